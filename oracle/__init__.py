@@ -1,0 +1,239 @@
+"""ctypes bindings of the CPU oracle (oracle/libcpu_ref.so) and, where it was built,
+of the reference's own GEFEL code (oracle/_ref/libssfref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker or the timed CPU baseline.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcpu_ref.so")
+REF_PATH = os.path.join(HERE, "_ref", "libssfref.so")
+
+_P = ctypes.c_void_p
+_L = ctypes.c_long
+_D = ctypes.c_double
+_lib = None
+_ref = None
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run make -C oracle)")
+        l = ctypes.CDLL(LIB_PATH)
+        sig = {
+            "cr_codenum": (_L, [_L, _L, _P, _P]),
+            "cr_elem_codes": (None, [_L, _L, _P, _L, _P, _P]),
+            "cr_scr_pattern": (_L, [_L, _L, _L, _P, _P, _P, _L]),
+            "cr_cr_pattern": (_L, [_L, _L, _L, _P, _P, _P, _L]),
+            "cr_scr_localized": (None, [_L, _P, _P, _P, _P, _P, _L]),
+            "cr_cr_localized": (None, [_L, _P, _P, _P, _P, _P, _L]),
+            "cr_scr_assemble": (None, [_L, _P, _P, _P, _L, _L, _P, _P]),
+            "cr_cr_assemble": (None, [_L, _P, _P, _P, _L, _L, _P, _P]),
+            "cr_mxv_scr": (None, [_L, _P, _P, _P, _P, _P]),
+            "cr_mxv_cr": (None, [_L, _P, _P, _P, _P, _P]),
+            "cr_scr_cg_prec": (ctypes.c_int, [_L, _P, _P, _P, _P, _P, _L, _D, _P, _P, _D, _L]),
+            "cr_scr_cg": (ctypes.c_int, [_L, _P, _P, _P, _P, _P, _L, _D, _P, _P, _D, _L]),
+            "cr_cr_cg_prec_noprec": (ctypes.c_int, [_L, _P, _P, _P, _P, _P, _L, _D, _P, _P, _D, _L]),
+            "cr_hex8_elast_ke": (None, [_P, _P, _P, _D, _D, _P]),
+            "cr_hex8_heat_ke": (None, [_P, _P, _P, _D, _P]),
+            "cr_quad4_pstress_ke": (None, [_P, _P, _D, _D, _P, _P]),
+            "cr_hex8_elast_ke_mesh": (None, [_L, _P, _P, _P, _P, _P]),
+            "cr_hex8_heat_ke_mesh": (None, [_L, _P, _P, _P, _P]),
+            "cr_quad4_pstress_ke_mesh": (None, [_L, _P, _P, _P, _P, _P, _P]),
+            "cr_dirichlet_lift": (None, [_L, _L, _P, _P, _P, _P]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(l, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = l
+    return _lib
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_PATH)
+
+
+def ref():
+    """The reference's own GEFEL code behind ref_harness.cpp (built only where
+    /root/reference exists)."""
+    global _ref
+    if _ref is None:
+        l = ctypes.CDLL(REF_PATH, mode=os.RTLD_LAZY | os.RTLD_LOCAL)
+        sig = {
+            "ref_scr_create": (_P, [_L, _L, _L, _P]),
+            "ref_scr_negm": (_L, [_P]),
+            "ref_scr_get": (None, [_P, _P, _P, _P]),
+            "ref_scr_set_a": (None, [_P, _P]),
+            "ref_scr_localized": (None, [_P, _P, _P, _L]),
+            "ref_scr_mxv": (None, [_P, _P, _P]),
+            "ref_scr_cg_prec": (None, [_P, _P, _P, _L, _D, _P, _P, _D, _L, _L]),
+            "ref_scr_cg": (None, [_P, _P, _P, _L, _D, _P, _P, _D, _L]),
+            "ref_scr_free": (None, [_P]),
+            "ref_cr_create": (_P, [_L, _L, _L, _P]),
+            "ref_cr_negm": (_L, [_P]),
+            "ref_cr_get": (None, [_P, _P, _P, _P]),
+            "ref_cr_localized": (None, [_P, _P, _P, _L]),
+            "ref_cr_mxv": (None, [_P, _P, _P]),
+            "ref_cr_cg_prec_noprec": (None, [_P, _P, _P, _L, _D, _P, _P, _D, _L]),
+            "ref_cr_free": (None, [_P]),
+            "ref_hex8_elast_ke": (None, [_P, _P, _P, _D, _D, _P]),
+            "ref_quad4_pstress_ke": (None, [_P, _P, _D, _D, _P, _P]),
+            "ref_hex8_heat_ke": (None, [_P, _P, _P, _D, _P]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(l, name)
+            f.restype = res
+            f.argtypes = args
+        _ref = l
+    return _ref
+
+
+# ------------------------------------------------------------------ helpers
+
+def i64(a):
+    return np.ascontiguousarray(a, dtype=np.int64)
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def scr_pattern(n, cn):
+    cn = i64(cn)
+    ne, ndofe = cn.shape
+    adr = np.zeros(n + 1, np.int64)
+    negm = lib().cr_scr_pattern(n, ne, ndofe, _ptr(cn), _ptr(adr), None, -1)
+    ci = np.zeros(negm, np.int64)
+    lib().cr_scr_pattern(n, ne, ndofe, _ptr(cn), _ptr(adr), _ptr(ci), negm)
+    return adr, ci
+
+
+def cr_pattern(n, cn):
+    cn = i64(cn)
+    ne, ndofe = cn.shape
+    adr = np.zeros(n + 1, np.int64)
+    negm = lib().cr_cr_pattern(n, ne, ndofe, _ptr(cn), _ptr(adr), None, -1)
+    ci = np.zeros(negm, np.int64)
+    lib().cr_cr_pattern(n, ne, ndofe, _ptr(cn), _ptr(adr), _ptr(ci), negm)
+    return adr, ci
+
+
+def element_matrices(prob):
+    """K_e for every element of a mesh.Problem, (ne, ndofe, ndofe)."""
+    from ssf_amd import mesh
+    ne, nd = prob.ne, prob.ndofe
+    ke = np.zeros((ne, nd, nd))
+    conn = i64(prob.conn)
+    if prob.kind == mesh.KIND_HEX8_ELAST:
+        lib().cr_hex8_elast_ke_mesh(ne, _ptr(conn), _ptr(f64(prob.coords)), _ptr(f64(prob.e)),
+                                    _ptr(f64(prob.nu)), _ptr(ke))
+    elif prob.kind == mesh.KIND_HEX8_HEAT:
+        lib().cr_hex8_heat_ke_mesh(ne, _ptr(conn), _ptr(f64(prob.coords)), _ptr(f64(prob.e)),
+                                   _ptr(ke))
+    else:
+        xy = f64(prob.coords[:, :2])
+        lib().cr_quad4_pstress_ke_mesh(ne, _ptr(conn), _ptr(xy), _ptr(f64(prob.e)),
+                                       _ptr(f64(prob.nu)), _ptr(f64(prob.thick)), _ptr(ke))
+    return ke
+
+
+def scr_assemble(n, adr, ci, cn, ke):
+    a = np.zeros(ci.shape[0])
+    cn = i64(cn)
+    ke = f64(ke)
+    lib().cr_scr_assemble(n, _ptr(adr), _ptr(ci), _ptr(a), cn.shape[0], cn.shape[1],
+                          _ptr(cn), _ptr(ke))
+    return a
+
+
+def cr_assemble(n, adr, ci, cn, ke):
+    a = np.zeros(ci.shape[0])
+    cn = i64(cn)
+    ke = f64(ke)
+    lib().cr_cr_assemble(n, _ptr(adr), _ptr(ci), _ptr(a), cn.shape[0], cn.shape[1],
+                         _ptr(cn), _ptr(ke))
+    return a
+
+
+def dirichlet_lift(prob, cn, ke):
+    rhs = np.zeros(prob.n)
+    cn = i64(cn)
+    lib().cr_dirichlet_lift(cn.shape[0], cn.shape[1], _ptr(cn), _ptr(f64(ke)),
+                            _ptr(f64(prob.pv)), _ptr(rhs))
+    return rhs
+
+
+def mxv_scr(adr, ci, a, b):
+    n = adr.shape[0] - 1
+    c = np.zeros(n)
+    lib().cr_mxv_scr(n, _ptr(adr), _ptr(ci), _ptr(f64(a)), _ptr(f64(b)), _ptr(c))
+    return c
+
+
+def mxv_cr(adr, ci, a, b):
+    n = adr.shape[0] - 1
+    c = np.zeros(n)
+    lib().cr_mxv_cr(n, _ptr(adr), _ptr(ci), _ptr(f64(a)), _ptr(f64(b)), _ptr(c))
+    return c
+
+
+def scr_cg_prec(adr, ci, a, y, ni, res, zero=1e-30, x0=None):
+    n = adr.shape[0] - 1
+    x = np.zeros(n) if x0 is None else f64(x0).copy()
+    ani = ctypes.c_long(-1)
+    ares = ctypes.c_double(-1.0)
+    st = lib().cr_scr_cg_prec(n, _ptr(adr), _ptr(ci), _ptr(f64(a)), _ptr(x), _ptr(f64(y)),
+                              ni, res, ctypes.byref(ani), ctypes.byref(ares), zero,
+                              0 if x0 is None else 1)
+    return x, ani.value, ares.value, st
+
+
+def scr_cg(adr, ci, a, y, ni, res, zero=1e-30):
+    n = adr.shape[0] - 1
+    x = np.zeros(n)
+    ani = ctypes.c_long(-1)
+    ares = ctypes.c_double(-1.0)
+    st = lib().cr_scr_cg(n, _ptr(adr), _ptr(ci), _ptr(f64(a)), _ptr(x), _ptr(f64(y)), ni, res,
+                         ctypes.byref(ani), ctypes.byref(ares), zero, 0)
+    return x, ani.value, ares.value, st
+
+
+def cr_cg_prec_noprec(adr, ci, a, y, ni, res, zero=1e-30):
+    n = adr.shape[0] - 1
+    x = np.zeros(n)
+    ani = ctypes.c_long(-1)
+    ares = ctypes.c_double(-1.0)
+    st = lib().cr_cr_cg_prec_noprec(n, _ptr(adr), _ptr(ci), _ptr(f64(a)), _ptr(x),
+                                    _ptr(f64(y)), ni, res, ctypes.byref(ani),
+                                    ctypes.byref(ares), zero, 0)
+    return x, ani.value, ares.value, st
+
+
+def assemble_problem(prob, storage="scr"):
+    """Oracle assembly of a mesh.Problem: (adr, ci, a, rhs[nrhs, n]) with the
+    Dirichlet lift added to every right-hand side."""
+    cn = prob.elem_codes()
+    ke = element_matrices(prob)
+    if storage == "scr":
+        adr, ci = scr_pattern(prob.n, cn)
+        a = scr_assemble(prob.n, adr, ci, cn, ke)
+    else:
+        adr, ci = cr_pattern(prob.n, cn)
+        a = cr_assemble(prob.n, adr, ci, cn, ke)
+    rhs = prob.rhs.copy()
+    if prob.pv.size:
+        rhs = rhs + dirichlet_lift(prob, cn, ke)[None, :]
+    return adr, ci, a, rhs

@@ -1,0 +1,824 @@
+/*
+ * oracle/cpu_ref.c -- CPU restatement of the SIFEL linear-system hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker and the timed CPU baseline); the product
+ * in ssf_amd/ never calls it. Build: make -C oracle (gcc -O2 -ffp-contract=off).
+ * Each function names the reference file:line whose loop order it restates; the
+ * pinning checks for each are listed in cpu_ref.h.
+ */
+#include "cpu_ref.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------ numbering */
+
+/* GEFEL/gtopology.cpp:1619-1680 */
+long cr_codenum(long nn, long ndofn, long *dof, const long *ordering)
+{
+  long i, j, k, ii, ndof = 0;
+  for (i = 0; i < nn * ndofn; i++)
+    if (dof[i] > ndof) ndof = dof[i];
+  ndof--;
+  if (ndof < 0) ndof = 0;
+  long *aux = (long *)malloc((size_t)(ndof > 0 ? ndof : 1) * sizeof(long));
+  for (i = 0; i < ndof; i++) aux[i] = -1;
+  ndof = 1;
+  for (i = 0; i < nn; i++) {
+    ii = ordering ? ordering[i] : i;
+    for (j = 0; j < ndofn; j++) {
+      k = dof[ii * ndofn + j];
+      if (k <= 0) continue;
+      if (k == 1) {
+        dof[ii * ndofn + j] = ndof;
+        ndof++;
+      } else if (aux[k - 2] == -1) {
+        dof[ii * ndofn + j] = ndof;
+        aux[k - 2] = ndof;
+        ndof++;
+      } else {
+        dof[ii * ndofn + j] = aux[k - 2];
+      }
+    }
+  }
+  free(aux);
+  return ndof - 1;
+}
+
+/* GEFEL/gtopology.cpp:597-626 (node-based branch) */
+void cr_elem_codes(long ne, long nne, const long *conn, long ndofn, const long *dof,
+                   long *cn)
+{
+  for (long e = 0; e < ne; e++) {
+    long k = 0;
+    for (long i = 0; i < nne; i++) {
+      long nd = conn[e * nne + i];
+      for (long j = 0; j < ndofn; j++) cn[e * nne * ndofn + k++] = dof[nd * ndofn + j];
+    }
+  }
+}
+
+/* ------------------------------------------------------------------- patterns */
+
+static int cmp_long(const void *a, const void *b)
+{
+  long x = *(const long *)a, y = *(const long *)b;
+  return (x > y) - (x < y);
+}
+
+/* Shared body of symcomprow::initiate (GEFEL/scr.cpp:112-330) and
+ * comprow::initiate (GEFEL/cr.cpp:140-320). lower=1 keeps only jj<=ii.
+ * Both reference sorts (sort_and_colindex_tko / sort_and_colindex) leave each
+ * row's column indices ascending and free of duplicates; that is what is built. */
+static long pattern(long n, long ne, long ndofe, const long *cn, long *adr, long *ci,
+                    long cap, int lower)
+{
+  long i, j, e, ii, jj;
+  long *cnt = (long *)calloc((size_t)n + 1, sizeof(long));
+  /* numcontr_elem */
+  for (e = 0; e < ne; e++) {
+    const long *c = cn + e * ndofe;
+    for (i = 0; i < ndofe; i++) {
+      ii = c[i] - 1;
+      if (ii < 0) continue;
+      for (j = 0; j < ndofe; j++) {
+        jj = c[j] - 1;
+        if (jj < 0) continue;
+        if (lower && ii < jj) continue;
+        cnt[ii]++;
+      }
+    }
+  }
+  /* addresses */
+  long *start = (long *)malloc(((size_t)n + 1) * sizeof(long));
+  long s = 0;
+  for (i = 0; i < n; i++) {
+    start[i] = s;
+    s += cnt[i];
+  }
+  start[n] = s;
+  long *aux = (long *)malloc((size_t)(s > 0 ? s : 1) * sizeof(long));
+  long *fill = (long *)malloc(((size_t)n + 1) * sizeof(long));
+  memcpy(fill, start, ((size_t)n + 1) * sizeof(long));
+  /* fillarray_elem */
+  for (e = 0; e < ne; e++) {
+    const long *c = cn + e * ndofe;
+    for (i = 0; i < ndofe; i++) {
+      ii = c[i] - 1;
+      if (ii < 0) continue;
+      for (j = 0; j < ndofe; j++) {
+        jj = c[j] - 1;
+        if (jj < 0) continue;
+        if (lower && ii < jj) continue;
+        aux[fill[ii]++] = jj;
+      }
+    }
+  }
+  /* sort + dedupe per row, then compact */
+  long negm = 0;
+  adr[0] = 0;
+  for (i = 0; i < n; i++) {
+    long lo = start[i], hi = fill[i];
+    qsort(aux + lo, (size_t)(hi - lo), sizeof(long), cmp_long);
+    long prev = -1;
+    for (j = lo; j < hi; j++) {
+      if (aux[j] == prev) continue;
+      prev = aux[j];
+      if (ci && cap >= 0 && negm < cap) ci[negm] = aux[j];
+      negm++;
+    }
+    adr[i + 1] = negm;
+  }
+  free(cnt);
+  free(start);
+  free(aux);
+  free(fill);
+  return negm;
+}
+
+long cr_scr_pattern(long n, long ne, long ndofe, const long *cn, long *adr, long *ci,
+                    long cap)
+{
+  return pattern(n, ne, ndofe, cn, adr, ci, cap, 1);
+}
+
+long cr_cr_pattern(long n, long ne, long ndofe, const long *cn, long *adr, long *ci,
+                   long cap)
+{
+  return pattern(n, ne, ndofe, cn, adr, ci, cap, 0);
+}
+
+/* ------------------------------------------------------------------ localize */
+
+static long find_slot(const long *adr, const long *ci, long row, long col)
+{
+  long lo = adr[row], hi = adr[row + 1] - 1;
+  while (lo <= hi) {
+    long mid = lo + (hi - lo) / 2;
+    if (ci[mid] == col) return mid;
+    if (ci[mid] < col) lo = mid + 1;
+    else hi = mid - 1;
+  }
+  return -1;
+}
+
+/* GEFEL/scr.cpp:472-521. The reference's forward/backward linear search finds the
+ * unique slot of (ii, jj); adds happen in (i, j) loop order. */
+void cr_scr_localized(long n, const long *adr, const long *ci, double *a,
+                      const double *lm, const long *cn, long nc)
+{
+  (void)n;
+  for (long i = 0; i < nc; i++) {
+    long ii = cn[i] - 1;
+    if (ii < 0) continue;
+    for (long j = 0; j < nc; j++) {
+      long jj = cn[j] - 1;
+      if (jj < 0) continue;
+      if (ii < jj) continue;
+      long k = find_slot(adr, ci, ii, jj);
+      if (k >= 0) a[k] += lm[i * nc + j];
+    }
+  }
+}
+
+/* GEFEL/cr.cpp:368-410 */
+void cr_cr_localized(long n, const long *adr, const long *ci, double *a,
+                     const double *lm, const long *cn, long nc)
+{
+  (void)n;
+  for (long i = 0; i < nc; i++) {
+    long ii = cn[i] - 1;
+    if (ii < 0) continue;
+    for (long j = 0; j < nc; j++) {
+      long jj = cn[j] - 1;
+      if (jj < 0) continue;
+      long k = find_slot(adr, ci, ii, jj);
+      if (k >= 0) a[k] += lm[i * nc + j];
+    }
+  }
+}
+
+void cr_scr_assemble(long n, const long *adr, const long *ci, double *a, long ne,
+                     long nc, const long *cn, const double *ke)
+{
+  for (long e = 0; e < ne; e++)
+    cr_scr_localized(n, adr, ci, a, ke + e * nc * nc, cn + e * nc, nc);
+}
+
+void cr_cr_assemble(long n, const long *adr, const long *ci, double *a, long ne,
+                    long nc, const long *cn, const double *ke)
+{
+  for (long e = 0; e < ne; e++)
+    cr_cr_localized(n, adr, ci, a, ke + e * nc * nc, cn + e * nc, nc);
+}
+
+/* ---------------------------------------------------------------------- SpMV */
+
+/* GEFEL/scr.cpp:655-670 */
+void cr_mxv_scr(long n, const long *adr, const long *ci, const double *a,
+                const double *b, double *c)
+{
+  for (long i = 0; i < n; i++) {
+    double s = 0.0, d = b[i];
+    for (long j = adr[i]; j < adr[i + 1]; j++) {
+      long ii = ci[j];
+      s += a[j] * b[ii];
+      c[ii] += a[j] * d;
+    }
+    c[i] = s;
+  }
+}
+
+/* GEFEL/cr.cpp:653-700 (serial branch) */
+void cr_mxv_cr(long n, const long *adr, const long *ci, const double *a,
+               const double *b, double *c)
+{
+  for (long i = 0; i < n; i++) {
+    double s = 0.0;
+    for (long j = adr[i]; j < adr[i + 1]; j++) s += a[j] * b[ci[j]];
+    c[i] = s;
+  }
+}
+
+/* GEFEL/vector.cpp:2052 ss */
+static double ss(const double *a, const double *b, long n)
+{
+  double s = 0.0;
+  for (long i = 0; i < n; i++) s += a[i] * b[i];
+  return s;
+}
+
+/* ------------------------------------------------------------------- solvers */
+
+/* GEFEL/scr.cpp:1093-1203, tprec = 1 */
+int cr_scr_cg_prec(long n, const long *adr, const long *ci, const double *a, double *x,
+                   const double *y, long ni, double res, long *ani, double *ares,
+                   double zero, long iv)
+{
+  long i, j;
+  double nom, denom, nory, alpha, beta, rr;
+  int status = 0;
+  double *d = (double *)malloc((size_t)n * sizeof(double));
+  double *r = (double *)malloc((size_t)n * sizeof(double));
+  double *p = (double *)malloc((size_t)n * sizeof(double));
+  double *h = (double *)malloc((size_t)n * sizeof(double));
+  if (iv == 0)
+    for (i = 0; i < n; i++) x[i] = 0.0;
+  /* mxv_scr accumulates into p before overwriting p[i]; the reference's p comes
+   * from `new double[n]` (uninitialised); every c[ii] with ii<i is overwritten
+   * when row ii... is visited earlier, so only entries ii>=i... matter: rows are
+   * visited in ascending order and c[i]=s happens last for row i, so each c[k]
+   * is set by row k before any later row adds into it (ci <= row). Zeroing is
+   * therefore not needed, but it keeps the oracle free of reads of garbage. */
+  memset(p, 0, (size_t)n * sizeof(double));
+  cr_mxv_scr(n, adr, ci, a, x, p);
+  nory = 0.0;
+  for (i = 0; i < n; i++) {
+    nory += y[i] * y[i];
+    r[i] = y[i] - p[i];
+  }
+  if (nory < zero) {
+    fprintf(stderr, "cr_scr_cg_prec: right hand side vector has zero norm\n");
+    *ares = nory;
+    *ani = 0;
+  }
+  for (i = 0; i < n; i++) h[i] = r[i] / a[adr[i + 1] - 1];
+  nom = ss(r, h, n);
+  for (i = 0; i < n; i++) d[i] = h[i];
+  for (i = 0; i < ni; i++) {
+    cr_mxv_scr(n, adr, ci, a, d, p);
+    denom = ss(d, p, n);
+    if (fabs(denom) < zero) {
+      *ares = nom;
+      *ani = i;
+      status = 2;
+      break;
+    }
+    alpha = nom / denom;
+    rr = 0.0;
+    for (j = 0; j < n; j++) {
+      x[j] += alpha * d[j];
+      r[j] -= alpha * p[j];
+      rr += r[j] * r[j];
+    }
+    for (j = 0; j < n; j++) h[j] = r[j] / a[adr[j + 1] - 1];
+    denom = nom;
+    nom = ss(r, h, n);
+    beta = nom / denom;
+    if (fabs(rr / nory) < res) {
+      *ani = i;
+      *ares = nom / nory;
+      status = 1;
+      break;
+    }
+    for (j = 0; j < n; j++) d[j] = beta * d[j] + h[j];
+  }
+  free(h);
+  free(p);
+  free(r);
+  free(d);
+  return status;
+}
+
+/* GEFEL/scr.cpp:786-880 */
+int cr_scr_cg(long n, const long *adr, const long *ci, const double *a, double *x,
+              const double *y, long ni, double res, long *ani, double *ares,
+              double zero, long iv)
+{
+  long i, j;
+  double nom, denom, nory, alpha, beta;
+  int status = 0;
+  double *d = (double *)malloc((size_t)n * sizeof(double));
+  double *r = (double *)malloc((size_t)n * sizeof(double));
+  double *p = (double *)calloc((size_t)n, sizeof(double));
+  if (iv == 0)
+    for (i = 0; i < n; i++) x[i] = 0.0;
+  cr_mxv_scr(n, adr, ci, a, x, p);
+  nory = 0.0;
+  nom = 0.0;
+  for (i = 0; i < n; i++) {
+    nory += y[i] * y[i];
+    r[i] = y[i] - p[i];
+    nom += r[i] * r[i];
+    d[i] = r[i];
+  }
+  if (nory < zero) {
+    *ares = nory;
+    *ani = 0;
+    for (i = 0; i < n; i++) x[i] = 0.0;
+    free(p);
+    free(r);
+    free(d);
+    return 3;
+  }
+  for (i = 0; i < ni; i++) {
+    cr_mxv_scr(n, adr, ci, a, d, p);
+    denom = ss(d, p, n);
+    if (fabs(denom) < zero) {
+      status = 2;
+      break;
+    }
+    alpha = nom / denom;
+    for (j = 0; j < n; j++) {
+      x[j] += alpha * d[j];
+      r[j] -= alpha * p[j];
+    }
+    denom = nom;
+    nom = ss(r, r, n);
+    if (sqrt(nom / nory) < res) {
+      status = 1;
+      break;
+    }
+    if (fabs(nom) < zero) {
+      status = 1;
+      break;
+    }
+    beta = nom / denom;
+    for (j = 0; j < n; j++) d[j] = beta * d[j] + r[j];
+  }
+  *ani = i;
+  *ares = nom;
+  free(p);
+  free(r);
+  free(d);
+  return status;
+}
+
+/* GEFEL/cr.cpp:1124-1232 with precond::preconditioning noprecond (h = r,
+ * GEFEL/precond.cpp:161-170). */
+int cr_cr_cg_prec_noprec(long n, const long *adr, const long *ci, const double *a,
+                         double *x, const double *y, long ni, double res, long *ani,
+                         double *ares, double zero, long iv)
+{
+  long i, j;
+  double nom, denom, nory, alpha, beta;
+  int status = 0;
+  double *d = (double *)malloc((size_t)n * sizeof(double));
+  double *r = (double *)malloc((size_t)n * sizeof(double));
+  double *p = (double *)malloc((size_t)n * sizeof(double));
+  double *h = (double *)malloc((size_t)n * sizeof(double));
+  if (iv == 0)
+    for (i = 0; i < n; i++) x[i] = 0.0;
+  cr_mxv_cr(n, adr, ci, a, x, p);
+  nory = 0.0;
+  for (i = 0; i < n; i++) {
+    nory += y[i] * y[i];
+    r[i] = y[i] - p[i];
+  }
+  if (nory < zero) {
+    *ares = nory;
+    *ani = 0;
+    free(h);
+    free(p);
+    free(r);
+    free(d);
+    return 3;
+  }
+  for (i = 0; i < n; i++) h[i] = r[i];
+  nom = ss(r, h, n);
+  for (i = 0; i < n; i++) d[i] = h[i];
+  for (i = 0; i < ni; i++) {
+    cr_mxv_cr(n, adr, ci, a, d, p);
+    denom = ss(d, p, n);
+    if (fabs(denom) < zero) {
+      status = 2;
+      break;
+    }
+    alpha = nom / denom;
+    for (j = 0; j < n; j++) {
+      x[j] += alpha * d[j];
+      r[j] -= alpha * p[j];
+    }
+    for (j = 0; j < n; j++) h[j] = r[j];
+    denom = nom;
+    nom = ss(r, h, n);
+    beta = nom / denom;
+    if (sqrt(nom / nory) < res) {
+      status = 1;
+      break;
+    }
+    for (j = 0; j < n; j++) d[j] = beta * d[j] + h[j];
+  }
+  /* the reference overwrites ares with nom after the loop (GEFEL/cr.cpp:1227) */
+  *ani = i;
+  *ares = nom;
+  free(h);
+  free(p);
+  free(r);
+  free(d);
+  return status;
+}
+
+/* ------------------------------------------------------------ element kernels */
+
+static const double GP2[2] = {-0.577350269189626, 0.577350269189626}; /* intp.cpp:26-27 */
+static const double GW2[2] = {1.0, 1.0};
+
+/* GEFEL/basefun.cpp:2317-2375 */
+static void dx_hex(double *n, double y, double z)
+{
+  n[0] = (1.0 + y) * (1.0 + z) / 8.0;
+  n[1] = (1.0 + y) * (1.0 + z) / (-8.0);
+  n[2] = (1.0 - y) * (1.0 + z) / (-8.0);
+  n[3] = (1.0 - y) * (1.0 + z) / 8.0;
+  n[4] = (1.0 + y) * (1.0 - z) / 8.0;
+  n[5] = (1.0 + y) * (1.0 - z) / (-8.0);
+  n[6] = (1.0 - y) * (1.0 - z) / (-8.0);
+  n[7] = (1.0 - y) * (1.0 - z) / 8.0;
+}
+static void dy_hex(double *n, double x, double z)
+{
+  n[0] = (1.0 + x) * (1.0 + z) / 8.0;
+  n[1] = (1.0 - x) * (1.0 + z) / 8.0;
+  n[2] = (1.0 - x) * (1.0 + z) / (-8.0);
+  n[3] = (1.0 + x) * (1.0 + z) / (-8.0);
+  n[4] = (1.0 + x) * (1.0 - z) / 8.0;
+  n[5] = (1.0 - x) * (1.0 - z) / 8.0;
+  n[6] = (1.0 - x) * (1.0 - z) / (-8.0);
+  n[7] = (1.0 + x) * (1.0 - z) / (-8.0);
+}
+static void dz_hex(double *n, double x, double y)
+{
+  n[0] = (1.0 + x) * (1.0 + y) / 8.0;
+  n[1] = (1.0 - x) * (1.0 + y) / 8.0;
+  n[2] = (1.0 - x) * (1.0 - y) / 8.0;
+  n[3] = (1.0 + x) * (1.0 - y) / 8.0;
+  n[4] = (1.0 + x) * (1.0 + y) / (-8.0);
+  n[5] = (1.0 - x) * (1.0 + y) / (-8.0);
+  n[6] = (1.0 - x) * (1.0 - y) / (-8.0);
+  n[7] = (1.0 + x) * (1.0 - y) / (-8.0);
+}
+
+/* GEFEL/difcalc.cpp:487-571 derivatives_3d for 8 nodes: on entry dx,dy,dz hold the
+ * natural derivatives, on exit the global ones; returns the Jacobian. */
+static double derivatives_3d(double *dx, double *dy, double *dz, const double *x,
+                             const double *y, const double *z, double xi, double eta,
+                             double zeta)
+{
+  double nx[8], ny[8], nz[8];
+  dx_hex(nx, eta, zeta);
+  dy_hex(ny, xi, zeta);
+  dz_hex(nz, xi, eta);
+  double d1 = 0.0, d2 = 0.0, d3 = 0.0, d4 = 0.0, d5 = 0.0, d6 = 0.0, d7 = 0.0, d8 = 0.0,
+         d9 = 0.0;
+  for (int i = 0; i < 8; i++) {
+    d1 += nx[i] * x[i];
+    d2 += ny[i] * x[i];
+    d3 += nz[i] * x[i];
+    d4 += nx[i] * y[i];
+    d5 += ny[i] * y[i];
+    d6 += nz[i] * y[i];
+    d7 += nx[i] * z[i];
+    d8 += ny[i] * z[i];
+    d9 += nz[i] * z[i];
+  }
+  double jac = d1 * d5 * d9 + d4 * d8 * d3 + d7 * d2 * d6 - d7 * d5 * d3 - d8 * d6 * d1 -
+               d9 * d4 * d2;
+  for (int i = 0; i < 8; i++) {
+    double ddx = dx[i], ddy = dy[i], ddz = dz[i];
+    dx[i] = (ddx * (d5 * d9 - d8 * d6) + ddy * (d7 * d6 - d9 * d4) +
+             ddz * (d4 * d8 - d7 * d5)) / jac;
+    dy[i] = (ddx * (d8 * d3 - d9 * d2) + ddy * (d1 * d9 - d7 * d3) +
+             ddz * (d7 * d2 - d8 * d1)) / jac;
+    dz[i] = (ddx * (d2 * d6 - d5 * d3) + ddy * (d4 * d3 - d6 * d1) +
+             ddz * (d1 * d5 - d4 * d2)) / jac;
+  }
+  return jac;
+}
+
+/* GEFEL/matrix.cpp:3126 bdbjac: d += (a^T b c) * jac with mtxm (:1362), mxm (:1184),
+ * cmulm (:1514) and addm (:1032) loop orders. a,c: m x n; b: m x m; d: n x n. */
+static void bdbjac(double *d, const double *a, const double *b, const double *c,
+                   double jac, int m, int n)
+{
+  double x[24 * 6];
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < m; j++) {
+      double aux = 0.0;
+      for (int k = 0; k < m; k++) aux += a[k * n + i] * b[k * m + j];
+      x[i * m + j] = aux;
+    }
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) {
+      double aux = 0.0;
+      for (int k = 0; k < m; k++) aux += x[i * m + k] * c[k * n + j];
+      aux *= jac;
+      d[i * n + j] = aux + d[i * n + j];
+    }
+}
+
+/* MEFEL/SRC/linhex.cpp:583-640 with geom_matrix :153-186 and
+ * MEFEL/SRC/elastisomat.cpp:395-410 matstiff_spacestr */
+void cr_hex8_elast_ke(const double *x, const double *y, const double *z, double e,
+                      double nu, double *ke)
+{
+  double d[36], gm[6 * 24], dx[8], dy[8], dz[8];
+  double g = e / 2.0 / (1.0 + nu);
+  double s = e / (1.0 + nu) / (1.0 - 2.0 * nu);
+  memset(d, 0, sizeof d);
+  d[0] = s * (1.0 - nu);
+  d[1] = s * nu;
+  d[2] = s * nu;
+  d[6] = d[1];
+  d[7] = d[0];
+  d[8] = d[1];
+  d[12] = d[1];
+  d[13] = d[1];
+  d[14] = d[0];
+  d[21] = g;
+  d[28] = g;
+  d[35] = g;
+  memset(ke, 0, 576 * sizeof(double));
+  for (int i = 0; i < 2; i++) {
+    double xi = GP2[i];
+    for (int j = 0; j < 2; j++) {
+      double eta = GP2[j];
+      for (int k = 0; k < 2; k++) {
+        double zeta = GP2[k];
+        dx_hex(dx, eta, zeta);
+        dy_hex(dy, xi, zeta);
+        dz_hex(dz, xi, eta);
+        double jac = derivatives_3d(dx, dy, dz, x, y, z, xi, eta, zeta);
+        memset(gm, 0, sizeof gm);
+        for (int a = 0; a < 8; a++) {
+          int c0 = 3 * a, c1 = 3 * a + 1, c2 = 3 * a + 2;
+          gm[0 * 24 + c0] = dx[a];
+          gm[1 * 24 + c1] = dy[a];
+          gm[2 * 24 + c2] = dz[a];
+          gm[3 * 24 + c1] = dz[a];
+          gm[3 * 24 + c2] = dy[a];
+          gm[4 * 24 + c0] = dz[a];
+          gm[4 * 24 + c2] = dx[a];
+          gm[5 * 24 + c0] = dy[a];
+          gm[5 * 24 + c1] = dx[a];
+        }
+        jac *= GW2[i] * GW2[j] * GW2[k];
+        if (jac < 0.0) jac = fabs(jac);
+        bdbjac(ke, gm, d, gm, jac, 6, 24);
+      }
+    }
+  }
+}
+
+/* GEFEL/matrix.cpp:3066-3110 bdbj: k += b^T d b * jac, upper triangle mirrored. */
+static void bdbj(double *k, const double *b, const double *d, double jac, int m, int n)
+{
+  double am[8 * 3];
+  int ll = 0;
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < m; j++) {
+      double s = 0.0;
+      int ii = i, jj = j;
+      for (int l = 0; l < m; l++) {
+        s += b[ii] * d[jj];
+        ii += n;
+        jj += m;
+      }
+      am[ll++] = s;
+    }
+  for (int i = 0; i < n; i++) {
+    int ii = i * n + i, mm = ii;
+    double s = 0.0;
+    int lo = i * m, hi = lo + m, jj = i;
+    for (int l = lo; l < hi; l++) {
+      s += am[l] * b[jj];
+      jj += n;
+    }
+    k[ii] += s * jac;
+    ii++;
+    mm += n;
+    for (int j = i + 1; j < n; j++) {
+      s = 0.0;
+      jj = j;
+      for (int l = lo; l < hi; l++) {
+        s += am[l] * b[jj];
+        jj += n;
+      }
+      k[ii] += s * jac;
+      k[mm] += s * jac;
+      ii++;
+      mm += n;
+    }
+  }
+}
+
+/* TRFEL/SRC/linhext.cpp:774-840 with grad_matrix :743-761 and
+ * TRFEL/SRC/isotrmat.cpp:124-135 matcond3d */
+void cr_hex8_heat_ke(const double *x, const double *y, const double *z, double kc,
+                     double *ke)
+{
+  double d[9], gm[3 * 8], dx[8], dy[8], dz[8];
+  memset(ke, 0, 64 * sizeof(double));
+  for (int i = 0; i < 2; i++) {
+    double xi = GP2[i], ww1 = GW2[i];
+    for (int j = 0; j < 2; j++) {
+      double eta = GP2[j], ww2 = GW2[j];
+      for (int k = 0; k < 2; k++) {
+        double zeta = GP2[k], ww3 = GW2[k];
+        dx_hex(dx, eta, zeta);
+        dy_hex(dy, xi, zeta);
+        dz_hex(dz, xi, eta);
+        double jac = derivatives_3d(dx, dy, dz, x, y, z, xi, eta, zeta);
+        for (int a = 0; a < 8; a++) {
+          gm[a] = dx[a];
+          gm[8 + a] = dy[a];
+          gm[16 + a] = dz[a];
+        }
+        memset(d, 0, sizeof d);
+        d[0] = kc;
+        d[4] = kc;
+        d[8] = kc;
+        if (jac < 0.0) jac = fabs(jac);
+        jac *= ww1 * ww2 * ww3;
+        bdbj(ke, gm, d, jac, 3, 8);
+        /* convective term: the zero 1x3 matrix of an isotropic medium adds +0.0 */
+        for (int q = 0; q < 64; q++) ke[q] = 0.0 + ke[q];
+      }
+    }
+  }
+}
+
+/* GEFEL/basefun.cpp:609-660 */
+static void dx_q4(double *n, double y)
+{
+  n[0] = 0.25 * (1.0 + y);
+  n[1] = 0.25 * (-1.0 - y);
+  n[2] = 0.25 * (-1.0 + y);
+  n[3] = 0.25 * (1.0 - y);
+}
+static void dy_q4(double *n, double x)
+{
+  n[0] = 0.25 * (1.0 + x);
+  n[1] = 0.25 * (1.0 - x);
+  n[2] = 0.25 * (-1.0 + x);
+  n[3] = 0.25 * (-1.0 - x);
+}
+
+/* MEFEL/SRC/plelemlq.cpp:611-690 with geom_matrix :294-314, derivatives_2d
+ * (GEFEL/difcalc.cpp:86-150), approx :127 and elastisomat.cpp:255-266 */
+void cr_quad4_pstress_ke(const double *x, const double *y, double e, double nu,
+                         const double *t, double *ke)
+{
+  double d[9], gm[3 * 8], dx[4], dy[4], nx[4], ny[4];
+  double c = e / (1.0 - nu * nu);
+  memset(d, 0, sizeof d);
+  d[0] = c;
+  d[1] = c * nu;
+  d[3] = c * nu;
+  d[4] = c;
+  d[8] = e / 2.0 / (1.0 + nu);
+  memset(ke, 0, 64 * sizeof(double));
+  for (int i = 0; i < 2; i++) {
+    double xi = GP2[i];
+    for (int j = 0; j < 2; j++) {
+      double eta = GP2[j];
+      dx_q4(dx, eta);
+      dy_q4(dy, xi);
+      dx_q4(nx, eta);
+      dy_q4(ny, xi);
+      double d1 = 0.0, d2 = 0.0, d3 = 0.0, d4 = 0.0;
+      for (int a = 0; a < 4; a++) {
+        d1 += nx[a] * x[a];
+        d2 += ny[a] * x[a];
+        d3 += nx[a] * y[a];
+        d4 += ny[a] * y[a];
+      }
+      double jac = d1 * d4 - d2 * d3;
+      for (int a = 0; a < 4; a++) {
+        double ddx = dx[a], ddy = dy[a];
+        dx[a] = (ddx * d4 - ddy * d3) / jac;
+        dy[a] = (ddy * d1 - ddx * d2) / jac;
+      }
+      memset(gm, 0, sizeof gm);
+      for (int a = 0; a < 4; a++) {
+        gm[0 * 8 + 2 * a] = dx[a];
+        gm[1 * 8 + 2 * a + 1] = dy[a];
+        gm[2 * 8 + 2 * a] = dy[a];
+        gm[2 * 8 + 2 * a + 1] = dx[a];
+      }
+      double bf[4];
+      bf[0] = 0.25 * (1.0 + xi) * (1.0 + eta);
+      bf[1] = 0.25 * (1.0 - xi) * (1.0 + eta);
+      bf[2] = 0.25 * (1.0 - xi) * (1.0 - eta);
+      bf[3] = 0.25 * (1.0 + xi) * (1.0 - eta);
+      double thick = 0.0;
+      for (int a = 0; a < 4; a++) thick += bf[a] * t[a];
+      jac *= thick * GW2[i] * GW2[j];
+      if (jac < 0.0) jac = fabs(jac);
+      bdbjac(ke, gm, d, gm, jac, 3, 8);
+    }
+  }
+}
+
+void cr_hex8_elast_ke_mesh(long ne, const long *conn, const double *xyz,
+                           const double *e, const double *nu, double *ke)
+{
+  for (long el = 0; el < ne; el++) {
+    double x[8], y[8], z[8];
+    for (int a = 0; a < 8; a++) {
+      long nd = conn[el * 8 + a];
+      x[a] = xyz[3 * nd];
+      y[a] = xyz[3 * nd + 1];
+      z[a] = xyz[3 * nd + 2];
+    }
+    cr_hex8_elast_ke(x, y, z, e[el], nu[el], ke + el * 576);
+  }
+}
+
+void cr_hex8_heat_ke_mesh(long ne, const long *conn, const double *xyz,
+                          const double *k, double *ke)
+{
+  for (long el = 0; el < ne; el++) {
+    double x[8], y[8], z[8];
+    for (int a = 0; a < 8; a++) {
+      long nd = conn[el * 8 + a];
+      x[a] = xyz[3 * nd];
+      y[a] = xyz[3 * nd + 1];
+      z[a] = xyz[3 * nd + 2];
+    }
+    cr_hex8_heat_ke(x, y, z, k[el], ke + el * 64);
+  }
+}
+
+void cr_quad4_pstress_ke_mesh(long ne, const long *conn, const double *xy,
+                              const double *e, const double *nu, const double *t,
+                              double *ke)
+{
+  for (long el = 0; el < ne; el++) {
+    double x[4], y[4], th[4];
+    for (int a = 0; a < 4; a++) {
+      long nd = conn[el * 4 + a];
+      x[a] = xy[2 * nd];
+      y[a] = xy[2 * nd + 1];
+      th[a] = t[el];
+    }
+    cr_quad4_pstress_ke(x, y, e[el], nu[el], th, ke + el * 64);
+  }
+}
+
+/* TRFEL/SRC/globmatt.cpp:1635-1710 (stationary, zero initial values) */
+void cr_dirichlet_lift(long ne, long nc, const long *cn, const double *ke,
+                       const double *pv, double *rhs)
+{
+  double r[64], f[64];
+  for (long el = 0; el < ne; el++) {
+    const long *c = cn + el * nc;
+    int any = 0;
+    for (long i = 0; i < nc; i++) {
+      r[i] = c[i] < 0 ? pv[-c[i] - 1] : 0.0;
+      if (c[i] < 0) any = 1;
+    }
+    (void)any;
+    const double *k = ke + el * nc * nc;
+    for (long i = 0; i < nc; i++) {
+      double s = 0.0;
+      for (long j = 0; j < nc; j++) s += k[i * nc + j] * r[j];
+      f[i] = s;
+    }
+    for (long i = 0; i < nc; i++) f[i] *= -1.0;
+    for (long i = 0; i < nc; i++)
+      if (c[i] >= 1) rhs[c[i] - 1] += f[i];
+  }
+}

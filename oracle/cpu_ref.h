@@ -1,0 +1,118 @@
+/*
+ * oracle/cpu_ref.h -- CPU restatement of the SIFEL linear-system hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY. Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load this library, and only as the checker or the timed
+ * CPU baseline. The product (ssf_amd/) never calls it.
+ *
+ * Every routine restates the reference loop order with plain double arithmetic;
+ * the library is compiled with -O2 -ffp-contract=off so that, like the
+ * reference's x86-64 build (no FMA), each operation rounds on its own.
+ * Integers are C `long` (64-bit), exactly as the reference's `long`.
+ *
+ * Pinning (what checks each function; see tests/test_oracle_*.py):
+ *   cr_codenum            gencodnum rules; tests vs closed-form numbering and flags
+ *   cr_scr_pattern        tmatstore.tex known-answer arrays; bitwise vs oracle/_ref
+ *   cr_cr_pattern         tmatstore.tex known-answer arrays; bitwise vs oracle/_ref
+ *   cr_*_localized        bitwise vs oracle/_ref; dense numpy assembly
+ *   cr_mxv_scr/cr         bitwise vs oracle/_ref; dense numpy product
+ *   cr_scr_cg_prec        bitwise (x, ani, ares) vs oracle/_ref; scipy direct solve;
+ *                         diagonal matrix converges in one iteration
+ *   cr_scr_cg, cr_cr_cg_prec_noprec   bitwise vs oracle/_ref; scipy direct solve
+ *   cr_hex8_elast_ke      bitwise vs oracle/_ref; symmetry, 6 rigid-body modes,
+ *                         patch test (uniform strain energy closed form)
+ *   cr_hex8_heat_ke       bitwise vs oracle/_ref; constants in kernel, closed form
+ *                         unit-cube matrix, linear temperature patch test
+ *   cr_quad4_pstress_ke   bitwise vs oracle/_ref; 3 rigid-body modes, patch test
+ */
+#ifndef SSF_ORACLE_CPU_REF_H
+#define SSF_ORACLE_CPU_REF_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* GEFEL/gtopology.cpp:1619-1680 gencodnum. dof[nn*ndofn] holds the flags on entry
+ * (0 fixed, 1 free, k>1 coupled group, <0 prescribed) and the code numbers on exit.
+ * ordering may be NULL (identity). Returns the number of unknowns. */
+long cr_codenum(long nn, long ndofn, long *dof, const long *ordering);
+
+/* GEFEL/gtopology.cpp:597-626 give_code_numbers for node-based code numbers. */
+void cr_elem_codes(long ne, long nne, const long *conn, long ndofn, const long *dof,
+                   long *cn);
+
+/* GEFEL/scr.cpp:556-570 symcomprow::initiate (numcontr_elem :112, addresses :211,
+ * fillarray_elem :163, sort_and_colindex_tko :317). adr[n+1] is always written;
+ * ci is written when ci != NULL and cap >= result. Returns negm. */
+long cr_scr_pattern(long n, long ne, long ndofe, const long *cn, long *adr, long *ci,
+                    long cap);
+/* GEFEL/cr.cpp:454-470 comprow::initiate (sort_and_colindex :247). */
+long cr_cr_pattern(long n, long ne, long ndofe, const long *cn, long *adr, long *ci,
+                   long cap);
+
+/* GEFEL/scr.cpp:472-521 symcomprow::localized and GEFEL/cr.cpp:368-410
+ * comprow::localized: a[] += element matrix lm[nc*nc] (row-major). */
+void cr_scr_localized(long n, const long *adr, const long *ci, double *a,
+                      const double *lm, const long *cn, long nc);
+void cr_cr_localized(long n, const long *adr, const long *ci, double *a,
+                     const double *lm, const long *cn, long nc);
+/* element loop of MEFEL/SRC/globmat.cpp:41-142 / TRFEL/SRC/globmatt.cpp:19-67:
+ * localize every element matrix ke[ne][nc*nc] in element order. */
+void cr_scr_assemble(long n, const long *adr, const long *ci, double *a, long ne,
+                     long nc, const long *cn, const double *ke);
+void cr_cr_assemble(long n, const long *adr, const long *ci, double *a, long ne,
+                    long nc, const long *cn, const double *ke);
+
+/* GEFEL/scr.cpp:655-670 mxv_scr and GEFEL/cr.cpp:653-700 mxv_cr (serial branch). */
+void cr_mxv_scr(long n, const long *adr, const long *ci, const double *a,
+                const double *b, double *c);
+void cr_mxv_cr(long n, const long *adr, const long *ci, const double *a,
+               const double *b, double *c);
+
+/* GEFEL/scr.cpp:1093-1203 cg_prec with tprec=1 (Jacobi, prec_diag_scr :897).
+ * Returns 1 when the stopping test fired, 2 on the zero-denominator exit,
+ * 0 when ni was exhausted (ani/ares then keep their input values, as in the
+ * reference). */
+int cr_scr_cg_prec(long n, const long *adr, const long *ci, const double *a, double *x,
+                   const double *y, long ni, double res, long *ani, double *ares,
+                   double zero, long iv);
+/* GEFEL/scr.cpp:786-880 symcomprow::cg (no preconditioner, sqrt test). */
+int cr_scr_cg(long n, const long *adr, const long *ci, const double *a, double *x,
+              const double *y, long ni, double res, long *ani, double *ares,
+              double zero, long iv);
+/* GEFEL/cr.cpp:1124-1232 comprow::cg_prec with a noprecond precond (h = r). */
+int cr_cr_cg_prec_noprec(long n, const long *adr, const long *ci, const double *a,
+                         double *x, const double *y, long ni, double res, long *ani,
+                         double *ares, double zero, long iv);
+
+/* MEFEL/SRC/linhex.cpp:583-640 gl_stiffness_matrix, 24x24 row-major. */
+void cr_hex8_elast_ke(const double *x, const double *y, const double *z, double e,
+                      double nu, double *ke);
+/* TRFEL/SRC/linhext.cpp:774-840 conductivity_matrix, 8x8 row-major. */
+void cr_hex8_heat_ke(const double *x, const double *y, const double *z, double k,
+                     double *ke);
+/* MEFEL/SRC/plelemlq.cpp:611-690 gl_stiffness_matrix, 8x8 row-major; t = nodal
+ * thickness (4 values). */
+void cr_quad4_pstress_ke(const double *x, const double *y, double e, double nu,
+                         const double *t, double *ke);
+
+/* Batched element matrices for a mesh: coordinates per node (xyz, stride 3 or 2),
+ * connectivity conn[ne*nne] 0-based, per-element material. */
+void cr_hex8_elast_ke_mesh(long ne, const long *conn, const double *xyz,
+                           const double *e, const double *nu, double *ke);
+void cr_hex8_heat_ke_mesh(long ne, const long *conn, const double *xyz,
+                          const double *k, double *ke);
+void cr_quad4_pstress_ke_mesh(long ne, const long *conn, const double *xy,
+                              const double *e, const double *nu, const double *t,
+                              double *ke);
+
+/* TRFEL/SRC/globmatt.cpp:1635-1700 assemble_init for a stationary problem with
+ * zero initial values: rhs += -(K_e r_e) with r_e the prescribed values pv[-cn-1]
+ * of the element's prescribed DOFs (MATRIX.cpp:3500 locglob scatter). */
+void cr_dirichlet_lift(long ne, long nc, const long *cn, const double *ke,
+                       const double *pv, double *rhs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

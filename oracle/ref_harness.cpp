@@ -1,0 +1,264 @@
+// Test infrastructure only: a thin extern "C" harness around the reference's own
+// GEFEL sources (compiled from /root/reference by oracle/Makefile into oracle/_ref/).
+// Nothing in the product links this; tests use it to pin the oracle restatement.
+//
+// Storage, SpMV and CG calls go straight into the reference classes
+// (GEFEL/scr.cpp, GEFEL/cr.cpp). The element routines compose the reference's
+// GEFEL building blocks (gauss_points, dx/dy/dz_bf_lin_hex_3d, derivatives_3d,
+// derivatives_2d, bf_lin_4_2d, bdbjac, bdbj) in the loop order of
+// MEFEL/SRC/linhex.cpp:583-640, MEFEL/SRC/plelemlq.cpp:611-690 and
+// TRFEL/SRC/linhext.cpp:774-840, because those element classes read MEFEL/TRFEL
+// globals (Mt, Mm, Tt, Tm) and cannot be called in isolation. The isotropic
+// material matrices (MEFEL/SRC/elastisomat.cpp:255-266,395-410 and
+// TRFEL/SRC/isotrmat.cpp:124-135) are restated here for the same reason.
+#include <math.h>
+#include <string.h>
+#include "scr.h"
+#include "cr.h"
+#include "precond.h"
+#include "matrix.h"
+#include "vector.h"
+#include "basefun.h"
+#include "difcalc.h"
+#include "intp.h"
+
+extern "C" {
+
+// ---------------------------------------------------------------- symcomprow
+void *ref_scr_create(long n, long ne, long ndofe, const long *cn)
+{
+  symcomprow *s = new symcomprow();
+  long *c = new long[ndofe];
+  s->allocadr(n);
+  for (long e = 0; e < ne; e++) {
+    memcpy(c, cn + e * ndofe, ndofe * sizeof(long));
+    s->numcontr_elem(c, ndofe);
+  }
+  s->addresses();
+  for (long e = 0; e < ne; e++) {
+    memcpy(c, cn + e * ndofe, ndofe * sizeof(long));
+    s->fillarray_elem(c, ndofe);
+  }
+  s->sort_and_colindex_tko();
+  delete[] c;
+  return s;
+}
+long ref_scr_negm(void *h) { return ((symcomprow *)h)->negm; }
+void ref_scr_get(void *h, long *adr, long *ci, double *a)
+{
+  symcomprow *s = (symcomprow *)h;
+  if (adr) memcpy(adr, s->adr, (s->n + 1) * sizeof(long));
+  if (ci) memcpy(ci, s->ci, s->negm * sizeof(long));
+  if (a) memcpy(a, s->a, s->negm * sizeof(double));
+}
+void ref_scr_set_a(void *h, const double *a)
+{
+  symcomprow *s = (symcomprow *)h;
+  memcpy(s->a, a, s->negm * sizeof(double));
+}
+void ref_scr_localized(void *h, double *lm, long *cn, long nc)
+{
+  ((symcomprow *)h)->localized(lm, cn, nc);
+}
+void ref_scr_mxv(void *h, double *b, double *c) { ((symcomprow *)h)->mxv_scr(b, c); }
+void ref_scr_cg_prec(void *h, double *x, double *y, long ni, double res, long *ani,
+                     double *ares, double zero, long iv, long tprec)
+{
+  long a = *ani;
+  double r = *ares;
+  ((symcomprow *)h)->cg_prec(x, y, ni, res, a, r, zero, iv, tprec, 0.0, NULL);
+  *ani = a;
+  *ares = r;
+}
+void ref_scr_cg(void *h, double *x, double *y, long ni, double res, long *ani,
+                double *ares, double zero, long iv)
+{
+  long a = *ani;
+  double r = *ares;
+  ((symcomprow *)h)->cg(x, y, ni, res, a, r, zero, iv);
+  *ani = a;
+  *ares = r;
+}
+void ref_scr_free(void *h) { delete (symcomprow *)h; }
+
+// ------------------------------------------------------------------ comprow
+void *ref_cr_create(long n, long ne, long ndofe, const long *cn)
+{
+  comprow *s = new comprow();
+  long *c = new long[ndofe];
+  s->allocadr(n);
+  for (long e = 0; e < ne; e++) {
+    memcpy(c, cn + e * ndofe, ndofe * sizeof(long));
+    s->numcontr_elem(c, ndofe);
+  }
+  s->addresses();
+  for (long e = 0; e < ne; e++) {
+    memcpy(c, cn + e * ndofe, ndofe * sizeof(long));
+    s->fillarray_elem(c, ndofe);
+  }
+  s->sort_and_colindex();
+  delete[] c;
+  return s;
+}
+long ref_cr_negm(void *h) { return ((comprow *)h)->negm; }
+void ref_cr_get(void *h, long *adr, long *ci, double *a)
+{
+  comprow *s = (comprow *)h;
+  if (adr) memcpy(adr, s->adr, (s->n + 1) * sizeof(long));
+  if (ci) memcpy(ci, s->ci, s->negm * sizeof(long));
+  if (a) memcpy(a, s->a, s->negm * sizeof(double));
+}
+void ref_cr_localized(void *h, double *lm, long *cn, long nc)
+{
+  ((comprow *)h)->localized(lm, cn, nc);
+}
+void ref_cr_mxv(void *h, double *b, double *c) { ((comprow *)h)->mxv_cr(b, c); }
+void ref_cr_cg_prec_noprec(void *h, double *x, double *y, long ni, double res, long *ani,
+                           double *ares, double zero, long iv)
+{
+  comprow *s = (comprow *)h;
+  precond pr;
+  pr.pt = noprecond;
+  pr.n = s->n;
+  long a = *ani;
+  double r = *ares;
+  s->cg_prec(pr, x, y, ni, res, a, r, zero, iv);
+  *ani = a;
+  *ares = r;
+}
+void ref_cr_free(void *h) { delete (comprow *)h; }
+
+// ------------------------------------------------------------ element routines
+// hex8 linear elasticity, order of MEFEL/SRC/linhex.cpp:583-640 with geom_matrix :153-186
+void ref_hex8_elast_ke(const double *xc, const double *yc, const double *zc, double e,
+                       double nu, double *ke)
+{
+  const long nne = 8, ndofe = 24;
+  vector x(nne), y(nne), z(nne), w(2), gp(2);
+  vector dx(nne), dy(nne), dz(nne);
+  matrix gm(6, ndofe), d(6, 6), sm(ndofe, ndofe);
+  for (long i = 0; i < nne; i++) { x[i] = xc[i]; y[i] = yc[i]; z[i] = zc[i]; }
+  // elastisomat::matstiff_spacestr
+  double g = e / 2.0 / (1.0 + nu);
+  double s = e / (1.0 + nu) / (1.0 - 2.0 * nu);
+  fillm(0.0, d);
+  d[0][0] = s * (1 - nu); d[0][1] = s * nu; d[0][2] = s * nu;
+  d[1][0] = d[0][1]; d[1][1] = d[0][0]; d[1][2] = d[0][1];
+  d[2][0] = d[0][1]; d[2][1] = d[0][1]; d[2][2] = d[0][0];
+  d[3][3] = g; d[4][4] = g; d[5][5] = g;
+
+  nullm(sm);
+  gauss_points(gp.a, w.a, 2);
+  for (long i = 0; i < 2; i++) {
+    double xi = gp[i];
+    for (long j = 0; j < 2; j++) {
+      double eta = gp[j];
+      for (long k = 0; k < 2; k++) {
+        double zeta = gp[k], jac;
+        dx_bf_lin_hex_3d(dx.a, eta, zeta);
+        dy_bf_lin_hex_3d(dy.a, xi, zeta);
+        dz_bf_lin_hex_3d(dz.a, xi, eta);
+        derivatives_3d(dx, dy, dz, jac, x, y, z, xi, eta, zeta);
+        nullm(gm);
+        long jj = 0, kk = 1, ll = 2;
+        for (long a = 0; a < nne; a++) {
+          gm[0][jj] = dx[a]; gm[1][kk] = dy[a]; gm[2][ll] = dz[a];
+          gm[3][kk] = dz[a]; gm[3][ll] = dy[a];
+          gm[4][jj] = dz[a]; gm[4][ll] = dx[a];
+          gm[5][jj] = dy[a]; gm[5][kk] = dx[a];
+          jj += 3; kk += 3; ll += 3;
+        }
+        jac *= w[i] * w[j] * w[k];
+        if (jac < 0.0) jac = fabs(jac);
+        bdbjac(sm, gm, d, gm, jac);
+      }
+    }
+  }
+  for (long i = 0; i < ndofe; i++)
+    for (long j = 0; j < ndofe; j++) ke[i * ndofe + j] = sm[i][j];
+}
+
+// quad4 plane stress, order of MEFEL/SRC/plelemlq.cpp:611-690 with geom_matrix :294-314
+void ref_quad4_pstress_ke(const double *xc, const double *yc, double e, double nu,
+                          const double *thick_nodes, double *ke)
+{
+  const long nne = 4, ndofe = 8;
+  vector x(nne), y(nne), t(nne), w(2), gp(2), dx(nne), dy(nne), bf(nne);
+  matrix gm(3, ndofe), d(3, 3), sm(ndofe, ndofe);
+  for (long i = 0; i < nne; i++) { x[i] = xc[i]; y[i] = yc[i]; t[i] = thick_nodes[i]; }
+  // elastisomat::matstiff_plstress
+  double c = e / (1.0 - nu * nu);
+  fillm(0.0, d);
+  d[0][0] = c; d[0][1] = c * nu; d[0][2] = 0.0;
+  d[1][0] = c * nu; d[1][1] = c; d[1][2] = 0.0;
+  d[2][0] = 0.0; d[2][1] = 0.0; d[2][2] = e / 2.0 / (1.0 + nu);
+
+  nullm(sm);
+  gauss_points(gp.a, w.a, 2);
+  for (long i = 0; i < 2; i++) {
+    double xi = gp[i];
+    for (long j = 0; j < 2; j++) {
+      double eta = gp[j], jac, thick;
+      dx_bf_lin_4_2d(dx.a, eta);
+      dy_bf_lin_4_2d(dy.a, xi);
+      derivatives_2d(dx, dy, jac, x, y, xi, eta);
+      nullm(gm);
+      long i1 = 0, i2 = 1;
+      for (long a = 0; a < nne; a++) {
+        gm[0][i1] = dx[a]; gm[1][i2] = dy[a];
+        gm[2][i1] = dy[a]; gm[2][i2] = dx[a];
+        i1 += 2; i2 += 2;
+      }
+      // planeelemlq::approx = scprd(bf_lin_4_2d, nodal thickness)
+      bf_lin_4_2d(bf.a, xi, eta);
+      scprd(bf, t, thick);
+      jac *= thick * w[i] * w[j];
+      if (jac < 0.0) jac = fabs(jac);
+      bdbjac(sm, gm, d, gm, jac);
+    }
+  }
+  for (long i = 0; i < ndofe; i++)
+    for (long j = 0; j < ndofe; j++) ke[i * ndofe + j] = sm[i][j];
+}
+
+// hex8 heat conduction, order of TRFEL/SRC/linhext.cpp:774-840 with grad_matrix :743-761
+void ref_hex8_heat_ke(const double *xc, const double *yc, const double *zc, double kk,
+                      double *ke)
+{
+  const long nne = 8;
+  vector x(nne), y(nne), z(nne), w(2), gp(2), dx(nne), dy(nne), dz(nne);
+  matrix gm(3, nne), d(3, 3), km(nne, nne), dc(1, 3), nmat(1, nne);
+  for (long i = 0; i < nne; i++) { x[i] = xc[i]; y[i] = yc[i]; z[i] = zc[i]; }
+  nullm(km);
+  gauss_points(gp.a, w.a, 2);
+  for (long i = 0; i < 2; i++) {
+    double xi = gp[i], ww1 = w[i];
+    for (long j = 0; j < 2; j++) {
+      double eta = gp[j], ww2 = w[j];
+      for (long k = 0; k < 2; k++) {
+        double zeta = gp[k], ww3 = w[k], jac;
+        dx_bf_lin_hex_3d(dx.a, eta, zeta);
+        dy_bf_lin_hex_3d(dy.a, xi, zeta);
+        dz_bf_lin_hex_3d(dz.a, xi, eta);
+        derivatives_3d(dx, dy, dz, jac, x, y, z, xi, eta, zeta);
+        nullm(gm);
+        for (long a = 0; a < nne; a++) { gm[0][a] = dx[a]; gm[1][a] = dy[a]; gm[2][a] = dz[a]; }
+        // isotrmat::matcond3d
+        fillm(0.0, d);
+        d[0][0] = kk; d[1][1] = kk; d[2][2] = kk;
+        if (jac < 0.0) jac = fabs(jac);
+        jac *= ww1 * ww2 * ww3;
+        bdbj(km.a, gm.a, d.a, jac, gm.m, gm.n);
+        // convective term: matcond2 leaves the zeroed 1x3 matrix untouched for an
+        // isotropic medium, so this adds exact zeros (kept for identical rounding)
+        nullm(dc);
+        bf_lin_hex_3d(nmat.a, xi, eta, zeta);
+        bdbjac(km, nmat, dc, gm, jac);
+      }
+    }
+  }
+  for (long i = 0; i < nne; i++)
+    for (long j = 0; j < nne; j++) ke[i * nne + j] = km[i][j];
+}
+
+}  // extern "C"
