@@ -1,0 +1,165 @@
+/*
+ * ssf_gpu.h -- C ABI of the MI355X (gfx950) linear-system path of SIFEL.
+ *
+ * The path: element stiffness / conductivity evaluation, assembly ("localize")
+ * into GEFEL compressed-row storage (symcomprow = symmetric lower rows with the
+ * diagonal last, comprow = full rows), and the conjugate-gradient solve behind
+ * gmatrix::solve_system. Host callers keep the gmatrix API (GEFEL/gmatrix.h:29-153);
+ * a gmatrix built on this library forwards each member below to one entry point.
+ *
+ * Conventions: plain pointers and sizes, 64-bit integers as the reference's
+ * `long`, code numbers 1-based with entries <= 0 skipped (GEFEL/scr.cpp:427-433).
+ * Every function returns SSF_OK (0) or a negative error code; ssf_last_error()
+ * gives the message (per calling thread). Host pointers are read/written
+ * synchronously; *_dev entry points take device pointers and are asynchronous
+ * on the context's stream. A context is used by one host thread at a time.
+ */
+#ifndef SSF_GPU_H
+#define SSF_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define SSF_OK 0
+#define SSF_EINVAL -1     /* bad argument (size, pointer, storage type)   */
+#define SSF_EHIP -2       /* HIP runtime error                           */
+#define SSF_ENOMEM -3     /* device memory or HBM budget exhausted        */
+#define SSF_ESTATE -4     /* call out of order (e.g. solve before values) */
+#define SSF_EUNSUP -5     /* combination the reference does not support   */
+
+/* identifiers kept from GEFEL/galias.h:59-62,74-77,103 */
+#define SSF_STORAGE_COMPRESSED_ROWS 10 /* comprow            */
+#define SSF_STORAGE_SYMM_COMP_ROWS 11  /* symcomprow        */
+#define SSF_SOLVER_CG 20               /* linsolvertype cg   */
+#define SSF_PREC_NOPRECOND 0           /* precondtype        */
+#define SSF_PREC_DIAG 1                /* diagprec (Jacobi)  */
+
+/* element kinds of the batched element kernels */
+#define SSF_ELEM_HEX8_ELAST 0  /* MEFEL linhex, 24x24, 3 DOF/node            */
+#define SSF_ELEM_HEX8_HEAT 1   /* TRFEL linhext with isotrmat, 8x8            */
+#define SSF_ELEM_QUAD4_PSTRESS 2 /* MEFEL planeelemlq, plane stress, 8x8      */
+
+/* arithmetic of a solve */
+#define SSF_PRECISION_FP64 0   /* plain FP64 PCG, recurrence of the reference   */
+#define SSF_PRECISION_MIXED 1  /* FP32 inner PCG inside FP64 iterative refinement */
+
+typedef struct ssf_ctx ssf_ctx;
+typedef struct ssf_matrix ssf_matrix;
+
+typedef struct ssf_solve_params {
+  int32_t linsolver;   /* SSF_SOLVER_CG                                          */
+  int32_t prec;        /* SSF_PREC_NOPRECOND / SSF_PREC_DIAG                      */
+  int64_t ni;          /* maximum number of iterations (gmatrix::ni)              */
+  double res;          /* required residual norm (gmatrix::res); squared test for
+                          symcomprow + diagprec as GEFEL/scr.cpp:1188             */
+  double zero;         /* computer zero (gmatrix::zero, 1e-30)                    */
+  int32_t iv;          /* 0: start from x = 0, 1: start from lhs                  */
+  int32_t precision;   /* SSF_PRECISION_*                                         */
+  double inner_res;    /* mixed: relative residual reduction of each inner solve  */
+  int64_t inner_ni;    /* mixed: maximum inner iterations per refinement step     */
+  int64_t max_outer;   /* mixed: maximum refinement steps                         */
+} ssf_solve_params;
+
+typedef struct ssf_solve_info {
+  int64_t ani;         /* as gmatrix::ani (left unchanged when ni is exhausted)   */
+  double ares;         /* as gmatrix::ares                                        */
+  int32_t status;      /* 1 converged, 2 zero denominator, 3 zero rhs, 0 ni spent,
+                          4 mixed: FP64 residual floor reached above res */
+  int32_t pad;
+  int64_t iterations;  /* SpMV-iterations actually run (inner ones for mixed)    */
+  int64_t outer;       /* mixed: refinement steps                                 */
+  double rel_residual; /* final ||b - A x|| / ||b|| in FP64                        */
+} ssf_solve_info;
+
+/* ------------------------------------------------------------------ context */
+const char *ssf_last_error(void);
+int ssf_ctx_create(int device, ssf_ctx **out);
+int ssf_ctx_destroy(ssf_ctx *ctx);
+int ssf_ctx_sync(ssf_ctx *ctx);
+/* the context's HIP stream (hipStream_t), for callers that enqueue around it */
+void *ssf_ctx_stream(ssf_ctx *ctx);
+
+/* ------------------------------------------------------------ storage setup */
+/* symcomprow::initiate / comprow::initiate (GEFEL/scr.cpp:556, GEFEL/cr.cpp:454,
+ * reached through gmatrix::initiate GEFEL/gmatrix.cpp:268): symbolic pattern
+ * from element code numbers cn[ne*ndofe]. storage = SSF_STORAGE_*. */
+int ssf_matrix_create_elem(ssf_ctx *ctx, int storage, int64_t n, int64_t ne,
+                           int64_t ndofe, const int64_t *cn, ssf_matrix **out);
+/* storage from existing reference arrays adr[n+1], ci[adr[n]] (0-based columns;
+ * symcomprow rows ascending with the diagonal last). */
+int ssf_matrix_create_csr(ssf_ctx *ctx, int storage, int64_t n, const int64_t *adr,
+                          const int64_t *ci, ssf_matrix **out);
+int ssf_matrix_destroy(ssf_matrix *m);
+/* number of stored entries in the reference layout (negm) and n */
+int ssf_matrix_sizes(const ssf_matrix *m, int64_t *n, int64_t *negm, int64_t *nnz_full);
+/* reference-layout arrays adr[n+1], ci[negm] (either may be NULL) */
+int ssf_matrix_pattern(const ssf_matrix *m, int64_t *adr, int64_t *ci);
+
+/* --------------------------------------------------------------- values */
+/* symcomprow::nullmat / comprow::nullmat (GEFEL/scr.cpp:100) */
+int ssf_matrix_zero(ssf_matrix *m);
+/* a[negm] in the reference layout -> device (replaces the host-side array a) */
+int ssf_matrix_set_values(ssf_matrix *m, const double *a);
+/* device -> a[negm] in the reference layout */
+int ssf_matrix_get_values(const ssf_matrix *m, double *a);
+/* gmatrix::localized (GEFEL/gmatrix.cpp:581) for a batch of element matrices:
+ * lm[ne][nc*nc] row-major added into the device values in element order of the
+ * colouring (symcomprow keeps ii>=jj, comprow all). cn[ne*nc] 1-based. */
+int ssf_matrix_localize_batch(ssf_matrix *m, int64_t ne, int64_t nc, const int64_t *cn,
+                              const double *lm);
+/* Batched element kernel + atomic-free coloured assembly straight into the device
+ * values (stiffness_matrix MEFEL/SRC/globmat.cpp:41 / conductivity_matrix
+ * TRFEL/SRC/globmatt.cpp:19 with linhex / linhext / planeelemlq kernels).
+ * xyz[nn*3] node coordinates (z ignored for quad4), conn[ne*nne] 0-based nodes,
+ * cn[ne*ndofe] code numbers, mat_a = E (elasticity) or k (heat), mat_b = nu,
+ * thick = thickness (quad4, may be NULL = 1). The pattern must have been built
+ * from the same cn. rhs_lift (may be NULL, n doubles, host) receives
+ * -K_e u_p for prescribed DOFs (cn<0 -> pv[-cn-1]) as assemble_init
+ * (TRFEL/SRC/globmatt.cpp:1635) computes. Values are accumulated (call
+ * ssf_matrix_zero first for a fresh assembly). */
+int ssf_matrix_assemble(ssf_matrix *m, int kind, int64_t nn, const double *xyz, int64_t ne,
+                        const int64_t *conn, const int64_t *cn, const double *mat_a,
+                        const double *mat_b, const double *thick, const double *pv,
+                        int64_t npv, double *rhs_lift);
+
+/* -------------------------------------------------------------- operations */
+/* gmatrix::gmxv (GEFEL/gmatrix.cpp:1704): b = A a, host vectors of length n */
+int ssf_gmxv(ssf_matrix *m, const double *a, double *b);
+/* same on device pointers, asynchronous */
+int ssf_gmxv_dev(ssf_matrix *m, const double *a_dev, double *b_dev);
+/* gmatrix::solve_system (GEFEL/gmatrix.cpp:1049), tlinsol = cg: dispatch on the
+ * storage and preconditioner as the reference does (symcomprow + diagprec ->
+ * symcomprow::cg_prec GEFEL/scr.cpp:1093; symcomprow + noprecond ->
+ * symcomprow::cg :786; comprow -> comprow::cg_prec GEFEL/cr.cpp:1124 with the
+ * noprecond preconditioner). lhs/rhs host vectors of length n. */
+int ssf_solve(ssf_matrix *m, const ssf_solve_params *p, const double *rhs, double *lhs,
+              ssf_solve_info *info);
+/* same with device vectors (rhs_dev read-only, lhs_dev in/out), synchronous */
+int ssf_solve_dev(ssf_matrix *m, const ssf_solve_params *p, const double *rhs_dev,
+                  double *lhs_dev, ssf_solve_info *info);
+
+/* ------------------------------------------------------------ diagnostics */
+/* Time `reps` launches of the FP64 (precision 0) or FP32 (1) SpMV on device
+ * vectors; returns the mean kernel time in microseconds measured with HIP
+ * events on the context's stream. */
+int ssf_spmv_bench(ssf_matrix *m, int precision, int reps, double *usec);
+/* SpMV autotuner: time every (lanes per row, blocks) tiling on this matrix for
+ * the FP64 (precision 0) or FP32 (1) SpMV and keep the fastest for later
+ * solves. The chosen tiling is returned (either pointer may be NULL). */
+int ssf_matrix_autotune(ssf_matrix *m, int precision, int reps, int *lanes, int *blocks);
+/* pin the SpMV tiling (lanes per row in {4,8,16,32,64}, blocks in [1, 2048]);
+ * results of a solve depend only on the matrix, the right-hand side and this
+ * tiling */
+int ssf_matrix_set_tune(ssf_matrix *m, int precision, int lanes, int blocks);
+/* bytes the SpMV of the given precision streams by the canonical count
+ * nnz*(s_v+4) + 4(n+1) + 2 s_v n, and the bytes of the format actually used */
+int ssf_spmv_bytes(const ssf_matrix *m, int precision, double *canonical, double *actual);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1010 @@
+// C ABI of libssf_gpu: storage setup, values, coloured assembly, SpMV and the
+// CG solves behind gmatrix::solve_system (see include/ssf_gpu.h).
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <thread>
+
+#include "ssf_kernels.hpp"
+
+namespace ssf {
+
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+
+static int host_threads()
+{
+  unsigned h = std::thread::hardware_concurrency();
+  if (h == 0) h = 4;
+  return (int)std::min(16u, h);
+}
+
+template <typename F>
+static void parallel_for(int64_t n, F f)
+{
+  const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(host_threads(), n / 4096 + 1));
+  if (nt == 1) {
+    f(0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; t++) {
+    const int64_t b = n * t / nt, e = n * (t + 1) / nt;
+    th.emplace_back([=]() { f(b, e); });
+  }
+  for (auto &x : th) x.join();
+}
+
+template <typename T>
+static hipError_t dalloc(T **p, size_t count)
+{
+  *p = nullptr;
+  if (count == 0) count = 1;
+  return hipMalloc((void **)p, count * sizeof(T));
+}
+
+template <typename T>
+static void dfree(T *&p)
+{
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+// ------------------------------------------------------------ patterns (host)
+
+// symcomprow::initiate / comprow::initiate semantics (GEFEL/scr.cpp:112-330,
+// GEFEL/cr.cpp:140-320): for every row ii the ascending, duplicate-free set of
+// jj over all element pairs with both code numbers > 0 (and jj <= ii for the
+// symmetric lower storage). Built per row from a DOF -> element incidence list.
+static int pattern_from_elements(int storage, int64_t n, int64_t ne, int64_t ndofe,
+                                 const int64_t *cn, std::vector<int64_t> &adr,
+                                 std::vector<int64_t> &ci)
+{
+  const bool lower = storage == SSF_STORAGE_SYMM_COMP_ROWS;
+  std::vector<int64_t> deg(n + 1, 0);
+  for (int64_t k = 0; k < ne * ndofe; k++) {
+    const int64_t c = cn[k];
+    SSF_CHECK(c <= n, SSF_EINVAL, "code number larger than the number of unknowns");
+    if (c > 0) deg[c - 1]++;
+  }
+  std::vector<int64_t> inc_ptr(n + 1, 0);
+  for (int64_t i = 0; i < n; i++) inc_ptr[i + 1] = inc_ptr[i] + deg[i];
+  std::vector<int64_t> inc(inc_ptr[n]);
+  std::vector<int64_t> fillp(inc_ptr.begin(), inc_ptr.end() - 1);
+  for (int64_t e = 0; e < ne; e++)
+    for (int64_t i = 0; i < ndofe; i++) {
+      const int64_t c = cn[e * ndofe + i];
+      if (c > 0) inc[fillp[c - 1]++] = e;
+    }
+  // row lengths, then columns
+  std::vector<int64_t> len(n, 0);
+  std::vector<std::vector<int64_t>> rows_tmp;  // unused; rows are rebuilt below
+  auto row_cols = [&](int64_t ii, std::vector<int64_t> &buf) {
+    buf.clear();
+    for (int64_t q = inc_ptr[ii]; q < inc_ptr[ii + 1]; q++) {
+      const int64_t e = inc[q];
+      for (int64_t j = 0; j < ndofe; j++) {
+        const int64_t jj = cn[e * ndofe + j] - 1;
+        if (jj < 0) continue;
+        if (lower && jj > ii) continue;
+        buf.push_back(jj);
+      }
+    }
+    std::sort(buf.begin(), buf.end());
+    buf.erase(std::unique(buf.begin(), buf.end()), buf.end());
+  };
+  parallel_for(n, [&](int64_t b, int64_t e) {
+    std::vector<int64_t> buf;
+    for (int64_t i = b; i < e; i++) {
+      row_cols(i, buf);
+      len[i] = (int64_t)buf.size();
+    }
+  });
+  adr.assign(n + 1, 0);
+  for (int64_t i = 0; i < n; i++) adr[i + 1] = adr[i] + len[i];
+  ci.assign(adr[n], 0);
+  parallel_for(n, [&](int64_t b, int64_t e) {
+    std::vector<int64_t> buf;
+    for (int64_t i = b; i < e; i++) {
+      row_cols(i, buf);
+      std::copy(buf.begin(), buf.end(), ci.begin() + adr[i]);
+    }
+  });
+  return SSF_OK;
+}
+
+// ------------------------------------------------------------- SpMV tuning
+
+// Deterministic default: lanes per row from the mean row length, blocks from n.
+static SpmvTune default_tune(int64_t n, int64_t nnz)
+{
+  SpmvTune t;
+  const double avg = n ? (double)nnz / (double)n : 1.0;
+  t.lanes = avg <= 6 ? 4 : avg <= 24 ? 8 : avg <= 56 ? 16 : avg <= 160 ? 32 : 64;
+  const int64_t rows_per_group_min = 4;
+  const int64_t groups = TPB / t.lanes;
+  int64_t nb = (n + groups * rows_per_group_min - 1) / (groups * rows_per_group_min);
+  nb = std::max<int64_t>(8, std::min<int64_t>(2048, nb));
+  nb = (nb + 7) / 8 * 8;
+  t.blocks = (int)nb;
+  return t;
+}
+
+}  // namespace ssf
+
+using namespace ssf;
+
+// =================================================================== C ABI
+
+extern "C" const char *ssf_last_error(void) { return g_err.c_str(); }
+
+extern "C" int ssf_ctx_create(int device, ssf_ctx **out)
+{
+  SSF_ENTER();
+  SSF_CHECK(out, SSF_EINVAL, "ssf_ctx_create: out is NULL");
+  int count = 0;
+  SSF_HIP(hipGetDeviceCount(&count));
+  SSF_CHECK(device >= 0 && device < count, SSF_EINVAL, "ssf_ctx_create: no such device");
+  SSF_HIP(hipSetDevice(device));
+  ssf_ctx *c = new ssf_ctx();
+  c->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cus = prop.multiProcessorCount;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    SSF_HIP(e);
+  }
+  *out = c;
+  return SSF_OK;
+}
+
+extern "C" int ssf_ctx_destroy(ssf_ctx *ctx)
+{
+  SSF_ENTER();
+  if (!ctx) return SSF_OK;
+  SSF_CHECK(ctx->live_matrices == 0, SSF_ESTATE,
+            "ssf_ctx_destroy: matrices created on this context are still alive");
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return SSF_OK;
+}
+
+extern "C" int ssf_ctx_sync(ssf_ctx *ctx)
+{
+  SSF_ENTER();
+  SSF_CHECK(ctx, SSF_EINVAL, "ssf_ctx_sync: NULL context");
+  SSF_HIP(hipStreamSynchronize(ctx->stream));
+  return SSF_OK;
+}
+
+extern "C" void *ssf_ctx_stream(ssf_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+static void free_workspace(Workspace &w)
+{
+  dfree(w.x); dfree(w.r); dfree(w.d); dfree(w.p); dfree(w.h); dfree(w.y);
+  dfree(w.x32); dfree(w.r32); dfree(w.d32); dfree(w.p32); dfree(w.h32); dfree(w.y32);
+  dfree(w.part); dfree(w.part32); dfree(w.st); dfree(w.st32);
+  if (w.st_host) (void)hipHostFree(w.st_host);
+  if (w.st32_host) (void)hipHostFree(w.st32_host);
+  w.st_host = nullptr;
+  w.st32_host = nullptr;
+  w.n = 0;
+}
+
+extern "C" int ssf_matrix_destroy(ssf_matrix *m)
+{
+  SSF_ENTER();
+  if (!m) return SSF_OK;
+  (void)hipSetDevice(m->ctx->device);
+  (void)hipStreamSynchronize(m->ctx->stream);
+  if (m->a_ref == m->val) m->a_ref = nullptr;  // comprow: the same allocation
+  dfree(m->a_ref);
+  dfree(m->rowptr); dfree(m->col); dfree(m->val); dfree(m->val32);
+  dfree(m->diag); dfree(m->diag32);
+  dfree(m->map_lo); dfree(m->map_up);
+  dfree(m->slot); dfree(m->order);
+  free_workspace(m->ws);
+  m->ctx->live_matrices--;
+  delete m;
+  return SSF_OK;
+}
+
+extern "C" int ssf_matrix_create_csr(ssf_ctx *ctx, int storage, int64_t n, const int64_t *adr,
+                                     const int64_t *ci, ssf_matrix **out)
+{
+  SSF_ENTER();
+  SSF_CHECK(ctx && out && adr && (ci || n == 0), SSF_EINVAL, "ssf_matrix_create_csr: NULL argument");
+  SSF_CHECK(storage == SSF_STORAGE_SYMM_COMP_ROWS || storage == SSF_STORAGE_COMPRESSED_ROWS,
+            SSF_EUNSUP, "ssf_matrix_create_csr: storage must be symm_comp_rows (11) or compressed_rows (10)");
+  SSF_CHECK(n >= 0 && n < (1LL << 31), SSF_EINVAL, "ssf_matrix_create_csr: n out of range");
+  SSF_CHECK(adr[0] == 0, SSF_EINVAL, "ssf_matrix_create_csr: adr[0] must be 0");
+  const int64_t negm = adr[n];
+  const bool sym = storage == SSF_STORAGE_SYMM_COMP_ROWS;
+  // validation: ascending, in range; symcomprow rows end with the diagonal
+  for (int64_t i = 0; i < n; i++) {
+    SSF_CHECK(adr[i + 1] >= adr[i], SSF_EINVAL, "ssf_matrix_create_csr: adr not monotone");
+    for (int64_t k = adr[i]; k < adr[i + 1]; k++) {
+      SSF_CHECK(ci[k] >= 0 && ci[k] < n, SSF_EINVAL, "ssf_matrix_create_csr: column index out of range");
+      if (k > adr[i]) SSF_CHECK(ci[k] > ci[k - 1], SSF_EINVAL, "ssf_matrix_create_csr: columns not ascending");
+      if (sym) SSF_CHECK(ci[k] <= i, SSF_EINVAL, "ssf_matrix_create_csr: symm_comp_rows holds the lower triangle only");
+    }
+    if (sym)
+      SSF_CHECK(adr[i + 1] > adr[i] && ci[adr[i + 1] - 1] == i, SSF_EINVAL,
+                "ssf_matrix_create_csr: every symm_comp_rows row must end with its diagonal");
+  }
+  SSF_HIP(hipSetDevice(ctx->device));
+  ssf_matrix *m = new ssf_matrix();
+  m->ctx = ctx;
+  ctx->live_matrices++;
+  m->storage = storage;
+  m->n = n;
+  m->negm = negm;
+  m->adr.assign(adr, adr + n + 1);
+  m->ci.assign(ci, ci + negm);
+  std::vector<int> rowptr(n + 1), col, lo, up;
+  if (!sym) {
+    for (int64_t i = 0; i <= n; i++) rowptr[i] = (int)adr[i];
+    col.resize(negm);
+    for (int64_t k = 0; k < negm; k++) col[k] = (int)ci[k];
+    m->nnz = negm;
+  } else {
+    std::vector<int64_t> cnt(n, 0);
+    for (int64_t i = 0; i < n; i++)
+      for (int64_t k = adr[i]; k < adr[i + 1]; k++) {
+        cnt[i]++;
+        if (ci[k] != i) cnt[ci[k]]++;
+      }
+    int64_t tot = 0;
+    for (int64_t i = 0; i < n; i++) {
+      rowptr[i] = (int)tot;
+      tot += cnt[i];
+    }
+    rowptr[n] = (int)tot;
+    if (tot >= (1LL << 31)) {
+      delete m;
+      set_error("ssf_matrix_create_csr: more than 2^31 entries in the full matrix");
+      return SSF_EUNSUP;
+    }
+    m->nnz = tot;
+    col.resize(tot);
+    lo.resize(negm);
+    up.resize(negm);
+    // full row i = lower part of scr row i (ascending, diagonal last), then the
+    // upper part: rows j > i holding column i, visited in ascending j
+    std::vector<int64_t> pos(n);
+    for (int64_t i = 0; i < n; i++) pos[i] = rowptr[i] + (adr[i + 1] - adr[i]);
+    for (int64_t j = 0; j < n; j++)
+      for (int64_t k = adr[j]; k < adr[j + 1]; k++) {
+        const int64_t s = rowptr[j] + (k - adr[j]);
+        col[s] = (int)ci[k];
+        lo[k] = (int)s;
+        if (ci[k] != j) {
+          const int64_t u = pos[ci[k]]++;
+          col[u] = (int)j;
+          up[k] = (int)u;
+        } else {
+          up[k] = -1;
+        }
+      }
+  }
+  hipError_t e = hipSuccess;
+  auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
+  chk(dalloc(&m->rowptr, n + 1));
+  chk(dalloc(&m->col, m->nnz));
+  chk(dalloc(&m->val, m->nnz));
+  chk(dalloc(&m->diag, n));
+  if (sym) {
+    chk(dalloc(&m->a_ref, negm));
+    chk(dalloc(&m->map_lo, negm));
+    chk(dalloc(&m->map_up, negm));
+  } else {
+    m->a_ref = m->val;
+  }
+  if (e == hipSuccess) {
+    chk(hipMemcpy(m->rowptr, rowptr.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice));
+    if (m->nnz) chk(hipMemcpy(m->col, col.data(), m->nnz * sizeof(int), hipMemcpyHostToDevice));
+    if (sym && negm) {
+      chk(hipMemcpy(m->map_lo, lo.data(), negm * sizeof(int), hipMemcpyHostToDevice));
+      chk(hipMemcpy(m->map_up, up.data(), negm * sizeof(int), hipMemcpyHostToDevice));
+    }
+    chk(hipMemsetAsync(m->a_ref, 0, std::max<int64_t>(1, negm) * sizeof(double), ctx->stream));
+    chk(hipMemsetAsync(m->val, 0, std::max<int64_t>(1, m->nnz) * sizeof(double), ctx->stream));
+    chk(hipMemsetAsync(m->diag, 0, std::max<int64_t>(1, n) * sizeof(double), ctx->stream));
+    chk(hipStreamSynchronize(ctx->stream));
+  }
+  if (e != hipSuccess) {
+    const std::string msg = std::string("ssf_matrix_create_csr: ") + hipGetErrorString(e);
+    ssf_matrix_destroy(m);
+    set_error(msg);
+    return e == hipErrorOutOfMemory ? SSF_ENOMEM : SSF_EHIP;
+  }
+  m->tune64 = default_tune(n, m->nnz);
+  m->tune32 = m->tune64;
+  *out = m;
+  return SSF_OK;
+}
+
+extern "C" int ssf_matrix_create_elem(ssf_ctx *ctx, int storage, int64_t n, int64_t ne,
+                                      int64_t ndofe, const int64_t *cn, ssf_matrix **out)
+{
+  SSF_ENTER();
+  SSF_CHECK(ctx && out && (cn || ne == 0), SSF_EINVAL, "ssf_matrix_create_elem: NULL argument");
+  SSF_CHECK(storage == SSF_STORAGE_SYMM_COMP_ROWS || storage == SSF_STORAGE_COMPRESSED_ROWS,
+            SSF_EUNSUP, "ssf_matrix_create_elem: storage must be symm_comp_rows (11) or compressed_rows (10)");
+  SSF_CHECK(n >= 0 && ne >= 0 && ndofe > 0, SSF_EINVAL, "ssf_matrix_create_elem: bad sizes");
+  std::vector<int64_t> adr, ci;
+  int rc = pattern_from_elements(storage, n, ne, ndofe, cn, adr, ci);
+  if (rc != SSF_OK) return rc;
+  if (storage == SSF_STORAGE_SYMM_COMP_ROWS)
+    for (int64_t i = 0; i < n; i++)
+      SSF_CHECK(adr[i + 1] > adr[i] && ci[adr[i + 1] - 1] == i, SSF_EINVAL,
+                "ssf_matrix_create_elem: unknown " + std::to_string(i + 1) + " belongs to no element");
+  return ssf_matrix_create_csr(ctx, storage, n, adr.data(), ci.data(), out);
+}
+
+extern "C" int ssf_matrix_sizes(const ssf_matrix *m, int64_t *n, int64_t *negm, int64_t *nnz_full)
+{
+  SSF_ENTER();
+  SSF_CHECK(m, SSF_EINVAL, "ssf_matrix_sizes: NULL matrix");
+  if (n) *n = m->n;
+  if (negm) *negm = m->negm;
+  if (nnz_full) *nnz_full = m->nnz;
+  return SSF_OK;
+}
+
+extern "C" int ssf_matrix_pattern(const ssf_matrix *m, int64_t *adr, int64_t *ci)
+{
+  SSF_ENTER();
+  SSF_CHECK(m, SSF_EINVAL, "ssf_matrix_pattern: NULL matrix");
+  if (adr) std::memcpy(adr, m->adr.data(), (m->n + 1) * sizeof(int64_t));
+  if (ci && m->negm) std::memcpy(ci, m->ci.data(), m->negm * sizeof(int64_t));
+  return SSF_OK;
+}
+
+// after the reference-layout values changed: full CSR values, diagonal
+static int values_changed(ssf_matrix *m)
+{
+  hipStream_t s = m->ctx->stream;
+  if (m->storage == SSF_STORAGE_SYMM_COMP_ROWS) SSF_HIP(expand_ref(m->negm, m->map_lo, m->map_up, m->a_ref, m->val, s));
+  SSF_HIP(extract_diag((int)m->n, m->rowptr, m->col, m->val, m->diag, nullptr, s));
+  m->values_set = true;
+  m->val32_valid = false;
+  return SSF_OK;
+}
+
+extern "C" int ssf_matrix_zero(ssf_matrix *m)
+{
+  SSF_ENTER();
+  SSF_CHECK(m, SSF_EINVAL, "ssf_matrix_zero: NULL matrix");
+  SSF_HIP(hipSetDevice(m->ctx->device));
+  SSF_HIP(hipMemsetAsync(m->a_ref, 0, std::max<int64_t>(1, m->negm) * sizeof(double), m->ctx->stream));
+  int rc = values_changed(m);
+  if (rc) return rc;
+  SSF_HIP(hipStreamSynchronize(m->ctx->stream));
+  return SSF_OK;
+}
+
+extern "C" int ssf_matrix_set_values(ssf_matrix *m, const double *a)
+{
+  SSF_ENTER();
+  SSF_CHECK(m && (a || m->negm == 0), SSF_EINVAL, "ssf_matrix_set_values: NULL argument");
+  SSF_HIP(hipSetDevice(m->ctx->device));
+  if (m->negm) SSF_HIP(hipMemcpyAsync(m->a_ref, a, m->negm * sizeof(double), hipMemcpyHostToDevice, m->ctx->stream));
+  int rc = values_changed(m);
+  if (rc) return rc;
+  SSF_HIP(hipStreamSynchronize(m->ctx->stream));
+  return SSF_OK;
+}
+
+extern "C" int ssf_matrix_get_values(const ssf_matrix *m, double *a)
+{
+  SSF_ENTER();
+  SSF_CHECK(m && (a || m->negm == 0), SSF_EINVAL, "ssf_matrix_get_values: NULL argument");
+  SSF_HIP(hipSetDevice(m->ctx->device));
+  if (m->negm) SSF_HIP(hipMemcpyAsync(a, m->a_ref, m->negm * sizeof(double), hipMemcpyDeviceToHost, m->ctx->stream));
+  SSF_HIP(hipStreamSynchronize(m->ctx->stream));
+  return SSF_OK;
+}
+
+// ----------------------------------------------------------------- assembly
+
+// Greedy first-fit colouring of the elements by shared nodes (at most 64
+// colours): elements of one colour share no node, hence no DOF and no matrix
+// entry, so one colour is assembled without atomics.
+static int colour_elements(int64_t nn, int64_t ne, int nne, const int64_t *conn,
+                           std::vector<int> &order, std::vector<int64_t> &start)
+{
+  std::vector<uint64_t> used(nn, 0);
+  std::vector<int> colour(ne);
+  int ncol = 0;
+  for (int64_t e = 0; e < ne; e++) {
+    uint64_t m = 0;
+    for (int a = 0; a < nne; a++) {
+      const int64_t nd = conn[e * nne + a];
+      SSF_CHECK(nd >= 0 && nd < nn, SSF_EINVAL, "element node index out of range");
+      m |= used[nd];
+    }
+    SSF_CHECK(~m != 0, SSF_EUNSUP, "element colouring needs more than 64 colours");
+    const int c = __builtin_ctzll(~m);
+    colour[e] = c;
+    ncol = std::max(ncol, c + 1);
+    for (int a = 0; a < nne; a++) used[conn[e * nne + a]] |= (1ULL << c);
+  }
+  start.assign(ncol + 1, 0);
+  for (int64_t e = 0; e < ne; e++) start[colour[e] + 1]++;
+  for (int c = 0; c < ncol; c++) start[c + 1] += start[c];
+  std::vector<int64_t> fillp(start.begin(), start.end() - 1);
+  order.resize(ne);
+  for (int64_t e = 0; e < ne; e++) order[fillp[colour[e]]++] = (int)e;
+  return SSF_OK;
+}
+
+// element set + slot map of the assembly (rebuilt when the code numbers change)
+static int prepare_assembly(ssf_matrix *m, int64_t nn, int64_t ne, int nne, int nd,
+                            const int64_t *conn, const int64_t *cn, bool aos,
+                            int **d_cn_out)
+{
+  hipStream_t s = m->ctx->stream;
+  const int64_t ncn = ne * nd;
+  bool same = m->slot && m->asm_ne == ne && m->asm_ndofe == nd &&
+              (int64_t)m->asm_cn_host.size() == ncn &&
+              std::equal(m->asm_cn_host.begin(), m->asm_cn_host.end(), cn);
+  std::vector<int> cn32(ncn);
+  for (int64_t k = 0; k < ncn; k++) {
+    SSF_CHECK(cn[k] <= m->n, SSF_EINVAL, "code number larger than the number of unknowns");
+    cn32[k] = (int)cn[k];
+  }
+  int *d_cn = nullptr;
+  SSF_HIP(dalloc(&d_cn, ncn));
+  SSF_HIP(hipMemcpy(d_cn, cn32.data(), ncn * sizeof(int), hipMemcpyHostToDevice));
+  *d_cn_out = d_cn;
+  if (same) return SSF_OK;
+  std::vector<int> order;
+  std::vector<int64_t> start;
+  int rc;
+  if (conn) rc = colour_elements(nn, ne, nne, conn, order, start);
+  else {  // no node list: colour by code numbers (DOFs as the shared resource)
+    std::vector<int64_t> c2(ncn);
+    for (int64_t k = 0; k < ncn; k++) c2[k] = std::max<int64_t>(cn[k], 0);
+    rc = colour_elements(m->n + 1, ne, nd, c2.data(), order, start);
+  }
+  if (rc) {
+    dfree(d_cn);
+    *d_cn_out = nullptr;
+    return rc;
+  }
+  dfree(m->slot);
+  dfree(m->order);
+  SSF_HIP(dalloc(&m->order, ne));
+  SSF_HIP(hipMemcpy(m->order, order.data(), ne * sizeof(int), hipMemcpyHostToDevice));
+  SSF_HIP(dalloc(&m->slot, ne * nd * nd));
+  // reference-layout pattern for the search
+  std::vector<int> adr32(m->n + 1), ci32(m->negm);
+  for (int64_t i = 0; i <= m->n; i++) adr32[i] = (int)m->adr[i];
+  for (int64_t k = 0; k < m->negm; k++) ci32[k] = (int)m->ci[k];
+  int *d_adr = nullptr, *d_ci = nullptr;
+  SSF_HIP(dalloc(&d_adr, m->n + 1));
+  SSF_HIP(dalloc(&d_ci, m->negm));
+  SSF_HIP(hipMemcpy(d_adr, adr32.data(), (m->n + 1) * sizeof(int), hipMemcpyHostToDevice));
+  if (m->negm) SSF_HIP(hipMemcpy(d_ci, ci32.data(), m->negm * sizeof(int), hipMemcpyHostToDevice));
+  SSF_HIP(build_slots(m->storage, aos ? 1 : 0, ne, nd, m->order, d_cn, d_adr, d_ci, m->slot, s));
+  SSF_HIP(hipStreamSynchronize(s));
+  dfree(d_adr);
+  dfree(d_ci);
+  m->asm_ne = ne;
+  m->asm_ndofe = nd;
+  m->colour_start = start;
+  m->asm_cn_host.assign(cn, cn + ncn);
+  return SSF_OK;
+}
+
+extern "C" int ssf_matrix_assemble(ssf_matrix *m, int kind, int64_t nn, const double *xyz,
+                                   int64_t ne, const int64_t *conn, const int64_t *cn,
+                                   const double *mat_a, const double *mat_b,
+                                   const double *thick, const double *pv, int64_t npv,
+                                   double *rhs_lift)
+{
+  SSF_ENTER();
+  SSF_CHECK(m && xyz && conn && cn && mat_a, SSF_EINVAL, "ssf_matrix_assemble: NULL argument");
+  SSF_CHECK(kind >= 0 && kind <= 2, SSF_EINVAL, "ssf_matrix_assemble: unknown element kind");
+  SSF_CHECK(kind == SSF_ELEM_HEX8_HEAT || mat_b, SSF_EINVAL, "ssf_matrix_assemble: Poisson ratios missing");
+  SSF_CHECK(ne >= 0 && nn >= 0 && ne < (1LL << 31) && nn < (1LL << 31), SSF_EINVAL, "ssf_matrix_assemble: bad sizes");
+  const int nne = kind == SSF_ELEM_QUAD4_PSTRESS ? 4 : 8;
+  const int nd = kind == SSF_ELEM_HEX8_ELAST ? 24 : 8;
+  SSF_CHECK((int64_t)ne * nd * nd < (1LL << 31) * 4, SSF_EUNSUP, "ssf_matrix_assemble: slot map too large");
+  for (int64_t k = 0; k < ne * nd; k++)
+    if (cn[k] < 0) SSF_CHECK(pv && -cn[k] <= npv, SSF_EINVAL, "ssf_matrix_assemble: prescribed value index out of range");
+  SSF_HIP(hipSetDevice(m->ctx->device));
+  hipStream_t s = m->ctx->stream;
+  int *d_cn = nullptr;
+  int rc = prepare_assembly(m, nn, ne, nne, nd, conn, cn, kind == SSF_ELEM_HEX8_ELAST, &d_cn);
+  if (rc) return rc;
+  std::vector<int> conn32(ne * nne);
+  for (int64_t k = 0; k < ne * nne; k++) conn32[k] = (int)conn[k];
+  int *d_conn = nullptr;
+  double *d_xyz = nullptr, *d_a = nullptr, *d_b = nullptr, *d_t = nullptr, *d_pv = nullptr, *d_lift = nullptr;
+  hipError_t e = hipSuccess;
+  auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
+  chk(dalloc(&d_conn, ne * nne));
+  chk(dalloc(&d_xyz, nn * 3));
+  chk(dalloc(&d_a, ne));
+  if (mat_b) chk(dalloc(&d_b, ne));
+  if (thick) chk(dalloc(&d_t, ne));
+  if (pv && npv) chk(dalloc(&d_pv, npv));
+  if (rhs_lift) chk(dalloc(&d_lift, m->n));
+  if (e == hipSuccess) {
+    chk(hipMemcpyAsync(d_conn, conn32.data(), ne * nne * sizeof(int), hipMemcpyHostToDevice, s));
+    chk(hipMemcpyAsync(d_xyz, xyz, nn * 3 * sizeof(double), hipMemcpyHostToDevice, s));
+    chk(hipMemcpyAsync(d_a, mat_a, ne * sizeof(double), hipMemcpyHostToDevice, s));
+    if (d_b) chk(hipMemcpyAsync(d_b, mat_b, ne * sizeof(double), hipMemcpyHostToDevice, s));
+    if (d_t) chk(hipMemcpyAsync(d_t, thick, ne * sizeof(double), hipMemcpyHostToDevice, s));
+    if (d_pv) chk(hipMemcpyAsync(d_pv, pv, npv * sizeof(double), hipMemcpyHostToDevice, s));
+    if (d_lift) chk(hipMemsetAsync(d_lift, 0, m->n * sizeof(double), s));
+  }
+  for (size_t c = 0; e == hipSuccess && c + 1 < m->colour_start.size(); c++)
+    chk(assemble_elements(kind, m->colour_start[c], m->colour_start[c + 1], ne, m->order, d_conn,
+                          d_cn, d_xyz, d_a, d_b, d_t, m->slot, m->a_ref, d_pv, d_lift, s));
+  if (e == hipSuccess) {
+    int r2 = values_changed(m);
+    if (r2) e = hipErrorUnknown;
+  }
+  if (e == hipSuccess && rhs_lift)
+    chk(hipMemcpyAsync(rhs_lift, d_lift, m->n * sizeof(double), hipMemcpyDeviceToHost, s));
+  chk(hipStreamSynchronize(s));
+  dfree(d_conn); dfree(d_xyz); dfree(d_a); dfree(d_b); dfree(d_t); dfree(d_pv); dfree(d_lift);
+  dfree(d_cn);
+  SSF_HIP(e);
+  return SSF_OK;
+}
+
+extern "C" int ssf_matrix_localize_batch(ssf_matrix *m, int64_t ne, int64_t nc, const int64_t *cn,
+                                         const double *lm)
+{
+  SSF_ENTER();
+  SSF_CHECK(m && cn && lm, SSF_EINVAL, "ssf_matrix_localize_batch: NULL argument");
+  SSF_CHECK(ne >= 0 && nc > 0 && nc <= 64, SSF_EINVAL, "ssf_matrix_localize_batch: bad sizes");
+  if (ne == 0) return SSF_OK;
+  SSF_HIP(hipSetDevice(m->ctx->device));
+  hipStream_t s = m->ctx->stream;
+  int *d_cn = nullptr;
+  int rc = prepare_assembly(m, 0, ne, (int)nc, (int)nc, nullptr, cn, false, &d_cn);
+  if (rc) return rc;
+  double *d_lm = nullptr;
+  SSF_HIP(dalloc(&d_lm, ne * nc * nc));
+  SSF_HIP(hipMemcpyAsync(d_lm, lm, ne * nc * nc * sizeof(double), hipMemcpyHostToDevice, s));
+  for (size_t c = 0; c + 1 < m->colour_start.size(); c++)
+    SSF_HIP(localize_elements(m->colour_start[c], m->colour_start[c + 1], ne, (int)nc, m->order,
+                              m->slot, d_lm, m->a_ref, s));
+  rc = values_changed(m);
+  SSF_HIP(hipStreamSynchronize(s));
+  dfree(d_lm);
+  dfree(d_cn);
+  return rc;
+}
+
+// --------------------------------------------------------------- workspace
+
+static int ensure_ws(ssf_matrix *m, bool mixed)
+{
+  Workspace &w = m->ws;
+  const int64_t n = std::max<int64_t>(1, m->n);
+  const int nb = std::max(m->tune64.blocks, m->tune32.blocks);
+  if (w.n != n || w.nb < nb) {
+    free_workspace(w);
+    w.n = n;
+    w.nb = std::max(nb, 2048);
+    SSF_HIP(dalloc(&w.x, n)); SSF_HIP(dalloc(&w.r, n)); SSF_HIP(dalloc(&w.d, n));
+    SSF_HIP(dalloc(&w.p, n)); SSF_HIP(dalloc(&w.h, n)); SSF_HIP(dalloc(&w.y, n));
+    SSF_HIP(dalloc(&w.part, 4 * (size_t)w.nb));
+    SSF_HIP(dalloc(&w.st, 1));
+    SSF_HIP(hipHostMalloc((void **)&w.st_host, 2 * sizeof(PcgState<double>), hipHostMallocDefault));
+  }
+  if (mixed && !w.x32) {
+    SSF_HIP(dalloc(&w.x32, n)); SSF_HIP(dalloc(&w.r32, n)); SSF_HIP(dalloc(&w.d32, n));
+    SSF_HIP(dalloc(&w.p32, n)); SSF_HIP(dalloc(&w.h32, n)); SSF_HIP(dalloc(&w.y32, n));
+    SSF_HIP(dalloc(&w.part32, 4 * (size_t)w.nb));
+    SSF_HIP(dalloc(&w.st32, 1));
+    SSF_HIP(hipHostMalloc((void **)&w.st32_host, 2 * sizeof(PcgState<float>), hipHostMallocDefault));
+  }
+  if (mixed && !m->val32_valid) {
+    if (!m->val32) SSF_HIP(dalloc(&m->val32, m->nnz));
+    if (!m->diag32) SSF_HIP(dalloc(&m->diag32, m->n));
+    SSF_HIP(to_f32(m->nnz, m->val, m->val32, 1.0, m->ctx->stream));
+    SSF_HIP(to_f32(m->n, m->diag, m->diag32, 1.0, m->ctx->stream));
+    m->val32_valid = true;
+  }
+  return SSF_OK;
+}
+
+// ----------------------------------------------------------------- SpMV
+
+extern "C" int ssf_gmxv_dev(ssf_matrix *m, const double *a_dev, double *b_dev)
+{
+  SSF_ENTER();
+  SSF_CHECK(m && a_dev && b_dev, SSF_EINVAL, "ssf_gmxv_dev: NULL argument");
+  SSF_CHECK(m->values_set, SSF_ESTATE, "ssf_gmxv_dev: matrix values were never set");
+  SSF_HIP(hipSetDevice(m->ctx->device));
+  if (m->n == 0) return SSF_OK;
+  SSF_HIP(spmv<double>(m->tune64, 0, (int)m->n, m->rowptr, m->col, m->val, a_dev, b_dev, nullptr,
+                       nullptr, nullptr, nullptr, m->ctx->stream));
+  return SSF_OK;
+}
+
+extern "C" int ssf_gmxv(ssf_matrix *m, const double *a, double *b)
+{
+  SSF_ENTER();
+  SSF_CHECK(m && a && b, SSF_EINVAL, "ssf_gmxv: NULL argument");
+  SSF_CHECK(m->values_set, SSF_ESTATE, "ssf_gmxv: matrix values were never set");
+  SSF_HIP(hipSetDevice(m->ctx->device));
+  int rc = ensure_ws(m, false);
+  if (rc) return rc;
+  hipStream_t s = m->ctx->stream;
+  SSF_HIP(hipMemcpyAsync(m->ws.d, a, m->n * sizeof(double), hipMemcpyHostToDevice, s));
+  rc = ssf_gmxv_dev(m, m->ws.d, m->ws.p);
+  if (rc) return rc;
+  SSF_HIP(hipMemcpyAsync(b, m->ws.p, m->n * sizeof(double), hipMemcpyDeviceToHost, s));
+  SSF_HIP(hipStreamSynchronize(s));
+  return SSF_OK;
+}
+
+// ------------------------------------------------------------------ solves
+
+namespace {
+
+constexpr int CHUNK = 16;  // iterations enqueued between two host checks
+
+template <typename T>
+int run_pcg(ssf_matrix *m, const SpmvTune &t, int mode, bool jac, const T *val, const T *diag,
+            T *x, const T *y, T *r, T *d, T *p, T *h, T *part, PcgState<T> *st,
+            PcgState<T> *st_host, bool havex, int64_t ni, T res, T zero, int64_t *launched)
+{
+  hipStream_t s = m->ctx->stream;
+  const int n = (int)m->n;
+  SSF_HIP(pcg_init<T>(t, jac, havex, n, m->rowptr, m->col, val, diag, x, y, r, h, d, part, st,
+                      zero, s));
+  hipEvent_t ev[2];
+  SSF_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+  SSF_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  int64_t it = 0;
+  int k = 0;
+  bool pending = false;
+  int rc = SSF_OK;
+  // chunk k is enqueued before the host waits for chunk k-1's state copy
+  while (true) {
+    bool launched_now = false;
+    if (it < ni) {
+      const int64_t c = std::min<int64_t>(CHUNK, ni - it);
+      for (int64_t q = 0; q < c; q++) {
+        hipError_t e = pcg_iteration<T>(t, mode, jac, n, m->rowptr, m->col, val, diag, x, r, d, p,
+                                        h, part, st, res, zero, s);
+        if (e != hipSuccess) {
+          set_error(std::string("pcg_iteration: ") + hipGetErrorString(e));
+          rc = SSF_EHIP;
+          break;
+        }
+      }
+      if (rc) break;
+      it += c;
+      (void)hipMemcpyAsync(&st_host[k & 1], st, sizeof(PcgState<T>), hipMemcpyDeviceToHost, s);
+      (void)hipEventRecord(ev[k & 1], s);
+      launched_now = true;
+    }
+    if (pending) {
+      (void)hipEventSynchronize(ev[(k - 1) & 1]);
+      if (st_host[(k - 1) & 1].done) break;
+    }
+    if (!launched_now) break;
+    pending = true;
+    k++;
+  }
+  (void)hipStreamSynchronize(s);
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  if (rc) return rc;
+  SSF_HIP(hipMemcpy(&st_host[0], st, sizeof(PcgState<T>), hipMemcpyDeviceToHost));
+  *launched = it;
+  return SSF_OK;
+}
+
+int true_residual(ssf_matrix *m, const double *x, const double *y, double *rel)
+{
+  hipStream_t s = m->ctx->stream;
+  Workspace &w = m->ws;
+  SSF_HIP(spmv<double>(m->tune64, 2, (int)m->n, m->rowptr, m->col, m->val, x, w.p, y, w.part,
+                       nullptr, nullptr, s));
+  SSF_HIP(reduce_to_host_slot(w.part, m->tune64.blocks, w.part + 2 * w.nb, s));
+  SSF_HIP(dot_partial(m->n, y, y, w.part, 256, s));
+  SSF_HIP(reduce_to_host_slot(w.part, 256, w.part + 2 * w.nb + 1, s));
+  double v[2];
+  SSF_HIP(hipMemcpyAsync(v, w.part + 2 * w.nb, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  SSF_HIP(hipStreamSynchronize(s));
+  *rel = v[1] > 0 ? std::sqrt(v[0] / v[1]) : 0.0;
+  return SSF_OK;
+}
+
+}  // namespace
+
+static int solve_fp64(ssf_matrix *m, int mode, bool jac, const ssf_solve_params *p,
+                      const double *y, double *x, ssf_solve_info *info)
+{
+  hipStream_t s = m->ctx->stream;
+  Workspace &w = m->ws;
+  if (!p->iv) SSF_HIP(hipMemsetAsync(x, 0, m->n * sizeof(double), s));
+  int64_t launched = 0;
+  int rc = run_pcg<double>(m, m->tune64, mode, jac, m->val, m->diag, x, y,
+                           w.r, w.d, w.p, w.h, w.part, w.st, w.st_host, p->iv != 0, p->ni, p->res,
+                           p->zero, &launched);
+  if (rc) return rc;
+  const PcgState<double> &st = w.st_host[0];
+  info->status = st.status;
+  info->outer = 0;
+  if (st.status == 3) {
+    // zero right-hand side: x = 0 (symcomprow::cg, GEFEL/scr.cpp:821-830); the
+    // reference's cg_prec would continue into 0/0 (GEFEL/scr.cpp:1117-1120)
+    if (mode != MODE_CR_CG_PREC) SSF_HIP(hipMemsetAsync(x, 0, m->n * sizeof(double), s));
+    info->ani = 0;
+    info->ares = st.ares;
+    info->iterations = 0;
+  } else if (st.status == 1 || st.status == 2) {
+    info->ani = st.ani;
+    info->ares = st.ares;
+    info->iterations = st.ani + 1;
+  } else {
+    // ni spent: cg_prec leaves ani/ares as they were; cg variants set ani = ni,
+    // ares = last r.h (GEFEL/scr.cpp:876, GEFEL/cr.cpp:1227)
+    if (mode != MODE_SCR_CG_PREC) {
+      info->ani = p->ni;
+      info->ares = st.nom[p->ni & 1];
+    }
+    info->iterations = launched;
+  }
+  return true_residual(m, x, y, &info->rel_residual);
+}
+
+// FP32-inner / FP64-refinement solve: r = b - A x in FP64, inner Jacobi-PCG on
+// the FP32 copy of A for A e = r / ||r|| to a relative reduction inner_res, x += ||r|| e.
+// Stops when ||r||^2 / ||b||^2 < res (the squared test of GEFEL/scr.cpp:1188).
+static int solve_mixed(ssf_matrix *m, const ssf_solve_params *p, const double *y, double *x,
+                       ssf_solve_info *info)
+{
+  hipStream_t s = m->ctx->stream;
+  Workspace &w = m->ws;
+  if (!p->iv) SSF_HIP(hipMemsetAsync(x, 0, m->n * sizeof(double), s));
+  const int n = (int)m->n;
+  double slots[2];
+  double *dslot = w.part + 3 * w.nb;
+  SSF_HIP(dot_partial(m->n, y, y, w.part, 256, s));
+  SSF_HIP(reduce_to_host_slot(w.part, 256, dslot + 1, s));
+  const int64_t max_outer = p->max_outer > 0 ? p->max_outer : 50;
+  const double inner_res = p->inner_res > 0 ? p->inner_res : 1e-4;
+  const int64_t inner_ni = p->inner_ni > 0 ? p->inner_ni : std::max<int64_t>(p->ni, 1);
+  int64_t total_inner = 0, outer = 0;
+  int status = 0;
+  double rel2 = 0.0, nory = 0.0, prev_rel2 = 0.0;
+  for (outer = 0; outer <= max_outer; outer++) {
+    // r = y - A x (FP64), ||r||^2
+    SSF_HIP(spmv<double>(m->tune64, 2, n, m->rowptr, m->col, m->val, x, w.r, y, w.part, nullptr,
+                         nullptr, s));
+    SSF_HIP(reduce_to_host_slot(w.part, m->tune64.blocks, dslot, s));
+    SSF_HIP(hipMemcpyAsync(slots, dslot, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    SSF_HIP(hipStreamSynchronize(s));
+    nory = slots[1];
+    if (nory < p->zero) {
+      status = 3;
+      SSF_HIP(hipMemsetAsync(x, 0, m->n * sizeof(double), s));
+      rel2 = nory;
+      break;
+    }
+    rel2 = slots[0] / nory;
+    if (rel2 < p->res) {
+      status = 1;
+      break;
+    }
+    // the FP64 residual stopped falling (less than 2x in the last refinement
+    // step): the FP64 accuracy floor of this system is reached above res
+    if (outer >= 2 && rel2 > 0.25 * prev_rel2) {
+      status = 4;
+      break;
+    }
+    prev_rel2 = rel2;
+    if (outer == max_outer || total_inner >= p->ni) break;
+    const double rn = std::sqrt(slots[0]);
+    float *y32 = w.y32;
+    SSF_HIP(to_f32(m->n, w.r, y32, 1.0 / rn, s));
+    SSF_HIP(hipMemsetAsync(w.x32, 0, m->n * sizeof(float), s));
+    int64_t launched = 0;
+    const int64_t budget = std::min<int64_t>(inner_ni, p->ni - total_inner);
+    int rc = run_pcg<float>(m, m->tune32, MODE_SCR_CG_PREC, true, m->val32, m->diag32, w.x32, y32,
+                            w.r32, w.d32, w.p32, w.h32, w.part32, w.st32, w.st32_host, false,
+                            budget, (float)(inner_res * inner_res), 1e-30f, &launched);
+    if (rc) return rc;
+    const PcgState<float> &st = w.st32_host[0];
+    total_inner += (st.status == 1 || st.status == 2) ? st.ani + 1 : launched;
+    SSF_HIP(axpy_f32(m->n, rn, w.x32, x, s));
+  }
+  info->status = status;
+  info->outer = outer;
+  info->iterations = total_inner;
+  if (status != 0) {
+    info->ani = total_inner;
+    info->ares = rel2;
+  }
+  info->rel_residual = std::sqrt(rel2);
+  return SSF_OK;
+}
+
+extern "C" int ssf_solve_dev(ssf_matrix *m, const ssf_solve_params *p, const double *rhs_dev,
+                             double *lhs_dev, ssf_solve_info *info)
+{
+  SSF_ENTER();
+  SSF_CHECK(m && p && info && rhs_dev && lhs_dev, SSF_EINVAL, "ssf_solve_dev: NULL argument");
+  SSF_CHECK(m->values_set, SSF_ESTATE, "ssf_solve_dev: matrix values were never set");
+  SSF_CHECK(p->linsolver == SSF_SOLVER_CG, SSF_EUNSUP, "ssf_solve_dev: only the cg solver (20) is on this path");
+  SSF_CHECK(p->ni >= 0, SSF_EINVAL, "ssf_solve_dev: negative ni");
+  int mode;
+  bool jac;
+  if (m->storage == SSF_STORAGE_SYMM_COMP_ROWS) {
+    if (p->prec == SSF_PREC_DIAG) { mode = MODE_SCR_CG_PREC; jac = true; }
+    else if (p->prec == SSF_PREC_NOPRECOND) { mode = MODE_SCR_CG; jac = false; }
+    else { set_error("ssf_solve_dev: symm_comp_rows supports noprecond (0) and diagprec (1) here"); return SSF_EUNSUP; }
+  } else {
+    if (p->prec == SSF_PREC_NOPRECOND) { mode = MODE_CR_CG_PREC; jac = false; }
+    else if (p->prec == SSF_PREC_DIAG && p->precision == SSF_PRECISION_MIXED) { mode = MODE_SCR_CG_PREC; jac = true; }
+    else {
+      set_error("ssf_solve_dev: compressed_rows + diagprec is not implemented by the reference "
+                "(precond::preconditioning, GEFEL/precond.cpp:161-183)");
+      return SSF_EUNSUP;
+    }
+  }
+  SSF_HIP(hipSetDevice(m->ctx->device));
+  const bool mixed = p->precision == SSF_PRECISION_MIXED;
+  SSF_CHECK(!mixed || jac, SSF_EUNSUP, "ssf_solve_dev: the mixed-precision solve is Jacobi-preconditioned (diagprec)");
+  int rc = ensure_ws(m, mixed);
+  if (rc) return rc;
+  if (m->n == 0) {
+    info->status = 1;
+    info->iterations = 0;
+    return SSF_OK;
+  }
+  info->rel_residual = 0.0;
+  return mixed ? solve_mixed(m, p, rhs_dev, lhs_dev, info)
+               : solve_fp64(m, mode, jac, p, rhs_dev, lhs_dev, info);
+}
+
+extern "C" int ssf_solve(ssf_matrix *m, const ssf_solve_params *p, const double *rhs, double *lhs,
+                         ssf_solve_info *info)
+{
+  SSF_ENTER();
+  SSF_CHECK(m && p && rhs && lhs && info, SSF_EINVAL, "ssf_solve: NULL argument");
+  SSF_HIP(hipSetDevice(m->ctx->device));
+  int rc = ensure_ws(m, p->precision == SSF_PRECISION_MIXED);
+  if (rc) return rc;
+  hipStream_t s = m->ctx->stream;
+  double *y = nullptr, *x = nullptr;
+  SSF_HIP(dalloc(&y, m->n));
+  SSF_HIP(dalloc(&x, m->n));
+  SSF_HIP(hipMemcpyAsync(y, rhs, m->n * sizeof(double), hipMemcpyHostToDevice, s));
+  if (p->iv) SSF_HIP(hipMemcpyAsync(x, lhs, m->n * sizeof(double), hipMemcpyHostToDevice, s));
+  rc = ssf_solve_dev(m, p, y, x, info);
+  if (rc == SSF_OK) {
+    SSF_HIP(hipMemcpyAsync(lhs, x, m->n * sizeof(double), hipMemcpyDeviceToHost, s));
+    SSF_HIP(hipStreamSynchronize(s));
+  }
+  dfree(y);
+  dfree(x);
+  return rc;
+}
+
+// ------------------------------------------------------------ diagnostics
+
+static int time_spmv(ssf_matrix *m, int precision, const SpmvTune &t, int reps, double *usec)
+{
+  hipStream_t s = m->ctx->stream;
+  Workspace &w = m->ws;
+  hipEvent_t a, b;
+  SSF_HIP(hipEventCreate(&a));
+  SSF_HIP(hipEventCreate(&b));
+  auto launch = [&]() -> hipError_t {
+    if (precision == 0)
+      return spmv<double>(t, 0, (int)m->n, m->rowptr, m->col, m->val, w.d, w.p, nullptr, nullptr,
+                          nullptr, nullptr, s);
+    return spmv<float>(t, 0, (int)m->n, m->rowptr, m->col, m->val32, w.d32, w.p32, nullptr,
+                       nullptr, nullptr, nullptr, s);
+  };
+  SSF_HIP(launch());
+  SSF_HIP(hipEventRecord(a, s));
+  for (int i = 0; i < reps; i++) SSF_HIP(launch());
+  SSF_HIP(hipEventRecord(b, s));
+  SSF_HIP(hipEventSynchronize(b));
+  float ms = 0.f;
+  SSF_HIP(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  *usec = 1000.0 * ms / reps;
+  return SSF_OK;
+}
+
+extern "C" int ssf_spmv_bench(ssf_matrix *m, int precision, int reps, double *usec)
+{
+  SSF_ENTER();
+  SSF_CHECK(m && usec && reps > 0, SSF_EINVAL, "ssf_spmv_bench: bad argument");
+  SSF_CHECK(m->values_set, SSF_ESTATE, "ssf_spmv_bench: matrix values were never set");
+  SSF_HIP(hipSetDevice(m->ctx->device));
+  int rc = ensure_ws(m, precision != 0);
+  if (rc) return rc;
+  SSF_HIP(fill(m->n, 1.0, m->ws.d, m->ctx->stream));
+  if (precision) SSF_HIP(to_f32(m->n, m->ws.d, m->ws.d32, 1.0, m->ctx->stream));
+  return time_spmv(m, precision, precision ? m->tune32 : m->tune64, reps, usec);
+}
+
+// SpMV autotuner: times every (lanes per row, blocks) candidate on this matrix
+// and keeps the fastest. Solves use the chosen tiling from then on.
+extern "C" int ssf_matrix_autotune(ssf_matrix *m, int precision, int reps, int *lanes, int *blocks)
+{
+  SSF_ENTER();
+  SSF_CHECK(m, SSF_EINVAL, "ssf_matrix_autotune: NULL matrix");
+  SSF_CHECK(m->values_set, SSF_ESTATE, "ssf_matrix_autotune: matrix values were never set");
+  SSF_HIP(hipSetDevice(m->ctx->device));
+  if (reps <= 0) reps = 5;
+  int rc = ensure_ws(m, precision != 0);
+  if (rc) return rc;
+  SSF_HIP(fill(m->n, 1.0, m->ws.d, m->ctx->stream));
+  if (precision) SSF_HIP(to_f32(m->n, m->ws.d, m->ws.d32, 1.0, m->ctx->stream));
+  const int lane_opts[] = {4, 8, 16, 32, 64};
+  const int block_opts[] = {256, 512, 1024, 2048};
+  SpmvTune best = precision ? m->tune32 : m->tune64;
+  double best_t = 1e300;
+  for (int l : lane_opts)
+    for (int b : block_opts) {
+      const int64_t groups = TPB / l;
+      if ((int64_t)b * groups > std::max<int64_t>(m->n, groups * 8)) continue;
+      if (b > m->ws.nb) continue;
+      SpmvTune t;
+      t.lanes = l;
+      t.blocks = b;
+      double us = 0;
+      rc = time_spmv(m, precision, t, reps, &us);
+      if (rc) return rc;
+      if (us < best_t) {
+        best_t = us;
+        best = t;
+      }
+    }
+  if (precision) m->tune32 = best;
+  else m->tune64 = best;
+  if (lanes) *lanes = best.lanes;
+  if (blocks) *blocks = best.blocks;
+  return SSF_OK;
+}
+
+extern "C" int ssf_matrix_set_tune(ssf_matrix *m, int precision, int lanes, int blocks)
+{
+  SSF_ENTER();
+  SSF_CHECK(m, SSF_EINVAL, "ssf_matrix_set_tune: NULL matrix");
+  SSF_CHECK(lanes == 4 || lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64, SSF_EINVAL,
+            "ssf_matrix_set_tune: lanes must be 4, 8, 16, 32 or 64");
+  SSF_CHECK(blocks >= 1 && blocks <= 2048, SSF_EINVAL, "ssf_matrix_set_tune: blocks out of [1, 2048]");
+  SpmvTune t;
+  t.lanes = lanes;
+  t.blocks = blocks;
+  if (precision) m->tune32 = t;
+  else m->tune64 = t;
+  return SSF_OK;
+}
+
+extern "C" int ssf_spmv_bytes(const ssf_matrix *m, int precision, double *canonical, double *actual)
+{
+  SSF_ENTER();
+  SSF_CHECK(m, SSF_EINVAL, "ssf_spmv_bytes: NULL matrix");
+  const double sv = precision ? 4.0 : 8.0;
+  const double b = (double)m->nnz * (sv + 4.0) + 4.0 * (double)(m->n + 1) + 2.0 * sv * (double)m->n;
+  if (canonical) *canonical = b;
+  if (actual) *actual = b;
+  return SSF_OK;
+}

@@ -1,0 +1,54 @@
+// Launchers of the gfx950 kernels (defined in ssf_kernels.hip).
+#pragma once
+
+#include "ssf_internal.hpp"
+
+namespace ssf {
+
+constexpr int TPB = 256;  // threads per block for every streaming kernel
+
+// y = A x (kind 0); y = A x with partial[b] = sum x_i y_i (kind 1);
+// y = b - A x with partial[b] = sum y_i^2 (kind 2). done (may be null) skips the
+// launch body once set; iter (may be null) is incremented by block 0.
+template <typename T>
+hipError_t spmv(const SpmvTune &t, int kind, int n, const int *rowptr, const int *col,
+                const T *val, const T *x, T *y, const T *b, T *partial, const int *done,
+                long long *iter, hipStream_t s);
+
+// PCG pieces (see ssf_kernels.hip for the recurrence).
+template <typename T>
+hipError_t pcg_init(const SpmvTune &t, bool jac, bool havex, int n, const int *rowptr,
+                    const int *col, const T *val, const T *diag, const T *x, const T *y,
+                    T *r, T *h, T *d, T *part, PcgState<T> *st, T zero, hipStream_t s);
+template <typename T>
+hipError_t pcg_iteration(const SpmvTune &t, int mode, bool jac, int n, const int *rowptr,
+                         const int *col, const T *val, const T *diag, T *x, T *r, T *d,
+                         T *p, T *h, T *part, PcgState<T> *st, T res, T zero,
+                         hipStream_t s);
+
+// element-wise helpers
+hipError_t expand_ref(int64_t negm, const int *map_lo, const int *map_up, const double *a,
+                      double *val, hipStream_t s);
+hipError_t extract_diag(int n, const int *rowptr, const int *col, const double *val,
+                        double *diag, float *diag32, hipStream_t s);
+hipError_t to_f32(int64_t n, const double *a, float *b, double scale, hipStream_t s);
+hipError_t axpy_f32(int64_t n, double alpha, const float *e, double *x, hipStream_t s);
+hipError_t dot_partial(int64_t n, const double *a, const double *b, double *part, int nb,
+                       hipStream_t s);
+hipError_t fill(int64_t n, double v, double *x, hipStream_t s);
+hipError_t reduce_to_host_slot(const double *part, int nb, double *out, hipStream_t s);
+
+// assembly
+hipError_t build_slots(int storage, int aos, int64_t ne_total, int nd, const int *order,
+                       const int *cn, const int *ref_adr, const int *ref_ci, int *slot,
+                       hipStream_t s);
+hipError_t assemble_elements(int kind, int64_t p0, int64_t p1, int64_t ne_total,
+                             const int *order, const int *conn, const int *cn,
+                             const double *xyz, const double *mat_a, const double *mat_b,
+                             const double *thick, const int *slot, double *a_ref,
+                             const double *pv, double *lift, hipStream_t s);
+hipError_t localize_elements(int64_t p0, int64_t p1, int64_t ne_total, int nd,
+                             const int *order, const int *slot, const double *lm,
+                             double *a_ref, hipStream_t s);
+
+}  // namespace ssf

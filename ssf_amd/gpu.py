@@ -1,0 +1,262 @@
+"""ctypes binding of libssf_gpu.so (the C ABI in include/ssf_gpu.h).
+
+The library is the product: this module only moves numpy arrays across the C
+ABI. Importing it raises if the HIP library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import mesh
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libssf_gpu.so")
+
+STORAGE_CR = 10
+STORAGE_SCR = 11
+SOLVER_CG = 20
+PREC_NONE = 0
+PREC_DIAG = 1
+FP64 = 0
+MIXED = 1
+
+_P = ctypes.c_void_p
+_L = ctypes.c_int64
+_D = ctypes.c_double
+_I = ctypes.c_int
+
+
+class SolveParams(ctypes.Structure):
+    _fields_ = [("linsolver", ctypes.c_int32), ("prec", ctypes.c_int32), ("ni", ctypes.c_int64),
+                ("res", ctypes.c_double), ("zero", ctypes.c_double), ("iv", ctypes.c_int32),
+                ("precision", ctypes.c_int32), ("inner_res", ctypes.c_double),
+                ("inner_ni", ctypes.c_int64), ("max_outer", ctypes.c_int64)]
+
+
+class SolveInfo(ctypes.Structure):
+    _fields_ = [("ani", ctypes.c_int64), ("ares", ctypes.c_double), ("status", ctypes.c_int32),
+                ("pad", ctypes.c_int32), ("iterations", ctypes.c_int64), ("outer", ctypes.c_int64),
+                ("rel_residual", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
+SIGNATURES = {
+    "ssf_last_error": (ctypes.c_char_p, []),
+    "ssf_ctx_create": (_I, [_I, ctypes.POINTER(_P)]),
+    "ssf_ctx_destroy": (_I, [_P]),
+    "ssf_ctx_sync": (_I, [_P]),
+    "ssf_ctx_stream": (_P, [_P]),
+    "ssf_matrix_create_elem": (_I, [_P, _I, _L, _L, _L, _P, ctypes.POINTER(_P)]),
+    "ssf_matrix_create_csr": (_I, [_P, _I, _L, _P, _P, ctypes.POINTER(_P)]),
+    "ssf_matrix_destroy": (_I, [_P]),
+    "ssf_matrix_sizes": (_I, [_P, _P, _P, _P]),
+    "ssf_matrix_pattern": (_I, [_P, _P, _P]),
+    "ssf_matrix_zero": (_I, [_P]),
+    "ssf_matrix_set_values": (_I, [_P, _P]),
+    "ssf_matrix_get_values": (_I, [_P, _P]),
+    "ssf_matrix_localize_batch": (_I, [_P, _L, _L, _P, _P]),
+    "ssf_matrix_assemble": (_I, [_P, _I, _L, _P, _L, _P, _P, _P, _P, _P, _P, _L, _P]),
+    "ssf_gmxv": (_I, [_P, _P, _P]),
+    "ssf_gmxv_dev": (_I, [_P, _P, _P]),
+    "ssf_solve": (_I, [_P, ctypes.POINTER(SolveParams), _P, _P, ctypes.POINTER(SolveInfo)]),
+    "ssf_solve_dev": (_I, [_P, ctypes.POINTER(SolveParams), _P, _P, ctypes.POINTER(SolveInfo)]),
+    "ssf_spmv_bench": (_I, [_P, _I, _I, ctypes.POINTER(_D)]),
+    "ssf_spmv_bytes": (_I, [_P, _I, ctypes.POINTER(_D), ctypes.POINTER(_D)]),
+    "ssf_matrix_autotune": (_I, [_P, _I, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "ssf_matrix_set_tune": (_I, [_P, _I, _I, _I]),
+}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"HIP library not built: {LIB_PATH} (run python ssf_amd/build.py)")
+        l = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(l, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = l
+    return _lib
+
+
+class SsfError(RuntimeError):
+    pass
+
+
+def _check(rc):
+    if rc != 0:
+        msg = lib().ssf_last_error().decode(errors="replace")
+        raise SsfError(f"ssf error {rc}: {msg}")
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _i64(a):
+    return np.ascontiguousarray(a, dtype=np.int64)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        self.h = _P()
+        _check(lib().ssf_ctx_create(device, ctypes.byref(self.h)))
+
+    def sync(self):
+        _check(lib().ssf_ctx_sync(self.h))
+
+    def close(self):
+        """Destroy the context; refused (and kept) while matrices on it are alive."""
+        if self.h and lib().ssf_ctx_destroy(self.h) == 0:
+            self.h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Matrix:
+    """A gmatrix whose symcomprow / comprow storage lives on the GPU."""
+
+    def __init__(self, ctx: Context, handle):
+        self.ctx = ctx
+        self.h = handle
+        n, negm, nnz = _L(), _L(), _L()
+        _check(lib().ssf_matrix_sizes(self.h, ctypes.byref(n), ctypes.byref(negm), ctypes.byref(nnz)))
+        self.n, self.negm, self.nnz = n.value, negm.value, nnz.value
+
+    @classmethod
+    def from_elements(cls, ctx: Context, storage: int, n: int, cn: np.ndarray) -> "Matrix":
+        cn = _i64(cn)
+        h = _P()
+        _check(lib().ssf_matrix_create_elem(ctx.h, storage, n, cn.shape[0], cn.shape[1], _p(cn),
+                                            ctypes.byref(h)))
+        return cls(ctx, h)
+
+    @classmethod
+    def from_csr(cls, ctx: Context, storage: int, adr: np.ndarray, ci: np.ndarray) -> "Matrix":
+        adr, ci = _i64(adr), _i64(ci)
+        h = _P()
+        _check(lib().ssf_matrix_create_csr(ctx.h, storage, adr.shape[0] - 1, _p(adr), _p(ci),
+                                           ctypes.byref(h)))
+        return cls(ctx, h)
+
+    def close(self):
+        if self.h:
+            lib().ssf_matrix_destroy(self.h)
+            self.h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def pattern(self):
+        adr = np.zeros(self.n + 1, np.int64)
+        ci = np.zeros(self.negm, np.int64)
+        _check(lib().ssf_matrix_pattern(self.h, _p(adr), _p(ci)))
+        return adr, ci
+
+    def zero(self):
+        _check(lib().ssf_matrix_zero(self.h))
+
+    def set_values(self, a):
+        a = _f64(a)
+        assert a.shape[0] == self.negm
+        _check(lib().ssf_matrix_set_values(self.h, _p(a)))
+
+    def get_values(self):
+        a = np.zeros(self.negm)
+        _check(lib().ssf_matrix_get_values(self.h, _p(a)))
+        return a
+
+    def localize_batch(self, cn, lm):
+        cn, lm = _i64(cn), _f64(lm)
+        ne, nc = cn.shape
+        _check(lib().ssf_matrix_localize_batch(self.h, ne, nc, _p(cn), _p(lm)))
+
+    def assemble(self, prob: mesh.Problem, lift: bool = True):
+        """Element kernels + coloured assembly; returns the Dirichlet lift (or None)."""
+        cn = _i64(prob.elem_codes())
+        conn = _i64(prob.conn)
+        xyz = _f64(prob.coords)
+        e = _f64(prob.e)
+        nu = _f64(prob.nu) if prob.kind != mesh.KIND_HEX8_HEAT else None
+        th = _f64(prob.thick) if prob.kind == mesh.KIND_QUAD4_PSTRESS else None
+        pv = _f64(prob.pv) if prob.pv.size else None
+        out = np.zeros(self.n) if (lift and pv is not None) else None
+        _check(lib().ssf_matrix_assemble(self.h, prob.kind, xyz.shape[0], _p(xyz), conn.shape[0],
+                                         _p(conn), _p(cn), _p(e), _p(nu), _p(th), _p(pv),
+                                         0 if pv is None else pv.shape[0], _p(out)))
+        return out
+
+    def gmxv(self, a):
+        a = _f64(a)
+        b = np.zeros(self.n)
+        _check(lib().ssf_gmxv(self.h, _p(a), _p(b)))
+        return b
+
+    def solve(self, rhs, ni=100000, res=1e-20, prec=PREC_DIAG, precision=FP64, zero=1e-30,
+              x0=None, inner_res=1e-4, inner_ni=0, max_outer=0):
+        rhs = _f64(rhs)
+        x = np.zeros(self.n) if x0 is None else _f64(x0).copy()
+        p = SolveParams(SOLVER_CG, prec, ni, res, zero, 0 if x0 is None else 1, precision,
+                        inner_res, inner_ni, max_outer)
+        info = SolveInfo(-1, -1.0, 0, 0, 0, 0, 0.0)
+        _check(lib().ssf_solve(self.h, ctypes.byref(p), _p(rhs), _p(x), ctypes.byref(info)))
+        return x, info
+
+    def solve_dev(self, rhs_ptr, x_ptr, ni=100000, res=1e-20, prec=PREC_DIAG, precision=FP64,
+                  zero=1e-30, iv=0, inner_res=1e-4, inner_ni=0, max_outer=0):
+        p = SolveParams(SOLVER_CG, prec, ni, res, zero, iv, precision, inner_res, inner_ni,
+                        max_outer)
+        info = SolveInfo(-1, -1.0, 0, 0, 0, 0, 0.0)
+        _check(lib().ssf_solve_dev(self.h, ctypes.byref(p), ctypes.c_void_p(rhs_ptr),
+                                   ctypes.c_void_p(x_ptr), ctypes.byref(info)))
+        return info
+
+    def spmv_bench(self, precision=FP64, reps=20):
+        us = _D()
+        _check(lib().ssf_spmv_bench(self.h, precision, reps, ctypes.byref(us)))
+        return us.value
+
+    def spmv_bytes(self, precision=FP64):
+        a, b = _D(), _D()
+        _check(lib().ssf_spmv_bytes(self.h, precision, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def autotune(self, precision=FP64, reps=5):
+        lanes, blocks = _I(), _I()
+        _check(lib().ssf_matrix_autotune(self.h, precision, reps, ctypes.byref(lanes),
+                                         ctypes.byref(blocks)))
+        return lanes.value, blocks.value
+
+    def set_tune(self, precision, lanes, blocks):
+        _check(lib().ssf_matrix_set_tune(self.h, precision, lanes, blocks))
+
+
+def build_problem_matrix(ctx: Context, prob: mesh.Problem, storage=STORAGE_SCR):
+    """Pattern + device assembly of a mesh.Problem; returns (matrix, rhs[nrhs, n])."""
+    m = Matrix.from_elements(ctx, storage, prob.n, prob.elem_codes())
+    lift = m.assemble(prob)
+    rhs = prob.rhs.copy()
+    if lift is not None:
+        rhs = rhs + lift[None, :]
+    return m, rhs

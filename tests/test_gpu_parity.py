@@ -1,0 +1,170 @@
+"""GPU parity of the HIP path against the oracle (CPU restatement of the reference)
+on the same seeded inputs, through the C ABI."""
+import numpy as np
+import pytest
+
+import oracle as O
+from ssf_amd import gpu, mesh
+
+pytestmark = pytest.mark.gpu
+
+SMALL = {
+    "hex8_elast": lambda: mesh.elastic_column(3, 4, 6, seed=11),
+    "cantilever": lambda: mesh.cantilever(8, 3, 3),
+    "hex8_heat": lambda: mesh.heat_cube(6, seed=12),
+    "quad4": lambda: mesh.plate(9, 5, seed=13, nrhs=2),
+}
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+@pytest.mark.parametrize("name", list(SMALL))
+@pytest.mark.parametrize("storage", ["scr", "cr"])
+def test_pattern_and_assembly(ctx, name, storage):
+    prob = SMALL[name]()
+    st = gpu.STORAGE_SCR if storage == "scr" else gpu.STORAGE_CR
+    m, rhs = gpu.build_problem_matrix(ctx, prob, st)
+    adr, ci, a, rhs_o = O.assemble_problem(prob, storage)
+    adr_g, ci_g = m.pattern()
+    assert np.array_equal(adr_g, adr) and np.array_equal(ci_g, ci)
+    a_g = m.get_values()
+    scale = np.abs(a).max()
+    assert np.abs(a_g - a).max() <= 1e-12 * scale
+    assert np.abs(rhs - rhs_o).max() <= 1e-12 * max(np.abs(rhs_o).max(), 1e-300)
+
+
+@pytest.mark.parametrize("name", list(SMALL))
+def test_gmxv(ctx, name):
+    prob = SMALL[name]()
+    adr, ci, a, _ = O.assemble_problem(prob, "scr")
+    m = gpu.Matrix.from_csr(ctx, gpu.STORAGE_SCR, adr, ci)
+    m.set_values(a)
+    b = np.random.default_rng(1).standard_normal(prob.n)
+    c = m.gmxv(b)
+    ref = O.mxv_scr(adr, ci, a, b)
+    assert rel(c, ref) <= 1e-14
+    adr2, ci2, a2, _ = O.assemble_problem(prob, "cr")
+    m2 = gpu.Matrix.from_csr(ctx, gpu.STORAGE_CR, adr2, ci2)
+    m2.set_values(a2)
+    assert rel(m2.gmxv(b), O.mxv_cr(adr2, ci2, a2, b)) <= 1e-14
+
+
+@pytest.mark.parametrize("name", list(SMALL))
+def test_pcg_fp64_parity(ctx, name):
+    prob = SMALL[name]()
+    adr, ci, a, rhs = O.assemble_problem(prob, "scr")
+    y = rhs[0]
+    xo, ani_o, ares_o, st_o = O.scr_cg_prec(adr, ci, a, y, 20000, 1e-24)
+    m = gpu.Matrix.from_csr(ctx, gpu.STORAGE_SCR, adr, ci)
+    m.set_values(a)
+    x, info = m.solve(y, ni=20000, res=1e-24)
+    # status 2 = |d.Ap| < zero (1e-30) before the residual test, as the reference
+    assert st_o in (1, 2) and info.status == st_o
+    assert rel(x, xo) <= 1e-8, rel(x, xo)
+    assert abs(info.ani - ani_o) <= max(2, ani_o // 50)
+    assert info.rel_residual < 1e-10
+
+
+@pytest.mark.parametrize("name", ["hex8_heat", "quad4"])
+def test_scr_cg_and_cr_cg(ctx, name):
+    prob = SMALL[name]()
+    adr, ci, a, rhs = O.assemble_problem(prob, "scr")
+    y = rhs[0]
+    xo, ani_o, ares_o, st_o = O.scr_cg(adr, ci, a, y, 20000, 1e-12)
+    m = gpu.Matrix.from_csr(ctx, gpu.STORAGE_SCR, adr, ci)
+    m.set_values(a)
+    x, info = m.solve(y, ni=20000, res=1e-12, prec=gpu.PREC_NONE)
+    assert rel(x, xo) <= 1e-8 and abs(info.ani - ani_o) <= max(2, ani_o // 50)
+    adr2, ci2, a2, _ = O.assemble_problem(prob, "cr")
+    xo2, ani2, ares2, st2 = O.cr_cg_prec_noprec(adr2, ci2, a2, y, 20000, 1e-12)
+    m2 = gpu.Matrix.from_csr(ctx, gpu.STORAGE_CR, adr2, ci2)
+    m2.set_values(a2)
+    x2, info2 = m2.solve(y, ni=20000, res=1e-12, prec=gpu.PREC_NONE)
+    assert rel(x2, xo2) <= 1e-8 and abs(info2.ani - ani2) <= max(2, ani2 // 50)
+
+
+def test_cr_jacobi_unsupported(ctx):
+    prob = SMALL["quad4"]()
+    adr, ci, a, rhs = O.assemble_problem(prob, "cr")
+    m = gpu.Matrix.from_csr(ctx, gpu.STORAGE_CR, adr, ci)
+    m.set_values(a)
+    with pytest.raises(gpu.SsfError):
+        m.solve(rhs[0], prec=gpu.PREC_DIAG)
+
+
+@pytest.mark.parametrize("name", list(SMALL))
+def test_mixed_precision(ctx, name):
+    prob = SMALL[name]()
+    adr, ci, a, rhs = O.assemble_problem(prob, "scr")
+    y = rhs[0]
+    xo, *_ = O.scr_cg_prec(adr, ci, a, y, 50000, 1e-30)
+    m = gpu.Matrix.from_csr(ctx, gpu.STORAGE_SCR, adr, ci)
+    m.set_values(a)
+    x, info = m.solve(y, ni=50000, res=1e-26, precision=gpu.MIXED, inner_res=1e-4)
+    # 1: residual test met; 4: FP64 residual floor of the system reached first
+    assert info.status in (1, 4)
+    assert rel(x, xo) <= 1e-8, rel(x, xo)
+
+
+def test_zero_rhs_and_ni_exhausted(ctx):
+    prob = SMALL["hex8_heat"]()
+    adr, ci, a, rhs = O.assemble_problem(prob, "scr")
+    m = gpu.Matrix.from_csr(ctx, gpu.STORAGE_SCR, adr, ci)
+    m.set_values(a)
+    x, info = m.solve(np.zeros(prob.n))
+    assert info.status == 3 and info.ani == 0 and not x.any()
+    x, info = m.solve(rhs[0], ni=3, res=1e-30)
+    assert info.status == 0 and info.ani == -1 and info.iterations == 3  # left unchanged
+    xo, ani_o, ares_o, st_o = O.scr_cg_prec(adr, ci, a, rhs[0], 3, 1e-30)
+    assert rel(x, xo) <= 1e-12
+
+
+def test_diagonal_one_iteration(ctx):
+    n = 50
+    adr = np.arange(n + 1, dtype=np.int64)
+    ci = np.arange(n, dtype=np.int64)
+    a = np.linspace(1.0, 7.0, n)
+    m = gpu.Matrix.from_csr(ctx, gpu.STORAGE_SCR, adr, ci)
+    m.set_values(a)
+    y = np.random.default_rng(3).standard_normal(n)
+    x, info = m.solve(y, res=1e-20)
+    assert info.status == 1 and info.ani == 0
+    assert rel(x, y / a) <= 1e-15
+
+
+def test_deterministic(ctx):
+    prob = mesh.heat_cube(10, seed=5)
+    m, rhs = gpu.build_problem_matrix(ctx, prob)
+    x1, i1 = m.solve(rhs[0], res=1e-24)
+    x2, i2 = m.solve(rhs[0], res=1e-24)
+    assert np.array_equal(x1, x2) and i1.ani == i2.ani
+
+
+def test_localize_batch(ctx):
+    prob = SMALL["hex8_elast"]()
+    adr, ci, a, _ = O.assemble_problem(prob, "scr")
+    cn = prob.elem_codes()
+    ke = O.element_matrices(prob)
+    m = gpu.Matrix.from_elements(ctx, gpu.STORAGE_SCR, prob.n, cn)
+    m.zero()
+    m.localize_batch(cn, ke.reshape(prob.ne, -1))
+    assert np.abs(m.get_values() - a).max() <= 1e-12 * np.abs(a).max()
+
+
+def test_cantilever_full_config(ctx):
+    """Cantilever 20x20x80 hex8 (105 840 unknowns) Jacobi-PCG to res 1e-20 on both."""
+    prob = mesh.cantilever()
+    m, rhs = gpu.build_problem_matrix(ctx, prob)
+    adr, ci = m.pattern()
+    a = m.get_values()
+    x, info = m.solve(rhs[0], ni=20000, res=1e-20)
+    adr_o, ci_o, a_o, rhs_o = O.assemble_problem(prob, "scr")
+    assert np.array_equal(adr, adr_o) and np.array_equal(ci, ci_o)
+    assert np.abs(a - a_o).max() <= 1e-12 * np.abs(a_o).max()
+    xo, ani_o, ares_o, st_o = O.scr_cg_prec(adr_o, ci_o, a_o, rhs_o[0], 20000, 1e-20)
+    print("cantilever ani gpu", info.ani, "oracle", ani_o, "rel", rel(x, xo), flush=True)
+    assert info.status == 1 and st_o == 1
+    assert rel(x, xo) <= 1e-8
+    assert abs(info.ani - ani_o) <= max(3, ani_o // 100)
