@@ -1,0 +1,127 @@
+#!/usr/bin/env python3
+"""Writes tests/golden/ref_vectors.npz: inputs and outputs of the reference's own
+GEFEL code (scr.cpp, cr.cpp, basefun/difcalc/intp/matrix.cpp) built by
+`make -C oracle ref` into oracle/_ref/libssfref.so. Run in a container where
+/root/reference exists; the fixture is data only (arrays of inputs and outputs).
+
+Cases (small seeded meshes from ssf_amd.mesh):
+  elast  hex8 elasticity 2x2x3, E lognormal, nu uniform, base clamped
+  cant   hex8 cantilever 4x2x2, x=0 clamped, traction on x=L
+  heat   hex8 heat 4^3 with T=0 / T=100 Dirichlet faces (lift on the rhs)
+  quad   quad4 plane stress 6x3, left edge clamped
+For each: element matrices K_e, scr and cr arrays (adr, ci, a) after localizing
+every K_e in element order, mxv_scr / mxv_cr of a seeded vector, and the
+cg_prec (Jacobi), scr cg and cr cg_prec (noprecond) results x, ani, ares.
+"""
+import ctypes
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+import oracle as O  # noqa: E402
+from oracle import _ptr, f64, i64  # noqa: E402
+from ssf_amd import mesh  # noqa: E402
+
+CASES = {
+    "elast": lambda: mesh.elastic_column(2, 2, 3, seed=21),
+    "cant": lambda: mesh.cantilever(4, 2, 2),
+    "heat": lambda: mesh.heat_cube(4, seed=22),
+    "quad": lambda: mesh.plate(6, 3, seed=23, nrhs=1),
+}
+
+
+def ref_ke(R, prob):
+    nd = prob.ndofe
+    out = np.zeros((prob.ne, nd, nd))
+    for e in range(prob.ne):
+        nodes = prob.conn[e]
+        x, y, z = (f64(prob.coords[nodes, k]) for k in range(3))
+        buf = np.zeros(nd * nd)
+        if prob.kind == mesh.KIND_HEX8_ELAST:
+            R.ref_hex8_elast_ke(_ptr(x), _ptr(y), _ptr(z), prob.e[e], prob.nu[e], _ptr(buf))
+        elif prob.kind == mesh.KIND_HEX8_HEAT:
+            R.ref_hex8_heat_ke(_ptr(x), _ptr(y), _ptr(z), prob.e[e], _ptr(buf))
+        else:
+            t = f64(np.full(4, prob.thick[e]))
+            R.ref_quad4_pstress_ke(_ptr(x), _ptr(y), prob.e[e], prob.nu[e], _ptr(t), _ptr(buf))
+        out[e] = buf.reshape(nd, nd)
+    return out
+
+
+def storage(R, prob, ke, kind):
+    cn = i64(prob.elem_codes())
+    ne, nd = cn.shape
+    create, negm, get, loc, free = (
+        (R.ref_scr_create, R.ref_scr_negm, R.ref_scr_get, R.ref_scr_localized, R.ref_scr_free)
+        if kind == "scr" else
+        (R.ref_cr_create, R.ref_cr_negm, R.ref_cr_get, R.ref_cr_localized, R.ref_cr_free))
+    h = create(prob.n, ne, nd, _ptr(cn))
+    for e in range(ne):
+        lm = f64(ke[e]).copy()
+        c = i64(cn[e]).copy()
+        loc(h, _ptr(lm), _ptr(c), nd)
+    k = negm(h)
+    adr = np.zeros(prob.n + 1, np.int64)
+    ci = np.zeros(k, np.int64)
+    a = np.zeros(k)
+    get(h, _ptr(adr), _ptr(ci), _ptr(a))
+    return h, adr, ci, a, free
+
+
+def solve(fn, h, n, y, ni, res, *extra):
+    x = np.zeros(n)
+    ani = ctypes.c_long(-7)
+    ares = ctypes.c_double(-7.0)
+    fn(h, _ptr(x), _ptr(f64(y).copy()), ni, res, ctypes.byref(ani), ctypes.byref(ares), 1e-30,
+       0, *extra)
+    return x, ani.value, ares.value
+
+
+def main():
+    if not O.ref_available():
+        sys.exit("oracle/_ref/libssfref.so missing: run make -C oracle ref")
+    R = O.ref()
+    out = {}
+    rng = np.random.default_rng(99)
+    for name, make in CASES.items():
+        prob = make()
+        ke = ref_ke(R, prob)
+        cn = prob.elem_codes()
+        y = prob.rhs[0].copy()
+        if prob.pv.size:
+            y = y + O.dirichlet_lift(prob, cn, ke)
+        out[f"{name}_ke"] = ke
+        out[f"{name}_cn"] = cn
+        out[f"{name}_rhs"] = y
+        b = rng.standard_normal(prob.n)
+        out[f"{name}_b"] = b
+        hs, adr, ci, a, fs = storage(R, prob, ke, "scr")
+        out[f"{name}_scr_adr"], out[f"{name}_scr_ci"], out[f"{name}_scr_a"] = adr, ci, a
+        c = np.zeros(prob.n)
+        R.ref_scr_mxv(hs, _ptr(f64(b).copy()), _ptr(c))
+        out[f"{name}_scr_mxv"] = c
+        x, ani, ares = solve(R.ref_scr_cg_prec, hs, prob.n, y, 5000, 1e-20, 1)
+        out[f"{name}_cgprec_x"], out[f"{name}_cgprec_ani"], out[f"{name}_cgprec_ares"] = x, ani, ares
+        x, ani, ares = solve(R.ref_scr_cg, hs, prob.n, y, 5000, 1e-10)
+        out[f"{name}_cg_x"], out[f"{name}_cg_ani"], out[f"{name}_cg_ares"] = x, ani, ares
+        fs(hs)
+        hc, adr, ci, a, fc = storage(R, prob, ke, "cr")
+        out[f"{name}_cr_adr"], out[f"{name}_cr_ci"], out[f"{name}_cr_a"] = adr, ci, a
+        c = np.zeros(prob.n)
+        R.ref_cr_mxv(hc, _ptr(f64(b).copy()), _ptr(c))
+        out[f"{name}_cr_mxv"] = c
+        x, ani, ares = solve(R.ref_cr_cg_prec_noprec, hc, prob.n, y, 5000, 1e-10)
+        out[f"{name}_crcg_x"], out[f"{name}_crcg_ani"], out[f"{name}_crcg_ares"] = x, ani, ares
+        fc(hc)
+        print(name, prob.n, "unknowns", flush=True)
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ref_vectors.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, os.path.getsize(path), "bytes")
+
+
+if __name__ == "__main__":
+    main()
