@@ -9,8 +9,10 @@ hex8 conduction kernel; inputs are resident in HBM before timing starts.
 A step is one solve of that system to relative error <= 1e-8 against x_ref
 (FP64 GPU PCG run to the limit of FP64, itself checked against the CPU
 restatement of the reference on smaller meshes by the tests):
-  * value      = solves/s of the mixed form (FP32 inner Jacobi-PCG, FP64 refinement)
-  * fp64       = solves/s of the plain FP64 GPU PCG (drop-in recurrence)
+  * value = solves/s of the mixed form (FP32 Jacobi-PCG, FP64 reliable updates)
+  * fp64  = solves/s of the plain FP64 GPU PCG (the drop-in recurrence)
+The roofline figure is the FP32 SpMV of the mixed solve: its bytes per launch
+divided by its mean duration sampled with HIP events inside the timed solves.
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -40,7 +42,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nel", type=int, default=128, help="elements per cube edge")
     ap.add_argument("--res", type=float, default=1e-22, help="squared relative residual test")
-    ap.add_argument("--inner-res", type=float, default=1e-4)
+    ap.add_argument("--interval", type=int, default=32,
+                    help="FP32 iterations between FP64 refinements (mixed)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-iters", type=int, default=20)
     args = ap.parse_args()
@@ -80,39 +83,47 @@ def main():
     xref_norm = torch.linalg.norm(xref).item()
 
     def rel_err():
-        return (torch.linalg.norm(x - xref).item() / xref_norm)
+        return torch.linalg.norm(x - xref).item() / xref_norm
 
     def fp64_solve():
         return m.solve_dev(b.data_ptr(), x.data_ptr(), ni=200000, res=args.res)
 
     def mixed_solve():
         return m.solve_dev(b.data_ptr(), x.data_ptr(), ni=200000, res=args.res,
-                           precision=gpu.MIXED, inner_res=args.inner_res)
+                           precision=gpu.MIXED, inner_ni=args.interval)
 
     results = {}
-    for name, fn in (("fp64", fp64_solve), ("mixed", mixed_solve)):
+    for name, fn, which in (("fp64", fp64_solve, 0), ("mixed", mixed_solve, 1)):
         for _ in range(args.warmup):
             fn()
         torch.cuda.synchronize()
+        ctx.set_profiling(True)
         t = time.perf_counter()
         infos = [fn() for _ in range(args.steps)]
         torch.cuda.synchronize()
         dt = time.perf_counter() - t
+        spmv_us, samples = ctx.kernel_time(which)
+        ctx.set_profiling(False)
         err = rel_err()
         results[name] = {"solves_per_s": args.steps / dt, "ms_per_solve": 1e3 * dt / args.steps,
                          "rel_err": err, "iterations": int(infos[-1].iterations),
-                         "outer": int(infos[-1].outer),
-                         "rel_residual": float(infos[-1].rel_residual)}
+                         "refinements": int(infos[-1].outer), "status": int(infos[-1].status),
+                         "rel_residual": float(infos[-1].rel_residual),
+                         "spmv_us_live": spmv_us, "spmv_samples": samples}
         log(f"[bench] {name}: {results[name]}")
 
-    # roofline of the dominant kernel: the FP32 SpMV of the inner solve
-    us32 = m.spmv_bench(gpu.MIXED, reps=50)
-    us64 = m.spmv_bench(gpu.FP64, reps=50)
-    b32, _ = m.spmv_bytes(gpu.MIXED)
-    b64, _ = m.spmv_bytes(gpu.FP64)
-    gbs32 = b32 / (us32 * 1e-6) / 1e9
-    gbs64 = b64 / (us64 * 1e-6) / 1e9
-    log(f"[bench] spmv fp32 {us32:.1f} us {gbs32:.0f} GB/s; fp64 {us64:.1f} us {gbs64:.0f} GB/s")
+    # bytes per SpMV launch (canonical CSR count and the format actually streamed)
+    b32_can, b32_act = m.spmv_bytes(gpu.MIXED)
+    b64_can, b64_act = m.spmv_bytes(gpu.FP64)
+    us32 = results["mixed"]["spmv_us_live"]
+    us64 = results["fp64"]["spmv_us_live"]
+    gbs32 = b32_act / (us32 * 1e-6) / 1e9 if us32 else 0.0
+    gbs64 = b64_act / (us64 * 1e-6) / 1e9 if us64 else 0.0
+    copy_bytes = 1 << 30
+    copy_us = ctx.copy_bench(copy_bytes, reps=20)
+    copy_gbs = 2 * copy_bytes / (copy_us * 1e-6) / 1e9
+    log(f"[bench] spmv fp32 {us32:.1f} us {gbs32:.0f} GB/s; fp64 {us64:.1f} us {gbs64:.0f} GB/s; "
+        f"copy {copy_gbs:.0f} GB/s")
 
     cpu = None
     if not args.no_cpu:
@@ -147,18 +158,19 @@ def main():
         "dtype": "fp32-inner/fp64-refinement",
         "data": "synthetic (seeded lognormal conductivity field, generated mesh)",
         "config": {"workload": f"heat hex8 {args.nel}^3, n={prob.n}, nnz={m.nnz}",
-                   "res": args.res, "inner_res": args.inner_res,
+                   "res": args.res, "refinement_interval": args.interval,
                    "spmv_tiling": {"fp64": tune64, "fp32": tune32}},
         "fp64_solves_per_s": fp64["solves_per_s"],
         "speedup_vs_fp64": mixed["solves_per_s"] / fp64["solves_per_s"],
         "rel_err": {"mixed": mixed["rel_err"], "fp64": fp64["rel_err"]},
-        "iterations": {"mixed_inner": mixed["iterations"], "mixed_outer": mixed["outer"],
+        "iterations": {"mixed_fp32": mixed["iterations"], "mixed_refinements": mixed["refinements"],
                        "fp64": fp64["iterations"]},
-        "spmv_gbs": {"fp32": gbs32, "fp64": gbs64},
+        "spmv_gbs": {"fp32": gbs32, "fp64": gbs64, "copy_kernel": copy_gbs},
         "roofline": {"bound": "hbm", "achieved": gbs32, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                      "frac": gbs32 * 1e9 / PEAK_HBM, "traffic": None,
-                     "kernel": "k_spmv<float> (FP32 CSR SpMV of the inner solve)",
-                     "bytes_per_launch": b32, "usec_per_launch": us32},
+                     "kernel": "k_spmv_stream<float> (FP32 SpMV of the mixed solve)",
+                     "bytes_per_launch": b32_act, "bytes_per_launch_canonical_csr": b32_can,
+                     "usec_per_launch": us32, "samples": mixed["spmv_samples"]},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -59,9 +59,11 @@ typedef struct ssf_solve_params {
   double zero;         /* computer zero (gmatrix::zero, 1e-30)                    */
   int32_t iv;          /* 0: start from x = 0, 1: start from lhs                  */
   int32_t precision;   /* SSF_PRECISION_*                                         */
-  double inner_res;    /* mixed: relative residual reduction of each inner solve  */
-  int64_t inner_ni;    /* mixed: maximum inner iterations per refinement step     */
-  int64_t max_outer;   /* mixed: maximum refinement steps                         */
+  double inner_res;    /* mixed: delta, relative FP32 residual reduction between two
+                          FP64 refreshes (0 = 0.1)                               */
+  int64_t inner_ni;    /* mixed: FP32 iterations between two FP64 refinements
+                          (reliable updates); 0 = 32                              */
+  int64_t max_outer;   /* reserved (0)                                            */
 } ssf_solve_params;
 
 typedef struct ssf_solve_info {
@@ -70,8 +72,8 @@ typedef struct ssf_solve_info {
   int32_t status;      /* 1 converged, 2 zero denominator, 3 zero rhs, 0 ni spent,
                           4 mixed: FP64 residual floor reached above res */
   int32_t pad;
-  int64_t iterations;  /* SpMV-iterations actually run (inner ones for mixed)    */
-  int64_t outer;       /* mixed: refinement steps                                 */
+  int64_t iterations;  /* PCG iterations actually run (FP32 ones for mixed)      */
+  int64_t outer;       /* mixed: FP64 refinements (residual recomputations)       */
   double rel_residual; /* final ||b - A x|| / ||b|| in FP64                        */
 } ssf_solve_info;
 
@@ -82,6 +84,12 @@ int ssf_ctx_destroy(ssf_ctx *ctx);
 int ssf_ctx_sync(ssf_ctx *ctx);
 /* the context's HIP stream (hipStream_t), for callers that enqueue around it */
 void *ssf_ctx_stream(ssf_ctx *ctx);
+/* Kernel sampling inside solves (on = 1 enables and resets): the first SpMV of
+ * every iteration chunk is bracketed by HIP events on the solve stream.
+ * ssf_ctx_kernel_time returns the mean duration (us) and sample count of the
+ * FP64 (which = 0) or FP32 (which = 1) operator since the last reset. */
+int ssf_ctx_set_profiling(ssf_ctx *ctx, int on);
+int ssf_ctx_kernel_time(ssf_ctx *ctx, int which, double *avg_us, int64_t *count);
 
 /* ------------------------------------------------------------ storage setup */
 /* symcomprow::initiate / comprow::initiate (GEFEL/scr.cpp:556, GEFEL/cr.cpp:454,
@@ -147,14 +155,17 @@ int ssf_solve_dev(ssf_matrix *m, const ssf_solve_params *p, const double *rhs_de
  * vectors; returns the mean kernel time in microseconds measured with HIP
  * events on the context's stream. */
 int ssf_spmv_bench(ssf_matrix *m, int precision, int reps, double *usec);
-/* SpMV autotuner: time every (lanes per row, blocks) tiling on this matrix for
- * the FP64 (precision 0) or FP32 (1) SpMV and keep the fastest for later
- * solves. The chosen tiling is returned (either pointer may be NULL). */
-int ssf_matrix_autotune(ssf_matrix *m, int precision, int reps, int *lanes, int *blocks);
-/* pin the SpMV tiling (lanes per row in {4,8,16,32,64}, blocks in [1, 2048]);
- * results of a solve depend only on the matrix, the right-hand side and this
- * tiling */
-int ssf_matrix_set_tune(ssf_matrix *m, int precision, int lanes, int blocks);
+/* SpMV autotuner: time every tiling of the FP64 (precision 0) or FP32 (1) SpMV
+ * on this matrix and keep the fastest for later solves. tune_out (may be NULL)
+ * receives {format (0 row-vector, 1 stream), lanes per row, blocks, idx16}. */
+int ssf_matrix_autotune(ssf_matrix *m, int precision, int reps, int32_t *tune_out);
+/* pin the SpMV tiling {format, lanes, blocks, idx16}; the result of a solve
+ * depends only on the matrix, the right-hand side and the two tilings */
+int ssf_matrix_set_tune(ssf_matrix *m, int precision, const int32_t *tune);
+/* Time `reps` launches of a plain device copy of `bytes` bytes (read + write
+ * 2*bytes of HBM per launch): the achievable-bandwidth reference of the
+ * roofline and the size-known kernel of the traffic-counter calibration. */
+int ssf_copy_bench(ssf_ctx *ctx, int64_t bytes, int reps, double *usec);
 /* bytes the SpMV of the given precision streams by the canonical count
  * nnz*(s_v+4) + 4(n+1) + 2 s_v n, and the bytes of the format actually used */
 int ssf_spmv_bytes(const ssf_matrix *m, int precision, double *canonical, double *actual);

@@ -118,11 +118,21 @@ static int pattern_from_elements(int storage, int64_t n, int64_t ne, int64_t ndo
 
 // ------------------------------------------------------------- SpMV tuning
 
-// Deterministic default: lanes per row from the mean row length, blocks from n.
-static SpmvTune default_tune(int64_t n, int64_t nnz)
+// Deterministic default: the stream kernel when the matrix has a tiling (lanes
+// per row from the mean row length, one to a few tiles per block), otherwise the
+// row-vector kernel.
+static SpmvTune default_tune(int64_t n, int64_t nnz, int ntiles, bool idx16)
 {
   SpmvTune t;
   const double avg = n ? (double)nnz / (double)n : 1.0;
+  if (ntiles > 0) {
+    t.format = 1;
+    t.idx16 = idx16 ? 1 : 0;
+    t.lanes = avg <= 24 ? 4 : avg <= 56 ? 8 : 16;
+    int64_t nb = std::max<int64_t>(8, std::min<int64_t>(1024, ntiles));
+    t.blocks = (int)((nb + 7) / 8 * 8);
+    return t;
+  }
   t.lanes = avg <= 6 ? 4 : avg <= 24 ? 8 : avg <= 56 ? 16 : avg <= 160 ? 32 : 64;
   const int64_t rows_per_group_min = 4;
   const int64_t groups = TPB / t.lanes;
@@ -131,6 +141,45 @@ static SpmvTune default_tune(int64_t n, int64_t nnz)
   nb = (nb + 7) / 8 * 8;
   t.blocks = (int)nb;
   return t;
+}
+
+// Stream tiling: consecutive whole rows with at most STREAM_CAP entries per tile;
+// 16-bit column offsets relative to each tile's first row when they all fit.
+static bool build_tiles(int64_t n, const std::vector<int> &rowptr, const std::vector<int> &col,
+                        std::vector<int> &tiles, std::vector<short> &col16)
+{
+  tiles.clear();
+  col16.clear();
+  tiles.push_back(0);
+  int64_t cur = 0;
+  for (int64_t i = 0; i < n; i++) {
+    const int64_t len = rowptr[i + 1] - rowptr[i];
+    if (len > STREAM_CAP) {
+      tiles.clear();
+      return false;
+    }
+    if (cur + len > STREAM_CAP) {
+      tiles.push_back((int)i);
+      cur = 0;
+    }
+    cur += len;
+  }
+  if (tiles.back() != n || tiles.size() == 1) tiles.push_back((int)n);
+  bool fits = true;
+  col16.resize(col.size());
+  for (size_t t = 0; t + 1 < tiles.size() && fits; t++) {
+    const int r0 = tiles[t];
+    for (int k = rowptr[r0]; k < rowptr[tiles[t + 1]]; k++) {
+      const int off = col[k] - r0;
+      if (off < -32768 || off > 32767) {
+        fits = false;
+        break;
+      }
+      col16[k] = (short)off;
+    }
+  }
+  if (!fits) col16.clear();
+  return true;
 }
 
 }  // namespace ssf
@@ -171,6 +220,9 @@ extern "C" int ssf_ctx_destroy(ssf_ctx *ctx)
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   (void)hipStreamDestroy(ctx->stream);
+  for (int a = 0; a < 2; a++)
+    for (int b = 0; b < 2; b++)
+      if (ctx->prof_ev[a][b]) (void)hipEventDestroy(ctx->prof_ev[a][b]);
   delete ctx;
   return SSF_OK;
 }
@@ -189,7 +241,9 @@ static void free_workspace(Workspace &w)
 {
   dfree(w.x); dfree(w.r); dfree(w.d); dfree(w.p); dfree(w.h); dfree(w.y);
   dfree(w.x32); dfree(w.r32); dfree(w.d32); dfree(w.p32); dfree(w.h32); dfree(w.y32);
-  dfree(w.part); dfree(w.part32); dfree(w.st); dfree(w.st32);
+  dfree(w.part); dfree(w.part32); dfree(w.st); dfree(w.st32); dfree(w.mix);
+  if (w.mix_host) (void)hipHostFree(w.mix_host);
+  w.mix_host = nullptr;
   if (w.st_host) (void)hipHostFree(w.st_host);
   if (w.st32_host) (void)hipHostFree(w.st32_host);
   w.st_host = nullptr;
@@ -209,6 +263,7 @@ extern "C" int ssf_matrix_destroy(ssf_matrix *m)
   dfree(m->diag); dfree(m->diag32);
   dfree(m->map_lo); dfree(m->map_up);
   dfree(m->slot); dfree(m->order);
+  dfree(m->tiles); dfree(m->col16);
   free_workspace(m->ws);
   m->ctx->live_matrices--;
   delete m;
@@ -293,12 +348,26 @@ extern "C" int ssf_matrix_create_csr(ssf_ctx *ctx, int storage, int64_t n, const
         }
       }
   }
+  std::vector<int> tiles;
+  std::vector<short> col16;
+  const bool have_tiles = build_tiles(n, rowptr, col, tiles, col16);
   hipError_t e = hipSuccess;
   auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
   chk(dalloc(&m->rowptr, n + 1));
   chk(dalloc(&m->col, m->nnz));
   chk(dalloc(&m->val, m->nnz));
   chk(dalloc(&m->diag, n));
+  if (have_tiles) {
+    m->ntiles = (int)tiles.size() - 1;
+    chk(dalloc(&m->tiles, tiles.size()));
+    if (e == hipSuccess)
+      chk(hipMemcpy(m->tiles, tiles.data(), tiles.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (!col16.empty()) {
+      chk(dalloc(&m->col16, col16.size()));
+      if (e == hipSuccess)
+        chk(hipMemcpy(m->col16, col16.data(), col16.size() * sizeof(short), hipMemcpyHostToDevice));
+    }
+  }
   if (sym) {
     chk(dalloc(&m->a_ref, negm));
     chk(dalloc(&m->map_lo, negm));
@@ -324,8 +393,10 @@ extern "C" int ssf_matrix_create_csr(ssf_ctx *ctx, int storage, int64_t n, const
     set_error(msg);
     return e == hipErrorOutOfMemory ? SSF_ENOMEM : SSF_EHIP;
   }
-  m->tune64 = default_tune(n, m->nnz);
-  m->tune32 = m->tune64;
+  // canonical int32 columns for the FP64 drop-in, 16-bit offsets where they fit
+  // for the FP32 inner solve
+  m->tune64 = default_tune(n, m->nnz, m->ntiles, false);
+  m->tune32 = default_tune(n, m->nnz, m->ntiles, m->col16 != nullptr);
   *out = m;
   return SSF_OK;
 }
@@ -611,6 +682,8 @@ static int ensure_ws(ssf_matrix *m, bool mixed)
     SSF_HIP(dalloc(&w.part32, 4 * (size_t)w.nb));
     SSF_HIP(dalloc(&w.st32, 1));
     SSF_HIP(hipHostMalloc((void **)&w.st32_host, 2 * sizeof(PcgState<float>), hipHostMallocDefault));
+    SSF_HIP(dalloc(&w.mix, 1));
+    SSF_HIP(hipHostMalloc((void **)&w.mix_host, 2 * sizeof(MixState), hipHostMallocDefault));
   }
   if (mixed && !m->val32_valid) {
     if (!m->val32) SSF_HIP(dalloc(&m->val32, m->nnz));
@@ -631,7 +704,7 @@ extern "C" int ssf_gmxv_dev(ssf_matrix *m, const double *a_dev, double *b_dev)
   SSF_CHECK(m->values_set, SSF_ESTATE, "ssf_gmxv_dev: matrix values were never set");
   SSF_HIP(hipSetDevice(m->ctx->device));
   if (m->n == 0) return SSF_OK;
-  SSF_HIP(spmv<double>(m->tune64, 0, (int)m->n, m->rowptr, m->col, m->val, a_dev, b_dev, nullptr,
+  SSF_HIP(spmv<double>(m->tune64, 0, m->csr(), m->val, a_dev, b_dev, nullptr,
                        nullptr, nullptr, nullptr, m->ctx->stream));
   return SSF_OK;
 }
@@ -655,6 +728,38 @@ extern "C" int ssf_gmxv(ssf_matrix *m, const double *a, double *b)
 
 // ------------------------------------------------------------------ solves
 
+static void prof_add(ssf_ctx *c, int which, int parity)
+{
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, c->prof_ev[parity][0], c->prof_ev[parity][1]) == hipSuccess) {
+    c->prof_us[which] += 1000.0 * ms;
+    c->prof_n[which] += 1;
+  }
+}
+
+extern "C" int ssf_ctx_set_profiling(ssf_ctx *ctx, int on)
+{
+  SSF_ENTER();
+  SSF_CHECK(ctx, SSF_EINVAL, "ssf_ctx_set_profiling: NULL context");
+  SSF_HIP(hipSetDevice(ctx->device));
+  if (on && !ctx->prof_ev[0][0])
+    for (int a = 0; a < 2; a++)
+      for (int b = 0; b < 2; b++) SSF_HIP(hipEventCreate(&ctx->prof_ev[a][b]));
+  ctx->prof = on != 0;
+  ctx->prof_us[0] = ctx->prof_us[1] = 0.0;
+  ctx->prof_n[0] = ctx->prof_n[1] = 0;
+  return SSF_OK;
+}
+
+extern "C" int ssf_ctx_kernel_time(ssf_ctx *ctx, int which, double *avg_us, int64_t *count)
+{
+  SSF_ENTER();
+  SSF_CHECK(ctx && avg_us && (which == 0 || which == 1), SSF_EINVAL, "ssf_ctx_kernel_time: bad argument");
+  *avg_us = ctx->prof_n[which] ? ctx->prof_us[which] / ctx->prof_n[which] : 0.0;
+  if (count) *count = ctx->prof_n[which];
+  return SSF_OK;
+}
+
 namespace {
 
 constexpr int CHUNK = 16;  // iterations enqueued between two host checks
@@ -666,7 +771,7 @@ int run_pcg(ssf_matrix *m, const SpmvTune &t, int mode, bool jac, const T *val, 
 {
   hipStream_t s = m->ctx->stream;
   const int n = (int)m->n;
-  SSF_HIP(pcg_init<T>(t, jac, havex, n, m->rowptr, m->col, val, diag, x, y, r, h, d, part, st,
+  SSF_HIP(pcg_init<T>(t, jac, havex, m->csr(), val, diag, x, y, r, h, d, part, st,
                       zero, s));
   hipEvent_t ev[2];
   SSF_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
@@ -675,14 +780,19 @@ int run_pcg(ssf_matrix *m, const SpmvTune &t, int mode, bool jac, const T *val, 
   int k = 0;
   bool pending = false;
   int rc = SSF_OK;
+  bool sample_valid[2] = {true, false};
+  const int which = sizeof(T) == 8 ? 0 : 1;
   // chunk k is enqueued before the host waits for chunk k-1's state copy
   while (true) {
     bool launched_now = false;
     if (it < ni) {
       const int64_t c = std::min<int64_t>(CHUNK, ni - it);
       for (int64_t q = 0; q < c; q++) {
-        hipError_t e = pcg_iteration<T>(t, mode, jac, n, m->rowptr, m->col, val, diag, x, r, d, p,
-                                        h, part, st, res, zero, s);
+        const bool smp = m->ctx->prof && q == 0;
+        hipError_t e = pcg_iteration<T>(t, mode, jac, m->csr(), val, diag, x, r, d, p,
+                                        h, part, st, res, zero, s,
+                                        smp ? m->ctx->prof_ev[k & 1][0] : nullptr,
+                                        smp ? m->ctx->prof_ev[k & 1][1] : nullptr);
         if (e != hipSuccess) {
           set_error(std::string("pcg_iteration: ") + hipGetErrorString(e));
           rc = SSF_EHIP;
@@ -697,6 +807,8 @@ int run_pcg(ssf_matrix *m, const SpmvTune &t, int mode, bool jac, const T *val, 
     }
     if (pending) {
       (void)hipEventSynchronize(ev[(k - 1) & 1]);
+      if (m->ctx->prof && sample_valid[(k - 1) & 1]) prof_add(m->ctx, which, (k - 1) & 1);
+      sample_valid[k & 1] = !st_host[(k - 1) & 1].done;
       if (st_host[(k - 1) & 1].done) break;
     }
     if (!launched_now) break;
@@ -716,7 +828,7 @@ int true_residual(ssf_matrix *m, const double *x, const double *y, double *rel)
 {
   hipStream_t s = m->ctx->stream;
   Workspace &w = m->ws;
-  SSF_HIP(spmv<double>(m->tune64, 2, (int)m->n, m->rowptr, m->col, m->val, x, w.p, y, w.part,
+  SSF_HIP(spmv<double>(m->tune64, 2, m->csr(), m->val, x, w.p, y, w.part,
                        nullptr, nullptr, s));
   SSF_HIP(reduce_to_host_slot(w.part, m->tune64.blocks, w.part + 2 * w.nb, s));
   SSF_HIP(dot_partial(m->n, y, y, w.part, 256, s));
@@ -767,75 +879,107 @@ static int solve_fp64(ssf_matrix *m, int mode, bool jac, const ssf_solve_params 
   return true_residual(m, x, y, &info->rel_residual);
 }
 
-// FP32-inner / FP64-refinement solve: r = b - A x in FP64, inner Jacobi-PCG on
-// the FP32 copy of A for A e = r / ||r|| to a relative reduction inner_res, x += ||r|| e.
-// Stops when ||r||^2 / ||b||^2 < res (the squared test of GEFEL/scr.cpp:1188).
+// Mixed-precision solve: FP32 Jacobi-PCG on float(A) with FP32-accumulated dots,
+// refined in FP64 by reliable updates every `interval` iterations (and whenever
+// the FP32 recurrence meets the test): x64 += x32, r = b - A x64 in FP64, the
+// FP32 residual is replaced by the FP64 one while the search direction is kept.
+// Stops when ||b - A x||^2 < res ||b||^2 in FP64 (the squared test of
+// GEFEL/scr.cpp:1188 applied to the true residual).
 static int solve_mixed(ssf_matrix *m, const ssf_solve_params *p, const double *y, double *x,
                        ssf_solve_info *info)
 {
   hipStream_t s = m->ctx->stream;
   Workspace &w = m->ws;
+  const CsrDev A = m->csr();
   if (!p->iv) SSF_HIP(hipMemsetAsync(x, 0, m->n * sizeof(double), s));
-  const int n = (int)m->n;
-  double slots[2];
+  const int64_t interval = p->inner_ni > 0 ? p->inner_ni : 32;
+  // ||b||^2 and the initial FP64 residual
   double *dslot = w.part + 3 * w.nb;
   SSF_HIP(dot_partial(m->n, y, y, w.part, 256, s));
   SSF_HIP(reduce_to_host_slot(w.part, 256, dslot + 1, s));
-  const int64_t max_outer = p->max_outer > 0 ? p->max_outer : 50;
-  const double inner_res = p->inner_res > 0 ? p->inner_res : 1e-4;
-  const int64_t inner_ni = p->inner_ni > 0 ? p->inner_ni : std::max<int64_t>(p->ni, 1);
-  int64_t total_inner = 0, outer = 0;
-  int status = 0;
-  double rel2 = 0.0, nory = 0.0, prev_rel2 = 0.0;
-  for (outer = 0; outer <= max_outer; outer++) {
-    // r = y - A x (FP64), ||r||^2
-    SSF_HIP(spmv<double>(m->tune64, 2, n, m->rowptr, m->col, m->val, x, w.r, y, w.part, nullptr,
-                         nullptr, s));
-    SSF_HIP(reduce_to_host_slot(w.part, m->tune64.blocks, dslot, s));
-    SSF_HIP(hipMemcpyAsync(slots, dslot, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  SSF_HIP(spmv<double>(m->tune64, 2, A, m->val, x, w.r, y, w.part, nullptr, nullptr, s));
+  SSF_HIP(reduce_to_host_slot(w.part, m->tune64.blocks, dslot, s));
+  double slots[2];
+  SSF_HIP(hipMemcpyAsync(slots, dslot, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  SSF_HIP(hipStreamSynchronize(s));
+  info->outer = 0;
+  info->iterations = 0;
+  if (slots[1] < p->zero) {  // zero right-hand side: x = 0 (GEFEL/scr.cpp:821-830)
+    SSF_HIP(hipMemsetAsync(x, 0, m->n * sizeof(double), s));
     SSF_HIP(hipStreamSynchronize(s));
-    nory = slots[1];
-    if (nory < p->zero) {
-      status = 3;
-      SSF_HIP(hipMemsetAsync(x, 0, m->n * sizeof(double), s));
-      rel2 = nory;
-      break;
-    }
-    rel2 = slots[0] / nory;
-    if (rel2 < p->res) {
-      status = 1;
-      break;
-    }
-    // the FP64 residual stopped falling (less than 2x in the last refinement
-    // step): the FP64 accuracy floor of this system is reached above res
-    if (outer >= 2 && rel2 > 0.25 * prev_rel2) {
-      status = 4;
-      break;
-    }
-    prev_rel2 = rel2;
-    if (outer == max_outer || total_inner >= p->ni) break;
-    const double rn = std::sqrt(slots[0]);
-    float *y32 = w.y32;
-    SSF_HIP(to_f32(m->n, w.r, y32, 1.0 / rn, s));
-    SSF_HIP(hipMemsetAsync(w.x32, 0, m->n * sizeof(float), s));
-    int64_t launched = 0;
-    const int64_t budget = std::min<int64_t>(inner_ni, p->ni - total_inner);
-    int rc = run_pcg<float>(m, m->tune32, MODE_SCR_CG_PREC, true, m->val32, m->diag32, w.x32, y32,
-                            w.r32, w.d32, w.p32, w.h32, w.part32, w.st32, w.st32_host, false,
-                            budget, (float)(inner_res * inner_res), 1e-30f, &launched);
-    if (rc) return rc;
-    const PcgState<float> &st = w.st32_host[0];
-    total_inner += (st.status == 1 || st.status == 2) ? st.ani + 1 : launched;
-    SSF_HIP(axpy_f32(m->n, rn, w.x32, x, s));
+    info->status = 3;
+    info->ani = 0;
+    info->ares = slots[1];
+    info->rel_residual = 0.0;
+    return SSF_OK;
   }
-  info->status = status;
-  info->outer = outer;
-  info->iterations = total_inner;
-  if (status != 0) {
-    info->ani = total_inner;
-    info->ares = rel2;
+  // FP32 recurrence on the residual scaled by 1/||r0|| (the device keeps the
+  // scale from here on), x32 = 0
+  const double scale = slots[0] > 0.0 ? 1.0 / std::sqrt(slots[0]) : 1.0;
+  SSF_HIP(to_f32(m->n, w.r, w.y32, scale, s));
+  SSF_HIP(hipMemsetAsync(w.x32, 0, m->n * sizeof(float), s));
+  SSF_HIP(pcg_init<float>(m->tune32, true, false, A, m->val32, m->diag32, w.x32, w.y32, w.r32,
+                          w.h32, w.d32, w.part32, w.st32, 1e-30f, s));
+  const double delta = (p->inner_res > 0.0 && p->inner_res < 1.0) ? p->inner_res : 0.1;
+  SSF_HIP(mixed_init(w.mix, w.st32, dslot, p->res, delta, s));
+  hipEvent_t ev[2];
+  SSF_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+  SSF_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  int64_t it = 0;
+  int k = 0;
+  bool pending = false;
+  int rc = SSF_OK;
+  bool sample_valid[2] = {true, false};
+  while (true) {
+    bool launched_now = false;
+    if (it < p->ni) {
+      const int64_t c = std::min<int64_t>(interval, p->ni - it);
+      for (int64_t q = 0; q < c && rc == SSF_OK; q++)
+        if (pcg_iteration<float>(m->tune32, MODE_MIXED_INNER, true, A, m->val32, m->diag32, w.x32,
+                                 w.r32, w.d32, w.p32, w.h32, w.part32, w.st32, 1.0f, 1e-30f, s,
+                                 (m->ctx->prof && q == 0) ? m->ctx->prof_ev[k & 1][0] : nullptr,
+                                 (m->ctx->prof && q == 0) ? m->ctx->prof_ev[k & 1][1] : nullptr) !=
+            hipSuccess)
+          rc = SSF_EHIP;
+      if (rc == SSF_OK &&
+          mixed_update(m->tune64, m->tune32, A, m->val, m->diag32, y, x, w.r, w.part, w.x32,
+                       w.r32, w.h32, w.d32, w.part32, w.st32, w.mix, p->res, s) != hipSuccess)
+        rc = SSF_EHIP;
+      if (rc) {
+        set_error("solve_mixed: kernel launch failed");
+        break;
+      }
+      it += c;
+      (void)hipMemcpyAsync(&w.mix_host[k & 1], w.mix, sizeof(MixState), hipMemcpyDeviceToHost, s);
+      (void)hipEventRecord(ev[k & 1], s);
+      launched_now = true;
+    }
+    if (pending) {
+      (void)hipEventSynchronize(ev[(k - 1) & 1]);
+      if (m->ctx->prof && sample_valid[(k - 1) & 1]) prof_add(m->ctx, 1, (k - 1) & 1);
+      sample_valid[k & 1] = !w.mix_host[(k - 1) & 1].final;
+      if (w.mix_host[(k - 1) & 1].final) break;
+    }
+    if (!launched_now) break;
+    pending = true;
+    k++;
   }
-  info->rel_residual = std::sqrt(rel2);
+  (void)hipStreamSynchronize(s);
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  if (rc) return rc;
+  MixState ms;
+  PcgState<float> st;
+  SSF_HIP(hipMemcpy(&ms, w.mix, sizeof ms, hipMemcpyDeviceToHost));
+  SSF_HIP(hipMemcpy(&st, w.st32, sizeof st, hipMemcpyDeviceToHost));
+  info->status = ms.final ? ms.status : 0;
+  info->outer = ms.updates;
+  info->iterations = st.iter + 1;
+  if (info->status != 0) {
+    info->ani = info->iterations;
+    info->ares = ms.rr64 / ms.nory64;
+  }
+  info->rel_residual = std::sqrt(ms.rr64 / ms.nory64);
   return SSF_OK;
 }
 
@@ -912,9 +1056,9 @@ static int time_spmv(ssf_matrix *m, int precision, const SpmvTune &t, int reps, 
   SSF_HIP(hipEventCreate(&b));
   auto launch = [&]() -> hipError_t {
     if (precision == 0)
-      return spmv<double>(t, 0, (int)m->n, m->rowptr, m->col, m->val, w.d, w.p, nullptr, nullptr,
+      return spmv<double>(t, 0, m->csr(), m->val, w.d, w.p, nullptr, nullptr,
                           nullptr, nullptr, s);
-    return spmv<float>(t, 0, (int)m->n, m->rowptr, m->col, m->val32, w.d32, w.p32, nullptr,
+    return spmv<float>(t, 0, m->csr(), m->val32, w.d32, w.p32, nullptr,
                        nullptr, nullptr, nullptr, s);
   };
   SSF_HIP(launch());
@@ -943,9 +1087,11 @@ extern "C" int ssf_spmv_bench(ssf_matrix *m, int precision, int reps, double *us
   return time_spmv(m, precision, precision ? m->tune32 : m->tune64, reps, usec);
 }
 
-// SpMV autotuner: times every (lanes per row, blocks) candidate on this matrix
-// and keeps the fastest. Solves use the chosen tiling from then on.
-extern "C" int ssf_matrix_autotune(ssf_matrix *m, int precision, int reps, int *lanes, int *blocks)
+// SpMV autotuner: times every tiling candidate (row-vector kernel: lanes per row x
+// blocks; stream kernel: lanes per row x blocks, with 16-bit column offsets for
+// the FP32 operator when they fit) on this matrix and keeps the fastest. Solves
+// use the chosen tiling from then on. tune_out = {format, lanes, blocks, idx16}.
+extern "C" int ssf_matrix_autotune(ssf_matrix *m, int precision, int reps, int32_t *tune_out)
 {
   SSF_ENTER();
   SSF_CHECK(m, SSF_EINVAL, "ssf_matrix_autotune: NULL matrix");
@@ -956,45 +1102,111 @@ extern "C" int ssf_matrix_autotune(ssf_matrix *m, int precision, int reps, int *
   if (rc) return rc;
   SSF_HIP(fill(m->n, 1.0, m->ws.d, m->ctx->stream));
   if (precision) SSF_HIP(to_f32(m->n, m->ws.d, m->ws.d32, 1.0, m->ctx->stream));
-  const int lane_opts[] = {4, 8, 16, 32, 64};
-  const int block_opts[] = {256, 512, 1024, 2048};
-  SpmvTune best = precision ? m->tune32 : m->tune64;
-  double best_t = 1e300;
-  for (int l : lane_opts)
-    for (int b : block_opts) {
+  std::vector<SpmvTune> cand;
+  for (int l : {4, 8, 16, 32, 64})
+    for (int b : {256, 512, 1024, 2048}) {
       const int64_t groups = TPB / l;
       if ((int64_t)b * groups > std::max<int64_t>(m->n, groups * 8)) continue;
-      if (b > m->ws.nb) continue;
       SpmvTune t;
+      t.format = 0;
       t.lanes = l;
       t.blocks = b;
-      double us = 0;
-      rc = time_spmv(m, precision, t, reps, &us);
-      if (rc) return rc;
-      if (us < best_t) {
-        best_t = us;
-        best = t;
-      }
+      cand.push_back(t);
     }
+  if (m->ntiles > 0)
+    for (int l : {4, 8, 16})
+      for (int b : {256, 512, 1024, 2048})
+        for (int i16 = 0; i16 <= ((precision && m->col16) ? 1 : 0); i16++) {
+          if (b > std::max(8, m->ntiles)) continue;
+          SpmvTune t;
+          t.format = 1;
+          t.lanes = l;
+          t.blocks = b;
+          t.idx16 = i16;
+          cand.push_back(t);
+        }
+  SpmvTune best = precision ? m->tune32 : m->tune64;
+  double best_t = 1e300;
+  for (const SpmvTune &t : cand) {
+    if (t.blocks > m->ws.nb) continue;
+    double us = 0;
+    rc = time_spmv(m, precision, t, reps, &us);
+    if (rc) return rc;
+    if (us < best_t) {
+      best_t = us;
+      best = t;
+    }
+  }
   if (precision) m->tune32 = best;
   else m->tune64 = best;
-  if (lanes) *lanes = best.lanes;
-  if (blocks) *blocks = best.blocks;
+  if (tune_out) {
+    tune_out[0] = best.format;
+    tune_out[1] = best.lanes;
+    tune_out[2] = best.blocks;
+    tune_out[3] = best.idx16;
+  }
   return SSF_OK;
 }
 
-extern "C" int ssf_matrix_set_tune(ssf_matrix *m, int precision, int lanes, int blocks)
+extern "C" int ssf_matrix_set_tune(ssf_matrix *m, int precision, const int32_t *tune)
 {
   SSF_ENTER();
-  SSF_CHECK(m, SSF_EINVAL, "ssf_matrix_set_tune: NULL matrix");
-  SSF_CHECK(lanes == 4 || lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64, SSF_EINVAL,
-            "ssf_matrix_set_tune: lanes must be 4, 8, 16, 32 or 64");
-  SSF_CHECK(blocks >= 1 && blocks <= 2048, SSF_EINVAL, "ssf_matrix_set_tune: blocks out of [1, 2048]");
+  SSF_CHECK(m && tune, SSF_EINVAL, "ssf_matrix_set_tune: NULL argument");
   SpmvTune t;
-  t.lanes = lanes;
-  t.blocks = blocks;
+  t.format = tune[0];
+  t.lanes = tune[1];
+  t.blocks = tune[2];
+  t.idx16 = tune[3];
+  SSF_CHECK(t.format == 0 || (t.format == 1 && m->ntiles > 0), SSF_EINVAL,
+            "ssf_matrix_set_tune: format must be 0 (row-vector) or 1 (stream, needs a tiling)");
+  if (t.format == 0)
+    SSF_CHECK(t.lanes == 4 || t.lanes == 8 || t.lanes == 16 || t.lanes == 32 || t.lanes == 64,
+              SSF_EINVAL, "ssf_matrix_set_tune: row-vector lanes must be 4, 8, 16, 32 or 64");
+  else
+    SSF_CHECK(t.lanes == 4 || t.lanes == 8 || t.lanes == 16, SSF_EINVAL,
+              "ssf_matrix_set_tune: stream lanes must be 4, 8 or 16");
+  SSF_CHECK(t.blocks >= 1 && t.blocks <= 2048, SSF_EINVAL, "ssf_matrix_set_tune: blocks out of [1, 2048]");
+  SSF_CHECK(t.idx16 == 0 || (precision && t.format == 1 && m->col16), SSF_EINVAL,
+            "ssf_matrix_set_tune: 16-bit columns only for the FP32 stream operator of a matrix that has them");
   if (precision) m->tune32 = t;
   else m->tune64 = t;
+  return SSF_OK;
+}
+
+extern "C" int ssf_copy_bench(ssf_ctx *ctx, int64_t bytes, int reps, double *usec)
+{
+  SSF_ENTER();
+  SSF_CHECK(ctx && usec && reps > 0 && bytes >= 16, SSF_EINVAL, "ssf_copy_bench: bad argument");
+  SSF_HIP(hipSetDevice(ctx->device));
+  const int64_t n = bytes / 16 * 2;  // doubles, a multiple of 2
+  double *a = nullptr, *b = nullptr;
+  SSF_HIP(dalloc(&a, n));
+  hipError_t e = dalloc(&b, n);
+  if (e != hipSuccess) {
+    dfree(a);
+    SSF_HIP(e);
+  }
+  hipStream_t s = ctx->stream;
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
+  chk(fill(n, 1.0, a, s));
+  chk(hipEventCreate(&t0));
+  chk(hipEventCreate(&t1));
+  if (e == hipSuccess) {
+    chk(copy_kernel(n, a, b, s));
+    chk(hipEventRecord(t0, s));
+    for (int i = 0; i < reps; i++) chk(copy_kernel(n, a, b, s));
+    chk(hipEventRecord(t1, s));
+    chk(hipEventSynchronize(t1));
+    float ms = 0.f;
+    chk(hipEventElapsedTime(&ms, t0, t1));
+    *usec = 1000.0 * ms / reps;
+  }
+  if (t0) (void)hipEventDestroy(t0);
+  if (t1) (void)hipEventDestroy(t1);
+  dfree(a);
+  dfree(b);
+  SSF_HIP(e);
   return SSF_OK;
 }
 
@@ -1004,7 +1216,11 @@ extern "C" int ssf_spmv_bytes(const ssf_matrix *m, int precision, double *canoni
   SSF_CHECK(m, SSF_EINVAL, "ssf_spmv_bytes: NULL matrix");
   const double sv = precision ? 4.0 : 8.0;
   const double b = (double)m->nnz * (sv + 4.0) + 4.0 * (double)(m->n + 1) + 2.0 * sv * (double)m->n;
+  const SpmvTune &t = precision ? m->tune32 : m->tune64;
+  const double ib = (t.format == 1 && t.idx16 && m->col16) ? 2.0 : 4.0;
+  double a = (double)m->nnz * (sv + ib) + 4.0 * (double)(m->n + 1) + 2.0 * sv * (double)m->n;
+  if (t.format == 1) a += 4.0 * (double)(m->ntiles + 1);
   if (canonical) *canonical = b;
-  if (actual) *actual = b;
+  if (actual) *actual = a;
   return SSF_OK;
 }

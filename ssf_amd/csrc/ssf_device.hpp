@@ -1,0 +1,78 @@
+// Device helpers shared by the gfx950 kernels of libssf_gpu.
+#pragma once
+
+#include "ssf_kernels.hpp"
+
+namespace ssf {
+
+// ----------------------------------------------------------------- helpers
+
+// Workgroups are dispatched round-robin over the 8 XCDs; remap so each XCD works
+// on one contiguous eighth of the rows and the x-vector window of neighbouring
+// rows stays in that XCD's L2.
+__device__ __forceinline__ int xcd_block(int b, int nb)
+{
+  return ((nb & 7) == 0) ? (b & 7) * (nb >> 3) + (b >> 3) : b;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Sum over the block; the result is valid in every thread. Fixed order.
+template <typename T>
+__device__ T block_sum(T v)
+{
+  __shared__ T lds[TPB / 64];
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+  __syncthreads();
+  T s = lds[0];
+#pragma unroll
+  for (int i = 1; i < TPB / 64; i++) s += lds[i];
+  __syncthreads();
+  return s;
+}
+
+template <typename T>
+__device__ T reduce_partials(const T *__restrict__ p, int nb)
+{
+  T s = T(0);
+  for (int i = threadIdx.x; i < nb; i += TPB) s += p[i];
+  return block_sum(s);
+}
+
+template <typename T>
+__device__ __forceinline__ T ldnt(const T *p)
+{
+  return __builtin_nontemporal_load(p);
+}
+
+// dot of one CSR row with x by G lanes; the sum is valid in the group's lane 0
+template <typename T, int G>
+__device__ __forceinline__ T row_dot(const int *__restrict__ rowptr, const int *__restrict__ col,
+                                     const T *__restrict__ val, const T *__restrict__ x,
+                                     int row, int lane)
+{
+  const int beg = rowptr[row], end = rowptr[row + 1];
+  T s = T(0);
+  for (int k = beg + lane; k < end; k += G) s += ldnt(val + k) * x[ldnt(col + k)];
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, G);
+  return s;
+}
+
+__device__ __forceinline__ void chunk(int n, int &r0, int &r1)
+{
+  const int nb = gridDim.x;
+  const int rpb = (n + nb - 1) / nb;
+  const int lb = xcd_block(blockIdx.x, nb);
+  r0 = lb * rpb;
+  r1 = min(n, r0 + rpb);
+}
+
+}  // namespace ssf

@@ -14,7 +14,7 @@
 //   3. each lane adds its tile entries (and the mirror of tile (0,1)) into the
 //      reference-layout values through the element's slot map (coloured, so no
 //      other wave touches the same entries), plus the Dirichlet lift.
-#include "ssf_kernels.hpp"
+#include "ssf_device.hpp"
 
 namespace ssf {
 

@@ -49,16 +49,44 @@ struct PcgState {
   int done;        // set once the recurrence has stopped
 };
 
+// Device state of the mixed-precision (reliable-update) solve.
+struct MixState {
+  double nory64;   // ||b||^2
+  double rr64;     // ||b - A x||^2 at the last refresh
+  double best;     // smallest rr64 that was a 4x improvement
+  double scale;    // FP32 quantities are the FP64 ones times scale = 1/||r|| at the last refresh
+  double delta2;   // the FP32 recurrence refreshes after reducing its residual by delta^2
+  long long updates;
+  int bad;         // consecutive refreshes whose FP32 reduction the FP64 residual did not follow
+  int since_best;  // refreshes since rr64 last improved by 10 %
+  int final;       // the solve has ended (status set)
+  int status;      // 1 converged, 4 FP64 floor reached above res
+};
+
 // Krylov recurrence variants of the reference.
 enum PcgMode : int {
   MODE_SCR_CG_PREC = 0,  // symcomprow::cg_prec, Jacobi, squared test   GEFEL/scr.cpp:1093
   MODE_SCR_CG = 1,       // symcomprow::cg, no preconditioner          GEFEL/scr.cpp:786
   MODE_CR_CG_PREC = 2,   // comprow::cg_prec with noprecond            GEFEL/cr.cpp:1124
+  MODE_MIXED_INNER = 3,  // FP32 recurrence of the mixed solve: the direction is
+                         // updated before the test rr < nory stops it
 };
 
 struct SpmvTune {
+  int format = 0;    // 0: row-vector kernel, 1: LDS-tiled stream kernel
   int lanes = 8;     // lanes (of a 64-wide wavefront) per row
-  int blocks = 1024; // blocks of 256 threads (rows split in contiguous chunks)
+  int blocks = 1024; // blocks of 256 threads (rows / tiles split in contiguous chunks)
+  int idx16 = 0;     // stream kernel reads 16-bit tile-relative column offsets
+};
+
+// Device view of a full CSR matrix and its stream tiling.
+struct CsrDev {
+  int n = 0;
+  const int *rowptr = nullptr;
+  const int *col = nullptr;
+  const short *col16 = nullptr;  // col - first row of the tile (null if it does not fit)
+  const int *tiles = nullptr;    // first row of each tile, ntiles + 1 entries
+  int ntiles = 0;
 };
 
 struct Workspace {  // vectors and reduction scratch of one solve shape
@@ -73,6 +101,8 @@ struct Workspace {  // vectors and reduction scratch of one solve shape
   PcgState<float> *st32 = nullptr;
   PcgState<double> *st_host = nullptr;  // pinned
   PcgState<float> *st32_host = nullptr; // pinned
+  MixState *mix = nullptr;
+  MixState *mix_host = nullptr;         // pinned, 2 entries
 };
 
 }  // namespace ssf
@@ -82,6 +112,12 @@ struct ssf_ctx {
   hipStream_t stream = nullptr;
   int cus = 256;
   int live_matrices = 0;  // a context outlives every matrix created on it
+  // kernel sampling inside solves: the first SpMV of every iteration chunk is
+  // bracketed by HIP events on the solve stream; [0] FP64, [1] FP32 operator
+  bool prof = false;
+  double prof_us[2] = {0.0, 0.0};
+  long long prof_n[2] = {0, 0};
+  hipEvent_t prof_ev[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [parity][start/end]
 };
 
 struct ssf_matrix {
@@ -110,7 +146,22 @@ struct ssf_matrix {
   int *order = nullptr;      // element ids sorted by colour
   std::vector<int64_t> colour_start;
   std::vector<int64_t> asm_cn_host;  // for identity checks
+  // stream tiling: whole rows, at most STREAM_CAP entries per tile
+  int *tiles = nullptr;
+  int ntiles = 0;
+  short *col16 = nullptr;
   ssf::SpmvTune tune64, tune32;
   bool tuned64 = false, tuned32 = false;
+  ssf::CsrDev csr() const
+  {
+    ssf::CsrDev c;
+    c.n = (int)n;
+    c.rowptr = rowptr;
+    c.col = col;
+    c.col16 = col16;
+    c.tiles = tiles;
+    c.ntiles = ntiles;
+    return c;
+  }
   ssf::Workspace ws;
 };

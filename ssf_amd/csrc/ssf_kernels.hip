@@ -7,372 +7,9 @@
 // every consumer block re-reduces the whole partial array in the same fixed order,
 // so each block holds bit-identical scalars without inter-block synchronisation
 // and a system gives the same result whatever else runs on the device.
-#include "ssf_kernels.hpp"
+#include "ssf_device.hpp"
 
 namespace ssf {
-
-// ----------------------------------------------------------------- helpers
-
-// Workgroups are dispatched round-robin over the 8 XCDs; remap so each XCD works
-// on one contiguous eighth of the rows and the x-vector window of neighbouring
-// rows stays in that XCD's L2.
-__device__ __forceinline__ int xcd_block(int b, int nb)
-{
-  return ((nb & 7) == 0) ? (b & 7) * (nb >> 3) + (b >> 3) : b;
-}
-
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v)
-{
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-
-// Sum over the block; the result is valid in every thread. Fixed order.
-template <typename T>
-__device__ T block_sum(T v)
-{
-  __shared__ T lds[TPB / 64];
-  v = wave_sum(v);
-  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
-  __syncthreads();
-  T s = lds[0];
-#pragma unroll
-  for (int i = 1; i < TPB / 64; i++) s += lds[i];
-  __syncthreads();
-  return s;
-}
-
-template <typename T>
-__device__ T reduce_partials(const T *__restrict__ p, int nb)
-{
-  T s = T(0);
-  for (int i = threadIdx.x; i < nb; i += TPB) s += p[i];
-  return block_sum(s);
-}
-
-template <typename T>
-__device__ __forceinline__ T ldnt(const T *p)
-{
-  return __builtin_nontemporal_load(p);
-}
-
-// dot of one CSR row with x by G lanes; the sum is valid in the group's lane 0
-template <typename T, int G>
-__device__ __forceinline__ T row_dot(const int *__restrict__ rowptr, const int *__restrict__ col,
-                                     const T *__restrict__ val, const T *__restrict__ x,
-                                     int row, int lane)
-{
-  const int beg = rowptr[row], end = rowptr[row + 1];
-  T s = T(0);
-  for (int k = beg + lane; k < end; k += G) s += ldnt(val + k) * x[ldnt(col + k)];
-#pragma unroll
-  for (int off = G / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, G);
-  return s;
-}
-
-__device__ __forceinline__ void chunk(int n, int &r0, int &r1)
-{
-  const int nb = gridDim.x;
-  const int rpb = (n + nb - 1) / nb;
-  const int lb = xcd_block(blockIdx.x, nb);
-  r0 = lb * rpb;
-  r1 = min(n, r0 + rpb);
-}
-
-// ------------------------------------------------------------------- SpMV
-
-template <typename T, int G, int KIND>
-__global__ __launch_bounds__(TPB) void k_spmv(int n, const int *__restrict__ rowptr,
-                                              const int *__restrict__ col,
-                                              const T *__restrict__ val,
-                                              const T *__restrict__ x, T *__restrict__ y,
-                                              const T *__restrict__ b,
-                                              T *__restrict__ partial, const int *done,
-                                              long long *iter)
-{
-  if (done && *done) return;
-  if (iter && blockIdx.x == 0 && threadIdx.x == 0) *iter += 1;
-  int r0, r1;
-  chunk(n, r0, r1);
-  const int grp = threadIdx.x / G, lane = threadIdx.x % G;
-  constexpr int NG = TPB / G;
-  T acc = T(0);
-  for (int row = r0 + grp; row < r1; row += NG) {
-    T s = row_dot<T, G>(rowptr, col, val, x, row, lane);
-    if (lane == 0) {
-      if (KIND == 2) {
-        s = b[row] - s;
-        acc += s * s;
-      } else if (KIND == 1) {
-        acc += x[row] * s;
-      }
-      y[row] = s;
-    }
-  }
-  if (KIND != 0) {
-    const T t = block_sum(acc);
-    if (threadIdx.x == 0) partial[blockIdx.x] = t;
-  }
-}
-
-template <typename T, int KIND>
-static hipError_t spmv_g(const SpmvTune &t, int n, const int *rowptr, const int *col,
-                         const T *val, const T *x, T *y, const T *b, T *partial,
-                         const int *done, long long *iter, hipStream_t s)
-{
-  dim3 grid(t.blocks), block(TPB);
-  switch (t.lanes) {
-    case 4: hipLaunchKernelGGL((k_spmv<T, 4, KIND>), grid, block, 0, s, n, rowptr, col, val, x, y, b, partial, done, iter); break;
-    case 8: hipLaunchKernelGGL((k_spmv<T, 8, KIND>), grid, block, 0, s, n, rowptr, col, val, x, y, b, partial, done, iter); break;
-    case 16: hipLaunchKernelGGL((k_spmv<T, 16, KIND>), grid, block, 0, s, n, rowptr, col, val, x, y, b, partial, done, iter); break;
-    case 32: hipLaunchKernelGGL((k_spmv<T, 32, KIND>), grid, block, 0, s, n, rowptr, col, val, x, y, b, partial, done, iter); break;
-    default: hipLaunchKernelGGL((k_spmv<T, 64, KIND>), grid, block, 0, s, n, rowptr, col, val, x, y, b, partial, done, iter); break;
-  }
-  return hipGetLastError();
-}
-
-template <typename T>
-hipError_t spmv(const SpmvTune &t, int kind, int n, const int *rowptr, const int *col,
-                const T *val, const T *x, T *y, const T *b, T *partial, const int *done,
-                long long *iter, hipStream_t s)
-{
-  if (kind == 0) return spmv_g<T, 0>(t, n, rowptr, col, val, x, y, b, partial, done, iter, s);
-  if (kind == 1) return spmv_g<T, 1>(t, n, rowptr, col, val, x, y, b, partial, done, iter, s);
-  return spmv_g<T, 2>(t, n, rowptr, col, val, x, y, b, partial, done, iter, s);
-}
-template hipError_t spmv<double>(const SpmvTune &, int, int, const int *, const int *,
-                                 const double *, const double *, double *, const double *,
-                                 double *, const int *, long long *, hipStream_t);
-template hipError_t spmv<float>(const SpmvTune &, int, int, const int *, const int *,
-                                const float *, const float *, float *, const float *,
-                                float *, const int *, long long *, hipStream_t);
-
-// --------------------------------------------------------------------- PCG
-//
-// Per iteration i (GEFEL/scr.cpp:1142-1197):
-//   K1  p = A d, partial d.p                      (k_spmv KIND 1, increments iter)
-//   K2  den = d.p; |den| < zero -> stop (status 2)  x += a d, r -= a p, h = M^-1 r,
-//       partials r.r and r.h                       (k_pcg_update)
-//   K3  rr = r.r, nom' = r.h, stopping test of the mode, else d = b d + h
-//                                                  (k_pcg_dir)
-// The stop of K2/K3 is decided identically by every block from the same
-// partials, so no block continues once another would have stopped.
-
-template <typename T>
-__device__ __forceinline__ T jdiv(T r, T dg);
-template <>
-__device__ __forceinline__ double jdiv<double>(double r, double dg) { return r / dg; }
-template <>
-__device__ __forceinline__ float jdiv<float>(float r, float dg) { return r / dg; }
-
-template <typename T, int G, bool JAC, bool HAVEX>
-__global__ __launch_bounds__(TPB) void k_pcg_init(int n, const int *__restrict__ rowptr,
-                                                  const int *__restrict__ col,
-                                                  const T *__restrict__ val,
-                                                  const T *__restrict__ diag,
-                                                  const T *__restrict__ x,
-                                                  const T *__restrict__ y, T *__restrict__ r,
-                                                  T *__restrict__ h, T *__restrict__ d,
-                                                  T *__restrict__ part)
-{
-  int r0, r1;
-  chunk(n, r0, r1);
-  const int grp = threadIdx.x / G, lane = threadIdx.x % G;
-  constexpr int NG = TPB / G;
-  T ayy = T(0), arh = T(0);
-  for (int row = r0 + grp; row < r1; row += NG) {
-    T s = HAVEX ? row_dot<T, G>(rowptr, col, val, x, row, lane) : T(0);
-    if (lane == 0) {
-      const T yi = y[row];
-      const T ri = yi - s;
-      const T hi = JAC ? jdiv<T>(ri, diag[row]) : ri;
-      r[row] = ri;
-      if (JAC) h[row] = hi;
-      d[row] = hi;
-      ayy += yi * yi;
-      arh += ri * hi;
-    }
-  }
-  const T a = block_sum(ayy);
-  const T c = block_sum(arh);
-  if (threadIdx.x == 0) {
-    part[blockIdx.x] = a;
-    part[2 * gridDim.x + blockIdx.x] = c;
-  }
-}
-
-template <typename T>
-__global__ __launch_bounds__(TPB) void k_pcg_init2(const T *__restrict__ part, int nb,
-                                                   PcgState<T> *st, T zero)
-{
-  const T nory = reduce_partials(part, nb);
-  const T nom = reduce_partials(part + 2 * nb, nb);
-  if (threadIdx.x == 0) {
-    st->nory = nory;
-    st->nom[0] = nom;
-    st->nom[1] = nom;
-    st->iter = -1;
-    st->status = 0;
-    st->done = 0;
-    st->scale = T(1);
-    if (nory < zero) {
-      st->status = 3;
-      st->done = 1;
-      st->ani = 0;
-      st->ares = nory;
-    }
-  }
-}
-
-template <typename T, bool JAC>
-__global__ __launch_bounds__(TPB) void k_pcg_update(int n, PcgState<T> *st,
-                                                    const T *__restrict__ pden, T zero,
-                                                    T *__restrict__ x, T *__restrict__ r,
-                                                    const T *__restrict__ d,
-                                                    const T *__restrict__ p,
-                                                    const T *__restrict__ diag,
-                                                    T *__restrict__ h, T *__restrict__ prr)
-{
-  if (st->done) return;
-  const int nb = gridDim.x;
-  const T den = reduce_partials(pden, nb);
-  const long long it = st->iter;
-  const T nom = st->nom[it & 1];
-  if (fabs(den) < zero) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      st->status = 2;
-      st->ani = it;
-      st->ares = nom;
-      st->done = 1;
-    }
-    return;
-  }
-  const T alpha = nom / den;
-  int r0, r1;
-  chunk(n, r0, r1);
-  T rr = T(0), rh = T(0);
-  for (int i = r0 + threadIdx.x; i < r1; i += TPB) {
-    x[i] += alpha * d[i];
-    const T ri = r[i] - alpha * p[i];
-    r[i] = ri;
-    rr += ri * ri;
-    if (JAC) {
-      const T hi = jdiv<T>(ri, diag[i]);
-      h[i] = hi;
-      rh += ri * hi;
-    }
-  }
-  const T a = block_sum(rr);
-  if (threadIdx.x == 0) prr[blockIdx.x] = a;
-  if (JAC) {
-    const T c = block_sum(rh);
-    if (threadIdx.x == 0) prr[nb + blockIdx.x] = c;
-  }
-}
-
-template <typename T, bool JAC, int MODE>
-__global__ __launch_bounds__(TPB) void k_pcg_dir(int n, PcgState<T> *st,
-                                                 const T *__restrict__ prr, T res, T zero,
-                                                 T *__restrict__ d, const T *__restrict__ h)
-{
-  if (st->done) return;
-  const int nb = gridDim.x;
-  const T rr = reduce_partials(prr, nb);
-  const T nomn = JAC ? reduce_partials(prr + nb, nb) : rr;
-  const long long it = st->iter;
-  const T nom = st->nom[it & 1];
-  const T nory = st->nory;
-  bool stop;
-  if (MODE == MODE_SCR_CG_PREC) stop = fabs(rr / nory) < res;                 // scr.cpp:1188
-  else if (MODE == MODE_SCR_CG) stop = (sqrt(nomn / nory) < res) || (fabs(nomn) < zero); // :862-870
-  else stop = sqrt(nomn / nory) < res;                                        // cr.cpp:1217
-  if (stop) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      st->ani = it;
-      st->ares = (MODE == MODE_SCR_CG_PREC) ? nomn / nory : nomn;
-      st->status = 1;
-      st->nom[(it + 1) & 1] = nomn;
-      st->done = 1;
-    }
-    return;
-  }
-  const T beta = nomn / nom;
-  int r0, r1;
-  chunk(n, r0, r1);
-  for (int i = r0 + threadIdx.x; i < r1; i += TPB) d[i] = beta * d[i] + h[i];
-  if (blockIdx.x == 0 && threadIdx.x == 0) st->nom[(it + 1) & 1] = nomn;
-}
-
-template <typename T, bool JAC, bool HAVEX>
-static void init_g(const SpmvTune &t, int n, const int *rowptr, const int *col, const T *val,
-                   const T *diag, const T *x, const T *y, T *r, T *h, T *d, T *part,
-                   hipStream_t s)
-{
-  dim3 grid(t.blocks), block(TPB);
-  switch (t.lanes) {
-    case 4: hipLaunchKernelGGL((k_pcg_init<T, 4, JAC, HAVEX>), grid, block, 0, s, n, rowptr, col, val, diag, x, y, r, h, d, part); break;
-    case 8: hipLaunchKernelGGL((k_pcg_init<T, 8, JAC, HAVEX>), grid, block, 0, s, n, rowptr, col, val, diag, x, y, r, h, d, part); break;
-    case 16: hipLaunchKernelGGL((k_pcg_init<T, 16, JAC, HAVEX>), grid, block, 0, s, n, rowptr, col, val, diag, x, y, r, h, d, part); break;
-    case 32: hipLaunchKernelGGL((k_pcg_init<T, 32, JAC, HAVEX>), grid, block, 0, s, n, rowptr, col, val, diag, x, y, r, h, d, part); break;
-    default: hipLaunchKernelGGL((k_pcg_init<T, 64, JAC, HAVEX>), grid, block, 0, s, n, rowptr, col, val, diag, x, y, r, h, d, part); break;
-  }
-}
-
-template <typename T>
-hipError_t pcg_init(const SpmvTune &t, bool jac, bool havex, int n, const int *rowptr,
-                    const int *col, const T *val, const T *diag, const T *x, const T *y,
-                    T *r, T *h, T *d, T *part, PcgState<T> *st, T zero, hipStream_t s)
-{
-  if (jac && havex) init_g<T, true, true>(t, n, rowptr, col, val, diag, x, y, r, h, d, part, s);
-  else if (jac) init_g<T, true, false>(t, n, rowptr, col, val, diag, x, y, r, h, d, part, s);
-  else if (havex) init_g<T, false, true>(t, n, rowptr, col, val, diag, x, y, r, h, d, part, s);
-  else init_g<T, false, false>(t, n, rowptr, col, val, diag, x, y, r, h, d, part, s);
-  hipLaunchKernelGGL((k_pcg_init2<T>), dim3(1), dim3(TPB), 0, s, part, t.blocks, st, zero);
-  return hipGetLastError();
-}
-template hipError_t pcg_init<double>(const SpmvTune &, bool, bool, int, const int *, const int *,
-                                     const double *, const double *, const double *,
-                                     const double *, double *, double *, double *, double *,
-                                     PcgState<double> *, double, hipStream_t);
-template hipError_t pcg_init<float>(const SpmvTune &, bool, bool, int, const int *, const int *,
-                                    const float *, const float *, const float *, const float *,
-                                    float *, float *, float *, float *, PcgState<float> *, float,
-                                    hipStream_t);
-
-template <typename T>
-hipError_t pcg_iteration(const SpmvTune &t, int mode, bool jac, int n, const int *rowptr,
-                         const int *col, const T *val, const T *diag, T *x, T *r, T *d,
-                         T *p, T *h, T *part, PcgState<T> *st, T res, T zero, hipStream_t s)
-{
-  hipError_t e = spmv<T>(t, 1, n, rowptr, col, val, d, p, nullptr, part, &st->done, &st->iter, s);
-  if (e != hipSuccess) return e;
-  dim3 grid(t.blocks), block(TPB);
-  T *prr = part + t.blocks;
-  if (jac) hipLaunchKernelGGL((k_pcg_update<T, true>), grid, block, 0, s, n, st, part, zero, x, r, d, p, diag, h, prr);
-  else hipLaunchKernelGGL((k_pcg_update<T, false>), grid, block, 0, s, n, st, part, zero, x, r, d, p, diag, h, prr);
-  const T *hh = jac ? h : r;
-  if (jac) {
-    if (mode == MODE_SCR_CG_PREC) hipLaunchKernelGGL((k_pcg_dir<T, true, MODE_SCR_CG_PREC>), grid, block, 0, s, n, st, prr, res, zero, d, hh);
-    else if (mode == MODE_SCR_CG) hipLaunchKernelGGL((k_pcg_dir<T, true, MODE_SCR_CG>), grid, block, 0, s, n, st, prr, res, zero, d, hh);
-    else hipLaunchKernelGGL((k_pcg_dir<T, true, MODE_CR_CG_PREC>), grid, block, 0, s, n, st, prr, res, zero, d, hh);
-  } else {
-    if (mode == MODE_SCR_CG_PREC) hipLaunchKernelGGL((k_pcg_dir<T, false, MODE_SCR_CG_PREC>), grid, block, 0, s, n, st, prr, res, zero, d, hh);
-    else if (mode == MODE_SCR_CG) hipLaunchKernelGGL((k_pcg_dir<T, false, MODE_SCR_CG>), grid, block, 0, s, n, st, prr, res, zero, d, hh);
-    else hipLaunchKernelGGL((k_pcg_dir<T, false, MODE_CR_CG_PREC>), grid, block, 0, s, n, st, prr, res, zero, d, hh);
-  }
-  return hipGetLastError();
-}
-template hipError_t pcg_iteration<double>(const SpmvTune &, int, bool, int, const int *,
-                                          const int *, const double *, const double *,
-                                          double *, double *, double *, double *, double *,
-                                          double *, PcgState<double> *, double, double,
-                                          hipStream_t);
-template hipError_t pcg_iteration<float>(const SpmvTune &, int, bool, int, const int *,
-                                         const int *, const float *, const float *, float *,
-                                         float *, float *, float *, float *, float *,
-                                         PcgState<float> *, float, float, hipStream_t);
 
 // ------------------------------------------------------- element-wise helpers
 
@@ -493,6 +130,38 @@ __global__ void k_reduce_slot(const double *__restrict__ part, int nb, double *o
 hipError_t reduce_to_host_slot(const double *part, int nb, double *out, hipStream_t s)
 {
   hipLaunchKernelGGL(k_reduce_slot, dim3(1), dim3(TPB), 0, s, part, nb, out);
+  return hipGetLastError();
+}
+
+// plain streaming copy (16-byte accesses): the HBM reference point of the
+// roofline and of the counter calibration
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+
+__global__ __launch_bounds__(TPB) void k_copy(int64_t n2, const dvec2 *__restrict__ a,
+                                              dvec2 *__restrict__ b)
+{
+  // 4 independent 16-byte loads in flight per thread before the stores
+  const int64_t stride = (int64_t)gridDim.x * TPB;
+  int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  for (; i + 3 * stride < n2; i += 4 * stride) {
+    const dvec2 v0 = __builtin_nontemporal_load(a + i);
+    const dvec2 v1 = __builtin_nontemporal_load(a + i + stride);
+    const dvec2 v2 = __builtin_nontemporal_load(a + i + 2 * stride);
+    const dvec2 v3 = __builtin_nontemporal_load(a + i + 3 * stride);
+    __builtin_nontemporal_store(v0, b + i);
+    __builtin_nontemporal_store(v1, b + i + stride);
+    __builtin_nontemporal_store(v2, b + i + 2 * stride);
+    __builtin_nontemporal_store(v3, b + i + 3 * stride);
+  }
+  for (; i < n2; i += stride) b[i] = __builtin_nontemporal_load(a + i);
+}
+
+hipError_t copy_kernel(int64_t n, const double *a, double *b, hipStream_t s)
+{
+  const int64_t n2 = n / 2;
+  if (n2 == 0) return hipSuccess;
+  const int64_t nb = std::min<int64_t>((n2 + TPB - 1) / TPB, 4096);
+  hipLaunchKernelGGL(k_copy, dim3((unsigned)nb), dim3(TPB), 0, s, n2, (const dvec2 *)a, (dvec2 *)b);
   return hipGetLastError();
 }
 

@@ -1,30 +1,41 @@
-// Launchers of the gfx950 kernels (defined in ssf_kernels.hip).
+// Launchers of the gfx950 kernels (defined in ssf_kernels.hip / ssf_elast.hip).
 #pragma once
 
 #include "ssf_internal.hpp"
 
 namespace ssf {
 
-constexpr int TPB = 256;  // threads per block for every streaming kernel
+constexpr int TPB = 256;          // threads per block for every streaming kernel
+constexpr int STREAM_KPT = 8;     // matrix entries per thread and tile in the stream SpMV
+constexpr int STREAM_CAP = TPB * STREAM_KPT;  // entries per stream tile
 
 // y = A x (kind 0); y = A x with partial[b] = sum x_i y_i (kind 1);
 // y = b - A x with partial[b] = sum y_i^2 (kind 2). done (may be null) skips the
 // launch body once set; iter (may be null) is incremented by block 0.
 template <typename T>
-hipError_t spmv(const SpmvTune &t, int kind, int n, const int *rowptr, const int *col,
-                const T *val, const T *x, T *y, const T *b, T *partial, const int *done,
-                long long *iter, hipStream_t s);
+hipError_t spmv(const SpmvTune &t, int kind, const CsrDev &A, const T *val, const T *x, T *y,
+                const T *b, T *partial, const int *done, long long *iter, hipStream_t s);
 
 // PCG pieces (see ssf_kernels.hip for the recurrence).
 template <typename T>
-hipError_t pcg_init(const SpmvTune &t, bool jac, bool havex, int n, const int *rowptr,
-                    const int *col, const T *val, const T *diag, const T *x, const T *y,
-                    T *r, T *h, T *d, T *part, PcgState<T> *st, T zero, hipStream_t s);
+hipError_t pcg_init(const SpmvTune &t, bool jac, bool havex, const CsrDev &A, const T *val,
+                    const T *diag, const T *x, const T *y, T *r, T *h, T *d, T *part,
+                    PcgState<T> *st, T zero, hipStream_t s);
 template <typename T>
-hipError_t pcg_iteration(const SpmvTune &t, int mode, bool jac, int n, const int *rowptr,
-                         const int *col, const T *val, const T *diag, T *x, T *r, T *d,
-                         T *p, T *h, T *part, PcgState<T> *st, T res, T zero,
-                         hipStream_t s);
+hipError_t pcg_iteration(const SpmvTune &t, int mode, bool jac, const CsrDev &A, const T *val,
+                         const T *diag, T *x, T *r, T *d, T *p, T *h, T *part,
+                         PcgState<T> *st, T res, T zero, hipStream_t s,
+                         hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+
+// mixed precision with reliable updates (ssf_solver.hip)
+hipError_t mixed_init(MixState *ms, PcgState<float> *st, const double *slots, double res,
+                      double delta, hipStream_t s);
+hipError_t mixed_update(const SpmvTune &t64, const SpmvTune &t32, const CsrDev &A,
+                        const double *val64, const float *diag32, const double *b,
+                        double *x64, double *r64, double *part64, float *x32, float *r32,
+                        float *h32, float *d32, float *part32, PcgState<float> *st,
+                        MixState *ms, double res, hipStream_t s);
+hipError_t copy_kernel(int64_t n, const double *a, double *b, hipStream_t s);
 
 // element-wise helpers
 hipError_t expand_ref(int64_t negm, const int *map_lo, const int *map_up, const double *a,

@@ -1,0 +1,563 @@
+// CSR SpMV (row-vector and LDS-tiled stream kernels) and the PCG recurrences of
+// the SIFEL solve path on gfx950.
+//
+// SpMV kernels, both deterministic for a fixed tiling:
+//  * row-vector: G lanes of a wavefront per row, strided over the row's entries,
+//    shuffle reduction; rows split in contiguous chunks per block.
+//  * stream: the rows are cut into tiles of whole rows holding at most
+//    STREAM_CAP entries. Each thread of a block issues STREAM_KPT independent,
+//    coalesced loads of values and column indices (32- or 16-bit, the latter
+//    relative to the tile's first row), multiplies by the gathered x and leaves
+//    the products in LDS; then G lanes per row sum their row's products in a
+//    fixed order. Tiles are split in contiguous ranges per block.
+//
+// PCG per iteration i (GEFEL/scr.cpp:1142-1197):
+//   K1  p = A d, partial d.p                       (SpMV kind 1, increments iter)
+//   K2  den = d.p; |den| < zero -> stop (status 2); x += a d, r -= a p, h = M^-1 r,
+//       partials r.r and r.h                        (k_pcg_update)
+//   K3  rr = r.r, nom' = r.h, stopping test of the mode, else d = b d + h
+//                                                   (k_pcg_dir)
+// Scalars are re-reduced from the per-block partials by every consumer block in
+// the same order, so all blocks take the same decision without inter-block
+// synchronisation.
+#include "ssf_device.hpp"
+
+namespace ssf {
+
+// ------------------------------------------------------------ row-vector SpMV
+
+template <typename T, int G, int KIND>
+__device__ __forceinline__ void finish_row(int row, T s, const T *__restrict__ x,
+                                           T *__restrict__ y, const T *__restrict__ b, T &acc)
+{
+  if (KIND == 2) {
+    s = b[row] - s;
+    acc += s * s;
+  } else if (KIND == 1) {
+    acc += x[row] * s;
+  }
+  y[row] = s;
+}
+
+template <typename T, int G, int KIND>
+__global__ __launch_bounds__(TPB) void k_spmv_vec(CsrDev A, const T *__restrict__ val,
+                                                  const T *__restrict__ x, T *__restrict__ y,
+                                                  const T *__restrict__ b,
+                                                  T *__restrict__ partial, const int *done,
+                                                  long long *iter)
+{
+  if (done && *done) return;
+  if (iter && blockIdx.x == 0 && threadIdx.x == 0) *iter += 1;
+  int r0, r1;
+  chunk(A.n, r0, r1);
+  const int grp = threadIdx.x / G, lane = threadIdx.x % G;
+  constexpr int NG = TPB / G;
+  T acc = T(0);
+  for (int row = r0 + grp; row < r1; row += NG) {
+    const T s = row_dot<T, G>(A.rowptr, A.col, val, x, row, lane);
+    if (lane == 0) finish_row<T, G, KIND>(row, s, x, y, b, acc);
+  }
+  if (KIND != 0) {
+    const T t = block_sum(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+  }
+}
+
+// ---------------------------------------------------------------- stream SpMV
+
+template <typename T, bool I16, int G, int KIND>
+__global__ __launch_bounds__(TPB) void k_spmv_stream(CsrDev A, const T *__restrict__ val,
+                                                     const T *__restrict__ x,
+                                                     T *__restrict__ y, const T *__restrict__ b,
+                                                     T *__restrict__ partial, const int *done,
+                                                     long long *iter)
+{
+  __shared__ T prod[STREAM_CAP];
+  if (done && *done) return;
+  if (iter && blockIdx.x == 0 && threadIdx.x == 0) *iter += 1;
+  const int nb = gridDim.x;
+  const int tpb = (A.ntiles + nb - 1) / nb;
+  const int lb = xcd_block(blockIdx.x, nb);
+  const int t0 = lb * tpb, t1 = min(A.ntiles, t0 + tpb);
+  const int grp = threadIdx.x / G, lane = threadIdx.x % G;
+  constexpr int NG = TPB / G;
+  T acc = T(0);
+  // software pipeline: the values and columns of tile t+1 are loaded into
+  // registers while tile t is reduced from LDS
+  T v[STREAM_KPT];
+  int c[STREAM_KPT];
+  auto load_tile = [&](int tile) {
+    const int r0 = A.tiles[tile];
+    const int s = A.rowptr[r0], e = A.rowptr[A.tiles[tile + 1]];
+#pragma unroll
+    for (int j = 0; j < STREAM_KPT; j++) {
+      const int k = s + j * TPB + threadIdx.x;
+      if (k < e) {
+        v[j] = ldnt(val + k);
+        c[j] = I16 ? r0 + (int)ldnt(A.col16 + k) : ldnt(A.col + k);
+      } else {
+        v[j] = T(0);
+        c[j] = -1;
+      }
+    }
+  };
+  if (t0 < t1) load_tile(t0);
+  for (int tile = t0; tile < t1; tile++) {
+    const int r0 = A.tiles[tile], r1 = A.tiles[tile + 1];
+    const int s = A.rowptr[r0];
+    // phase 1: products of the streamed entries with the gathered x
+#pragma unroll
+    for (int j = 0; j < STREAM_KPT; j++)
+      if (c[j] >= 0) prod[j * TPB + threadIdx.x] = v[j] * x[c[j]];
+    if (tile + 1 < t1) load_tile(tile + 1);
+    __syncthreads();
+    // phase 2: G lanes per row, fixed order
+    const int nrows = r1 - r0;
+    for (int rr = grp; rr < nrows; rr += NG) {
+      const int row = r0 + rr;
+      const int k0 = A.rowptr[row] - s, k1 = A.rowptr[row + 1] - s;
+      T sum = T(0);
+      for (int k = k0 + lane; k < k1; k += G) sum += prod[k];
+#pragma unroll
+      for (int off = G / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, G);
+      if (lane == 0) finish_row<T, G, KIND>(row, sum, x, y, b, acc);
+    }
+    __syncthreads();
+  }
+  if (KIND != 0) {
+    const T t = block_sum(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+  }
+}
+
+template <typename T, int KIND>
+static hipError_t spmv_k(const SpmvTune &t, const CsrDev &A, const T *val, const T *x, T *y,
+                         const T *b, T *partial, const int *done, long long *iter,
+                         hipStream_t s)
+{
+  dim3 grid(t.blocks), block(TPB);
+#define SSF_VEC(G) hipLaunchKernelGGL((k_spmv_vec<T, G, KIND>), grid, block, 0, s, A, val, x, y, b, partial, done, iter)
+#define SSF_STR(I, G) hipLaunchKernelGGL((k_spmv_stream<T, I, G, KIND>), grid, block, 0, s, A, val, x, y, b, partial, done, iter)
+  if (t.format == 1 && A.ntiles > 0) {
+    const bool i16 = t.idx16 && A.col16;
+    switch (t.lanes) {
+      case 4: if (i16) SSF_STR(true, 4); else SSF_STR(false, 4); break;
+      case 8: if (i16) SSF_STR(true, 8); else SSF_STR(false, 8); break;
+      default: if (i16) SSF_STR(true, 16); else SSF_STR(false, 16); break;
+    }
+  } else {
+    switch (t.lanes) {
+      case 4: SSF_VEC(4); break;
+      case 8: SSF_VEC(8); break;
+      case 16: SSF_VEC(16); break;
+      case 32: SSF_VEC(32); break;
+      default: SSF_VEC(64); break;
+    }
+  }
+#undef SSF_VEC
+#undef SSF_STR
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t spmv(const SpmvTune &t, int kind, const CsrDev &A, const T *val, const T *x, T *y,
+                const T *b, T *partial, const int *done, long long *iter, hipStream_t s)
+{
+  if (kind == 0) return spmv_k<T, 0>(t, A, val, x, y, b, partial, done, iter, s);
+  if (kind == 1) return spmv_k<T, 1>(t, A, val, x, y, b, partial, done, iter, s);
+  return spmv_k<T, 2>(t, A, val, x, y, b, partial, done, iter, s);
+}
+template hipError_t spmv<double>(const SpmvTune &, int, const CsrDev &, const double *,
+                                 const double *, double *, const double *, double *,
+                                 const int *, long long *, hipStream_t);
+template hipError_t spmv<float>(const SpmvTune &, int, const CsrDev &, const float *,
+                                const float *, float *, const float *, float *, const int *,
+                                long long *, hipStream_t);
+
+// ----------------------------------------------------------------------- PCG
+
+// r = y - q (q = A x, or 0), h = M^-1 r, d = h, partials y.y and r.h
+template <typename T, bool JAC, bool HAVEX>
+__global__ __launch_bounds__(TPB) void k_pcg_init(int n, const T *__restrict__ diag,
+                                                  const T *__restrict__ y, T *__restrict__ r,
+                                                  T *__restrict__ h, T *__restrict__ d,
+                                                  T *__restrict__ part)
+{
+  int r0, r1;
+  chunk(n, r0, r1);
+  T ayy = T(0), arh = T(0);
+  for (int i = r0 + threadIdx.x; i < r1; i += TPB) {
+    const T yi = y[i];
+    const T ri = HAVEX ? yi - r[i] : yi;
+    const T hi = JAC ? ri / diag[i] : ri;
+    r[i] = ri;
+    if (JAC) h[i] = hi;
+    d[i] = hi;
+    ayy += yi * yi;
+    arh += ri * hi;
+  }
+  const T a = block_sum(ayy);
+  const T c = block_sum(arh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = a;
+    part[2 * gridDim.x + blockIdx.x] = c;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_pcg_init2(const T *__restrict__ part, int nb,
+                                                   PcgState<T> *st, T zero)
+{
+  const T nory = reduce_partials(part, nb);
+  const T nom = reduce_partials(part + 2 * nb, nb);
+  if (threadIdx.x == 0) {
+    st->nory = nory;
+    st->nom[0] = nom;
+    st->nom[1] = nom;
+    st->iter = -1;
+    st->status = 0;
+    st->done = 0;
+    st->scale = T(1);
+    if (nory < zero) {
+      st->status = 3;
+      st->done = 1;
+      st->ani = 0;
+      st->ares = nory;
+    }
+  }
+}
+
+template <typename T, bool JAC>
+__global__ __launch_bounds__(TPB) void k_pcg_update(int n, PcgState<T> *st,
+                                                    const T *__restrict__ pden, int npden,
+                                                    T zero, T *__restrict__ x,
+                                                    T *__restrict__ r, const T *__restrict__ d,
+                                                    const T *__restrict__ p,
+                                                    const T *__restrict__ diag,
+                                                    T *__restrict__ h, T *__restrict__ prr)
+{
+  if (st->done) return;
+  const int nb = gridDim.x;
+  const T den = reduce_partials(pden, npden);
+  const long long it = st->iter;
+  const T nom = st->nom[it & 1];
+  if (fabs(den) < zero) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->status = 2;
+      st->ani = it;
+      st->ares = nom;
+      st->done = 1;
+    }
+    return;
+  }
+  const T alpha = nom / den;
+  int r0, r1;
+  chunk(n, r0, r1);
+  T rr = T(0), rh = T(0);
+  for (int i = r0 + threadIdx.x; i < r1; i += TPB) {
+    x[i] += alpha * d[i];
+    const T ri = r[i] - alpha * p[i];
+    r[i] = ri;
+    rr += ri * ri;
+    if (JAC) {
+      const T hi = ri / diag[i];
+      h[i] = hi;
+      rh += ri * hi;
+    }
+  }
+  const T a = block_sum(rr);
+  if (threadIdx.x == 0) prr[blockIdx.x] = a;
+  if (JAC) {
+    const T c = block_sum(rh);
+    if (threadIdx.x == 0) prr[nb + blockIdx.x] = c;
+  }
+}
+
+template <typename T, bool JAC, int MODE>
+__global__ __launch_bounds__(TPB) void k_pcg_dir(int n, PcgState<T> *st,
+                                                 const T *__restrict__ prr, T res, T zero,
+                                                 T *__restrict__ d, const T *__restrict__ h)
+{
+  if (st->done) return;
+  const int nb = gridDim.x;
+  const T rr = reduce_partials(prr, nb);
+  const T nomn = JAC ? reduce_partials(prr + nb, nb) : rr;
+  const long long it = st->iter;
+  const T nom = st->nom[it & 1];
+  const T nory = st->nory;
+  if (MODE == MODE_MIXED_INNER) {
+    // the direction is always updated, then rr < nory (the refresh threshold)
+    // pauses the recurrence until the FP64 refresh
+    const T beta = nomn / nom;
+    int r0, r1;
+    chunk(n, r0, r1);
+    for (int i = r0 + threadIdx.x; i < r1; i += TPB) d[i] = beta * d[i] + h[i];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->nom[(it + 1) & 1] = nomn;
+      if (rr < nory) {
+        st->ani = it;
+        st->status = 1;
+        st->done = 1;
+      }
+    }
+    return;
+  }
+  bool stop;
+  if (MODE == MODE_SCR_CG_PREC) stop = fabs(rr / nory) < res;                         // scr.cpp:1188
+  else if (MODE == MODE_SCR_CG) stop = (sqrt(nomn / nory) < res) || (fabs(nomn) < zero);  // :862-870
+  else stop = sqrt(nomn / nory) < res;                                                // cr.cpp:1217
+  if (stop) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->ani = it;
+      st->ares = (MODE == MODE_SCR_CG_PREC) ? nomn / nory : nomn;
+      st->status = 1;
+      st->nom[(it + 1) & 1] = nomn;
+      st->done = 1;
+    }
+    return;
+  }
+  const T beta = nomn / nom;
+  int r0, r1;
+  chunk(n, r0, r1);
+  for (int i = r0 + threadIdx.x; i < r1; i += TPB) d[i] = beta * d[i] + h[i];
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->nom[(it + 1) & 1] = nomn;
+}
+
+// the element-wise PCG kernels use at most this many blocks
+static int vec_blocks(const SpmvTune &t, int n)
+{
+  int nb = std::min(t.blocks, 1024);
+  const int need = (n + TPB * 4 - 1) / (TPB * 4);
+  nb = std::max(8, std::min(nb, need));
+  return (nb + 7) / 8 * 8;
+}
+
+template <typename T>
+hipError_t pcg_init(const SpmvTune &t, bool jac, bool havex, const CsrDev &A, const T *val,
+                    const T *diag, const T *x, const T *y, T *r, T *h, T *d, T *part,
+                    PcgState<T> *st, T zero, hipStream_t s)
+{
+  if (havex) {
+    hipError_t e = spmv<T>(t, 0, A, val, x, r, nullptr, nullptr, nullptr, nullptr, s);
+    if (e != hipSuccess) return e;
+  }
+  const int nb = vec_blocks(t, A.n);
+  dim3 grid(nb), block(TPB);
+  if (jac && havex) hipLaunchKernelGGL((k_pcg_init<T, true, true>), grid, block, 0, s, A.n, diag, y, r, h, d, part);
+  else if (jac) hipLaunchKernelGGL((k_pcg_init<T, true, false>), grid, block, 0, s, A.n, diag, y, r, h, d, part);
+  else if (havex) hipLaunchKernelGGL((k_pcg_init<T, false, true>), grid, block, 0, s, A.n, diag, y, r, h, d, part);
+  else hipLaunchKernelGGL((k_pcg_init<T, false, false>), grid, block, 0, s, A.n, diag, y, r, h, d, part);
+  hipLaunchKernelGGL((k_pcg_init2<T>), dim3(1), dim3(TPB), 0, s, part, nb, st, zero);
+  return hipGetLastError();
+}
+template hipError_t pcg_init<double>(const SpmvTune &, bool, bool, const CsrDev &, const double *,
+                                     const double *, const double *, const double *, double *,
+                                     double *, double *, double *, PcgState<double> *, double,
+                                     hipStream_t);
+template hipError_t pcg_init<float>(const SpmvTune &, bool, bool, const CsrDev &, const float *,
+                                    const float *, const float *, const float *, float *, float *,
+                                    float *, float *, PcgState<float> *, float, hipStream_t);
+
+// Partial layout in part[]: [0, t.blocks) d.p of K1; then, for the element-wise
+// kernels with nv blocks, [P1, P1 + nv) r.r and [P1 + nv, P1 + 2 nv) r.h, with
+// P1 = t.blocks.
+template <typename T>
+hipError_t pcg_iteration(const SpmvTune &t, int mode, bool jac, const CsrDev &A, const T *val,
+                         const T *diag, T *x, T *r, T *d, T *p, T *h, T *part,
+                         PcgState<T> *st, T res, T zero, hipStream_t s, hipEvent_t ev0,
+                         hipEvent_t ev1)
+{
+  if (ev0) (void)hipEventRecord(ev0, s);
+  hipError_t e = spmv<T>(t, 1, A, val, d, p, nullptr, part, &st->done, &st->iter, s);
+  if (e != hipSuccess) return e;
+  if (ev1) (void)hipEventRecord(ev1, s);
+  const int nv = vec_blocks(t, A.n);
+  dim3 grid(nv), block(TPB);
+  T *prr = part + t.blocks;
+  const int n = A.n;
+  if (jac) hipLaunchKernelGGL((k_pcg_update<T, true>), grid, block, 0, s, n, st, part, t.blocks, zero, x, r, d, p, diag, h, prr);
+  else hipLaunchKernelGGL((k_pcg_update<T, false>), grid, block, 0, s, n, st, part, t.blocks, zero, x, r, d, p, diag, h, prr);
+  const T *hh = jac ? h : r;
+#define SSF_DIR(J, M) hipLaunchKernelGGL((k_pcg_dir<T, J, M>), grid, block, 0, s, n, st, prr, res, zero, d, hh)
+  if (jac) {
+    if (mode == MODE_SCR_CG_PREC) SSF_DIR(true, MODE_SCR_CG_PREC);
+    else if (mode == MODE_SCR_CG) SSF_DIR(true, MODE_SCR_CG);
+    else if (mode == MODE_MIXED_INNER) SSF_DIR(true, MODE_MIXED_INNER);
+    else SSF_DIR(true, MODE_CR_CG_PREC);
+  } else {
+    if (mode == MODE_SCR_CG_PREC) SSF_DIR(false, MODE_SCR_CG_PREC);
+    else if (mode == MODE_SCR_CG) SSF_DIR(false, MODE_SCR_CG);
+    else SSF_DIR(false, MODE_CR_CG_PREC);
+  }
+#undef SSF_DIR
+  return hipGetLastError();
+}
+template hipError_t pcg_iteration<double>(const SpmvTune &, int, bool, const CsrDev &,
+                                          const double *, const double *, double *, double *,
+                                          double *, double *, double *, double *,
+                                          PcgState<double> *, double, double, hipStream_t,
+                                          hipEvent_t, hipEvent_t);
+template hipError_t pcg_iteration<float>(const SpmvTune &, int, bool, const CsrDev &,
+                                         const float *, const float *, float *, float *, float *,
+                                         float *, float *, float *, PcgState<float> *, float,
+                                         float, hipStream_t, hipEvent_t, hipEvent_t);
+
+}  // namespace ssf
+
+// ------------------------------------------- mixed precision: reliable updates
+//
+// FP32 Jacobi-PCG on float(A) for the residual scaled by s = 1/||r|| (the FP64
+// residual norm at the last refresh, so FP32 values stay O(1)). Every K
+// iterations (and whenever the FP32 recurrence meets the test) the correction is
+// folded into the FP64 solution and the residual is recomputed in FP64:
+//   x64 += x32 / s, x32 = 0;  r64 = b - A x64 (FP64);  s' = 1/||r64||;
+//   r32 = s' r64, h32 = M^-1 r32, d32 *= s'/s;  FP64 test ||r64||^2 < res ||b||^2.
+// The FP32 search direction is carried across the refresh (rescaled with the
+// residual, which leaves alpha*d consistent), so the Krylov recurrence continues
+// on the corrected residual and the iteration count stays close to the FP64
+// recurrence while each iteration moves the FP32 traffic.
+
+namespace ssf {
+
+__global__ __launch_bounds__(TPB) void k_mx_xupdate(int n, const MixState *ms,
+                                                    double *__restrict__ x64,
+                                                    float *__restrict__ x32)
+{
+  if (ms->final) return;
+  const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (i >= n) return;
+  x64[i] += (double)x32[i] / ms->scale;
+  x32[i] = 0.0f;
+}
+
+__device__ __forceinline__ double refresh_scale(double rr64, double old)
+{
+  return rr64 > 0.0 ? 1.0 / sqrt(rr64) : old;
+}
+
+__global__ __launch_bounds__(TPB) void k_mx_refresh(int n, const MixState *ms,
+                                                    const double *__restrict__ part64, int nb64,
+                                                    const double *__restrict__ r64,
+                                                    const float *__restrict__ diag32,
+                                                    float *__restrict__ r32,
+                                                    float *__restrict__ h32,
+                                                    float *__restrict__ d32,
+                                                    float *__restrict__ prr)
+{
+  if (ms->final) return;
+  const double rr64 = reduce_partials(part64, nb64);
+  const double s = refresh_scale(rr64, ms->scale);
+  const float c = (float)(s / ms->scale);
+  // a true residual 1e4 x above its best means the carried direction is spoiled:
+  // restart it from the preconditioned residual
+  const bool restart = rr64 > 1e4 * ms->best;
+  int r0, r1;
+  chunk(n, r0, r1);
+  float rr = 0.f, rh = 0.f;
+  for (int i = r0 + threadIdx.x; i < r1; i += TPB) {
+    const float ri = (float)(r64[i] * s);
+    const float hi = ri / diag32[i];
+    r32[i] = ri;
+    h32[i] = hi;
+    d32[i] = restart ? hi : d32[i] * c;
+    rr += ri * ri;
+    rh += ri * hi;
+  }
+  const float a = block_sum(rr);
+  const float b = block_sum(rh);
+  if (threadIdx.x == 0) {
+    prr[blockIdx.x] = a;
+    prr[gridDim.x + blockIdx.x] = b;
+  }
+}
+
+__global__ __launch_bounds__(TPB) void k_mx_check(MixState *ms, PcgState<float> *st,
+                                                  const double *__restrict__ part64, int nb64,
+                                                  const float *__restrict__ prr, int nv,
+                                                  double res)
+{
+  if (ms->final) return;
+  const double rr64 = reduce_partials(part64, nb64);
+  const float nomn = reduce_partials(prr + nv, nv);
+  if (threadIdx.x != 0) return;
+  const double prev = ms->rr64;
+  ms->rr64 = rr64;
+  ms->updates += 1;
+  if (rr64 < res * ms->nory64) {
+    ms->final = 1;
+    ms->status = 1;
+    st->done = 1;
+    return;
+  }
+  // FP64 accuracy floor above res: the FP32 recurrence met its reduction three
+  // refreshes in a row while the FP64 residual did not even halve, or the FP64
+  // residual has not improved by 10 % in ten refreshes
+  const bool met = st->done && st->status == 1;
+  ms->bad = (met && rr64 > 0.25 * prev) ? ms->bad + 1 : 0;
+  if (rr64 < 0.81 * ms->best) ms->since_best = 0;
+  else ms->since_best += 1;
+  if (rr64 < ms->best) ms->best = rr64;
+  if (ms->bad >= 3 || ms->since_best >= 10) {
+    ms->final = 1;
+    ms->status = 4;
+    return;
+  }
+  const double s = refresh_scale(rr64, ms->scale);
+  ms->scale = s;
+  // next refresh once the scaled FP32 residual (norm 1 now) falls below
+  // delta, or below the final target if that is closer
+  st->nory = (float)fmax(ms->delta2, res * ms->nory64 * s * s);
+  st->nom[(st->iter + 1) & 1] = nomn;
+  st->done = 0;
+  st->status = 0;
+}
+
+__global__ void k_mx_init(MixState *ms, PcgState<float> *st, const double *__restrict__ slots,
+                          double res, double delta)
+{
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  ms->nory64 = slots[1];
+  ms->rr64 = slots[0];
+  ms->best = slots[0];
+  ms->scale = refresh_scale(slots[0], 1.0);
+  ms->delta2 = delta * delta;
+  ms->bad = 0;
+  ms->since_best = 0;
+  ms->updates = 0;
+  ms->final = 0;
+  ms->status = 0;
+  st->nory = (float)fmax(ms->delta2, res * slots[1] * ms->scale * ms->scale);
+  if (slots[0] < res * slots[1]) {
+    ms->final = 1;
+    ms->status = 1;
+    st->done = 1;
+  }
+}
+
+hipError_t mixed_init(MixState *ms, PcgState<float> *st, const double *slots, double res,
+                      double delta, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_mx_init, dim3(1), dim3(64), 0, s, ms, st, slots, res, delta);
+  return hipGetLastError();
+}
+
+hipError_t mixed_update(const SpmvTune &t64, const SpmvTune &t32, const CsrDev &A,
+                        const double *val64, const float *diag32, const double *b,
+                        double *x64, double *r64, double *part64, float *x32, float *r32,
+                        float *h32, float *d32, float *part32, PcgState<float> *st,
+                        MixState *ms, double res, hipStream_t s)
+{
+  const int n = A.n;
+  hipLaunchKernelGGL(k_mx_xupdate, dim3((n + TPB - 1) / TPB), dim3(TPB), 0, s, n, ms, x64, x32);
+  hipError_t e = spmv<double>(t64, 2, A, val64, x64, r64, b, part64, &ms->final, nullptr, s);
+  if (e != hipSuccess) return e;
+  const int nv = vec_blocks(t32, n);
+  float *prr = part32 + t32.blocks;
+  hipLaunchKernelGGL(k_mx_refresh, dim3(nv), dim3(TPB), 0, s, n, ms, part64, t64.blocks, r64,
+                     diag32, r32, h32, d32, prr);
+  hipLaunchKernelGGL(k_mx_check, dim3(1), dim3(TPB), 0, s, ms, st, part64, t64.blocks, prr, nv,
+                     res);
+  return hipGetLastError();
+}
+
+}  // namespace ssf

@@ -68,8 +68,11 @@ SIGNATURES = {
     "ssf_solve_dev": (_I, [_P, ctypes.POINTER(SolveParams), _P, _P, ctypes.POINTER(SolveInfo)]),
     "ssf_spmv_bench": (_I, [_P, _I, _I, ctypes.POINTER(_D)]),
     "ssf_spmv_bytes": (_I, [_P, _I, ctypes.POINTER(_D), ctypes.POINTER(_D)]),
-    "ssf_matrix_autotune": (_I, [_P, _I, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
-    "ssf_matrix_set_tune": (_I, [_P, _I, _I, _I]),
+    "ssf_copy_bench": (_I, [_P, _L, _I, ctypes.POINTER(_D)]),
+    "ssf_ctx_set_profiling": (_I, [_P, _I]),
+    "ssf_ctx_kernel_time": (_I, [_P, _I, ctypes.POINTER(_D), ctypes.POINTER(_L)]),
+    "ssf_matrix_autotune": (_I, [_P, _I, _I, _P]),
+    "ssf_matrix_set_tune": (_I, [_P, _I, _P]),
 }
 
 _lib = None
@@ -118,6 +121,21 @@ class Context:
 
     def sync(self):
         _check(lib().ssf_ctx_sync(self.h))
+
+    def set_profiling(self, on: bool):
+        _check(lib().ssf_ctx_set_profiling(self.h, 1 if on else 0))
+
+    def kernel_time(self, which: int):
+        """(mean us, samples) of the FP64 (0) / FP32 (1) SpMV sampled inside solves."""
+        us, n = _D(), _L()
+        _check(lib().ssf_ctx_kernel_time(self.h, which, ctypes.byref(us), ctypes.byref(n)))
+        return us.value, n.value
+
+    def copy_bench(self, nbytes: int, reps: int = 20) -> float:
+        """Mean time (us) of a device copy of nbytes (2*nbytes of HBM traffic)."""
+        us = _D()
+        _check(lib().ssf_copy_bench(self.h, nbytes, reps, ctypes.byref(us)))
+        return us.value
 
     def close(self):
         """Destroy the context; refused (and kept) while matrices on it are alive."""
@@ -243,13 +261,14 @@ class Matrix:
         return a.value, b.value
 
     def autotune(self, precision=FP64, reps=5):
-        lanes, blocks = _I(), _I()
-        _check(lib().ssf_matrix_autotune(self.h, precision, reps, ctypes.byref(lanes),
-                                         ctypes.byref(blocks)))
-        return lanes.value, blocks.value
+        """Returns the chosen SpMV tiling (format, lanes, blocks, idx16)."""
+        out = np.zeros(4, np.int32)
+        _check(lib().ssf_matrix_autotune(self.h, precision, reps, _p(out)))
+        return tuple(int(v) for v in out)
 
-    def set_tune(self, precision, lanes, blocks):
-        _check(lib().ssf_matrix_set_tune(self.h, precision, lanes, blocks))
+    def set_tune(self, precision, tune):
+        t = np.ascontiguousarray(tune, dtype=np.int32)
+        _check(lib().ssf_matrix_set_tune(self.h, precision, _p(t)))
 
 
 def build_problem_matrix(ctx: Context, prob: mesh.Problem, storage=STORAGE_SCR):

@@ -102,7 +102,7 @@ def test_mixed_precision(ctx, name):
     xo, *_ = O.scr_cg_prec(adr, ci, a, y, 50000, 1e-30)
     m = gpu.Matrix.from_csr(ctx, gpu.STORAGE_SCR, adr, ci)
     m.set_values(a)
-    x, info = m.solve(y, ni=50000, res=1e-26, precision=gpu.MIXED, inner_res=1e-4)
+    x, info = m.solve(y, ni=50000, res=1e-26, precision=gpu.MIXED)
     # 1: residual test met; 4: FP64 residual floor of the system reached first
     assert info.status in (1, 4)
     assert rel(x, xo) <= 1e-8, rel(x, xo)
@@ -168,3 +168,42 @@ def test_cantilever_full_config(ctx):
     assert info.status == 1 and st_o == 1
     assert rel(x, xo) <= 1e-8
     assert abs(info.ani - ani_o) <= max(3, ani_o // 100)
+
+
+TILINGS64 = [(0, l, 64, 0) for l in (4, 8, 16, 32, 64)] + [(1, l, 64, 0) for l in (4, 8, 16)]
+TILINGS32 = [(0, 8, 64, 0), (1, 4, 64, 0), (1, 8, 64, 1), (1, 16, 16, 1)]
+
+
+@pytest.mark.parametrize("name", ["hex8_elast", "hex8_heat", "quad4"])
+def test_spmv_tilings(ctx, name):
+    """Every SpMV tiling (row-vector and stream kernels, 32/16-bit columns) gives the
+    oracle's product / solution."""
+    prob = SMALL[name]()
+    adr, ci, a, rhs = O.assemble_problem(prob, "scr")
+    m = gpu.Matrix.from_csr(ctx, gpu.STORAGE_SCR, adr, ci)
+    m.set_values(a)
+    b = np.random.default_rng(2).standard_normal(prob.n)
+    ref = O.mxv_scr(adr, ci, a, b)
+    for t in TILINGS64:
+        m.set_tune(gpu.FP64, t)
+        assert rel(m.gmxv(b), ref) <= 1e-14, t
+    xo, *_ = O.scr_cg_prec(adr, ci, a, rhs[0], 50000, 1e-30)
+    m.set_tune(gpu.FP64, TILINGS64[-1])
+    for t in TILINGS32:
+        m.set_tune(gpu.MIXED, t)
+        x, info = m.solve(rhs[0], ni=50000, res=1e-26, precision=gpu.MIXED)
+        assert info.status in (1, 4) and rel(x, xo) <= 1e-8, (t, rel(x, xo))
+
+
+def test_autotune_keeps_parity(ctx):
+    prob = mesh.heat_cube(12, seed=8)
+    m, rhs = gpu.build_problem_matrix(ctx, prob)
+    t64 = m.autotune(gpu.FP64, reps=2)
+    t32 = m.autotune(gpu.MIXED, reps=2)
+    assert t64[3] == 0  # the FP64 operator keeps canonical 32-bit columns
+    adr, ci = m.pattern()
+    a = m.get_values()
+    xo, *_ = O.scr_cg_prec(adr, ci, a, rhs[0], 50000, 1e-30)
+    for precision in (gpu.FP64, gpu.MIXED):
+        x, info = m.solve(rhs[0], ni=50000, res=1e-24, precision=precision)
+        assert rel(x, xo) <= 1e-8, (precision, t64, t32)
