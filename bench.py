@@ -142,6 +142,16 @@ def main():
                          f"FP64 recurrence"}
         log(f"[bench] cpu: {per_it * 1e3:.1f} ms/iteration -> {cpu['value']:.4f} solves/s")
 
+    # HBM traffic per launch measured by rocprofv3 counters (calibrated on the copy
+    # kernel) for this workload and tiling, when such a measurement is committed
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "spmv_traffic_heat128.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            tr = json.load(fh)
+        if tr["n"] == prob.n and tr["nnz"] == m.nnz and list(tr["fp32"]["tiling"]) == list(tune32):
+            traffic = tr["fp32"]["bytes_per_launch"]
+
     mixed = results["mixed"]
     fp64 = results["fp64"]
     line = {
@@ -167,7 +177,7 @@ def main():
                        "fp64": fp64["iterations"]},
         "spmv_gbs": {"fp32": gbs32, "fp64": gbs64, "copy_kernel": copy_gbs},
         "roofline": {"bound": "hbm", "achieved": gbs32, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-                     "frac": gbs32 * 1e9 / PEAK_HBM, "traffic": None,
+                     "frac": gbs32 * 1e9 / PEAK_HBM, "traffic": traffic,
                      "kernel": "k_spmv_stream<float> (FP32 SpMV of the mixed solve)",
                      "bytes_per_launch": b32_act, "bytes_per_launch_canonical_csr": b32_can,
                      "usec_per_launch": us32, "samples": mixed["spmv_samples"]},
