@@ -150,6 +150,34 @@ int ssf_solve(ssf_matrix *m, const ssf_solve_params *p, const double *rhs, doubl
 int ssf_solve_dev(ssf_matrix *m, const ssf_solve_params *p, const double *rhs_dev,
                   double *lhs_dev, ssf_solve_info *info);
 
+/* ------------------------------------------------------------------ batches */
+/* Many assembled systems solved together, each with nrhs right-hand sides
+ * (1, 2, 4 or 8). Replaces the loop of gmatrix::solve_system calls over the
+ * load cases of one matrix (MEFEL/SRC/lssolver.cpp:78-79) and over independent
+ * meshes: one launch sequence serves every (system, rhs) pair, each pair runs
+ * the symcomprow::cg_prec recurrence (GEFEL/scr.cpp:1093-1203, Jacobi) on its
+ * own and stops on its own test. The matrices' values are copied at creation
+ * (later changes to them are not seen); the matrices may be destroyed after.
+ * Vectors are interleaved: entry (system s, row i, rhs r) is at
+ * (row0[s] + i) * nrhs + r, row0 from ssf_batch_sizes. */
+typedef struct ssf_batch ssf_batch;
+int ssf_batch_create(ssf_ctx *ctx, int64_t nsys, ssf_matrix *const *mats, int32_t nrhs,
+                     ssf_batch **out);
+int ssf_batch_destroy(ssf_batch *b);
+/* total rows, stored entries, and row0[nsys + 1] (may be NULL) */
+int ssf_batch_sizes(const ssf_batch *b, int64_t *nrows, int64_t *nnz, int64_t *row0);
+/* solve every pair: p as for ssf_solve_dev (cg + diagprec; FP64 or mixed), infos
+ * receives nsys * nrhs results in pair order s * nrhs + r; the result of a pair
+ * does not depend on the other pairs of the batch. Synchronous. */
+int ssf_batch_solve_dev(ssf_batch *b, const ssf_solve_params *p, const double *rhs_dev,
+                        double *lhs_dev, ssf_solve_info *infos);
+/* same with host vectors */
+int ssf_batch_solve(ssf_batch *b, const ssf_solve_params *p, const double *rhs, double *lhs,
+                    ssf_solve_info *infos);
+/* time the lane counts of the batched SpMM for precision 0 (FP64) / 1 (FP32)
+ * and keep the fastest; lanes_out (may be NULL) receives it */
+int ssf_batch_autotune(ssf_batch *b, int precision, int reps, int32_t *lanes_out);
+
 /* ------------------------------------------------------------ diagnostics */
 /* Time `reps` launches of the FP64 (precision 0) or FP32 (1) SpMV on device
  * vectors; returns the mean kernel time in microseconds measured with HIP

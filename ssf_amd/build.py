@@ -9,7 +9,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libssf_gpu.so")
-SOURCES = ["ssf_kernels.hip", "ssf_solver.hip", "ssf_elast.hip", "ssf_api.hip"]
+SOURCES = ["ssf_kernels.hip", "ssf_solver.hip", "ssf_elast.hip", "ssf_api.hip", "ssf_batch.hip",
+           "ssf_batch_api.hip"]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     # packed FP32 instructions are disabled: hipcc forms them itself from float code

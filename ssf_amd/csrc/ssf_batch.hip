@@ -1,0 +1,754 @@
+// Batched Jacobi-PCG over many independent systems with R right-hand sides each
+// (gfx950). The systems are concatenated block-diagonally; vectors are stored
+// interleaved, v[row * R + r], so the SpMM gathers one contiguous R-vector per
+// column. Every block owns a contiguous row range of exactly one system, so every
+// scalar of a system is reduced from that system's blocks only, in a fixed order:
+// a system's result does not depend on the other systems in the batch.
+//
+// Per iteration (for each active (system, rhs) pair, the recurrence of
+// GEFEL/scr.cpp:1142-1197):
+//   B1  P = A D, partials d.p              (k_bspmm, kind 1)
+//   B2  x += a d, r -= a p, h = r / diag, partials r.r, r.h   (k_bupdate)
+//   B3  test, d = h + b d                   (k_bdir)
+// Mixed precision adds, once per chunk, the reliable update of ssf_solver.hip
+// per pair: x64 += x32 / s, r64 = b - A x64 in FP64, FP32 residual refresh.
+#include "ssf_batch.hpp"
+
+namespace ssf {
+
+// ----------------------------------------------------------------- SpMM
+
+// G = R * L lanes per row: lane (l, r) sums entries l, l + L, ... of the row for
+// right-hand side r; the L partial sums are combined with shuffles (fixed order).
+template <typename T, int R, int L, int KIND>
+__global__ __launch_bounds__(TPB) void k_bspmm(BatchDev B, const T *__restrict__ val,
+                                               const T *__restrict__ x, T *__restrict__ y,
+                                               const T *__restrict__ b,
+                                               T *__restrict__ partial,
+                                               const int *__restrict__ sys_done)
+{
+  constexpr int G = R * L;
+  constexpr int NG = TPB / G;
+  const int blk = blockIdx.x;
+  const int s = B.blk_sys[blk];
+  if (sys_done && sys_done[s]) return;
+  const int r0 = B.blk_r0[blk], r1 = B.blk_r1[blk];
+  const int grp = threadIdx.x / G, lane = threadIdx.x % G;
+  const int r = lane % R, l = lane / R;
+  T acc = T(0);
+  for (int row = r0 + grp; row < r1; row += NG) {
+    const int beg = B.rowptr[row], end = B.rowptr[row + 1];
+    T sum = T(0);
+    for (int k = beg + l; k < end; k += L) sum += val[k] * x[(int64_t)B.col[k] * R + r];
+#pragma unroll
+    for (int off = G / 2; off >= R; off >>= 1) sum += __shfl_xor(sum, off, G);
+    if (l == 0) {
+      const int64_t q = (int64_t)row * R + r;
+      if (KIND == 2) {
+        sum = b[q] - sum;
+        acc += sum * sum;
+      } else if (KIND == 1) {
+        acc += x[q] * sum;
+      }
+      y[q] = sum;
+    }
+  }
+  if (KIND != 0) {
+    // per right-hand side: lanes with the same r hold partial sums
+    __shared__ T red[TPB];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x < R) {
+      T t = T(0);
+      for (int i = threadIdx.x; i < TPB; i += G)
+        for (int j = 0; j < L; j++) t += red[i + j * R];
+      partial[(int64_t)blk * R + threadIdx.x] = t;
+    }
+  }
+}
+
+// sum of a per-system quantity over the blocks of system s (fixed order)
+template <typename T, int R>
+__device__ __forceinline__ T sys_sum(const BatchDev &B, const T *__restrict__ part, int s, int r)
+{
+  const int b0 = B.sys_blk0[s], b1 = B.sys_blk0[s + 1];
+  T t = T(0);
+  for (int b = b0; b < b1; b++) t += part[(int64_t)b * R + r];
+  return t;
+}
+
+// block-wide per-rhs reduction: thread t holds a value for rhs t % R
+template <typename T, int R>
+__device__ __forceinline__ void block_sum_r(T v, T *out_lds)
+{
+  __shared__ T red[TPB];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  if (threadIdx.x < R) {
+    T t = T(0);
+    for (int i = threadIdx.x; i < TPB; i += R) t += red[i];
+    out_lds[threadIdx.x] = t;
+  }
+  __syncthreads();
+}
+
+// ------------------------------------------------------------- B2: update
+
+template <typename T, int R>
+__global__ __launch_bounds__(TPB) void k_bupdate(BatchDev B, BatchState<T> st,
+                                                 const T *__restrict__ pden, T zero,
+                                                 T *__restrict__ x, T *__restrict__ rr_,
+                                                 const T *__restrict__ d,
+                                                 const T *__restrict__ p,
+                                                 const T *__restrict__ diag,
+                                                 T *__restrict__ h, T *__restrict__ prr)
+{
+  __shared__ T alpha[R];
+  __shared__ T outa[R], outb[R];
+  const int blk = blockIdx.x;
+  const int s = B.blk_sys[blk];
+  if (st.sys_done[s]) return;
+  if (threadIdx.x < R) {
+    const int r = threadIdx.x;
+    const int q = s * R + r;
+    T a = T(0);
+    if (!st.done[q]) {
+      const T den = sys_sum<T, R>(B, pden, s, r);
+      const T nom = st.nom[2 * q + (st.iter[q] & 1)];
+      if (fabs(den) < zero) {
+        a = T(0);
+        if (blk == B.sys_blk0[s]) {  // first block records the stop, applied by k_badvance
+          st.brk[q] = 1;
+          st.ani[q] = st.iter[q];
+          st.ares[q] = nom;
+        }
+      } else {
+        a = nom / den;
+      }
+    }
+    alpha[r] = a;
+  }
+  __syncthreads();
+  const int64_t e0 = (int64_t)B.blk_r0[blk] * R, e1 = (int64_t)B.blk_r1[blk] * R;
+  const int r = threadIdx.x % R;  // TPB is a multiple of R
+  const T a = alpha[r];
+  T srr = T(0), srh = T(0);
+  for (int64_t e = e0 + threadIdx.x; e < e1; e += TPB) {
+    if (a != T(0)) {
+      x[e] += a * d[e];
+      rr_[e] -= a * p[e];
+    }
+    const T ri = rr_[e];
+    const T hi = ri / diag[e / R];
+    h[e] = hi;
+    srr += ri * ri;
+    srh += ri * hi;
+  }
+  block_sum_r<T, R>(srr, outa);
+  block_sum_r<T, R>(srh, outb);
+  if (threadIdx.x < R) {
+    prr[(int64_t)blk * R + threadIdx.x] = outa[threadIdx.x];
+    prr[(int64_t)(B.nblk + blk) * R + threadIdx.x] = outb[threadIdx.x];
+  }
+}
+
+// -------------------------------------------------------------- B3: direction
+
+template <typename T, int R, bool MIXED>
+__global__ __launch_bounds__(TPB) void k_bdir(BatchDev B, BatchState<T> st,
+                                              const T *__restrict__ prr, T res,
+                                              T *__restrict__ d, const T *__restrict__ h)
+{
+  __shared__ T beta[R];
+  __shared__ int upd[R];
+  const int blk = blockIdx.x;
+  const int s = B.blk_sys[blk];
+  if (st.sys_done[s]) return;
+  const bool first = blk == B.sys_blk0[s];
+  if (threadIdx.x < R) {
+    const int r = threadIdx.x;
+    const int q = s * R + r;
+    int u = 0;
+    T bt = T(0);
+    // state changes are left pending for k_badvance: every block of the system
+    // reads the same iter / done / nom while this kernel runs
+    if (!st.done[q] && !st.brk[q]) {
+      const T rr = sys_sum<T, R>(B, prr, s, r);
+      const T nomn = sys_sum<T, R>(B, prr + (int64_t)B.nblk * R, s, r);
+      const long long it = st.iter[q];
+      const T nom = st.nom[2 * q + (it & 1)];
+      bt = nomn / nom;
+      if (MIXED) {
+        u = 1;  // direction always updated; rr < nory pauses until the refresh
+        if (first) {
+          st.nom[2 * q + ((it + 1) & 1)] = nomn;
+          st.pend[q] = PEND_ADVANCE | (rr < st.nory[q] ? PEND_STOP : 0);
+          st.pend_status[q] = 1;
+        }
+      } else {
+        const bool stop = fabs(rr / st.nory[q]) < res;  // GEFEL/scr.cpp:1188
+        u = stop ? 0 : 1;
+        if (first) {
+          st.nom[2 * q + ((it + 1) & 1)] = nomn;
+          if (stop) {
+            st.ani[q] = it;
+            st.ares[q] = nomn / st.nory[q];
+            st.pend[q] = PEND_STOP;
+            st.pend_status[q] = 1;
+          } else {
+            st.pend[q] = PEND_ADVANCE;
+          }
+        }
+      }
+    }
+    beta[r] = bt;
+    upd[r] = u;
+  }
+  __syncthreads();
+  const int64_t e0 = (int64_t)B.blk_r0[blk] * R, e1 = (int64_t)B.blk_r1[blk] * R;
+  const int r = threadIdx.x % R;
+  if (!upd[r]) return;
+  const T bt = beta[r];
+  for (int64_t e = e0 + threadIdx.x; e < e1; e += TPB) d[e] = bt * d[e] + h[e];
+}
+
+// applies the pending state changes of the iteration, then: a system is done when
+// all its right-hand sides are; counts the active pairs for the host
+template <typename T, int R>
+__global__ void k_badvance(BatchDev B, BatchState<T> st, int *active)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= B.nsys) return;
+  int a = 0;
+  for (int r = 0; r < R; r++) {
+    const int q = s * R + r;
+    if (st.brk[q]) {  // |d.Ap| < zero in k_bupdate (GEFEL/scr.cpp:1148-1152)
+      st.done[q] = 1;
+      st.status[q] = 2;
+      st.brk[q] = 0;
+      st.pend[q] = 0;
+    }
+    const int pd = st.pend[q];
+    if (pd) {
+      if (pd & PEND_ADVANCE) st.iter[q] += 1;
+      if (pd & PEND_STOP) {
+        st.done[q] = 1;
+        st.status[q] = st.pend_status[q];
+      }
+      st.pend[q] = 0;
+    }
+    a += st.done[q] ? 0 : 1;
+  }
+  st.sys_done[s] = a == 0;
+  if (a && active) atomicAdd(active, a);
+}
+
+// --------------------------------------------------------------- init
+
+// r = y (x = 0), h = r / diag, d = h, partials y.y, r.h
+template <typename T, int R>
+__global__ __launch_bounds__(TPB) void k_binit(BatchDev B, const T *__restrict__ y,
+                                               const T *__restrict__ diag, T *__restrict__ r_,
+                                               T *__restrict__ h, T *__restrict__ d,
+                                               T *__restrict__ prr)
+{
+  __shared__ T outa[R], outb[R];
+  const int blk = blockIdx.x;
+  const int64_t e0 = (int64_t)B.blk_r0[blk] * R, e1 = (int64_t)B.blk_r1[blk] * R;
+  T syy = T(0), srh = T(0);
+  for (int64_t e = e0 + threadIdx.x; e < e1; e += TPB) {
+    const T yi = y[e];
+    const T hi = yi / diag[e / R];
+    r_[e] = yi;
+    h[e] = hi;
+    d[e] = hi;
+    syy += yi * yi;
+    srh += yi * hi;
+  }
+  block_sum_r<T, R>(syy, outa);
+  block_sum_r<T, R>(srh, outb);
+  if (threadIdx.x < R) {
+    prr[(int64_t)blk * R + threadIdx.x] = outa[threadIdx.x];
+    prr[(int64_t)(B.nblk + blk) * R + threadIdx.x] = outb[threadIdx.x];
+  }
+}
+
+template <typename T, int R>
+__global__ void k_binit2(BatchDev B, BatchState<T> st, const T *__restrict__ prr, T zero)
+{
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= B.nsys * R) return;
+  const int s = q / R, r = q % R;
+  const T nory = sys_sum<T, R>(B, prr, s, r);
+  const T nom = sys_sum<T, R>(B, prr + (int64_t)B.nblk * R, s, r);
+  st.nory[q] = nory;
+  st.nom[2 * q] = nom;
+  st.nom[2 * q + 1] = nom;
+  st.iter[q] = 0;
+  st.status[q] = 0;
+  st.done[q] = 0;
+  st.pend[q] = 0;
+  st.brk[q] = 0;
+  st.ani[q] = -1;
+  st.ares[q] = T(0);
+  if (nory < zero) {  // zero right-hand side: x = 0
+    st.status[q] = 3;
+    st.done[q] = 1;
+    st.ani[q] = 0;
+    st.ares[q] = nory;
+  }
+  if (r == 0) st.sys_done[s] = 0;
+}
+
+// ------------------------------------------------- mixed: reliable updates
+
+struct BatchMix {
+  double *nory64, *rr64, *best, *scale, *delta2;
+  int *bad, *since_best, *final, *status;
+  long long *updates;
+};
+
+template <int R>
+__global__ __launch_bounds__(TPB) void k_bmx_xupdate(BatchDev B, BatchMix mx,
+                                                     const int *__restrict__ sys_final,
+                                                     double *__restrict__ x64,
+                                                     float *__restrict__ x32)
+{
+  const int blk = blockIdx.x;
+  const int s = B.blk_sys[blk];
+  if (sys_final[s]) return;
+  const int64_t e0 = (int64_t)B.blk_r0[blk] * R, e1 = (int64_t)B.blk_r1[blk] * R;
+  const int r = threadIdx.x % R;
+  const int q = s * R + r;
+  if (mx.final[q]) return;
+  const double sc = mx.scale[q];
+  for (int64_t e = e0 + threadIdx.x; e < e1; e += TPB) {
+    x64[e] += (double)x32[e] / sc;
+    x32[e] = 0.0f;
+  }
+}
+
+template <int R>
+__global__ __launch_bounds__(TPB) void k_bmx_refresh(BatchDev B, BatchMix mx,
+                                                     const int *__restrict__ sys_final,
+                                                     const double *__restrict__ part64,
+                                                     const double *__restrict__ r64,
+                                                     const float *__restrict__ diag32,
+                                                     float *__restrict__ r32,
+                                                     float *__restrict__ h32,
+                                                     float *__restrict__ d32,
+                                                     float *__restrict__ prr)
+{
+  __shared__ double scl[R];
+  __shared__ float cfac[R];
+  __shared__ int rst[R], fin[R];
+  __shared__ float outa[R], outb[R];
+  const int blk = blockIdx.x;
+  const int s = B.blk_sys[blk];
+  if (sys_final[s]) return;
+  if (threadIdx.x < R) {
+    const int q = s * R + threadIdx.x;
+    const double rr64 = sys_sum<double, R>(B, part64, s, threadIdx.x);
+    const double old = mx.scale[q];
+    const double sc = rr64 > 0.0 ? 1.0 / sqrt(rr64) : old;
+    scl[threadIdx.x] = sc;
+    cfac[threadIdx.x] = (float)(sc / old);
+    rst[threadIdx.x] = rr64 > 1e4 * mx.best[q];
+    fin[threadIdx.x] = mx.final[q];
+  }
+  __syncthreads();
+  const int64_t e0 = (int64_t)B.blk_r0[blk] * R, e1 = (int64_t)B.blk_r1[blk] * R;
+  const int r = threadIdx.x % R;
+  float srr = 0.f, srh = 0.f;
+  if (!fin[r]) {
+    const double sc = scl[r];
+    const float c = cfac[r];
+    const bool restart = rst[r];
+    for (int64_t e = e0 + threadIdx.x; e < e1; e += TPB) {
+      const float ri = (float)(r64[e] * sc);
+      const float hi = ri / diag32[e / R];
+      r32[e] = ri;
+      h32[e] = hi;
+      d32[e] = restart ? hi : d32[e] * c;
+      srr += ri * ri;
+      srh += ri * hi;
+    }
+  }
+  block_sum_r<float, R>(srr, outa);
+  block_sum_r<float, R>(srh, outb);
+  if (threadIdx.x < R) {
+    prr[(int64_t)blk * R + threadIdx.x] = outa[threadIdx.x];
+    prr[(int64_t)(B.nblk + blk) * R + threadIdx.x] = outb[threadIdx.x];
+  }
+}
+
+template <int R>
+__global__ void k_bmx_check(BatchDev B, BatchMix mx, BatchState<float> st,
+                            const double *__restrict__ part64, const float *__restrict__ prr,
+                            double res)
+{
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= B.nsys * R) return;
+  if (mx.final[q]) return;
+  const int s = q / R, r = q % R;
+  const double rr64 = sys_sum<double, R>(B, part64, s, r);
+  const float nomn = sys_sum<float, R>(B, prr + (int64_t)B.nblk * R, s, r);
+  const double prev = mx.rr64[q];
+  mx.rr64[q] = rr64;
+  mx.updates[q] += 1;
+  if (rr64 < res * mx.nory64[q]) {
+    mx.final[q] = 1;
+    mx.status[q] = 1;
+    st.done[q] = 1;
+    return;
+  }
+  const bool met = st.done[q] && st.status[q] == 1;
+  mx.bad[q] = (met && rr64 > 0.25 * prev) ? mx.bad[q] + 1 : 0;
+  if (rr64 < 0.81 * mx.best[q]) mx.since_best[q] = 0;
+  else mx.since_best[q] += 1;
+  if (rr64 < mx.best[q]) mx.best[q] = rr64;
+  if (mx.bad[q] >= 3 || mx.since_best[q] >= 10) {
+    mx.final[q] = 1;
+    mx.status[q] = 4;
+    st.done[q] = 1;
+    return;
+  }
+  const double sc = rr64 > 0.0 ? 1.0 / sqrt(rr64) : mx.scale[q];
+  mx.scale[q] = sc;
+  st.nory[q] = (float)fmax(mx.delta2[q], res * mx.nory64[q] * sc * sc);
+  st.nom[2 * q + (st.iter[q] & 1)] = nomn;
+  st.done[q] = 0;
+  st.status[q] = 0;
+  st.pend[q] = 0;
+  st.brk[q] = 0;
+}
+
+// system-level final flags for the mixed solve; counts the unfinished pairs
+template <int R>
+__global__ void k_bmx_flags(BatchDev B, BatchMix mx, BatchState<float> st, int *sys_final,
+                            int *active)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= B.nsys) return;
+  int a = 0, run = 0;
+  for (int r = 0; r < R; r++) {
+    a += mx.final[s * R + r] ? 0 : 1;
+    run += (mx.final[s * R + r] || st.done[s * R + r]) ? 0 : 1;
+  }
+  if (a == 0) sys_final[s] = 1;
+  st.sys_done[s] = run == 0;  // the FP32 kernels skip a system with nothing to iterate
+  if (a) atomicAdd(active, a);
+}
+
+template <int R>
+__global__ void k_bmx_init(BatchDev B, BatchMix mx, BatchState<float> st,
+                           const double *__restrict__ part64, const double *__restrict__ pyy,
+                           double res, double delta, int *sys_final)
+{
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= B.nsys * R) return;
+  const int s = q / R, r = q % R;
+  const double rr = sys_sum<double, R>(B, part64, s, r);
+  const double yy = sys_sum<double, R>(B, pyy, s, r);
+  mx.nory64[q] = yy;
+  mx.rr64[q] = rr;
+  mx.best[q] = rr;
+  mx.scale[q] = rr > 0.0 ? 1.0 / sqrt(rr) : 1.0;
+  mx.delta2[q] = delta * delta;
+  mx.bad[q] = 0;
+  mx.since_best[q] = 0;
+  mx.updates[q] = 0;
+  mx.final[q] = 0;
+  mx.status[q] = 0;
+  if (r == 0) sys_final[s] = 0;
+  if (yy < 1e-300) {  // zero right-hand side: x = 0 stays
+    mx.final[q] = 1;
+    mx.status[q] = 3;
+  } else if (rr < res * yy) {
+    mx.final[q] = 1;
+    mx.status[q] = 1;
+  }
+}
+
+// residual of the FP64 refresh in the units of the FP32 recurrence
+template <int R>
+__global__ __launch_bounds__(TPB) void k_bmx_scale_rhs(BatchDev B, BatchMix mx,
+                                                       const double *__restrict__ r64,
+                                                       float *__restrict__ y32)
+{
+  const int blk = blockIdx.x;
+  const int s = B.blk_sys[blk];
+  const int64_t e0 = (int64_t)B.blk_r0[blk] * R, e1 = (int64_t)B.blk_r1[blk] * R;
+  const int r = threadIdx.x % R;
+  const double sc = mx.scale[s * R + r];
+  for (int64_t e = e0 + threadIdx.x; e < e1; e += TPB) y32[e] = (float)(r64[e] * sc);
+}
+
+template <int R>
+__global__ void k_bmx_init32(BatchDev B, BatchMix mx, BatchState<float> st, double res)
+{
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= B.nsys * R) return;
+  const double sc = mx.scale[q];
+  st.nory[q] = (float)fmax(mx.delta2[q], res * mx.nory64[q] * sc * sc);
+  if (mx.final[q]) st.done[q] = 1;
+  st.status[q] = 0;
+}
+
+__global__ __launch_bounds__(TPB) void k_bdot_yy(BatchDev B, int R, const double *__restrict__ y,
+                                                 double *__restrict__ part)
+{
+  // per block and rhs: sum y^2 (R <= 8)
+  __shared__ double red[TPB];
+  const int blk = blockIdx.x;
+  const int64_t e0 = (int64_t)B.blk_r0[blk] * R, e1 = (int64_t)B.blk_r1[blk] * R;
+  double a = 0.0;
+  for (int64_t e = e0 + threadIdx.x; e < e1; e += TPB) a += y[e] * y[e];
+  red[threadIdx.x] = a;
+  __syncthreads();
+  if (threadIdx.x < R) {
+    double t = 0.0;
+    for (int i = threadIdx.x; i < TPB; i += R) t += red[i];
+    part[(int64_t)blk * R + threadIdx.x] = t;
+  }
+}
+
+// ----------------------------------------------------------- dispatch
+
+template <typename T, int R>
+static hipError_t bspmm_r(const BatchTune &t, int kind, const BatchDev &B, const T *val,
+                          const T *x, T *y, const T *b, T *partial, const int *sys_done,
+                          hipStream_t s)
+{
+  dim3 grid(B.nblk), block(TPB);
+#define SSF_BS(L, K) hipLaunchKernelGGL((k_bspmm<T, R, L, K>), grid, block, 0, s, B, val, x, y, b, partial, sys_done)
+#define SSF_BSK(L)              \
+  do {                          \
+    if (kind == 0) SSF_BS(L, 0); \
+    else if (kind == 1) SSF_BS(L, 1); \
+    else SSF_BS(L, 2);          \
+  } while (0)
+  const int L = t.lanes / R;
+  if (R * 16 <= 64 && L >= 16) SSF_BSK((R * 16 <= 64 ? 16 : 1));
+  else if (R * 8 <= 64 && L >= 8) SSF_BSK((R * 8 <= 64 ? 8 : 1));
+  else if (R * 4 <= 64 && L >= 4) SSF_BSK((R * 4 <= 64 ? 4 : 1));
+  else if (R * 2 <= 64 && L >= 2) SSF_BSK((R * 2 <= 64 ? 2 : 1));
+  else SSF_BSK(1);
+#undef SSF_BSK
+#undef SSF_BS
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t bspmm(const BatchTune &t, int kind, const BatchDev &B, const T *val, const T *x,
+                 T *y, const T *b, T *partial, const int *sys_done, hipStream_t s)
+{
+  switch (B.R) {
+    case 1: return bspmm_r<T, 1>(t, kind, B, val, x, y, b, partial, sys_done, s);
+    case 2: return bspmm_r<T, 2>(t, kind, B, val, x, y, b, partial, sys_done, s);
+    case 4: return bspmm_r<T, 4>(t, kind, B, val, x, y, b, partial, sys_done, s);
+    default: return bspmm_r<T, 8>(t, kind, B, val, x, y, b, partial, sys_done, s);
+  }
+}
+template hipError_t bspmm<double>(const BatchTune &, int, const BatchDev &, const double *,
+                                  const double *, double *, const double *, double *,
+                                  const int *, hipStream_t);
+template hipError_t bspmm<float>(const BatchTune &, int, const BatchDev &, const float *,
+                                 const float *, float *, const float *, float *, const int *,
+                                 hipStream_t);
+
+template <typename T, int R>
+static hipError_t biter_r(const BatchTune &t, const BatchDev &B, const BatchVec<T> &v,
+                          BatchState<T> st, T res, T zero, bool mixed, hipStream_t s)
+{
+  hipError_t e = bspmm_r<T, R>(t, 1, B, v.val, v.d, v.p, nullptr, v.part, st.sys_done, s);
+  if (e != hipSuccess) return e;
+  dim3 grid(B.nblk), block(TPB);
+  T *prr = v.part + (int64_t)B.nblk * R;
+  hipLaunchKernelGGL((k_bupdate<T, R>), grid, block, 0, s, B, st, v.part, zero, v.x, v.r, v.d,
+                     v.p, v.diag, v.h, prr);
+  if (mixed)
+    hipLaunchKernelGGL((k_bdir<T, R, true>), grid, block, 0, s, B, st, prr, res, v.d, v.h);
+  else
+    hipLaunchKernelGGL((k_bdir<T, R, false>), grid, block, 0, s, B, st, prr, res, v.d, v.h);
+  hipLaunchKernelGGL((k_badvance<T, R>), dim3((B.nsys + TPB - 1) / TPB), dim3(TPB), 0, s, B, st,
+                     nullptr);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t batch_iteration(const BatchTune &t, const BatchDev &B, const BatchVec<T> &v,
+                           BatchState<T> st, T res, T zero, bool mixed, hipStream_t s)
+{
+  switch (B.R) {
+    case 1: return biter_r<T, 1>(t, B, v, st, res, zero, mixed, s);
+    case 2: return biter_r<T, 2>(t, B, v, st, res, zero, mixed, s);
+    case 4: return biter_r<T, 4>(t, B, v, st, res, zero, mixed, s);
+    default: return biter_r<T, 8>(t, B, v, st, res, zero, mixed, s);
+  }
+}
+template hipError_t batch_iteration<double>(const BatchTune &, const BatchDev &,
+                                            const BatchVec<double> &, BatchState<double>, double,
+                                            double, bool, hipStream_t);
+template hipError_t batch_iteration<float>(const BatchTune &, const BatchDev &,
+                                           const BatchVec<float> &, BatchState<float>, float,
+                                           float, bool, hipStream_t);
+
+template <typename T, int R>
+static hipError_t binit_r(const BatchDev &B, const BatchVec<T> &v, BatchState<T> st, T zero,
+                          hipStream_t s)
+{
+  T *prr = v.part + (int64_t)B.nblk * R;
+  hipLaunchKernelGGL((k_binit<T, R>), dim3(B.nblk), dim3(TPB), 0, s, B, v.y, v.diag, v.r, v.h,
+                     v.d, prr);
+  const int nq = B.nsys * R;
+  hipLaunchKernelGGL((k_binit2<T, R>), dim3((nq + TPB - 1) / TPB), dim3(TPB), 0, s, B, st, prr,
+                     zero);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t batch_init(const BatchDev &B, const BatchVec<T> &v, BatchState<T> st, T zero,
+                      hipStream_t s)
+{
+  switch (B.R) {
+    case 1: return binit_r<T, 1>(B, v, st, zero, s);
+    case 2: return binit_r<T, 2>(B, v, st, zero, s);
+    case 4: return binit_r<T, 4>(B, v, st, zero, s);
+    default: return binit_r<T, 8>(B, v, st, zero, s);
+  }
+}
+template hipError_t batch_init<double>(const BatchDev &, const BatchVec<double> &,
+                                       BatchState<double>, double, hipStream_t);
+template hipError_t batch_init<float>(const BatchDev &, const BatchVec<float> &,
+                                      BatchState<float>, float, hipStream_t);
+
+template <typename T>
+hipError_t batch_flags(const BatchDev &B, BatchState<T> st, int *active, hipStream_t s)
+{
+  const unsigned g = (unsigned)((B.nsys + TPB - 1) / TPB);
+  switch (B.R) {
+    case 1: hipLaunchKernelGGL((k_badvance<T, 1>), dim3(g), dim3(TPB), 0, s, B, st, active); break;
+    case 2: hipLaunchKernelGGL((k_badvance<T, 2>), dim3(g), dim3(TPB), 0, s, B, st, active); break;
+    case 4: hipLaunchKernelGGL((k_badvance<T, 4>), dim3(g), dim3(TPB), 0, s, B, st, active); break;
+    default: hipLaunchKernelGGL((k_badvance<T, 8>), dim3(g), dim3(TPB), 0, s, B, st, active); break;
+  }
+  return hipGetLastError();
+}
+template hipError_t batch_flags<double>(const BatchDev &, BatchState<double>, int *, hipStream_t);
+template hipError_t batch_flags<float>(const BatchDev &, BatchState<float>, int *, hipStream_t);
+
+// ---------------------------------------------------- mixed-precision steps
+
+__global__ void k_zero_int(int *p) { *p = 0; }
+
+template <int R>
+static BatchMix mixview(const BatchMixDev &m)
+{
+  BatchMix x;
+  x.nory64 = m.nory64; x.rr64 = m.rr64; x.best = m.best; x.scale = m.scale; x.delta2 = m.delta2;
+  x.bad = m.bad; x.since_best = m.since_best; x.final = m.final; x.status = m.status;
+  x.updates = m.updates;
+  return x;
+}
+
+template <int R>
+static hipError_t bmx_start_r(const BatchTune &t64, const BatchTune &t32, const BatchDev &B,
+                              const BatchVec<double> &v64, const BatchVec<float> &v32,
+                              BatchState<float> st, const BatchMixDev &md, double res,
+                              double delta, hipStream_t s)
+{
+  BatchMix mx = mixview<R>(md);
+  dim3 grid(B.nblk), block(TPB);
+  const int nq = B.nsys * R;
+  const unsigned gq = (unsigned)((nq + TPB - 1) / TPB);
+  // y.y and the initial FP64 residual r = b - A x
+  double *pyy = v64.part + 2 * (int64_t)B.nblk * R;
+  hipLaunchKernelGGL(k_bdot_yy, grid, block, 0, s, B, R, v64.y, pyy);
+  hipError_t e = bspmm_r<double, R>(t64, 2, B, v64.val, v64.x, v64.r, v64.y, v64.part, nullptr, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_bmx_init<R>), dim3(gq), dim3(TPB), 0, s, B, mx, st, v64.part, pyy, res,
+                     delta, md.sys_final);
+  // FP32 recurrence on the scaled residual (x32 = 0 set by the caller)
+  hipLaunchKernelGGL((k_bmx_scale_rhs<R>), grid, block, 0, s, B, mx, v64.r, v32.y);
+  e = binit_r<float, R>(B, v32, st, 1e-30f, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_bmx_init32<R>), dim3(gq), dim3(TPB), 0, s, B, mx, st, res);
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s, md.active);
+  hipLaunchKernelGGL((k_bmx_flags<R>), dim3((B.nsys + TPB - 1) / TPB), dim3(TPB), 0, s, B, mx, st,
+                     md.sys_final, md.active);
+  return hipGetLastError();
+}
+
+template <int R>
+static hipError_t bmx_refresh_r(const BatchTune &t64, const BatchDev &B,
+                                const BatchVec<double> &v64, const BatchVec<float> &v32,
+                                BatchState<float> st, const BatchMixDev &md, double res,
+                                hipStream_t s)
+{
+  BatchMix mx = mixview<R>(md);
+  dim3 grid(B.nblk), block(TPB);
+  const int nq = B.nsys * R;
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s, md.active);
+  hipLaunchKernelGGL((k_bmx_xupdate<R>), grid, block, 0, s, B, mx, md.sys_final, v64.x, v32.x);
+  hipError_t e = bspmm_r<double, R>(t64, 2, B, v64.val, v64.x, v64.r, v64.y, v64.part,
+                                    md.sys_final, s);
+  if (e != hipSuccess) return e;
+  float *prr = v32.part + (int64_t)B.nblk * R;
+  hipLaunchKernelGGL((k_bmx_refresh<R>), grid, block, 0, s, B, mx, md.sys_final, v64.part, v64.r,
+                     v32.diag, v32.r, v32.h, v32.d, prr);
+  hipLaunchKernelGGL((k_bmx_check<R>), dim3((nq + TPB - 1) / TPB), dim3(TPB), 0, s, B, mx, st,
+                     v64.part, prr, res);
+  hipLaunchKernelGGL((k_bmx_flags<R>), dim3((B.nsys + TPB - 1) / TPB), dim3(TPB), 0, s, B, mx, st,
+                     md.sys_final, md.active);
+  return hipGetLastError();
+}
+
+__global__ void k_offset_copy(int64_t n, const int *__restrict__ src, int *__restrict__ dst,
+                              int add)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i] + add;
+}
+
+hipError_t offset_copy(int64_t n, const int *src, int *dst, int add, hipStream_t s)
+{
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_offset_copy, dim3((unsigned)((n + TPB - 1) / TPB)), dim3(TPB), 0, s, n,
+                     src, dst, add);
+  return hipGetLastError();
+}
+
+hipError_t zero_int(int *p, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s, p);
+  return hipGetLastError();
+}
+
+hipError_t batch_mixed_start(const BatchTune &t64, const BatchTune &t32, const BatchDev &B,
+                             const BatchVec<double> &v64, const BatchVec<float> &v32,
+                             BatchState<float> st, const BatchMixDev &md, double res,
+                             double delta, hipStream_t s)
+{
+  switch (B.R) {
+    case 1: return bmx_start_r<1>(t64, t32, B, v64, v32, st, md, res, delta, s);
+    case 2: return bmx_start_r<2>(t64, t32, B, v64, v32, st, md, res, delta, s);
+    case 4: return bmx_start_r<4>(t64, t32, B, v64, v32, st, md, res, delta, s);
+    default: return bmx_start_r<8>(t64, t32, B, v64, v32, st, md, res, delta, s);
+  }
+}
+
+hipError_t batch_mixed_refresh(const BatchTune &t64, const BatchDev &B,
+                               const BatchVec<double> &v64, const BatchVec<float> &v32,
+                               BatchState<float> st, const BatchMixDev &md, double res,
+                               hipStream_t s)
+{
+  switch (B.R) {
+    case 1: return bmx_refresh_r<1>(t64, B, v64, v32, st, md, res, s);
+    case 2: return bmx_refresh_r<2>(t64, B, v64, v32, st, md, res, s);
+    case 4: return bmx_refresh_r<4>(t64, B, v64, v32, st, md, res, s);
+    default: return bmx_refresh_r<8>(t64, B, v64, v32, st, md, res, s);
+  }
+}
+
+}  // namespace ssf

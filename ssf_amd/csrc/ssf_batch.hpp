@@ -1,0 +1,79 @@
+// Batched (many systems x R right-hand sides) Jacobi-PCG on gfx950: device views.
+#pragma once
+
+#include "ssf_device.hpp"
+
+namespace ssf {
+
+constexpr int PEND_ADVANCE = 1;  // iteration completed: iter += 1
+constexpr int PEND_STOP = 2;     // recurrence stops (status in pend_status)
+
+// block-diagonal matrix of the batch and its block table: block b covers rows
+// [blk_r0[b], blk_r1[b]) of system blk_sys[b]; the blocks of system s are
+// [sys_blk0[s], sys_blk0[s+1])
+struct BatchDev {
+  int nsys = 0;
+  int R = 1;
+  int nblk = 0;
+  int64_t nrows = 0;
+  const int *rowptr = nullptr;  // global rows, global (block-diagonal) columns
+  const int *col = nullptr;
+  const int *blk_sys = nullptr, *blk_r0 = nullptr, *blk_r1 = nullptr, *sys_blk0 = nullptr;
+};
+
+struct BatchTune {
+  int lanes = 8;  // lanes per row (R * L)
+};
+
+// per (system, rhs) pair q = s * R + r
+template <typename T>
+struct BatchState {
+  T *nory = nullptr;
+  T *nom = nullptr;  // 2 per pair, ping-pong by iteration parity
+  long long *iter = nullptr;
+  int *done = nullptr, *status = nullptr, *pend = nullptr, *pend_status = nullptr,
+      *brk = nullptr;
+  long long *ani = nullptr;
+  T *ares = nullptr;
+  int *sys_done = nullptr;  // per system
+};
+
+template <typename T>
+struct BatchVec {  // interleaved [row * R + r]
+  const T *val = nullptr, *diag = nullptr;
+  T *x = nullptr, *r = nullptr, *d = nullptr, *p = nullptr, *h = nullptr, *y = nullptr;
+  T *part = nullptr;  // 3 * nblk * R
+};
+
+struct BatchMixDev {
+  double *nory64 = nullptr, *rr64 = nullptr, *best = nullptr, *scale = nullptr,
+         *delta2 = nullptr;
+  int *bad = nullptr, *since_best = nullptr, *final = nullptr, *status = nullptr;
+  long long *updates = nullptr;
+  int *sys_final = nullptr;
+  int *active = nullptr;
+};
+
+template <typename T>
+hipError_t bspmm(const BatchTune &t, int kind, const BatchDev &B, const T *val, const T *x,
+                 T *y, const T *b, T *partial, const int *sys_done, hipStream_t s);
+template <typename T>
+hipError_t batch_init(const BatchDev &B, const BatchVec<T> &v, BatchState<T> st, T zero,
+                      hipStream_t s);
+template <typename T>
+hipError_t batch_iteration(const BatchTune &t, const BatchDev &B, const BatchVec<T> &v,
+                           BatchState<T> st, T res, T zero, bool mixed, hipStream_t s);
+template <typename T>
+hipError_t batch_flags(const BatchDev &B, BatchState<T> st, int *active, hipStream_t s);
+hipError_t offset_copy(int64_t n, const int *src, int *dst, int add, hipStream_t s);
+hipError_t zero_int(int *p, hipStream_t s);
+hipError_t batch_mixed_start(const BatchTune &t64, const BatchTune &t32, const BatchDev &B,
+                             const BatchVec<double> &v64, const BatchVec<float> &v32,
+                             BatchState<float> st, const BatchMixDev &md, double res,
+                             double delta, hipStream_t s);
+hipError_t batch_mixed_refresh(const BatchTune &t64, const BatchDev &B,
+                               const BatchVec<double> &v64, const BatchVec<float> &v32,
+                               BatchState<float> st, const BatchMixDev &md, double res,
+                               hipStream_t s);
+
+}  // namespace ssf

@@ -1,0 +1,493 @@
+// C ABI of the batched solver: many assembled matrices (independent systems, each
+// with R right-hand sides) solved together. Iteration chunks are captured once
+// into a hipGraph and replayed until every (system, rhs) pair has finished.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "ssf_batch.hpp"
+
+using namespace ssf;
+
+namespace {
+
+template <typename T>
+hipError_t balloc(T **p, size_t count)
+{
+  *p = nullptr;
+  return hipMalloc((void **)p, std::max<size_t>(count, 1) * sizeof(T));
+}
+template <typename T>
+void bfree(T *&p)
+{
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+template <typename T>
+struct StateArrays {
+  BatchState<T> s;
+  hipError_t alloc(int64_t nq, int64_t nsys)
+  {
+    hipError_t e = hipSuccess;
+    auto c = [&](hipError_t x) { if (e == hipSuccess) e = x; };
+    c(balloc(&s.nory, nq)); c(balloc(&s.nom, 2 * nq)); c(balloc(&s.iter, nq));
+    c(balloc(&s.done, nq)); c(balloc(&s.status, nq)); c(balloc(&s.pend, nq));
+    c(balloc(&s.pend_status, nq)); c(balloc(&s.brk, nq)); c(balloc(&s.ani, nq));
+    c(balloc(&s.ares, nq)); c(balloc(&s.sys_done, nsys));
+    return e;
+  }
+  void release()
+  {
+    bfree(s.nory); bfree(s.nom); bfree(s.iter); bfree(s.done); bfree(s.status); bfree(s.pend);
+    bfree(s.pend_status); bfree(s.brk); bfree(s.ani); bfree(s.ares); bfree(s.sys_done);
+  }
+};
+
+}  // namespace
+
+struct ssf_batch {
+  ssf_ctx *ctx = nullptr;
+  int nsys = 0, R = 1, nblk = 0;
+  int64_t nrows = 0, nnz = 0;
+  std::vector<int64_t> row0;  // first global row of each system, nsys + 1
+  int *rowptr = nullptr, *col = nullptr;
+  double *val = nullptr, *diag = nullptr;
+  float *val32 = nullptr, *diag32 = nullptr;
+  int *blk_sys = nullptr, *blk_r0 = nullptr, *blk_r1 = nullptr, *sys_blk0 = nullptr;
+  // vectors
+  double *x = nullptr, *r = nullptr, *d = nullptr, *p = nullptr, *h = nullptr, *y = nullptr,
+         *part = nullptr;
+  float *x32 = nullptr, *r32 = nullptr, *d32 = nullptr, *p32 = nullptr, *h32 = nullptr,
+        *y32 = nullptr, *part32 = nullptr;
+  StateArrays<double> st64;
+  StateArrays<float> st32;
+  BatchMixDev mix;
+  int *active = nullptr;
+  int *active_host = nullptr;  // pinned, 2 entries
+  BatchTune tune64, tune32;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};  // [precision]
+  int graph_chunk[2] = {0, 0};
+  double graph_res[2] = {0.0, 0.0}, graph_zero[2] = {0.0, 0.0};
+  BatchDev dev() const
+  {
+    BatchDev b;
+    b.nsys = nsys;
+    b.R = R;
+    b.nblk = nblk;
+    b.nrows = nrows;
+    b.rowptr = rowptr;
+    b.col = col;
+    b.blk_sys = blk_sys;
+    b.blk_r0 = blk_r0;
+    b.blk_r1 = blk_r1;
+    b.sys_blk0 = sys_blk0;
+    return b;
+  }
+  template <typename T>
+  BatchVec<T> vec() const;
+};
+
+template <>
+BatchVec<double> ssf_batch::vec<double>() const
+{
+  BatchVec<double> v;
+  v.val = val; v.diag = diag; v.x = x; v.r = r; v.d = d; v.p = p; v.h = h; v.y = y; v.part = part;
+  return v;
+}
+template <>
+BatchVec<float> ssf_batch::vec<float>() const
+{
+  BatchVec<float> v;
+  v.val = val32; v.diag = diag32; v.x = x32; v.r = r32; v.d = d32; v.p = p32; v.h = h32;
+  v.y = y32; v.part = part32;
+  return v;
+}
+
+static void batch_release(ssf_batch *b)
+{
+  for (int k = 0; k < 2; k++)
+    if (b->graph[k]) (void)hipGraphExecDestroy(b->graph[k]);
+  bfree(b->rowptr); bfree(b->col); bfree(b->val); bfree(b->diag); bfree(b->val32); bfree(b->diag32);
+  bfree(b->blk_sys); bfree(b->blk_r0); bfree(b->blk_r1); bfree(b->sys_blk0);
+  bfree(b->x); bfree(b->r); bfree(b->d); bfree(b->p); bfree(b->h); bfree(b->y); bfree(b->part);
+  bfree(b->x32); bfree(b->r32); bfree(b->d32); bfree(b->p32); bfree(b->h32); bfree(b->y32);
+  bfree(b->part32);
+  b->st64.release();
+  b->st32.release();
+  BatchMixDev &m = b->mix;
+  bfree(m.nory64); bfree(m.rr64); bfree(m.best); bfree(m.scale); bfree(m.delta2); bfree(m.bad);
+  bfree(m.since_best); bfree(m.final); bfree(m.status); bfree(m.updates); bfree(m.sys_final);
+  bfree(b->active);
+  if (b->active_host) (void)hipHostFree(b->active_host);
+  b->active_host = nullptr;
+}
+
+extern "C" int ssf_batch_destroy(ssf_batch *b)
+{
+  SSF_ENTER();
+  if (!b) return SSF_OK;
+  (void)hipSetDevice(b->ctx->device);
+  (void)hipStreamSynchronize(b->ctx->stream);
+  batch_release(b);
+  b->ctx->live_matrices--;
+  delete b;
+  return SSF_OK;
+}
+
+extern "C" int ssf_batch_create(ssf_ctx *ctx, int64_t nsys, ssf_matrix *const *mats, int32_t nrhs,
+                                ssf_batch **out)
+{
+  SSF_ENTER();
+  SSF_CHECK(ctx && mats && out && nsys > 0, SSF_EINVAL, "ssf_batch_create: bad argument");
+  SSF_CHECK(nrhs == 1 || nrhs == 2 || nrhs == 4 || nrhs == 8, SSF_EINVAL,
+            "ssf_batch_create: nrhs must be 1, 2, 4 or 8");
+  int64_t nrows = 0, nnz = 0;
+  for (int64_t s = 0; s < nsys; s++) {
+    SSF_CHECK(mats[s] && mats[s]->ctx == ctx, SSF_EINVAL, "ssf_batch_create: matrix of another context");
+    SSF_CHECK(mats[s]->values_set, SSF_ESTATE, "ssf_batch_create: matrix values were never set");
+    nrows += mats[s]->n;
+    nnz += mats[s]->nnz;
+  }
+  SSF_CHECK(nnz < (1LL << 31) && nrows * nrhs < (1LL << 40), SSF_EUNSUP,
+            "ssf_batch_create: batch larger than 2^31 matrix entries");
+  SSF_HIP(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  ssf_batch *b = new ssf_batch();
+  b->ctx = ctx;
+  ctx->live_matrices++;
+  b->nsys = (int)nsys;
+  b->R = nrhs;
+  b->nrows = nrows;
+  b->nnz = nnz;
+  b->row0.assign(nsys + 1, 0);
+  for (int64_t s = 0; s < nsys; s++) b->row0[s + 1] = b->row0[s] + mats[s]->n;
+  // block table: contiguous row ranges inside one system
+  const int64_t rpb = std::max<int64_t>(64, 2048 / nrhs);
+  std::vector<int> bsys, br0, br1, sblk0(nsys + 1, 0);
+  for (int64_t s = 0; s < nsys; s++) {
+    sblk0[s] = (int)bsys.size();
+    const int64_t n = mats[s]->n;
+    const int64_t nb = std::max<int64_t>(1, (n + rpb - 1) / rpb);
+    for (int64_t k = 0; k < nb; k++) {
+      bsys.push_back((int)s);
+      br0.push_back((int)(b->row0[s] + n * k / nb));
+      br1.push_back((int)(b->row0[s] + n * (k + 1) / nb));
+    }
+  }
+  sblk0[nsys] = (int)bsys.size();
+  b->nblk = (int)bsys.size();
+  const int64_t nq = nsys * nrhs, nv = nrows * nrhs;
+  hipError_t e = hipSuccess;
+  auto c = [&](hipError_t x) { if (e == hipSuccess) e = x; };
+  c(balloc(&b->rowptr, nrows + 1)); c(balloc(&b->col, nnz)); c(balloc(&b->val, nnz));
+  c(balloc(&b->diag, nrows));
+  c(balloc(&b->blk_sys, b->nblk)); c(balloc(&b->blk_r0, b->nblk)); c(balloc(&b->blk_r1, b->nblk));
+  c(balloc(&b->sys_blk0, nsys + 1));
+  c(balloc(&b->x, nv)); c(balloc(&b->r, nv)); c(balloc(&b->d, nv)); c(balloc(&b->p, nv));
+  c(balloc(&b->h, nv)); c(balloc(&b->y, nv)); c(balloc(&b->part, 3 * (int64_t)b->nblk * nrhs));
+  c(b->st64.alloc(nq, nsys));
+  c(balloc(&b->active, 1));
+  if (e == hipSuccess) c(hipHostMalloc((void **)&b->active_host, 2 * sizeof(int), hipHostMallocDefault));
+  if (e == hipSuccess) {
+    c(hipMemcpy(b->blk_sys, bsys.data(), bsys.size() * sizeof(int), hipMemcpyHostToDevice));
+    c(hipMemcpy(b->blk_r0, br0.data(), br0.size() * sizeof(int), hipMemcpyHostToDevice));
+    c(hipMemcpy(b->blk_r1, br1.data(), br1.size() * sizeof(int), hipMemcpyHostToDevice));
+    c(hipMemcpy(b->sys_blk0, sblk0.data(), sblk0.size() * sizeof(int), hipMemcpyHostToDevice));
+    // block-diagonal CSR: rowptr and columns shifted by the system offsets
+    int64_t k0 = 0;
+    for (int64_t s = 0; s < nsys && e == hipSuccess; s++) {
+      const ssf_matrix *m = mats[s];
+      c(offset_copy(m->n, m->rowptr, b->rowptr + b->row0[s], (int)k0, st));
+      c(offset_copy(m->nnz, m->col, b->col + k0, (int)b->row0[s], st));
+      c(hipMemcpyAsync(b->val + k0, m->val, m->nnz * sizeof(double), hipMemcpyDeviceToDevice, st));
+      c(hipMemcpyAsync(b->diag + b->row0[s], m->diag, m->n * sizeof(double), hipMemcpyDeviceToDevice, st));
+      k0 += m->nnz;
+    }
+    const int last = (int)nnz;
+    c(hipMemcpyAsync(b->rowptr + nrows, &last, sizeof(int), hipMemcpyHostToDevice, st));
+    c(hipStreamSynchronize(st));
+  }
+  if (e != hipSuccess) {
+    const std::string msg = std::string("ssf_batch_create: ") + hipGetErrorString(e);
+    ssf_batch_destroy(b);
+    set_error(msg);
+    return e == hipErrorOutOfMemory ? SSF_ENOMEM : SSF_EHIP;
+  }
+  // lanes per row: 4 per right-hand side. The default does not depend on the
+  // batch's contents, so a system's summation order (and result) is the same
+  // alone and among others; only ssf_batch_autotune changes it.
+  b->tune64.lanes = std::min(64, 4 * nrhs);
+  b->tune32 = b->tune64;
+  *out = b;
+  return SSF_OK;
+}
+
+static int ensure_mixed(ssf_batch *b)
+{
+  if (b->val32) return SSF_OK;
+  const int64_t nq = (int64_t)b->nsys * b->R, nv = b->nrows * b->R;
+  hipError_t e = hipSuccess;
+  auto c = [&](hipError_t x) { if (e == hipSuccess) e = x; };
+  c(balloc(&b->val32, b->nnz)); c(balloc(&b->diag32, b->nrows));
+  c(balloc(&b->x32, nv)); c(balloc(&b->r32, nv)); c(balloc(&b->d32, nv)); c(balloc(&b->p32, nv));
+  c(balloc(&b->h32, nv)); c(balloc(&b->y32, nv));
+  c(balloc(&b->part32, 3 * (int64_t)b->nblk * b->R));
+  c(b->st32.alloc(nq, b->nsys));
+  BatchMixDev &m = b->mix;
+  c(balloc(&m.nory64, nq)); c(balloc(&m.rr64, nq)); c(balloc(&m.best, nq)); c(balloc(&m.scale, nq));
+  c(balloc(&m.delta2, nq)); c(balloc(&m.bad, nq)); c(balloc(&m.since_best, nq));
+  c(balloc(&m.final, nq)); c(balloc(&m.status, nq)); c(balloc(&m.updates, nq));
+  c(balloc(&m.sys_final, b->nsys));
+  m.active = b->active;
+  if (e == hipSuccess) c(to_f32(b->nnz, b->val, b->val32, 1.0, b->ctx->stream));
+  if (e == hipSuccess) c(to_f32(b->nrows, b->diag, b->diag32, 1.0, b->ctx->stream));
+  SSF_HIP(e);
+  return SSF_OK;
+}
+
+// one chunk: `c` iterations, then the count of unfinished pairs into `active`
+// (the mixed solve ends its chunk with the FP64 refresh, which counts them)
+static hipError_t enqueue_chunk(ssf_batch *b, int mixed, int64_t c, const ssf_solve_params *p)
+{
+  hipStream_t s = b->ctx->stream;
+  const BatchDev B = b->dev();
+  hipError_t e = hipSuccess;
+  for (int64_t q = 0; q < c && e == hipSuccess; q++) {
+    if (mixed)
+      e = batch_iteration<float>(b->tune32, B, b->vec<float>(), b->st32.s, 1.0f, 1e-30f, true, s);
+    else
+      e = batch_iteration<double>(b->tune64, B, b->vec<double>(), b->st64.s, p->res, p->zero,
+                                  false, s);
+  }
+  if (e != hipSuccess) return e;
+  if (mixed)
+    return batch_mixed_refresh(b->tune64, B, b->vec<double>(), b->vec<float>(), b->st32.s, b->mix,
+                               p->res, s);
+  e = zero_int(b->active, s);
+  if (e == hipSuccess) e = batch_flags<double>(B, b->st64.s, b->active, s);
+  return e;
+}
+
+// a full chunk is captured once into a graph (kernel arguments include res and
+// zero) and replayed; the graph holds kernel nodes only
+static int chunk_graph(ssf_batch *b, int mixed, int chunk, const ssf_solve_params *p)
+{
+  if (b->graph[mixed] && b->graph_chunk[mixed] == chunk && b->graph_res[mixed] == p->res &&
+      b->graph_zero[mixed] == p->zero)
+    return SSF_OK;
+  if (b->graph[mixed]) (void)hipGraphExecDestroy(b->graph[mixed]);
+  b->graph[mixed] = nullptr;
+  hipStream_t s = b->ctx->stream;
+  SSF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  hipError_t e = enqueue_chunk(b, mixed, chunk, p);
+  hipGraph_t g = nullptr;
+  hipError_t e2 = hipStreamEndCapture(s, &g);
+  if (e == hipSuccess) e = e2;
+  if (e == hipSuccess) e = hipGraphInstantiate(&b->graph[mixed], g, nullptr, nullptr, 0);
+  if (g) (void)hipGraphDestroy(g);
+  SSF_HIP(e);
+  b->graph_chunk[mixed] = chunk;
+  b->graph_res[mixed] = p->res;
+  b->graph_zero[mixed] = p->zero;
+  return SSF_OK;
+}
+
+extern "C" int ssf_batch_solve_dev(ssf_batch *b, const ssf_solve_params *p, const double *rhs_dev,
+                                   double *lhs_dev, ssf_solve_info *infos)
+{
+  SSF_ENTER();
+  SSF_CHECK(b && p && rhs_dev && lhs_dev && infos, SSF_EINVAL, "ssf_batch_solve_dev: NULL argument");
+  SSF_CHECK(p->linsolver == SSF_SOLVER_CG && p->prec == SSF_PREC_DIAG, SSF_EUNSUP,
+            "ssf_batch_solve_dev: the batched solver is Jacobi-PCG (cg + diagprec)");
+  SSF_HIP(hipSetDevice(b->ctx->device));
+  const bool mixed = p->precision == SSF_PRECISION_MIXED;
+  if (mixed) {
+    int rc = ensure_mixed(b);
+    if (rc) return rc;
+  }
+  hipStream_t s = b->ctx->stream;
+  const int64_t nv = b->nrows * b->R, nq = (int64_t)b->nsys * b->R;
+  const BatchDev B = b->dev();
+  SSF_HIP(hipMemcpyAsync(b->y, rhs_dev, nv * sizeof(double), hipMemcpyDeviceToDevice, s));
+  if (p->iv) SSF_HIP(hipMemcpyAsync(b->x, lhs_dev, nv * sizeof(double), hipMemcpyDeviceToDevice, s));
+  else SSF_HIP(hipMemsetAsync(b->x, 0, nv * sizeof(double), s));
+  const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(p->inner_ni > 0 ? p->inner_ni : 32,
+                                                                 std::max<int64_t>(p->ni, 1)));
+  if (mixed) {
+    SSF_HIP(hipMemsetAsync(b->x32, 0, nv * sizeof(float), s));
+    const double delta = (p->inner_res > 0.0 && p->inner_res < 1.0) ? p->inner_res : 0.1;
+    SSF_HIP(batch_mixed_start(b->tune64, b->tune32, B, b->vec<double>(), b->vec<float>(), b->st32.s,
+                              b->mix, p->res, delta, s));
+  } else {
+    SSF_CHECK(!p->iv, SSF_EUNSUP, "ssf_batch_solve_dev: the FP64 batched solve starts from x = 0");
+    SSF_HIP(batch_init<double>(B, b->vec<double>(), b->st64.s, p->zero, s));
+  }
+  int rc = chunk_graph(b, mixed ? 1 : 0, chunk, p);
+  if (rc) return rc;
+  hipEvent_t ev[2];
+  SSF_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+  SSF_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  int64_t it = 0;
+  int k = 0;
+  bool pending = false;
+  hipError_t e = hipSuccess;
+  while (true) {
+    bool launched = false;
+    if (it < p->ni) {
+      // the last chunk is shortened so that exactly ni iterations run
+      const int64_t c = std::min<int64_t>(chunk, p->ni - it);
+      e = c == chunk ? hipGraphLaunch(b->graph[mixed ? 1 : 0], s)
+                     : enqueue_chunk(b, mixed ? 1 : 0, c, p);
+      if (e != hipSuccess) break;
+      it += c;
+      (void)hipMemcpyAsync(&b->active_host[k & 1], b->active, sizeof(int), hipMemcpyDeviceToHost, s);
+      (void)hipEventRecord(ev[k & 1], s);
+      launched = true;
+    }
+    if (pending) {
+      (void)hipEventSynchronize(ev[(k - 1) & 1]);
+      if (b->active_host[(k - 1) & 1] == 0) break;
+    }
+    if (!launched) break;
+    pending = true;
+    k++;
+  }
+  (void)hipStreamSynchronize(s);
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  SSF_HIP(e);
+  SSF_HIP(hipMemcpyAsync(lhs_dev, b->x, nv * sizeof(double), hipMemcpyDeviceToDevice, s));
+  // per-pair results
+  std::vector<int> status(nq), done(nq);
+  std::vector<long long> iter(nq), ani(nq);
+  if (mixed) {
+    std::vector<int> fin(nq);
+    std::vector<long long> upd(nq);
+    std::vector<double> rr(nq), ny(nq);
+    SSF_HIP(hipMemcpy(fin.data(), b->mix.final, nq * sizeof(int), hipMemcpyDeviceToHost));
+    SSF_HIP(hipMemcpy(status.data(), b->mix.status, nq * sizeof(int), hipMemcpyDeviceToHost));
+    SSF_HIP(hipMemcpy(upd.data(), b->mix.updates, nq * sizeof(long long), hipMemcpyDeviceToHost));
+    SSF_HIP(hipMemcpy(rr.data(), b->mix.rr64, nq * sizeof(double), hipMemcpyDeviceToHost));
+    SSF_HIP(hipMemcpy(ny.data(), b->mix.nory64, nq * sizeof(double), hipMemcpyDeviceToHost));
+    SSF_HIP(hipMemcpy(iter.data(), b->st32.s.iter, nq * sizeof(long long), hipMemcpyDeviceToHost));
+    for (int64_t q = 0; q < nq; q++) {
+      ssf_solve_info &in = infos[q];
+      in.status = fin[q] ? status[q] : 0;
+      in.iterations = iter[q];
+      in.outer = upd[q];
+      in.ani = iter[q];
+      in.ares = ny[q] > 0 ? rr[q] / ny[q] : 0.0;
+      in.rel_residual = ny[q] > 0 ? std::sqrt(rr[q] / ny[q]) : 0.0;
+    }
+  } else {
+    std::vector<double> ares(nq);
+    SSF_HIP(hipMemcpy(status.data(), b->st64.s.status, nq * sizeof(int), hipMemcpyDeviceToHost));
+    SSF_HIP(hipMemcpy(iter.data(), b->st64.s.iter, nq * sizeof(long long), hipMemcpyDeviceToHost));
+    SSF_HIP(hipMemcpy(ani.data(), b->st64.s.ani, nq * sizeof(long long), hipMemcpyDeviceToHost));
+    SSF_HIP(hipMemcpy(ares.data(), b->st64.s.ares, nq * sizeof(double), hipMemcpyDeviceToHost));
+    for (int64_t q = 0; q < nq; q++) {
+      ssf_solve_info &in = infos[q];
+      in.status = status[q];
+      in.outer = 0;
+      in.iterations = status[q] == 3 ? 0 : (status[q] ? ani[q] + 1 : iter[q]);
+      if (status[q]) {
+        in.ani = ani[q];
+        in.ares = ares[q];
+      }
+      in.rel_residual = -1.0;
+    }
+  }
+  SSF_HIP(hipStreamSynchronize(s));
+  return SSF_OK;
+}
+
+extern "C" int ssf_batch_solve(ssf_batch *b, const ssf_solve_params *p, const double *rhs,
+                               double *lhs, ssf_solve_info *infos)
+{
+  SSF_ENTER();
+  SSF_CHECK(b && p && rhs && lhs && infos, SSF_EINVAL, "ssf_batch_solve: NULL argument");
+  SSF_HIP(hipSetDevice(b->ctx->device));
+  const int64_t nv = b->nrows * b->R;
+  double *y = nullptr, *x = nullptr;
+  SSF_HIP(balloc(&y, nv));
+  hipError_t e = balloc(&x, nv);
+  if (e != hipSuccess) {
+    bfree(y);
+    SSF_HIP(e);
+  }
+  hipStream_t s = b->ctx->stream;
+  int rc = SSF_OK;
+  e = hipMemcpyAsync(y, rhs, nv * sizeof(double), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && p->iv) e = hipMemcpyAsync(x, lhs, nv * sizeof(double), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) rc = ssf_batch_solve_dev(b, p, y, x, infos);
+  if (e == hipSuccess && rc == SSF_OK) {
+    e = hipMemcpyAsync(lhs, x, nv * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+  }
+  bfree(y);
+  bfree(x);
+  SSF_HIP(e);
+  return rc;
+}
+
+extern "C" int ssf_batch_sizes(const ssf_batch *b, int64_t *nrows, int64_t *nnz, int64_t *row0)
+{
+  SSF_ENTER();
+  SSF_CHECK(b, SSF_EINVAL, "ssf_batch_sizes: NULL batch");
+  if (nrows) *nrows = b->nrows;
+  if (nnz) *nnz = b->nnz;
+  if (row0) std::memcpy(row0, b->row0.data(), (b->nsys + 1) * sizeof(int64_t));
+  return SSF_OK;
+}
+
+// lanes per row of the batched SpMM (a multiple of R up to 64); the autotuner
+// times the candidates on this batch and keeps the fastest
+extern "C" int ssf_batch_autotune(ssf_batch *b, int precision, int reps, int32_t *lanes_out)
+{
+  SSF_ENTER();
+  SSF_CHECK(b, SSF_EINVAL, "ssf_batch_autotune: NULL batch");
+  SSF_HIP(hipSetDevice(b->ctx->device));
+  if (precision) {
+    int rc = ensure_mixed(b);
+    if (rc) return rc;
+  }
+  if (reps <= 0) reps = 5;
+  hipStream_t s = b->ctx->stream;
+  const BatchDev B = b->dev();
+  hipEvent_t t0, t1;
+  SSF_HIP(hipEventCreate(&t0));
+  SSF_HIP(hipEventCreate(&t1));
+  float best = 1e30f;
+  int best_l = precision ? b->tune32.lanes : b->tune64.lanes;
+  for (int l = b->R; l <= 64; l *= 2) {
+    BatchTune t;
+    t.lanes = l;
+    auto launch = [&]() -> hipError_t {
+      if (precision) return bspmm<float>(t, 0, B, b->val32, b->d32, b->p32, nullptr, nullptr, nullptr, s);
+      return bspmm<double>(t, 0, B, b->val, b->d, b->p, nullptr, nullptr, nullptr, s);
+    };
+    SSF_HIP(launch());
+    SSF_HIP(hipEventRecord(t0, s));
+    for (int i = 0; i < reps; i++) SSF_HIP(launch());
+    SSF_HIP(hipEventRecord(t1, s));
+    SSF_HIP(hipEventSynchronize(t1));
+    float ms = 0.f;
+    SSF_HIP(hipEventElapsedTime(&ms, t0, t1));
+    if (ms < best) {
+      best = ms;
+      best_l = l;
+    }
+  }
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  if (precision) b->tune32.lanes = best_l;
+  else b->tune64.lanes = best_l;
+  // a new tiling invalidates the captured chunk
+  const int k = precision ? 1 : 0;
+  if (b->graph[k]) (void)hipGraphExecDestroy(b->graph[k]);
+  b->graph[k] = nullptr;
+  if (lanes_out) *lanes_out = best_l;
+  return SSF_OK;
+}

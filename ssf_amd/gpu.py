@@ -73,6 +73,12 @@ SIGNATURES = {
     "ssf_ctx_kernel_time": (_I, [_P, _I, ctypes.POINTER(_D), ctypes.POINTER(_L)]),
     "ssf_matrix_autotune": (_I, [_P, _I, _I, _P]),
     "ssf_matrix_set_tune": (_I, [_P, _I, _P]),
+    "ssf_batch_create": (_I, [_P, _L, _P, ctypes.c_int32, ctypes.POINTER(_P)]),
+    "ssf_batch_destroy": (_I, [_P]),
+    "ssf_batch_sizes": (_I, [_P, _P, _P, _P]),
+    "ssf_batch_solve": (_I, [_P, ctypes.POINTER(SolveParams), _P, _P, _P]),
+    "ssf_batch_solve_dev": (_I, [_P, ctypes.POINTER(SolveParams), _P, _P, _P]),
+    "ssf_batch_autotune": (_I, [_P, _I, _I, _P]),
 }
 
 _lib = None
@@ -83,6 +89,15 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"HIP library not built: {LIB_PATH} (run python ssf_amd/build.py)")
+        # One HIP runtime per process: torch ships its own libamdhip64. Loaded
+        # first, it is the runtime this library binds to (same soname), and torch
+        # tensors and this library's calls share one device context. Loaded
+        # after this library, a second runtime would start and torch would see
+        # no GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         l = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(l, name)
@@ -269,6 +284,80 @@ class Matrix:
     def set_tune(self, precision, tune):
         t = np.ascontiguousarray(tune, dtype=np.int32)
         _check(lib().ssf_matrix_set_tune(self.h, precision, _p(t)))
+
+
+class Batch:
+    """Independent systems x nrhs right-hand sides solved together on the GPU.
+
+    Vectors are interleaved: entry (system s, row i, rhs r) sits at
+    (row0[s] + i) * nrhs + r.
+    """
+
+    def __init__(self, ctx: Context, mats, nrhs: int = 1):
+        self.ctx = ctx
+        self.nsys = len(mats)
+        self.nrhs = nrhs
+        hs = (_P * self.nsys)(*[m.h for m in mats])
+        self.h = _P()
+        _check(lib().ssf_batch_create(ctx.h, self.nsys, hs, nrhs, ctypes.byref(self.h)))
+        nrows, nnz = _L(), _L()
+        self.row0 = np.zeros(self.nsys + 1, np.int64)
+        _check(lib().ssf_batch_sizes(self.h, ctypes.byref(nrows), ctypes.byref(nnz),
+                                     _p(self.row0)))
+        self.nrows, self.nnz = nrows.value, nnz.value
+
+    def close(self):
+        if self.h:
+            lib().ssf_batch_destroy(self.h)
+            self.h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def pack(self, rhs_list):
+        """[nsys] arrays of shape (nrhs, n_s) -> interleaved vector."""
+        v = np.zeros((self.nrows, self.nrhs))
+        for s, r in enumerate(rhs_list):
+            r = np.asarray(r, dtype=np.float64).reshape(self.nrhs, -1)
+            v[self.row0[s]:self.row0[s + 1], :] = r.T
+        return v.reshape(-1)
+
+    def unpack(self, v):
+        v = np.asarray(v).reshape(self.nrows, self.nrhs)
+        return [v[self.row0[s]:self.row0[s + 1], :].T.copy() for s in range(self.nsys)]
+
+    def _params(self, ni, res, precision, zero, iv, inner_res, inner_ni):
+        return SolveParams(SOLVER_CG, PREC_DIAG, ni, res, zero, iv, precision, inner_res,
+                           inner_ni, 0)
+
+    def _infos(self):
+        return (SolveInfo * (self.nsys * self.nrhs))()
+
+    def solve(self, rhs_list, ni=100000, res=1e-20, precision=FP64, zero=1e-30, x0=None,
+              inner_res=1e-4, inner_ni=0):
+        """Returns ([nsys] arrays (nrhs, n_s), infos[nsys * nrhs])."""
+        y = self.pack(rhs_list)
+        x = np.zeros_like(y) if x0 is None else self.pack(x0)
+        p = self._params(ni, res, precision, zero, 0 if x0 is None else 1, inner_res, inner_ni)
+        infos = self._infos()
+        _check(lib().ssf_batch_solve(self.h, ctypes.byref(p), _p(y), _p(x), infos))
+        return self.unpack(x), list(infos)
+
+    def solve_dev(self, rhs_ptr, x_ptr, ni=100000, res=1e-20, precision=FP64, zero=1e-30, iv=0,
+                  inner_res=1e-4, inner_ni=0):
+        p = self._params(ni, res, precision, zero, iv, inner_res, inner_ni)
+        infos = self._infos()
+        _check(lib().ssf_batch_solve_dev(self.h, ctypes.byref(p), ctypes.c_void_p(rhs_ptr),
+                                         ctypes.c_void_p(x_ptr), infos))
+        return list(infos)
+
+    def autotune(self, precision=FP64, reps=5):
+        out = ctypes.c_int32()
+        _check(lib().ssf_batch_autotune(self.h, precision, reps, ctypes.byref(out)))
+        return out.value
 
 
 def build_problem_matrix(ctx: Context, prob: mesh.Problem, storage=STORAGE_SCR):
