@@ -35,6 +35,102 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def bench_batched(ctx, args, gpu, mesh, torch):
+    """512 independent quad4 plane-stress plates (1e3..6e4 DOF, seed 4), 8 load cases
+    each: 4096 solves in one batch. Compared with the drop-in path, which solves
+    the 4096 systems one after the other with the FP64 GPU PCG (timed on every
+    `seq_stride`-th plate, all 8 load cases, and scaled to the 512 plates)."""
+    t0 = time.time()
+    probs = mesh.plate_batch(args.batch_count, seed=4, nrhs=8)
+    mats, rhs = [], []
+    for p in probs:
+        m, r = gpu.build_problem_matrix(ctx, p)
+        mats.append(m)
+        rhs.append(r)
+    b = gpu.Batch(ctx, mats, 8)
+    ndof = int(sum(p.n for p in probs))
+    log(f"[batch] {len(probs)} plates, {ndof} DOF, {b.nnz} entries, 8 rhs "
+        f"({time.time() - t0:.1f}s to generate and assemble)")
+    lanes64 = b.autotune(gpu.FP64)
+    lanes32 = b.autotune(gpu.MIXED)
+    dev = torch.device("cuda:0")
+    y = torch.tensor(b.pack(rhs), dtype=torch.float64, device=dev)
+    x = torch.zeros_like(y)
+    xref = torch.zeros_like(y)
+    torch.cuda.synchronize()
+    b.solve_dev(y.data_ptr(), xref.data_ptr(), ni=40000, res=1e-28)
+    nq = len(probs) * 8
+    row0 = b.row0
+
+    idx = torch.repeat_interleave(torch.arange(len(probs), device=dev),
+                                  torch.tensor(np.diff(row0), device=dev))
+
+    def pair_errors():
+        """relative error of every (plate, load case) pair, in pair order s * 8 + r"""
+        d = (x - xref).view(-1, 8)
+        rr = xref.view(-1, 8)
+        num = torch.zeros(len(probs), 8, dtype=torch.float64, device=dev)
+        den = torch.zeros(len(probs), 8, dtype=torch.float64, device=dev)
+        num.index_add_(0, idx, d * d)
+        den.index_add_(0, idx, rr * rr)
+        return torch.sqrt(num / den).reshape(-1).cpu().numpy()
+
+    out = {}
+    for name, prec in (("fp64", gpu.FP64), ("mixed", gpu.MIXED)):
+        for _ in range(args.warmup):
+            b.solve_dev(y.data_ptr(), x.data_ptr(), ni=40000, res=args.batch_res, precision=prec)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            infos = b.solve_dev(y.data_ptr(), x.data_ptr(), ni=40000, res=args.batch_res,
+                                precision=prec)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / args.steps
+        its = [i.iterations for i in infos]
+        err = pair_errors()
+        st = np.array([i.status for i in infos])
+        by_status = {int(k): {"pairs": int((st == k).sum()), "max_rel_err": float(err[st == k].max())}
+                     for k in np.unique(st)}
+        out[name] = {"solves_per_s": nq / dt, "ms_per_batch": 1e3 * dt,
+                     "max_rel_err": float(err.max()), "by_status": by_status,
+                     "max_iterations": int(max(its)), "mean_iterations": float(np.mean(its))}
+        log(f"[batch] {name}: {out[name]}")
+
+    # the drop-in path: one FP64 solve per (plate, load case), one after the other
+    stride = args.seq_stride
+    sample = list(range(0, len(probs), stride))
+    ys = [torch.tensor(rhs[s], dtype=torch.float64, device=dev) for s in sample]
+    xs = [torch.zeros(probs[s].n, dtype=torch.float64, device=dev) for s in sample]
+    for j, s in enumerate(sample[:4]):
+        mats[s].solve_dev(ys[j][0].data_ptr(), xs[j].data_ptr(), ni=40000, res=args.batch_res)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    nseq = 0
+    for j, s in enumerate(sample):
+        for r in range(8):
+            mats[s].solve_dev(ys[j][r].data_ptr(), xs[j].data_ptr(), ni=40000, res=args.batch_res)
+            nseq += 1
+    torch.cuda.synchronize()
+    dt_seq = (time.perf_counter() - t) * (len(probs) / len(sample))
+    seq = nq / dt_seq
+    log(f"[batch] sequential FP64 drop-in: {nseq} solves timed, {seq:.1f} solves/s (scaled)")
+    for m in mats:
+        m.close()
+    return {
+        "workload": f"{len(probs)} quad4 plates (seed 4), {ndof} DOF, 8 load cases each",
+        "solves": nq, "res": args.batch_res,
+        "batched_mixed_solves_per_s": out["mixed"]["solves_per_s"],
+        "batched_fp64_solves_per_s": out["fp64"]["solves_per_s"],
+        "sequential_fp64_solves_per_s": seq,
+        "speedup_mixed_vs_sequential_fp64": out["mixed"]["solves_per_s"] / seq,
+        "speedup_mixed_vs_batched_fp64": out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"],
+        "max_rel_err": {"mixed": out["mixed"]["max_rel_err"], "fp64": out["fp64"]["max_rel_err"]},
+        "detail": out, "lanes": {"fp64": lanes64, "fp32": lanes32},
+        "sequential_sample": f"every {stride}th plate ({len(sample)} plates x 8 load cases), "
+                             f"time scaled by {len(probs) / len(sample):.0f}",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -46,6 +142,11 @@ def main():
                     help="FP32 iterations between FP64 refinements (mixed)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-iters", type=int, default=20)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched leg")
+    ap.add_argument("--batch-count", type=int, default=512)
+    ap.add_argument("--batch-res", type=float, default=1e-20)
+    ap.add_argument("--seq-stride", type=int, default=16)
+    ap.add_argument("--only-batch", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; the path shards by independent "
@@ -55,6 +156,10 @@ def main():
     import torch
     from ssf_amd import gpu, mesh
 
+    if args.only_batch:
+        ctx = gpu.Context(0)
+        print(json.dumps(bench_batched(ctx, args, gpu, mesh, torch)), flush=True)
+        return
     t0 = time.time()
     prob = mesh.heat_cube(args.nel, seed=2)
     log(f"[bench] problem {prob.name}: n={prob.n} ne={prob.ne} ({time.time() - t0:.1f}s)")
@@ -152,6 +257,13 @@ def main():
         if tr["n"] == prob.n and tr["nnz"] == m.nnz and list(tr["fp32"]["tiling"]) == list(tune32):
             traffic = tr["fp32"]["bytes_per_launch"]
 
+    batched = None
+    if not args.no_batch:
+        m.close()
+        del b, x, xref
+        torch.cuda.empty_cache()
+        batched = bench_batched(ctx, args, gpu, mesh, torch)
+
     mixed = results["mixed"]
     fp64 = results["fp64"]
     line = {
@@ -182,6 +294,7 @@ def main():
                      "bytes_per_launch": b32_act, "bytes_per_launch_canonical_csr": b32_can,
                      "usec_per_launch": us32, "samples": mixed["spmv_samples"]},
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(line), flush=True)
 

@@ -260,7 +260,7 @@ extern "C" int ssf_matrix_destroy(ssf_matrix *m)
   if (m->a_ref == m->val) m->a_ref = nullptr;  // comprow: the same allocation
   dfree(m->a_ref);
   dfree(m->rowptr); dfree(m->col); dfree(m->val); dfree(m->val32);
-  dfree(m->diag); dfree(m->diag32);
+  dfree(m->diag); dfree(m->diag32); dfree(m->norm_a);
   dfree(m->map_lo); dfree(m->map_up);
   dfree(m->slot); dfree(m->order);
   dfree(m->tiles); dfree(m->col16);
@@ -690,6 +690,8 @@ static int ensure_ws(ssf_matrix *m, bool mixed)
     if (!m->diag32) SSF_HIP(dalloc(&m->diag32, m->n));
     SSF_HIP(to_f32(m->nnz, m->val, m->val32, 1.0, m->ctx->stream));
     SSF_HIP(to_f32(m->n, m->diag, m->diag32, 1.0, m->ctx->stream));
+    if (!m->norm_a) SSF_HIP(dalloc(&m->norm_a, 1));
+    SSF_HIP(row_absmax((int)m->n, m->rowptr, m->val, w.part, 1024, m->norm_a, m->ctx->stream));
     m->val32_valid = true;
   }
   return SSF_OK;
@@ -921,7 +923,7 @@ static int solve_mixed(ssf_matrix *m, const ssf_solve_params *p, const double *y
   SSF_HIP(pcg_init<float>(m->tune32, true, false, A, m->val32, m->diag32, w.x32, w.y32, w.r32,
                           w.h32, w.d32, w.part32, w.st32, 1e-30f, s));
   const double delta = (p->inner_res > 0.0 && p->inner_res < 1.0) ? p->inner_res : 0.1;
-  SSF_HIP(mixed_init(w.mix, w.st32, dslot, p->res, delta, s));
+  SSF_HIP(mixed_init(w.mix, w.st32, dslot, m->norm_a, p->res, delta, s));
   hipEvent_t ev[2];
   SSF_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
   SSF_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
@@ -942,7 +944,8 @@ static int solve_mixed(ssf_matrix *m, const ssf_solve_params *p, const double *y
             hipSuccess)
           rc = SSF_EHIP;
       if (rc == SSF_OK &&
-          mixed_update(m->tune64, m->tune32, A, m->val, m->diag32, y, x, w.r, w.part, w.x32,
+          mixed_update(m->tune64, m->tune32, A, m->val, m->diag32, y, x, w.r, w.part,
+                       w.part + w.nb, w.x32,
                        w.r32, w.h32, w.d32, w.part32, w.st32, w.mix, p->res, s) != hipSuccess)
         rc = SSF_EHIP;
       if (rc) {

@@ -16,6 +16,51 @@
 
 namespace ssf {
 
+// ------------------------------------------------------------ reductions
+
+// sum of a per-system quantity over the blocks of system s (fixed order)
+template <typename T, int R>
+__device__ __forceinline__ T sys_sum(const BatchDev &B, const T *__restrict__ part, int s, int r)
+{
+  const int b0 = B.sys_blk0[s], b1 = B.sys_blk0[s + 1];
+  T t = T(0);
+  for (int b = b0; b < b1; b++) t += part[(int64_t)b * R + r];
+  return t;
+}
+
+// block-wide per-rhs reduction: thread t holds a value for rhs t % R; a fixed
+// tree over LDS (pairs t, t + w share their rhs), result in out_lds[0, R)
+template <typename T, int R>
+__device__ __forceinline__ void block_sum_r(T v, T *out_lds)
+{
+  __shared__ T red[TPB];
+  red[threadIdx.x] = v;
+  __syncthreads();
+#pragma unroll
+  for (int w = TPB / 2; w >= R; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < R) out_lds[threadIdx.x] = red[threadIdx.x];
+  __syncthreads();
+}
+
+// Sum over the blocks of system s of part[b * R + r], for every r, by the whole
+// block: thread (j, r) adds the system's blocks j, j + TPB/R, ... (counted from
+// the system's first block), then the fixed tree. The order depends only on the
+// system's own block count, not on its place in the batch.
+template <typename T, int R>
+__device__ __forceinline__ void sys_sum_block(const BatchDev &B, const T *__restrict__ part,
+                                              int s, T *out_lds)
+{
+  const int b0 = B.sys_blk0[s], b1 = B.sys_blk0[s + 1];
+  constexpr int NJ = TPB / R;
+  const int r = threadIdx.x % R;
+  T t = T(0);
+  for (int b = b0 + (int)threadIdx.x / R; b < b1; b += NJ) t += part[(int64_t)b * R + r];
+  block_sum_r<T, R>(t, out_lds);
+}
+
 // ----------------------------------------------------------------- SpMM
 
 // G = R * L lanes per row: lane (l, r) sums entries l, l + L, ... of the row for
@@ -54,42 +99,11 @@ __global__ __launch_bounds__(TPB) void k_bspmm(BatchDev B, const T *__restrict__
     }
   }
   if (KIND != 0) {
-    // per right-hand side: lanes with the same r hold partial sums
-    __shared__ T red[TPB];
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    if (threadIdx.x < R) {
-      T t = T(0);
-      for (int i = threadIdx.x; i < TPB; i += G)
-        for (int j = 0; j < L; j++) t += red[i + j * R];
-      partial[(int64_t)blk * R + threadIdx.x] = t;
-    }
+    // per right-hand side: thread t holds a partial sum for rhs t % R
+    __shared__ T outp[R];
+    block_sum_r<T, R>(acc, outp);
+    if (threadIdx.x < R) partial[(int64_t)blk * R + threadIdx.x] = outp[threadIdx.x];
   }
-}
-
-// sum of a per-system quantity over the blocks of system s (fixed order)
-template <typename T, int R>
-__device__ __forceinline__ T sys_sum(const BatchDev &B, const T *__restrict__ part, int s, int r)
-{
-  const int b0 = B.sys_blk0[s], b1 = B.sys_blk0[s + 1];
-  T t = T(0);
-  for (int b = b0; b < b1; b++) t += part[(int64_t)b * R + r];
-  return t;
-}
-
-// block-wide per-rhs reduction: thread t holds a value for rhs t % R
-template <typename T, int R>
-__device__ __forceinline__ void block_sum_r(T v, T *out_lds)
-{
-  __shared__ T red[TPB];
-  red[threadIdx.x] = v;
-  __syncthreads();
-  if (threadIdx.x < R) {
-    T t = T(0);
-    for (int i = threadIdx.x; i < TPB; i += R) t += red[i];
-    out_lds[threadIdx.x] = t;
-  }
-  __syncthreads();
 }
 
 // ------------------------------------------------------------- B2: update
@@ -104,16 +118,17 @@ __global__ __launch_bounds__(TPB) void k_bupdate(BatchDev B, BatchState<T> st,
                                                  T *__restrict__ h, T *__restrict__ prr)
 {
   __shared__ T alpha[R];
-  __shared__ T outa[R], outb[R];
+  __shared__ T outa[R], outb[R], dens[R];
   const int blk = blockIdx.x;
   const int s = B.blk_sys[blk];
   if (st.sys_done[s]) return;
+  sys_sum_block<T, R>(B, pden, s, dens);
   if (threadIdx.x < R) {
     const int r = threadIdx.x;
     const int q = s * R + r;
     T a = T(0);
     if (!st.done[q]) {
-      const T den = sys_sum<T, R>(B, pden, s, r);
+      const T den = dens[r];
       const T nom = st.nom[2 * q + (st.iter[q] & 1)];
       if (fabs(den) < zero) {
         a = T(0);
@@ -161,10 +176,13 @@ __global__ __launch_bounds__(TPB) void k_bdir(BatchDev B, BatchState<T> st,
 {
   __shared__ T beta[R];
   __shared__ int upd[R];
+  __shared__ T rrs[R], noms[R];
   const int blk = blockIdx.x;
   const int s = B.blk_sys[blk];
   if (st.sys_done[s]) return;
   const bool first = blk == B.sys_blk0[s];
+  sys_sum_block<T, R>(B, prr, s, rrs);
+  sys_sum_block<T, R>(B, prr + (int64_t)B.nblk * R, s, noms);
   if (threadIdx.x < R) {
     const int r = threadIdx.x;
     const int q = s * R + r;
@@ -173,8 +191,8 @@ __global__ __launch_bounds__(TPB) void k_bdir(BatchDev B, BatchState<T> st,
     // state changes are left pending for k_badvance: every block of the system
     // reads the same iter / done / nom while this kernel runs
     if (!st.done[q] && !st.brk[q]) {
-      const T rr = sys_sum<T, R>(B, prr, s, r);
-      const T nomn = sys_sum<T, R>(B, prr + (int64_t)B.nblk * R, s, r);
+      const T rr = rrs[r];
+      const T nomn = noms[r];
       const long long it = st.iter[q];
       const T nom = st.nom[2 * q + (it & 1)];
       bt = nomn / nom;
@@ -303,29 +321,40 @@ __global__ void k_binit2(BatchDev B, BatchState<T> st, const T *__restrict__ prr
 // ------------------------------------------------- mixed: reliable updates
 
 struct BatchMix {
-  double *nory64, *rr64, *best, *scale, *delta2;
+  double *nory64, *rr64, *best, *scale, *delta2, *norm_a;
   int *bad, *since_best, *final, *status;
   long long *updates;
 };
 
+// x64 += x32 / s, x32 = 0 for the unfinished pairs; block partials of ||x64||^2
+// per rhs into px[blk * R + r]
 template <int R>
 __global__ __launch_bounds__(TPB) void k_bmx_xupdate(BatchDev B, BatchMix mx,
                                                      const int *__restrict__ sys_final,
                                                      double *__restrict__ x64,
-                                                     float *__restrict__ x32)
+                                                     float *__restrict__ x32,
+                                                     double *__restrict__ px)
 {
+  __shared__ double outa[R];
   const int blk = blockIdx.x;
   const int s = B.blk_sys[blk];
   if (sys_final[s]) return;
   const int64_t e0 = (int64_t)B.blk_r0[blk] * R, e1 = (int64_t)B.blk_r1[blk] * R;
   const int r = threadIdx.x % R;
   const int q = s * R + r;
-  if (mx.final[q]) return;
-  const double sc = mx.scale[q];
-  for (int64_t e = e0 + threadIdx.x; e < e1; e += TPB) {
-    x64[e] += (double)x32[e] / sc;
-    x32[e] = 0.0f;
+  const bool fin = mx.final[q];
+  const double inv = 1.0 / mx.scale[q];
+  double xx = 0.0;
+  if (!fin) {
+    for (int64_t e = e0 + threadIdx.x; e < e1; e += TPB) {
+      const double xi = x64[e] + (double)x32[e] * inv;
+      x64[e] = xi;
+      x32[e] = 0.0f;
+      xx += xi * xi;
+    }
   }
+  block_sum_r<double, R>(xx, outa);
+  if (threadIdx.x < R) px[(int64_t)blk * R + threadIdx.x] = outa[threadIdx.x];
 }
 
 template <int R>
@@ -382,10 +411,15 @@ __global__ __launch_bounds__(TPB) void k_bmx_refresh(BatchDev B, BatchMix mx,
   }
 }
 
+// below this the FP64 residual is within the rounding error of its own
+// computation: at most m eps (|b| + |A||x|) per row for m entries per row,
+// typically sqrt(m) eps; 64 eps (||b|| + ||A||_inf ||x||) in 2-norm
+constexpr double BMIX_FLOOR = 64.0 * 2.220446049250313e-16;
+
 template <int R>
 __global__ void k_bmx_check(BatchDev B, BatchMix mx, BatchState<float> st,
                             const double *__restrict__ part64, const float *__restrict__ prr,
-                            double res)
+                            const double *__restrict__ px, double res)
 {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= B.nsys * R) return;
@@ -407,7 +441,11 @@ __global__ void k_bmx_check(BatchDev B, BatchMix mx, BatchState<float> st,
   if (rr64 < 0.81 * mx.best[q]) mx.since_best[q] = 0;
   else mx.since_best[q] += 1;
   if (rr64 < mx.best[q]) mx.best[q] = rr64;
-  if (mx.bad[q] >= 3 || mx.since_best[q] >= 10) {
+  // stuck at the FP64 floor: within the rounding floor and no longer improving
+  // (far above it the CG residual norm may stall or grow for many refreshes)
+  const double xx = sys_sum<double, R>(B, px, s, r);
+  const double fl = BMIX_FLOOR * (sqrt(mx.nory64[q]) + mx.norm_a[s] * sqrt(xx));
+  if (rr64 < fl * fl && (mx.bad[q] >= 3 || mx.since_best[q] >= 10)) {
     mx.final[q] = 1;
     mx.status[q] = 4;
     st.done[q] = 1;
@@ -647,6 +685,7 @@ static BatchMix mixview(const BatchMixDev &m)
 {
   BatchMix x;
   x.nory64 = m.nory64; x.rr64 = m.rr64; x.best = m.best; x.scale = m.scale; x.delta2 = m.delta2;
+  x.norm_a = m.norm_a;
   x.bad = m.bad; x.since_best = m.since_best; x.final = m.final; x.status = m.status;
   x.updates = m.updates;
   return x;
@@ -690,7 +729,9 @@ static hipError_t bmx_refresh_r(const BatchTune &t64, const BatchDev &B,
   dim3 grid(B.nblk), block(TPB);
   const int nq = B.nsys * R;
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s, md.active);
-  hipLaunchKernelGGL((k_bmx_xupdate<R>), grid, block, 0, s, B, mx, md.sys_final, v64.x, v32.x);
+  double *px = v64.part + (int64_t)B.nblk * R;  // free in the mixed solve
+  hipLaunchKernelGGL((k_bmx_xupdate<R>), grid, block, 0, s, B, mx, md.sys_final, v64.x, v32.x,
+                     px);
   hipError_t e = bspmm_r<double, R>(t64, 2, B, v64.val, v64.x, v64.r, v64.y, v64.part,
                                     md.sys_final, s);
   if (e != hipSuccess) return e;
@@ -698,7 +739,7 @@ static hipError_t bmx_refresh_r(const BatchTune &t64, const BatchDev &B,
   hipLaunchKernelGGL((k_bmx_refresh<R>), grid, block, 0, s, B, mx, md.sys_final, v64.part, v64.r,
                      v32.diag, v32.r, v32.h, v32.d, prr);
   hipLaunchKernelGGL((k_bmx_check<R>), dim3((nq + TPB - 1) / TPB), dim3(TPB), 0, s, B, mx, st,
-                     v64.part, prr, res);
+                     v64.part, prr, px, res);
   hipLaunchKernelGGL((k_bmx_flags<R>), dim3((B.nsys + TPB - 1) / TPB), dim3(TPB), 0, s, B, mx, st,
                      md.sys_final, md.active);
   return hipGetLastError();
@@ -716,6 +757,47 @@ hipError_t offset_copy(int64_t n, const int *src, int *dst, int add, hipStream_t
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_offset_copy, dim3((unsigned)((n + TPB - 1) / TPB)), dim3(TPB), 0, s, n,
                      src, dst, add);
+  return hipGetLastError();
+}
+
+// ||A||_inf per system: block maxima of the absolute row sums, then the maximum
+// over the blocks of each system (order-independent)
+__global__ __launch_bounds__(TPB) void k_bnorm_blk(BatchDev B, const double *__restrict__ val,
+                                                   double *__restrict__ blkmax)
+{
+  __shared__ double lds[TPB];
+  const int blk = blockIdx.x;
+  double m = 0.0;
+  for (int i = B.blk_r0[blk] + threadIdx.x; i < B.blk_r1[blk]; i += TPB) {
+    double t = 0.0;
+    for (int k = B.rowptr[i]; k < B.rowptr[i + 1]; k++) t += fabs(val[k]);
+    m = fmax(m, t);
+  }
+  lds[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = TPB / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) lds[threadIdx.x] = fmax(lds[threadIdx.x], lds[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) blkmax[blk] = lds[0];
+}
+
+__global__ void k_bnorm_sys(BatchDev B, const double *__restrict__ blkmax,
+                            double *__restrict__ norm_a)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= B.nsys) return;
+  double m = 0.0;
+  for (int b = B.sys_blk0[s]; b < B.sys_blk0[s + 1]; b++) m = fmax(m, blkmax[b]);
+  norm_a[s] = m;
+}
+
+hipError_t batch_norm_inf(const BatchDev &B, const double *val, double *blkmax, double *norm_a,
+                          hipStream_t s)
+{
+  hipLaunchKernelGGL(k_bnorm_blk, dim3(B.nblk), dim3(TPB), 0, s, B, val, blkmax);
+  hipLaunchKernelGGL(k_bnorm_sys, dim3((B.nsys + TPB - 1) / TPB), dim3(TPB), 0, s, B, blkmax,
+                     norm_a);
   return hipGetLastError();
 }
 

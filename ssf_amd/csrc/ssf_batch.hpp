@@ -48,6 +48,7 @@ struct BatchVec {  // interleaved [row * R + r]
 struct BatchMixDev {
   double *nory64 = nullptr, *rr64 = nullptr, *best = nullptr, *scale = nullptr,
          *delta2 = nullptr;
+  double *norm_a = nullptr;  // per system: ||A||_inf
   int *bad = nullptr, *since_best = nullptr, *final = nullptr, *status = nullptr;
   long long *updates = nullptr;
   int *sys_final = nullptr;
@@ -65,6 +66,9 @@ hipError_t batch_iteration(const BatchTune &t, const BatchDev &B, const BatchVec
                            BatchState<T> st, T res, T zero, bool mixed, hipStream_t s);
 template <typename T>
 hipError_t batch_flags(const BatchDev &B, BatchState<T> st, int *active, hipStream_t s);
+// per system ||A||_inf of the FP64 values into norm_a[nsys]; blkmax: nblk doubles
+hipError_t batch_norm_inf(const BatchDev &B, const double *val, double *blkmax, double *norm_a,
+                          hipStream_t s);
 hipError_t offset_copy(int64_t n, const int *src, int *dst, int add, hipStream_t s);
 hipError_t zero_int(int *p, hipStream_t s);
 hipError_t batch_mixed_start(const BatchTune &t64, const BatchTune &t32, const BatchDev &B,

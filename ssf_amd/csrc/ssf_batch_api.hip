@@ -119,6 +119,7 @@ static void batch_release(ssf_batch *b)
   BatchMixDev &m = b->mix;
   bfree(m.nory64); bfree(m.rr64); bfree(m.best); bfree(m.scale); bfree(m.delta2); bfree(m.bad);
   bfree(m.since_best); bfree(m.final); bfree(m.status); bfree(m.updates); bfree(m.sys_final);
+  bfree(m.norm_a);
   bfree(b->active);
   if (b->active_host) (void)hipHostFree(b->active_host);
   b->active_host = nullptr;
@@ -240,9 +241,11 @@ static int ensure_mixed(ssf_batch *b)
   c(balloc(&m.delta2, nq)); c(balloc(&m.bad, nq)); c(balloc(&m.since_best, nq));
   c(balloc(&m.final, nq)); c(balloc(&m.status, nq)); c(balloc(&m.updates, nq));
   c(balloc(&m.sys_final, b->nsys));
+  c(balloc(&m.norm_a, b->nsys));
   m.active = b->active;
   if (e == hipSuccess) c(to_f32(b->nnz, b->val, b->val32, 1.0, b->ctx->stream));
   if (e == hipSuccess) c(to_f32(b->nrows, b->diag, b->diag32, 1.0, b->ctx->stream));
+  if (e == hipSuccess) c(batch_norm_inf(b->dev(), b->val, b->part, m.norm_a, b->ctx->stream));
   SSF_HIP(e);
   return SSF_OK;
 }

@@ -56,6 +56,7 @@ struct MixState {
   double best;     // smallest rr64 that was a 4x improvement
   double scale;    // FP32 quantities are the FP64 ones times scale = 1/||r|| at the last refresh
   double delta2;   // the FP32 recurrence refreshes after reducing its residual by delta^2
+  double norm_a;   // ||A||_inf (FP64 values), for the rounding floor of b - A x
   long long updates;
   int bad;         // consecutive refreshes whose FP32 reduction the FP64 residual did not follow
   int since_best;  // refreshes since rr64 last improved by 10 %
@@ -134,6 +135,7 @@ struct ssf_matrix {
   float *val32 = nullptr;
   double *diag = nullptr;
   float *diag32 = nullptr;
+  double *norm_a = nullptr;  // ||A||_inf, with the FP32 copy (mixed solves)
   // reference layout values on the device and the scatter into the full CSR
   double *a_ref = nullptr;   // == val for comprow
   int *map_lo = nullptr;     // ref entry -> full slot (lower / same)

@@ -28,13 +28,17 @@ hipError_t pcg_iteration(const SpmvTune &t, int mode, bool jac, const CsrDev &A,
                          hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 
 // mixed precision with reliable updates (ssf_solver.hip)
-hipError_t mixed_init(MixState *ms, PcgState<float> *st, const double *slots, double res,
-                      double delta, hipStream_t s);
+hipError_t mixed_init(MixState *ms, PcgState<float> *st, const double *slots,
+                      const double *norm_a, double res, double delta, hipStream_t s);
+// px: vec_blocks(t32, n) doubles of scratch (partials of ||x64||^2)
 hipError_t mixed_update(const SpmvTune &t64, const SpmvTune &t32, const CsrDev &A,
                         const double *val64, const float *diag32, const double *b,
-                        double *x64, double *r64, double *part64, float *x32, float *r32,
-                        float *h32, float *d32, float *part32, PcgState<float> *st,
+                        double *x64, double *r64, double *part64, double *px, float *x32,
+                        float *r32, float *h32, float *d32, float *part32, PcgState<float> *st,
                         MixState *ms, double res, hipStream_t s);
+// ||A||_inf into *out (device); partial: nb doubles of scratch
+hipError_t row_absmax(int n, const int *rowptr, const double *val, double *partial, int nb,
+                      double *out, hipStream_t s);
 hipError_t copy_kernel(int64_t n, const double *a, double *b, hipStream_t s);
 
 // element-wise helpers

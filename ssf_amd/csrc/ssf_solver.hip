@@ -419,15 +419,37 @@ template hipError_t pcg_iteration<float>(const SpmvTune &, int, bool, const CsrD
 
 namespace ssf {
 
+// x64 += x32 / s, x32 = 0; block partials of ||x64||^2 into px
 __global__ __launch_bounds__(TPB) void k_mx_xupdate(int n, const MixState *ms,
                                                     double *__restrict__ x64,
-                                                    float *__restrict__ x32)
+                                                    float *__restrict__ x32,
+                                                    double *__restrict__ px)
 {
   if (ms->final) return;
-  const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
-  if (i >= n) return;
-  x64[i] += (double)x32[i] / ms->scale;
-  x32[i] = 0.0f;
+  const double inv = 1.0 / ms->scale;
+  int r0, r1;
+  chunk(n, r0, r1);
+  double xx = 0.0;
+  for (int i = r0 + threadIdx.x; i < r1; i += TPB) {
+    const double xi = x64[i] + (double)x32[i] * inv;
+    x64[i] = xi;
+    x32[i] = 0.0f;
+    xx += xi * xi;
+  }
+  const double t = block_sum(xx);
+  if (threadIdx.x == 0) px[blockIdx.x] = t;
+}
+
+// The computed residual fl(b - A x) carries a rounding error of at most
+// m eps (|b| + |A||x|) per row for m entries per row, typically sqrt(m) eps; in
+// 2-norm about MIX_FLOOR (||b|| + ||A||_inf ||x||) with MIX_FLOOR = 64 eps. A
+// solve is only declared stuck at the FP64 floor below that level.
+constexpr double MIX_FLOOR = 64.0 * 2.220446049250313e-16;
+
+__device__ __forceinline__ bool below_floor(double rr64, double nory64, double norm_a, double xx)
+{
+  const double f = MIX_FLOOR * (sqrt(nory64) + norm_a * sqrt(xx));
+  return rr64 < f * f;
 }
 
 __device__ __forceinline__ double refresh_scale(double rr64, double old)
@@ -474,11 +496,12 @@ __global__ __launch_bounds__(TPB) void k_mx_refresh(int n, const MixState *ms,
 __global__ __launch_bounds__(TPB) void k_mx_check(MixState *ms, PcgState<float> *st,
                                                   const double *__restrict__ part64, int nb64,
                                                   const float *__restrict__ prr, int nv,
-                                                  double res)
+                                                  const double *__restrict__ px, double res)
 {
   if (ms->final) return;
   const double rr64 = reduce_partials(part64, nb64);
   const float nomn = reduce_partials(prr + nv, nv);
+  const double xx = reduce_partials(px, nv);
   if (threadIdx.x != 0) return;
   const double prev = ms->rr64;
   ms->rr64 = rr64;
@@ -489,15 +512,18 @@ __global__ __launch_bounds__(TPB) void k_mx_check(MixState *ms, PcgState<float> 
     st->done = 1;
     return;
   }
-  // FP64 accuracy floor above res: the FP32 recurrence met its reduction three
-  // refreshes in a row while the FP64 residual did not even halve, or the FP64
-  // residual has not improved by 10 % in ten refreshes
+  // FP64 accuracy floor above res: the residual is within the rounding floor
+  // of its own computation, and either the FP32 recurrence met its reduction
+  // three refreshes in a row while the FP64 residual did not even halve, or the
+  // FP64 residual has not improved by 10 % in ten refreshes. (Far above that
+  // floor the residual norm of CG may stall or grow for hundreds of iterations
+  // on ill-conditioned systems while the error keeps falling.)
   const bool met = st->done && st->status == 1;
   ms->bad = (met && rr64 > 0.25 * prev) ? ms->bad + 1 : 0;
   if (rr64 < 0.81 * ms->best) ms->since_best = 0;
   else ms->since_best += 1;
   if (rr64 < ms->best) ms->best = rr64;
-  if (ms->bad >= 3 || ms->since_best >= 10) {
+  if (below_floor(rr64, ms->nory64, ms->norm_a, xx) && (ms->bad >= 3 || ms->since_best >= 10)) {
     ms->final = 1;
     ms->status = 4;
     return;
@@ -513,9 +539,10 @@ __global__ __launch_bounds__(TPB) void k_mx_check(MixState *ms, PcgState<float> 
 }
 
 __global__ void k_mx_init(MixState *ms, PcgState<float> *st, const double *__restrict__ slots,
-                          double res, double delta)
+                          const double *__restrict__ norm_a, double res, double delta)
 {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  ms->norm_a = *norm_a;
   ms->nory64 = slots[1];
   ms->rr64 = slots[0];
   ms->best = slots[0];
@@ -534,29 +561,77 @@ __global__ void k_mx_init(MixState *ms, PcgState<float> *st, const double *__res
   }
 }
 
-hipError_t mixed_init(MixState *ms, PcgState<float> *st, const double *slots, double res,
-                      double delta, hipStream_t s)
+hipError_t mixed_init(MixState *ms, PcgState<float> *st, const double *slots,
+                      const double *norm_a, double res, double delta, hipStream_t s)
 {
-  hipLaunchKernelGGL(k_mx_init, dim3(1), dim3(64), 0, s, ms, st, slots, res, delta);
+  hipLaunchKernelGGL(k_mx_init, dim3(1), dim3(64), 0, s, ms, st, slots, norm_a, res, delta);
   return hipGetLastError();
 }
 
 hipError_t mixed_update(const SpmvTune &t64, const SpmvTune &t32, const CsrDev &A,
                         const double *val64, const float *diag32, const double *b,
-                        double *x64, double *r64, double *part64, float *x32, float *r32,
-                        float *h32, float *d32, float *part32, PcgState<float> *st,
+                        double *x64, double *r64, double *part64, double *px, float *x32,
+                        float *r32, float *h32, float *d32, float *part32, PcgState<float> *st,
                         MixState *ms, double res, hipStream_t s)
 {
   const int n = A.n;
-  hipLaunchKernelGGL(k_mx_xupdate, dim3((n + TPB - 1) / TPB), dim3(TPB), 0, s, n, ms, x64, x32);
+  const int nv = vec_blocks(t32, n);
+  hipLaunchKernelGGL(k_mx_xupdate, dim3(nv), dim3(TPB), 0, s, n, ms, x64, x32, px);
   hipError_t e = spmv<double>(t64, 2, A, val64, x64, r64, b, part64, &ms->final, nullptr, s);
   if (e != hipSuccess) return e;
-  const int nv = vec_blocks(t32, n);
   float *prr = part32 + t32.blocks;
   hipLaunchKernelGGL(k_mx_refresh, dim3(nv), dim3(TPB), 0, s, n, ms, part64, t64.blocks, r64,
                      diag32, r32, h32, d32, prr);
   hipLaunchKernelGGL(k_mx_check, dim3(1), dim3(TPB), 0, s, ms, st, part64, t64.blocks, prr, nv,
-                     res);
+                     px, res);
+  return hipGetLastError();
+}
+
+// ||A||_inf of a CSR matrix: per-block maxima of the absolute row sums into
+// partial[0, gridDim), then their maximum into *out (order-independent)
+__global__ __launch_bounds__(TPB) void k_row_absmax(int n, const int *__restrict__ rowptr,
+                                                    const double *__restrict__ val,
+                                                    double *__restrict__ partial)
+{
+  int r0, r1;
+  chunk(n, r0, r1);
+  double m = 0.0;
+  for (int i = r0 + threadIdx.x; i < r1; i += TPB) {
+    double s = 0.0;
+    for (int k = rowptr[i]; k < rowptr[i + 1]; k++) s += fabs(val[k]);
+    m = fmax(m, s);
+  }
+  __shared__ double lds[TPB];
+  lds[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = TPB / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) lds[threadIdx.x] = fmax(lds[threadIdx.x], lds[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = lds[0];
+}
+
+__global__ __launch_bounds__(TPB) void k_max_partials(const double *__restrict__ partial, int nb,
+                                                      double *__restrict__ out)
+{
+  __shared__ double lds[TPB];
+  double m = 0.0;
+  for (int i = threadIdx.x; i < nb; i += TPB) m = fmax(m, partial[i]);
+  lds[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = TPB / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) lds[threadIdx.x] = fmax(lds[threadIdx.x], lds[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = lds[0];
+}
+
+hipError_t row_absmax(int n, const int *rowptr, const double *val, double *partial, int nb,
+                      double *out, hipStream_t s)
+{
+  if (n <= 0) return hipMemsetAsync(out, 0, sizeof(double), s);
+  hipLaunchKernelGGL(k_row_absmax, dim3(nb), dim3(TPB), 0, s, n, rowptr, val, partial);
+  hipLaunchKernelGGL(k_max_partials, dim3(1), dim3(TPB), 0, s, partial, nb, out);
   return hipGetLastError();
 }
 
