@@ -178,6 +178,40 @@ int ssf_batch_solve(ssf_batch *b, const ssf_solve_params *p, const double *rhs, 
  * and keep the fastest; lanes_out (may be NULL) receives it */
 int ssf_batch_autotune(ssf_batch *b, int precision, int reps, int32_t *lanes_out);
 
+/* ------------------------------------------------------------------ serving */
+/* A request loop for element-mesh solves on one GPU: requests are admitted
+ * under an HBM budget and solved with continuous batching (requests join and
+ * leave the resident set between iteration chunks). Each request runs what a
+ * SIFEL problem does for one load case: gmatrix::initiate (pattern,
+ * GEFEL/gmatrix.cpp:268), stiffness_matrix / conductivity_matrix assembly
+ * (MEFEL/SRC/globmat.cpp:41, TRFEL/SRC/globmatt.cpp:19) with the Dirichlet lift,
+ * and gmatrix::solve_system with symcomprow::cg_prec (GEFEL/scr.cpp:1093, Jacobi,
+ * FP64). Its result does not depend on the other requests in flight. Matrices
+ * and solver state stay on the device until the request is finished. */
+typedef struct ssf_server ssf_server;
+/* p: cg + diagprec, precision FP64, iv 0 (ni, res, zero used); chunk: iterations
+ * between admissions (0 = 32); max_slots: resident requests at most (0 = 1024);
+ * prep_threads: host threads building and assembling requests (0 = 2). */
+int ssf_server_create(int device, int64_t budget_bytes, const ssf_solve_params *p,
+                      int32_t chunk, int32_t max_slots, int32_t prep_threads,
+                      ssf_server **out);
+/* stops accepting, finishes every request in flight, releases everything */
+int ssf_server_destroy(ssf_server *s);
+/* one request, arguments as ssf_matrix_assemble plus the number of unknowns n
+ * and the load vector rhs[n] (the lift of prescribed values is added); the
+ * arrays are copied. ticket receives the request's id. */
+int ssf_server_submit(ssf_server *s, int kind, int64_t n, int64_t nn, const double *xyz,
+                      int64_t ne, const int64_t *conn, const int64_t *cn, const double *mat_a,
+                      const double *mat_b, const double *thick, const double *pv, int64_t npv,
+                      const double *rhs, int64_t *ticket);
+/* blocks until the request is finished; x[n] and info (either may be NULL) get
+ * its result; times[4] (may be NULL): submitted, admitted under the budget,
+ * assembled, finished, in seconds since the server was created */
+int ssf_server_wait(ssf_server *s, int64_t ticket, double *x, ssf_solve_info *info,
+                    double *times);
+/* stats[4]: finished requests, loop steps, peak reserved bytes, reserved bytes */
+int ssf_server_stats(ssf_server *s, int64_t *stats);
+
 /* ------------------------------------------------------------ diagnostics */
 /* Time `reps` launches of the FP64 (precision 0) or FP32 (1) SpMV on device
  * vectors; returns the mean kernel time in microseconds measured with HIP

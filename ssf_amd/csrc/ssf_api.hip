@@ -59,9 +59,8 @@ static void dfree(T *&p)
 // GEFEL/cr.cpp:140-320): for every row ii the ascending, duplicate-free set of
 // jj over all element pairs with both code numbers > 0 (and jj <= ii for the
 // symmetric lower storage). Built per row from a DOF -> element incidence list.
-static int pattern_from_elements(int storage, int64_t n, int64_t ne, int64_t ndofe,
-                                 const int64_t *cn, std::vector<int64_t> &adr,
-                                 std::vector<int64_t> &ci)
+int pattern_from_elements(int storage, int64_t n, int64_t ne, int64_t ndofe, const int64_t *cn,
+                          std::vector<int64_t> &adr, std::vector<int64_t> &ci)
 {
   const bool lower = storage == SSF_STORAGE_SYMM_COMP_ROWS;
   std::vector<int64_t> deg(n + 1, 0);
@@ -268,6 +267,26 @@ extern "C" int ssf_matrix_destroy(ssf_matrix *m)
   m->ctx->live_matrices--;
   delete m;
   return SSF_OK;
+}
+
+void ssf::matrix_drop_assembly(ssf_matrix *m, bool host_pattern)
+{
+  (void)hipStreamSynchronize(m->ctx->stream);
+  dfree(m->slot);
+  dfree(m->order);
+  m->asm_ne = 0;
+  m->asm_ndofe = 0;
+  m->colour_start.clear();
+  m->asm_cn_host.clear();
+  m->asm_cn_host.shrink_to_fit();
+  if (host_pattern) {  // the matrix can then only be solved and multiplied
+    std::vector<int64_t>().swap(m->adr);
+    std::vector<int64_t>().swap(m->ci);
+    if (m->a_ref == m->val) m->a_ref = nullptr;
+    dfree(m->a_ref);
+    dfree(m->map_lo);
+    dfree(m->map_up);
+  }
 }
 
 extern "C" int ssf_matrix_create_csr(ssf_ctx *ctx, int storage, int64_t n, const int64_t *adr,

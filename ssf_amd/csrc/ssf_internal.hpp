@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -112,7 +113,9 @@ struct ssf_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   int cus = 256;
-  int live_matrices = 0;  // a context outlives every matrix created on it
+  // a context outlives every matrix created on it; matrices of one context may
+  // be created and destroyed on different host threads (served solves)
+  std::atomic<int> live_matrices{0};
   // kernel sampling inside solves: the first SpMV of every iteration chunk is
   // bracketed by HIP events on the solve stream; [0] FP64, [1] FP32 operator
   bool prof = false;
@@ -167,3 +170,12 @@ struct ssf_matrix {
   }
   ssf::Workspace ws;
 };
+
+namespace ssf {
+// symbolic pattern of symcomprow / comprow from element code numbers (host)
+int pattern_from_elements(int storage, int64_t n, int64_t ne, int64_t ndofe, const int64_t *cn,
+                          std::vector<int64_t> &adr, std::vector<int64_t> &ci);
+// releases the element-assembly structures (slot map, colour order) of a matrix
+// and, if host_pattern, its host copy of the reference-layout pattern
+void matrix_drop_assembly(ssf_matrix *m, bool host_pattern);
+}  // namespace ssf

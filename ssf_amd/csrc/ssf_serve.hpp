@@ -1,0 +1,48 @@
+// Served solves on gfx950: device views of the resident requests ("slots").
+#pragma once
+
+#include "ssf_device.hpp"
+
+namespace ssf {
+
+// One resident request: its own matrix (full CSR of the gmatrix) and its own
+// solve vectors, wherever they were allocated. Kernels reach them through this
+// descriptor, so admitting or retiring a request moves no matrix data.
+struct SlotDesc {
+  const int *rowptr;
+  const int *col;
+  const double *val;
+  const double *diag;
+  const double *y;
+  double *x, *r, *d, *p, *h;
+  int n;
+  int sid;         // index of the slot's scalars in SlotState (stable while resident)
+  int blk0, nblk;  // the slot's blocks in the block table
+  int fresh;       // admitted since the last step: initialise before iterating
+};
+
+// Per-slot scalars of the cg_prec recurrence (GEFEL/scr.cpp:1093-1203).
+struct SlotState {
+  double *nory, *nom, *ares;  // nom: 2 per slot (iteration parity)
+  long long *iter, *ani;
+  int *done, *status, *pend, *pend_status, *brk;
+};
+
+struct ServeDev {
+  int nslot = 0, nblk = 0;
+  const SlotDesc *slots = nullptr;
+  const int *blk_slot = nullptr, *blk_r0 = nullptr, *blk_r1 = nullptr;  // local rows
+  double *part = nullptr;  // 3 * nblk
+};
+
+// rows of a slot per block: the partition depends only on the slot's size
+constexpr int SERVE_RPB = 1024;
+
+hipError_t serve_init(const ServeDev &S, SlotState st, double zero, hipStream_t s);
+// ni: a slot stops with status 0 after its ni-th iteration
+hipError_t serve_iteration(const ServeDev &S, SlotState st, double res, double zero,
+                           long long ni, hipStream_t s);
+// applies pending state; counts unfinished slots into *active (zeroed first)
+hipError_t serve_flags(const ServeDev &S, SlotState st, int *active, hipStream_t s);
+
+}  // namespace ssf

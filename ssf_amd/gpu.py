@@ -79,6 +79,13 @@ SIGNATURES = {
     "ssf_batch_solve": (_I, [_P, ctypes.POINTER(SolveParams), _P, _P, _P]),
     "ssf_batch_solve_dev": (_I, [_P, ctypes.POINTER(SolveParams), _P, _P, _P]),
     "ssf_batch_autotune": (_I, [_P, _I, _I, _P]),
+    "ssf_server_create": (_I, [_I, _L, ctypes.POINTER(SolveParams), ctypes.c_int32, ctypes.c_int32,
+                               ctypes.c_int32, ctypes.POINTER(_P)]),
+    "ssf_server_destroy": (_I, [_P]),
+    "ssf_server_submit": (_I, [_P, _I, _L, _L, _P, _L, _P, _P, _P, _P, _P, _P, _L, _P,
+                               ctypes.POINTER(_L)]),
+    "ssf_server_wait": (_I, [_P, _L, _P, ctypes.POINTER(SolveInfo), _P]),
+    "ssf_server_stats": (_I, [_P, _P]),
 }
 
 _lib = None
@@ -358,6 +365,63 @@ class Batch:
         out = ctypes.c_int32()
         _check(lib().ssf_batch_autotune(self.h, precision, reps, ctypes.byref(out)))
         return out.value
+
+
+class Server:
+    """Request loop on one GPU: element-mesh solves admitted under an HBM budget
+    and solved with continuous batching (FP64 Jacobi-PCG per request)."""
+
+    def __init__(self, device=0, budget_bytes=200 << 30, ni=100000, res=1e-20, zero=1e-30,
+                 chunk=32, max_slots=1024, prep_threads=2):
+        p = SolveParams(SOLVER_CG, PREC_DIAG, ni, res, zero, 0, FP64, 0.0, 0, 0)
+        self.h = _P()
+        _check(lib().ssf_server_create(device, int(budget_bytes), ctypes.byref(p), chunk,
+                                       max_slots, prep_threads, ctypes.byref(self.h)))
+        self._n = {}
+
+    def submit(self, prob: mesh.Problem, rhs=None) -> int:
+        """Queue one solve of prob (load case rhs, default prob.rhs[0]); returns a ticket."""
+        y = _f64(prob.rhs[0] if rhs is None else rhs)
+        cn = _i64(prob.elem_codes())
+        conn = _i64(prob.conn)
+        xyz = _f64(prob.coords)
+        e = _f64(prob.e)
+        nu = _f64(prob.nu) if prob.kind != mesh.KIND_HEX8_HEAT else None
+        th = _f64(prob.thick) if prob.kind == mesh.KIND_QUAD4_PSTRESS else None
+        pv = _f64(prob.pv) if prob.pv.size else None
+        t = _L()
+        _check(lib().ssf_server_submit(self.h, prob.kind, prob.n, xyz.shape[0], _p(xyz),
+                                       conn.shape[0], _p(conn), _p(cn), _p(e), _p(nu), _p(th),
+                                       _p(pv), 0 if pv is None else pv.shape[0], _p(y),
+                                       ctypes.byref(t)))
+        self._n[t.value] = prob.n
+        return t.value
+
+    def wait(self, ticket: int):
+        """Returns (x, info, times[submitted, admitted, assembled, finished])."""
+        n = self._n.pop(ticket)
+        x = np.zeros(n)
+        info = SolveInfo(-1, -1.0, 0, 0, 0, 0, 0.0)
+        times = np.zeros(4)
+        _check(lib().ssf_server_wait(self.h, ticket, _p(x), ctypes.byref(info), _p(times)))
+        return x, info, times
+
+    def stats(self):
+        s = np.zeros(4, np.int64)
+        _check(lib().ssf_server_stats(self.h, _p(s)))
+        return {"finished": int(s[0]), "steps": int(s[1]), "peak_bytes": int(s[2]),
+                "reserved_bytes": int(s[3])}
+
+    def close(self):
+        if self.h:
+            lib().ssf_server_destroy(self.h)
+            self.h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def build_problem_matrix(ctx: Context, prob: mesh.Problem, storage=STORAGE_SCR):
