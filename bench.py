@@ -131,6 +131,68 @@ def bench_batched(ctx, args, gpu, mesh, torch):
     }
 
 
+def bench_served(ctx, args, gpu, mesh, torch):
+    """Served solves: a Poisson stream of requests (seed 5), half hex8 elasticity
+    columns and half heat cubes of 5e3..2e6 DOF (log-uniform), admitted under a
+    200 GB HBM budget. Latency = finished - submitted per request (server clock).
+    Compared with the drop-in path, which builds, assembles and solves the same
+    requests one after the other with the FP64 GPU PCG."""
+    rng = np.random.default_rng(5)
+    t0 = time.time()
+    probs = [mesh.served_request(rng) for _ in range(args.serve_count)]
+    ndof = [p.n for p in probs]
+    log(f"[serve] {len(probs)} requests, {sum(ndof)} DOF in total (max {max(ndof)}), "
+        f"generated in {time.time() - t0:.1f}s")
+    out = {"workload": f"{len(probs)} requests (seed 5): hex8 elasticity / heat, "
+                       f"5e3..2e6 DOF, {sum(ndof)} DOF in total",
+           "res": args.serve_res, "budget_bytes": 200 << 30, "rates": {}}
+    for rate in args.serve_rates:
+        arr = np.cumsum(np.random.default_rng(7).exponential(1.0 / rate, len(probs)))
+        srv = gpu.Server(budget_bytes=200 << 30, res=args.serve_res, chunk=32, prep_threads=4)
+        try:
+            tickets = []
+            start = time.perf_counter()
+            for p, ta in zip(probs, arr):
+                dt = ta - (time.perf_counter() - start)
+                if dt > 0:
+                    time.sleep(dt)
+                tickets.append(srv.submit(p))
+            res = [srv.wait(t) for t in tickets]
+            st = srv.stats()
+        finally:
+            srv.close()
+        times = np.array([r[2] for r in res])
+        lat = times[:, 3] - times[:, 0]
+        span = times[:, 3].max() - times[:, 0].min()
+        conv = sum(1 for r in res if r[1].status == 1)
+        out["rates"][str(rate)] = {
+            "offered_req_per_s": rate, "solves_per_s": len(probs) / span,
+            "p50_latency_s": float(np.percentile(lat, 50)),
+            "p99_latency_s": float(np.percentile(lat, 99)),
+            "mean_queue_s": float(np.mean(times[:, 1] - times[:, 0])),
+            "converged": conv, "peak_bytes": st["peak_bytes"], "steps": st["steps"]}
+        log(f"[serve] rate {rate}/s: {out['rates'][str(rate)]}")
+        xs = [r[0] for r in res]
+    # the drop-in path, one request after the other: pattern, assembly, FP64 solve
+    t = time.perf_counter()
+    seq_x = []
+    for p in probs:
+        m, rhs = gpu.build_problem_matrix(ctx, p)
+        x, info = m.solve(rhs[0], ni=100000, res=args.serve_res)
+        seq_x.append(x)
+        m.close()
+    dt = time.perf_counter() - t
+    out["sequential_fp64_solves_per_s"] = len(probs) / dt
+    # the served results against the drop-in ones (same recurrence, FP64)
+    out["max_rel_diff_vs_sequential"] = float(max(
+        np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(xs, seq_x)))
+    best = max(v["solves_per_s"] for v in out["rates"].values())
+    out["speedup_served_vs_sequential_fp64"] = best / out["sequential_fp64_solves_per_s"]
+    log(f"[serve] sequential drop-in {out['sequential_fp64_solves_per_s']:.2f} solves/s, "
+        f"served best {best:.2f}; max rel diff {out['max_rel_diff_vs_sequential']:.1e}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -147,6 +209,11 @@ def main():
     ap.add_argument("--batch-res", type=float, default=1e-20)
     ap.add_argument("--seq-stride", type=int, default=16)
     ap.add_argument("--only-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served leg")
+    ap.add_argument("--only-serve", action="store_true")
+    ap.add_argument("--serve-count", type=int, default=24)
+    ap.add_argument("--serve-res", type=float, default=1e-22)
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[20.0])
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; the path shards by independent "
@@ -159,6 +226,10 @@ def main():
     if args.only_batch:
         ctx = gpu.Context(0)
         print(json.dumps(bench_batched(ctx, args, gpu, mesh, torch)), flush=True)
+        return
+    if args.only_serve:
+        ctx = gpu.Context(0)
+        print(json.dumps(bench_served(ctx, args, gpu, mesh, torch)), flush=True)
         return
     t0 = time.time()
     prob = mesh.heat_cube(args.nel, seed=2)
@@ -263,6 +334,9 @@ def main():
         del b, x, xref
         torch.cuda.empty_cache()
         batched = bench_batched(ctx, args, gpu, mesh, torch)
+    served = None
+    if not args.no_serve:
+        served = bench_served(ctx, args, gpu, mesh, torch)
 
     mixed = results["mixed"]
     fp64 = results["fp64"]
@@ -295,6 +369,7 @@ def main():
                      "usec_per_launch": us32, "samples": mixed["spmv_samples"]},
         "cpu_baseline": cpu,
         "batched": batched,
+        "served": served,
     }
     print(json.dumps(line), flush=True)
 

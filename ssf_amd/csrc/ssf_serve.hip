@@ -2,13 +2,14 @@
 // resident request ("slot") by one iteration of the cg_prec recurrence
 // (GEFEL/scr.cpp:1142-1197), each slot through its own descriptor.
 //
-// Block b of a launch works on rows [blk_r0[b], blk_r1[b]) of slot blk_slot[b];
-// a slot's blocks are contiguous in the table and its rows are split in chunks
-// of SERVE_RPB, so its partition, and with it every summation order, depends on
-// its own size only: a request's result is the same alone and among others.
-// Scalars of a slot are re-reduced from its blocks' partials by each of its
-// blocks (no atomics, no inter-block waits); state changes decided inside a
-// kernel are left pending and applied by k_sv_advance.
+// Block b of a launch works on the stream tiles [blk_t0[b], blk_t1[b]) of slot
+// blk_slot[b] (whole rows, at most STREAM_CAP entries per tile, the tiling of the
+// slot's own matrix). Every block leaves its partial sums; a small kernel with
+// one block per slot then sums a slot's partials in block order into the
+// slot's scalars (d.p, r.r, r.h), read by the next kernel. The summation order
+// depends on the slot's own tiling only, so a request's result is the same alone
+// and among others. State changes decided inside a kernel are left pending and
+// applied by k_sv_advance.
 #include "ssf_serve.hpp"
 
 namespace ssf {
@@ -16,47 +17,102 @@ namespace ssf {
 constexpr int SV_ADVANCE = 1;
 constexpr int SV_STOP = 2;
 
-// sum of part[b0 + i], i < nb, by the whole block: thread t adds i = t, t + TPB,
-// ..., then the fixed block_sum tree (depends on nb only)
-__device__ __forceinline__ double slot_sum(const double *__restrict__ part, int b0, int nb)
+// block partials (a, b) of the running kernel into part[2 * block]
+__device__ __forceinline__ void block_partials2(double *__restrict__ part, double a, double b)
 {
-  double t = 0.0;
-  for (int i = threadIdx.x; i < nb; i += TPB) t += part[b0 + i];
-  return block_sum(t);
+  const double ta = block_sum(a);
+  const double tb = block_sum(b);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = ta;
+    part[2 * blockIdx.x + 1] = tb;
+  }
 }
 
-// p = A d for the slot's rows, partial d.p
-template <int G>
+// One block per slot: the slot's sums of the partials of the previous kernel,
+// in block order, into out = st.sc + 3 sid + off (count values). Slots that the
+// previous kernel skipped are skipped here too (done, or not fresh for init).
+__global__ __launch_bounds__(TPB) void k_sv_reduce(ServeDev S, SlotState st, int off, int count,
+                                                   int fresh_only)
+{
+  const SlotDesc D = S.slots[blockIdx.x];
+  const int sid = D.sid;
+  if (fresh_only ? !D.fresh : st.done[sid]) return;
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < D.nblk; i += TPB) {
+    a += S.part[2 * (D.blk0 + i)];
+    b += S.part[2 * (D.blk0 + i) + 1];
+  }
+  a = block_sum(a);
+  b = block_sum(b);
+  if (threadIdx.x == 0) {
+    st.sc[3 * sid + off] = a;
+    if (count > 1) st.sc[3 * sid + off + 1] = b;
+  }
+}
+
+// p = A d over the block's tiles: coalesced loads of the tile's entries into
+// registers (the next tile's while this one is summed), products with the
+// gathered d into LDS, then `lanes` lanes per row sum their row in a fixed order
 __global__ __launch_bounds__(TPB) void k_sv_spmv(ServeDev S, SlotState st)
 {
+  __shared__ double prod[STREAM_CAP];
   const int blk = blockIdx.x;
   const SlotDesc D = S.slots[S.blk_slot[blk]];
   const int sid = D.sid;
   if (st.done[sid]) return;
-  const int r0 = S.blk_r0[blk], r1 = S.blk_r1[blk];
-  const int grp = threadIdx.x / G, lane = threadIdx.x % G;
-  constexpr int NG = TPB / G;
-  double acc = 0.0;
-  for (int row = r0 + grp; row < r1; row += NG) {
-    const double s = row_dot<double, G>(D.rowptr, D.col, D.val, D.d, row, lane);
-    if (lane == 0) {
-      D.p[row] = s;
-      acc += D.d[row] * s;
+  const int t0 = S.blk_t0[blk], t1 = S.blk_t1[blk];
+  const int g = D.lanes;
+  const int grp = threadIdx.x / g, lane = threadIdx.x % g, ng = TPB / g;
+  double v[STREAM_KPT];
+  int c[STREAM_KPT];
+  auto load_tile = [&](int tile) {
+    const int s = D.rowptr[D.tiles[tile]], e = D.rowptr[D.tiles[tile + 1]];
+#pragma unroll
+    for (int j = 0; j < STREAM_KPT; j++) {
+      const int k = s + j * TPB + threadIdx.x;
+      if (k < e) {
+        v[j] = ldnt(D.val + k);
+        c[j] = ldnt(D.col + k);
+      } else {
+        v[j] = 0.0;
+        c[j] = -1;
+      }
     }
+  };
+  double acc = 0.0;
+  if (t0 < t1) load_tile(t0);
+  for (int tile = t0; tile < t1; tile++) {
+    const int r0 = D.tiles[tile], r1 = D.tiles[tile + 1];
+    const int s = D.rowptr[r0];
+#pragma unroll
+    for (int j = 0; j < STREAM_KPT; j++)
+      if (c[j] >= 0) prod[j * TPB + threadIdx.x] = v[j] * D.d[c[j]];
+    if (tile + 1 < t1) load_tile(tile + 1);
+    __syncthreads();
+    for (int row = r0 + grp; row < r1; row += ng) {
+      const int k0 = D.rowptr[row] - s, k1 = D.rowptr[row + 1] - s;
+      double sum = 0.0;
+      for (int k = k0 + lane; k < k1; k += g) sum += prod[k];
+      for (int off = g / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, g);
+      if (lane == 0) {
+        D.p[row] = sum;
+        acc += D.d[row] * sum;
+      }
+    }
+    __syncthreads();
   }
-  const double t = block_sum(acc);
-  if (threadIdx.x == 0) S.part[blk] = t;
+  block_partials2(S.part, acc, 0.0);
 }
 
 // den = d.p; |den| < zero stops the slot (status 2); x += a d, r -= a p,
-// h = r / diag; partials r.r and r.h
+// h = r / diag; slot sums r.r and r.h
 __global__ __launch_bounds__(TPB) void k_sv_update(ServeDev S, SlotState st, double zero)
 {
   const int blk = blockIdx.x;
   const SlotDesc D = S.slots[S.blk_slot[blk]];
   const int sid = D.sid;
   if (st.done[sid]) return;
-  const double den = slot_sum(S.part, D.blk0, D.nblk);
+  const double den = st.sc[3 * sid];
   const double nom = st.nom[2 * sid + (st.iter[sid] & 1)];
   double a;
   if (fabs(den) < zero) {  // GEFEL/scr.cpp:1148-1152
@@ -69,7 +125,7 @@ __global__ __launch_bounds__(TPB) void k_sv_update(ServeDev S, SlotState st, dou
   } else {
     a = nom / den;
   }
-  const int r0 = S.blk_r0[blk], r1 = S.blk_r1[blk];
+  const int r0 = D.tiles[S.blk_t0[blk]], r1 = D.tiles[S.blk_t1[blk]];
   double srr = 0.0, srh = 0.0;
   for (int i = r0 + threadIdx.x; i < r1; i += TPB) {
     if (a != 0.0) {
@@ -82,12 +138,7 @@ __global__ __launch_bounds__(TPB) void k_sv_update(ServeDev S, SlotState st, dou
     srr += ri * ri;
     srh += ri * hi;
   }
-  const double trr = block_sum(srr);
-  const double trh = block_sum(srh);
-  if (threadIdx.x == 0) {
-    S.part[S.nblk + blk] = trr;
-    S.part[2 * S.nblk + blk] = trh;
-  }
+  block_partials2(S.part, srr, srh);
 }
 
 // stopping test |r.r / nory| < res (GEFEL/scr.cpp:1188), else d = h + b d; the
@@ -99,8 +150,8 @@ __global__ __launch_bounds__(TPB) void k_sv_dir(ServeDev S, SlotState st, double
   const SlotDesc D = S.slots[S.blk_slot[blk]];
   const int sid = D.sid;
   if (st.done[sid] || st.brk[sid]) return;
-  const double rr = slot_sum(S.part + S.nblk, D.blk0, D.nblk);
-  const double nomn = slot_sum(S.part + 2 * S.nblk, D.blk0, D.nblk);
+  const double rr = st.sc[3 * sid + 1];
+  const double nomn = st.sc[3 * sid + 2];
   const long long it = st.iter[sid];
   const double nom = st.nom[2 * sid + (it & 1)];
   const bool stop = fabs(rr / st.nory[sid]) < res;
@@ -120,7 +171,7 @@ __global__ __launch_bounds__(TPB) void k_sv_dir(ServeDev S, SlotState st, double
   }
   if (stop) return;
   const double bt = nomn / nom;
-  const int r0 = S.blk_r0[blk], r1 = S.blk_r1[blk];
+  const int r0 = D.tiles[S.blk_t0[blk]], r1 = D.tiles[S.blk_t1[blk]];
   for (int i = r0 + threadIdx.x; i < r1; i += TPB) D.d[i] = bt * D.d[i] + D.h[i];
 }
 
@@ -148,13 +199,13 @@ __global__ void k_sv_advance(ServeDev S, SlotState st, int *active)
   if (active && !st.done[sid]) atomicAdd(active, 1);
 }
 
-// fresh slots: x = 0, r = y, h = r / diag, d = h; partials y.y and r.h
-__global__ __launch_bounds__(TPB) void k_sv_init(ServeDev S)
+// fresh slots: x = 0, r = y, h = r / diag, d = h; slot sums y.y and r.h
+__global__ __launch_bounds__(TPB) void k_sv_init(ServeDev S, SlotState st)
 {
   const int blk = blockIdx.x;
   const SlotDesc D = S.slots[S.blk_slot[blk]];
   if (!D.fresh) return;
-  const int r0 = S.blk_r0[blk], r1 = S.blk_r1[blk];
+  const int r0 = D.tiles[S.blk_t0[blk]], r1 = D.tiles[S.blk_t1[blk]];
   double syy = 0.0, srh = 0.0;
   for (int i = r0 + threadIdx.x; i < r1; i += TPB) {
     const double yi = D.y[i];
@@ -166,26 +217,19 @@ __global__ __launch_bounds__(TPB) void k_sv_init(ServeDev S)
     syy += yi * yi;
     srh += yi * hi;
   }
-  const double a = block_sum(syy);
-  const double b = block_sum(srh);
-  if (threadIdx.x == 0) {
-    S.part[S.nblk + blk] = a;
-    S.part[2 * S.nblk + blk] = b;
-  }
+  block_partials2(S.part, syy, srh);
 }
 
-// one block per slot: initial scalars of the fresh slots; the fresh mark is
+// one thread per slot: initial scalars of the fresh slots; the fresh mark is
 // cleared so the iteration kernels take the slot from now on
-__global__ __launch_bounds__(TPB) void k_sv_init2(ServeDev S, SlotDesc *slots, SlotState st,
-                                                  double zero)
+__global__ void k_sv_init2(ServeDev S, SlotDesc *slots, SlotState st, double zero)
 {
-  const int k = blockIdx.x;
-  const SlotDesc D = slots[k];
-  if (!D.fresh) return;
-  const double nory = slot_sum(S.part + S.nblk, D.blk0, D.nblk);
-  const double nom = slot_sum(S.part + 2 * S.nblk, D.blk0, D.nblk);
-  if (threadIdx.x != 0) return;
-  const int sid = D.sid;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= S.nslot) return;
+  if (!slots[k].fresh) return;
+  const int sid = slots[k].sid;
+  const double nory = st.sc[3 * sid + 1];
+  const double nom = st.sc[3 * sid + 2];
   st.nory[sid] = nory;
   st.nom[2 * sid] = nom;
   st.nom[2 * sid + 1] = nom;
@@ -208,8 +252,9 @@ __global__ __launch_bounds__(TPB) void k_sv_init2(ServeDev S, SlotDesc *slots, S
 hipError_t serve_init(const ServeDev &S, SlotState st, double zero, hipStream_t s)
 {
   if (S.nslot == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_sv_init, dim3(S.nblk), dim3(TPB), 0, s, S);
-  hipLaunchKernelGGL(k_sv_init2, dim3(S.nslot), dim3(TPB), 0, s, S,
+  hipLaunchKernelGGL(k_sv_init, dim3(S.nblk), dim3(TPB), 0, s, S, st);
+  hipLaunchKernelGGL(k_sv_reduce, dim3(S.nslot), dim3(TPB), 0, s, S, st, 1, 2, 1);
+  hipLaunchKernelGGL(k_sv_init2, dim3((S.nslot + TPB - 1) / TPB), dim3(TPB), 0, s, S,
                      const_cast<SlotDesc *>(S.slots), st, zero);
   return hipGetLastError();
 }
@@ -218,8 +263,10 @@ hipError_t serve_iteration(const ServeDev &S, SlotState st, double res, double z
                            long long ni, hipStream_t s)
 {
   if (S.nslot == 0) return hipSuccess;
-  hipLaunchKernelGGL((k_sv_spmv<8>), dim3(S.nblk), dim3(TPB), 0, s, S, st);
+  hipLaunchKernelGGL(k_sv_spmv, dim3(S.nblk), dim3(TPB), 0, s, S, st);
+  hipLaunchKernelGGL(k_sv_reduce, dim3(S.nslot), dim3(TPB), 0, s, S, st, 0, 1, 0);
   hipLaunchKernelGGL(k_sv_update, dim3(S.nblk), dim3(TPB), 0, s, S, st, zero);
+  hipLaunchKernelGGL(k_sv_reduce, dim3(S.nslot), dim3(TPB), 0, s, S, st, 1, 2, 0);
   hipLaunchKernelGGL(k_sv_dir, dim3(S.nblk), dim3(TPB), 0, s, S, st, res, ni);
   hipLaunchKernelGGL(k_sv_advance, dim3((S.nslot + TPB - 1) / TPB), dim3(TPB), 0, s, S, st,
                      nullptr);

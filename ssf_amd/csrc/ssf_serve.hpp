@@ -5,25 +5,29 @@
 
 namespace ssf {
 
-// One resident request: its own matrix (full CSR of the gmatrix) and its own
-// solve vectors, wherever they were allocated. Kernels reach them through this
-// descriptor, so admitting or retiring a request moves no matrix data.
+// One resident request: its own matrix (full CSR of the gmatrix, cut into stream
+// tiles of whole rows) and its own solve vectors, wherever they were allocated.
+// Kernels reach them through this descriptor, so admitting or retiring a
+// request moves no matrix data.
 struct SlotDesc {
   const int *rowptr;
   const int *col;
   const double *val;
   const double *diag;
+  const int *tiles;  // first row of each tile, ntiles + 1 entries
   const double *y;
   double *x, *r, *d, *p, *h;
   int n;
   int sid;         // index of the slot's scalars in SlotState (stable while resident)
   int blk0, nblk;  // the slot's blocks in the block table
+  int lanes;       // lanes per row of the SpMV row sums (4, 8 or 16)
   int fresh;       // admitted since the last step: initialise before iterating
 };
 
 // Per-slot scalars of the cg_prec recurrence (GEFEL/scr.cpp:1093-1203).
 struct SlotState {
   double *nory, *nom, *ares;  // nom: 2 per slot (iteration parity)
+  double *sc;                 // 3 per slot: d.p, r.r, r.h of the running iteration
   long long *iter, *ani;
   int *done, *status, *pend, *pend_status, *brk;
 };
@@ -31,12 +35,12 @@ struct SlotState {
 struct ServeDev {
   int nslot = 0, nblk = 0;
   const SlotDesc *slots = nullptr;
-  const int *blk_slot = nullptr, *blk_r0 = nullptr, *blk_r1 = nullptr;  // local rows
-  double *part = nullptr;  // 3 * nblk
+  const int *blk_slot = nullptr, *blk_t0 = nullptr, *blk_t1 = nullptr;  // local tiles
+  double *part = nullptr;  // 2 per block
 };
 
-// rows of a slot per block: the partition depends only on the slot's size
-constexpr int SERVE_RPB = 1024;
+// stream tiles of a slot per block: the partition depends only on the slot
+constexpr int SERVE_TILES_PER_BLOCK = 4;
 
 hipError_t serve_init(const ServeDev &S, SlotState st, double zero, hipStream_t s);
 // ni: a slot stops with status 0 after its ni-th iteration

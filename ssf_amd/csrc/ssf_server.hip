@@ -97,7 +97,7 @@ struct ssf_server {
   int max_slots = 0;  // resident requests at most (size of the per-slot state)
   int cap_slots = 0, cap_blk = 0;
   SlotDesc *d_slots = nullptr;
-  int *d_blk_slot = nullptr, *d_blk_r0 = nullptr, *d_blk_r1 = nullptr;
+  int *d_blk_slot = nullptr, *d_blk_t0 = nullptr, *d_blk_t1 = nullptr;
   double *d_part = nullptr;
   SlotState st{};
   int *d_active = nullptr;
@@ -175,6 +175,10 @@ void prep_main(ssf_server *S, int w)
     if (rc == SSF_OK) {
       rc = ssf_matrix_create_csr(ctx, SSF_STORAGE_SYMM_COMP_ROWS, q->n, adr.data(), ci.data(),
                                  &q->m);
+      if (rc == SSF_OK && q->m->ntiles == 0) {  // a row longer than one stream tile
+        set_error("ssf_server: matrix rows longer than the stream tile are not served");
+        rc = SSF_EUNSUP;
+      }
       std::vector<int64_t>().swap(adr);
       std::vector<int64_t>().swap(ci);
       std::vector<double> lift(q->n, 0.0);
@@ -254,15 +258,15 @@ hipError_t ensure_tables(ssf_server *S, int nslot, int nblk)
   }
   if (nblk > S->cap_blk) {
     sfree(S->d_blk_slot);
-    sfree(S->d_blk_r0);
-    sfree(S->d_blk_r1);
+    sfree(S->d_blk_t0);
+    sfree(S->d_blk_t1);
     sfree(S->d_part);
     int cap = std::max(1024, S->cap_blk);
     while (cap < nblk) cap *= 2;
     if ((e = salloc(&S->d_blk_slot, cap)) != hipSuccess) return e;
-    if ((e = salloc(&S->d_blk_r0, cap)) != hipSuccess) return e;
-    if ((e = salloc(&S->d_blk_r1, cap)) != hipSuccess) return e;
-    if ((e = salloc(&S->d_part, 3 * (size_t)cap)) != hipSuccess) return e;
+    if ((e = salloc(&S->d_blk_t0, cap)) != hipSuccess) return e;
+    if ((e = salloc(&S->d_blk_t1, cap)) != hipSuccess) return e;
+    if ((e = salloc(&S->d_part, 2 * (size_t)cap)) != hipSuccess) return e;
     S->cap_blk = cap;
   }
   return e;
@@ -287,17 +291,21 @@ hipError_t upload_tables(ssf_server *S, const std::vector<char> &fresh)
     D.d = q->vec + 3 * q->n;
     D.p = q->vec + 4 * q->n;
     D.h = q->vec + 5 * q->n;
+    D.tiles = m->tiles;
     D.n = (int)q->n;
     D.sid = q->sid;
     D.fresh = fresh[k];
+    // lanes per row from the mean row length (depends on the matrix only)
+    const double avg = q->n ? (double)m->nnz / (double)q->n : 1.0;
+    D.lanes = avg <= 32.0 ? 4 : (avg <= 96.0 ? 8 : 16);
     D.blk0 = (int)bs.size();
-    const int64_t nb = std::max<int64_t>(1, (q->n + SERVE_RPB - 1) / SERVE_RPB);
-    for (int64_t j = 0; j < nb; j++) {
+    const int nt = m->ntiles;
+    for (int t = 0; t < nt; t += SERVE_TILES_PER_BLOCK) {
       bs.push_back((int)k);
-      b0.push_back((int)(q->n * j / nb));
-      b1.push_back((int)(q->n * (j + 1) / nb));
+      b0.push_back(t);
+      b1.push_back(std::min(nt, t + SERVE_TILES_PER_BLOCK));
     }
-    D.nblk = (int)nb;
+    D.nblk = (int)bs.size() - D.blk0;
   }
   S->nblk = (int)bs.size();
   hipError_t e = ensure_tables(S, (int)desc.size(), S->nblk);
@@ -309,9 +317,9 @@ hipError_t upload_tables(ssf_server *S, const std::vector<char> &fresh)
     if (e == hipSuccess)
       e = hipMemcpyAsync(S->d_blk_slot, bs.data(), bs.size() * sizeof(int), hipMemcpyHostToDevice, s);
     if (e == hipSuccess)
-      e = hipMemcpyAsync(S->d_blk_r0, b0.data(), b0.size() * sizeof(int), hipMemcpyHostToDevice, s);
+      e = hipMemcpyAsync(S->d_blk_t0, b0.data(), b0.size() * sizeof(int), hipMemcpyHostToDevice, s);
     if (e == hipSuccess)
-      e = hipMemcpyAsync(S->d_blk_r1, b1.data(), b1.size() * sizeof(int), hipMemcpyHostToDevice, s);
+      e = hipMemcpyAsync(S->d_blk_t1, b1.data(), b1.size() * sizeof(int), hipMemcpyHostToDevice, s);
     // the host vectors go out of scope: wait for the copies
     if (e == hipSuccess) e = hipStreamSynchronize(s);
   }
@@ -325,8 +333,8 @@ ServeDev serve_view(ssf_server *S)
   v.nblk = S->nblk;
   v.slots = S->d_slots;
   v.blk_slot = S->d_blk_slot;
-  v.blk_r0 = S->d_blk_r0;
-  v.blk_r1 = S->d_blk_r1;
+  v.blk_t0 = S->d_blk_t0;
+  v.blk_t1 = S->d_blk_t1;
   v.part = S->d_part;
   return v;
 }
@@ -446,6 +454,7 @@ hipError_t alloc_state(ssf_server *S, int cap)
   auto c = [&](hipError_t x) { if (e == hipSuccess) e = x; };
   SlotState &t = S->st;
   c(salloc(&t.nory, cap)); c(salloc(&t.nom, 2 * (size_t)cap)); c(salloc(&t.ares, cap));
+  c(salloc(&t.sc, 3 * (size_t)cap));
   c(salloc(&t.iter, cap)); c(salloc(&t.ani, cap)); c(salloc(&t.done, cap));
   c(salloc(&t.status, cap)); c(salloc(&t.pend, cap)); c(salloc(&t.pend_status, cap));
   c(salloc(&t.brk, cap));
@@ -459,10 +468,11 @@ hipError_t alloc_state(ssf_server *S, int cap)
 void free_all(ssf_server *S)
 {
   SlotState &t = S->st;
-  sfree(t.nory); sfree(t.nom); sfree(t.ares); sfree(t.iter); sfree(t.ani); sfree(t.done);
+  sfree(t.nory); sfree(t.nom); sfree(t.ares); sfree(t.sc);
+  sfree(t.iter); sfree(t.ani); sfree(t.done);
   sfree(t.status); sfree(t.pend); sfree(t.pend_status); sfree(t.brk);
   sfree(S->d_active);
-  sfree(S->d_slots); sfree(S->d_blk_slot); sfree(S->d_blk_r0); sfree(S->d_blk_r1);
+  sfree(S->d_slots); sfree(S->d_blk_slot); sfree(S->d_blk_t0); sfree(S->d_blk_t1);
   sfree(S->d_part);
   if (S->h_done) (void)hipHostFree(S->h_done);
   S->h_done = nullptr;
