@@ -10,7 +10,7 @@
  * reference's x86-64 build (no FMA), each operation rounds on its own.
  * Integers are C `long` (64-bit), exactly as the reference's `long`.
  *
- * Pinning (what checks each function; see tests/test_oracle_*.py):
+ * Pinning (what checks each function; see tests/test_oracle.py):
  *   cr_codenum            gencodnum rules; tests vs closed-form numbering and flags
  *   cr_scr_pattern        tmatstore.tex known-answer arrays; bitwise vs oracle/_ref
  *   cr_cr_pattern         tmatstore.tex known-answer arrays; bitwise vs oracle/_ref
