@@ -120,10 +120,16 @@ int pattern_from_elements(int storage, int64_t n, int64_t ne, int64_t ndofe, con
 // Deterministic default: the stream kernel when the matrix has a tiling (lanes
 // per row from the mean row length, one to a few tiles per block), otherwise the
 // row-vector kernel.
-static SpmvTune default_tune(int64_t n, int64_t nnz, int ntiles, bool idx16)
+static SpmvTune default_tune(int64_t n, int64_t nnz, int ntiles, int nslice, bool idx16)
 {
   SpmvTune t;
   const double avg = n ? (double)nnz / (double)n : 1.0;
+  if (nslice > 0) {  // SELL-64: four slices per block pass, at most 2048 blocks
+    t.format = 2;
+    int64_t nb = std::max<int64_t>(8, std::min<int64_t>(2048, (nslice + 3) / 4));
+    t.blocks = (int)((nb + 7) / 8 * 8);
+    return t;
+  }
   if (ntiles > 0) {
     t.format = 1;
     t.idx16 = idx16 ? 1 : 0;
@@ -178,6 +184,48 @@ static bool build_tiles(int64_t n, const std::vector<int> &rowptr, const std::ve
     }
   }
   if (!fits) col16.clear();
+  return true;
+}
+
+// SELL-64 layout of a full CSR: slices of 64 rows, each row's entries in
+// column order, padded to the slice's longest row (rounded up to SELL_V) with
+// value 0 at the row's own column; src maps a SELL entry to its CSR entry.
+static bool build_sell(int64_t n, const std::vector<int> &rowptr, const std::vector<int> &col,
+                       std::vector<int> &off, std::vector<int> &scol, std::vector<int> &src)
+{
+  const int64_t nslice = (n + 63) / 64;
+  off.assign(nslice + 1, 0);
+  int64_t tot = 0;
+  for (int64_t sl = 0; sl < nslice; sl++) {
+    int64_t w = 0;
+    for (int64_t r = sl * 64; r < std::min<int64_t>(n, sl * 64 + 64); r++)
+      w = std::max<int64_t>(w, rowptr[r + 1] - rowptr[r]);
+    w = (w + SELL_V - 1) / SELL_V * SELL_V;
+    off[sl] = (int)tot;
+    tot += w * 64;
+    if (tot >= (1LL << 31)) return false;
+  }
+  off[nslice] = (int)tot;
+  scol.assign(tot, 0);
+  src.assign(tot, -1);
+  parallel_for(nslice, [&](int64_t b, int64_t e) {
+    for (int64_t sl = b; sl < e; sl++) {
+      const int64_t w = (off[sl + 1] - off[sl]) / 64;
+      for (int64_t l = 0; l < 64; l++) {
+        const int64_t r = sl * 64 + l;
+        const int64_t len = r < n ? rowptr[r + 1] - rowptr[r] : 0;
+        for (int64_t q = 0; q < w; q++) {
+          const int64_t k = off[sl] + (q / SELL_V) * 64 * SELL_V + l * SELL_V + q % SELL_V;
+          if (q < len) {
+            scol[k] = col[rowptr[r] + q];
+            src[k] = rowptr[r] + (int)q;
+          } else {
+            scol[k] = r < n ? (int)r : 0;
+          }
+        }
+      }
+    }
+  });
   return true;
 }
 
@@ -263,6 +311,7 @@ extern "C" int ssf_matrix_destroy(ssf_matrix *m)
   dfree(m->map_lo); dfree(m->map_up);
   dfree(m->slot); dfree(m->order);
   dfree(m->tiles); dfree(m->col16);
+  dfree(m->sell_off); dfree(m->sell_col); dfree(m->sell_src); dfree(m->sell_v64); dfree(m->sell_v32);
   free_workspace(m->ws);
   m->ctx->live_matrices--;
   delete m;
@@ -370,6 +419,8 @@ extern "C" int ssf_matrix_create_csr(ssf_ctx *ctx, int storage, int64_t n, const
   std::vector<int> tiles;
   std::vector<short> col16;
   const bool have_tiles = build_tiles(n, rowptr, col, tiles, col16);
+  std::vector<int> soff, scol, ssrc;
+  const bool have_sell = n > 0 && build_sell(n, rowptr, col, soff, scol, ssrc);
   hipError_t e = hipSuccess;
   auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
   chk(dalloc(&m->rowptr, n + 1));
@@ -385,6 +436,20 @@ extern "C" int ssf_matrix_create_csr(ssf_ctx *ctx, int storage, int64_t n, const
       chk(dalloc(&m->col16, col16.size()));
       if (e == hipSuccess)
         chk(hipMemcpy(m->col16, col16.data(), col16.size() * sizeof(short), hipMemcpyHostToDevice));
+    }
+  }
+  if (have_sell) {
+    m->nslice = (int)soff.size() - 1;
+    m->sell_tot = soff.back();
+    chk(dalloc(&m->sell_off, soff.size()));
+    chk(dalloc(&m->sell_col, m->sell_tot));
+    chk(dalloc(&m->sell_src, m->sell_tot));
+    chk(dalloc(&m->sell_v64, m->sell_tot));
+    if (e == hipSuccess) {
+      chk(hipMemcpy(m->sell_off, soff.data(), soff.size() * sizeof(int), hipMemcpyHostToDevice));
+      chk(hipMemcpy(m->sell_col, scol.data(), scol.size() * sizeof(int), hipMemcpyHostToDevice));
+      chk(hipMemcpy(m->sell_src, ssrc.data(), ssrc.size() * sizeof(int), hipMemcpyHostToDevice));
+      chk(hipMemsetAsync(m->sell_v64, 0, std::max<int64_t>(1, m->sell_tot) * sizeof(double), ctx->stream));
     }
   }
   if (sym) {
@@ -414,8 +479,8 @@ extern "C" int ssf_matrix_create_csr(ssf_ctx *ctx, int storage, int64_t n, const
   }
   // canonical int32 columns for the FP64 drop-in, 16-bit offsets where they fit
   // for the FP32 inner solve
-  m->tune64 = default_tune(n, m->nnz, m->ntiles, false);
-  m->tune32 = default_tune(n, m->nnz, m->ntiles, m->col16 != nullptr);
+  m->tune64 = default_tune(n, m->nnz, m->ntiles, m->nslice, false);
+  m->tune32 = default_tune(n, m->nnz, m->ntiles, m->nslice, m->col16 != nullptr);
   *out = m;
   return SSF_OK;
 }
@@ -463,6 +528,7 @@ static int values_changed(ssf_matrix *m)
   hipStream_t s = m->ctx->stream;
   if (m->storage == SSF_STORAGE_SYMM_COMP_ROWS) SSF_HIP(expand_ref(m->negm, m->map_lo, m->map_up, m->a_ref, m->val, s));
   SSF_HIP(extract_diag((int)m->n, m->rowptr, m->col, m->val, m->diag, nullptr, s));
+  if (m->sell_src) SSF_HIP(sell_gather(m->sell_tot, m->sell_src, m->val, m->sell_v64, s));
   m->values_set = true;
   m->val32_valid = false;
   return SSF_OK;
@@ -688,7 +754,7 @@ static int ensure_ws(ssf_matrix *m, bool mixed)
   if (w.n != n || w.nb < nb) {
     free_workspace(w);
     w.n = n;
-    w.nb = std::max(nb, 2048);
+    w.nb = std::max(nb, 4096);
     SSF_HIP(dalloc(&w.x, n)); SSF_HIP(dalloc(&w.r, n)); SSF_HIP(dalloc(&w.d, n));
     SSF_HIP(dalloc(&w.p, n)); SSF_HIP(dalloc(&w.h, n)); SSF_HIP(dalloc(&w.y, n));
     SSF_HIP(dalloc(&w.part, 4 * (size_t)w.nb));
@@ -709,6 +775,10 @@ static int ensure_ws(ssf_matrix *m, bool mixed)
     if (!m->diag32) SSF_HIP(dalloc(&m->diag32, m->n));
     SSF_HIP(to_f32(m->nnz, m->val, m->val32, 1.0, m->ctx->stream));
     SSF_HIP(to_f32(m->n, m->diag, m->diag32, 1.0, m->ctx->stream));
+    if (m->sell_v64) {
+      if (!m->sell_v32) SSF_HIP(dalloc(&m->sell_v32, m->sell_tot));
+      SSF_HIP(to_f32(m->sell_tot, m->sell_v64, m->sell_v32, 1.0, m->ctx->stream));
+    }
     if (!m->norm_a) SSF_HIP(dalloc(&m->norm_a, 1));
     SSF_HIP(row_absmax((int)m->n, m->rowptr, m->val, w.part, 1024, m->norm_a, m->ctx->stream));
     m->val32_valid = true;
@@ -1147,6 +1217,14 @@ extern "C" int ssf_matrix_autotune(ssf_matrix *m, int precision, int reps, int32
           t.idx16 = i16;
           cand.push_back(t);
         }
+  if (m->nslice > 0)
+    for (int b : {512, 1024, 2048, 4096}) {
+      if (b > std::max(8, (m->nslice + 3) / 4)) continue;
+      SpmvTune t;
+      t.format = 2;
+      t.blocks = b;
+      cand.push_back(t);
+    }
   SpmvTune best = precision ? m->tune32 : m->tune64;
   double best_t = 1e300;
   for (const SpmvTune &t : cand) {
@@ -1179,15 +1257,16 @@ extern "C" int ssf_matrix_set_tune(ssf_matrix *m, int precision, const int32_t *
   t.lanes = tune[1];
   t.blocks = tune[2];
   t.idx16 = tune[3];
-  SSF_CHECK(t.format == 0 || (t.format == 1 && m->ntiles > 0), SSF_EINVAL,
-            "ssf_matrix_set_tune: format must be 0 (row-vector) or 1 (stream, needs a tiling)");
+  SSF_CHECK(t.format == 0 || (t.format == 1 && m->ntiles > 0) || (t.format == 2 && m->nslice > 0),
+            SSF_EINVAL,
+            "ssf_matrix_set_tune: format must be 0 (row-vector), 1 (stream, needs a tiling) or 2 (SELL-64)");
   if (t.format == 0)
     SSF_CHECK(t.lanes == 4 || t.lanes == 8 || t.lanes == 16 || t.lanes == 32 || t.lanes == 64,
               SSF_EINVAL, "ssf_matrix_set_tune: row-vector lanes must be 4, 8, 16, 32 or 64");
-  else
+  else if (t.format == 1)
     SSF_CHECK(t.lanes == 4 || t.lanes == 8 || t.lanes == 16, SSF_EINVAL,
               "ssf_matrix_set_tune: stream lanes must be 4, 8 or 16");
-  SSF_CHECK(t.blocks >= 1 && t.blocks <= 2048, SSF_EINVAL, "ssf_matrix_set_tune: blocks out of [1, 2048]");
+  SSF_CHECK(t.blocks >= 1 && t.blocks <= 4096, SSF_EINVAL, "ssf_matrix_set_tune: blocks out of [1, 4096]");
   SSF_CHECK(t.idx16 == 0 || (precision && t.format == 1 && m->col16), SSF_EINVAL,
             "ssf_matrix_set_tune: 16-bit columns only for the FP32 stream operator of a matrix that has them");
   if (precision) m->tune32 = t;
@@ -1242,6 +1321,8 @@ extern "C" int ssf_spmv_bytes(const ssf_matrix *m, int precision, double *canoni
   const double ib = (t.format == 1 && t.idx16 && m->col16) ? 2.0 : 4.0;
   double a = (double)m->nnz * (sv + ib) + 4.0 * (double)(m->n + 1) + 2.0 * sv * (double)m->n;
   if (t.format == 1) a += 4.0 * (double)(m->ntiles + 1);
+  if (t.format == 2 && m->nslice > 0)  // padded slices, slice offsets instead of row pointers
+    a = (double)m->sell_tot * (sv + 4.0) + 4.0 * (double)(m->nslice + 1) + 2.0 * sv * (double)m->n;
   if (canonical) *canonical = b;
   if (actual) *actual = a;
   return SSF_OK;

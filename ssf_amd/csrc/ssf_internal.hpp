@@ -75,7 +75,7 @@ enum PcgMode : int {
 };
 
 struct SpmvTune {
-  int format = 0;    // 0: row-vector kernel, 1: LDS-tiled stream kernel
+  int format = 0;    // 0: row-vector kernel, 1: LDS-tiled stream kernel, 2: SELL-64 slices
   int lanes = 8;     // lanes (of a 64-wide wavefront) per row
   int blocks = 1024; // blocks of 256 threads (rows / tiles split in contiguous chunks)
   int idx16 = 0;     // stream kernel reads 16-bit tile-relative column offsets
@@ -89,7 +89,17 @@ struct CsrDev {
   const short *col16 = nullptr;  // col - first row of the tile (null if it does not fit)
   const int *tiles = nullptr;    // first row of each tile, ntiles + 1 entries
   int ntiles = 0;
+  // SELL-64: slice s = rows 64s..64s+63, stored column-major in groups of
+  // SELL_V entries per row: entry q of row 64s + l at
+  // sell_off[s] + (q / SELL_V) * 64 * SELL_V + l * SELL_V + q % SELL_V
+  int nslice = 0;
+  const int *sell_off = nullptr;  // nslice + 1, multiples of 64 * SELL_V
+  const int *sell_col = nullptr;  // padding: column of a row of the slice, value 0
+  const double *sell_v64 = nullptr;
+  const float *sell_v32 = nullptr;
 };
+
+constexpr int SELL_V = 4;  // entries per lane and load (16-byte FP32, 32-byte FP64 loads)
 
 struct Workspace {  // vectors and reduction scratch of one solve shape
   int64_t n = 0;
@@ -155,6 +165,12 @@ struct ssf_matrix {
   int *tiles = nullptr;
   int ntiles = 0;
   short *col16 = nullptr;
+  // SELL-64 layout of the full CSR (values gathered from val through sell_src)
+  int nslice = 0;
+  int64_t sell_tot = 0;
+  int *sell_off = nullptr, *sell_col = nullptr, *sell_src = nullptr;
+  double *sell_v64 = nullptr;
+  float *sell_v32 = nullptr;
   ssf::SpmvTune tune64, tune32;
   bool tuned64 = false, tuned32 = false;
   ssf::CsrDev csr() const
@@ -166,6 +182,11 @@ struct ssf_matrix {
     c.col16 = col16;
     c.tiles = tiles;
     c.ntiles = ntiles;
+    c.nslice = nslice;
+    c.sell_off = sell_off;
+    c.sell_col = sell_col;
+    c.sell_v64 = sell_v64;
+    c.sell_v32 = sell_v32;
     return c;
   }
   ssf::Workspace ws;

@@ -156,6 +156,24 @@ __global__ __launch_bounds__(TPB) void k_copy(int64_t n2, const dvec2 *__restric
   for (; i < n2; i += stride) b[i] = __builtin_nontemporal_load(a + i);
 }
 
+__global__ __launch_bounds__(TPB) void k_sell_gather(int64_t tot, const int *__restrict__ src,
+                                                     const double *__restrict__ val,
+                                                     double *__restrict__ out)
+{
+  const int64_t k = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (k >= tot) return;
+  const int j = src[k];
+  out[k] = j >= 0 ? val[j] : 0.0;
+}
+
+hipError_t sell_gather(int64_t tot, const int *src, const double *val, double *out, hipStream_t s)
+{
+  if (tot == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sell_gather, dim3((unsigned)((tot + TPB - 1) / TPB)), dim3(TPB), 0, s, tot,
+                     src, val, out);
+  return hipGetLastError();
+}
+
 hipError_t copy_kernel(int64_t n, const double *a, double *b, hipStream_t s)
 {
   const int64_t n2 = n / 2;

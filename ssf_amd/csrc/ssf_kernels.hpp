@@ -40,6 +40,8 @@ hipError_t mixed_update(const SpmvTune &t64, const SpmvTune &t32, const CsrDev &
 hipError_t row_absmax(int n, const int *rowptr, const double *val, double *partial, int nb,
                       double *out, hipStream_t s);
 hipError_t copy_kernel(int64_t n, const double *a, double *b, hipStream_t s);
+// SELL values from the full CSR values: out[k] = src[k] >= 0 ? val[src[k]] : 0
+hipError_t sell_gather(int64_t tot, const int *src, const double *val, double *out, hipStream_t s);
 
 // element-wise helpers
 hipError_t expand_ref(int64_t negm, const int *map_lo, const int *map_up, const double *a,

@@ -130,6 +130,62 @@ __global__ __launch_bounds__(TPB) void k_spmv_stream(CsrDev A, const T *__restri
   }
 }
 
+// ---------------------------------------------------------------- SELL SpMV
+
+template <typename T>
+__device__ __forceinline__ const T *sell_values(const CsrDev &A);
+template <>
+__device__ __forceinline__ const double *sell_values<double>(const CsrDev &A)
+{
+  return A.sell_v64;
+}
+template <>
+__device__ __forceinline__ const float *sell_values<float>(const CsrDev &A)
+{
+  return A.sell_v32;
+}
+
+// One wavefront per 64-row slice, one lane per row: the lane streams its row's
+// entries SELL_V at a time (one 16/32-byte value load and one 16-byte column
+// load), gathers x and sums in column order. No LDS, no barrier. A block's waves
+// take the slices of its contiguous slice range round-robin.
+template <typename T, int KIND>
+__global__ __launch_bounds__(TPB) void k_spmv_sell(CsrDev A, const T *__restrict__ x,
+                                                   T *__restrict__ y, const T *__restrict__ b,
+                                                   T *__restrict__ partial, const int *done,
+                                                   long long *iter)
+{
+  typedef T tv __attribute__((ext_vector_type(SELL_V)));
+  typedef int iv __attribute__((ext_vector_type(SELL_V)));
+  if (done && *done) return;
+  if (iter && blockIdx.x == 0 && threadIdx.x == 0) *iter += 1;
+  const T *__restrict__ sval = sell_values<T>(A);
+  const int nb = gridDim.x;
+  const int spb = (A.nslice + nb - 1) / nb;
+  const int lb = xcd_block(blockIdx.x, nb);
+  const int s0 = lb * spb, s1 = min(A.nslice, s0 + spb);
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  T acc = T(0);
+  for (int sl = s0 + wave; sl < s1; sl += TPB / 64) {
+    const int off = A.sell_off[sl];
+    const int w = (A.sell_off[sl + 1] - off) / 64;  // a multiple of SELL_V
+    T sum = T(0);
+    for (int q = 0; q < w; q += SELL_V) {
+      const int k = off + q * 64 + lane * SELL_V;
+      const tv vv = ldnt((const tv *)(sval + k));
+      const iv cc = ldnt((const iv *)(A.sell_col + k));
+#pragma unroll
+      for (int u = 0; u < SELL_V; u++) sum += vv[u] * x[cc[u]];
+    }
+    const int row = sl * 64 + lane;
+    if (row < A.n) finish_row<T, 1, KIND>(row, sum, x, y, b, acc);
+  }
+  if (KIND != 0) {
+    const T t = block_sum(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+  }
+}
+
 template <typename T, int KIND>
 static hipError_t spmv_k(const SpmvTune &t, const CsrDev &A, const T *val, const T *x, T *y,
                          const T *b, T *partial, const int *done, long long *iter,
@@ -138,7 +194,9 @@ static hipError_t spmv_k(const SpmvTune &t, const CsrDev &A, const T *val, const
   dim3 grid(t.blocks), block(TPB);
 #define SSF_VEC(G) hipLaunchKernelGGL((k_spmv_vec<T, G, KIND>), grid, block, 0, s, A, val, x, y, b, partial, done, iter)
 #define SSF_STR(I, G) hipLaunchKernelGGL((k_spmv_stream<T, I, G, KIND>), grid, block, 0, s, A, val, x, y, b, partial, done, iter)
-  if (t.format == 1 && A.ntiles > 0) {
+  if (t.format == 2 && A.nslice > 0) {
+    hipLaunchKernelGGL((k_spmv_sell<T, KIND>), grid, block, 0, s, A, x, y, b, partial, done, iter);
+  } else if (t.format == 1 && A.ntiles > 0) {
     const bool i16 = t.idx16 && A.col16;
     switch (t.lanes) {
       case 4: if (i16) SSF_STR(true, 4); else SSF_STR(false, 4); break;

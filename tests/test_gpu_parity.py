@@ -170,8 +170,9 @@ def test_cantilever_full_config(ctx):
     assert abs(info.ani - ani_o) <= max(3, ani_o // 100)
 
 
-TILINGS64 = [(0, l, 64, 0) for l in (4, 8, 16, 32, 64)] + [(1, l, 64, 0) for l in (4, 8, 16)]
-TILINGS32 = [(0, 8, 64, 0), (1, 4, 64, 0), (1, 8, 64, 1), (1, 16, 16, 1)]
+TILINGS64 = ([(0, l, 64, 0) for l in (4, 8, 16, 32, 64)] + [(1, l, 64, 0) for l in (4, 8, 16)]
+             + [(2, 0, b, 0) for b in (1, 8, 64)])
+TILINGS32 = [(0, 8, 64, 0), (1, 4, 64, 0), (1, 8, 64, 1), (1, 16, 16, 1), (2, 0, 8, 0)]
 
 
 @pytest.mark.parametrize("name", ["hex8_elast", "hex8_heat", "quad4"])
