@@ -191,7 +191,8 @@ static bool build_tiles(int64_t n, const std::vector<int> &rowptr, const std::ve
 // column order, padded to the slice's longest row (rounded up to SELL_V) with
 // value 0 at the row's own column; src maps a SELL entry to its CSR entry.
 static bool build_sell(int64_t n, const std::vector<int> &rowptr, const std::vector<int> &col,
-                       std::vector<int> &off, std::vector<int> &scol, std::vector<int> &src)
+                       std::vector<int> &off, std::vector<int> &scol, std::vector<int> &src,
+                       std::vector<short> &scol16)
 {
   const int64_t nslice = (n + 63) / 64;
   off.assign(nslice + 1, 0);
@@ -220,12 +221,31 @@ static bool build_sell(int64_t n, const std::vector<int> &rowptr, const std::vec
             scol[k] = col[rowptr[r] + q];
             src[k] = rowptr[r] + (int)q;
           } else {
-            scol[k] = r < n ? (int)r : 0;
+            scol[k] = (int)std::min<int64_t>(r, n - 1);  // in the slice
           }
         }
       }
     }
   });
+  // 16-bit column offsets from each slice's first row, when every offset fits
+  std::vector<char> fits(nslice, 1);
+  scol16.assign(tot, 0);
+  parallel_for(nslice, [&](int64_t b, int64_t e) {
+    for (int64_t sl = b; sl < e; sl++)
+      for (int64_t k = off[sl]; k < off[sl + 1]; k++) {
+        const int64_t o = (int64_t)scol[k] - sl * 64;
+        if (o < -32768 || o > 32767) {
+          fits[sl] = 0;
+          break;
+        }
+        scol16[k] = (short)o;
+      }
+  });
+  for (char f : fits)
+    if (!f) {
+      std::vector<short>().swap(scol16);
+      break;
+    }
   return true;
 }
 
@@ -312,6 +332,7 @@ extern "C" int ssf_matrix_destroy(ssf_matrix *m)
   dfree(m->slot); dfree(m->order);
   dfree(m->tiles); dfree(m->col16);
   dfree(m->sell_off); dfree(m->sell_col); dfree(m->sell_src); dfree(m->sell_v64); dfree(m->sell_v32);
+  dfree(m->sell_col16);
   free_workspace(m->ws);
   m->ctx->live_matrices--;
   delete m;
@@ -420,7 +441,8 @@ extern "C" int ssf_matrix_create_csr(ssf_ctx *ctx, int storage, int64_t n, const
   std::vector<short> col16;
   const bool have_tiles = build_tiles(n, rowptr, col, tiles, col16);
   std::vector<int> soff, scol, ssrc;
-  const bool have_sell = n > 0 && build_sell(n, rowptr, col, soff, scol, ssrc);
+  std::vector<short> scol16;
+  const bool have_sell = n > 0 && build_sell(n, rowptr, col, soff, scol, ssrc, scol16);
   hipError_t e = hipSuccess;
   auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
   chk(dalloc(&m->rowptr, n + 1));
@@ -449,6 +471,12 @@ extern "C" int ssf_matrix_create_csr(ssf_ctx *ctx, int storage, int64_t n, const
       chk(hipMemcpy(m->sell_off, soff.data(), soff.size() * sizeof(int), hipMemcpyHostToDevice));
       chk(hipMemcpy(m->sell_col, scol.data(), scol.size() * sizeof(int), hipMemcpyHostToDevice));
       chk(hipMemcpy(m->sell_src, ssrc.data(), ssrc.size() * sizeof(int), hipMemcpyHostToDevice));
+      if (!scol16.empty()) {
+        chk(dalloc(&m->sell_col16, m->sell_tot));
+        if (e == hipSuccess)
+          chk(hipMemcpy(m->sell_col16, scol16.data(), scol16.size() * sizeof(short),
+                        hipMemcpyHostToDevice));
+      }
       chk(hipMemsetAsync(m->sell_v64, 0, std::max<int64_t>(1, m->sell_tot) * sizeof(double), ctx->stream));
     }
   }
@@ -481,6 +509,7 @@ extern "C" int ssf_matrix_create_csr(ssf_ctx *ctx, int storage, int64_t n, const
   // for the FP32 inner solve
   m->tune64 = default_tune(n, m->nnz, m->ntiles, m->nslice, false);
   m->tune32 = default_tune(n, m->nnz, m->ntiles, m->nslice, m->col16 != nullptr);
+  if (m->sell_col16) m->tune64.idx16 = m->tune32.idx16 = (m->tune64.format == 2) ? 1 : 0;
   *out = m;
   return SSF_OK;
 }
@@ -1218,13 +1247,15 @@ extern "C" int ssf_matrix_autotune(ssf_matrix *m, int precision, int reps, int32
           cand.push_back(t);
         }
   if (m->nslice > 0)
-    for (int b : {512, 1024, 2048, 4096}) {
-      if (b > std::max(8, (m->nslice + 3) / 4)) continue;
-      SpmvTune t;
-      t.format = 2;
-      t.blocks = b;
-      cand.push_back(t);
-    }
+    for (int b : {512, 1024, 2048, 4096})
+      for (int i16 = 0; i16 <= (m->sell_col16 ? 1 : 0); i16++) {
+        if (b > std::max(8, (m->nslice + 3) / 4)) continue;
+        SpmvTune t;
+        t.format = 2;
+        t.blocks = b;
+        t.idx16 = i16;
+        cand.push_back(t);
+      }
   SpmvTune best = precision ? m->tune32 : m->tune64;
   double best_t = 1e300;
   for (const SpmvTune &t : cand) {
@@ -1267,8 +1298,11 @@ extern "C" int ssf_matrix_set_tune(ssf_matrix *m, int precision, const int32_t *
     SSF_CHECK(t.lanes == 4 || t.lanes == 8 || t.lanes == 16, SSF_EINVAL,
               "ssf_matrix_set_tune: stream lanes must be 4, 8 or 16");
   SSF_CHECK(t.blocks >= 1 && t.blocks <= 4096, SSF_EINVAL, "ssf_matrix_set_tune: blocks out of [1, 4096]");
-  SSF_CHECK(t.idx16 == 0 || (precision && t.format == 1 && m->col16), SSF_EINVAL,
-            "ssf_matrix_set_tune: 16-bit columns only for the FP32 stream operator of a matrix that has them");
+  SSF_CHECK(t.idx16 == 0 || (precision && t.format == 1 && m->col16) ||
+                (t.format == 2 && m->sell_col16),
+            SSF_EINVAL,
+            "ssf_matrix_set_tune: 16-bit columns for the FP32 stream operator or the SELL operator "
+            "of a matrix that has them");
   if (precision) m->tune32 = t;
   else m->tune64 = t;
   return SSF_OK;
@@ -1318,11 +1352,12 @@ extern "C" int ssf_spmv_bytes(const ssf_matrix *m, int precision, double *canoni
   const double sv = precision ? 4.0 : 8.0;
   const double b = (double)m->nnz * (sv + 4.0) + 4.0 * (double)(m->n + 1) + 2.0 * sv * (double)m->n;
   const SpmvTune &t = precision ? m->tune32 : m->tune64;
-  const double ib = (t.format == 1 && t.idx16 && m->col16) ? 2.0 : 4.0;
+  const double ib = ((t.format == 1 && t.idx16 && m->col16) ||
+                     (t.format == 2 && t.idx16 && m->sell_col16)) ? 2.0 : 4.0;
   double a = (double)m->nnz * (sv + ib) + 4.0 * (double)(m->n + 1) + 2.0 * sv * (double)m->n;
   if (t.format == 1) a += 4.0 * (double)(m->ntiles + 1);
   if (t.format == 2 && m->nslice > 0)  // padded slices, slice offsets instead of row pointers
-    a = (double)m->sell_tot * (sv + 4.0) + 4.0 * (double)(m->nslice + 1) + 2.0 * sv * (double)m->n;
+    a = (double)m->sell_tot * (sv + ib) + 4.0 * (double)(m->nslice + 1) + 2.0 * sv * (double)m->n;
   if (canonical) *canonical = b;
   if (actual) *actual = a;
   return SSF_OK;

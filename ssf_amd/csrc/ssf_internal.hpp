@@ -95,6 +95,7 @@ struct CsrDev {
   int nslice = 0;
   const int *sell_off = nullptr;  // nslice + 1, multiples of 64 * SELL_V
   const int *sell_col = nullptr;  // padding: column of a row of the slice, value 0
+  const short *sell_col16 = nullptr;  // sell_col - 64 s (null if an offset does not fit)
   const double *sell_v64 = nullptr;
   const float *sell_v32 = nullptr;
 };
@@ -169,6 +170,7 @@ struct ssf_matrix {
   int nslice = 0;
   int64_t sell_tot = 0;
   int *sell_off = nullptr, *sell_col = nullptr, *sell_src = nullptr;
+  short *sell_col16 = nullptr;
   double *sell_v64 = nullptr;
   float *sell_v32 = nullptr;
   ssf::SpmvTune tune64, tune32;
@@ -185,6 +187,7 @@ struct ssf_matrix {
     c.nslice = nslice;
     c.sell_off = sell_off;
     c.sell_col = sell_col;
+    c.sell_col16 = sell_col16;
     c.sell_v64 = sell_v64;
     c.sell_v32 = sell_v32;
     return c;

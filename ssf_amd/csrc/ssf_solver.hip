@@ -149,7 +149,7 @@ __device__ __forceinline__ const float *sell_values<float>(const CsrDev &A)
 // entries SELL_V at a time (one 16/32-byte value load and one 16-byte column
 // load), gathers x and sums in column order. No LDS, no barrier. A block's waves
 // take the slices of its contiguous slice range round-robin.
-template <typename T, int KIND>
+template <typename T, bool I16, int KIND>
 __global__ __launch_bounds__(TPB) void k_spmv_sell(CsrDev A, const T *__restrict__ x,
                                                    T *__restrict__ y, const T *__restrict__ b,
                                                    T *__restrict__ partial, const int *done,
@@ -157,6 +157,7 @@ __global__ __launch_bounds__(TPB) void k_spmv_sell(CsrDev A, const T *__restrict
 {
   typedef T tv __attribute__((ext_vector_type(SELL_V)));
   typedef int iv __attribute__((ext_vector_type(SELL_V)));
+  typedef short sv __attribute__((ext_vector_type(SELL_V)));
   if (done && *done) return;
   if (iter && blockIdx.x == 0 && threadIdx.x == 0) *iter += 1;
   const T *__restrict__ sval = sell_values<T>(A);
@@ -173,9 +174,16 @@ __global__ __launch_bounds__(TPB) void k_spmv_sell(CsrDev A, const T *__restrict
     for (int q = 0; q < w; q += SELL_V) {
       const int k = off + q * 64 + lane * SELL_V;
       const tv vv = ldnt((const tv *)(sval + k));
-      const iv cc = ldnt((const iv *)(A.sell_col + k));
+      if (I16) {  // 16-bit offsets from the slice's first row
+        const sv cc = ldnt((const sv *)(A.sell_col16 + k));
+        const T *__restrict__ xs = x + sl * 64;
 #pragma unroll
-      for (int u = 0; u < SELL_V; u++) sum += vv[u] * x[cc[u]];
+        for (int u = 0; u < SELL_V; u++) sum += vv[u] * xs[cc[u]];
+      } else {
+        const iv cc = ldnt((const iv *)(A.sell_col + k));
+#pragma unroll
+        for (int u = 0; u < SELL_V; u++) sum += vv[u] * x[cc[u]];
+      }
     }
     const int row = sl * 64 + lane;
     if (row < A.n) finish_row<T, 1, KIND>(row, sum, x, y, b, acc);
@@ -195,7 +203,10 @@ static hipError_t spmv_k(const SpmvTune &t, const CsrDev &A, const T *val, const
 #define SSF_VEC(G) hipLaunchKernelGGL((k_spmv_vec<T, G, KIND>), grid, block, 0, s, A, val, x, y, b, partial, done, iter)
 #define SSF_STR(I, G) hipLaunchKernelGGL((k_spmv_stream<T, I, G, KIND>), grid, block, 0, s, A, val, x, y, b, partial, done, iter)
   if (t.format == 2 && A.nslice > 0) {
-    hipLaunchKernelGGL((k_spmv_sell<T, KIND>), grid, block, 0, s, A, x, y, b, partial, done, iter);
+    if (t.idx16 && A.sell_col16)
+      hipLaunchKernelGGL((k_spmv_sell<T, true, KIND>), grid, block, 0, s, A, x, y, b, partial, done, iter);
+    else
+      hipLaunchKernelGGL((k_spmv_sell<T, false, KIND>), grid, block, 0, s, A, x, y, b, partial, done, iter);
   } else if (t.format == 1 && A.ntiles > 0) {
     const bool i16 = t.idx16 && A.col16;
     switch (t.lanes) {
