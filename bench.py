@@ -209,6 +209,10 @@ def main():
     ap.add_argument("--batch-res", type=float, default=1e-20)
     ap.add_argument("--seq-stride", type=int, default=16)
     ap.add_argument("--only-batch", action="store_true")
+    ap.add_argument("--tiling64", type=int, nargs=4, default=None,
+                    help="pin the FP64 SpMV tiling (format lanes blocks idx16) instead of autotuning")
+    ap.add_argument("--tiling32", type=int, nargs=4, default=None,
+                    help="pin the FP32 SpMV tiling instead of autotuning")
     ap.add_argument("--no-serve", action="store_true", help="skip the served leg")
     ap.add_argument("--only-serve", action="store_true")
     ap.add_argument("--serve-count", type=int, default=24)
@@ -242,8 +246,16 @@ def main():
     lift = m.assemble(prob)
     rhs = prob.rhs[0] + lift
     log(f"[bench] assembled ({time.time() - t0:.1f}s)")
-    tune64 = m.autotune(gpu.FP64)
-    tune32 = m.autotune(gpu.MIXED)
+    if args.tiling64:
+        m.set_tune(gpu.FP64, args.tiling64)
+        tune64 = tuple(args.tiling64)
+    else:
+        tune64 = m.autotune(gpu.FP64)
+    if args.tiling32:
+        m.set_tune(gpu.MIXED, args.tiling32)
+        tune32 = tuple(args.tiling32)
+    else:
+        tune32 = m.autotune(gpu.MIXED)
     log(f"[bench] spmv tiling fp64 {tune64} fp32 {tune32}")
 
     dev = torch.device("cuda:0")
@@ -325,7 +337,9 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as fh:
             tr = json.load(fh)
-        if tr["n"] == prob.n and tr["nnz"] == m.nnz and list(tr["fp32"]["tiling"]) == list(tune32):
+        t = tr["fp32"]["tiling"]
+        if (tr["n"] == prob.n and tr["nnz"] == m.nnz and t[0] == tune32[0] and t[3] == tune32[3]
+                and (t[0] == 2 or t[1] == tune32[1])):
             traffic = tr["fp32"]["bytes_per_launch"]
 
     batched = None
