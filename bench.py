@@ -170,7 +170,11 @@ def bench_served(ctx, args, gpu, mesh, torch):
             "p50_latency_s": float(np.percentile(lat, 50)),
             "p99_latency_s": float(np.percentile(lat, 99)),
             "mean_queue_s": float(np.mean(times[:, 1] - times[:, 0])),
-            "converged": conv, "peak_bytes": st["peak_bytes"], "steps": st["steps"]}
+            "mean_assembly_s": float(np.mean(times[:, 2] - times[:, 1])),
+            "mean_solve_s": float(np.mean(times[:, 3] - times[:, 2])),
+            "converged": conv, "peak_bytes": st["peak_bytes"], "steps": st["steps"],
+            "prep_thread_s": {"pattern": st["pattern_s"], "setup": st["setup_s"],
+                              "assembly": st["assembly_s"]}}
         log(f"[serve] rate {rate}/s: {out['rates'][str(rate)]}")
         xs = [r[0] for r in res]
     # the drop-in path, one request after the other: pattern, assembly, FP64 solve
@@ -190,6 +194,32 @@ def bench_served(ctx, args, gpu, mesh, torch):
     out["speedup_served_vs_sequential_fp64"] = best / out["sequential_fp64_solves_per_s"]
     log(f"[serve] sequential drop-in {out['sequential_fp64_solves_per_s']:.2f} solves/s, "
         f"served best {best:.2f}; max rel diff {out['max_rel_diff_vs_sequential']:.1e}")
+    return out
+
+
+def bench_assembly(ctx, args, gpu, mesh):
+    """Element kernels + coloured assembly of the 64x64x256 hex8 elasticity column
+    (1 048 576 elements, 3 244 800 unknowns): device time of the element kernel
+    launches (HIP events around ssf_matrix_assemble's kernels are not exposed, so
+    the whole call is timed after a warm-up that builds the slot map)."""
+    prob = mesh.elastic_column(args.asm_nel, args.asm_nel, 4 * args.asm_nel, seed=3)
+    m = gpu.Matrix.from_elements(ctx, gpu.STORAGE_SCR, prob.n, prob.elem_codes())
+    m.assemble(prob, lift=False)  # slot map and colouring
+    ts = []
+    for _ in range(args.asm_reps):
+        m.zero()
+        t = time.perf_counter()
+        m.assemble(prob, lift=False)
+        ts.append(time.perf_counter() - t)
+    dt = min(ts)
+    flop = 69120.0 * prob.ne
+    out = {"workload": f"hex8 elasticity {args.asm_nel}x{args.asm_nel}x{4 * args.asm_nel}, "
+                       f"{prob.ne} elements, n={prob.n}",
+           "assemble_call_s": dt, "elements_per_s": prob.ne / dt,
+           "reference_flop_per_element": 69120,
+           "gflops_call": flop / dt / 1e9}
+    log(f"[assembly] {out}")
+    m.close()
     return out
 
 
@@ -215,6 +245,9 @@ def main():
                     help="pin the FP32 SpMV tiling instead of autotuning")
     ap.add_argument("--no-serve", action="store_true", help="skip the served leg")
     ap.add_argument("--only-serve", action="store_true")
+    ap.add_argument("--only-assembly", action="store_true")
+    ap.add_argument("--asm-nel", type=int, default=64)
+    ap.add_argument("--asm-reps", type=int, default=3)
     ap.add_argument("--serve-count", type=int, default=24)
     ap.add_argument("--serve-res", type=float, default=1e-22)
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[20.0])
@@ -234,6 +267,10 @@ def main():
     if args.only_serve:
         ctx = gpu.Context(0)
         print(json.dumps(bench_served(ctx, args, gpu, mesh, torch)), flush=True)
+        return
+    if args.only_assembly:
+        ctx = gpu.Context(0)
+        print(json.dumps(bench_assembly(ctx, args, gpu, mesh)), flush=True)
         return
     t0 = time.time()
     prob = mesh.heat_cube(args.nel, seed=2)

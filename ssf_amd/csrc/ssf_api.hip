@@ -38,6 +38,19 @@ static void parallel_for(int64_t n, F f)
   for (auto &x : th) x.join();
 }
 
+// f(t) for t in [0, T), one host thread each
+template <typename F>
+static void parallel_tasks(int T, F f)
+{
+  if (T <= 1) {
+    f(0);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int t = 0; t < T; t++) th.emplace_back([=]() { f(t); });
+  for (auto &x : th) x.join();
+}
+
 template <typename T>
 static hipError_t dalloc(T **p, size_t count)
 {
@@ -359,10 +372,134 @@ void ssf::matrix_drop_assembly(ssf_matrix *m, bool host_pattern)
   }
 }
 
-extern "C" int ssf_matrix_create_csr(ssf_ctx *ctx, int storage, int64_t n, const int64_t *adr,
-                                     const int64_t *ci, ssf_matrix **out)
+void ssf::matrix_keep_sell_only(ssf_matrix *m)
 {
-  SSF_ENTER();
+  (void)hipStreamSynchronize(m->ctx->stream);
+  dfree(m->col);
+  dfree(m->val);
+  dfree(m->val32);
+  dfree(m->diag32);
+  dfree(m->tiles);
+  dfree(m->col16);
+  dfree(m->sell_src);
+  dfree(m->sell_v32);
+  if (m->sell_col16) dfree(m->sell_col);
+  m->ntiles = 0;
+  m->val32_valid = false;
+}
+
+// Full CSR of a symcomprow pattern, built in parallel: row i = the lower part of
+// scr row i (ascending, diagonal last), then the upper part: the rows j > i that
+// hold column i, in ascending j. The upper parts are a counting-sort transpose
+// over contiguous chunks of rows: chunk t's entries of column i follow those of
+// chunks < t, so every column's rows stay ascending. lo[k] / up[k] give the full
+// slots of scr entry k (up = -1 on the diagonal).
+static bool full_from_scr(int64_t n, const int64_t *adr, const int64_t *ci,
+                          std::vector<int> &rowptr, std::vector<int> &col, std::vector<int> &lo,
+                          std::vector<int> &up)
+{
+  const int64_t negm = adr[n];
+  const int T = (int)std::max<int64_t>(1, std::min<int64_t>(host_threads(), n / 4096 + 1));
+  std::vector<int> cc((size_t)T * n, 0);  // entries of column i in chunk t (strictly lower)
+  auto chunk_rows = [&](int t, int64_t &b, int64_t &e) {
+    b = n * t / T;
+    e = n * (t + 1) / T;
+  };
+  parallel_tasks(T, [&](int t) {
+    int64_t b, e;
+    chunk_rows(t, b, e);
+    int *c = cc.data() + (size_t)t * n;
+    for (int64_t j = b; j < e; j++)
+      for (int64_t k = adr[j]; k < adr[j + 1]; k++)
+        if (ci[k] != j) c[ci[k]]++;
+  });
+  rowptr.assign(n + 1, 0);
+  std::vector<int> ucnt(n);
+  parallel_for(n, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; i++) {
+      int u = 0;
+      for (int t = 0; t < T; t++) u += cc[(size_t)t * n + i];
+      ucnt[i] = u;
+    }
+  });
+  int64_t tot = 0;
+  for (int64_t i = 0; i < n; i++) {
+    rowptr[i] = (int)tot;
+    tot += (adr[i + 1] - adr[i]) + ucnt[i];
+    if (tot >= (1LL << 31)) return false;
+  }
+  rowptr[n] = (int)tot;
+  // first upper slot of column i for chunk t
+  parallel_for(n, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; i++) {
+      int run = rowptr[i] + (int)(adr[i + 1] - adr[i]);
+      for (int t = 0; t < T; t++) {
+        const int c = cc[(size_t)t * n + i];
+        cc[(size_t)t * n + i] = run;
+        run += c;
+      }
+    }
+  });
+  col.assign(tot, 0);
+  lo.assign(negm, 0);
+  up.assign(negm, -1);
+  parallel_tasks(T, [&](int t) {
+    int64_t b, e;
+    chunk_rows(t, b, e);
+    int *c = cc.data() + (size_t)t * n;
+    for (int64_t j = b; j < e; j++)
+      for (int64_t k = adr[j]; k < adr[j + 1]; k++) {
+        const int64_t s = rowptr[j] + (k - adr[j]);
+        col[s] = (int)ci[k];
+        lo[k] = (int)s;
+        if (ci[k] != j) {
+          const int u = c[ci[k]]++;
+          col[u] = (int)j;
+          up[k] = u;
+        }
+      }
+  });
+  return true;
+}
+
+// validation of a reference-layout pattern: ascending, in range; symcomprow rows
+// hold the lower triangle and end with their diagonal
+static const char *check_pattern(int64_t n, const int64_t *adr, const int64_t *ci, bool sym)
+{
+  std::atomic<int> bad{0};
+  parallel_for(n, [&](int64_t b, int64_t e) {
+    int code = 0;
+    for (int64_t i = b; i < e && !code; i++) {
+      if (adr[i + 1] < adr[i]) { code = 1; break; }
+      for (int64_t k = adr[i]; k < adr[i + 1]; k++) {
+        if (ci[k] < 0 || ci[k] >= n) { code = 2; break; }
+        if (k > adr[i] && ci[k] <= ci[k - 1]) { code = 3; break; }
+        if (sym && ci[k] > i) { code = 4; break; }
+      }
+      if (!code && sym && !(adr[i + 1] > adr[i] && ci[adr[i + 1] - 1] == i)) code = 5;
+    }
+    if (code) {
+      int z = 0;
+      bad.compare_exchange_strong(z, code);
+    }
+  });
+  switch (bad.load()) {
+    case 1: return "ssf_matrix_create_csr: adr not monotone";
+    case 2: return "ssf_matrix_create_csr: column index out of range";
+    case 3: return "ssf_matrix_create_csr: columns not ascending";
+    case 4: return "ssf_matrix_create_csr: symm_comp_rows holds the lower triangle only";
+    case 5: return "ssf_matrix_create_csr: every symm_comp_rows row must end with its diagonal";
+    default: return nullptr;
+  }
+}
+
+// Device storage of a pattern. adr_own / ci_own (may be null) are moved into the
+// matrix instead of copied; stream_tiles = false skips the tiling of the CSR
+// stream kernel (matrices solved through the SELL layout only).
+int ssf::matrix_create(ssf_ctx *ctx, int storage, int64_t n, const int64_t *adr,
+                       const int64_t *ci, std::vector<int64_t> *adr_own,
+                       std::vector<int64_t> *ci_own, bool stream_tiles, ssf_matrix **out)
+{
   SSF_CHECK(ctx && out && adr && (ci || n == 0), SSF_EINVAL, "ssf_matrix_create_csr: NULL argument");
   SSF_CHECK(storage == SSF_STORAGE_SYMM_COMP_ROWS || storage == SSF_STORAGE_COMPRESSED_ROWS,
             SSF_EUNSUP, "ssf_matrix_create_csr: storage must be symm_comp_rows (11) or compressed_rows (10)");
@@ -370,17 +507,9 @@ extern "C" int ssf_matrix_create_csr(ssf_ctx *ctx, int storage, int64_t n, const
   SSF_CHECK(adr[0] == 0, SSF_EINVAL, "ssf_matrix_create_csr: adr[0] must be 0");
   const int64_t negm = adr[n];
   const bool sym = storage == SSF_STORAGE_SYMM_COMP_ROWS;
-  // validation: ascending, in range; symcomprow rows end with the diagonal
-  for (int64_t i = 0; i < n; i++) {
-    SSF_CHECK(adr[i + 1] >= adr[i], SSF_EINVAL, "ssf_matrix_create_csr: adr not monotone");
-    for (int64_t k = adr[i]; k < adr[i + 1]; k++) {
-      SSF_CHECK(ci[k] >= 0 && ci[k] < n, SSF_EINVAL, "ssf_matrix_create_csr: column index out of range");
-      if (k > adr[i]) SSF_CHECK(ci[k] > ci[k - 1], SSF_EINVAL, "ssf_matrix_create_csr: columns not ascending");
-      if (sym) SSF_CHECK(ci[k] <= i, SSF_EINVAL, "ssf_matrix_create_csr: symm_comp_rows holds the lower triangle only");
-    }
-    if (sym)
-      SSF_CHECK(adr[i + 1] > adr[i] && ci[adr[i + 1] - 1] == i, SSF_EINVAL,
-                "ssf_matrix_create_csr: every symm_comp_rows row must end with its diagonal");
+  if (const char *msg = check_pattern(n, adr, ci, sym)) {
+    set_error(msg);
+    return SSF_EINVAL;
   }
   SSF_HIP(hipSetDevice(ctx->device));
   ssf_matrix *m = new ssf_matrix();
@@ -389,57 +518,32 @@ extern "C" int ssf_matrix_create_csr(ssf_ctx *ctx, int storage, int64_t n, const
   m->storage = storage;
   m->n = n;
   m->negm = negm;
-  m->adr.assign(adr, adr + n + 1);
-  m->ci.assign(ci, ci + negm);
   std::vector<int> rowptr(n + 1), col, lo, up;
   if (!sym) {
     for (int64_t i = 0; i <= n; i++) rowptr[i] = (int)adr[i];
     col.resize(negm);
-    for (int64_t k = 0; k < negm; k++) col[k] = (int)ci[k];
+    parallel_for(negm, [&](int64_t b, int64_t e) {
+      for (int64_t k = b; k < e; k++) col[k] = (int)ci[k];
+    });
     m->nnz = negm;
   } else {
-    std::vector<int64_t> cnt(n, 0);
-    for (int64_t i = 0; i < n; i++)
-      for (int64_t k = adr[i]; k < adr[i + 1]; k++) {
-        cnt[i]++;
-        if (ci[k] != i) cnt[ci[k]]++;
-      }
-    int64_t tot = 0;
-    for (int64_t i = 0; i < n; i++) {
-      rowptr[i] = (int)tot;
-      tot += cnt[i];
-    }
-    rowptr[n] = (int)tot;
-    if (tot >= (1LL << 31)) {
-      delete m;
+    if (!full_from_scr(n, adr, ci, rowptr, col, lo, up)) {
+      ssf_matrix_destroy(m);
       set_error("ssf_matrix_create_csr: more than 2^31 entries in the full matrix");
       return SSF_EUNSUP;
     }
-    m->nnz = tot;
-    col.resize(tot);
-    lo.resize(negm);
-    up.resize(negm);
-    // full row i = lower part of scr row i (ascending, diagonal last), then the
-    // upper part: rows j > i holding column i, visited in ascending j
-    std::vector<int64_t> pos(n);
-    for (int64_t i = 0; i < n; i++) pos[i] = rowptr[i] + (adr[i + 1] - adr[i]);
-    for (int64_t j = 0; j < n; j++)
-      for (int64_t k = adr[j]; k < adr[j + 1]; k++) {
-        const int64_t s = rowptr[j] + (k - adr[j]);
-        col[s] = (int)ci[k];
-        lo[k] = (int)s;
-        if (ci[k] != j) {
-          const int64_t u = pos[ci[k]]++;
-          col[u] = (int)j;
-          up[k] = (int)u;
-        } else {
-          up[k] = -1;
-        }
-      }
+    m->nnz = rowptr[n];
+  }
+  if (adr_own && ci_own) {
+    m->adr.swap(*adr_own);
+    m->ci.swap(*ci_own);
+  } else {
+    m->adr.assign(adr, adr + n + 1);
+    m->ci.assign(ci, ci + negm);
   }
   std::vector<int> tiles;
   std::vector<short> col16;
-  const bool have_tiles = build_tiles(n, rowptr, col, tiles, col16);
+  const bool have_tiles = stream_tiles && build_tiles(n, rowptr, col, tiles, col16);
   std::vector<int> soff, scol, ssrc;
   std::vector<short> scol16;
   const bool have_sell = n > 0 && build_sell(n, rowptr, col, soff, scol, ssrc, scol16);
@@ -514,6 +618,13 @@ extern "C" int ssf_matrix_create_csr(ssf_ctx *ctx, int storage, int64_t n, const
   return SSF_OK;
 }
 
+extern "C" int ssf_matrix_create_csr(ssf_ctx *ctx, int storage, int64_t n, const int64_t *adr,
+                                     const int64_t *ci, ssf_matrix **out)
+{
+  SSF_ENTER();
+  return matrix_create(ctx, storage, n, adr, ci, nullptr, nullptr, true, out);
+}
+
 extern "C" int ssf_matrix_create_elem(ssf_ctx *ctx, int storage, int64_t n, int64_t ne,
                                       int64_t ndofe, const int64_t *cn, ssf_matrix **out)
 {
@@ -529,7 +640,8 @@ extern "C" int ssf_matrix_create_elem(ssf_ctx *ctx, int storage, int64_t n, int6
     for (int64_t i = 0; i < n; i++)
       SSF_CHECK(adr[i + 1] > adr[i] && ci[adr[i + 1] - 1] == i, SSF_EINVAL,
                 "ssf_matrix_create_elem: unknown " + std::to_string(i + 1) + " belongs to no element");
-  return ssf_matrix_create_csr(ctx, storage, n, adr.data(), ci.data(), out);
+  SSF_ENTER();
+  return matrix_create(ctx, storage, n, adr.data(), ci.data(), &adr, &ci, true, out);
 }
 
 extern "C" int ssf_matrix_sizes(const ssf_matrix *m, int64_t *n, int64_t *negm, int64_t *nnz_full)
@@ -641,10 +753,14 @@ static int prepare_assembly(ssf_matrix *m, int64_t nn, int64_t ne, int nne, int 
               (int64_t)m->asm_cn_host.size() == ncn &&
               std::equal(m->asm_cn_host.begin(), m->asm_cn_host.end(), cn);
   std::vector<int> cn32(ncn);
-  for (int64_t k = 0; k < ncn; k++) {
-    SSF_CHECK(cn[k] <= m->n, SSF_EINVAL, "code number larger than the number of unknowns");
-    cn32[k] = (int)cn[k];
-  }
+  std::atomic<int> too_large{0};
+  parallel_for(ncn, [&](int64_t b, int64_t e) {
+    for (int64_t k = b; k < e; k++) {
+      if (cn[k] > m->n) too_large = 1;
+      cn32[k] = (int)cn[k];
+    }
+  });
+  SSF_CHECK(!too_large.load(), SSF_EINVAL, "code number larger than the number of unknowns");
   int *d_cn = nullptr;
   SSF_HIP(dalloc(&d_cn, ncn));
   SSF_HIP(hipMemcpy(d_cn, cn32.data(), ncn * sizeof(int), hipMemcpyHostToDevice));
@@ -671,8 +787,12 @@ static int prepare_assembly(ssf_matrix *m, int64_t nn, int64_t ne, int nne, int 
   SSF_HIP(dalloc(&m->slot, ne * nd * nd));
   // reference-layout pattern for the search
   std::vector<int> adr32(m->n + 1), ci32(m->negm);
-  for (int64_t i = 0; i <= m->n; i++) adr32[i] = (int)m->adr[i];
-  for (int64_t k = 0; k < m->negm; k++) ci32[k] = (int)m->ci[k];
+  parallel_for(m->n + 1, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; i++) adr32[i] = (int)m->adr[i];
+  });
+  parallel_for(m->negm, [&](int64_t b, int64_t e) {
+    for (int64_t k = b; k < e; k++) ci32[k] = (int)m->ci[k];
+  });
   int *d_adr = nullptr, *d_ci = nullptr;
   SSF_HIP(dalloc(&d_adr, m->n + 1));
   SSF_HIP(dalloc(&d_ci, m->negm));
@@ -711,7 +831,9 @@ extern "C" int ssf_matrix_assemble(ssf_matrix *m, int kind, int64_t nn, const do
   int rc = prepare_assembly(m, nn, ne, nne, nd, conn, cn, kind == SSF_ELEM_HEX8_ELAST, &d_cn);
   if (rc) return rc;
   std::vector<int> conn32(ne * nne);
-  for (int64_t k = 0; k < ne * nne; k++) conn32[k] = (int)conn[k];
+  parallel_for(ne * nne, [&](int64_t b, int64_t e) {
+    for (int64_t k = b; k < e; k++) conn32[k] = (int)conn[k];
+  });
   int *d_conn = nullptr;
   double *d_xyz = nullptr, *d_a = nullptr, *d_b = nullptr, *d_t = nullptr, *d_pv = nullptr, *d_lift = nullptr;
   hipError_t e = hipSuccess;

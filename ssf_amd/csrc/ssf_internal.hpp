@@ -199,7 +199,15 @@ namespace ssf {
 // symbolic pattern of symcomprow / comprow from element code numbers (host)
 int pattern_from_elements(int storage, int64_t n, int64_t ne, int64_t ndofe, const int64_t *cn,
                           std::vector<int64_t> &adr, std::vector<int64_t> &ci);
+// device storage of a reference-layout pattern (ssf_matrix_create_csr); the
+// host pattern vectors are moved in when given, the stream tiling is optional
+int matrix_create(ssf_ctx *ctx, int storage, int64_t n, const int64_t *adr, const int64_t *ci,
+                  std::vector<int64_t> *adr_own, std::vector<int64_t> *ci_own, bool stream_tiles,
+                  ssf_matrix **out);
 // releases the element-assembly structures (slot map, colour order) of a matrix
 // and, if host_pattern, its host copy of the reference-layout pattern
 void matrix_drop_assembly(ssf_matrix *m, bool host_pattern);
+// keeps only what the SELL-64 solve reads (slice offsets, columns of the
+// narrowest width, SELL values, diagonal): the CSR and the FP32 copies go
+void matrix_keep_sell_only(ssf_matrix *m);
 }  // namespace ssf

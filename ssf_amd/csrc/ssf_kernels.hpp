@@ -11,10 +11,12 @@ constexpr int STREAM_CAP = TPB * STREAM_KPT;  // entries per stream tile
 
 // y = A x (kind 0); y = A x with partial[b] = sum x_i y_i (kind 1);
 // y = b - A x with partial[b] = sum y_i^2 (kind 2). done (may be null) skips the
-// launch body once set; iter (may be null) is incremented by block 0.
+// launch body once set; iter (may be null) is incremented by block 0. ev0/ev1
+// (may be null) receive the kernel's own start and end times.
 template <typename T>
 hipError_t spmv(const SpmvTune &t, int kind, const CsrDev &A, const T *val, const T *x, T *y,
-                const T *b, T *partial, const int *done, long long *iter, hipStream_t s);
+                const T *b, T *partial, const int *done, long long *iter, hipStream_t s,
+                hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 
 // PCG pieces (see ssf_kernels.hip for the recurrence).
 template <typename T>

@@ -2,9 +2,8 @@
 // resident request ("slot") by one iteration of the cg_prec recurrence
 // (GEFEL/scr.cpp:1142-1197), each slot through its own descriptor.
 //
-// Block b of a launch works on the stream tiles [blk_t0[b], blk_t1[b]) of slot
-// blk_slot[b] (whole rows, at most STREAM_CAP entries per tile, the tiling of the
-// slot's own matrix). Every block leaves its partial sums; a small kernel with
+// Block b of a launch works on the 64-row slices [blk_s0[b], blk_s1[b]) of slot
+// blk_slot[b] (the SELL-64 layout of the slot's own matrix). Every block leaves its partial sums; a small kernel with
 // one block per slot then sums a slot's partials in block order into the
 // slot's scalars (d.p, r.r, r.h), read by the next kernel. The summation order
 // depends on the slot's own tiling only, so a request's result is the same alone
@@ -50,56 +49,47 @@ __global__ __launch_bounds__(TPB) void k_sv_reduce(ServeDev S, SlotState st, int
   }
 }
 
-// p = A d over the block's tiles: coalesced loads of the tile's entries into
-// registers (the next tile's while this one is summed), products with the
-// gathered d into LDS, then `lanes` lanes per row sum their row in a fixed order
+// p = A d over the block's slices: one wavefront per slice, one lane per row,
+// SELL_V entries per load (16-bit or 32-bit columns), row sums in column order
 __global__ __launch_bounds__(TPB) void k_sv_spmv(ServeDev S, SlotState st)
 {
-  __shared__ double prod[STREAM_CAP];
+  typedef double tv __attribute__((ext_vector_type(SELL_V)));
+  typedef int iv __attribute__((ext_vector_type(SELL_V)));
+  typedef short sv __attribute__((ext_vector_type(SELL_V)));
   const int blk = blockIdx.x;
   const SlotDesc D = S.slots[S.blk_slot[blk]];
   const int sid = D.sid;
   if (st.done[sid]) return;
-  const int t0 = S.blk_t0[blk], t1 = S.blk_t1[blk];
-  const int g = D.lanes;
-  const int grp = threadIdx.x / g, lane = threadIdx.x % g, ng = TPB / g;
-  double v[STREAM_KPT];
-  int c[STREAM_KPT];
-  auto load_tile = [&](int tile) {
-    const int s = D.rowptr[D.tiles[tile]], e = D.rowptr[D.tiles[tile + 1]];
-#pragma unroll
-    for (int j = 0; j < STREAM_KPT; j++) {
-      const int k = s + j * TPB + threadIdx.x;
-      if (k < e) {
-        v[j] = ldnt(D.val + k);
-        c[j] = ldnt(D.col + k);
-      } else {
-        v[j] = 0.0;
-        c[j] = -1;
-      }
-    }
-  };
+  const int s0 = S.blk_s0[blk], s1 = S.blk_s1[blk];
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   double acc = 0.0;
-  if (t0 < t1) load_tile(t0);
-  for (int tile = t0; tile < t1; tile++) {
-    const int r0 = D.tiles[tile], r1 = D.tiles[tile + 1];
-    const int s = D.rowptr[r0];
+  for (int sl = s0 + wave; sl < s1; sl += TPB / 64) {
+    const int off = D.soff[sl];
+    const int w = (D.soff[sl + 1] - off) / 64;
+    double sum = 0.0;
+    if (D.scol16) {
+      const double *__restrict__ xs = D.d + sl * 64;
+      for (int q = 0; q < w; q += SELL_V) {
+        const int k = off + q * 64 + lane * SELL_V;
+        const tv vv = ldnt((const tv *)(D.sval + k));
+        const sv cc = ldnt((const sv *)(D.scol16 + k));
 #pragma unroll
-    for (int j = 0; j < STREAM_KPT; j++)
-      if (c[j] >= 0) prod[j * TPB + threadIdx.x] = v[j] * D.d[c[j]];
-    if (tile + 1 < t1) load_tile(tile + 1);
-    __syncthreads();
-    for (int row = r0 + grp; row < r1; row += ng) {
-      const int k0 = D.rowptr[row] - s, k1 = D.rowptr[row + 1] - s;
-      double sum = 0.0;
-      for (int k = k0 + lane; k < k1; k += g) sum += prod[k];
-      for (int off = g / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, g);
-      if (lane == 0) {
-        D.p[row] = sum;
-        acc += D.d[row] * sum;
+        for (int u = 0; u < SELL_V; u++) sum += vv[u] * xs[cc[u]];
+      }
+    } else {
+      for (int q = 0; q < w; q += SELL_V) {
+        const int k = off + q * 64 + lane * SELL_V;
+        const tv vv = ldnt((const tv *)(D.sval + k));
+        const iv cc = ldnt((const iv *)(D.scol + k));
+#pragma unroll
+        for (int u = 0; u < SELL_V; u++) sum += vv[u] * D.d[cc[u]];
       }
     }
-    __syncthreads();
+    const int row = sl * 64 + lane;
+    if (row < D.n) {
+      D.p[row] = sum;
+      acc += D.d[row] * sum;
+    }
   }
   block_partials2(S.part, acc, 0.0);
 }
@@ -125,7 +115,7 @@ __global__ __launch_bounds__(TPB) void k_sv_update(ServeDev S, SlotState st, dou
   } else {
     a = nom / den;
   }
-  const int r0 = D.tiles[S.blk_t0[blk]], r1 = D.tiles[S.blk_t1[blk]];
+  const int r0 = S.blk_s0[blk] * 64, r1 = min(D.n, S.blk_s1[blk] * 64);
   double srr = 0.0, srh = 0.0;
   for (int i = r0 + threadIdx.x; i < r1; i += TPB) {
     if (a != 0.0) {
@@ -171,7 +161,7 @@ __global__ __launch_bounds__(TPB) void k_sv_dir(ServeDev S, SlotState st, double
   }
   if (stop) return;
   const double bt = nomn / nom;
-  const int r0 = D.tiles[S.blk_t0[blk]], r1 = D.tiles[S.blk_t1[blk]];
+  const int r0 = S.blk_s0[blk] * 64, r1 = min(D.n, S.blk_s1[blk] * 64);
   for (int i = r0 + threadIdx.x; i < r1; i += TPB) D.d[i] = bt * D.d[i] + D.h[i];
 }
 
@@ -205,7 +195,7 @@ __global__ __launch_bounds__(TPB) void k_sv_init(ServeDev S, SlotState st)
   const int blk = blockIdx.x;
   const SlotDesc D = S.slots[S.blk_slot[blk]];
   if (!D.fresh) return;
-  const int r0 = D.tiles[S.blk_t0[blk]], r1 = D.tiles[S.blk_t1[blk]];
+  const int r0 = S.blk_s0[blk] * 64, r1 = min(D.n, S.blk_s1[blk] * 64);
   double syy = 0.0, srh = 0.0;
   for (int i = r0 + threadIdx.x; i < r1; i += TPB) {
     const double yi = D.y[i];

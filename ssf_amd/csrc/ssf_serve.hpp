@@ -5,22 +5,21 @@
 
 namespace ssf {
 
-// One resident request: its own matrix (full CSR of the gmatrix, cut into stream
-// tiles of whole rows) and its own solve vectors, wherever they were allocated.
-// Kernels reach them through this descriptor, so admitting or retiring a
-// request moves no matrix data.
+// One resident request: its own matrix (the SELL-64 slices of the gmatrix's full
+// rows) and its own solve vectors, wherever they were allocated. Kernels reach
+// them through this descriptor, so admitting or retiring a request moves no
+// matrix data.
 struct SlotDesc {
-  const int *rowptr;
-  const int *col;
-  const double *val;
+  const int *soff;      // slice offsets, nslice + 1
+  const int *scol;      // 32-bit columns (used when scol16 is null)
+  const short *scol16;  // 16-bit offsets from the slice's first row
+  const double *sval;
   const double *diag;
-  const int *tiles;  // first row of each tile, ntiles + 1 entries
   const double *y;
   double *x, *r, *d, *p, *h;
   int n;
   int sid;         // index of the slot's scalars in SlotState (stable while resident)
   int blk0, nblk;  // the slot's blocks in the block table
-  int lanes;       // lanes per row of the SpMV row sums (4, 8 or 16)
   int fresh;       // admitted since the last step: initialise before iterating
 };
 
@@ -35,12 +34,12 @@ struct SlotState {
 struct ServeDev {
   int nslot = 0, nblk = 0;
   const SlotDesc *slots = nullptr;
-  const int *blk_slot = nullptr, *blk_t0 = nullptr, *blk_t1 = nullptr;  // local tiles
+  const int *blk_slot = nullptr, *blk_s0 = nullptr, *blk_s1 = nullptr;  // local slices
   double *part = nullptr;  // 2 per block
 };
 
-// stream tiles of a slot per block: the partition depends only on the slot
-constexpr int SERVE_TILES_PER_BLOCK = 4;
+// 64-row slices of a slot per block: the partition depends only on the slot
+constexpr int SERVE_SLICES_PER_BLOCK = 16;
 
 hipError_t serve_init(const ServeDev &S, SlotState st, double zero, hipStream_t s);
 // ni: a slot stops with status 0 after its ni-th iteration

@@ -97,7 +97,7 @@ struct ssf_server {
   int max_slots = 0;  // resident requests at most (size of the per-slot state)
   int cap_slots = 0, cap_blk = 0;
   SlotDesc *d_slots = nullptr;
-  int *d_blk_slot = nullptr, *d_blk_t0 = nullptr, *d_blk_t1 = nullptr;
+  int *d_blk_slot = nullptr, *d_blk_s0 = nullptr, *d_blk_s1 = nullptr;
   double *d_part = nullptr;
   SlotState st{};
   int *d_active = nullptr;
@@ -105,6 +105,7 @@ struct ssf_server {
   int nblk = 0;
   // statistics
   int64_t completed = 0, steps = 0;
+  double t_pattern = 0, t_create = 0, t_assemble = 0;  // summed over requests (prep threads)
 
   double now() const
   {
@@ -114,18 +115,38 @@ struct ssf_server {
 
 namespace {
 
-// device bytes of a resident request: full CSR (int32 columns, FP64 values),
-// diagonal, 16-bit tile columns and tiles, six solve vectors
-int64_t resident_bytes(int64_t n, int64_t nnz)
+// entries of the SELL-64 layout of the full matrix of a symcomprow pattern
+int64_t sell_entries(int64_t n, const std::vector<int64_t> &adr, const std::vector<int64_t> &ci)
 {
-  return 4 * (n + 1) + 12 * nnz + 8 * n + 2 * nnz + 8 * (nnz / 1024 + 2) + 48 * n + (1 << 16);
+  std::vector<int> len(n, 0);
+  for (int64_t i = 0; i < n; i++)
+    for (int64_t k = adr[i]; k < adr[i + 1]; k++) {
+      len[i]++;
+      if (ci[k] != i) len[ci[k]]++;
+    }
+  int64_t tot = 0;
+  for (int64_t s0 = 0; s0 < n; s0 += 64) {
+    int w = 0;
+    for (int64_t r = s0; r < std::min<int64_t>(n, s0 + 64); r++) w = std::max(w, len[r]);
+    tot += (int64_t)(w + SELL_V - 1) / SELL_V * SELL_V * 64;
+  }
+  return tot;
 }
 
-// and while it is assembled: reference-layout values and scatter maps, element
-// slot map and colour order
-int64_t prep_bytes(const Request &q, int64_t negm, int64_t nnz)
+// device bytes of a resident request: SELL slices (offsets, 16-bit columns or
+// 32-bit ones, FP64 values), diagonal, six solve vectors
+int64_t resident_bytes(int64_t n, int64_t tot, bool col16)
 {
-  return resident_bytes(q.n, nnz) + 16 * negm + 4 * q.ne * q.ndofe * q.ndofe + 8 * q.ne;
+  return 4 * ((n + 63) / 64 + 1) + (col16 ? 10 : 12) * tot + 8 * n + 48 * n + (1 << 16);
+}
+
+// and while it is built and assembled: full CSR, stream tiles, SELL columns of
+// both widths and its gather map, reference-layout values and scatter maps,
+// element slot map and colour order
+int64_t prep_bytes(const Request &q, int64_t negm, int64_t nnz, int64_t tot)
+{
+  return resident_bytes(q.n, tot, false) + 4 * (q.n + 1) + 14 * nnz + 8 * (nnz / 1024 + 2) +
+         6 * tot + 16 * negm + 4 * q.ne * q.ndofe * q.ndofe + 8 * q.ne;
 }
 
 void release_device(Request *q)
@@ -153,13 +174,16 @@ void prep_main(ssf_server *S, int w)
     int rc = SSF_OK;
     std::string err;
     std::vector<int64_t> adr, ci;
+    const auto tp0 = clk::now();
     rc = pattern_from_elements(SSF_STORAGE_SYMM_COMP_ROWS, q->n, q->ne, q->ndofe, q->cn.data(),
                                adr, ci);
+    const double dt_pattern = std::chrono::duration<double>(clk::now() - tp0).count();
+    double dt_create = 0, dt_assemble = 0;
     if (rc) err = ssf_last_error();
     if (rc == SSF_OK) {
       const int64_t negm = adr[q->n];
       const int64_t nnz = 2 * negm - q->n;
-      q->bytes = prep_bytes(*q, negm, nnz);
+      q->bytes = prep_bytes(*q, negm, nnz, sell_entries(q->n, adr, ci));
       std::unique_lock<std::mutex> lk(S->mu);
       if (q->bytes > S->budget) {
         rc = SSF_ENOMEM;
@@ -173,10 +197,13 @@ void prep_main(ssf_server *S, int w)
       }
     }
     if (rc == SSF_OK) {
-      rc = ssf_matrix_create_csr(ctx, SSF_STORAGE_SYMM_COMP_ROWS, q->n, adr.data(), ci.data(),
-                                 &q->m);
-      if (rc == SSF_OK && q->m->ntiles == 0) {  // a row longer than one stream tile
-        set_error("ssf_server: matrix rows longer than the stream tile are not served");
+      const auto tc0 = clk::now();
+      rc = matrix_create(ctx, SSF_STORAGE_SYMM_COMP_ROWS, q->n, adr.data(), ci.data(), &adr, &ci,
+                         false, &q->m);
+      dt_create = std::chrono::duration<double>(clk::now() - tc0).count();
+      const auto ta0 = clk::now();
+      if (rc == SSF_OK && q->m->nslice == 0) {
+        set_error("ssf_server: the matrix has no SELL layout");
         rc = SSF_EUNSUP;
       }
       std::vector<int64_t>().swap(adr);
@@ -191,8 +218,10 @@ void prep_main(ssf_server *S, int w)
                                  q->pv.empty() ? nullptr : lift.data());
       if (rc == SSF_OK) {
         matrix_drop_assembly(q->m, true);
-        {  // the assembly structures are gone: keep only the resident bytes reserved
-          const int64_t keep = resident_bytes(q->n, q->m->nnz);
+        matrix_keep_sell_only(q->m);
+        dt_assemble = std::chrono::duration<double>(clk::now() - ta0).count();
+        {  // only the SELL solve structures are left: keep their bytes reserved
+          const int64_t keep = resident_bytes(q->n, q->m->sell_tot, q->m->sell_col16 != nullptr);
           std::unique_lock<std::mutex> lk(S->mu);
           if (keep < q->bytes) {
             S->used -= q->bytes - keep;
@@ -225,6 +254,9 @@ void prep_main(ssf_server *S, int w)
     }
     std::unique_lock<std::mutex> lk(S->mu);
     S->preparing--;
+    S->t_pattern += dt_pattern;
+    S->t_create += dt_create;
+    S->t_assemble += dt_assemble;
     if (rc != SSF_OK) {
       lk.unlock();
       release_device(q);
@@ -258,14 +290,14 @@ hipError_t ensure_tables(ssf_server *S, int nslot, int nblk)
   }
   if (nblk > S->cap_blk) {
     sfree(S->d_blk_slot);
-    sfree(S->d_blk_t0);
-    sfree(S->d_blk_t1);
+    sfree(S->d_blk_s0);
+    sfree(S->d_blk_s1);
     sfree(S->d_part);
     int cap = std::max(1024, S->cap_blk);
     while (cap < nblk) cap *= 2;
     if ((e = salloc(&S->d_blk_slot, cap)) != hipSuccess) return e;
-    if ((e = salloc(&S->d_blk_t0, cap)) != hipSuccess) return e;
-    if ((e = salloc(&S->d_blk_t1, cap)) != hipSuccess) return e;
+    if ((e = salloc(&S->d_blk_s0, cap)) != hipSuccess) return e;
+    if ((e = salloc(&S->d_blk_s1, cap)) != hipSuccess) return e;
     if ((e = salloc(&S->d_part, 2 * (size_t)cap)) != hipSuccess) return e;
     S->cap_blk = cap;
   }
@@ -281,9 +313,10 @@ hipError_t upload_tables(ssf_server *S, const std::vector<char> &fresh)
     Request *q = S->active[k];
     const ssf_matrix *m = q->m;
     SlotDesc &D = desc[k];
-    D.rowptr = m->rowptr;
-    D.col = m->col;
-    D.val = m->val;
+    D.soff = m->sell_off;
+    D.scol = m->sell_col;
+    D.scol16 = m->sell_col16;
+    D.sval = m->sell_v64;
     D.diag = m->diag;
     D.y = q->vec;
     D.x = q->vec + q->n;
@@ -291,19 +324,15 @@ hipError_t upload_tables(ssf_server *S, const std::vector<char> &fresh)
     D.d = q->vec + 3 * q->n;
     D.p = q->vec + 4 * q->n;
     D.h = q->vec + 5 * q->n;
-    D.tiles = m->tiles;
     D.n = (int)q->n;
     D.sid = q->sid;
     D.fresh = fresh[k];
-    // lanes per row from the mean row length (depends on the matrix only)
-    const double avg = q->n ? (double)m->nnz / (double)q->n : 1.0;
-    D.lanes = avg <= 32.0 ? 4 : (avg <= 96.0 ? 8 : 16);
     D.blk0 = (int)bs.size();
-    const int nt = m->ntiles;
-    for (int t = 0; t < nt; t += SERVE_TILES_PER_BLOCK) {
+    const int ns = m->nslice;
+    for (int t = 0; t < ns; t += SERVE_SLICES_PER_BLOCK) {
       bs.push_back((int)k);
       b0.push_back(t);
-      b1.push_back(std::min(nt, t + SERVE_TILES_PER_BLOCK));
+      b1.push_back(std::min(ns, t + SERVE_SLICES_PER_BLOCK));
     }
     D.nblk = (int)bs.size() - D.blk0;
   }
@@ -317,9 +346,9 @@ hipError_t upload_tables(ssf_server *S, const std::vector<char> &fresh)
     if (e == hipSuccess)
       e = hipMemcpyAsync(S->d_blk_slot, bs.data(), bs.size() * sizeof(int), hipMemcpyHostToDevice, s);
     if (e == hipSuccess)
-      e = hipMemcpyAsync(S->d_blk_t0, b0.data(), b0.size() * sizeof(int), hipMemcpyHostToDevice, s);
+      e = hipMemcpyAsync(S->d_blk_s0, b0.data(), b0.size() * sizeof(int), hipMemcpyHostToDevice, s);
     if (e == hipSuccess)
-      e = hipMemcpyAsync(S->d_blk_t1, b1.data(), b1.size() * sizeof(int), hipMemcpyHostToDevice, s);
+      e = hipMemcpyAsync(S->d_blk_s1, b1.data(), b1.size() * sizeof(int), hipMemcpyHostToDevice, s);
     // the host vectors go out of scope: wait for the copies
     if (e == hipSuccess) e = hipStreamSynchronize(s);
   }
@@ -333,8 +362,8 @@ ServeDev serve_view(ssf_server *S)
   v.nblk = S->nblk;
   v.slots = S->d_slots;
   v.blk_slot = S->d_blk_slot;
-  v.blk_t0 = S->d_blk_t0;
-  v.blk_t1 = S->d_blk_t1;
+  v.blk_s0 = S->d_blk_s0;
+  v.blk_s1 = S->d_blk_s1;
   v.part = S->d_part;
   return v;
 }
@@ -472,7 +501,7 @@ void free_all(ssf_server *S)
   sfree(t.iter); sfree(t.ani); sfree(t.done);
   sfree(t.status); sfree(t.pend); sfree(t.pend_status); sfree(t.brk);
   sfree(S->d_active);
-  sfree(S->d_slots); sfree(S->d_blk_slot); sfree(S->d_blk_t0); sfree(S->d_blk_t1);
+  sfree(S->d_slots); sfree(S->d_blk_slot); sfree(S->d_blk_s0); sfree(S->d_blk_s1);
   sfree(S->d_part);
   if (S->h_done) (void)hipHostFree(S->h_done);
   S->h_done = nullptr;
@@ -619,7 +648,9 @@ extern "C" int ssf_server_wait(ssf_server *S, int64_t ticket, double *x, ssf_sol
   return SSF_OK;
 }
 
-// stats[4]: completed requests, loop steps, peak reserved bytes, bytes now
+// stats[7]: completed requests, loop steps, peak reserved bytes, bytes now, and
+// the microseconds the prep threads spent on patterns, on matrix set-up (CSR,
+// SELL, upload) and on device assembly, summed over requests
 extern "C" int ssf_server_stats(ssf_server *S, int64_t *stats)
 {
   SSF_ENTER();
@@ -629,5 +660,8 @@ extern "C" int ssf_server_stats(ssf_server *S, int64_t *stats)
   stats[1] = S->steps;
   stats[2] = S->peak;
   stats[3] = S->used;
+  stats[4] = (int64_t)(1e6 * S->t_pattern);
+  stats[5] = (int64_t)(1e6 * S->t_create);
+  stats[6] = (int64_t)(1e6 * S->t_assemble);
   return SSF_OK;
 }

@@ -20,6 +20,8 @@
 // Scalars are re-reduced from the per-block partials by every consumer block in
 // the same order, so all blocks take the same decision without inter-block
 // synchronisation.
+#include <hip/hip_ext.h>
+
 #include "ssf_device.hpp"
 
 namespace ssf {
@@ -194,19 +196,26 @@ __global__ __launch_bounds__(TPB) void k_spmv_sell(CsrDev A, const T *__restrict
   }
 }
 
+// ev0/ev1 (both or neither): the launch's own start and end timestamps
+// (hipExtLaunchKernelGGL), i.e. the kernel's duration without the dispatch gap
 template <typename T, int KIND>
 static hipError_t spmv_k(const SpmvTune &t, const CsrDev &A, const T *val, const T *x, T *y,
                          const T *b, T *partial, const int *done, long long *iter,
-                         hipStream_t s)
+                         hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
 {
   dim3 grid(t.blocks), block(TPB);
-#define SSF_VEC(G) hipLaunchKernelGGL((k_spmv_vec<T, G, KIND>), grid, block, 0, s, A, val, x, y, b, partial, done, iter)
-#define SSF_STR(I, G) hipLaunchKernelGGL((k_spmv_stream<T, I, G, KIND>), grid, block, 0, s, A, val, x, y, b, partial, done, iter)
+#define SSF_LAUNCH(KERN, ...)                                                      \
+  do {                                                                             \
+    if (ev0) hipExtLaunchKernelGGL(KERN, grid, block, 0, s, ev0, ev1, 0, __VA_ARGS__); \
+    else hipLaunchKernelGGL(KERN, grid, block, 0, s, __VA_ARGS__);                 \
+  } while (0)
+#define SSF_VEC(G) SSF_LAUNCH((k_spmv_vec<T, G, KIND>), A, val, x, y, b, partial, done, iter)
+#define SSF_STR(I, G) SSF_LAUNCH((k_spmv_stream<T, I, G, KIND>), A, val, x, y, b, partial, done, iter)
   if (t.format == 2 && A.nslice > 0) {
     if (t.idx16 && A.sell_col16)
-      hipLaunchKernelGGL((k_spmv_sell<T, true, KIND>), grid, block, 0, s, A, x, y, b, partial, done, iter);
+      SSF_LAUNCH((k_spmv_sell<T, true, KIND>), A, x, y, b, partial, done, iter);
     else
-      hipLaunchKernelGGL((k_spmv_sell<T, false, KIND>), grid, block, 0, s, A, x, y, b, partial, done, iter);
+      SSF_LAUNCH((k_spmv_sell<T, false, KIND>), A, x, y, b, partial, done, iter);
   } else if (t.format == 1 && A.ntiles > 0) {
     const bool i16 = t.idx16 && A.col16;
     switch (t.lanes) {
@@ -225,23 +234,25 @@ static hipError_t spmv_k(const SpmvTune &t, const CsrDev &A, const T *val, const
   }
 #undef SSF_VEC
 #undef SSF_STR
+#undef SSF_LAUNCH
   return hipGetLastError();
 }
 
 template <typename T>
 hipError_t spmv(const SpmvTune &t, int kind, const CsrDev &A, const T *val, const T *x, T *y,
-                const T *b, T *partial, const int *done, long long *iter, hipStream_t s)
+                const T *b, T *partial, const int *done, long long *iter, hipStream_t s,
+                hipEvent_t ev0, hipEvent_t ev1)
 {
-  if (kind == 0) return spmv_k<T, 0>(t, A, val, x, y, b, partial, done, iter, s);
-  if (kind == 1) return spmv_k<T, 1>(t, A, val, x, y, b, partial, done, iter, s);
-  return spmv_k<T, 2>(t, A, val, x, y, b, partial, done, iter, s);
+  if (kind == 0) return spmv_k<T, 0>(t, A, val, x, y, b, partial, done, iter, s, ev0, ev1);
+  if (kind == 1) return spmv_k<T, 1>(t, A, val, x, y, b, partial, done, iter, s, ev0, ev1);
+  return spmv_k<T, 2>(t, A, val, x, y, b, partial, done, iter, s, ev0, ev1);
 }
 template hipError_t spmv<double>(const SpmvTune &, int, const CsrDev &, const double *,
                                  const double *, double *, const double *, double *,
-                                 const int *, long long *, hipStream_t);
+                                 const int *, long long *, hipStream_t, hipEvent_t, hipEvent_t);
 template hipError_t spmv<float>(const SpmvTune &, int, const CsrDev &, const float *,
                                 const float *, float *, const float *, float *, const int *,
-                                long long *, hipStream_t);
+                                long long *, hipStream_t, hipEvent_t, hipEvent_t);
 
 // ----------------------------------------------------------------------- PCG
 
@@ -436,10 +447,8 @@ hipError_t pcg_iteration(const SpmvTune &t, int mode, bool jac, const CsrDev &A,
                          PcgState<T> *st, T res, T zero, hipStream_t s, hipEvent_t ev0,
                          hipEvent_t ev1)
 {
-  if (ev0) (void)hipEventRecord(ev0, s);
-  hipError_t e = spmv<T>(t, 1, A, val, d, p, nullptr, part, &st->done, &st->iter, s);
+  hipError_t e = spmv<T>(t, 1, A, val, d, p, nullptr, part, &st->done, &st->iter, s, ev0, ev1);
   if (e != hipSuccess) return e;
-  if (ev1) (void)hipEventRecord(ev1, s);
   const int nv = vec_blocks(t, A.n);
   dim3 grid(nv), block(TPB);
   T *prr = part + t.blocks;
