@@ -51,18 +51,22 @@ static void parallel_tasks(int T, F f)
   for (auto &x : th) x.join();
 }
 
+// Device memory of matrices and workspaces is stream-ordered (hipMallocAsync /
+// hipFreeAsync on the owning context's stream): releasing it never waits for
+// work on other streams, e.g. a served request's solve while another request
+// is being assembled.
 template <typename T>
-static hipError_t dalloc(T **p, size_t count)
+static hipError_t dalloc(T **p, size_t count, hipStream_t s)
 {
   *p = nullptr;
   if (count == 0) count = 1;
-  return hipMalloc((void **)p, count * sizeof(T));
+  return hipMallocAsync((void **)p, count * sizeof(T), s);
 }
 
 template <typename T>
-static void dfree(T *&p)
+static void dfree(T *&p, hipStream_t s)
 {
-  if (p) (void)hipFree(p);
+  if (p) (void)hipFreeAsync(p, s);
   p = nullptr;
 }
 
@@ -299,6 +303,7 @@ extern "C" int ssf_ctx_destroy(ssf_ctx *ctx)
             "ssf_ctx_destroy: matrices created on this context are still alive");
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
+  (void)hipStreamSynchronize(ctx->stream);  // stream-ordered frees complete first
   (void)hipStreamDestroy(ctx->stream);
   for (int a = 0; a < 2; a++)
     for (int b = 0; b < 2; b++)
@@ -317,11 +322,11 @@ extern "C" int ssf_ctx_sync(ssf_ctx *ctx)
 
 extern "C" void *ssf_ctx_stream(ssf_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
 
-static void free_workspace(Workspace &w)
+static void free_workspace(Workspace &w, hipStream_t s)
 {
-  dfree(w.x); dfree(w.r); dfree(w.d); dfree(w.p); dfree(w.h); dfree(w.y);
-  dfree(w.x32); dfree(w.r32); dfree(w.d32); dfree(w.p32); dfree(w.h32); dfree(w.y32);
-  dfree(w.part); dfree(w.part32); dfree(w.st); dfree(w.st32); dfree(w.mix);
+  dfree(w.x, s); dfree(w.r, s); dfree(w.d, s); dfree(w.p, s); dfree(w.h, s); dfree(w.y, s);
+  dfree(w.x32, s); dfree(w.r32, s); dfree(w.d32, s); dfree(w.p32, s); dfree(w.h32, s); dfree(w.y32, s);
+  dfree(w.part, s); dfree(w.part32, s); dfree(w.st, s); dfree(w.st32, s); dfree(w.mix, s);
   if (w.mix_host) (void)hipHostFree(w.mix_host);
   w.mix_host = nullptr;
   if (w.st_host) (void)hipHostFree(w.st_host);
@@ -336,17 +341,18 @@ extern "C" int ssf_matrix_destroy(ssf_matrix *m)
   SSF_ENTER();
   if (!m) return SSF_OK;
   (void)hipSetDevice(m->ctx->device);
-  (void)hipStreamSynchronize(m->ctx->stream);
+  // the frees are ordered after the work queued on the matrix's stream; work the
+  // caller queued on other streams must have finished (as for any release)
   if (m->a_ref == m->val) m->a_ref = nullptr;  // comprow: the same allocation
-  dfree(m->a_ref);
-  dfree(m->rowptr); dfree(m->col); dfree(m->val); dfree(m->val32);
-  dfree(m->diag); dfree(m->diag32); dfree(m->norm_a);
-  dfree(m->map_lo); dfree(m->map_up);
-  dfree(m->slot); dfree(m->order);
-  dfree(m->tiles); dfree(m->col16);
-  dfree(m->sell_off); dfree(m->sell_col); dfree(m->sell_src); dfree(m->sell_v64); dfree(m->sell_v32);
-  dfree(m->sell_col16);
-  free_workspace(m->ws);
+  dfree(m->a_ref, m->ctx->stream);
+  dfree(m->rowptr, m->ctx->stream); dfree(m->col, m->ctx->stream); dfree(m->val, m->ctx->stream); dfree(m->val32, m->ctx->stream);
+  dfree(m->diag, m->ctx->stream); dfree(m->diag32, m->ctx->stream); dfree(m->norm_a, m->ctx->stream);
+  dfree(m->map_lo, m->ctx->stream); dfree(m->map_up, m->ctx->stream);
+  dfree(m->slot, m->ctx->stream); dfree(m->order, m->ctx->stream);
+  dfree(m->tiles, m->ctx->stream); dfree(m->col16, m->ctx->stream);
+  dfree(m->sell_off, m->ctx->stream); dfree(m->sell_col, m->ctx->stream); dfree(m->sell_src, m->ctx->stream); dfree(m->sell_v64, m->ctx->stream); dfree(m->sell_v32, m->ctx->stream);
+  dfree(m->sell_col16, m->ctx->stream);
+  free_workspace(m->ws, m->ctx->stream);
   m->ctx->live_matrices--;
   delete m;
   return SSF_OK;
@@ -355,8 +361,8 @@ extern "C" int ssf_matrix_destroy(ssf_matrix *m)
 void ssf::matrix_drop_assembly(ssf_matrix *m, bool host_pattern)
 {
   (void)hipStreamSynchronize(m->ctx->stream);
-  dfree(m->slot);
-  dfree(m->order);
+  dfree(m->slot, m->ctx->stream);
+  dfree(m->order, m->ctx->stream);
   m->asm_ne = 0;
   m->asm_ndofe = 0;
   m->colour_start.clear();
@@ -366,24 +372,24 @@ void ssf::matrix_drop_assembly(ssf_matrix *m, bool host_pattern)
     std::vector<int64_t>().swap(m->adr);
     std::vector<int64_t>().swap(m->ci);
     if (m->a_ref == m->val) m->a_ref = nullptr;
-    dfree(m->a_ref);
-    dfree(m->map_lo);
-    dfree(m->map_up);
+    dfree(m->a_ref, m->ctx->stream);
+    dfree(m->map_lo, m->ctx->stream);
+    dfree(m->map_up, m->ctx->stream);
   }
 }
 
 void ssf::matrix_keep_sell_only(ssf_matrix *m)
 {
   (void)hipStreamSynchronize(m->ctx->stream);
-  dfree(m->col);
-  dfree(m->val);
-  dfree(m->val32);
-  dfree(m->diag32);
-  dfree(m->tiles);
-  dfree(m->col16);
-  dfree(m->sell_src);
-  dfree(m->sell_v32);
-  if (m->sell_col16) dfree(m->sell_col);
+  dfree(m->col, m->ctx->stream);
+  dfree(m->val, m->ctx->stream);
+  dfree(m->val32, m->ctx->stream);
+  dfree(m->diag32, m->ctx->stream);
+  dfree(m->tiles, m->ctx->stream);
+  dfree(m->col16, m->ctx->stream);
+  dfree(m->sell_src, m->ctx->stream);
+  dfree(m->sell_v32, m->ctx->stream);
+  if (m->sell_col16) dfree(m->sell_col, m->ctx->stream);
   m->ntiles = 0;
   m->val32_valid = false;
 }
@@ -549,54 +555,54 @@ int ssf::matrix_create(ssf_ctx *ctx, int storage, int64_t n, const int64_t *adr,
   const bool have_sell = n > 0 && build_sell(n, rowptr, col, soff, scol, ssrc, scol16);
   hipError_t e = hipSuccess;
   auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
-  chk(dalloc(&m->rowptr, n + 1));
-  chk(dalloc(&m->col, m->nnz));
-  chk(dalloc(&m->val, m->nnz));
-  chk(dalloc(&m->diag, n));
+  chk(dalloc(&m->rowptr, n + 1, ctx->stream));
+  chk(dalloc(&m->col, m->nnz, ctx->stream));
+  chk(dalloc(&m->val, m->nnz, ctx->stream));
+  chk(dalloc(&m->diag, n, ctx->stream));
   if (have_tiles) {
     m->ntiles = (int)tiles.size() - 1;
-    chk(dalloc(&m->tiles, tiles.size()));
+    chk(dalloc(&m->tiles, tiles.size(), ctx->stream));
     if (e == hipSuccess)
-      chk(hipMemcpy(m->tiles, tiles.data(), tiles.size() * sizeof(int), hipMemcpyHostToDevice));
+      chk(hipMemcpyAsync(m->tiles, tiles.data(), tiles.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
     if (!col16.empty()) {
-      chk(dalloc(&m->col16, col16.size()));
+      chk(dalloc(&m->col16, col16.size(), ctx->stream));
       if (e == hipSuccess)
-        chk(hipMemcpy(m->col16, col16.data(), col16.size() * sizeof(short), hipMemcpyHostToDevice));
+        chk(hipMemcpyAsync(m->col16, col16.data(), col16.size() * sizeof(short), hipMemcpyHostToDevice, ctx->stream));
     }
   }
   if (have_sell) {
     m->nslice = (int)soff.size() - 1;
     m->sell_tot = soff.back();
-    chk(dalloc(&m->sell_off, soff.size()));
-    chk(dalloc(&m->sell_col, m->sell_tot));
-    chk(dalloc(&m->sell_src, m->sell_tot));
-    chk(dalloc(&m->sell_v64, m->sell_tot));
+    chk(dalloc(&m->sell_off, soff.size(), ctx->stream));
+    chk(dalloc(&m->sell_col, m->sell_tot, ctx->stream));
+    chk(dalloc(&m->sell_src, m->sell_tot, ctx->stream));
+    chk(dalloc(&m->sell_v64, m->sell_tot, ctx->stream));
     if (e == hipSuccess) {
-      chk(hipMemcpy(m->sell_off, soff.data(), soff.size() * sizeof(int), hipMemcpyHostToDevice));
-      chk(hipMemcpy(m->sell_col, scol.data(), scol.size() * sizeof(int), hipMemcpyHostToDevice));
-      chk(hipMemcpy(m->sell_src, ssrc.data(), ssrc.size() * sizeof(int), hipMemcpyHostToDevice));
+      chk(hipMemcpyAsync(m->sell_off, soff.data(), soff.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+      chk(hipMemcpyAsync(m->sell_col, scol.data(), scol.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+      chk(hipMemcpyAsync(m->sell_src, ssrc.data(), ssrc.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
       if (!scol16.empty()) {
-        chk(dalloc(&m->sell_col16, m->sell_tot));
+        chk(dalloc(&m->sell_col16, m->sell_tot, ctx->stream));
         if (e == hipSuccess)
-          chk(hipMemcpy(m->sell_col16, scol16.data(), scol16.size() * sizeof(short),
-                        hipMemcpyHostToDevice));
+          chk(hipMemcpyAsync(m->sell_col16, scol16.data(), scol16.size() * sizeof(short),
+                             hipMemcpyHostToDevice, ctx->stream));
       }
       chk(hipMemsetAsync(m->sell_v64, 0, std::max<int64_t>(1, m->sell_tot) * sizeof(double), ctx->stream));
     }
   }
   if (sym) {
-    chk(dalloc(&m->a_ref, negm));
-    chk(dalloc(&m->map_lo, negm));
-    chk(dalloc(&m->map_up, negm));
+    chk(dalloc(&m->a_ref, negm, ctx->stream));
+    chk(dalloc(&m->map_lo, negm, ctx->stream));
+    chk(dalloc(&m->map_up, negm, ctx->stream));
   } else {
     m->a_ref = m->val;
   }
   if (e == hipSuccess) {
-    chk(hipMemcpy(m->rowptr, rowptr.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice));
-    if (m->nnz) chk(hipMemcpy(m->col, col.data(), m->nnz * sizeof(int), hipMemcpyHostToDevice));
+    chk(hipMemcpyAsync(m->rowptr, rowptr.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    if (m->nnz) chk(hipMemcpyAsync(m->col, col.data(), m->nnz * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
     if (sym && negm) {
-      chk(hipMemcpy(m->map_lo, lo.data(), negm * sizeof(int), hipMemcpyHostToDevice));
-      chk(hipMemcpy(m->map_up, up.data(), negm * sizeof(int), hipMemcpyHostToDevice));
+      chk(hipMemcpyAsync(m->map_lo, lo.data(), negm * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+      chk(hipMemcpyAsync(m->map_up, up.data(), negm * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
     }
     chk(hipMemsetAsync(m->a_ref, 0, std::max<int64_t>(1, negm) * sizeof(double), ctx->stream));
     chk(hipMemsetAsync(m->val, 0, std::max<int64_t>(1, m->nnz) * sizeof(double), ctx->stream));
@@ -762,10 +768,13 @@ static int prepare_assembly(ssf_matrix *m, int64_t nn, int64_t ne, int nne, int 
   });
   SSF_CHECK(!too_large.load(), SSF_EINVAL, "code number larger than the number of unknowns");
   int *d_cn = nullptr;
-  SSF_HIP(dalloc(&d_cn, ncn));
-  SSF_HIP(hipMemcpy(d_cn, cn32.data(), ncn * sizeof(int), hipMemcpyHostToDevice));
+  SSF_HIP(dalloc(&d_cn, ncn, s));
+  SSF_HIP(hipMemcpyAsync(d_cn, cn32.data(), ncn * sizeof(int), hipMemcpyHostToDevice, s));
   *d_cn_out = d_cn;
-  if (same) return SSF_OK;
+  if (same) {
+    SSF_HIP(hipStreamSynchronize(s));  // cn32 is released on return
+    return SSF_OK;
+  }
   std::vector<int> order;
   std::vector<int64_t> start;
   int rc;
@@ -776,15 +785,15 @@ static int prepare_assembly(ssf_matrix *m, int64_t nn, int64_t ne, int nne, int 
     rc = colour_elements(m->n + 1, ne, nd, c2.data(), order, start);
   }
   if (rc) {
-    dfree(d_cn);
+    dfree(d_cn, s);
     *d_cn_out = nullptr;
     return rc;
   }
-  dfree(m->slot);
-  dfree(m->order);
-  SSF_HIP(dalloc(&m->order, ne));
-  SSF_HIP(hipMemcpy(m->order, order.data(), ne * sizeof(int), hipMemcpyHostToDevice));
-  SSF_HIP(dalloc(&m->slot, ne * nd * nd));
+  dfree(m->slot, s);
+  dfree(m->order, s);
+  SSF_HIP(dalloc(&m->order, ne, s));
+  SSF_HIP(hipMemcpyAsync(m->order, order.data(), ne * sizeof(int), hipMemcpyHostToDevice, s));
+  SSF_HIP(dalloc(&m->slot, ne * nd * nd, s));
   // reference-layout pattern for the search
   std::vector<int> adr32(m->n + 1), ci32(m->negm);
   parallel_for(m->n + 1, [&](int64_t b, int64_t e) {
@@ -794,14 +803,15 @@ static int prepare_assembly(ssf_matrix *m, int64_t nn, int64_t ne, int nne, int 
     for (int64_t k = b; k < e; k++) ci32[k] = (int)m->ci[k];
   });
   int *d_adr = nullptr, *d_ci = nullptr;
-  SSF_HIP(dalloc(&d_adr, m->n + 1));
-  SSF_HIP(dalloc(&d_ci, m->negm));
-  SSF_HIP(hipMemcpy(d_adr, adr32.data(), (m->n + 1) * sizeof(int), hipMemcpyHostToDevice));
-  if (m->negm) SSF_HIP(hipMemcpy(d_ci, ci32.data(), m->negm * sizeof(int), hipMemcpyHostToDevice));
+  SSF_HIP(dalloc(&d_adr, m->n + 1, s));
+  SSF_HIP(dalloc(&d_ci, m->negm, s));
+  SSF_HIP(hipMemcpyAsync(d_adr, adr32.data(), (m->n + 1) * sizeof(int), hipMemcpyHostToDevice, s));
+  if (m->negm)
+    SSF_HIP(hipMemcpyAsync(d_ci, ci32.data(), m->negm * sizeof(int), hipMemcpyHostToDevice, s));
   SSF_HIP(build_slots(m->storage, aos ? 1 : 0, ne, nd, m->order, d_cn, d_adr, d_ci, m->slot, s));
   SSF_HIP(hipStreamSynchronize(s));
-  dfree(d_adr);
-  dfree(d_ci);
+  dfree(d_adr, s);
+  dfree(d_ci, s);
   m->asm_ne = ne;
   m->asm_ndofe = nd;
   m->colour_start = start;
@@ -838,13 +848,13 @@ extern "C" int ssf_matrix_assemble(ssf_matrix *m, int kind, int64_t nn, const do
   double *d_xyz = nullptr, *d_a = nullptr, *d_b = nullptr, *d_t = nullptr, *d_pv = nullptr, *d_lift = nullptr;
   hipError_t e = hipSuccess;
   auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
-  chk(dalloc(&d_conn, ne * nne));
-  chk(dalloc(&d_xyz, nn * 3));
-  chk(dalloc(&d_a, ne));
-  if (mat_b) chk(dalloc(&d_b, ne));
-  if (thick) chk(dalloc(&d_t, ne));
-  if (pv && npv) chk(dalloc(&d_pv, npv));
-  if (rhs_lift) chk(dalloc(&d_lift, m->n));
+  chk(dalloc(&d_conn, ne * nne, s));
+  chk(dalloc(&d_xyz, nn * 3, s));
+  chk(dalloc(&d_a, ne, s));
+  if (mat_b) chk(dalloc(&d_b, ne, s));
+  if (thick) chk(dalloc(&d_t, ne, s));
+  if (pv && npv) chk(dalloc(&d_pv, npv, s));
+  if (rhs_lift) chk(dalloc(&d_lift, m->n, s));
   if (e == hipSuccess) {
     chk(hipMemcpyAsync(d_conn, conn32.data(), ne * nne * sizeof(int), hipMemcpyHostToDevice, s));
     chk(hipMemcpyAsync(d_xyz, xyz, nn * 3 * sizeof(double), hipMemcpyHostToDevice, s));
@@ -864,8 +874,8 @@ extern "C" int ssf_matrix_assemble(ssf_matrix *m, int kind, int64_t nn, const do
   if (e == hipSuccess && rhs_lift)
     chk(hipMemcpyAsync(rhs_lift, d_lift, m->n * sizeof(double), hipMemcpyDeviceToHost, s));
   chk(hipStreamSynchronize(s));
-  dfree(d_conn); dfree(d_xyz); dfree(d_a); dfree(d_b); dfree(d_t); dfree(d_pv); dfree(d_lift);
-  dfree(d_cn);
+  dfree(d_conn, s); dfree(d_xyz, s); dfree(d_a, s); dfree(d_b, s); dfree(d_t, s); dfree(d_pv, s); dfree(d_lift, s);
+  dfree(d_cn, s);
   SSF_HIP(e);
   return SSF_OK;
 }
@@ -883,15 +893,15 @@ extern "C" int ssf_matrix_localize_batch(ssf_matrix *m, int64_t ne, int64_t nc, 
   int rc = prepare_assembly(m, 0, ne, (int)nc, (int)nc, nullptr, cn, false, &d_cn);
   if (rc) return rc;
   double *d_lm = nullptr;
-  SSF_HIP(dalloc(&d_lm, ne * nc * nc));
+  SSF_HIP(dalloc(&d_lm, ne * nc * nc, s));
   SSF_HIP(hipMemcpyAsync(d_lm, lm, ne * nc * nc * sizeof(double), hipMemcpyHostToDevice, s));
   for (size_t c = 0; c + 1 < m->colour_start.size(); c++)
     SSF_HIP(localize_elements(m->colour_start[c], m->colour_start[c + 1], ne, (int)nc, m->order,
                               m->slot, d_lm, m->a_ref, s));
   rc = values_changed(m);
   SSF_HIP(hipStreamSynchronize(s));
-  dfree(d_lm);
-  dfree(d_cn);
+  dfree(d_lm, s);
+  dfree(d_cn, s);
   return rc;
 }
 
@@ -903,34 +913,34 @@ static int ensure_ws(ssf_matrix *m, bool mixed)
   const int64_t n = std::max<int64_t>(1, m->n);
   const int nb = std::max(m->tune64.blocks, m->tune32.blocks);
   if (w.n != n || w.nb < nb) {
-    free_workspace(w);
+    free_workspace(w, m->ctx->stream);
     w.n = n;
     w.nb = std::max(nb, 4096);
-    SSF_HIP(dalloc(&w.x, n)); SSF_HIP(dalloc(&w.r, n)); SSF_HIP(dalloc(&w.d, n));
-    SSF_HIP(dalloc(&w.p, n)); SSF_HIP(dalloc(&w.h, n)); SSF_HIP(dalloc(&w.y, n));
-    SSF_HIP(dalloc(&w.part, 4 * (size_t)w.nb));
-    SSF_HIP(dalloc(&w.st, 1));
+    SSF_HIP(dalloc(&w.x, n, m->ctx->stream)); SSF_HIP(dalloc(&w.r, n, m->ctx->stream)); SSF_HIP(dalloc(&w.d, n, m->ctx->stream));
+    SSF_HIP(dalloc(&w.p, n, m->ctx->stream)); SSF_HIP(dalloc(&w.h, n, m->ctx->stream)); SSF_HIP(dalloc(&w.y, n, m->ctx->stream));
+    SSF_HIP(dalloc(&w.part, 4 * (size_t)w.nb, m->ctx->stream));
+    SSF_HIP(dalloc(&w.st, 1, m->ctx->stream));
     SSF_HIP(hipHostMalloc((void **)&w.st_host, 2 * sizeof(PcgState<double>), hipHostMallocDefault));
   }
   if (mixed && !w.x32) {
-    SSF_HIP(dalloc(&w.x32, n)); SSF_HIP(dalloc(&w.r32, n)); SSF_HIP(dalloc(&w.d32, n));
-    SSF_HIP(dalloc(&w.p32, n)); SSF_HIP(dalloc(&w.h32, n)); SSF_HIP(dalloc(&w.y32, n));
-    SSF_HIP(dalloc(&w.part32, 4 * (size_t)w.nb));
-    SSF_HIP(dalloc(&w.st32, 1));
+    SSF_HIP(dalloc(&w.x32, n, m->ctx->stream)); SSF_HIP(dalloc(&w.r32, n, m->ctx->stream)); SSF_HIP(dalloc(&w.d32, n, m->ctx->stream));
+    SSF_HIP(dalloc(&w.p32, n, m->ctx->stream)); SSF_HIP(dalloc(&w.h32, n, m->ctx->stream)); SSF_HIP(dalloc(&w.y32, n, m->ctx->stream));
+    SSF_HIP(dalloc(&w.part32, 4 * (size_t)w.nb, m->ctx->stream));
+    SSF_HIP(dalloc(&w.st32, 1, m->ctx->stream));
     SSF_HIP(hipHostMalloc((void **)&w.st32_host, 2 * sizeof(PcgState<float>), hipHostMallocDefault));
-    SSF_HIP(dalloc(&w.mix, 1));
+    SSF_HIP(dalloc(&w.mix, 1, m->ctx->stream));
     SSF_HIP(hipHostMalloc((void **)&w.mix_host, 2 * sizeof(MixState), hipHostMallocDefault));
   }
   if (mixed && !m->val32_valid) {
-    if (!m->val32) SSF_HIP(dalloc(&m->val32, m->nnz));
-    if (!m->diag32) SSF_HIP(dalloc(&m->diag32, m->n));
+    if (!m->val32) SSF_HIP(dalloc(&m->val32, m->nnz, m->ctx->stream));
+    if (!m->diag32) SSF_HIP(dalloc(&m->diag32, m->n, m->ctx->stream));
     SSF_HIP(to_f32(m->nnz, m->val, m->val32, 1.0, m->ctx->stream));
     SSF_HIP(to_f32(m->n, m->diag, m->diag32, 1.0, m->ctx->stream));
     if (m->sell_v64) {
-      if (!m->sell_v32) SSF_HIP(dalloc(&m->sell_v32, m->sell_tot));
+      if (!m->sell_v32) SSF_HIP(dalloc(&m->sell_v32, m->sell_tot, m->ctx->stream));
       SSF_HIP(to_f32(m->sell_tot, m->sell_v64, m->sell_v32, 1.0, m->ctx->stream));
     }
-    if (!m->norm_a) SSF_HIP(dalloc(&m->norm_a, 1));
+    if (!m->norm_a) SSF_HIP(dalloc(&m->norm_a, 1, m->ctx->stream));
     SSF_HIP(row_absmax((int)m->n, m->rowptr, m->val, w.part, 1024, m->norm_a, m->ctx->stream));
     m->val32_valid = true;
   }
@@ -1274,8 +1284,8 @@ extern "C" int ssf_solve(ssf_matrix *m, const ssf_solve_params *p, const double 
   if (rc) return rc;
   hipStream_t s = m->ctx->stream;
   double *y = nullptr, *x = nullptr;
-  SSF_HIP(dalloc(&y, m->n));
-  SSF_HIP(dalloc(&x, m->n));
+  SSF_HIP(dalloc(&y, m->n, s));
+  SSF_HIP(dalloc(&x, m->n, s));
   SSF_HIP(hipMemcpyAsync(y, rhs, m->n * sizeof(double), hipMemcpyHostToDevice, s));
   if (p->iv) SSF_HIP(hipMemcpyAsync(x, lhs, m->n * sizeof(double), hipMemcpyHostToDevice, s));
   rc = ssf_solve_dev(m, p, y, x, info);
@@ -1283,8 +1293,8 @@ extern "C" int ssf_solve(ssf_matrix *m, const ssf_solve_params *p, const double 
     SSF_HIP(hipMemcpyAsync(lhs, x, m->n * sizeof(double), hipMemcpyDeviceToHost, s));
     SSF_HIP(hipStreamSynchronize(s));
   }
-  dfree(y);
-  dfree(x);
+  dfree(y, s);
+  dfree(x, s);
   return rc;
 }
 
@@ -1436,14 +1446,14 @@ extern "C" int ssf_copy_bench(ssf_ctx *ctx, int64_t bytes, int reps, double *use
   SSF_CHECK(ctx && usec && reps > 0 && bytes >= 16, SSF_EINVAL, "ssf_copy_bench: bad argument");
   SSF_HIP(hipSetDevice(ctx->device));
   const int64_t n = bytes / 16 * 2;  // doubles, a multiple of 2
+  hipStream_t s = ctx->stream;
   double *a = nullptr, *b = nullptr;
-  SSF_HIP(dalloc(&a, n));
-  hipError_t e = dalloc(&b, n);
+  SSF_HIP(dalloc(&a, n, s));
+  hipError_t e = dalloc(&b, n, s);
   if (e != hipSuccess) {
-    dfree(a);
+    dfree(a, s);
     SSF_HIP(e);
   }
-  hipStream_t s = ctx->stream;
   hipEvent_t t0 = nullptr, t1 = nullptr;
   auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
   chk(fill(n, 1.0, a, s));
@@ -1461,8 +1471,8 @@ extern "C" int ssf_copy_bench(ssf_ctx *ctx, int64_t bytes, int reps, double *use
   }
   if (t0) (void)hipEventDestroy(t0);
   if (t1) (void)hipEventDestroy(t1);
-  dfree(a);
-  dfree(b);
+  dfree(a, s);
+  dfree(b, s);
   SSF_HIP(e);
   return SSF_OK;
 }

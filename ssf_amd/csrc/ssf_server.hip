@@ -149,11 +149,13 @@ int64_t prep_bytes(const Request &q, int64_t negm, int64_t nnz, int64_t tot)
          6 * tot + 16 * negm + 4 * q.ne * q.ndofe * q.ndofe + 8 * q.ne;
 }
 
-void release_device(Request *q)
+// s: the stream the release is ordered on (device memory is stream-ordered)
+void release_device(Request *q, hipStream_t s)
 {
   if (q->m) ssf_matrix_destroy(q->m);
   q->m = nullptr;
-  sfree(q->vec);
+  if (q->vec) (void)hipFreeAsync(q->vec, s);
+  q->vec = nullptr;
 }
 
 // prep worker: requests from the pending queue to the ready queue
@@ -230,7 +232,9 @@ void prep_main(ssf_server *S, int w)
           }
         }
         for (int64_t i = 0; i < q->n; i++) lift[i] += q->rhs[i];
-        hipError_t e = salloc(&q->vec, 6 * (size_t)std::max<int64_t>(q->n, 1));
+        hipError_t e = hipMallocAsync((void **)&q->vec,
+                                      6 * (size_t)std::max<int64_t>(q->n, 1) * sizeof(double),
+                                      ctx->stream);
         if (e == hipSuccess)
           e = hipMemcpyAsync(q->vec, lift.data(), q->n * sizeof(double), hipMemcpyHostToDevice,
                              ctx->stream);
@@ -259,7 +263,7 @@ void prep_main(ssf_server *S, int w)
     S->t_assemble += dt_assemble;
     if (rc != SSF_OK) {
       lk.unlock();
-      release_device(q);
+      release_device(q, ctx->stream);
       lk.lock();
       if (q->t_admit > 0) S->used -= q->bytes;
       q->rc = rc;
@@ -427,7 +431,7 @@ void loop_main(ssf_server *S)
       fail = std::string("ssf_server: ") + hipGetErrorString(e);
       // fail every resident request and stop iterating
       for (Request *q : S->active) {
-        release_device(q);
+        release_device(q, s);
         finish(S, q, SSF_EHIP, fail);
       }
       S->active.clear();
@@ -459,7 +463,7 @@ void loop_main(ssf_server *S)
       q->info.ares = ares;
       q->info.rel_residual = -1.0;
       S->free_sids.push_back(q->sid);
-      release_device(q);
+      release_device(q, s);
       finish(S, q, e2 == hipSuccess ? SSF_OK : SSF_EHIP,
              e2 == hipSuccess ? std::string() : std::string(hipGetErrorString(e2)));
     }
@@ -468,7 +472,7 @@ void loop_main(ssf_server *S)
       std::vector<char> fresh(S->active.size(), 0);
       if (upload_tables(S, fresh) != hipSuccess) {
         for (Request *q : S->active) {
-          release_device(q);
+          release_device(q, s);
           finish(S, q, SSF_EHIP, "ssf_server: table upload failed");
         }
         S->active.clear();
@@ -522,7 +526,7 @@ extern "C" int ssf_server_destroy(ssf_server *S)
     if (t.joinable()) t.join();
   if (S->loop.joinable()) S->loop.join();
   (void)hipSetDevice(S->device);
-  for (auto &kv : S->all) release_device(kv.second.get());
+  for (auto &kv : S->all) release_device(kv.second.get(), S->ctx->stream);
   free_all(S);
   for (ssf_ctx *c : S->prep_ctx) ssf_ctx_destroy(c);
   if (S->ctx) ssf_ctx_destroy(S->ctx);
