@@ -352,6 +352,8 @@ extern "C" int ssf_matrix_destroy(ssf_matrix *m)
   dfree(m->tiles, m->ctx->stream); dfree(m->col16, m->ctx->stream);
   dfree(m->sell_off, m->ctx->stream); dfree(m->sell_col, m->ctx->stream); dfree(m->sell_src, m->ctx->stream); dfree(m->sell_v64, m->ctx->stream); dfree(m->sell_v32, m->ctx->stream);
   dfree(m->sell_col16, m->ctx->stream);
+  dfree(m->sell_lo, m->ctx->stream); dfree(m->sell_up, m->ctx->stream);
+  dfree(m->d_adr32, m->ctx->stream); dfree(m->d_ci32, m->ctx->stream);
   free_workspace(m->ws, m->ctx->stream);
   m->ctx->live_matrices--;
   delete m;
@@ -390,6 +392,11 @@ void ssf::matrix_keep_sell_only(ssf_matrix *m)
   dfree(m->sell_src, m->ctx->stream);
   dfree(m->sell_v32, m->ctx->stream);
   if (m->sell_col16) dfree(m->sell_col, m->ctx->stream);
+  dfree(m->sell_lo, m->ctx->stream);
+  dfree(m->sell_up, m->ctx->stream);
+  dfree(m->d_adr32, m->ctx->stream);
+  dfree(m->d_ci32, m->ctx->stream);
+  dfree(m->a_ref, m->ctx->stream);
   m->ntiles = 0;
   m->val32_valid = false;
 }
@@ -624,6 +631,80 @@ int ssf::matrix_create(ssf_ctx *ctx, int storage, int64_t n, const int64_t *adr,
   return SSF_OK;
 }
 
+int ssf::matrix_create_sell(ssf_ctx *ctx, int64_t n, std::vector<int64_t> &adr,
+                            std::vector<int64_t> &ci, ssf_matrix **out)
+{
+  SSF_CHECK(ctx && out && (int64_t)adr.size() == n + 1, SSF_EINVAL, "matrix_create_sell: bad argument");
+  SSF_CHECK(n > 0 && n < (1LL << 31) && adr[n] < (1LL << 31) && 2 * adr[n] < (1LL << 31),
+            SSF_EUNSUP, "matrix_create_sell: more than 2^31 entries");
+  if (const char *msg = check_pattern(n, adr.data(), ci.data(), true)) {
+    set_error(msg);
+    return SSF_EINVAL;
+  }
+  SSF_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const int64_t negm = adr[n];
+  std::vector<int> adr32(n + 1), ci32(negm);
+  parallel_for(n + 1, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; i++) adr32[i] = (int)adr[i];
+  });
+  parallel_for(negm, [&](int64_t b, int64_t e) {
+    for (int64_t k = b; k < e; k++) ci32[k] = (int)ci[k];
+  });
+  ssf_matrix *m = new ssf_matrix();
+  m->ctx = ctx;
+  ctx->live_matrices++;
+  m->storage = SSF_STORAGE_SYMM_COMP_ROWS;
+  m->n = n;
+  m->negm = negm;
+  m->nnz = 2 * negm - n;
+  m->sell_only = true;
+  hipError_t e = hipSuccess;
+  auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
+  chk(dalloc(&m->d_adr32, n + 1, s));
+  chk(dalloc(&m->d_ci32, negm, s));
+  if (e == hipSuccess) {
+    chk(hipMemcpyAsync(m->d_adr32, adr32.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice, s));
+    chk(hipMemcpyAsync(m->d_ci32, ci32.data(), negm * sizeof(int), hipMemcpyHostToDevice, s));
+    chk(hipStreamSynchronize(s));
+  }
+  SellLayout L;
+  if (e == hipSuccess) chk(sell_layout_build((int)n, negm, m->d_adr32, m->d_ci32, s, &L));
+  if (e == hipSuccess) {
+    m->nslice = L.nslice;
+    m->sell_tot = L.tot;
+    m->sell_off = L.off;
+    m->sell_col = L.col;
+    m->sell_col16 = L.col16;
+    m->sell_lo = L.lo;
+    m->sell_up = L.up;
+    chk(dalloc(&m->a_ref, negm, s));
+    chk(dalloc(&m->sell_v64, m->sell_tot, s));
+    chk(dalloc(&m->diag, n, s));
+  }
+  if (e == hipSuccess) {
+    chk(hipMemsetAsync(m->a_ref, 0, negm * sizeof(double), s));
+    chk(hipMemsetAsync(m->sell_v64, 0, std::max<int64_t>(1, m->sell_tot) * sizeof(double), s));
+    chk(hipMemsetAsync(m->diag, 0, n * sizeof(double), s));
+    chk(hipStreamSynchronize(s));
+  }
+  if (e != hipSuccess) {
+    const std::string msg = std::string("matrix_create_sell: ") + hipGetErrorString(e);
+    ssf_matrix_destroy(m);
+    set_error(msg);
+    return e == hipErrorOutOfMemory ? SSF_ENOMEM : SSF_EHIP;
+  }
+  m->adr.swap(adr);
+  m->ci.swap(ci);
+  SpmvTune t;
+  t.format = 2;
+  t.blocks = (int)((std::max<int64_t>(8, std::min<int64_t>(2048, (m->nslice + 3) / 4)) + 7) / 8 * 8);
+  t.idx16 = m->sell_col16 ? 1 : 0;
+  m->tune64 = m->tune32 = t;
+  *out = m;
+  return SSF_OK;
+}
+
 extern "C" int ssf_matrix_create_csr(ssf_ctx *ctx, int storage, int64_t n, const int64_t *adr,
                                      const int64_t *ci, ssf_matrix **out)
 {
@@ -673,6 +754,13 @@ extern "C" int ssf_matrix_pattern(const ssf_matrix *m, int64_t *adr, int64_t *ci
 static int values_changed(ssf_matrix *m)
 {
   hipStream_t s = m->ctx->stream;
+  if (m->sell_only) {
+    SSF_HIP(sell_expand((int)m->n, m->negm, m->d_adr32, m->sell_lo, m->sell_up, m->a_ref,
+                        m->sell_v64, m->diag, s));
+    m->values_set = true;
+    m->val32_valid = false;
+    return SSF_OK;
+  }
   if (m->storage == SSF_STORAGE_SYMM_COMP_ROWS) SSF_HIP(expand_ref(m->negm, m->map_lo, m->map_up, m->a_ref, m->val, s));
   SSF_HIP(extract_diag((int)m->n, m->rowptr, m->col, m->val, m->diag, nullptr, s));
   if (m->sell_src) SSF_HIP(sell_gather(m->sell_tot, m->sell_src, m->val, m->sell_v64, s));
@@ -794,6 +882,16 @@ static int prepare_assembly(ssf_matrix *m, int64_t nn, int64_t ne, int nne, int 
   SSF_HIP(dalloc(&m->order, ne, s));
   SSF_HIP(hipMemcpyAsync(m->order, order.data(), ne * sizeof(int), hipMemcpyHostToDevice, s));
   SSF_HIP(dalloc(&m->slot, ne * nd * nd, s));
+  if (m->sell_only) {  // the device pattern is at hand
+    SSF_HIP(build_slots(m->storage, aos ? 1 : 0, ne, nd, m->order, d_cn, m->d_adr32, m->d_ci32,
+                        m->slot, s));
+    SSF_HIP(hipStreamSynchronize(s));
+    m->asm_ne = ne;
+    m->asm_ndofe = nd;
+    m->colour_start = start;
+    m->asm_cn_host.assign(cn, cn + ncn);
+    return SSF_OK;
+  }
   // reference-layout pattern for the search
   std::vector<int> adr32(m->n + 1), ci32(m->negm);
   parallel_for(m->n + 1, [&](int64_t b, int64_t e) {
@@ -909,6 +1007,8 @@ extern "C" int ssf_matrix_localize_batch(ssf_matrix *m, int64_t ne, int64_t nc, 
 
 static int ensure_ws(ssf_matrix *m, bool mixed)
 {
+  SSF_CHECK(!(mixed && m->sell_only), SSF_EUNSUP,
+            "the mixed-precision solve needs the full-CSR copy this matrix does not keep");
   Workspace &w = m->ws;
   const int64_t n = std::max<int64_t>(1, m->n);
   const int nb = std::max(m->tune64.blocks, m->tune32.blocks);
@@ -1358,6 +1458,7 @@ extern "C" int ssf_matrix_autotune(ssf_matrix *m, int precision, int reps, int32
   std::vector<SpmvTune> cand;
   for (int l : {4, 8, 16, 32, 64})
     for (int b : {256, 512, 1024, 2048}) {
+      if (!m->rowptr) continue;  // SELL-only matrix
       const int64_t groups = TPB / l;
       if ((int64_t)b * groups > std::max<int64_t>(m->n, groups * 8)) continue;
       SpmvTune t;
@@ -1420,7 +1521,8 @@ extern "C" int ssf_matrix_set_tune(ssf_matrix *m, int precision, const int32_t *
   t.lanes = tune[1];
   t.blocks = tune[2];
   t.idx16 = tune[3];
-  SSF_CHECK(t.format == 0 || (t.format == 1 && m->ntiles > 0) || (t.format == 2 && m->nslice > 0),
+  SSF_CHECK((t.format == 0 && m->rowptr) || (t.format == 1 && m->ntiles > 0) ||
+                (t.format == 2 && m->nslice > 0),
             SSF_EINVAL,
             "ssf_matrix_set_tune: format must be 0 (row-vector), 1 (stream, needs a tiling) or 2 (SELL-64)");
   if (t.format == 0)

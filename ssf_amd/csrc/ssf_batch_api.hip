@@ -148,6 +148,8 @@ extern "C" int ssf_batch_create(ssf_ctx *ctx, int64_t nsys, ssf_matrix *const *m
   for (int64_t s = 0; s < nsys; s++) {
     SSF_CHECK(mats[s] && mats[s]->ctx == ctx, SSF_EINVAL, "ssf_batch_create: matrix of another context");
     SSF_CHECK(mats[s]->values_set, SSF_ESTATE, "ssf_batch_create: matrix values were never set");
+    SSF_CHECK(mats[s]->rowptr && mats[s]->col && mats[s]->val, SSF_EUNSUP,
+              "ssf_batch_create: the matrix keeps no full CSR");
     nrows += mats[s]->n;
     nnz += mats[s]->nnz;
   }

@@ -173,6 +173,10 @@ struct ssf_matrix {
   short *sell_col16 = nullptr;
   double *sell_v64 = nullptr;
   float *sell_v32 = nullptr;
+  // SELL-only matrices (laid out on the device, no full CSR): SELL positions of
+  // the reference-layout entries and the device copy of the pattern
+  bool sell_only = false;
+  int *sell_lo = nullptr, *sell_up = nullptr, *d_adr32 = nullptr, *d_ci32 = nullptr;
   ssf::SpmvTune tune64, tune32;
   bool tuned64 = false, tuned32 = false;
   ssf::CsrDev csr() const
@@ -204,6 +208,10 @@ int pattern_from_elements(int storage, int64_t n, int64_t ne, int64_t ndofe, con
 int matrix_create(ssf_ctx *ctx, int storage, int64_t n, const int64_t *adr, const int64_t *ci,
                   std::vector<int64_t> *adr_own, std::vector<int64_t> *ci_own, bool stream_tiles,
                   ssf_matrix **out);
+// a symcomprow matrix held in the SELL-64 layout only, laid out on the device
+// from the pattern (moved in); assembled, multiplied and solved in FP64
+int matrix_create_sell(ssf_ctx *ctx, int64_t n, std::vector<int64_t> &adr,
+                       std::vector<int64_t> &ci, ssf_matrix **out);
 // releases the element-assembly structures (slot map, colour order) of a matrix
 // and, if host_pattern, its host copy of the reference-layout pattern
 void matrix_drop_assembly(ssf_matrix *m, bool host_pattern);

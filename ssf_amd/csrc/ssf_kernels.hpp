@@ -45,6 +45,23 @@ hipError_t copy_kernel(int64_t n, const double *a, double *b, hipStream_t s);
 // SELL values from the full CSR values: out[k] = src[k] >= 0 ? val[src[k]] : 0
 hipError_t sell_gather(int64_t tot, const int *src, const double *val, double *out, hipStream_t s);
 
+// SELL-64 layout of the full matrix of a symcomprow pattern, built on the device
+// (ssf_layout.hip). lo[k] / up[k]: SELL positions of scr entry k in its row and,
+// off the diagonal, mirrored in its column's row (-1 on the diagonal). Exactly
+// one of col (32-bit) and col16 (16-bit offsets from the slice's first row) is
+// returned, col16 when every offset fits.
+struct SellLayout {
+  int nslice = 0;
+  int64_t tot = 0;
+  int *off = nullptr, *col = nullptr, *lo = nullptr, *up = nullptr;
+  short *col16 = nullptr;
+};
+hipError_t sell_layout_build(int n, int64_t negm, const int *adr, const int *ci, hipStream_t s,
+                             SellLayout *out);
+// SELL values and diagonal from the reference-layout values a_ref
+hipError_t sell_expand(int n, int64_t negm, const int *adr, const int *lo, const int *up,
+                       const double *a_ref, double *sv, double *diag, hipStream_t s);
+
 // element-wise helpers
 hipError_t expand_ref(int64_t negm, const int *map_lo, const int *map_up, const double *a,
                       double *val, hipStream_t s);

@@ -140,13 +140,14 @@ int64_t resident_bytes(int64_t n, int64_t tot, bool col16)
   return 4 * ((n + 63) / 64 + 1) + (col16 ? 10 : 12) * tot + 8 * n + 48 * n + (1 << 16);
 }
 
-// and while it is built and assembled: full CSR, stream tiles, SELL columns of
-// both widths and its gather map, reference-layout values and scatter maps,
-// element slot map and colour order
-int64_t prep_bytes(const Request &q, int64_t negm, int64_t nnz, int64_t tot)
+// and while it is laid out and assembled: the device pattern, the layout build's
+// temporaries (row index, sort keys and values, twice), SELL columns of both
+// widths, SELL positions and values of the reference layout, element slot map
+// and colour order
+int64_t prep_bytes(const Request &q, int64_t negm, int64_t tot)
 {
-  return resident_bytes(q.n, tot, false) + 4 * (q.n + 1) + 14 * nnz + 8 * (nnz / 1024 + 2) +
-         6 * tot + 16 * negm + 4 * q.ne * q.ndofe * q.ndofe + 8 * q.ne;
+  return resident_bytes(q.n, tot, false) + 2 * tot + 4 * (q.n + 1) + 4 * negm + 28 * negm +
+         24 * q.n + 16 * negm + 8 * negm + 4 * q.ne * q.ndofe * q.ndofe + 8 * q.ne;
 }
 
 // s: the stream the release is ordered on (device memory is stream-ordered)
@@ -185,7 +186,7 @@ void prep_main(ssf_server *S, int w)
     if (rc == SSF_OK) {
       const int64_t negm = adr[q->n];
       const int64_t nnz = 2 * negm - q->n;
-      q->bytes = prep_bytes(*q, negm, nnz, sell_entries(q->n, adr, ci));
+      q->bytes = prep_bytes(*q, negm, sell_entries(q->n, adr, ci));
       std::unique_lock<std::mutex> lk(S->mu);
       if (q->bytes > S->budget) {
         rc = SSF_ENOMEM;
@@ -200,8 +201,7 @@ void prep_main(ssf_server *S, int w)
     }
     if (rc == SSF_OK) {
       const auto tc0 = clk::now();
-      rc = matrix_create(ctx, SSF_STORAGE_SYMM_COMP_ROWS, q->n, adr.data(), ci.data(), &adr, &ci,
-                         false, &q->m);
+      rc = matrix_create_sell(ctx, q->n, adr, ci, &q->m);
       dt_create = std::chrono::duration<double>(clk::now() - tc0).count();
       const auto ta0 = clk::now();
       if (rc == SSF_OK && q->m->nslice == 0) {
