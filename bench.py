@@ -125,7 +125,7 @@ def bench_batched(ctx, args, gpu, mesh, torch):
         "speedup_mixed_vs_sequential_fp64": out["mixed"]["solves_per_s"] / seq,
         "speedup_mixed_vs_batched_fp64": out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"],
         "max_rel_err": {"mixed": out["mixed"]["max_rel_err"], "fp64": out["fp64"]["max_rel_err"]},
-        "detail": out, "lanes": {"fp64": lanes64, "fp32": lanes32},
+        "detail": out, "spmm_format_lanes": {"fp64": lanes64, "fp32": lanes32},
         "sequential_sample": f"every {stride}th plate ({len(sample)} plates x 8 load cases), "
                              f"time scaled by {len(probs) / len(sample):.0f}",
     }

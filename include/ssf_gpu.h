@@ -174,9 +174,11 @@ int ssf_batch_solve_dev(ssf_batch *b, const ssf_solve_params *p, const double *r
 /* same with host vectors */
 int ssf_batch_solve(ssf_batch *b, const ssf_solve_params *p, const double *rhs, double *lhs,
                     ssf_solve_info *infos);
-/* time the lane counts of the batched SpMM for precision 0 (FP64) / 1 (FP32)
- * and keep the fastest; lanes_out (may be NULL) receives it */
-int ssf_batch_autotune(ssf_batch *b, int precision, int reps, int32_t *lanes_out);
+/* time the batched SpMM formats for precision 0 (FP64) / 1 (FP32): SELL-64
+ * slices (format 2, the default) and the CSR row-vector kernel with R..64
+ * lanes per row (format 0); keep the fastest. tune_out (may be NULL) receives
+ * {format, lanes}. */
+int ssf_batch_autotune(ssf_batch *b, int precision, int reps, int32_t *tune_out);
 
 /* ------------------------------------------------------------------ serving */
 /* A request loop for element-mesh solves on one GPU: requests are admitted

@@ -18,6 +18,79 @@ namespace ssf {
 
 // ------------------------------------------------------------ reductions
 
+// SELL-64 SpMM: one wavefront per slice, one lane per row holding the R sums of
+// its row; each entry is loaded once (SELL_V per load) and its column's R
+// values of x are read contiguously. Rows past the block's end are padding.
+template <typename T, int R, bool I16, int KIND>
+__global__ __launch_bounds__(TPB) void k_bsell(BatchDev B, const T *__restrict__ val,
+                                               const T *__restrict__ x, T *__restrict__ y,
+                                               const T *__restrict__ b,
+                                               T *__restrict__ partial,
+                                               const int *__restrict__ sys_done)
+{
+  typedef T tv __attribute__((ext_vector_type(SELL_V)));
+  typedef int iv __attribute__((ext_vector_type(SELL_V)));
+  typedef short sv __attribute__((ext_vector_type(SELL_V)));
+  const int blk = blockIdx.x;
+  const int s = B.blk_sys[blk];
+  if (sys_done && sys_done[s]) return;
+  const int rend = B.blk_r1[blk];
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  T acc[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) acc[r] = T(0);
+  for (int sl = B.blk_s0[blk] + wave; sl < B.blk_s1[blk]; sl += TPB / 64) {
+    const int off = B.soff[sl];
+    const int w = (B.soff[sl + 1] - off) / 64;
+    const int row0 = B.slice_row0[sl];
+    T sum[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) sum[r] = T(0);
+    for (int q = 0; q < w; q += SELL_V) {
+      const int k = off + q * 64 + lane * SELL_V;
+      const tv vv = __builtin_nontemporal_load((const tv *)(val + k));
+      int cc[SELL_V];
+      if (I16) {
+        const sv c16 = __builtin_nontemporal_load((const sv *)(B.scol16 + k));
+#pragma unroll
+        for (int u = 0; u < SELL_V; u++) cc[u] = row0 + c16[u];
+      } else {
+        const iv c32 = __builtin_nontemporal_load((const iv *)(B.scol + k));
+#pragma unroll
+        for (int u = 0; u < SELL_V; u++) cc[u] = c32[u];
+      }
+#pragma unroll
+      for (int u = 0; u < SELL_V; u++) {
+        const T *__restrict__ xr = x + (int64_t)cc[u] * R;
+#pragma unroll
+        for (int r = 0; r < R; r++) sum[r] += vv[u] * xr[r];
+      }
+    }
+    const int row = row0 + lane;
+    if (row < rend) {
+      const int64_t q0 = (int64_t)row * R;
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        T v = sum[r];
+        if (KIND == 2) {
+          v = b[q0 + r] - v;
+          acc[r] += v * v;
+        } else if (KIND == 1) {
+          acc[r] += x[q0 + r] * v;
+        }
+        y[q0 + r] = v;
+      }
+    }
+  }
+  if (KIND != 0) {
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const T t = block_sum(acc[r]);
+      if (threadIdx.x == 0) partial[(int64_t)blk * R + r] = t;
+    }
+  }
+}
+
 // sum of a per-system quantity over the blocks of system s (fixed order)
 template <typename T, int R>
 __device__ __forceinline__ T sys_sum(const BatchDev &B, const T *__restrict__ part, int s, int r)
@@ -555,10 +628,19 @@ __global__ __launch_bounds__(TPB) void k_bdot_yy(BatchDev B, int R, const double
 
 template <typename T, int R>
 static hipError_t bspmm_r(const BatchTune &t, int kind, const BatchDev &B, const T *val,
-                          const T *x, T *y, const T *b, T *partial, const int *sys_done,
-                          hipStream_t s)
+                          const T *sval, const T *x, T *y, const T *b, T *partial,
+                          const int *sys_done, hipStream_t s)
 {
   dim3 grid(B.nblk), block(TPB);
+  if (t.format == 2 && B.soff && sval) {
+#define SSF_BSELL(I, K) hipLaunchKernelGGL((k_bsell<T, R, I, K>), grid, block, 0, s, B, sval, x, y, b, partial, sys_done)
+    const bool i16 = B.scol16 != nullptr;
+    if (kind == 0) { if (i16) SSF_BSELL(true, 0); else SSF_BSELL(false, 0); }
+    else if (kind == 1) { if (i16) SSF_BSELL(true, 1); else SSF_BSELL(false, 1); }
+    else { if (i16) SSF_BSELL(true, 2); else SSF_BSELL(false, 2); }
+#undef SSF_BSELL
+    return hipGetLastError();
+  }
 #define SSF_BS(L, K) hipLaunchKernelGGL((k_bspmm<T, R, L, K>), grid, block, 0, s, B, val, x, y, b, partial, sys_done)
 #define SSF_BSK(L)              \
   do {                          \
@@ -578,28 +660,28 @@ static hipError_t bspmm_r(const BatchTune &t, int kind, const BatchDev &B, const
 }
 
 template <typename T>
-hipError_t bspmm(const BatchTune &t, int kind, const BatchDev &B, const T *val, const T *x,
-                 T *y, const T *b, T *partial, const int *sys_done, hipStream_t s)
+hipError_t bspmm(const BatchTune &t, int kind, const BatchDev &B, const T *val, const T *sval,
+                 const T *x, T *y, const T *b, T *partial, const int *sys_done, hipStream_t s)
 {
   switch (B.R) {
-    case 1: return bspmm_r<T, 1>(t, kind, B, val, x, y, b, partial, sys_done, s);
-    case 2: return bspmm_r<T, 2>(t, kind, B, val, x, y, b, partial, sys_done, s);
-    case 4: return bspmm_r<T, 4>(t, kind, B, val, x, y, b, partial, sys_done, s);
-    default: return bspmm_r<T, 8>(t, kind, B, val, x, y, b, partial, sys_done, s);
+    case 1: return bspmm_r<T, 1>(t, kind, B, val, sval, x, y, b, partial, sys_done, s);
+    case 2: return bspmm_r<T, 2>(t, kind, B, val, sval, x, y, b, partial, sys_done, s);
+    case 4: return bspmm_r<T, 4>(t, kind, B, val, sval, x, y, b, partial, sys_done, s);
+    default: return bspmm_r<T, 8>(t, kind, B, val, sval, x, y, b, partial, sys_done, s);
   }
 }
 template hipError_t bspmm<double>(const BatchTune &, int, const BatchDev &, const double *,
-                                  const double *, double *, const double *, double *,
-                                  const int *, hipStream_t);
+                                  const double *, const double *, double *, const double *,
+                                  double *, const int *, hipStream_t);
 template hipError_t bspmm<float>(const BatchTune &, int, const BatchDev &, const float *,
-                                 const float *, float *, const float *, float *, const int *,
-                                 hipStream_t);
+                                 const float *, const float *, float *, const float *, float *,
+                                 const int *, hipStream_t);
 
 template <typename T, int R>
 static hipError_t biter_r(const BatchTune &t, const BatchDev &B, const BatchVec<T> &v,
                           BatchState<T> st, T res, T zero, bool mixed, hipStream_t s)
 {
-  hipError_t e = bspmm_r<T, R>(t, 1, B, v.val, v.d, v.p, nullptr, v.part, st.sys_done, s);
+  hipError_t e = bspmm_r<T, R>(t, 1, B, v.val, v.sval, v.d, v.p, nullptr, v.part, st.sys_done, s);
   if (e != hipSuccess) return e;
   dim3 grid(B.nblk), block(TPB);
   T *prr = v.part + (int64_t)B.nblk * R;
@@ -704,7 +786,8 @@ static hipError_t bmx_start_r(const BatchTune &t64, const BatchTune &t32, const 
   // y.y and the initial FP64 residual r = b - A x
   double *pyy = v64.part + 2 * (int64_t)B.nblk * R;
   hipLaunchKernelGGL(k_bdot_yy, grid, block, 0, s, B, R, v64.y, pyy);
-  hipError_t e = bspmm_r<double, R>(t64, 2, B, v64.val, v64.x, v64.r, v64.y, v64.part, nullptr, s);
+  hipError_t e = bspmm_r<double, R>(t64, 2, B, v64.val, v64.sval, v64.x, v64.r, v64.y, v64.part,
+                                    nullptr, s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL((k_bmx_init<R>), dim3(gq), dim3(TPB), 0, s, B, mx, st, v64.part, pyy, res,
                      delta, md.sys_final);
@@ -732,8 +815,8 @@ static hipError_t bmx_refresh_r(const BatchTune &t64, const BatchDev &B,
   double *px = v64.part + (int64_t)B.nblk * R;  // free in the mixed solve
   hipLaunchKernelGGL((k_bmx_xupdate<R>), grid, block, 0, s, B, mx, md.sys_final, v64.x, v32.x,
                      px);
-  hipError_t e = bspmm_r<double, R>(t64, 2, B, v64.val, v64.x, v64.r, v64.y, v64.part,
-                                    md.sys_final, s);
+  hipError_t e = bspmm_r<double, R>(t64, 2, B, v64.val, v64.sval, v64.x, v64.r, v64.y,
+                                    v64.part, md.sys_final, s);
   if (e != hipSuccess) return e;
   float *prr = v32.part + (int64_t)B.nblk * R;
   hipLaunchKernelGGL((k_bmx_refresh<R>), grid, block, 0, s, B, mx, md.sys_final, v64.part, v64.r,

@@ -19,10 +19,18 @@ struct BatchDev {
   const int *rowptr = nullptr;  // global rows, global (block-diagonal) columns
   const int *col = nullptr;
   const int *blk_sys = nullptr, *blk_r0 = nullptr, *blk_r1 = nullptr, *sys_blk0 = nullptr;
+  // SELL-64 slices of the systems, concatenated: slice g covers global rows
+  // slice_row0[g] .. +63 of one system; block b covers slices [blk_s0[b], blk_s1[b])
+  const int *soff = nullptr;        // entry offsets, total slices + 1
+  const int *scol = nullptr;        // global 32-bit columns (when scol16 is null)
+  const short *scol16 = nullptr;    // offsets from the slice's first row
+  const int *slice_row0 = nullptr;
+  const int *blk_s0 = nullptr, *blk_s1 = nullptr;
 };
 
 struct BatchTune {
-  int lanes = 8;  // lanes per row (R * L)
+  int format = 2;  // 0: CSR row-vector SpMM, 2: SELL-64 (one lane per row, R sums)
+  int lanes = 8;   // format 0: lanes per row (R * L)
 };
 
 // per (system, rhs) pair q = s * R + r
@@ -41,6 +49,7 @@ struct BatchState {
 template <typename T>
 struct BatchVec {  // interleaved [row * R + r]
   const T *val = nullptr, *diag = nullptr;
+  const T *sval = nullptr;  // SELL values
   T *x = nullptr, *r = nullptr, *d = nullptr, *p = nullptr, *h = nullptr, *y = nullptr;
   T *part = nullptr;  // 3 * nblk * R
 };
@@ -55,9 +64,12 @@ struct BatchMixDev {
   int *active = nullptr;
 };
 
+// Y = A X (kind 0), with per-(block, rhs) partials of d.p (kind 1), or
+// Y = B - A X with partials of |y|^2 (kind 2); val: CSR values (format 0),
+// sval: SELL values (format 2)
 template <typename T>
-hipError_t bspmm(const BatchTune &t, int kind, const BatchDev &B, const T *val, const T *x,
-                 T *y, const T *b, T *partial, const int *sys_done, hipStream_t s);
+hipError_t bspmm(const BatchTune &t, int kind, const BatchDev &B, const T *val, const T *sval,
+                 const T *x, T *y, const T *b, T *partial, const int *sys_done, hipStream_t s);
 template <typename T>
 hipError_t batch_init(const BatchDev &B, const BatchVec<T> &v, BatchState<T> st, T zero,
                       hipStream_t s);

@@ -56,6 +56,13 @@ struct ssf_batch {
   double *val = nullptr, *diag = nullptr;
   float *val32 = nullptr, *diag32 = nullptr;
   int *blk_sys = nullptr, *blk_r0 = nullptr, *blk_r1 = nullptr, *sys_blk0 = nullptr;
+  // SELL-64 slices of the systems, concatenated
+  int nslice = 0;
+  int64_t stot = 0;
+  int *soff = nullptr, *scol = nullptr, *slice_row0 = nullptr, *blk_s0 = nullptr, *blk_s1 = nullptr;
+  short *scol16 = nullptr;
+  double *sval64 = nullptr;
+  float *sval32 = nullptr;
   // vectors
   double *x = nullptr, *r = nullptr, *d = nullptr, *p = nullptr, *h = nullptr, *y = nullptr,
          *part = nullptr;
@@ -83,6 +90,12 @@ struct ssf_batch {
     b.blk_r0 = blk_r0;
     b.blk_r1 = blk_r1;
     b.sys_blk0 = sys_blk0;
+    b.soff = soff;
+    b.scol = scol;
+    b.scol16 = scol16;
+    b.slice_row0 = slice_row0;
+    b.blk_s0 = blk_s0;
+    b.blk_s1 = blk_s1;
     return b;
   }
   template <typename T>
@@ -94,6 +107,7 @@ BatchVec<double> ssf_batch::vec<double>() const
 {
   BatchVec<double> v;
   v.val = val; v.diag = diag; v.x = x; v.r = r; v.d = d; v.p = p; v.h = h; v.y = y; v.part = part;
+  v.sval = sval64;
   return v;
 }
 template <>
@@ -102,6 +116,7 @@ BatchVec<float> ssf_batch::vec<float>() const
   BatchVec<float> v;
   v.val = val32; v.diag = diag32; v.x = x32; v.r = r32; v.d = d32; v.p = p32; v.h = h32;
   v.y = y32; v.part = part32;
+  v.sval = sval32;
   return v;
 }
 
@@ -111,6 +126,8 @@ static void batch_release(ssf_batch *b)
     if (b->graph[k]) (void)hipGraphExecDestroy(b->graph[k]);
   bfree(b->rowptr); bfree(b->col); bfree(b->val); bfree(b->diag); bfree(b->val32); bfree(b->diag32);
   bfree(b->blk_sys); bfree(b->blk_r0); bfree(b->blk_r1); bfree(b->sys_blk0);
+  bfree(b->soff); bfree(b->scol); bfree(b->scol16); bfree(b->slice_row0); bfree(b->blk_s0);
+  bfree(b->blk_s1); bfree(b->sval64); bfree(b->sval32);
   bfree(b->x); bfree(b->r); bfree(b->d); bfree(b->p); bfree(b->h); bfree(b->y); bfree(b->part);
   bfree(b->x32); bfree(b->r32); bfree(b->d32); bfree(b->p32); bfree(b->h32); bfree(b->y32);
   bfree(b->part32);
@@ -150,10 +167,14 @@ extern "C" int ssf_batch_create(ssf_ctx *ctx, int64_t nsys, ssf_matrix *const *m
     SSF_CHECK(mats[s]->values_set, SSF_ESTATE, "ssf_batch_create: matrix values were never set");
     SSF_CHECK(mats[s]->rowptr && mats[s]->col && mats[s]->val, SSF_EUNSUP,
               "ssf_batch_create: the matrix keeps no full CSR");
+    SSF_CHECK(mats[s]->n > 0 && mats[s]->nslice > 0, SSF_EINVAL,
+              "ssf_batch_create: every system needs at least one unknown");
     nrows += mats[s]->n;
     nnz += mats[s]->nnz;
   }
-  SSF_CHECK(nnz < (1LL << 31) && nrows * nrhs < (1LL << 40), SSF_EUNSUP,
+  int64_t stot_all = 0;
+  for (int64_t s = 0; s < nsys; s++) stot_all += mats[s]->sell_tot;
+  SSF_CHECK(nnz < (1LL << 31) && stot_all < (1LL << 31) && nrows * nrhs < (1LL << 40), SSF_EUNSUP,
             "ssf_batch_create: batch larger than 2^31 matrix entries");
   SSF_HIP(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
@@ -166,21 +187,34 @@ extern "C" int ssf_batch_create(ssf_ctx *ctx, int64_t nsys, ssf_matrix *const *m
   b->nnz = nnz;
   b->row0.assign(nsys + 1, 0);
   for (int64_t s = 0; s < nsys; s++) b->row0[s + 1] = b->row0[s] + mats[s]->n;
-  // block table: contiguous row ranges inside one system
-  const int64_t rpb = std::max<int64_t>(64, 2048 / nrhs);
-  std::vector<int> bsys, br0, br1, sblk0(nsys + 1, 0);
+  // block table: each block covers consecutive 64-row slices of one system
+  // (2048 / nrhs rows at most), so its rows are one slice group of the SELL
+  // layout and its scalar partials belong to one system
+  const int spb = (int)std::max<int64_t>(1, std::max<int64_t>(64, 2048 / nrhs) / 64);
+  std::vector<int> bsys, br0, br1, bs0, bs1, sblk0(nsys + 1, 0), srow0;
+  std::vector<int64_t> sbase(nsys + 1, 0);
+  int gslice = 0;
   for (int64_t s = 0; s < nsys; s++) {
     sblk0[s] = (int)bsys.size();
     const int64_t n = mats[s]->n;
-    const int64_t nb = std::max<int64_t>(1, (n + rpb - 1) / rpb);
-    for (int64_t k = 0; k < nb; k++) {
+    const int ns = mats[s]->nslice;
+    for (int k = 0; k < ns; k += spb) {
       bsys.push_back((int)s);
-      br0.push_back((int)(b->row0[s] + n * k / nb));
-      br1.push_back((int)(b->row0[s] + n * (k + 1) / nb));
+      bs0.push_back(gslice + k);
+      bs1.push_back(gslice + std::min(ns, k + spb));
+      br0.push_back((int)(b->row0[s] + 64LL * k));
+      br1.push_back((int)(b->row0[s] + std::min<int64_t>(n, 64LL * (k + spb))));
     }
+    for (int k = 0; k < ns; k++) srow0.push_back((int)(b->row0[s] + 64LL * k));
+    gslice += ns;
+    sbase[s + 1] = sbase[s] + mats[s]->sell_tot;
   }
   sblk0[nsys] = (int)bsys.size();
   b->nblk = (int)bsys.size();
+  b->nslice = gslice;
+  b->stot = sbase[nsys];
+  bool all16 = true;
+  for (int64_t s = 0; s < nsys; s++) all16 = all16 && mats[s]->sell_col16 != nullptr;
   const int64_t nq = nsys * nrhs, nv = nrows * nrhs;
   hipError_t e = hipSuccess;
   auto c = [&](hipError_t x) { if (e == hipSuccess) e = x; };
@@ -192,6 +226,10 @@ extern "C" int ssf_batch_create(ssf_ctx *ctx, int64_t nsys, ssf_matrix *const *m
   c(balloc(&b->h, nv)); c(balloc(&b->y, nv)); c(balloc(&b->part, 3 * (int64_t)b->nblk * nrhs));
   c(b->st64.alloc(nq, nsys));
   c(balloc(&b->active, 1));
+  c(balloc(&b->soff, b->nslice + 1)); c(balloc(&b->slice_row0, b->nslice));
+  c(balloc(&b->blk_s0, b->nblk)); c(balloc(&b->blk_s1, b->nblk)); c(balloc(&b->sval64, b->stot));
+  if (all16) c(balloc(&b->scol16, b->stot));
+  else c(balloc(&b->scol, b->stot));
   if (e == hipSuccess) c(hipHostMalloc((void **)&b->active_host, 2 * sizeof(int), hipHostMallocDefault));
   if (e == hipSuccess) {
     c(hipMemcpy(b->blk_sys, bsys.data(), bsys.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -210,6 +248,29 @@ extern "C" int ssf_batch_create(ssf_ctx *ctx, int64_t nsys, ssf_matrix *const *m
     }
     const int last = (int)nnz;
     c(hipMemcpyAsync(b->rowptr + nrows, &last, sizeof(int), hipMemcpyHostToDevice, st));
+    // SELL slices: offsets shifted by each system's base, columns as they are
+    // (16-bit ones are relative to their slice) or shifted to global rows
+    std::vector<int> soff_all(b->nslice + 1, 0), tmp;
+    int g = 0;
+    for (int64_t s = 0; s < nsys && e == hipSuccess; s++) {
+      const ssf_matrix *m = mats[s];
+      tmp.resize(m->nslice + 1);
+      c(hipMemcpy(tmp.data(), m->sell_off, (m->nslice + 1) * sizeof(int), hipMemcpyDeviceToHost));
+      for (int k = 0; k < m->nslice; k++) soff_all[g + k] = (int)(sbase[s] + tmp[k]);
+      g += m->nslice;
+      c(hipMemcpyAsync(b->sval64 + sbase[s], m->sell_v64, m->sell_tot * sizeof(double),
+                       hipMemcpyDeviceToDevice, st));
+      if (all16)
+        c(hipMemcpyAsync(b->scol16 + sbase[s], m->sell_col16, m->sell_tot * sizeof(short),
+                         hipMemcpyDeviceToDevice, st));
+      else
+        c(offset_copy(m->sell_tot, m->sell_col, b->scol + sbase[s], (int)b->row0[s], st));
+    }
+    soff_all[b->nslice] = (int)b->stot;
+    c(hipMemcpy(b->soff, soff_all.data(), soff_all.size() * sizeof(int), hipMemcpyHostToDevice));
+    c(hipMemcpy(b->slice_row0, srow0.data(), srow0.size() * sizeof(int), hipMemcpyHostToDevice));
+    c(hipMemcpy(b->blk_s0, bs0.data(), bs0.size() * sizeof(int), hipMemcpyHostToDevice));
+    c(hipMemcpy(b->blk_s1, bs1.data(), bs1.size() * sizeof(int), hipMemcpyHostToDevice));
     c(hipStreamSynchronize(st));
   }
   if (e != hipSuccess) {
@@ -218,9 +279,11 @@ extern "C" int ssf_batch_create(ssf_ctx *ctx, int64_t nsys, ssf_matrix *const *m
     set_error(msg);
     return e == hipErrorOutOfMemory ? SSF_ENOMEM : SSF_EHIP;
   }
-  // lanes per row: 4 per right-hand side. The default does not depend on the
-  // batch's contents, so a system's summation order (and result) is the same
-  // alone and among others; only ssf_batch_autotune changes it.
+  // SELL SpMM by default; the CSR alternative with 4 lanes per right-hand side.
+  // The default does not depend on the batch's contents, so a system's
+  // summation order (and result) is the same alone and among others; only
+  // ssf_batch_autotune changes it.
+  b->tune64.format = 2;
   b->tune64.lanes = std::min(64, 4 * nrhs);
   b->tune32 = b->tune64;
   *out = b;
@@ -245,7 +308,9 @@ static int ensure_mixed(ssf_batch *b)
   c(balloc(&m.sys_final, b->nsys));
   c(balloc(&m.norm_a, b->nsys));
   m.active = b->active;
+  c(balloc(&b->sval32, b->stot));
   if (e == hipSuccess) c(to_f32(b->nnz, b->val, b->val32, 1.0, b->ctx->stream));
+  if (e == hipSuccess) c(to_f32(b->stot, b->sval64, b->sval32, 1.0, b->ctx->stream));
   if (e == hipSuccess) c(to_f32(b->nrows, b->diag, b->diag32, 1.0, b->ctx->stream));
   if (e == hipSuccess) c(batch_norm_inf(b->dev(), b->val, b->part, m.norm_a, b->ctx->stream));
   SSF_HIP(e);
@@ -447,9 +512,10 @@ extern "C" int ssf_batch_sizes(const ssf_batch *b, int64_t *nrows, int64_t *nnz,
   return SSF_OK;
 }
 
-// lanes per row of the batched SpMM (a multiple of R up to 64); the autotuner
-// times the candidates on this batch and keeps the fastest
-extern "C" int ssf_batch_autotune(ssf_batch *b, int precision, int reps, int32_t *lanes_out)
+// SpMM format of the batch: SELL-64 (format 2) or the CSR row-vector kernel with
+// R..64 lanes per row (format 0); the autotuner times the candidates on this
+// batch and keeps the fastest. tune_out (may be NULL) receives {format, lanes}.
+extern "C" int ssf_batch_autotune(ssf_batch *b, int precision, int reps, int32_t *tune_out)
 {
   SSF_ENTER();
   SSF_CHECK(b, SSF_EINVAL, "ssf_batch_autotune: NULL batch");
@@ -465,13 +531,25 @@ extern "C" int ssf_batch_autotune(ssf_batch *b, int precision, int reps, int32_t
   SSF_HIP(hipEventCreate(&t0));
   SSF_HIP(hipEventCreate(&t1));
   float best = 1e30f;
-  int best_l = precision ? b->tune32.lanes : b->tune64.lanes;
+  BatchTune bt = precision ? b->tune32 : b->tune64;
+  std::vector<BatchTune> cand;
+  {
+    BatchTune t;
+    t.format = 2;
+    t.lanes = bt.lanes;
+    cand.push_back(t);
+  }
   for (int l = b->R; l <= 64; l *= 2) {
     BatchTune t;
+    t.format = 0;
     t.lanes = l;
+    cand.push_back(t);
+  }
+  for (const BatchTune &t : cand) {
     auto launch = [&]() -> hipError_t {
-      if (precision) return bspmm<float>(t, 0, B, b->val32, b->d32, b->p32, nullptr, nullptr, nullptr, s);
-      return bspmm<double>(t, 0, B, b->val, b->d, b->p, nullptr, nullptr, nullptr, s);
+      if (precision)
+        return bspmm<float>(t, 0, B, b->val32, b->sval32, b->d32, b->p32, nullptr, nullptr, nullptr, s);
+      return bspmm<double>(t, 0, B, b->val, b->sval64, b->d, b->p, nullptr, nullptr, nullptr, s);
     };
     SSF_HIP(launch());
     SSF_HIP(hipEventRecord(t0, s));
@@ -482,17 +560,20 @@ extern "C" int ssf_batch_autotune(ssf_batch *b, int precision, int reps, int32_t
     SSF_HIP(hipEventElapsedTime(&ms, t0, t1));
     if (ms < best) {
       best = ms;
-      best_l = l;
+      bt = t;
     }
   }
   (void)hipEventDestroy(t0);
   (void)hipEventDestroy(t1);
-  if (precision) b->tune32.lanes = best_l;
-  else b->tune64.lanes = best_l;
+  if (precision) b->tune32 = bt;
+  else b->tune64 = bt;
   // a new tiling invalidates the captured chunk
   const int k = precision ? 1 : 0;
   if (b->graph[k]) (void)hipGraphExecDestroy(b->graph[k]);
   b->graph[k] = nullptr;
-  if (lanes_out) *lanes_out = best_l;
+  if (tune_out) {
+    tune_out[0] = bt.format;
+    tune_out[1] = bt.lanes;
+  }
   return SSF_OK;
 }

@@ -362,9 +362,10 @@ class Batch:
         return list(infos)
 
     def autotune(self, precision=FP64, reps=5):
-        out = ctypes.c_int32()
-        _check(lib().ssf_batch_autotune(self.h, precision, reps, ctypes.byref(out)))
-        return out.value
+        """Returns the chosen SpMM (format, lanes): 2 = SELL-64, 0 = CSR row-vector."""
+        out = np.zeros(2, np.int32)
+        _check(lib().ssf_batch_autotune(self.h, precision, reps, _p(out)))
+        return int(out[0]), int(out[1])
 
 
 class Server:
