@@ -11,8 +11,9 @@ A step is one solve of that system to relative error <= 1e-8 against x_ref
 restatement of the reference on smaller meshes by the tests):
   * value = solves/s of the mixed form (FP32 Jacobi-PCG, FP64 reliable updates)
   * fp64  = solves/s of the plain FP64 GPU PCG (the drop-in recurrence)
-The roofline figure is the FP32 SpMV of the mixed solve: its bytes per launch
-divided by its mean duration sampled with HIP events inside the timed solves.
+The roofline figure is the FP32 SpMV of the mixed solve: the bytes of the
+format it streams per launch, divided by its mean duration, sampled inside the
+timed solves with the launch's own start/stop events (hipExtLaunchKernelGGL).
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -415,7 +416,10 @@ def main():
         "spmv_gbs": {"fp32": gbs32, "fp64": gbs64, "copy_kernel": copy_gbs},
         "roofline": {"bound": "hbm", "achieved": gbs32, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                      "frac": gbs32 * 1e9 / PEAK_HBM, "traffic": traffic,
-                     "kernel": "k_spmv_stream<float> (FP32 SpMV of the mixed solve)",
+                     "kernel": {2: "k_spmv_sell<float, %s, 1> (FP32 SELL-64 SpMV of the mixed solve)"
+                                % ("true" if tune32[3] else "false"),
+                                1: "k_spmv_stream<float> (FP32 stream SpMV of the mixed solve)"}.get(
+                                    tune32[0], "k_spmv_vec<float> (FP32 SpMV of the mixed solve)"),
                      "bytes_per_launch": b32_act, "bytes_per_launch_canonical_csr": b32_can,
                      "usec_per_launch": us32, "samples": mixed["spmv_samples"]},
         "cpu_baseline": cpu,
