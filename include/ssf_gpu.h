@@ -216,7 +216,49 @@ int ssf_server_wait(ssf_server *s, int64_t ticket, double *x, ssf_solve_info *in
  * and on device assembly, summed over requests */
 int ssf_server_stats(ssf_server *s, int64_t *stats);
 
+/* ------------------------------------------------------- post-solve element pass */
+/* Results at the integration points and the internal forces, from a solution.
+ * Replaces the element loops run after the solve: compute_ipstrains
+ * (MEFEL/SRC/elemswitch.cpp:41), compute_ipstresses (:1215) and
+ * loc_internal_forces (MEFEL/SRC/globmat.cpp:506) for the elastic kinds, with
+ * linhex::gl_ip_strains / elem_integration (MEFEL/SRC/linhex.cpp:1110,2157),
+ * planeelemlq::gl_ip_strains / elem_integration (MEFEL/SRC/plelemlq.cpp:1173,2307)
+ * and elastisomat::nlstresses (MEFEL/SRC/elastisomat.cpp:627); and
+ * compute_ipgrads / compute_ipfluxes / internal_fluxes
+ * (TRFEL/SRC/elemswitcht.cpp:1260,1111,1845) with linhext::intpointgrad /
+ * internal_fluxes (TRFEL/SRC/linhext.cpp:432,1086) for heat. The element set
+ * (mesh, code numbers, material) stays on the device across calls. */
+typedef struct ssf_elements ssf_elements;
+/* arguments as ssf_matrix_assemble; n = number of unknowns (code numbers <= n) */
+int ssf_elements_create(ssf_ctx *ctx, int kind, int64_t n, int64_t nn, const double *xyz,
+                        int64_t ne, const int64_t *conn, const int64_t *cn, const double *mat_a,
+                        const double *mat_b, const double *thick, ssf_elements **out);
+int ssf_elements_destroy(ssf_elements *el);
+/* integration points per element (8 hex8, 4 quad4), components per point
+ * (hex8 elasticity 6: eps x, y, z, gamma yz, xz, xy; hex8 heat 3: x, y, z;
+ * quad4 plane stress 4: eps x, y, gamma xy, eps z with stress x, y, xy, 0) and
+ * DOFs per element */
+int ssf_elements_sizes(const ssf_elements *el, int32_t *ngp, int32_t *ncomp, int32_t *ndofe);
+/* For the solution lhs[n] and prescribed values pv[npv] (eldispl,
+ * MEFEL/SRC/globmat.cpp:1207: cn > 0 -> lhs[cn-1], cn < 0 -> pv[-cn-1], 0 -> 0):
+ * strain[ne][ngp][ncomp] (heat: temperature gradients), stress[ne][ngp][ncomp]
+ * (heat: fluxes q = -k grad T) and ifor[n], the element forces int B^T sigma dV
+ * (heat: internal fluxes) added per unknown in element order (locglob,
+ * GEFEL/matrix.cpp:3500). Any output may be NULL. Host arrays, synchronous. */
+int ssf_elements_results(ssf_elements *el, const double *lhs, const double *pv, int64_t npv,
+                         double *strain, double *stress, double *ifor);
+/* same on device pointers, asynchronous on the context's stream */
+int ssf_elements_results_dev(ssf_elements *el, const double *lhs_dev, const double *pv_dev,
+                             int64_t npv, double *strain_dev, double *stress_dev,
+                             double *ifor_dev);
+
 /* ------------------------------------------------------------ diagnostics */
+/* Time `reps` calls of ssf_elements_results_dev: usec[0] = mean duration of the
+ * element kernel (its own start and end events), usec[1] = mean duration of the
+ * whole call (element kernel and the per-unknown force sums). */
+int ssf_elements_bench(ssf_elements *el, const double *lhs_dev, const double *pv_dev,
+                       int64_t npv, double *strain_dev, double *stress_dev, double *ifor_dev,
+                       int reps, double *usec);
 /* Time `reps` launches of the FP64 (precision 0) or FP32 (1) SpMV on device
  * vectors; returns the mean kernel time in microseconds measured with HIP
  * events on the context's stream. */

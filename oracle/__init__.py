@@ -53,6 +53,12 @@ def lib():
             "cr_hex8_heat_ke_mesh": (None, [_L, _P, _P, _P, _P]),
             "cr_quad4_pstress_ke_mesh": (None, [_L, _P, _P, _P, _P, _P, _P]),
             "cr_dirichlet_lift": (None, [_L, _L, _P, _P, _P, _P]),
+            "cr_eldispl": (None, [_L, _P, _P, _P, _P]),
+            "cr_hex8_elast_ip": (None, [_P, _P, _P, _D, _D, _P, _P, _P, _P]),
+            "cr_quad4_pstress_ip": (None, [_P, _P, _D, _D, _P, _P, _P, _P, _P]),
+            "cr_hex8_heat_ip": (None, [_P, _P, _P, _D, _P, _P, _P, _P]),
+            "cr_elem_results_mesh": (None, [ctypes.c_int, _L, _P, _P, _P, _P, _P, _P, _L, _P,
+                                            _P, _P, _P, _P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(l, name)
@@ -92,6 +98,10 @@ def ref():
             "ref_hex8_elast_ke": (None, [_P, _P, _P, _D, _D, _P]),
             "ref_quad4_pstress_ke": (None, [_P, _P, _D, _D, _P, _P]),
             "ref_hex8_heat_ke": (None, [_P, _P, _P, _D, _P]),
+            "ref_hex8_elast_ip": (None, [_P, _P, _P, _D, _D, _P, _P, _P, _P]),
+            "ref_quad4_pstress_ip": (None, [_P, _P, _D, _D, _P, _P, _P, _P, _P]),
+            "ref_hex8_heat_ip": (None, [_P, _P, _P, _D, _P, _P, _P, _P]),
+            "ref_locglob": (None, [_P, _P, _P, _L]),
         }
         for name, (res, args) in sig.items():
             f = getattr(l, name)
@@ -237,3 +247,44 @@ def assemble_problem(prob, storage="scr"):
     if prob.pv.size:
         rhs = rhs + dirichlet_lift(prob, cn, ke)[None, :]
     return adr, ci, a, rhs
+
+
+# integration points and components per point of the post-solve element pass:
+# hex8 elasticity (eps_x, eps_y, eps_z, gamma_yz, gamma_xz, gamma_xy), hex8 heat
+# (gradient / flux x, y, z), quad4 plane stress (eps_x, eps_y, gamma_xy, eps_z)
+POST_SHAPE = {0: (8, 6), 1: (8, 3), 2: (4, 4)}
+
+
+def elem_results(prob, lhs):
+    """Oracle post-solve pass of a mesh.Problem for the solution lhs[n]:
+    (ip strains or gradients [ne, ngp, nc], ip stresses or fluxes, internal forces [n])."""
+    ngp, nc = POST_SHAPE[prob.kind]
+    eps = np.zeros((prob.ne, ngp, nc))
+    sig = np.zeros((prob.ne, ngp, nc))
+    ifor = np.zeros(prob.n)
+    pv = f64(prob.pv) if prob.pv.size else np.zeros(1)
+    lib().cr_elem_results_mesh(prob.kind, prob.ne, _ptr(i64(prob.conn)), _ptr(f64(prob.coords)),
+                               _ptr(i64(prob.elem_codes())), _ptr(f64(prob.e)),
+                               _ptr(f64(prob.nu)), _ptr(f64(prob.thick)), prob.n,
+                               _ptr(f64(lhs)), _ptr(pv), _ptr(eps), _ptr(sig), _ptr(ifor))
+    return eps, sig, ifor
+
+
+def elem_ip(kind, x, y, z, ma, mb, th, r):
+    """One element of the post-solve pass: (ip values, ip values, element forces)."""
+    ngp, nc = POST_SHAPE[kind]
+    nd = r.shape[0]
+    eps = np.zeros((ngp, nc))
+    sig = np.zeros((ngp, nc))
+    f = np.zeros(nd)
+    x, y, z, r = f64(x), f64(y), f64(z), f64(r)
+    if kind == 0:
+        lib().cr_hex8_elast_ip(_ptr(x), _ptr(y), _ptr(z), ma, mb, _ptr(r), _ptr(eps), _ptr(sig),
+                               _ptr(f))
+    elif kind == 1:
+        lib().cr_hex8_heat_ip(_ptr(x), _ptr(y), _ptr(z), ma, _ptr(r), _ptr(eps), _ptr(sig), _ptr(f))
+    else:
+        t = f64(np.full(4, th))
+        lib().cr_quad4_pstress_ip(_ptr(x), _ptr(y), ma, mb, _ptr(t), _ptr(r), _ptr(eps), _ptr(sig),
+                                  _ptr(f))
+    return eps, sig, f

@@ -822,3 +822,270 @@ void cr_dirichlet_lift(long ne, long nc, const long *cn, const double *ke,
       if (c[i] >= 1) rhs[c[i] - 1] += f[i];
   }
 }
+
+/* ============================================= post-solve element pass ===== */
+
+/* MEFEL/SRC/globmat.cpp:1305-1311 eldispl (linear statics branch) and
+ * TRFEL/SRC/globmatt.cpp:350-363 select_nodalval (stationary problem) */
+void cr_eldispl(long nc, const long *cn, const double *lhs, const double *pv, double *r)
+{
+  for (long i = 0; i < nc; i++) {
+    long ii = cn[i];
+    if (ii < 0) r[i] = pv[-ii - 1];
+    if (ii == 0) r[i] = 0.0;
+    if (ii > 0) r[i] = lhs[ii - 1];
+  }
+}
+
+/* GEFEL/matrix.cpp:1711-1730 mxv: v[i] = sum_j u[j] a[i][j], a row-major m x n */
+static void mxv_rm(const double *a, const double *u, double *v, int m, int n)
+{
+  for (int i = 0; i < m; i++) {
+    double aux = 0.0;
+    for (int j = 0; j < n; j++) aux += u[j] * a[i * n + j];
+    v[i] = aux;
+  }
+}
+
+/* GEFEL/matrix.cpp:1857-1876 mtxv: v[i] = sum_j u[j] a[j][i], a row-major m x n */
+static void mtxv_rm(const double *a, const double *u, double *v, int m, int n)
+{
+  for (int i = 0; i < n; i++) {
+    double aux = 0.0;
+    for (int j = 0; j < m; j++) aux += u[j] * a[j * n + i];
+    v[i] = aux;
+  }
+}
+
+/* MEFEL/SRC/linhex.cpp:153-184 geom_matrix: the 6x24 strain-displacement matrix
+ * (rows eps_x, eps_y, eps_z, gamma_yz, gamma_xz, gamma_xy); returns the signed
+ * Jacobian */
+static double hex_gm(double *gm, const double *x, const double *y, const double *z,
+                     double xi, double eta, double zeta)
+{
+  double dx[8], dy[8], dz[8];
+  dx_hex(dx, eta, zeta);
+  dy_hex(dy, xi, zeta);
+  dz_hex(dz, xi, eta);
+  double jac = derivatives_3d(dx, dy, dz, x, y, z, xi, eta, zeta);
+  memset(gm, 0, 6 * 24 * sizeof(double));
+  for (int a = 0; a < 8; a++) {
+    int c0 = 3 * a, c1 = 3 * a + 1, c2 = 3 * a + 2;
+    gm[0 * 24 + c0] = dx[a];
+    gm[1 * 24 + c1] = dy[a];
+    gm[2 * 24 + c2] = dz[a];
+    gm[3 * 24 + c1] = dz[a];
+    gm[3 * 24 + c2] = dy[a];
+    gm[4 * 24 + c0] = dz[a];
+    gm[4 * 24 + c2] = dx[a];
+    gm[5 * 24 + c0] = dy[a];
+    gm[5 * 24 + c1] = dx[a];
+  }
+  return jac;
+}
+
+/* hex8 elasticity after the solve, one element with nodal displacements r[24]:
+ *   strains    linhex::gl_ip_strains      MEFEL/SRC/linhex.cpp:1110-1145
+ *   stresses   elastisomat::nlstresses    MEFEL/SRC/elastisomat.cpp:627-651 (D eps,
+ *              D of matstiff_spacestr :395-410)
+ *   forces     linhex::elem_integration   MEFEL/SRC/linhex.cpp:2157-2191 with the
+ *              stresses (locstress), signed Jacobian as there
+ * eps, sig: [8 integration points][6]; f[24]. */
+void cr_hex8_elast_ip(const double *x, const double *y, const double *z, double e,
+                      double nu, const double *r, double *eps, double *sig, double *f)
+{
+  double d[36], gm[6 * 24], contr[24];
+  double g = e / 2.0 / (1.0 + nu);
+  double s = e / (1.0 + nu) / (1.0 - 2.0 * nu);
+  memset(d, 0, sizeof d);
+  d[0] = s * (1 - nu);
+  d[1] = s * nu;
+  d[2] = s * nu;
+  d[6] = d[1];
+  d[7] = d[0];
+  d[8] = d[1];
+  d[12] = d[1];
+  d[13] = d[1];
+  d[14] = d[0];
+  d[21] = g;
+  d[28] = g;
+  d[35] = g;
+  for (int q = 0; q < 24; q++) f[q] = 0.0;
+  int ip = 0;
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++)
+      for (int k = 0; k < 2; k++) {
+        double jac = hex_gm(gm, x, y, z, GP2[i], GP2[j], GP2[k]);
+        double *ep = eps + 6 * ip, *sp = sig + 6 * ip;
+        mxv_rm(gm, r, ep, 6, 24);
+        mxv_rm(d, ep, sp, 6, 6);
+        mtxv_rm(gm, sp, contr, 6, 24);
+        double c = jac * GW2[i] * GW2[j] * GW2[k];
+        for (int q = 0; q < 24; q++) contr[q] *= c;
+        for (int q = 0; q < 24; q++) f[q] = contr[q] + f[q];
+        ip++;
+      }
+}
+
+/* quad4 plane stress after the solve, one element with r[8], nodal thickness t[4]:
+ *   strains    planeelemlq::gl_ip_strains    MEFEL/SRC/plelemlq.cpp:1173-1205
+ *   stresses   elastisomat::nlstresses       MEFEL/SRC/elastisomat.cpp:627-651: the
+ *              strain vector has 4 components, D (matstiff_plstress :255-266) is 4x4
+ *              with a zero last row and column, and eps_z = -nu/(1-nu)(eps_x+eps_y)
+ *              is stored into the fourth strain component after the stresses
+ *   forces     planeelemlq::elem_integration MEFEL/SRC/plelemlq.cpp:2307-2359
+ *              (jac w_i w_j thick, thick = approx at the point, signed Jacobian)
+ * eps, sig: [4 integration points][4]; f[8]. The fourth strain component enters
+ * the stresses only through zero entries of D; a fresh state (0) is used. */
+void cr_quad4_pstress_ip(const double *x, const double *y, double e, double nu,
+                         const double *t, const double *r, double *eps, double *sig,
+                         double *f)
+{
+  double d[16], gm[3 * 8], dx[4], dy[4], nx[4], ny[4], contr[8];
+  double c = e / (1.0 - nu * nu);
+  memset(d, 0, sizeof d);
+  d[0] = c;
+  d[1] = c * nu;
+  d[4] = c * nu;
+  d[5] = c;
+  d[10] = e / 2.0 / (1.0 + nu);
+  for (int q = 0; q < 8; q++) f[q] = 0.0;
+  int ip = 0;
+  for (int i = 0; i < 2; i++) {
+    double xi = GP2[i];
+    for (int j = 0; j < 2; j++) {
+      double eta = GP2[j];
+      double bf[4];
+      bf[0] = 0.25 * (1.0 + xi) * (1.0 + eta);
+      bf[1] = 0.25 * (1.0 - xi) * (1.0 + eta);
+      bf[2] = 0.25 * (1.0 - xi) * (1.0 - eta);
+      bf[3] = 0.25 * (1.0 + xi) * (1.0 - eta);
+      double thick = 0.0;
+      for (int a = 0; a < 4; a++) thick += bf[a] * t[a];
+      dx_q4(dx, eta);
+      dy_q4(dy, xi);
+      dx_q4(nx, eta);
+      dy_q4(ny, xi);
+      double d1 = 0.0, d2 = 0.0, d3 = 0.0, d4 = 0.0;
+      for (int a = 0; a < 4; a++) {
+        d1 += nx[a] * x[a];
+        d2 += ny[a] * x[a];
+        d3 += nx[a] * y[a];
+        d4 += ny[a] * y[a];
+      }
+      double jac = d1 * d4 - d2 * d3;
+      for (int a = 0; a < 4; a++) {
+        double ddx = dx[a], ddy = dy[a];
+        dx[a] = (ddx * d4 - ddy * d3) / jac;
+        dy[a] = (ddy * d1 - ddx * d2) / jac;
+      }
+      memset(gm, 0, sizeof gm);
+      for (int a = 0; a < 4; a++) {
+        gm[0 * 8 + 2 * a] = dx[a];
+        gm[1 * 8 + 2 * a + 1] = dy[a];
+        gm[2 * 8 + 2 * a] = dy[a];
+        gm[2 * 8 + 2 * a + 1] = dx[a];
+      }
+      double *ep = eps + 4 * ip, *sp = sig + 4 * ip;
+      mxv_rm(gm, r, ep, 3, 8);
+      ep[3] = 0.0;
+      mxv_rm(d, ep, sp, 4, 4);
+      ep[3] = -nu / (1.0 - nu) * (ep[0] + ep[1]);
+      mtxv_rm(gm, sp, contr, 3, 8);
+      double cc = jac * GW2[i] * GW2[j] * thick;
+      for (int q = 0; q < 8; q++) contr[q] *= cc;
+      for (int q = 0; q < 8; q++) f[q] = contr[q] + f[q];
+      ip++;
+    }
+  }
+}
+
+/* hex8 heat conduction after the solve, one element with nodal temperatures r[8]:
+ *   gradients  linhext::intpointgrad       TRFEL/SRC/linhext.cpp:432-498 (grad_matrix :743)
+ *   fluxes     transmat::computenlfluxes   TRFEL/SRC/transmat.cpp:1123-1148, q = -D grad
+ *              with D = k I (isotrmat::matcond3d); the advection term of an isotropic
+ *              medium adds an exact zero and is left out
+ *   internal fluxes  linhext::internal_fluxes  TRFEL/SRC/linhext.cpp:1086-1146 (|J|)
+ * grad, flux: [8 integration points][3]; f[8]. */
+void cr_hex8_heat_ip(const double *x, const double *y, const double *z, double kc,
+                     const double *r, double *grad, double *flux, double *f)
+{
+  double gm[3 * 8], dx[8], dy[8], dz[8], d[9], contr[8];
+  memset(d, 0, sizeof d);
+  d[0] = kc;
+  d[4] = kc;
+  d[8] = kc;
+  for (int q = 0; q < 8; q++) f[q] = 0.0;
+  int ip = 0;
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++)
+      for (int k = 0; k < 2; k++) {
+        double xi = GP2[i], eta = GP2[j], zeta = GP2[k];
+        dx_hex(dx, eta, zeta);
+        dy_hex(dy, xi, zeta);
+        dz_hex(dz, xi, eta);
+        double jac = derivatives_3d(dx, dy, dz, x, y, z, xi, eta, zeta);
+        for (int a = 0; a < 8; a++) {
+          gm[a] = dx[a];
+          gm[8 + a] = dy[a];
+          gm[16 + a] = dz[a];
+        }
+        double *gp = grad + 3 * ip, *fp = flux + 3 * ip;
+        mxv_rm(gm, r, gp, 3, 8);
+        for (int jj = 0; jj < 3; jj++) {
+          fp[jj] = 0.0;
+          for (int kk = 0; kk < 3; kk++) fp[jj] -= d[jj * 3 + kk] * gp[kk];
+        }
+        mtxv_rm(gm, fp, contr, 3, 8);
+        if (jac < 0.0) jac = fabs(jac);
+        double c = jac * GW2[i] * GW2[j] * GW2[k];
+        for (int q = 0; q < 8; q++) contr[q] *= c;
+        for (int q = 0; q < 8; q++) f[q] = contr[q] + f[q];
+        ip++;
+      }
+}
+
+/* The element loops of compute_ipstrains (MEFEL/SRC/elemswitch.cpp:41),
+ * compute_ipstresses (:1215) and loc_internal_forces (MEFEL/SRC/globmat.cpp:506-650),
+ * or compute_ipgrads / compute_ipfluxes / internal_fluxes (TRFEL/SRC/elemswitcht.cpp:
+ * 1260, 1111, 1845), for one solution lhs[n]: per element the nodal values by
+ * eldispl, the integration-point values into eps/sig ([ne][ngp][ncomp], either may
+ * be NULL) and the element forces added into ifor[n] in element order by locglob
+ * (GEFEL/matrix.cpp:3500-3515; ifor is cleared first, may be NULL). xyz has stride
+ * 3; th (quad4) is the element thickness, equal at the four nodes. */
+void cr_elem_results_mesh(int kind, long ne, const long *conn, const double *xyz,
+                          const long *cn, const double *ma, const double *mb,
+                          const double *th, long n, const double *lhs, const double *pv,
+                          double *eps, double *sig, double *ifor)
+{
+  const int nne = kind == 2 ? 4 : 8;
+  const int nd = kind == 0 ? 24 : 8;
+  const int ngp = kind == 2 ? 4 : 8;
+  const int nc = kind == 0 ? 6 : (kind == 1 ? 3 : 4);
+  double x[8], y[8], z[8], t[4], r[24], f[24], e1[48], s1[48];
+  if (ifor)
+    for (long i = 0; i < n; i++) ifor[i] = 0.0;
+  for (long el = 0; el < ne; el++) {
+    for (int a = 0; a < nne; a++) {
+      long nd0 = conn[el * nne + a];
+      x[a] = xyz[3 * nd0];
+      y[a] = xyz[3 * nd0 + 1];
+      z[a] = xyz[3 * nd0 + 2];
+    }
+    const long *c = cn + el * nd;
+    cr_eldispl(nd, c, lhs, pv, r);
+    if (kind == 0) {
+      cr_hex8_elast_ip(x, y, z, ma[el], mb[el], r, e1, s1, f);
+    } else if (kind == 1) {
+      cr_hex8_heat_ip(x, y, z, ma[el], r, e1, s1, f);
+    } else {
+      for (int a = 0; a < 4; a++) t[a] = th ? th[el] : 1.0;
+      cr_quad4_pstress_ip(x, y, ma[el], mb[el], t, r, e1, s1, f);
+    }
+    if (eps) memcpy(eps + el * ngp * nc, e1, ngp * nc * sizeof(double));
+    if (sig) memcpy(sig + el * ngp * nc, s1, ngp * nc * sizeof(double));
+    if (ifor)
+      for (int i = 0; i < nd; i++)
+        if (c[i] >= 1) ifor[c[i] - 1] += f[i];
+  }
+}

@@ -24,6 +24,13 @@
  *   cr_hex8_heat_ke       bitwise vs oracle/_ref; constants in kernel, closed form
  *                         unit-cube matrix, linear temperature patch test
  *   cr_quad4_pstress_ke   bitwise vs oracle/_ref; 3 rigid-body modes, patch test
+ *   cr_eldispl            code-number rule on a hand-built table (all three branches)
+ *   cr_*_ip, cr_elem_results_mesh
+ *                         bitwise vs oracle/_ref (fixtures from the reference's
+ *                         solutions); patch tests (exact constant strain / gradient
+ *                         under linear fields), rigid-body motions give zero strain
+ *                         and force, element forces = K_e r with the pinned K_e,
+ *                         assembled forces = A x - lift, D / -k closed forms
  */
 #ifndef SSF_ORACLE_CPU_REF_H
 #define SSF_ORACLE_CPU_REF_H
@@ -111,6 +118,34 @@ void cr_quad4_pstress_ke_mesh(long ne, const long *conn, const double *xy,
  * of the element's prescribed DOFs (MATRIX.cpp:3500 locglob scatter). */
 void cr_dirichlet_lift(long ne, long nc, const long *cn, const double *ke,
                        const double *pv, double *rhs);
+
+/* ---- post-solve element pass (strains/stresses or gradients/fluxes at the
+ * integration points, internal forces / fluxes) */
+
+/* MEFEL/SRC/globmat.cpp:1305-1311 eldispl and TRFEL/SRC/globmatt.cpp:350-363
+ * select_nodalval: r[i] = lhs[cn-1] (cn > 0), pv[-cn-1] (cn < 0), 0 (cn = 0). */
+void cr_eldispl(long nc, const long *cn, const double *lhs, const double *pv, double *r);
+/* MEFEL/SRC/linhex.cpp:1110-1145, elastisomat.cpp:627-651, linhex.cpp:2157-2191:
+ * eps/sig [8][6] (x, y, z, yz, xz, xy; engineering shears), element forces f[24]. */
+void cr_hex8_elast_ip(const double *x, const double *y, const double *z, double e,
+                      double nu, const double *r, double *eps, double *sig, double *f);
+/* MEFEL/SRC/plelemlq.cpp:1173-1205, elastisomat.cpp:627-651, plelemlq.cpp:2307-2359:
+ * eps/sig [4][4] (x, y, xy, z), f[8]; t = nodal thickness. */
+void cr_quad4_pstress_ip(const double *x, const double *y, double e, double nu,
+                         const double *t, const double *r, double *eps, double *sig,
+                         double *f);
+/* TRFEL/SRC/linhext.cpp:432-498, transmat.cpp:1123-1148, linhext.cpp:1086-1146:
+ * grad/flux [8][3], internal fluxes f[8]. */
+void cr_hex8_heat_ip(const double *x, const double *y, const double *z, double k,
+                     const double *r, double *grad, double *flux, double *f);
+/* element loop of compute_ipstrains / compute_ipstresses / loc_internal_forces
+ * (MEFEL/SRC/elemswitch.cpp:41,1215, globmat.cpp:506-650) and of compute_ipgrads /
+ * compute_ipfluxes / internal_fluxes (TRFEL/SRC/elemswitcht.cpp:1260,1111,1845);
+ * kind 0 hex8 elasticity, 1 hex8 heat, 2 quad4 plane stress. */
+void cr_elem_results_mesh(int kind, long ne, const long *conn, const double *xyz,
+                          const long *cn, const double *ma, const double *mb,
+                          const double *th, long n, const double *lhs, const double *pv,
+                          double *eps, double *sig, double *ifor);
 
 #ifdef __cplusplus
 }

@@ -261,4 +261,160 @@ void ref_hex8_heat_ke(const double *xc, const double *yc, const double *zc, doub
     for (long j = 0; j < nne; j++) ke[i * nne + j] = km[i][j];
 }
 
+// ------------------------------------------------------- post-solve element pass
+// Integration-point values and element forces, composed from GEFEL's mxv, mtxv,
+// cmulv, addv, scprd and the shape-function routines in the loop orders of
+// linhex::gl_ip_strains / elem_integration (MEFEL/SRC/linhex.cpp:1110-1145,
+// 2157-2191), planeelemlq::gl_ip_strains / elem_integration (MEFEL/SRC/plelemlq.cpp:
+// 1173-1205, 2307-2359), linhext::intpointgrad / internal_fluxes (TRFEL/SRC/
+// linhext.cpp:432-498, 1086-1146). The material laws of elastisomat::nlstresses
+// (MEFEL/SRC/elastisomat.cpp:627-651) and transmat::computenlfluxes (TRFEL/SRC/
+// transmat.cpp:1123-1148) are restated, as the material classes read MEFEL/TRFEL
+// globals.
+static void ref_hex_gm(matrix &gm, vector &x, vector &y, vector &z, double xi, double eta,
+                       double zeta, double &jac)
+{
+  const long nne = 8;
+  vector dx(nne), dy(nne), dz(nne);
+  dx_bf_lin_hex_3d(dx.a, eta, zeta);
+  dy_bf_lin_hex_3d(dy.a, xi, zeta);
+  dz_bf_lin_hex_3d(dz.a, xi, eta);
+  derivatives_3d(dx, dy, dz, jac, x, y, z, xi, eta, zeta);
+  nullm(gm);
+  long jj = 0, kk = 1, ll = 2;
+  for (long a = 0; a < nne; a++) {
+    gm[0][jj] = dx[a]; gm[1][kk] = dy[a]; gm[2][ll] = dz[a];
+    gm[3][kk] = dz[a]; gm[3][ll] = dy[a];
+    gm[4][jj] = dz[a]; gm[4][ll] = dx[a];
+    gm[5][jj] = dy[a]; gm[5][kk] = dx[a];
+    jj += 3; kk += 3; ll += 3;
+  }
+}
+
+void ref_hex8_elast_ip(const double *xc, const double *yc, const double *zc, double e,
+                       double nu, const double *rr, double *eps_out, double *sig_out,
+                       double *f_out)
+{
+  const long nne = 8, ndofe = 24;
+  vector x(nne), y(nne), z(nne), r(ndofe), w(2), gp(2), eps(6), sig(6), contr(ndofe),
+      nv(ndofe);
+  matrix gm(6, ndofe), d(6, 6);
+  for (long i = 0; i < nne; i++) { x[i] = xc[i]; y[i] = yc[i]; z[i] = zc[i]; }
+  for (long i = 0; i < ndofe; i++) r[i] = rr[i];
+  double g = e / 2.0 / (1.0 + nu);
+  double s = e / (1.0 + nu) / (1.0 - 2.0 * nu);
+  fillm(0.0, d);
+  d[0][0] = s * (1 - nu); d[0][1] = s * nu; d[0][2] = s * nu;
+  d[1][0] = d[0][1]; d[1][1] = d[0][0]; d[1][2] = d[0][1];
+  d[2][0] = d[0][1]; d[2][1] = d[0][1]; d[2][2] = d[0][0];
+  d[3][3] = g; d[4][4] = g; d[5][5] = g;
+  gauss_points(gp.a, w.a, 2);
+  nullv(nv);
+  long ip = 0;
+  for (long i = 0; i < 2; i++)
+    for (long j = 0; j < 2; j++)
+      for (long k = 0; k < 2; k++) {
+        double jac;
+        ref_hex_gm(gm, x, y, z, gp[i], gp[j], gp[k], jac);
+        mxv(gm, r, eps);
+        mxv(d, eps, sig);
+        mtxv(gm, sig, contr);
+        cmulv(jac * w[i] * w[j] * w[k], contr);
+        addv(contr, nv, nv);
+        for (long c = 0; c < 6; c++) { eps_out[6 * ip + c] = eps[c]; sig_out[6 * ip + c] = sig[c]; }
+        ip++;
+      }
+  for (long i = 0; i < ndofe; i++) f_out[i] = nv[i];
+}
+
+void ref_quad4_pstress_ip(const double *xc, const double *yc, double e, double nu,
+                          const double *thick_nodes, const double *rr, double *eps_out,
+                          double *sig_out, double *f_out)
+{
+  const long nne = 4, ndofe = 8;
+  vector x(nne), y(nne), t(nne), r(ndofe), w(2), gp(2), dx(nne), dy(nne), bf(nne), eps3(3),
+      eps(4), sig(4), ipv(3), contr(ndofe), nv(ndofe);
+  matrix gm(3, ndofe), d(4, 4);
+  for (long i = 0; i < nne; i++) { x[i] = xc[i]; y[i] = yc[i]; t[i] = thick_nodes[i]; }
+  for (long i = 0; i < ndofe; i++) r[i] = rr[i];
+  double c = e / (1.0 - nu * nu);
+  fillm(0.0, d);
+  d[0][0] = c; d[0][1] = c * nu; d[0][2] = 0.0;
+  d[1][0] = c * nu; d[1][1] = c; d[1][2] = 0.0;
+  d[2][0] = 0.0; d[2][1] = 0.0; d[2][2] = e / 2.0 / (1.0 + nu);
+  gauss_points(gp.a, w.a, 2);
+  nullv(nv);
+  long ip = 0;
+  for (long i = 0; i < 2; i++) {
+    double xi = gp[i];
+    for (long j = 0; j < 2; j++) {
+      double eta = gp[j], jac, thick;
+      bf_lin_4_2d(bf.a, xi, eta);
+      scprd(bf, t, thick);
+      dx_bf_lin_4_2d(dx.a, eta);
+      dy_bf_lin_4_2d(dy.a, xi);
+      derivatives_2d(dx, dy, jac, x, y, xi, eta);
+      nullm(gm);
+      long i1 = 0, i2 = 1;
+      for (long a = 0; a < nne; a++) {
+        gm[0][i1] = dx[a]; gm[1][i2] = dy[a];
+        gm[2][i1] = dy[a]; gm[2][i2] = dx[a];
+        i1 += 2; i2 += 2;
+      }
+      mxv(gm, r, eps3);
+      eps[0] = eps3[0]; eps[1] = eps3[1]; eps[2] = eps3[2]; eps[3] = 0.0;
+      mxv(d, eps, sig);
+      eps[3] = -nu / (1.0 - nu) * (eps[0] + eps[1]);
+      ipv[0] = sig[0]; ipv[1] = sig[1]; ipv[2] = sig[2];
+      mtxv(gm, ipv, contr);
+      cmulv(jac * w[i] * w[j] * thick, contr);
+      addv(contr, nv, nv);
+      for (long q = 0; q < 4; q++) { eps_out[4 * ip + q] = eps[q]; sig_out[4 * ip + q] = sig[q]; }
+      ip++;
+    }
+  }
+  for (long i = 0; i < ndofe; i++) f_out[i] = nv[i];
+}
+
+void ref_hex8_heat_ip(const double *xc, const double *yc, const double *zc, double kk,
+                      const double *tt, double *grad_out, double *flux_out, double *f_out)
+{
+  const long nne = 8;
+  vector x(nne), y(nne), z(nne), t(nne), w(2), gp(2), dx(nne), dy(nne), dz(nne), grad(3),
+      fl(3), contr(nne), nv(nne);
+  matrix gm(3, nne), d(3, 3);
+  for (long i = 0; i < nne; i++) { x[i] = xc[i]; y[i] = yc[i]; z[i] = zc[i]; t[i] = tt[i]; }
+  fillm(0.0, d);
+  d[0][0] = kk; d[1][1] = kk; d[2][2] = kk;
+  gauss_points(gp.a, w.a, 2);
+  nullv(nv);
+  long ip = 0;
+  for (long i = 0; i < 2; i++)
+    for (long j = 0; j < 2; j++)
+      for (long k = 0; k < 2; k++) {
+        double xi = gp[i], eta = gp[j], zeta = gp[k], jac;
+        dx_bf_lin_hex_3d(dx.a, eta, zeta);
+        dy_bf_lin_hex_3d(dy.a, xi, zeta);
+        dz_bf_lin_hex_3d(dz.a, xi, eta);
+        derivatives_3d(dx, dy, dz, jac, x, y, z, xi, eta, zeta);
+        nullm(gm);
+        for (long a = 0; a < nne; a++) { gm[0][a] = dx[a]; gm[1][a] = dy[a]; gm[2][a] = dz[a]; }
+        mxv(gm, t, grad);
+        for (long jj = 0; jj < 3; jj++) {
+          fl[jj] = 0.0;
+          for (long q = 0; q < 3; q++) fl[jj] -= d[jj][q] * grad[q];
+        }
+        mtxv(gm, fl, contr);
+        if (jac < 0.0) jac = fabs(jac);
+        cmulv(jac * w[i] * w[j] * w[k], contr);
+        addv(contr, nv, nv);
+        for (long c = 0; c < 3; c++) { grad_out[3 * ip + c] = grad[c]; flux_out[3 * ip + c] = fl[c]; }
+        ip++;
+      }
+  for (long i = 0; i < nne; i++) f_out[i] = nv[i];
+}
+
+// GEFEL/matrix.cpp:3500 locglob
+void ref_locglob(double *gv, const double *lv, const long *cn, long n) { locglob(gv, lv, cn, n); }
+
 }  // extern "C"

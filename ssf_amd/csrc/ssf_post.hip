@@ -1,0 +1,729 @@
+// Post-solve element pass on the device: integration-point strains and stresses
+// (gradients and fluxes for heat) and the internal forces (internal fluxes)
+// assembled into the global vector.
+//
+// Reference path, per element in element order:
+//   eldispl                 MEFEL/SRC/globmat.cpp:1207-1311 (select_nodalval,
+//                           TRFEL/SRC/globmatt.cpp:342-363, for heat)
+//   strains                 linhex::gl_ip_strains MEFEL/SRC/linhex.cpp:1110-1145,
+//                           planeelemlq::gl_ip_strains MEFEL/SRC/plelemlq.cpp:1173-1205,
+//                           linhext::intpointgrad TRFEL/SRC/linhext.cpp:432-498
+//   stresses                elastisomat::nlstresses MEFEL/SRC/elastisomat.cpp:627-651,
+//                           transmat::computenlfluxes TRFEL/SRC/transmat.cpp:1123-1148
+//   element forces          linhex::elem_integration MEFEL/SRC/linhex.cpp:2157-2191,
+//                           planeelemlq::elem_integration MEFEL/SRC/plelemlq.cpp:2307-2359,
+//                           linhext::internal_fluxes TRFEL/SRC/linhext.cpp:1086-1146
+//   assembly                loc_internal_forces MEFEL/SRC/globmat.cpp:506-650 /
+//                           internal_fluxes TRFEL/SRC/elemswitcht.cpp:1845, locglob
+//                           GEFEL/matrix.cpp:3500
+//
+// Layout of the work: one lane per integration point, NGP lanes per element
+// (8 for the hexahedra, 4 for the quadrilateral), elements contiguous within a
+// wavefront. Lane g loads node g (coordinates, and the nodal values picked by the
+// code numbers) into LDS; every lane then evaluates its own point from the
+// element's LDS copy and writes its strain / stress records, which are contiguous
+// over the wavefront. The element forces are summed over the points in point
+// order through LDS and written per element; a second kernel adds them per
+// unknown in element order, from a table of (element, local DOF) entries sorted
+// by code number (a stable radix sort of the element code numbers, built once).
+//
+// Floating-point contraction is off in the arithmetic below and every sum runs
+// in the reference's order (the dense mxv / mtxv loops minus their exact-zero
+// terms), so the results equal the reference's x86-64 arithmetic bit for bit,
+// up to the sign of exact zeros.
+#include <hip/hip_ext.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <vector>
+
+#include "ssf_device.hpp"
+
+namespace ssf {
+namespace {
+
+template <int KIND>
+struct Shape;
+template <>
+struct Shape<SSF_ELEM_HEX8_ELAST> {
+  static constexpr int NNE = 8, NGP = 8, DPN = 3, NC = 6;
+};
+template <>
+struct Shape<SSF_ELEM_HEX8_HEAT> {
+  static constexpr int NNE = 8, NGP = 8, DPN = 1, NC = 3;
+};
+template <>
+struct Shape<SSF_ELEM_QUAD4_PSTRESS> {
+  static constexpr int NNE = 4, NGP = 4, DPN = 2, NC = 4;
+};
+
+constexpr double kGP = 0.577350269189626;  // GEFEL/intp.cpp:26-27, weights 1
+
+// natural derivatives of the trilinear hexahedron, GEFEL/basefun.cpp:2317-2375
+__device__ __forceinline__ void hex_dx(double *n, double y, double z)
+{
+#pragma clang fp contract(off)
+  n[0] = (1.0 + y) * (1.0 + z) / 8.0;
+  n[1] = (1.0 + y) * (1.0 + z) / (-8.0);
+  n[2] = (1.0 - y) * (1.0 + z) / (-8.0);
+  n[3] = (1.0 - y) * (1.0 + z) / 8.0;
+  n[4] = (1.0 + y) * (1.0 - z) / 8.0;
+  n[5] = (1.0 + y) * (1.0 - z) / (-8.0);
+  n[6] = (1.0 - y) * (1.0 - z) / (-8.0);
+  n[7] = (1.0 - y) * (1.0 - z) / 8.0;
+}
+__device__ __forceinline__ void hex_dy(double *n, double x, double z)
+{
+#pragma clang fp contract(off)
+  n[0] = (1.0 + x) * (1.0 + z) / 8.0;
+  n[1] = (1.0 - x) * (1.0 + z) / 8.0;
+  n[2] = (1.0 - x) * (1.0 + z) / (-8.0);
+  n[3] = (1.0 + x) * (1.0 + z) / (-8.0);
+  n[4] = (1.0 + x) * (1.0 - z) / 8.0;
+  n[5] = (1.0 - x) * (1.0 - z) / 8.0;
+  n[6] = (1.0 - x) * (1.0 - z) / (-8.0);
+  n[7] = (1.0 + x) * (1.0 - z) / (-8.0);
+}
+__device__ __forceinline__ void hex_dz(double *n, double x, double y)
+{
+#pragma clang fp contract(off)
+  n[0] = (1.0 + x) * (1.0 + y) / 8.0;
+  n[1] = (1.0 - x) * (1.0 + y) / 8.0;
+  n[2] = (1.0 - x) * (1.0 - y) / 8.0;
+  n[3] = (1.0 + x) * (1.0 - y) / 8.0;
+  n[4] = (1.0 + x) * (1.0 + y) / (-8.0);
+  n[5] = (1.0 - x) * (1.0 + y) / (-8.0);
+  n[6] = (1.0 - x) * (1.0 - y) / (-8.0);
+  n[7] = (1.0 + x) * (1.0 - y) / (-8.0);
+}
+
+// GEFEL/difcalc.cpp:487-571 derivatives_3d: natural -> global derivatives in
+// place; returns the signed Jacobian
+__device__ __forceinline__ double hex_global(double *dx, double *dy, double *dz,
+                                             const double *x, const double *y,
+                                             const double *z)
+{
+#pragma clang fp contract(off)
+  double d1 = 0.0, d2 = 0.0, d3 = 0.0, d4 = 0.0, d5 = 0.0, d6 = 0.0, d7 = 0.0, d8 = 0.0,
+         d9 = 0.0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    d1 += dx[i] * x[i];
+    d2 += dy[i] * x[i];
+    d3 += dz[i] * x[i];
+    d4 += dx[i] * y[i];
+    d5 += dy[i] * y[i];
+    d6 += dz[i] * y[i];
+    d7 += dx[i] * z[i];
+    d8 += dy[i] * z[i];
+    d9 += dz[i] * z[i];
+  }
+  const double jac = d1 * d5 * d9 + d4 * d8 * d3 + d7 * d2 * d6 - d7 * d5 * d3 - d8 * d6 * d1 -
+                     d9 * d4 * d2;
+  const double c00 = d5 * d9 - d8 * d6, c01 = d7 * d6 - d9 * d4, c02 = d4 * d8 - d7 * d5;
+  const double c10 = d8 * d3 - d9 * d2, c11 = d1 * d9 - d7 * d3, c12 = d7 * d2 - d8 * d1;
+  const double c20 = d2 * d6 - d5 * d3, c21 = d4 * d3 - d6 * d1, c22 = d1 * d5 - d4 * d2;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const double ddx = dx[i], ddy = dy[i], ddz = dz[i];
+    dx[i] = (ddx * c00 + ddy * c01 + ddz * c02) / jac;
+    dy[i] = (ddx * c10 + ddy * c11 + ddz * c12) / jac;
+    dz[i] = (ddx * c20 + ddy * c21 + ddz * c22) / jac;
+  }
+  return jac;
+}
+
+// One integration point of an element: writes the point's NC strain and stress
+// components and its force contributions contr[ND] (already times the point's
+// Jacobian factor).
+template <int KIND>
+struct Point;
+
+// hex8 elasticity: rows of the strain-displacement matrix (linhex.cpp:167-183)
+// eps_x, eps_y, eps_z, gamma_yz, gamma_xz, gamma_xy; D of matstiff_spacestr
+// (elastisomat.cpp:395-410)
+template <>
+struct Point<SSF_ELEM_HEX8_ELAST> {
+  __device__ static void eval(int g, const double *x, const double *y, const double *z,
+                              const double *r, double em, double nu, double th, double *eps,
+                              double *sig, double *contr)
+  {
+#pragma clang fp contract(off)
+    (void)th;
+    const double xi = (g & 4) ? kGP : -kGP, eta = (g & 2) ? kGP : -kGP,
+                 zeta = (g & 1) ? kGP : -kGP;
+    double bx[8], by[8], bz[8];
+    hex_dx(bx, eta, zeta);
+    hex_dy(by, xi, zeta);
+    hex_dz(bz, xi, eta);
+    const double jac = hex_global(bx, by, bz, x, y, z);
+    double e0 = 0.0, e1 = 0.0, e2 = 0.0, e3 = 0.0, e4 = 0.0, e5 = 0.0;
+#pragma unroll
+    for (int a = 0; a < 8; a++) e0 += r[3 * a] * bx[a];
+#pragma unroll
+    for (int a = 0; a < 8; a++) e1 += r[3 * a + 1] * by[a];
+#pragma unroll
+    for (int a = 0; a < 8; a++) e2 += r[3 * a + 2] * bz[a];
+#pragma unroll
+    for (int a = 0; a < 8; a++) {
+      e3 += r[3 * a + 1] * bz[a];
+      e3 += r[3 * a + 2] * by[a];
+    }
+#pragma unroll
+    for (int a = 0; a < 8; a++) {
+      e4 += r[3 * a] * bz[a];
+      e4 += r[3 * a + 2] * bx[a];
+    }
+#pragma unroll
+    for (int a = 0; a < 8; a++) {
+      e5 += r[3 * a] * by[a];
+      e5 += r[3 * a + 1] * bx[a];
+    }
+    const double gm = em / 2.0 / (1.0 + nu);
+    const double s = em / (1.0 + nu) / (1.0 - 2.0 * nu);
+    const double d00 = s * (1.0 - nu), d01 = s * nu;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    s0 += e0 * d00; s0 += e1 * d01; s0 += e2 * d01;
+    s1 += e0 * d01; s1 += e1 * d00; s1 += e2 * d01;
+    s2 += e0 * d01; s2 += e1 * d01; s2 += e2 * d00;
+    const double s3 = e3 * gm, s4 = e4 * gm, s5 = e5 * gm;
+    eps[0] = e0; eps[1] = e1; eps[2] = e2; eps[3] = e3; eps[4] = e4; eps[5] = e5;
+    sig[0] = s0; sig[1] = s1; sig[2] = s2; sig[3] = s3; sig[4] = s4; sig[5] = s5;
+#pragma unroll
+    for (int a = 0; a < 8; a++) {
+      double c = 0.0;
+      c += s0 * bx[a]; c += s4 * bz[a]; c += s5 * by[a];
+      contr[3 * a] = c * jac;
+      c = 0.0;
+      c += s1 * by[a]; c += s3 * bz[a]; c += s5 * bx[a];
+      contr[3 * a + 1] = c * jac;
+      c = 0.0;
+      c += s2 * bz[a]; c += s3 * by[a]; c += s4 * bx[a];
+      contr[3 * a + 2] = c * jac;
+    }
+  }
+};
+
+// hex8 heat: gradient = grad N t, flux q = -k grad (isotrmat), internal fluxes
+// with |J| (linhext.cpp:1129-1132)
+template <>
+struct Point<SSF_ELEM_HEX8_HEAT> {
+  __device__ static void eval(int g, const double *x, const double *y, const double *z,
+                              const double *r, double kc, double nu, double th, double *eps,
+                              double *sig, double *contr)
+  {
+#pragma clang fp contract(off)
+    (void)nu;
+    (void)th;
+    const double xi = (g & 4) ? kGP : -kGP, eta = (g & 2) ? kGP : -kGP,
+                 zeta = (g & 1) ? kGP : -kGP;
+    double bx[8], by[8], bz[8];
+    hex_dx(bx, eta, zeta);
+    hex_dy(by, xi, zeta);
+    hex_dz(bz, xi, eta);
+    double jac = hex_global(bx, by, bz, x, y, z);
+    double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+#pragma unroll
+    for (int a = 0; a < 8; a++) g0 += r[a] * bx[a];
+#pragma unroll
+    for (int a = 0; a < 8; a++) g1 += r[a] * by[a];
+#pragma unroll
+    for (int a = 0; a < 8; a++) g2 += r[a] * bz[a];
+    const double q0 = 0.0 - kc * g0, q1 = 0.0 - kc * g1, q2 = 0.0 - kc * g2;
+    eps[0] = g0; eps[1] = g1; eps[2] = g2;
+    sig[0] = q0; sig[1] = q1; sig[2] = q2;
+    if (jac < 0.0) jac = fabs(jac);
+#pragma unroll
+    for (int a = 0; a < 8; a++) {
+      double c = 0.0;
+      c += q0 * bx[a]; c += q1 * by[a]; c += q2 * bz[a];
+      contr[a] = c * jac;
+    }
+  }
+};
+
+// quad4 plane stress: rows eps_x, eps_y, gamma_xy (plelemlq.cpp:294-314); the
+// stored strain gains eps_z = -nu/(1-nu)(eps_x+eps_y) and the stress a zero
+// fourth component (elastisomat.cpp:640-643); thickness interpolated at the
+// point (bf_lin_4_2d, GEFEL/basefun.cpp:609-615)
+template <>
+struct Point<SSF_ELEM_QUAD4_PSTRESS> {
+  __device__ static void eval(int g, const double *x, const double *y, const double *z,
+                              const double *r, double em, double nu, double th, double *eps,
+                              double *sig, double *contr)
+  {
+#pragma clang fp contract(off)
+    (void)z;
+    const double xi = (g & 2) ? kGP : -kGP, eta = (g & 1) ? kGP : -kGP;
+    double bf[4];
+    bf[0] = 0.25 * (1.0 + xi) * (1.0 + eta);
+    bf[1] = 0.25 * (1.0 - xi) * (1.0 + eta);
+    bf[2] = 0.25 * (1.0 - xi) * (1.0 - eta);
+    bf[3] = 0.25 * (1.0 + xi) * (1.0 - eta);
+    double thick = 0.0;
+#pragma unroll
+    for (int a = 0; a < 4; a++) thick += bf[a] * th;
+    double nx[4], ny[4];
+    nx[0] = 0.25 * (1.0 + eta); nx[1] = 0.25 * (-1.0 - eta);
+    nx[2] = 0.25 * (-1.0 + eta); nx[3] = 0.25 * (1.0 - eta);
+    ny[0] = 0.25 * (1.0 + xi); ny[1] = 0.25 * (1.0 - xi);
+    ny[2] = 0.25 * (-1.0 + xi); ny[3] = 0.25 * (-1.0 - xi);
+    double d1 = 0.0, d2 = 0.0, d3 = 0.0, d4 = 0.0;
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      d1 += nx[a] * x[a];
+      d2 += ny[a] * x[a];
+      d3 += nx[a] * y[a];
+      d4 += ny[a] * y[a];
+    }
+    const double jac = d1 * d4 - d2 * d3;
+    double bx[4], by[4];
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      bx[a] = (nx[a] * d4 - ny[a] * d3) / jac;
+      by[a] = (ny[a] * d1 - nx[a] * d2) / jac;
+    }
+    double e0 = 0.0, e1 = 0.0, e2 = 0.0;
+#pragma unroll
+    for (int a = 0; a < 4; a++) e0 += r[2 * a] * bx[a];
+#pragma unroll
+    for (int a = 0; a < 4; a++) e1 += r[2 * a + 1] * by[a];
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      e2 += r[2 * a] * by[a];
+      e2 += r[2 * a + 1] * bx[a];
+    }
+    const double c = em / (1.0 - nu * nu);
+    const double cn = c * nu, d22 = em / 2.0 / (1.0 + nu);
+    double s0 = 0.0, s1 = 0.0;
+    s0 += e0 * c; s0 += e1 * cn;
+    s1 += e0 * cn; s1 += e1 * c;
+    const double s2 = e2 * d22;
+    eps[0] = e0; eps[1] = e1; eps[2] = e2; eps[3] = -nu / (1.0 - nu) * (e0 + e1);
+    sig[0] = s0; sig[1] = s1; sig[2] = s2; sig[3] = 0.0;
+    const double w = jac * thick;
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      double q = 0.0;
+      q += s0 * bx[a]; q += s2 * by[a];
+      contr[2 * a] = q * w;
+      q = 0.0;
+      q += s1 * by[a]; q += s2 * bx[a];
+      contr[2 * a + 1] = q * w;
+    }
+  }
+};
+
+template <int KIND>
+__global__ __launch_bounds__(TPB) void k_post(int64_t ne, const int *__restrict__ conn,
+                                              const int *__restrict__ cn,
+                                              const double *__restrict__ xyz,
+                                              const double *__restrict__ ma,
+                                              const double *__restrict__ mb,
+                                              const double *__restrict__ th,
+                                              const double *__restrict__ lhs,
+                                              const double *__restrict__ pv,
+                                              double *__restrict__ eps_out,
+                                              double *__restrict__ sig_out,
+                                              double *__restrict__ fe)
+{
+  using S = Shape<KIND>;
+  constexpr int NNE = S::NNE, NGP = S::NGP, DPN = S::DPN, NC = S::NC, ND = NNE * DPN;
+  constexpr int EPB = TPB / NGP, NPASS = ND / NGP;
+  static_assert(NNE == NGP && ND % NGP == 0, "one node per lane, whole passes");
+  __shared__ double s_x[EPB][NNE], s_y[EPB][NNE], s_z[EPB][NNE];
+  __shared__ double s_r[EPB][ND];
+  __shared__ double s_c[EPB][NGP][NGP + 1];
+  const int le = threadIdx.x / NGP, g = threadIdx.x % NGP;
+  const int64_t e = (int64_t)xcd_block(blockIdx.x, gridDim.x) * EPB + le;
+  const bool active = e < ne;
+  if (active) {
+    const int64_t nd = conn[e * NNE + g];
+    s_x[le][g] = xyz[3 * nd];
+    s_y[le][g] = xyz[3 * nd + 1];
+    s_z[le][g] = xyz[3 * nd + 2];
+#pragma unroll
+    for (int c = 0; c < DPN; c++) {
+      const int k = cn[e * ND + g * DPN + c];
+      s_r[le][g * DPN + c] = k > 0 ? lhs[k - 1] : (k < 0 ? pv[-k - 1] : 0.0);
+    }
+  }
+  __syncthreads();
+  double contr[ND];
+  if (active) {
+    double eps[NC], sig[NC];
+    Point<KIND>::eval(g, s_x[le], s_y[le], s_z[le], s_r[le], ma[e],
+                      KIND == SSF_ELEM_HEX8_HEAT ? 0.0 : mb[e], th ? th[e] : 1.0, eps, sig,
+                      contr);
+    const int64_t o = (e * NGP + g) * NC;
+    if (eps_out) {
+#pragma unroll
+      for (int c = 0; c < NC; c++) eps_out[o + c] = eps[c];
+    }
+    if (sig_out) {
+#pragma unroll
+      for (int c = 0; c < NC; c++) sig_out[o + c] = sig[c];
+    }
+  }
+  if (!fe) return;  // uniform over the grid: no lane reaches the barriers below
+  // element forces: f = contr_0 + 0, then contr_g + f in point order
+#pragma unroll
+  for (int p = 0; p < NPASS; p++) {
+    if (active) {
+#pragma unroll
+      for (int q = 0; q < NGP; q++) s_c[le][g][q] = contr[p * NGP + q];
+    }
+    __syncthreads();
+    if (active) {
+      double f = 0.0;
+#pragma unroll
+      for (int h = 0; h < NGP; h++) f = s_c[le][h][g] + f;
+      fe[e * ND + p * NGP + g] = f;
+    }
+    __syncthreads();
+  }
+}
+
+// ifor[d] = sum over the element entries of unknown d, in element order
+// (locglob into a cleared vector)
+__global__ __launch_bounds__(TPB) void k_post_gather(int n, const int *__restrict__ start,
+                                                     const int *__restrict__ src,
+                                                     const double *__restrict__ fe,
+                                                     double *__restrict__ ifor)
+{
+  const int d = blockIdx.x * TPB + threadIdx.x;
+  if (d >= n) return;
+  double s = 0.0;
+  for (int k = start[d], k1 = start[d + 1]; k < k1; k++) s = s + fe[src[k]];
+  ifor[d] = s;
+}
+
+// sort keys of the (element, local DOF) entries: unknown (code number - 1), or n
+// for a fixed / prescribed DOF (sorted last, not summed)
+__global__ void k_post_keys(int64_t nent, int n, const int *__restrict__ cn, int *__restrict__ key,
+                            int *__restrict__ val)
+{
+  const int64_t k = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (k >= nent) return;
+  const int c = cn[k];
+  key[k] = c > 0 ? c - 1 : n;
+  val[k] = (int)k;
+}
+
+// start[d] = first sorted entry with key >= d, for d = 0..n
+__global__ void k_post_start(int64_t nent, int n, const int *__restrict__ skey,
+                             int *__restrict__ start)
+{
+  const int64_t p = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (p >= nent) return;
+  const int kp = skey[p];
+  const int kprev = p > 0 ? skey[p - 1] : -1;
+  for (int d = kprev + 1; d <= kp && d <= n; d++) start[d] = (int)p;
+  if (p == nent - 1)
+    for (int d = kp + 1; d <= n; d++) start[d] = (int)nent;
+}
+
+__global__ void k_post_fill_int(int64_t m, int v, int *__restrict__ a)
+{
+  const int64_t k = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (k < m) a[k] = v;
+}
+
+template <typename T>
+hipError_t dalloc(T **p, size_t count, hipStream_t s)
+{
+  *p = nullptr;
+  if (count == 0) count = 1;
+  return hipMallocAsync((void **)p, count * sizeof(T), s);
+}
+
+template <typename T>
+void dfree(T *&p, hipStream_t s)
+{
+  if (p) (void)hipFreeAsync(p, s);
+  p = nullptr;
+}
+
+inline unsigned blocks_for(int64_t m, int64_t per) { return (unsigned)((m + per - 1) / per); }
+
+}  // namespace
+}  // namespace ssf
+
+struct ssf_elements {
+  ssf_ctx *ctx = nullptr;
+  int kind = 0;
+  int64_t n = 0, nn = 0, ne = 0;
+  int nne = 0, nd = 0, ngp = 0, nc = 0;
+  int64_t maxpv = 0;  // largest prescribed-value index the code numbers refer to
+  int *conn = nullptr, *cn = nullptr;
+  double *xyz = nullptr, *ma = nullptr, *mb = nullptr, *th = nullptr;
+  int *start = nullptr, *src = nullptr;  // per-unknown entry lists
+  double *fe = nullptr;                  // element forces [ne][nd]
+};
+
+using namespace ssf;
+
+static void post_shape(int kind, int &nne, int &nd, int &ngp, int &nc)
+{
+  nne = kind == SSF_ELEM_QUAD4_PSTRESS ? 4 : 8;
+  nd = kind == SSF_ELEM_HEX8_ELAST ? 24 : 8;
+  ngp = kind == SSF_ELEM_QUAD4_PSTRESS ? 4 : 8;
+  nc = kind == SSF_ELEM_HEX8_ELAST ? 6 : (kind == SSF_ELEM_HEX8_HEAT ? 3 : 4);
+}
+
+extern "C" int ssf_elements_destroy(ssf_elements *el)
+{
+  SSF_ENTER();
+  if (!el) return SSF_OK;
+  (void)hipSetDevice(el->ctx->device);
+  hipStream_t s = el->ctx->stream;
+  dfree(el->conn, s);
+  dfree(el->cn, s);
+  dfree(el->xyz, s);
+  dfree(el->ma, s);
+  dfree(el->mb, s);
+  dfree(el->th, s);
+  dfree(el->start, s);
+  dfree(el->src, s);
+  dfree(el->fe, s);
+  delete el;
+  return SSF_OK;
+}
+
+extern "C" int ssf_elements_create(ssf_ctx *ctx, int kind, int64_t n, int64_t nn,
+                                   const double *xyz, int64_t ne, const int64_t *conn,
+                                   const int64_t *cn, const double *mat_a, const double *mat_b,
+                                   const double *thick, ssf_elements **out)
+{
+  SSF_ENTER();
+  SSF_CHECK(out, SSF_EINVAL, "ssf_elements_create: NULL output");
+  *out = nullptr;
+  SSF_CHECK(ctx && xyz && conn && cn && mat_a, SSF_EINVAL, "ssf_elements_create: NULL argument");
+  SSF_CHECK(kind >= 0 && kind <= 2, SSF_EINVAL, "ssf_elements_create: unknown element kind");
+  SSF_CHECK(kind == SSF_ELEM_HEX8_HEAT || mat_b, SSF_EINVAL,
+            "ssf_elements_create: Poisson ratios missing");
+  int nne, nd, ngp, nc;
+  post_shape(kind, nne, nd, ngp, nc);
+  SSF_CHECK(n >= 0 && nn > 0 && ne > 0 && n < (1LL << 31) - 1 && nn < (1LL << 31) &&
+                ne * nd < (1LL << 31) && ne * ngp * nc < (1LL << 40),
+            SSF_EINVAL, "ssf_elements_create: bad sizes");
+  int64_t maxpv = 0;
+  std::vector<int> conn32(ne * nne), cn32(ne * nd);
+  for (int64_t k = 0; k < ne * nne; k++) {
+    SSF_CHECK(conn[k] >= 0 && conn[k] < nn, SSF_EINVAL, "ssf_elements_create: node id out of range");
+    conn32[k] = (int)conn[k];
+  }
+  for (int64_t k = 0; k < ne * nd; k++) {
+    SSF_CHECK(cn[k] <= n && cn[k] > -(1LL << 31), SSF_EINVAL,
+              "ssf_elements_create: code number out of range");
+    maxpv = std::max<int64_t>(maxpv, -cn[k]);
+    cn32[k] = (int)cn[k];
+  }
+  SSF_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  ssf_elements *el = new ssf_elements();
+  el->ctx = ctx;
+  el->kind = kind;
+  el->n = n;
+  el->nn = nn;
+  el->ne = ne;
+  el->nne = nne;
+  el->nd = nd;
+  el->ngp = ngp;
+  el->nc = nc;
+  el->maxpv = maxpv;
+  const int64_t nent = ne * nd;
+  int *key = nullptr, *skey = nullptr, *val = nullptr;
+  void *tmp = nullptr;
+  hipError_t e = hipSuccess;
+  auto c = [&](hipError_t x) { if (e == hipSuccess) e = x; };
+  c(dalloc(&el->conn, ne * nne, s));
+  c(dalloc(&el->cn, nent, s));
+  c(dalloc(&el->xyz, nn * 3, s));
+  c(dalloc(&el->ma, ne, s));
+  if (mat_b && kind != SSF_ELEM_HEX8_HEAT) c(dalloc(&el->mb, ne, s));
+  if (thick && kind == SSF_ELEM_QUAD4_PSTRESS) c(dalloc(&el->th, ne, s));
+  c(dalloc(&el->start, n + 1, s));
+  c(dalloc(&el->src, nent, s));
+  c(dalloc(&el->fe, nent, s));
+  c(dalloc(&key, nent, s));
+  c(dalloc(&skey, nent, s));
+  c(dalloc(&val, nent, s));
+  if (e == hipSuccess) {
+    c(hipMemcpyAsync(el->conn, conn32.data(), ne * nne * sizeof(int), hipMemcpyHostToDevice, s));
+    c(hipMemcpyAsync(el->cn, cn32.data(), nent * sizeof(int), hipMemcpyHostToDevice, s));
+    c(hipMemcpyAsync(el->xyz, xyz, nn * 3 * sizeof(double), hipMemcpyHostToDevice, s));
+    c(hipMemcpyAsync(el->ma, mat_a, ne * sizeof(double), hipMemcpyHostToDevice, s));
+    if (el->mb) c(hipMemcpyAsync(el->mb, mat_b, ne * sizeof(double), hipMemcpyHostToDevice, s));
+    if (el->th) c(hipMemcpyAsync(el->th, thick, ne * sizeof(double), hipMemcpyHostToDevice, s));
+  }
+  // entries of each unknown in element order: stable sort by code number
+  int bits = 1;
+  while (bits < 31 && (1LL << bits) <= n) bits++;
+  size_t tb = 0;
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_post_keys, dim3(blocks_for(nent, TPB)), dim3(TPB), 0, s, nent, (int)n,
+                       el->cn, key, val);
+    c(hipGetLastError());
+  }
+  if (e == hipSuccess)
+    c(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, key, skey, val, el->src, (int)nent, 0, bits,
+                                         s));
+  if (e == hipSuccess) c(hipMallocAsync(&tmp, std::max<size_t>(tb, 1), s));
+  if (e == hipSuccess)
+    c(hipcub::DeviceRadixSort::SortPairs(tmp, tb, key, skey, val, el->src, (int)nent, 0, bits, s));
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_post_fill_int, dim3(blocks_for(n + 1, TPB)), dim3(TPB), 0, s, n + 1,
+                       (int)nent, el->start);
+    c(hipGetLastError());
+  }
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_post_start, dim3(blocks_for(nent, TPB)), dim3(TPB), 0, s, nent, (int)n,
+                       skey, el->start);
+    c(hipGetLastError());
+  }
+  c(hipStreamSynchronize(s));
+  if (tmp) (void)hipFreeAsync(tmp, s);
+  dfree(key, s);
+  dfree(skey, s);
+  dfree(val, s);
+  if (e != hipSuccess) {
+    ssf_elements_destroy(el);
+    SSF_HIP(e);
+  }
+  *out = el;
+  return SSF_OK;
+}
+
+extern "C" int ssf_elements_sizes(const ssf_elements *el, int32_t *ngp, int32_t *ncomp,
+                                  int32_t *ndofe)
+{
+  SSF_ENTER();
+  SSF_CHECK(el, SSF_EINVAL, "ssf_elements_sizes: NULL element set");
+  if (ngp) *ngp = el->ngp;
+  if (ncomp) *ncomp = el->nc;
+  if (ndofe) *ndofe = el->nd;
+  return SSF_OK;
+}
+
+static hipError_t post_launch(ssf_elements *el, const double *lhs, const double *pv,
+                              double *eps, double *sig, double *ifor, hipEvent_t ev0,
+                              hipEvent_t ev1)
+{
+  hipStream_t s = el->ctx->stream;
+  double *fe = ifor ? el->fe : nullptr;
+  const int epb = TPB / el->ngp;
+  const dim3 grid(blocks_for(el->ne, epb)), block(TPB);
+#define SSF_POST(K)                                                                          \
+  do {                                                                                       \
+    if (ev0)                                                                                 \
+      hipExtLaunchKernelGGL((k_post<K>), grid, block, 0, s, ev0, ev1, 0, el->ne, el->conn,   \
+                            el->cn, el->xyz, el->ma, el->mb, el->th, lhs, pv, eps, sig, fe); \
+    else                                                                                     \
+      hipLaunchKernelGGL((k_post<K>), grid, block, 0, s, el->ne, el->conn, el->cn, el->xyz,  \
+                         el->ma, el->mb, el->th, lhs, pv, eps, sig, fe);                     \
+  } while (0)
+  if (el->kind == SSF_ELEM_HEX8_ELAST) SSF_POST(SSF_ELEM_HEX8_ELAST);
+  else if (el->kind == SSF_ELEM_HEX8_HEAT) SSF_POST(SSF_ELEM_HEX8_HEAT);
+  else SSF_POST(SSF_ELEM_QUAD4_PSTRESS);
+#undef SSF_POST
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || !ifor || el->n == 0) return e;
+  hipLaunchKernelGGL(k_post_gather, dim3(blocks_for(el->n, TPB)), dim3(TPB), 0, s, (int)el->n,
+                     el->start, el->src, el->fe, ifor);
+  return hipGetLastError();
+}
+
+extern "C" int ssf_elements_results_dev(ssf_elements *el, const double *lhs_dev,
+                                        const double *pv_dev, int64_t npv, double *strain_dev,
+                                        double *stress_dev, double *ifor_dev)
+{
+  SSF_ENTER();
+  SSF_CHECK(el, SSF_EINVAL, "ssf_elements_results_dev: NULL element set");
+  SSF_CHECK(lhs_dev || el->n == 0, SSF_EINVAL, "ssf_elements_results_dev: NULL solution");
+  SSF_CHECK(npv >= el->maxpv && (pv_dev || el->maxpv == 0), SSF_EINVAL,
+            "ssf_elements_results_dev: prescribed values missing for the code numbers");
+  SSF_HIP(hipSetDevice(el->ctx->device));
+  SSF_HIP(post_launch(el, lhs_dev, pv_dev, strain_dev, stress_dev, ifor_dev, nullptr, nullptr));
+  return SSF_OK;
+}
+
+extern "C" int ssf_elements_results(ssf_elements *el, const double *lhs, const double *pv,
+                                    int64_t npv, double *strain, double *stress, double *ifor)
+{
+  SSF_ENTER();
+  SSF_CHECK(el, SSF_EINVAL, "ssf_elements_results: NULL element set");
+  SSF_CHECK(lhs || el->n == 0, SSF_EINVAL, "ssf_elements_results: NULL solution");
+  SSF_CHECK(npv >= el->maxpv && (pv || el->maxpv == 0), SSF_EINVAL,
+            "ssf_elements_results: prescribed values missing for the code numbers");
+  SSF_HIP(hipSetDevice(el->ctx->device));
+  hipStream_t s = el->ctx->stream;
+  const int64_t nip = el->ne * el->ngp * el->nc;
+  double *d_lhs = nullptr, *d_pv = nullptr, *d_eps = nullptr, *d_sig = nullptr, *d_f = nullptr;
+  hipError_t e = hipSuccess;
+  auto c = [&](hipError_t x) { if (e == hipSuccess) e = x; };
+  c(dalloc(&d_lhs, el->n, s));
+  if (pv && npv) c(dalloc(&d_pv, npv, s));
+  if (strain) c(dalloc(&d_eps, nip, s));
+  if (stress) c(dalloc(&d_sig, nip, s));
+  if (ifor) c(dalloc(&d_f, el->n, s));
+  if (e == hipSuccess && el->n)
+    c(hipMemcpyAsync(d_lhs, lhs, el->n * sizeof(double), hipMemcpyHostToDevice, s));
+  if (e == hipSuccess && d_pv)
+    c(hipMemcpyAsync(d_pv, pv, npv * sizeof(double), hipMemcpyHostToDevice, s));
+  if (e == hipSuccess) c(post_launch(el, d_lhs, d_pv, d_eps, d_sig, d_f, nullptr, nullptr));
+  if (e == hipSuccess && strain)
+    c(hipMemcpyAsync(strain, d_eps, nip * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (e == hipSuccess && stress)
+    c(hipMemcpyAsync(stress, d_sig, nip * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (e == hipSuccess && ifor && el->n)
+    c(hipMemcpyAsync(ifor, d_f, el->n * sizeof(double), hipMemcpyDeviceToHost, s));
+  c(hipStreamSynchronize(s));
+  dfree(d_lhs, s);
+  dfree(d_pv, s);
+  dfree(d_eps, s);
+  dfree(d_sig, s);
+  dfree(d_f, s);
+  SSF_HIP(e);
+  return SSF_OK;
+}
+
+extern "C" int ssf_elements_bench(ssf_elements *el, const double *lhs_dev, const double *pv_dev,
+                                  int64_t npv, double *strain_dev, double *stress_dev,
+                                  double *ifor_dev, int reps, double *usec)
+{
+  SSF_ENTER();
+  SSF_CHECK(el && usec && reps > 0, SSF_EINVAL, "ssf_elements_bench: bad argument");
+  SSF_CHECK(lhs_dev || el->n == 0, SSF_EINVAL, "ssf_elements_bench: NULL solution");
+  SSF_CHECK(npv >= el->maxpv && (pv_dev || el->maxpv == 0), SSF_EINVAL,
+            "ssf_elements_bench: prescribed values missing for the code numbers");
+  SSF_HIP(hipSetDevice(el->ctx->device));
+  hipStream_t s = el->ctx->stream;
+  hipEvent_t k0, k1, c0, c1;
+  SSF_HIP(hipEventCreate(&k0));
+  SSF_HIP(hipEventCreate(&k1));
+  SSF_HIP(hipEventCreate(&c0));
+  SSF_HIP(hipEventCreate(&c1));
+  hipError_t e = post_launch(el, lhs_dev, pv_dev, strain_dev, stress_dev, ifor_dev, nullptr,
+                             nullptr);  // warm-up
+  double tk = 0.0, tc = 0.0;
+  for (int r = 0; r < reps && e == hipSuccess; r++) {
+    if (e == hipSuccess) e = hipEventRecord(c0, s);
+    if (e == hipSuccess) e = post_launch(el, lhs_dev, pv_dev, strain_dev, stress_dev, ifor_dev, k0, k1);
+    if (e == hipSuccess) e = hipEventRecord(c1, s);
+    if (e == hipSuccess) e = hipEventSynchronize(c1);
+    float a = 0.f, b = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&a, k0, k1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&b, c0, c1);
+    tk += a;
+    tc += b;
+  }
+  (void)hipEventDestroy(k0);
+  (void)hipEventDestroy(k1);
+  (void)hipEventDestroy(c0);
+  (void)hipEventDestroy(c1);
+  SSF_HIP(e);
+  usec[0] = 1e3 * tk / reps;
+  usec[1] = 1e3 * tc / reps;
+  return SSF_OK;
+}

@@ -86,6 +86,12 @@ SIGNATURES = {
                                ctypes.POINTER(_L)]),
     "ssf_server_wait": (_I, [_P, _L, _P, ctypes.POINTER(SolveInfo), _P]),
     "ssf_server_stats": (_I, [_P, _P]),
+    "ssf_elements_create": (_I, [_P, _I, _L, _L, _P, _L, _P, _P, _P, _P, _P, ctypes.POINTER(_P)]),
+    "ssf_elements_destroy": (_I, [_P]),
+    "ssf_elements_sizes": (_I, [_P, _P, _P, _P]),
+    "ssf_elements_results": (_I, [_P, _P, _P, _L, _P, _P, _P]),
+    "ssf_elements_results_dev": (_I, [_P, _P, _P, _L, _P, _P, _P]),
+    "ssf_elements_bench": (_I, [_P, _P, _P, _L, _P, _P, _P, _I, _P]),
 }
 
 _lib = None
@@ -417,6 +423,66 @@ class Server:
     def close(self):
         if self.h:
             lib().ssf_server_destroy(self.h)
+            self.h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Elements:
+    """Device-resident element set of a mesh.Problem for the post-solve element
+    pass: integration-point strains / stresses (heat: gradients / fluxes) and the
+    internal forces (heat: internal fluxes) of a solution."""
+
+    def __init__(self, ctx: Context, prob: mesh.Problem):
+        self.ctx = ctx
+        self.n = prob.n
+        self.ne = prob.ne
+        cn = _i64(prob.elem_codes())
+        conn = _i64(prob.conn)
+        xyz = _f64(prob.coords)
+        e = _f64(prob.e)
+        nu = _f64(prob.nu) if prob.kind != mesh.KIND_HEX8_HEAT else None
+        th = _f64(prob.thick) if prob.kind == mesh.KIND_QUAD4_PSTRESS else None
+        self.pv = _f64(prob.pv) if prob.pv.size else None
+        self.h = _P()
+        _check(lib().ssf_elements_create(ctx.h, prob.kind, prob.n, xyz.shape[0], _p(xyz),
+                                         conn.shape[0], _p(conn), _p(cn), _p(e), _p(nu), _p(th),
+                                         ctypes.byref(self.h)))
+        a = np.zeros(3, np.int32)
+        _check(lib().ssf_elements_sizes(self.h, _p(a[0:1]), _p(a[1:2]), _p(a[2:3])))
+        self.ngp, self.ncomp, self.ndofe = (int(v) for v in a)
+
+    def results(self, lhs, pv=None, strain=True, stress=True, forces=True):
+        """(strain [ne, ngp, ncomp], stress [ne, ngp, ncomp], ifor [n]); outputs
+        not asked for are None. pv defaults to the problem's prescribed values."""
+        pv = self.pv if pv is None else _f64(pv)
+        shp = (self.ne, self.ngp, self.ncomp)
+        eps = np.zeros(shp) if strain else None
+        sig = np.zeros(shp) if stress else None
+        f = np.zeros(self.n) if forces else None
+        _check(lib().ssf_elements_results(self.h, _p(_f64(lhs)), _p(pv),
+                                          0 if pv is None else pv.shape[0], _p(eps), _p(sig),
+                                          _p(f)))
+        return eps, sig, f
+
+    def results_dev(self, lhs_ptr, pv_ptr, npv, strain_ptr, stress_ptr, ifor_ptr):
+        _check(lib().ssf_elements_results_dev(self.h, lhs_ptr, pv_ptr, npv, strain_ptr,
+                                              stress_ptr, ifor_ptr))
+
+    def bench(self, lhs_ptr, pv_ptr, npv, strain_ptr, stress_ptr, ifor_ptr, reps=10):
+        """(element kernel us, whole call us), means over reps calls"""
+        us = np.zeros(2)
+        _check(lib().ssf_elements_bench(self.h, lhs_ptr, pv_ptr, npv, strain_ptr, stress_ptr,
+                                        ifor_ptr, reps, _p(us)))
+        return float(us[0]), float(us[1])
+
+    def close(self):
+        if self.h:
+            lib().ssf_elements_destroy(self.h)
             self.h = _P()
 
     def __del__(self):

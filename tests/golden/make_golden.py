@@ -12,6 +12,10 @@ Cases (small seeded meshes from ssf_amd.mesh):
 For each: element matrices K_e, scr and cr arrays (adr, ci, a) after localizing
 every K_e in element order, mxv_scr / mxv_cr of a seeded vector, and the
 cg_prec (Jacobi), scr cg and cr cg_prec (noprecond) results x, ani, ares.
+Post-solve element pass (post_*): for a seeded nodal solution post_x, the
+integration-point strains / stresses (gradients / fluxes for heat) and element
+forces of the harness's ref_*_ip, gathered by eldispl's code-number rule and added
+into the global vector by the reference's locglob in element order.
 """
 import ctypes
 import os
@@ -81,6 +85,38 @@ def solve(fn, h, n, y, ni, res, *extra):
     return x, ani.value, ares.value
 
 
+def post_pass(R, prob, x):
+    ngp, nc = O.POST_SHAPE[prob.kind]
+    cn = i64(prob.elem_codes())
+    pv = f64(prob.pv) if prob.pv.size else np.zeros(1)
+    eps = np.zeros((prob.ne, ngp, nc))
+    sig = np.zeros((prob.ne, ngp, nc))
+    ifor = np.zeros(prob.n)
+    for e in range(prob.ne):
+        nodes = prob.conn[e]
+        xx, yy, zz = (f64(prob.coords[nodes, k]) for k in range(3))
+        c = cn[e]
+        # eldispl (MEFEL/SRC/globmat.cpp:1305-1311)
+        r = np.array([pv[-k - 1] if k < 0 else (x[k - 1] if k > 0 else 0.0) for k in c])
+        e1 = np.zeros(ngp * nc)
+        s1 = np.zeros(ngp * nc)
+        f1 = np.zeros(prob.ndofe)
+        if prob.kind == mesh.KIND_HEX8_ELAST:
+            R.ref_hex8_elast_ip(_ptr(xx), _ptr(yy), _ptr(zz), prob.e[e], prob.nu[e], _ptr(r),
+                                _ptr(e1), _ptr(s1), _ptr(f1))
+        elif prob.kind == mesh.KIND_HEX8_HEAT:
+            R.ref_hex8_heat_ip(_ptr(xx), _ptr(yy), _ptr(zz), prob.e[e], _ptr(r), _ptr(e1),
+                               _ptr(s1), _ptr(f1))
+        else:
+            t = f64(np.full(4, prob.thick[e]))
+            R.ref_quad4_pstress_ip(_ptr(xx), _ptr(yy), prob.e[e], prob.nu[e], _ptr(t), _ptr(r),
+                                   _ptr(e1), _ptr(s1), _ptr(f1))
+        eps[e] = e1.reshape(ngp, nc)
+        sig[e] = s1.reshape(ngp, nc)
+        R.ref_locglob(_ptr(ifor), _ptr(f1), _ptr(i64(c).copy()), prob.ndofe)
+    return eps, sig, ifor
+
+
 def main():
     if not O.ref_available():
         sys.exit("oracle/_ref/libssfref.so missing: run make -C oracle ref")
@@ -117,6 +153,11 @@ def main():
         x, ani, ares = solve(R.ref_cr_cg_prec_noprec, hc, prob.n, y, 5000, 1e-10)
         out[f"{name}_crcg_x"], out[f"{name}_crcg_ani"], out[f"{name}_crcg_ares"] = x, ani, ares
         fc(hc)
+        # own generator per case, so the arrays above keep their values
+        xp = np.random.default_rng(1000 + list(CASES).index(name)).standard_normal(prob.n) * 1e-3
+        out[f"{name}_post_x"] = xp
+        out[f"{name}_post_eps"], out[f"{name}_post_sig"], out[f"{name}_post_ifor"] = \
+            post_pass(R, prob, xp)
         print(name, prob.n, "unknowns", flush=True)
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ref_vectors.npz")
     np.savez_compressed(path, **out)

@@ -1,0 +1,191 @@
+"""GPU post-solve element pass (ssf_elements_*) against the oracle, through the
+C ABI: integration-point strains / stresses (heat: gradients / fluxes) and the
+assembled internal forces, bitwise on the oracle's own seeded cases and the
+golden fixtures written by the reference; at larger sizes through properties
+that hold at any size (patch fields give their constant strain everywhere, the
+assembled forces equal A x - lift with the GPU's own SpMV and lift)."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from ssf_amd import gpu, mesh
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "ref_vectors.npz")
+GOLD_CASES = {
+    "elast": lambda: mesh.elastic_column(2, 2, 3, seed=21),
+    "cant": lambda: mesh.cantilever(4, 2, 2),
+    "heat": lambda: mesh.heat_cube(4, seed=22),
+    "quad": lambda: mesh.plate(6, 3, seed=23, nrhs=1),
+}
+CASES = {
+    "hex8_elast": lambda: mesh.elastic_column(5, 4, 7, seed=31),
+    "cantilever": lambda: mesh.cantilever(10, 3, 4),
+    "hex8_heat": lambda: mesh.heat_cube(9, seed=32),
+    "quad4": lambda: mesh.plate(37, 11, seed=33, nrhs=1),
+}
+
+
+def same(a, b):
+    return np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("name", list(GOLD_CASES))
+def test_post_matches_reference_fixture(ctx, name):
+    g = np.load(GOLD)
+    prob = GOLD_CASES[name]()
+    el = gpu.Elements(ctx, prob)
+    eps, sig, ifor = el.results(g[f"{name}_post_x"])
+    assert same(eps, g[f"{name}_post_eps"]) and same(sig, g[f"{name}_post_sig"])
+    assert same(ifor, g[f"{name}_post_ifor"])
+    el.close()
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_post_bitwise_vs_oracle(ctx, name):
+    prob = CASES[name]()
+    x = np.random.default_rng(41).standard_normal(prob.n) * 1e-3
+    el = gpu.Elements(ctx, prob)
+    eps, sig, ifor = el.results(x)
+    eo, so, fo = O.elem_results(prob, x)
+    assert same(eps, eo) and same(sig, so) and same(ifor, fo)
+    # outputs left out, and a second call, change nothing
+    e2, s2, f2 = el.results(x, strain=False, stress=False)
+    assert e2 is None and s2 is None and same(f2, fo)
+    e3, s3, f3 = el.results(x, forces=False)
+    assert f3 is None and same(e3, eo) and same(s3, so)
+    el.close()
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_post_after_gpu_solve(ctx, name):
+    """Post pass of the GPU's own solution: equals the oracle's post pass of the
+    same vector, and the internal forces balance the loads at the unknowns."""
+    prob = CASES[name]()
+    m, rhs = gpu.build_problem_matrix(ctx, prob)
+    x, info = m.solve(rhs[0], ni=20000, res=1e-20)
+    assert info.status == 1
+    el = gpu.Elements(ctx, prob)
+    eps, sig, ifor = el.results(x)
+    eo, so, fo = O.elem_results(prob, x)
+    assert same(eps, eo) and same(sig, so) and same(ifor, fo)
+    # K_ff x + K_fp u_p = f_ext: the internal forces equal the applied loads
+    # (heat: internal fluxes = -(loads), here zero)
+    sign = -1.0 if prob.kind == mesh.KIND_HEX8_HEAT else 1.0
+    ext = prob.rhs[0]
+    scale = np.abs(m.gmxv(x)).max()
+    assert np.abs(ifor - sign * ext).max() <= 1e-9 * scale
+
+
+def test_post_device_pointers(ctx):
+    torch = pytest.importorskip("torch")
+    prob = CASES["hex8_elast"]()
+    x = np.random.default_rng(42).standard_normal(prob.n) * 1e-3
+    el = gpu.Elements(ctx, prob)
+    eh, sh, fh = el.results(x)
+    dev = torch.device("cuda:0")
+    xd = torch.tensor(x, dtype=torch.float64, device=dev)
+    ed = torch.zeros((prob.ne, el.ngp, el.ncomp), dtype=torch.float64, device=dev)
+    sd = torch.zeros_like(ed)
+    fd = torch.zeros(prob.n, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    el.results_dev(xd.data_ptr(), None, 0, ed.data_ptr(), sd.data_ptr(), fd.data_ptr())
+    ctx.sync()
+    assert same(ed.cpu().numpy(), eh) and same(sd.cpu().numpy(), sh) and same(fd.cpu().numpy(), fh)
+    tk, tc = el.bench(xd.data_ptr(), None, 0, ed.data_ptr(), sd.data_ptr(), fd.data_ptr(), reps=3)
+    assert 0.0 < tk <= tc
+    assert same(fd.cpu().numpy(), fh)
+    el.close()
+
+
+def test_post_patch_fields_full_size(ctx):
+    """Linear fields that satisfy the problems' Dirichlet data give their exact
+    constant strain / gradient at every integration point, with the closed-form
+    stresses / fluxes of each element's material."""
+    # heat: T = 100 z is the exact solution of the cube with T=0 / T=100 faces
+    prob = mesh.heat_cube(40, seed=5)
+    z = prob.coords[:, 2]
+    x = np.zeros(prob.n)
+    m = prob.dof[:, 0] > 0
+    x[prob.dof[m, 0] - 1] = 100.0 * z[m]
+    el = gpu.Elements(ctx, prob)
+    grad, flux, f = el.results(x)
+    assert np.abs(grad - np.array([0.0, 0.0, 100.0])).max() <= 1e-11 * 100
+    kq = prob.e[:, None, None] * np.array([0.0, 0.0, 100.0])
+    assert np.abs(flux + kq).max() <= 1e-11 * np.abs(kq).max()
+    el.close()
+    # elasticity: u = z v vanishes on the clamped base
+    prob = mesh.elastic_column(12, 10, 40, seed=6)
+    v = np.array([1e-3, -2e-3, 3e-3])
+    u = prob.coords[:, 2:3] * v[None, :]
+    x = np.zeros(prob.n)
+    m = prob.dof > 0
+    x[prob.dof[m] - 1] = u[m]
+    el = gpu.Elements(ctx, prob)
+    eps, sig, f = el.results(x)
+    ev = np.array([0.0, 0.0, v[2], v[1], v[0], 0.0])
+    assert np.abs(eps - ev).max() <= 1e-12 * np.abs(v).max()
+    E, nu = prob.e[:, None, None], prob.nu[:, None, None]
+    lam, mu = E * nu / ((1 + nu) * (1 - 2 * nu)), E / (2 * (1 + nu))
+    sv = np.concatenate([lam * v[2] * np.ones((1, 1, 2)), (lam + 2 * mu) * v[2] * np.ones((1, 1, 1)),
+                         mu * v[1] * np.ones((1, 1, 1)), mu * v[0] * np.ones((1, 1, 1)),
+                         np.zeros((1, 1, 1))], axis=2) * np.ones((1, el.ngp, 1))
+    assert np.abs(sig - sv).max() <= 1e-12 * np.abs(sv).max()
+    el.close()
+    # quad4: u = x v vanishes on the clamped left edge
+    prob = mesh.plate(120, 40, seed=7, nrhs=1)
+    v = np.array([2e-3, -1e-3])
+    u = prob.coords[:, 0:1] * v[None, :]
+    x = np.zeros(prob.n)
+    m = prob.dof > 0
+    x[prob.dof[m] - 1] = u[m]
+    el = gpu.Elements(ctx, prob)
+    eps, sig, f = el.results(x)
+    nu = prob.nu[0]
+    ev = np.array([v[0], 0.0, v[1], -nu / (1 - nu) * v[0]])
+    assert np.abs(eps - ev).max() <= 1e-12 * np.abs(v).max()
+    el.close()
+
+
+@pytest.mark.parametrize("make", [lambda: mesh.heat_cube(36, seed=8),
+                                  lambda: mesh.elastic_column(10, 10, 30, seed=9),
+                                  lambda: mesh.plate(150, 60, seed=10, nrhs=1)])
+def test_post_forces_equal_a_x_minus_lift_full_size(ctx, make):
+    """Size-independent: the assembled element forces of any vector equal the
+    GPU's own A x minus its Dirichlet lift (heat: the negative), to rounding."""
+    prob = make()
+    m, rhs = gpu.build_problem_matrix(ctx, prob)
+    lift = rhs[0] - prob.rhs[0]
+    x = np.random.default_rng(43).standard_normal(prob.n) * 1e-3
+    el = gpu.Elements(ctx, prob)
+    _, _, f = el.results(x, strain=False, stress=False)
+    ax = m.gmxv(x) - lift
+    sign = -1.0 if prob.kind == mesh.KIND_HEX8_HEAT else 1.0
+    assert np.abs(f - sign * ax).max() <= 1e-11 * np.abs(ax).max()
+    # a sample of elements checked bit for bit against the oracle
+    rng = np.random.default_rng(44)
+    cn = prob.elem_codes()
+    pv = prob.pv if prob.pv.size else np.zeros(1)
+    eps, sig, _ = el.results(x, forces=False)
+    for e in rng.choice(prob.ne, 64, replace=False):
+        nodes = prob.conn[e]
+        r = np.array([pv[-k - 1] if k < 0 else (x[k - 1] if k > 0 else 0.0) for k in cn[e]])
+        eo, so, _ = O.elem_ip(prob.kind, prob.coords[nodes, 0], prob.coords[nodes, 1],
+                              prob.coords[nodes, 2], prob.e[e], prob.nu[e], prob.thick[e], r)
+        assert same(eps[e], eo) and same(sig[e], so)
+    el.close()
+
+
+def test_post_argument_errors(ctx):
+    prob = mesh.heat_cube(3, seed=1)
+    el = gpu.Elements(ctx, prob)
+    with pytest.raises(gpu.SsfError):
+        el.results(np.zeros(prob.n), pv=np.zeros(1))  # code numbers refer to pv[1]
+    el.close()
+    bad = mesh.heat_cube(3, seed=1)
+    bad.n = bad.n - 1  # a code number beyond n
+    with pytest.raises(gpu.SsfError):
+        gpu.Elements(ctx, bad)
