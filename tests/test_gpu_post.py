@@ -130,9 +130,11 @@ def test_post_patch_fields_full_size(ctx):
     assert np.abs(eps - ev).max() <= 1e-12 * np.abs(v).max()
     E, nu = prob.e[:, None, None], prob.nu[:, None, None]
     lam, mu = E * nu / ((1 + nu) * (1 - 2 * nu)), E / (2 * (1 + nu))
-    sv = np.concatenate([lam * v[2] * np.ones((1, 1, 2)), (lam + 2 * mu) * v[2] * np.ones((1, 1, 1)),
-                         mu * v[1] * np.ones((1, 1, 1)), mu * v[0] * np.ones((1, 1, 1)),
-                         np.zeros((1, 1, 1))], axis=2) * np.ones((1, el.ngp, 1))
+    sv = np.zeros((prob.ne, 1, 6))
+    sv[:, :, 0:2] = lam * v[2]
+    sv[:, :, 2:3] = (lam + 2 * mu) * v[2]
+    sv[:, :, 3:4] = mu * v[1]
+    sv[:, :, 4:5] = mu * v[0]
     assert np.abs(sig - sv).max() <= 1e-12 * np.abs(sv).max()
     el.close()
     # quad4: u = x v vanishes on the clamped left edge
