@@ -14,6 +14,10 @@ restatement of the reference on smaller meshes by the tests):
 The roofline figure is the FP32 SpMV of the mixed solve: the bytes of the
 format it streams per launch, divided by its mean duration, sampled inside the
 timed solves with the launch's own start/stop events (hipExtLaunchKernelGGL).
+The post-solve element pass (temperature gradients and fluxes at the 8 points of
+every element, assembled internal fluxes) runs on the mixed solution and is
+reported under "post" with its own bytes and kernel time; --only-assembly times
+the element kernels and the post pass of the 64x64x256 elasticity column.
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -198,7 +202,7 @@ def bench_served(ctx, args, gpu, mesh, torch):
     return out
 
 
-def bench_assembly(ctx, args, gpu, mesh):
+def bench_assembly(ctx, args, gpu, mesh, torch=None):
     """Element kernels + coloured assembly of the 64x64x256 hex8 elasticity column
     (1 048 576 elements, 3 244 800 unknowns): device time of the element kernel
     launches (HIP events around ssf_matrix_assemble's kernels are not exposed, so
@@ -221,6 +225,70 @@ def bench_assembly(ctx, args, gpu, mesh):
            "gflops_call": flop / dt / 1e9}
     log(f"[assembly] {out}")
     m.close()
+    if torch is not None:
+        x = torch.tensor(np.random.default_rng(1).standard_normal(prob.n) * 1e-3,
+                         dtype=torch.float64, device=torch.device("cuda:0"))
+        out["post"] = bench_post(ctx, args, gpu, mesh, torch, prob, x,
+                                 f"hex8 elasticity {args.asm_nel}x{args.asm_nel}x{4 * args.asm_nel}")
+    return out
+
+
+def post_bytes(prob, ngp, ncomp, mesh):
+    """Algorithmic HBM bytes of one post-solve pass: every input read once (element
+    nodes and code numbers as int32, material, node coordinates, the solution and
+    the prescribed values) and every output written once (strains, stresses, the
+    assembled forces). Returns (element kernel bytes, whole pass bytes)."""
+    ne, nne, nd = prob.ne, prob.nne, prob.ndofe
+    nmat = 1 + (prob.kind != mesh.KIND_HEX8_HEAT) + (prob.kind == mesh.KIND_QUAD4_PSTRESS)
+    rd = ne * (4 * nne + 4 * nd + 8 * nmat) + 24 * prob.coords.shape[0] + 8 * prob.n
+    rd += 8 * prob.pv.size
+    ip = 2 * ne * ngp * ncomp * 8
+    return rd + ip, rd + ip + 8 * prob.n
+
+
+def bench_post(ctx, args, gpu, mesh, torch, prob, x_dev, label):
+    """Post-solve element pass (integration-point strains / stresses or gradients /
+    fluxes, assembled internal forces) of the solution x_dev on prob: mean device
+    time of the element kernel (its own start / stop events) and of the whole call,
+    against the algorithmic bytes; CPU baseline: the oracle's post pass on a
+    smaller mesh of the same kind, scaled per element."""
+    el = gpu.Elements(ctx, prob)
+    dev = torch.device("cuda:0")
+    shp = (prob.ne, el.ngp, el.ncomp)
+    eps = torch.empty(shp, dtype=torch.float64, device=dev)
+    sig = torch.empty(shp, dtype=torch.float64, device=dev)
+    ifor = torch.empty(prob.n, dtype=torch.float64, device=dev)
+    pv = torch.tensor(prob.pv, dtype=torch.float64, device=dev) if prob.pv.size else None
+    torch.cuda.synchronize()
+    pvp = pv.data_ptr() if pv is not None else None
+    npv = int(prob.pv.size)
+    tk, tc = el.bench(x_dev.data_ptr(), pvp, npv, eps.data_ptr(), sig.data_ptr(),
+                      ifor.data_ptr(), reps=args.post_reps)
+    bk, bc = post_bytes(prob, el.ngp, el.ncomp, mesh)
+    out = {"workload": f"{label}, {prob.ne} elements, n={prob.n}",
+           "element_kernel_us": tk, "call_us": tc,
+           "element_kernel_bytes": bk, "call_bytes": bc,
+           "element_kernel_gbs": bk / (tk * 1e-6) / 1e9, "call_gbs": bc / (tc * 1e-6) / 1e9,
+           "elements_per_s": prob.ne / (tc * 1e-6)}
+    out["element_kernel_frac"] = out["element_kernel_gbs"] * 1e9 / PEAK_HBM
+    el.close()
+    del eps, sig, ifor, pv
+    torch.cuda.empty_cache()
+    if not args.no_cpu:
+        import oracle as O
+        small = (mesh.heat_cube(32, seed=2) if prob.kind == mesh.KIND_HEX8_HEAT else
+                 mesh.elastic_column(16, 16, 64, seed=3) if prob.kind == mesh.KIND_HEX8_ELAST
+                 else mesh.plate(256, 128, seed=4, nrhs=1))
+        xs = np.random.default_rng(0).standard_normal(small.n) * 1e-3
+        t = time.perf_counter()
+        O.elem_results(small, xs)
+        dt = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": small.ne / dt, "unit": "elements/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"oracle post pass (cr_elem_results_mesh, 1 thread) on "
+                                         f"{small.name}, {small.ne} elements"}
+        out["speedup_vs_cpu"] = out["elements_per_s"] / out["cpu_baseline"]["value"]
+    log(f"[post] {out}")
     return out
 
 
@@ -249,6 +317,8 @@ def main():
     ap.add_argument("--only-assembly", action="store_true")
     ap.add_argument("--asm-nel", type=int, default=64)
     ap.add_argument("--asm-reps", type=int, default=3)
+    ap.add_argument("--post-reps", type=int, default=10)
+    ap.add_argument("--no-post", action="store_true", help="skip the post-solve element pass")
     ap.add_argument("--serve-count", type=int, default=24)
     ap.add_argument("--serve-res", type=float, default=1e-22)
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[20.0])
@@ -271,7 +341,7 @@ def main():
         return
     if args.only_assembly:
         ctx = gpu.Context(0)
-        print(json.dumps(bench_assembly(ctx, args, gpu, mesh)), flush=True)
+        print(json.dumps(bench_assembly(ctx, args, gpu, mesh, torch)), flush=True)
         return
     t0 = time.time()
     prob = mesh.heat_cube(args.nel, seed=2)
@@ -380,6 +450,10 @@ def main():
                 and (t[0] == 2 or t[1] == tune32[1])):
             traffic = tr["fp32"]["bytes_per_launch"]
 
+    post = None
+    if not args.no_post:
+        post = bench_post(ctx, args, gpu, mesh, torch, prob, x, f"heat hex8 {args.nel}^3")
+
     batched = None
     if not args.no_batch:
         m.close()
@@ -423,6 +497,7 @@ def main():
                      "bytes_per_launch": b32_act, "bytes_per_launch_canonical_csr": b32_can,
                      "usec_per_launch": us32, "samples": mixed["spmv_samples"]},
         "cpu_baseline": cpu,
+        "post": post,
         "batched": batched,
         "served": served,
     }

@@ -59,46 +59,69 @@ struct Shape<SSF_ELEM_QUAD4_PSTRESS> {
 
 constexpr double kGP = 0.577350269189626;  // GEFEL/intp.cpp:26-27, weights 1
 
-// natural derivatives of the trilinear hexahedron, GEFEL/basefun.cpp:2317-2375
-__device__ __forceinline__ void hex_dx(double *n, double y, double z)
+// The NGP lanes of an element are consecutive lanes of one wavefront and only
+// they touch the element's LDS records: a wavefront-local barrier with a
+// workgroup-scope fence (waits for the LDS writes) orders them.
+__device__ __forceinline__ void wave_sync()
 {
-#pragma clang fp contract(off)
-  n[0] = (1.0 + y) * (1.0 + z) / 8.0;
-  n[1] = (1.0 + y) * (1.0 + z) / (-8.0);
-  n[2] = (1.0 - y) * (1.0 + z) / (-8.0);
-  n[3] = (1.0 - y) * (1.0 + z) / 8.0;
-  n[4] = (1.0 + y) * (1.0 - z) / 8.0;
-  n[5] = (1.0 + y) * (1.0 - z) / (-8.0);
-  n[6] = (1.0 - y) * (1.0 - z) / (-8.0);
-  n[7] = (1.0 - y) * (1.0 - z) / 8.0;
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+  __builtin_amdgcn_wave_barrier();
 }
-__device__ __forceinline__ void hex_dy(double *n, double x, double z)
+
+// Natural derivatives of the trilinear hexahedron (GEFEL/basefun.cpp:2317-2375),
+// e.g. dN_0/dxi = (1 + eta)(1 + zeta) / 8 for node 0 at (+,+,+). At a Gauss point
+// (+-g, +-g, +-g) each factor is 1 + g or 1 - g, so the product / 8 is one of three
+// constants P, M, Q, folded at compile time with the same IEEE operations
+// ((1+g)(1+g)/8 and so on; the / (-8.0) forms are their exact negatives): the
+// selected constant equals the reference's runtime value bit for bit.
+constexpr double kP = (1.0 + kGP) * (1.0 + kGP) / 8.0;
+constexpr double kM = (1.0 + kGP) * (1.0 - kGP) / 8.0;
+constexpr double kQ = (1.0 - kGP) * (1.0 - kGP) / 8.0;
+
+__device__ __forceinline__ double pick(bool p1, bool p2)
 {
-#pragma clang fp contract(off)
-  n[0] = (1.0 + x) * (1.0 + z) / 8.0;
-  n[1] = (1.0 - x) * (1.0 + z) / 8.0;
-  n[2] = (1.0 - x) * (1.0 + z) / (-8.0);
-  n[3] = (1.0 + x) * (1.0 + z) / (-8.0);
-  n[4] = (1.0 + x) * (1.0 - z) / 8.0;
-  n[5] = (1.0 - x) * (1.0 - z) / 8.0;
-  n[6] = (1.0 - x) * (1.0 - z) / (-8.0);
-  n[7] = (1.0 + x) * (1.0 - z) / (-8.0);
+  return p1 ? (p2 ? kP : kM) : (p2 ? kM : kQ);
 }
-__device__ __forceinline__ void hex_dz(double *n, double x, double y)
+
+// node a sits at (sx, sy, sz) in {+1,-1}^3: (+,+,+), (-,+,+), (-,-,+), (+,-,+),
+// then the same at zeta = -1; Gauss point g has xi > 0 iff g & 4, eta > 0 iff
+// g & 2, zeta > 0 iff g & 1 (MEFEL/SRC/linhex.cpp:598-606 loop order)
+__device__ __forceinline__ void hex_nat(int g, double *nx, double *ny, double *nz)
 {
-#pragma clang fp contract(off)
-  n[0] = (1.0 + x) * (1.0 + y) / 8.0;
-  n[1] = (1.0 - x) * (1.0 + y) / 8.0;
-  n[2] = (1.0 - x) * (1.0 - y) / 8.0;
-  n[3] = (1.0 + x) * (1.0 - y) / 8.0;
-  n[4] = (1.0 + x) * (1.0 + y) / (-8.0);
-  n[5] = (1.0 - x) * (1.0 + y) / (-8.0);
-  n[6] = (1.0 - x) * (1.0 - y) / (-8.0);
-  n[7] = (1.0 + x) * (1.0 - y) / (-8.0);
+  const bool gx = g & 4, gy = g & 2, gz = g & 1;
+#pragma unroll
+  for (int a = 0; a < 8; a++) {
+    const bool px = (a == 0 || a == 3 || a == 4 || a == 7);
+    const bool py = (a == 0 || a == 1 || a == 4 || a == 5);
+    const bool pz = a < 4;
+    const double vx = pick(py == gy, pz == gz), vy = pick(px == gx, pz == gz),
+                 vz = pick(px == gx, py == gy);
+    nx[a] = px ? vx : -vx;
+    ny[a] = py ? vy : -vy;
+    nz[a] = pz ? vz : -vz;
+  }
+}
+
+// a / b rounded to nearest, given y = RN(1/b): Markstein's division method with
+// FMA remainders. q = RN(a y) is within 2 ulp of a/b; one correction
+// q + (a - b q) y leaves a faithful quotient, for which the remainder a - b q is
+// exact and, by Markstein's theorem (y the correctly rounded reciprocal, q
+// faithful), the second correction rounds to RN(a / b). Valid while a - b q does
+// not underflow (|a| above ~2^-960), far below the values of element geometry.
+__device__ __forceinline__ double div_rn(double a, double b, double y)
+{
+  double q = a * y;
+  double r = __builtin_fma(-b, q, a);
+  q = __builtin_fma(r, y, q);
+  r = __builtin_fma(-b, q, a);
+  return __builtin_fma(r, y, q);
 }
 
 // GEFEL/difcalc.cpp:487-571 derivatives_3d: natural -> global derivatives in
-// place; returns the signed Jacobian
+// place; returns the signed Jacobian. The 24 quotients by the Jacobian share one
+// IEEE reciprocal (div_rn). A degenerate element (zero Jacobian) gives NaN
+// derivatives where the reference's division gives infinities.
 __device__ __forceinline__ double hex_global(double *dx, double *dy, double *dz,
                                              const double *x, const double *y,
                                              const double *z)
@@ -123,40 +146,39 @@ __device__ __forceinline__ double hex_global(double *dx, double *dy, double *dz,
   const double c00 = d5 * d9 - d8 * d6, c01 = d7 * d6 - d9 * d4, c02 = d4 * d8 - d7 * d5;
   const double c10 = d8 * d3 - d9 * d2, c11 = d1 * d9 - d7 * d3, c12 = d7 * d2 - d8 * d1;
   const double c20 = d2 * d6 - d5 * d3, c21 = d4 * d3 - d6 * d1, c22 = d1 * d5 - d4 * d2;
+  const double rj = 1.0 / jac;
 #pragma unroll
   for (int i = 0; i < 8; i++) {
     const double ddx = dx[i], ddy = dy[i], ddz = dz[i];
-    dx[i] = (ddx * c00 + ddy * c01 + ddz * c02) / jac;
-    dy[i] = (ddx * c10 + ddy * c11 + ddz * c12) / jac;
-    dz[i] = (ddx * c20 + ddy * c21 + ddz * c22) / jac;
+    dx[i] = div_rn(ddx * c00 + ddy * c01 + ddz * c02, jac, rj);
+    dy[i] = div_rn(ddx * c10 + ddy * c11 + ddz * c12, jac, rj);
+    dz[i] = div_rn(ddx * c20 + ddy * c21 + ddz * c22, jac, rj);
   }
   return jac;
 }
 
-// One integration point of an element: writes the point's NC strain and stress
-// components and its force contributions contr[ND] (already times the point's
-// Jacobian factor).
+// One integration point of an element. eval() writes the point's NC strain and
+// stress components and keeps what its force contributions need; force(c, a) is
+// the contribution to DOF c of node a, already times the point's Jacobian factor.
 template <int KIND>
 struct Point;
 
 // hex8 elasticity: rows of the strain-displacement matrix (linhex.cpp:167-183)
 // eps_x, eps_y, eps_z, gamma_yz, gamma_xz, gamma_xy; D of matstiff_spacestr
-// (elastisomat.cpp:395-410)
+// (elastisomat.cpp:395-410); forces B^T sigma jac (signed, linhex.cpp:2181-2184)
 template <>
 struct Point<SSF_ELEM_HEX8_ELAST> {
-  __device__ static void eval(int g, const double *x, const double *y, const double *z,
-                              const double *r, double em, double nu, double th, double *eps,
-                              double *sig, double *contr)
+  double bx[8], by[8], bz[8];
+  double s0, s1, s2, s3, s4, s5, w;
+
+  __device__ void eval(int g, const double *x, const double *y, const double *z,
+                       const double *r, double em, double nu, double th, double *eps,
+                       double *sig)
   {
 #pragma clang fp contract(off)
     (void)th;
-    const double xi = (g & 4) ? kGP : -kGP, eta = (g & 2) ? kGP : -kGP,
-                 zeta = (g & 1) ? kGP : -kGP;
-    double bx[8], by[8], bz[8];
-    hex_dx(bx, eta, zeta);
-    hex_dy(by, xi, zeta);
-    hex_dz(bz, xi, eta);
-    const double jac = hex_global(bx, by, bz, x, y, z);
+    hex_nat(g, bx, by, bz);
+    w = hex_global(bx, by, bz, x, y, z);
     double e0 = 0.0, e1 = 0.0, e2 = 0.0, e3 = 0.0, e4 = 0.0, e5 = 0.0;
 #pragma unroll
     for (int a = 0; a < 8; a++) e0 += r[3 * a] * bx[a];
@@ -182,46 +204,46 @@ struct Point<SSF_ELEM_HEX8_ELAST> {
     const double gm = em / 2.0 / (1.0 + nu);
     const double s = em / (1.0 + nu) / (1.0 - 2.0 * nu);
     const double d00 = s * (1.0 - nu), d01 = s * nu;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    s0 = 0.0;
+    s1 = 0.0;
+    s2 = 0.0;
     s0 += e0 * d00; s0 += e1 * d01; s0 += e2 * d01;
     s1 += e0 * d01; s1 += e1 * d00; s1 += e2 * d01;
     s2 += e0 * d01; s2 += e1 * d01; s2 += e2 * d00;
-    const double s3 = e3 * gm, s4 = e4 * gm, s5 = e5 * gm;
+    s3 = e3 * gm;
+    s4 = e4 * gm;
+    s5 = e5 * gm;
     eps[0] = e0; eps[1] = e1; eps[2] = e2; eps[3] = e3; eps[4] = e4; eps[5] = e5;
     sig[0] = s0; sig[1] = s1; sig[2] = s2; sig[3] = s3; sig[4] = s4; sig[5] = s5;
-#pragma unroll
-    for (int a = 0; a < 8; a++) {
-      double c = 0.0;
-      c += s0 * bx[a]; c += s4 * bz[a]; c += s5 * by[a];
-      contr[3 * a] = c * jac;
-      c = 0.0;
-      c += s1 * by[a]; c += s3 * bz[a]; c += s5 * bx[a];
-      contr[3 * a + 1] = c * jac;
-      c = 0.0;
-      c += s2 * bz[a]; c += s3 * by[a]; c += s4 * bx[a];
-      contr[3 * a + 2] = c * jac;
-    }
+  }
+  // column 3a + c of B^T sigma: the nonzero rows of that column in row order
+  __device__ double force(int c, int a) const
+  {
+#pragma clang fp contract(off)
+    double q = 0.0;
+    if (c == 0) { q += s0 * bx[a]; q += s4 * bz[a]; q += s5 * by[a]; }
+    else if (c == 1) { q += s1 * by[a]; q += s3 * bz[a]; q += s5 * bx[a]; }
+    else { q += s2 * bz[a]; q += s3 * by[a]; q += s4 * bx[a]; }
+    return q * w;
   }
 };
 
 // hex8 heat: gradient = grad N t, flux q = -k grad (isotrmat), internal fluxes
-// with |J| (linhext.cpp:1129-1132)
+// with |J| (linhext.cpp:1124-1132)
 template <>
 struct Point<SSF_ELEM_HEX8_HEAT> {
-  __device__ static void eval(int g, const double *x, const double *y, const double *z,
-                              const double *r, double kc, double nu, double th, double *eps,
-                              double *sig, double *contr)
+  double bx[8], by[8], bz[8];
+  double q0, q1, q2, w;
+
+  __device__ void eval(int g, const double *x, const double *y, const double *z,
+                       const double *r, double kc, double nu, double th, double *eps,
+                       double *sig)
   {
 #pragma clang fp contract(off)
     (void)nu;
     (void)th;
-    const double xi = (g & 4) ? kGP : -kGP, eta = (g & 2) ? kGP : -kGP,
-                 zeta = (g & 1) ? kGP : -kGP;
-    double bx[8], by[8], bz[8];
-    hex_dx(bx, eta, zeta);
-    hex_dy(by, xi, zeta);
-    hex_dz(bz, xi, eta);
-    double jac = hex_global(bx, by, bz, x, y, z);
+    hex_nat(g, bx, by, bz);
+    const double jac = hex_global(bx, by, bz, x, y, z);
     double g0 = 0.0, g1 = 0.0, g2 = 0.0;
 #pragma unroll
     for (int a = 0; a < 8; a++) g0 += r[a] * bx[a];
@@ -229,28 +251,35 @@ struct Point<SSF_ELEM_HEX8_HEAT> {
     for (int a = 0; a < 8; a++) g1 += r[a] * by[a];
 #pragma unroll
     for (int a = 0; a < 8; a++) g2 += r[a] * bz[a];
-    const double q0 = 0.0 - kc * g0, q1 = 0.0 - kc * g1, q2 = 0.0 - kc * g2;
+    q0 = 0.0 - kc * g0;
+    q1 = 0.0 - kc * g1;
+    q2 = 0.0 - kc * g2;
     eps[0] = g0; eps[1] = g1; eps[2] = g2;
     sig[0] = q0; sig[1] = q1; sig[2] = q2;
-    if (jac < 0.0) jac = fabs(jac);
-#pragma unroll
-    for (int a = 0; a < 8; a++) {
-      double c = 0.0;
-      c += q0 * bx[a]; c += q1 * by[a]; c += q2 * bz[a];
-      contr[a] = c * jac;
-    }
+    w = jac < 0.0 ? fabs(jac) : jac;
+  }
+  __device__ double force(int c, int a) const
+  {
+#pragma clang fp contract(off)
+    (void)c;
+    double q = 0.0;
+    q += q0 * bx[a]; q += q1 * by[a]; q += q2 * bz[a];
+    return q * w;
   }
 };
 
 // quad4 plane stress: rows eps_x, eps_y, gamma_xy (plelemlq.cpp:294-314); the
 // stored strain gains eps_z = -nu/(1-nu)(eps_x+eps_y) and the stress a zero
 // fourth component (elastisomat.cpp:640-643); thickness interpolated at the
-// point (bf_lin_4_2d, GEFEL/basefun.cpp:609-615)
+// point (bf_lin_4_2d, GEFEL/basefun.cpp:609-615); forces B^T sigma jac thick
 template <>
 struct Point<SSF_ELEM_QUAD4_PSTRESS> {
-  __device__ static void eval(int g, const double *x, const double *y, const double *z,
-                              const double *r, double em, double nu, double th, double *eps,
-                              double *sig, double *contr)
+  double bx[4], by[4];
+  double s0, s1, s2, w;
+
+  __device__ void eval(int g, const double *x, const double *y, const double *z,
+                       const double *r, double em, double nu, double th, double *eps,
+                       double *sig)
   {
 #pragma clang fp contract(off)
     (void)z;
@@ -277,7 +306,6 @@ struct Point<SSF_ELEM_QUAD4_PSTRESS> {
       d4 += ny[a] * y[a];
     }
     const double jac = d1 * d4 - d2 * d3;
-    double bx[4], by[4];
 #pragma unroll
     for (int a = 0; a < 4; a++) {
       bx[a] = (nx[a] * d4 - ny[a] * d3) / jac;
@@ -295,25 +323,27 @@ struct Point<SSF_ELEM_QUAD4_PSTRESS> {
     }
     const double c = em / (1.0 - nu * nu);
     const double cn = c * nu, d22 = em / 2.0 / (1.0 + nu);
-    double s0 = 0.0, s1 = 0.0;
+    s0 = 0.0;
+    s1 = 0.0;
     s0 += e0 * c; s0 += e1 * cn;
     s1 += e0 * cn; s1 += e1 * c;
-    const double s2 = e2 * d22;
+    s2 = e2 * d22;
     eps[0] = e0; eps[1] = e1; eps[2] = e2; eps[3] = -nu / (1.0 - nu) * (e0 + e1);
     sig[0] = s0; sig[1] = s1; sig[2] = s2; sig[3] = 0.0;
-    const double w = jac * thick;
-#pragma unroll
-    for (int a = 0; a < 4; a++) {
-      double q = 0.0;
-      q += s0 * bx[a]; q += s2 * by[a];
-      contr[2 * a] = q * w;
-      q = 0.0;
-      q += s1 * by[a]; q += s2 * bx[a];
-      contr[2 * a + 1] = q * w;
-    }
+    w = jac * thick;
+  }
+  __device__ double force(int c, int a) const
+  {
+#pragma clang fp contract(off)
+    double q = 0.0;
+    if (c == 0) { q += s0 * bx[a]; q += s2 * by[a]; }
+    else { q += s1 * by[a]; q += s2 * bx[a]; }
+    return q * w;
   }
 };
 
+// Element forces are kept per element in component-major order,
+// fe[e][c][a] = force on DOF c of node a (local DOF a * DPN + c).
 template <int KIND>
 __global__ __launch_bounds__(TPB) void k_post(int64_t ne, const int *__restrict__ conn,
                                               const int *__restrict__ cn,
@@ -329,11 +359,11 @@ __global__ __launch_bounds__(TPB) void k_post(int64_t ne, const int *__restrict_
 {
   using S = Shape<KIND>;
   constexpr int NNE = S::NNE, NGP = S::NGP, DPN = S::DPN, NC = S::NC, ND = NNE * DPN;
-  constexpr int EPB = TPB / NGP, NPASS = ND / NGP;
-  static_assert(NNE == NGP && ND % NGP == 0, "one node per lane, whole passes");
+  constexpr int EPB = TPB / NGP;
+  static_assert(NNE == NGP && 64 % NGP == 0, "one node per lane, elements within a wavefront");
   __shared__ double s_x[EPB][NNE], s_y[EPB][NNE], s_z[EPB][NNE];
   __shared__ double s_r[EPB][ND];
-  __shared__ double s_c[EPB][NGP][NGP + 1];
+  __shared__ double s_c[EPB][NGP][NNE + 1];
   const int le = threadIdx.x / NGP, g = threadIdx.x % NGP;
   const int64_t e = (int64_t)xcd_block(blockIdx.x, gridDim.x) * EPB + le;
   const bool active = e < ne;
@@ -348,66 +378,82 @@ __global__ __launch_bounds__(TPB) void k_post(int64_t ne, const int *__restrict_
       s_r[le][g * DPN + c] = k > 0 ? lhs[k - 1] : (k < 0 ? pv[-k - 1] : 0.0);
     }
   }
-  __syncthreads();
-  double contr[ND];
+  wave_sync();
+  Point<KIND> pt;
   if (active) {
     double eps[NC], sig[NC];
-    Point<KIND>::eval(g, s_x[le], s_y[le], s_z[le], s_r[le], ma[e],
-                      KIND == SSF_ELEM_HEX8_HEAT ? 0.0 : mb[e], th ? th[e] : 1.0, eps, sig,
-                      contr);
+    pt.eval(g, s_x[le], s_y[le], s_z[le], s_r[le], ma[e],
+            KIND == SSF_ELEM_HEX8_HEAT ? 0.0 : mb[e], th ? th[e] : 1.0, eps, sig);
     const int64_t o = (e * NGP + g) * NC;
     if (eps_out) {
 #pragma unroll
-      for (int c = 0; c < NC; c++) eps_out[o + c] = eps[c];
+      for (int c = 0; c < NC; c++) __builtin_nontemporal_store(eps[c], eps_out + o + c);
     }
     if (sig_out) {
 #pragma unroll
-      for (int c = 0; c < NC; c++) sig_out[o + c] = sig[c];
+      for (int c = 0; c < NC; c++) __builtin_nontemporal_store(sig[c], sig_out + o + c);
     }
   }
-  if (!fe) return;  // uniform over the grid: no lane reaches the barriers below
-  // element forces: f = contr_0 + 0, then contr_g + f in point order
+  if (!fe) return;
+  // element forces, one DOF component per pass: f = contr_0 + 0, then
+  // contr_g + f in point order (elem_integration's addv)
 #pragma unroll
-  for (int p = 0; p < NPASS; p++) {
+  for (int c = 0; c < DPN; c++) {
     if (active) {
 #pragma unroll
-      for (int q = 0; q < NGP; q++) s_c[le][g][q] = contr[p * NGP + q];
+      for (int a = 0; a < NNE; a++) s_c[le][g][a] = pt.force(c, a);
     }
-    __syncthreads();
+    wave_sync();
     if (active) {
       double f = 0.0;
 #pragma unroll
       for (int h = 0; h < NGP; h++) f = s_c[le][h][g] + f;
-      fe[e * ND + p * NGP + g] = f;
+      fe[e * ND + c * NNE + g] = f;
     }
-    __syncthreads();
+    wave_sync();
   }
 }
 
 // ifor[d] = sum over the element entries of unknown d, in element order
-// (locglob into a cleared vector)
+// (locglob into a cleared vector). The entry indices and values are loaded eight
+// at a time so that their loads overlap; the sum keeps the entry order.
 __global__ __launch_bounds__(TPB) void k_post_gather(int n, const int *__restrict__ start,
                                                      const int *__restrict__ src,
                                                      const double *__restrict__ fe,
                                                      double *__restrict__ ifor)
 {
-  const int d = blockIdx.x * TPB + threadIdx.x;
+  const int d = xcd_block(blockIdx.x, gridDim.x) * TPB + threadIdx.x;
   if (d >= n) return;
+  const int k1 = start[d + 1];
   double s = 0.0;
-  for (int k = start[d], k1 = start[d + 1]; k < k1; k++) s = s + fe[src[k]];
+  for (int k = start[d]; k < k1; k += 8) {
+    int idx[8];
+    double v[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) idx[j] = k + j < k1 ? src[k + j] : -1;
+#pragma unroll
+    for (int j = 0; j < 8; j++) v[j] = idx[j] >= 0 ? fe[idx[j]] : 0.0;
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+      if (idx[j] >= 0) s = s + v[j];
+  }
   ifor[d] = s;
 }
 
 // sort keys of the (element, local DOF) entries: unknown (code number - 1), or n
-// for a fixed / prescribed DOF (sorted last, not summed)
-__global__ void k_post_keys(int64_t nent, int n, const int *__restrict__ cn, int *__restrict__ key,
-                            int *__restrict__ val)
+// for a fixed / prescribed DOF (sorted last, not summed); values: the entry's
+// place in the component-major element forces
+__global__ void k_post_keys(int64_t nent, int n, int nne, int dpn, const int *__restrict__ cn,
+                            int *__restrict__ key, int *__restrict__ val)
 {
   const int64_t k = (int64_t)blockIdx.x * TPB + threadIdx.x;
   if (k >= nent) return;
   const int c = cn[k];
+  const int nd = nne * dpn;
+  const int64_t e = k / nd;
+  const int i = (int)(k - e * nd);
   key[k] = c > 0 ? c - 1 : n;
-  val[k] = (int)k;
+  val[k] = (int)(e * nd + (i % dpn) * nne + i / dpn);
 }
 
 // start[d] = first sorted entry with key >= d, for d = 0..n
@@ -458,7 +504,7 @@ struct ssf_elements {
   int *conn = nullptr, *cn = nullptr;
   double *xyz = nullptr, *ma = nullptr, *mb = nullptr, *th = nullptr;
   int *start = nullptr, *src = nullptr;  // per-unknown entry lists
-  double *fe = nullptr;                  // element forces [ne][nd]
+  double *fe = nullptr;                  // element forces [ne][dpn][nne]
 };
 
 using namespace ssf;
@@ -563,7 +609,7 @@ extern "C" int ssf_elements_create(ssf_ctx *ctx, int kind, int64_t n, int64_t nn
   size_t tb = 0;
   if (e == hipSuccess) {
     hipLaunchKernelGGL(k_post_keys, dim3(blocks_for(nent, TPB)), dim3(TPB), 0, s, nent, (int)n,
-                       el->cn, key, val);
+                       nne, nd / nne, el->cn, key, val);
     c(hipGetLastError());
   }
   if (e == hipSuccess)

@@ -61,6 +61,24 @@ def test_post_bitwise_vs_oracle(ctx, name):
 
 
 @pytest.mark.parametrize("name", list(CASES))
+def test_post_bitwise_on_distorted_mesh(ctx, name):
+    """Node coordinates moved at random (up to 30 % of a cell): general Jacobians,
+    so every quotient, product and sum of the pass meets arbitrary operands."""
+    prob = CASES[name]()
+    rng = np.random.default_rng(45)
+    # cell size per axis of the structured grid (0 along the plate's z)
+    h = np.array([np.ptp(prob.coords[:, d]) / max(1, np.unique(prob.coords[:, d]).size - 1)
+                  for d in range(3)])
+    prob.coords = prob.coords + rng.uniform(-0.3, 0.3, prob.coords.shape) * h
+    x = rng.standard_normal(prob.n) * 1e-3
+    el = gpu.Elements(ctx, prob)
+    eps, sig, ifor = el.results(x)
+    eo, so, fo = O.elem_results(prob, x)
+    assert same(eps, eo) and same(sig, so) and same(ifor, fo)
+    el.close()
+
+
+@pytest.mark.parametrize("name", list(CASES))
 def test_post_after_gpu_solve(ctx, name):
     """Post pass of the GPU's own solution: equals the oracle's post pass of the
     same vector, and the internal forces balance the loads at the unknowns."""
