@@ -84,8 +84,9 @@ int ssf_ctx_destroy(ssf_ctx *ctx);
 int ssf_ctx_sync(ssf_ctx *ctx);
 /* the context's HIP stream (hipStream_t), for callers that enqueue around it */
 void *ssf_ctx_stream(ssf_ctx *ctx);
-/* Kernel sampling inside solves (on = 1 enables and resets): the first SpMV of
- * every iteration chunk is bracketed by HIP events on the solve stream.
+/* Kernel sampling inside solves (on = 1 enables and resets): the second SpMV of
+ * every iteration chunk is bracketed by its own start / end events
+ * (hipExtLaunchKernelGGL) on the solve stream.
  * ssf_ctx_kernel_time returns the mean duration (us) and sample count of the
  * FP64 (which = 0) or FP32 (which = 1) operator since the last reset. */
 int ssf_ctx_set_profiling(ssf_ctx *ctx, int on);

@@ -1139,8 +1139,11 @@ int run_pcg(ssf_matrix *m, const SpmvTune &t, int mode, bool jac, const T *val, 
     bool launched_now = false;
     if (it < ni) {
       const int64_t c = std::min<int64_t>(CHUNK, ni - it);
+      // the sampled SpMV is the chunk's second: the first follows the state copy
+      // and starts on cold caches
+      const int64_t qs = c > 1 ? 1 : 0;
       for (int64_t q = 0; q < c; q++) {
-        const bool smp = m->ctx->prof && q == 0;
+        const bool smp = m->ctx->prof && q == qs;
         hipError_t e = pcg_iteration<T>(t, mode, jac, m->csr(), val, diag, x, r, d, p,
                                         h, part, st, res, zero, s,
                                         smp ? m->ctx->prof_ev[k & 1][0] : nullptr,
@@ -1286,11 +1289,14 @@ static int solve_mixed(ssf_matrix *m, const ssf_solve_params *p, const double *y
     bool launched_now = false;
     if (it < p->ni) {
       const int64_t c = std::min<int64_t>(interval, p->ni - it);
+      // sampled: the chunk's second FP32 SpMV (the first follows the FP64 refresh
+      // and starts on cold caches; the inner recurrence never stops after one step)
+      const int64_t qs = c > 1 ? 1 : 0;
       for (int64_t q = 0; q < c && rc == SSF_OK; q++)
         if (pcg_iteration<float>(m->tune32, MODE_MIXED_INNER, true, A, m->val32, m->diag32, w.x32,
                                  w.r32, w.d32, w.p32, w.h32, w.part32, w.st32, 1.0f, 1e-30f, s,
-                                 (m->ctx->prof && q == 0) ? m->ctx->prof_ev[k & 1][0] : nullptr,
-                                 (m->ctx->prof && q == 0) ? m->ctx->prof_ev[k & 1][1] : nullptr) !=
+                                 (m->ctx->prof && q == qs) ? m->ctx->prof_ev[k & 1][0] : nullptr,
+                                 (m->ctx->prof && q == qs) ? m->ctx->prof_ev[k & 1][1] : nullptr) !=
             hipSuccess)
           rc = SSF_EHIP;
       if (rc == SSF_OK &&

@@ -127,7 +127,7 @@ struct ssf_ctx {
   // a context outlives every matrix created on it; matrices of one context may
   // be created and destroyed on different host threads (served solves)
   std::atomic<int> live_matrices{0};
-  // kernel sampling inside solves: the first SpMV of every iteration chunk is
+  // kernel sampling inside solves: the second SpMV of every iteration chunk is
   // bracketed by HIP events on the solve stream; [0] FP64, [1] FP32 operator
   bool prof = false;
   double prof_us[2] = {0.0, 0.0};

@@ -98,6 +98,25 @@ def test_post_after_gpu_solve(ctx, name):
     assert np.abs(ifor - sign * ext).max() <= 1e-9 * scale
 
 
+@pytest.mark.parametrize("name", ["hex8_elast", "hex8_heat", "quad4"])
+def test_post_coupled_dofs_take_the_unknown_order_path(ctx, name):
+    """Code numbers shared by two nodes (coupled DOFs) and an unknown no element
+    refers to: the forces are summed per unknown, still in element order."""
+    prob = CASES[name]()
+    free = np.nonzero((prob.dof > 0).all(axis=1))[0]
+    i, j = free[1], free[-2]
+    unused = prob.dof[j, 0]
+    prob.dof = prob.dof.copy()
+    prob.dof[j, 0] = prob.dof[i, 0]  # node j's first DOF coupled to node i's
+    x = np.random.default_rng(46).standard_normal(prob.n) * 1e-3
+    el = gpu.Elements(ctx, prob)
+    eps, sig, ifor = el.results(x)
+    eo, so, fo = O.elem_results(prob, x)
+    assert same(eps, eo) and same(sig, so) and same(ifor, fo)
+    assert ifor[unused - 1] == 0.0
+    el.close()
+
+
 def test_post_device_pointers(ctx):
     torch = pytest.importorskip("torch")
     prob = CASES["hex8_elast"]()
