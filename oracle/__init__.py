@@ -59,6 +59,10 @@ def lib():
             "cr_hex8_heat_ip": (None, [_P, _P, _P, _D, _P, _P, _P, _P]),
             "cr_elem_results_mesh": (None, [ctypes.c_int, _L, _P, _P, _P, _P, _P, _P, _L, _P,
                                             _P, _P, _P, _P]),
+            "cr_hex8_heat_cap": (None, [_P, _P, _P, _D, _P, _P]),
+            "cr_hex8_heat_cap_mesh": (None, [_L, _P, _P, _P, _P, _P]),
+            "cr_np_step_linear": (ctypes.c_int, [_L, _P, _P, _P, _P, _P, _D, _D, _P, _P, _P, _P,
+                                                 _P, _P, _L, _D, _D, _P, _P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(l, name)
@@ -102,6 +106,8 @@ def ref():
             "ref_quad4_pstress_ip": (None, [_P, _P, _D, _D, _P, _P, _P, _P, _P]),
             "ref_hex8_heat_ip": (None, [_P, _P, _P, _D, _P, _P, _P, _P]),
             "ref_locglob": (None, [_P, _P, _P, _L]),
+            "ref_hex8_heat_cap": (None, [_P, _P, _P, _D, _P, _P]),
+            "ref_np_step_linear": (None, [_P, _P, _P, _L, _D, _D, _P, _P, _P, _L, _D, _D, _P, _P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(l, name)
@@ -288,3 +294,29 @@ def elem_ip(kind, x, y, z, ma, mb, th, r):
         lib().cr_quad4_pstress_ip(_ptr(x), _ptr(y), ma, mb, _ptr(t), _ptr(r), _ptr(eps), _ptr(sig),
                                   _ptr(f))
     return eps, sig, f
+
+
+def capacity_matrices(prob, c, rho=None):
+    """C_e of every element of a hex8 heat mesh, (ne, 8, 8): specific heat c[ne],
+    density rho[ne] (None = 1), equal at the element's nodes."""
+    ne = prob.ne
+    cm = np.zeros((ne, 8, 8))
+    lib().cr_hex8_heat_cap_mesh(ne, _ptr(i64(prob.conn)), _ptr(f64(prob.coords)), _ptr(f64(c)),
+                                None if rho is None else _ptr(f64(rho)), _ptr(cm))
+    return cm
+
+
+def np_step_linear(adr, ci, aK, aC, alpha, dt, rhs, lhs, tdlhs, ni=20000, res=1e-20,
+                   zero=1e-30):
+    """One generalized-trapezoidal step; lhs and tdlhs are updated in place.
+    Returns (status, ani, ares, aA)."""
+    n = adr.shape[0] - 1
+    aA = np.zeros(aK.shape[0])
+    d, p, f = np.zeros(n), np.zeros(n), np.zeros(n)
+    ani = ctypes.c_long(-1)
+    ares = ctypes.c_double(-1.0)
+    st = lib().cr_np_step_linear(n, _ptr(adr), _ptr(ci), _ptr(f64(aK)), _ptr(f64(aC)), _ptr(aA),
+                                 alpha, dt, _ptr(f64(rhs)), _ptr(lhs), _ptr(tdlhs), _ptr(d),
+                                 _ptr(p), _ptr(f), ni, res, zero, ctypes.byref(ani),
+                                 ctypes.byref(ares))
+    return st, ani.value, ares.value, aA

@@ -1089,3 +1089,97 @@ void cr_elem_results_mesh(int kind, long ne, const long *conn, const double *xyz
         if (c[i] >= 1) ifor[c[i] - 1] += f[i];
   }
 }
+
+/* ====================================== nonstationary heat (one linear step) === */
+
+/* TRFEL/SRC/linhext.cpp:979-1029 capacity_matrix: cm = sum over the 2x2x2 points of
+ * N^T N |J| rho c (rho = approx of the nodal densities, linhext.cpp:195-204; c of
+ * isotrmat::matcap, TRFEL/SRC/isotrmat.cpp:146-156), nnj (GEFEL/matrix.cpp:3149-3174)
+ * with N the 1x8 row of the trilinear functions (GEFEL/basefun.cpp:2299-2309). */
+void cr_hex8_heat_cap(const double *x, const double *y, const double *z, double c,
+                      const double *dens, double *cm)
+{
+  double dx[8], dy[8], dz[8], n[8];
+  memset(cm, 0, 64 * sizeof(double));
+  for (int i = 0; i < 2; i++) {
+    double xi = GP2[i], w1 = GW2[i];
+    for (int j = 0; j < 2; j++) {
+      double eta = GP2[j], w2 = GW2[j];
+      for (int k = 0; k < 2; k++) {
+        double zeta = GP2[k], w3 = GW2[k];
+        /* jac_3d (GEFEL/difcalc.cpp:650-700): the determinant of derivatives_3d */
+        dx_hex(dx, eta, zeta);
+        dy_hex(dy, xi, zeta);
+        dz_hex(dz, xi, eta);
+        double jac = derivatives_3d(dx, dy, dz, x, y, z, xi, eta, zeta);
+        n[0] = (1.0 + xi) * (1.0 + eta) * (1.0 + zeta) / 8.0;
+        n[1] = (1.0 - xi) * (1.0 + eta) * (1.0 + zeta) / 8.0;
+        n[2] = (1.0 - xi) * (1.0 - eta) * (1.0 + zeta) / 8.0;
+        n[3] = (1.0 + xi) * (1.0 - eta) * (1.0 + zeta) / 8.0;
+        n[4] = (1.0 + xi) * (1.0 + eta) * (1.0 - zeta) / 8.0;
+        n[5] = (1.0 - xi) * (1.0 + eta) * (1.0 - zeta) / 8.0;
+        n[6] = (1.0 - xi) * (1.0 - eta) * (1.0 - zeta) / 8.0;
+        n[7] = (1.0 + xi) * (1.0 - eta) * (1.0 - zeta) / 8.0;
+        double rho = 0.0;
+        for (int a = 0; a < 8; a++) rho += n[a] * dens[a];
+        if (jac < 0.0) jac = fabs(jac);
+        jac *= w1 * w2 * w3 * rho * c;
+        for (int a = 0; a < 8; a++) {
+          double s = 0.0;
+          s += n[a] * n[a];
+          cm[a * 8 + a] += s * jac;
+          for (int b = a + 1; b < 8; b++) {
+            s = 0.0;
+            s += n[a] * n[b];
+            cm[a * 8 + b] += s * jac;
+            cm[b * 8 + a] += s * jac;
+          }
+        }
+      }
+    }
+  }
+}
+
+void cr_hex8_heat_cap_mesh(long ne, const long *conn, const double *xyz, const double *c,
+                           const double *rho, double *cm)
+{
+  for (long el = 0; el < ne; el++) {
+    double x[8], y[8], z[8], d[8];
+    for (int a = 0; a < 8; a++) {
+      long nd = conn[el * 8 + a];
+      x[a] = xyz[3 * nd];
+      y[a] = xyz[3 * nd + 1];
+      z[a] = xyz[3 * nd + 2];
+      d[a] = rho ? rho[el] : 1.0;
+    }
+    cr_hex8_heat_cap(x, y, z, c[el], d, cm + el * 64);
+  }
+}
+
+/* TRFEL/SRC/npsolvert.cpp:367-482 one_step_linear (generalized trapezoidal rule,
+ * v-form) on symcomprow matrices K (conductivity) and C (capacity) of one pattern:
+ *   d = lhs + (1 - alpha) dt v          (:422-424)
+ *   p = K d                              (:428, mxv_scr GEFEL/scr.cpp:655)
+ *   A = K * (dt alpha) + 1.0 * C        (:432-433, scalmat_scr :760, addmat_scr :682)
+ *   f = rhs - p                          (:441-443)
+ *   A v = f by cg_prec from v (iv = 1)   (:449)
+ *   lhs = d + alpha dt v                 (:454-456)
+ * aA (negm) receives A's values; d, p, f are work vectors of length n. Returns
+ * the cg_prec status. */
+int cr_np_step_linear(long n, const long *adr, const long *ci, const double *aK,
+                      const double *aC, double *aA, double alpha, double dt,
+                      const double *rhs, double *lhs, double *tdlhs, double *d, double *p,
+                      double *f, long ni, double res, double zero, long *ani, double *ares)
+{
+  long negm = adr[n];
+  for (long j = 0; j < n; j++) d[j] = lhs[j] + (1.0 - alpha) * dt * tdlhs[j];
+  cr_mxv_scr(n, adr, ci, aK, d, p);
+  for (long i = 0; i < negm; i++) aA[i] = aK[i];
+  double s = dt * alpha;
+  for (long i = 0; i < negm; i++) aA[i] *= s;
+  for (long i = 0; i < negm; i++) aA[i] += 1.0 * aC[i];
+  for (long j = 0; j < n; j++) f[j] = rhs[j] - p[j];
+  int st = cr_scr_cg_prec(n, adr, ci, aA, tdlhs, f, ni, res, ani, ares, zero, 1);
+  for (long j = 0; j < n; j++) lhs[j] = d[j] + alpha * dt * tdlhs[j];
+  return st;
+}

@@ -31,6 +31,11 @@
  *                         under linear fields), rigid-body motions give zero strain
  *                         and force, element forces = K_e r with the pinned K_e,
  *                         assembled forces = A x - lift, D / -k closed forms
+ *   cr_hex8_heat_cap      bitwise vs oracle/_ref; unit-cube closed form (exact with
+ *                         2x2x2 points); sum of entries = rho c V
+ *   cr_np_step_linear     bitwise vs oracle/_ref (the reference's own symcomprow
+ *                         mxv_scr / scalmat_scr / addmat_scr / cg_prec); amplification
+ *                         factor of the trapezoidal rule on one unknown; steady state
  */
 #ifndef SSF_ORACLE_CPU_REF_H
 #define SSF_ORACLE_CPU_REF_H
@@ -146,6 +151,20 @@ void cr_elem_results_mesh(int kind, long ne, const long *conn, const double *xyz
                           const long *cn, const double *ma, const double *mb,
                           const double *th, long n, const double *lhs, const double *pv,
                           double *eps, double *sig, double *ifor);
+
+/* ---- nonstationary heat */
+/* TRFEL/SRC/linhext.cpp:979-1029 capacity_matrix, 8x8 row-major; dens = nodal
+ * densities (8), c = specific heat (isotrmat::matcap). */
+void cr_hex8_heat_cap(const double *x, const double *y, const double *z, double c,
+                      const double *dens, double *cm);
+void cr_hex8_heat_cap_mesh(long ne, const long *conn, const double *xyz, const double *c,
+                           const double *rho, double *cm);
+/* TRFEL/SRC/npsolvert.cpp:367-482 one_step_linear on symcomprow K and C (same
+ * pattern); returns the cg_prec status (see the definition for the steps). */
+int cr_np_step_linear(long n, const long *adr, const long *ci, const double *aK,
+                      const double *aC, double *aA, double alpha, double dt,
+                      const double *rhs, double *lhs, double *tdlhs, double *d, double *p,
+                      double *f, long ni, double res, double zero, long *ani, double *ares);
 
 #ifdef __cplusplus
 }

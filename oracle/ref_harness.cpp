@@ -417,4 +417,64 @@ void ref_hex8_heat_ip(const double *xc, const double *yc, const double *zc, doub
 // GEFEL/matrix.cpp:3500 locglob
 void ref_locglob(double *gv, const double *lv, const long *cn, long n) { locglob(gv, lv, cn, n); }
 
+// ------------------------------------------------------- nonstationary heat
+// capacity matrix in the loop order of linhext::capacity_matrix
+// (TRFEL/SRC/linhext.cpp:979-1029): GEFEL's gauss_points, jac_3d, bf_lin_hex_3d,
+// scprd (linhext::approx) and nnj; c restates isotrmat::matcap
+void ref_hex8_heat_cap(const double *xc, const double *yc, const double *zc, double c,
+                       const double *dens_nodes, double *cm_out)
+{
+  const long nne = 8;
+  vector x(nne), y(nne), z(nne), w(2), gp(2), dens(nne), bf(nne);
+  matrix n(1, nne), cm(nne, nne);
+  for (long i = 0; i < nne; i++) { x[i] = xc[i]; y[i] = yc[i]; z[i] = zc[i]; dens[i] = dens_nodes[i]; }
+  gauss_points(gp.a, w.a, 2);
+  nullm(cm);
+  for (long i = 0; i < 2; i++) {
+    double xi = gp[i], w1 = w[i];
+    for (long j = 0; j < 2; j++) {
+      double eta = gp[j], w2 = w[j];
+      for (long k = 0; k < 2; k++) {
+        double zeta = gp[k], w3 = w[k], jac, rho;
+        jac_3d(jac, x, y, z, xi, eta, zeta);
+        nullm(n);
+        bf_lin_hex_3d(n.a, xi, eta, zeta);
+        bf_lin_hex_3d(bf.a, xi, eta, zeta);
+        scprd(bf, dens, rho);
+        if (jac < 0.0) jac = fabs(jac);
+        jac *= w1 * w2 * w3 * rho * c;
+        nnj(cm.a, n.a, jac, n.m, n.n);
+      }
+    }
+  }
+  for (long i = 0; i < nne; i++)
+    for (long j = 0; j < nne; j++) cm_out[i * nne + j] = cm[i][j];
+}
+
+// one_step_linear (TRFEL/SRC/npsolvert.cpp:367-482) on the reference's own
+// symcomprow objects: K->mxv_scr, A->scalmat_scr, A->addmat_scr, A->cg_prec; the
+// vector loops of the step are restated
+void ref_np_step_linear(void *hK, void *hC, void *hA, long n, double alpha, double dt,
+                        const double *rhs, double *lhs, double *tdlhs, long ni, double res,
+                        double zero, long *ani, double *ares)
+{
+  symcomprow *K = (symcomprow *)hK, *C = (symcomprow *)hC, *A = (symcomprow *)hA;
+  double *d = new double[n], *p = new double[n], *f = new double[n];
+  for (long j = 0; j < n; j++) d[j] = lhs[j] + (1.0 - alpha) * dt * tdlhs[j];
+  K->mxv_scr(d, p);
+  memcpy(A->a, K->a, K->negm * sizeof(double));
+  A->scalmat_scr(dt * alpha);
+  A->addmat_scr(1.0, *C);
+  for (long j = 0; j < n; j++) f[j] = rhs[j] - p[j];
+  long a = *ani;
+  double r = *ares;
+  A->cg_prec(tdlhs, f, ni, res, a, r, zero, 1, 1, 0.0, NULL);
+  *ani = a;
+  *ares = r;
+  for (long j = 0; j < n; j++) lhs[j] = d[j] + alpha * dt * tdlhs[j];
+  delete[] d;
+  delete[] p;
+  delete[] f;
+}
+
 }  // extern "C"

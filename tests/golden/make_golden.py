@@ -12,6 +12,10 @@ Cases (small seeded meshes from ssf_amd.mesh):
 For each: element matrices K_e, scr and cr arrays (adr, ci, a) after localizing
 every K_e in element order, mxv_scr / mxv_cr of a seeded vector, and the
 cg_prec (Jacobi), scr cg and cr cg_prec (noprecond) results x, ani, ares.
+Nonstationary heat (heat_np_*): capacity matrices of the heat case (seeded c and
+rho) from the harness's ref_hex8_heat_cap, and three steps of one_step_linear on
+the reference's own symcomprow objects (alpha 0.5, dt 0.01, from T = v = 0 with
+the Dirichlet lift as the right-hand side): T, v, ani and ares after each step.
 Post-solve element pass (post_*): for a seeded nodal solution post_x, the
 integration-point strains / stresses (gradients / fluxes for heat) and element
 forces of the harness's ref_*_ip, gathered by eldispl's code-number rule and added
@@ -159,6 +163,41 @@ def main():
         out[f"{name}_post_eps"], out[f"{name}_post_sig"], out[f"{name}_post_ifor"] = \
             post_pass(R, prob, xp)
         print(name, prob.n, "unknowns", flush=True)
+    # nonstationary heat on the heat case
+    prob = CASES["heat"]()
+    grng = np.random.default_rng(2000)
+    c = grng.uniform(0.5, 2.0, prob.ne)
+    rho = grng.uniform(0.5, 2.0, prob.ne)
+    cm = np.zeros((prob.ne, 8, 8))
+    for e in range(prob.ne):
+        nodes = prob.conn[e]
+        xx, yy, zz = (f64(prob.coords[nodes, k]) for k in range(3))
+        buf = np.zeros(64)
+        R.ref_hex8_heat_cap(_ptr(xx), _ptr(yy), _ptr(zz), c[e], _ptr(f64(np.full(8, rho[e]))),
+                            _ptr(buf))
+        cm[e] = buf.reshape(8, 8)
+    out["heat_np_c"], out["heat_np_rho"], out["heat_np_cap"] = c, rho, cm
+    cn = i64(prob.elem_codes())
+    ke = ref_ke(R, prob)
+    hK, _, _, aK, _ = storage(R, prob, ke, "scr")
+    hC, _, _, aC, _ = storage(R, prob, cm, "scr")
+    hA, _, _, _, free = storage(R, prob, ke, "scr")
+    y = prob.rhs[0] + O.dirichlet_lift(prob, cn, ke)
+    T, v = np.zeros(prob.n), np.zeros(prob.n)
+    traj = []
+    for step in range(3):
+        ani = ctypes.c_long(-7)
+        ares = ctypes.c_double(-7.0)
+        R.ref_np_step_linear(hK, hC, hA, prob.n, 0.5, 0.01, _ptr(f64(y).copy()), _ptr(T), _ptr(v),
+                             5000, 1e-20, 1e-30, ctypes.byref(ani), ctypes.byref(ares))
+        traj.append((T.copy(), v.copy(), ani.value, ares.value))
+    for h in (hK, hC, hA):
+        free(h)
+    out["heat_np_T"] = np.array([t[0] for t in traj])
+    out["heat_np_v"] = np.array([t[1] for t in traj])
+    out["heat_np_ani"] = np.array([t[2] for t in traj])
+    out["heat_np_ares"] = np.array([t[3] for t in traj])
+    print("heat np", prob.n, "unknowns", flush=True)
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ref_vectors.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, os.path.getsize(path), "bytes")
