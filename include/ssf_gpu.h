@@ -42,6 +42,9 @@ extern "C" {
 #define SSF_ELEM_HEX8_ELAST 0  /* MEFEL linhex, 24x24, 3 DOF/node            */
 #define SSF_ELEM_HEX8_HEAT 1   /* TRFEL linhext with isotrmat, 8x8            */
 #define SSF_ELEM_QUAD4_PSTRESS 2 /* MEFEL planeelemlq, plane stress, 8x8      */
+#define SSF_ELEM_HEX8_HEAT_CAP 3 /* TRFEL linhext capacity matrix, 8x8:
+                                    mat_a = specific heat c, mat_b = density
+                                    (may be NULL = 1)                          */
 
 /* arithmetic of a solve */
 #define SSF_PRECISION_FP64 0   /* plain FP64 PCG, recurrence of the reference   */
@@ -129,11 +132,20 @@ int ssf_matrix_localize_batch(ssf_matrix *m, int64_t ne, int64_t nc, const int64
  * from the same cn. rhs_lift (may be NULL, n doubles, host) receives
  * -K_e u_p for prescribed DOFs (cn<0 -> pv[-cn-1]) as assemble_init
  * (TRFEL/SRC/globmatt.cpp:1635) computes. Values are accumulated (call
- * ssf_matrix_zero first for a fresh assembly). */
+ * ssf_matrix_zero first for a fresh assembly). Kind SSF_ELEM_HEX8_HEAT_CAP
+ * assembles the capacity matrix of linhext (TRFEL/SRC/linhext.cpp:979). */
 int ssf_matrix_assemble(ssf_matrix *m, int kind, int64_t nn, const double *xyz, int64_t ne,
                         const int64_t *conn, const int64_t *cn, const double *mat_a,
                         const double *mat_b, const double *thick, const double *pv,
                         int64_t npv, double *rhs_lift);
+
+/* gmatrix::scalgm (GEFEL/gmatrix.cpp:1848; symcomprow::scalmat_scr
+ * GEFEL/scr.cpp:760, comprow::scalmat_cr GEFEL/cr.cpp:996): a *= c */
+int ssf_matrix_scale(ssf_matrix *m, double c);
+/* gmatrix::addgm (GEFEL/gmatrix.cpp:1791; symcomprow::addmat_scr GEFEL/scr.cpp:682,
+ * comprow::addmat_cr GEFEL/cr.cpp:981): a += c * other.a. Both matrices must have
+ * the same storage and the same pattern (SSF_EINVAL otherwise). */
+int ssf_matrix_add(ssf_matrix *m, double c, const ssf_matrix *other);
 
 /* -------------------------------------------------------------- operations */
 /* gmatrix::gmxv (GEFEL/gmatrix.cpp:1704): b = A a, host vectors of length n */
@@ -150,6 +162,21 @@ int ssf_solve(ssf_matrix *m, const ssf_solve_params *p, const double *rhs, doubl
 /* same with device vectors (rhs_dev read-only, lhs_dev in/out), synchronous */
 int ssf_solve_dev(ssf_matrix *m, const ssf_solve_params *p, const double *rhs_dev,
                   double *lhs_dev, ssf_solve_info *info);
+
+/* one_step_linear (TRFEL/SRC/npsolvert.cpp:367-482): one step of the generalized
+ * trapezoidal rule in the v-form for C dT/dt + K T = f, on device vectors:
+ *   d = lhs + (1 - alpha) dt tdlhs;  p = K d;  A = K * (dt alpha) + 1.0 * C;
+ *   f = rhs - p;  A tdlhs = f by ssf_solve_dev from the current tdlhs (the
+ *   iv field of p is ignored: the previous rate is the start vector);
+ *   lhs = d + alpha dt tdlhs.
+ * K (conductivity), C (capacity) and A (the system matrix, overwritten) share
+ * one context, storage and pattern. A is re-formed only when K's or C's values
+ * or dt * alpha change since the last step. rhs_dev = f_{n+1} (prescribed
+ * fluxes and the Dirichlet lift); lhs_dev = nodal values, tdlhs_dev = their time
+ * derivatives, both updated. Synchronous. */
+int ssf_np_step_linear(ssf_matrix *K, ssf_matrix *C, ssf_matrix *A, double alpha, double dt,
+                       const ssf_solve_params *p, const double *rhs_dev, double *lhs_dev,
+                       double *tdlhs_dev, ssf_solve_info *info);
 
 /* ------------------------------------------------------------------ batches */
 /* Many assembled systems solved together, each with nrhs right-hand sides

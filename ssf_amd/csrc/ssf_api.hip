@@ -51,6 +51,37 @@ static void parallel_tasks(int T, F f)
   for (auto &x : th) x.join();
 }
 
+// FNV-1a fingerprint of a reference-layout pattern: chunks of 2^20 entries hashed
+// in parallel and combined in order, so the value does not depend on the thread
+// count
+static uint64_t fnv(uint64_t h, const int64_t *v, int64_t n)
+{
+  const unsigned char *b = (const unsigned char *)v;
+  for (int64_t k = 0; k < n * 8; k++) {
+    h ^= b[k];
+    h *= 1099511628211ULL;
+  }
+  return h;
+}
+
+static uint64_t pattern_fingerprint(const std::vector<int64_t> &adr, const std::vector<int64_t> &ci)
+{
+  constexpr int64_t CH = 1 << 20;
+  const int64_t nch = ((int64_t)ci.size() + CH - 1) / CH;
+  std::vector<uint64_t> part(nch);
+  parallel_for(nch, [&](int64_t b, int64_t e) {
+    for (int64_t c = b; c < e; c++)
+      part[c] = fnv(1469598103934665603ULL, ci.data() + c * CH,
+                    std::min<int64_t>(CH, (int64_t)ci.size() - c * CH));
+  });
+  uint64_t h = fnv(1469598103934665603ULL, adr.data(), (int64_t)adr.size());
+  for (uint64_t v : part) {
+    h ^= v;
+    h *= 1099511628211ULL;
+  }
+  return h;
+}
+
 // Device memory of matrices and workspaces is stream-ordered (hipMallocAsync /
 // hipFreeAsync on the owning context's stream): releasing it never waits for
 // work on other streams, e.g. a served request's solve while another request
@@ -554,6 +585,7 @@ int ssf::matrix_create(ssf_ctx *ctx, int storage, int64_t n, const int64_t *adr,
     m->adr.assign(adr, adr + n + 1);
     m->ci.assign(ci, ci + negm);
   }
+  m->pattern_fp = pattern_fingerprint(m->adr, m->ci);
   std::vector<int> tiles;
   std::vector<short> col16;
   const bool have_tiles = stream_tiles && build_tiles(n, rowptr, col, tiles, col16);
@@ -696,6 +728,7 @@ int ssf::matrix_create_sell(ssf_ctx *ctx, int64_t n, std::vector<int64_t> &adr,
   }
   m->adr.swap(adr);
   m->ci.swap(ci);
+  m->pattern_fp = pattern_fingerprint(m->adr, m->ci);
   SpmvTune t;
   t.format = 2;
   t.blocks = (int)((std::max<int64_t>(8, std::min<int64_t>(2048, (m->nslice + 3) / 4)) + 7) / 8 * 8);
@@ -754,6 +787,7 @@ extern "C" int ssf_matrix_pattern(const ssf_matrix *m, int64_t *adr, int64_t *ci
 static int values_changed(ssf_matrix *m)
 {
   hipStream_t s = m->ctx->stream;
+  m->values_version++;
   if (m->sell_only) {
     SSF_HIP(sell_expand((int)m->n, m->negm, m->d_adr32, m->sell_lo, m->sell_up, m->a_ref,
                         m->sell_v64, m->diag, s));
@@ -925,14 +959,16 @@ extern "C" int ssf_matrix_assemble(ssf_matrix *m, int kind, int64_t nn, const do
 {
   SSF_ENTER();
   SSF_CHECK(m && xyz && conn && cn && mat_a, SSF_EINVAL, "ssf_matrix_assemble: NULL argument");
-  SSF_CHECK(kind >= 0 && kind <= 2, SSF_EINVAL, "ssf_matrix_assemble: unknown element kind");
-  SSF_CHECK(kind == SSF_ELEM_HEX8_HEAT || mat_b, SSF_EINVAL, "ssf_matrix_assemble: Poisson ratios missing");
+  SSF_CHECK(kind >= 0 && kind <= 3, SSF_EINVAL, "ssf_matrix_assemble: unknown element kind");
+  SSF_CHECK(kind == SSF_ELEM_HEX8_HEAT || kind == SSF_ELEM_HEX8_HEAT_CAP || mat_b, SSF_EINVAL,
+            "ssf_matrix_assemble: Poisson ratios missing");
   SSF_CHECK(ne >= 0 && nn >= 0 && ne < (1LL << 31) && nn < (1LL << 31), SSF_EINVAL, "ssf_matrix_assemble: bad sizes");
   const int nne = kind == SSF_ELEM_QUAD4_PSTRESS ? 4 : 8;
   const int nd = kind == SSF_ELEM_HEX8_ELAST ? 24 : 8;
   SSF_CHECK((int64_t)ne * nd * nd < (1LL << 31) * 4, SSF_EUNSUP, "ssf_matrix_assemble: slot map too large");
-  for (int64_t k = 0; k < ne * nd; k++)
-    if (cn[k] < 0) SSF_CHECK(pv && -cn[k] <= npv, SSF_EINVAL, "ssf_matrix_assemble: prescribed value index out of range");
+  if (rhs_lift)  // prescribed values are read by the lift only
+    for (int64_t k = 0; k < ne * nd; k++)
+      if (cn[k] < 0) SSF_CHECK(pv && -cn[k] <= npv, SSF_EINVAL, "ssf_matrix_assemble: prescribed value index out of range");
   SSF_HIP(hipSetDevice(m->ctx->device));
   hipStream_t s = m->ctx->stream;
   int *d_cn = nullptr;
@@ -1600,5 +1636,101 @@ extern "C" int ssf_spmv_bytes(const ssf_matrix *m, int precision, double *canoni
     a = (double)m->sell_tot * (sv + ib) + 4.0 * (double)(m->nslice + 1) + 2.0 * sv * (double)m->n;
   if (canonical) *canonical = b;
   if (actual) *actual = a;
+  return SSF_OK;
+}
+
+// ------------------------------------------------ matrix combinations and the
+// nonstationary step
+
+static int same_pattern(const ssf_matrix *a, const ssf_matrix *b)
+{
+  return a->ctx == b->ctx && a->storage == b->storage && a->n == b->n && a->negm == b->negm &&
+         a->pattern_fp == b->pattern_fp;
+}
+
+extern "C" int ssf_matrix_scale(ssf_matrix *m, double c)
+{
+  SSF_ENTER();
+  SSF_CHECK(m, SSF_EINVAL, "ssf_matrix_scale: NULL matrix");
+  SSF_CHECK(m->a_ref, SSF_EUNSUP, "ssf_matrix_scale: the matrix keeps no reference-layout values");
+  SSF_HIP(hipSetDevice(m->ctx->device));
+  SSF_HIP(vals_scale(m->negm, m->a_ref, c, m->ctx->stream));
+  int rc = values_changed(m);
+  if (rc) return rc;
+  SSF_HIP(hipStreamSynchronize(m->ctx->stream));
+  return SSF_OK;
+}
+
+extern "C" int ssf_matrix_add(ssf_matrix *m, double c, const ssf_matrix *other)
+{
+  SSF_ENTER();
+  SSF_CHECK(m && other, SSF_EINVAL, "ssf_matrix_add: NULL matrix");
+  SSF_CHECK(m->a_ref && other->a_ref, SSF_EUNSUP,
+            "ssf_matrix_add: the matrices keep no reference-layout values");
+  SSF_CHECK(same_pattern(m, other), SSF_EINVAL,
+            "ssf_matrix_add: the matrices differ in context, storage or pattern");
+  SSF_HIP(hipSetDevice(m->ctx->device));
+  SSF_HIP(vals_add(m->negm, m->a_ref, c, other->a_ref, m->ctx->stream));
+  int rc = values_changed(m);
+  if (rc) return rc;
+  SSF_HIP(hipStreamSynchronize(m->ctx->stream));
+  return SSF_OK;
+}
+
+extern "C" int ssf_np_step_linear(ssf_matrix *K, ssf_matrix *C, ssf_matrix *A, double alpha,
+                                  double dt, const ssf_solve_params *p, const double *rhs_dev,
+                                  double *lhs_dev, double *tdlhs_dev, ssf_solve_info *info)
+{
+  SSF_ENTER();
+  SSF_CHECK(K && C && A && p && rhs_dev && lhs_dev && tdlhs_dev, SSF_EINVAL,
+            "ssf_np_step_linear: NULL argument");
+  SSF_CHECK(K != A && C != A, SSF_EINVAL, "ssf_np_step_linear: A must be a third matrix");
+  SSF_CHECK(K->a_ref && C->a_ref && A->a_ref && K->values_set && C->values_set, SSF_ESTATE,
+            "ssf_np_step_linear: K and C need their values");
+  SSF_CHECK(same_pattern(K, C) && same_pattern(K, A), SSF_EINVAL,
+            "ssf_np_step_linear: K, C and A differ in context, storage or pattern");
+  SSF_HIP(hipSetDevice(K->ctx->device));
+  hipStream_t s = K->ctx->stream;
+  const int64_t n = K->n;
+  // A = K * (dt alpha) + 1.0 * C (npsolvert.cpp:432-433), unless already formed
+  const double sc = dt * alpha;
+  if (A->np_k != K || A->np_c != C || A->np_kv != K->values_version ||
+      A->np_cv != C->values_version || A->np_s != sc || !A->values_set) {
+    SSF_HIP(vals_combine(A->negm, A->a_ref, K->a_ref, sc, C->a_ref, s));
+    int rc = values_changed(A);
+    if (rc) return rc;
+    A->np_k = K;
+    A->np_c = C;
+    A->np_kv = K->values_version;
+    A->np_cv = C->values_version;
+    A->np_s = sc;
+  }
+  double *d = nullptr, *pv = nullptr, *f = nullptr;
+  hipError_t e = hipSuccess;
+  auto c = [&](hipError_t x) { if (e == hipSuccess) e = x; };
+  c(dalloc(&d, n, s));
+  c(dalloc(&pv, n, s));
+  c(dalloc(&f, n, s));
+  // d = lhs + (1 - alpha) dt v (:422-424)
+  if (e == hipSuccess) c(axpy_rn(n, d, lhs_dev, (1.0 - alpha) * dt, tdlhs_dev, s));
+  int rc = SSF_OK;
+  // p = K d (:428)
+  if (e == hipSuccess) rc = ssf_gmxv_dev(K, d, pv);
+  // f = rhs - p (:441-443)
+  if (e == hipSuccess && rc == SSF_OK) c(vec_sub(n, f, rhs_dev, pv, s));
+  // A v = f from the current v (:449)
+  if (e == hipSuccess && rc == SSF_OK) {
+    ssf_solve_params q = *p;
+    q.iv = 1;
+    rc = ssf_solve_dev(A, &q, f, tdlhs_dev, info);
+  }
+  // lhs = d + alpha dt v (:454-456)
+  if (e == hipSuccess && rc == SSF_OK) c(axpy_rn(n, lhs_dev, d, alpha * dt, tdlhs_dev, s));
+  c(hipStreamSynchronize(s));
+  dfree(d, s);
+  dfree(pv, s);
+  dfree(f, s);
+  if (rc) return rc;
+  SSF_HIP(e);
   return SSF_OK;
 }

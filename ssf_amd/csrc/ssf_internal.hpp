@@ -179,6 +179,14 @@ struct ssf_matrix {
   int *sell_lo = nullptr, *sell_up = nullptr, *d_adr32 = nullptr, *d_ci32 = nullptr;
   ssf::SpmvTune tune64, tune32;
   bool tuned64 = false, tuned32 = false;
+  // fingerprint of the reference-layout pattern (adr, ci), for operations that
+  // combine two matrices' values; counts every change of the values
+  uint64_t pattern_fp = 0;
+  uint64_t values_version = 0;
+  // as the system matrix of a nonstationary step: the inputs it was formed from
+  const ssf_matrix *np_k = nullptr, *np_c = nullptr;
+  uint64_t np_kv = 0, np_cv = 0;
+  double np_s = 0.0;
   ssf::CsrDev csr() const
   {
     ssf::CsrDev c;

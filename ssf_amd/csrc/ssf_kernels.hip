@@ -7,6 +7,8 @@
 // every consumer block re-reduces the whole partial array in the same fixed order,
 // so each block holds bit-identical scalars without inter-block synchronisation
 // and a system gives the same result whatever else runs on the device.
+#include <algorithm>
+
 #include "ssf_device.hpp"
 
 namespace ssf {
@@ -346,6 +348,41 @@ __device__ void ke_quad4_pstress(const double *x, const double *y, double e, dou
   }
 }
 
+// hex8 heat capacity (TRFEL/SRC/linhext.cpp:979-1029): C_e = sum_gp N^T N |J| rho c
+// with rho the bilinear interpolation of equal nodal densities (linhext::approx)
+__device__ void ke_hex8_cap(const double *x, const double *y, const double *z, double c,
+                            double rho_e, double *K)
+{
+#pragma unroll
+  for (int q = 0; q < 64; q++) K[q] = 0.0;
+  for (int g = 0; g < 8; g++) {
+    const double xi = (g & 4) ? GPT : -GPT, eta = (g & 2) ? GPT : -GPT, zeta = (g & 1) ? GPT : -GPT;
+    double dx[8], dy[8], dz[8];
+    const double jac = hex_global(x, y, z, xi, eta, zeta, dx, dy, dz);
+    double n[8];
+    n[0] = (1.0 + xi) * (1.0 + eta) * (1.0 + zeta) / 8.0;
+    n[1] = (1.0 - xi) * (1.0 + eta) * (1.0 + zeta) / 8.0;
+    n[2] = (1.0 - xi) * (1.0 - eta) * (1.0 + zeta) / 8.0;
+    n[3] = (1.0 + xi) * (1.0 - eta) * (1.0 + zeta) / 8.0;
+    n[4] = (1.0 + xi) * (1.0 + eta) * (1.0 - zeta) / 8.0;
+    n[5] = (1.0 - xi) * (1.0 + eta) * (1.0 - zeta) / 8.0;
+    n[6] = (1.0 - xi) * (1.0 - eta) * (1.0 - zeta) / 8.0;
+    n[7] = (1.0 + xi) * (1.0 - eta) * (1.0 - zeta) / 8.0;
+    double rho = 0.0;
+#pragma unroll
+    for (int a = 0; a < 8; a++) rho += n[a] * rho_e;
+    const double w = jac * (rho * c);  // hex_global returns |J|
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+#pragma unroll
+      for (int j = i; j < 8; j++) {
+        const double v = (n[i] * n[j]) * w;
+        K[i * 8 + j] += v;
+        if (j != i) K[j * 8 + i] += v;
+      }
+  }
+}
+
 // one thread per element of a colour; adds K_e into the reference-layout values
 // through the slot map, and -K_e u_p into the Dirichlet lift (colour-exclusive).
 template <int KIND>
@@ -366,14 +403,15 @@ __global__ __launch_bounds__(TPB) void k_assemble8(int64_t p0, int64_t p1, int64
   if (p >= p1) return;
   const int e = order[p];
   double K[64];
-  if (KIND == SSF_ELEM_HEX8_HEAT) {
+  if (KIND == SSF_ELEM_HEX8_HEAT || KIND == SSF_ELEM_HEX8_HEAT_CAP) {
     double x[8], y[8], z[8];
 #pragma unroll
     for (int a = 0; a < 8; a++) {
       const int nd = conn[(int64_t)e * 8 + a];
       x[a] = xyz[3 * (int64_t)nd]; y[a] = xyz[3 * (int64_t)nd + 1]; z[a] = xyz[3 * (int64_t)nd + 2];
     }
-    ke_hex8_heat(x, y, z, ma[e], K);
+    if (KIND == SSF_ELEM_HEX8_HEAT) ke_hex8_heat(x, y, z, ma[e], K);
+    else ke_hex8_cap(x, y, z, ma[e], mb ? mb[e] : 1.0, K);
   } else {
     double x[4], y[4];
 #pragma unroll
@@ -428,6 +466,8 @@ hipError_t assemble_elements(int kind, int64_t p0, int64_t p1, int64_t ne_total,
   const unsigned nb = (unsigned)((p1 - p0 + TPB - 1) / TPB);
   if (kind == SSF_ELEM_HEX8_HEAT)
     hipLaunchKernelGGL((k_assemble8<SSF_ELEM_HEX8_HEAT>), dim3(nb), dim3(TPB), 0, s, p0, p1, ne_total, order, conn, cn, xyz, mat_a, mat_b, thick, slot, a_ref, pv, lift);
+  else if (kind == SSF_ELEM_HEX8_HEAT_CAP)
+    hipLaunchKernelGGL((k_assemble8<SSF_ELEM_HEX8_HEAT_CAP>), dim3(nb), dim3(TPB), 0, s, p0, p1, ne_total, order, conn, cn, xyz, mat_a, mat_b, thick, slot, a_ref, pv, lift);
   else
     hipLaunchKernelGGL((k_assemble8<SSF_ELEM_QUAD4_PSTRESS>), dim3(nb), dim3(TPB), 0, s, p0, p1, ne_total, order, conn, cn, xyz, mat_a, mat_b, thick, slot, a_ref, pv, lift);
   return hipGetLastError();
@@ -453,6 +493,87 @@ hipError_t localize_elements(int64_t p0, int64_t p1, int64_t ne_total, int nd, c
   if (p1 <= p0) return hipSuccess;
   hipLaunchKernelGGL(k_localize, dim3((unsigned)((p1 - p0 + TPB - 1) / TPB)), dim3(TPB), 0, s,
                      p0, p1, ne_total, nd, order, slot, lm, a_ref);
+  return hipGetLastError();
+}
+
+}  // namespace ssf
+
+namespace ssf {
+
+// ------------------------------------------------- value and vector updates
+// Each operation rounds as the reference's loops do (no contraction): the
+// nonstationary step and the matrix combinations stay bit-identical to it.
+
+__global__ void k_vals_scale(int64_t n, double *__restrict__ a, double c)
+{
+#pragma clang fp contract(off)
+  const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (i < n) a[i] *= c;
+}
+
+__global__ void k_vals_add(int64_t n, double *__restrict__ a, double c, const double *__restrict__ b)
+{
+#pragma clang fp contract(off)
+  const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (i < n) a[i] += c * b[i];
+}
+
+// out = k * s, then out += 1.0 * cm (scalmat then addmat)
+__global__ void k_vals_combine(int64_t n, double *__restrict__ out, const double *__restrict__ k,
+                               double s, const double *__restrict__ cm)
+{
+#pragma clang fp contract(off)
+  const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (i >= n) return;
+  double v = k[i];
+  v *= s;
+  v += 1.0 * cm[i];
+  out[i] = v;
+}
+
+// y = x + c * v
+__global__ void k_axpy_rn(int64_t n, double *__restrict__ y, const double *__restrict__ x, double c,
+                          const double *__restrict__ v)
+{
+#pragma clang fp contract(off)
+  const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (i < n) y[i] = x[i] + c * v[i];
+}
+
+// f = a - b
+__global__ void k_sub(int64_t n, double *__restrict__ f, const double *__restrict__ a,
+                      const double *__restrict__ b)
+{
+  const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (i < n) f[i] = a[i] - b[i];
+}
+
+static unsigned grid_for(int64_t n) { return (unsigned)((std::max<int64_t>(n, 1) + TPB - 1) / TPB); }
+
+hipError_t vals_scale(int64_t n, double *a, double c, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_vals_scale, dim3(grid_for(n)), dim3(TPB), 0, s, n, a, c);
+  return hipGetLastError();
+}
+hipError_t vals_add(int64_t n, double *a, double c, const double *b, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_vals_add, dim3(grid_for(n)), dim3(TPB), 0, s, n, a, c, b);
+  return hipGetLastError();
+}
+hipError_t vals_combine(int64_t n, double *out, const double *k, double sc, const double *cm,
+                        hipStream_t s)
+{
+  hipLaunchKernelGGL(k_vals_combine, dim3(grid_for(n)), dim3(TPB), 0, s, n, out, k, sc, cm);
+  return hipGetLastError();
+}
+hipError_t axpy_rn(int64_t n, double *y, const double *x, double c, const double *v, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_axpy_rn, dim3(grid_for(n)), dim3(TPB), 0, s, n, y, x, c, v);
+  return hipGetLastError();
+}
+hipError_t vec_sub(int64_t n, double *f, const double *a, const double *b, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_sub, dim3(grid_for(n)), dim3(TPB), 0, s, n, f, a, b);
   return hipGetLastError();
 }
 

@@ -78,6 +78,15 @@ hipError_t reduce_to_host_slot(const double *part, int nb, double *out, hipStrea
 hipError_t build_slots(int storage, int aos, int64_t ne_total, int nd, const int *order,
                        const int *cn, const int *ref_adr, const int *ref_ci, int *slot,
                        hipStream_t s);
+// value and vector updates rounded as the reference's loops (no contraction)
+hipError_t vals_scale(int64_t n, double *a, double c, hipStream_t s);             // a *= c
+hipError_t vals_add(int64_t n, double *a, double c, const double *b, hipStream_t s);  // a += c b
+hipError_t vals_combine(int64_t n, double *out, const double *k, double sc, const double *cm,
+                        hipStream_t s);                                            // out = k sc + 1.0 cm
+hipError_t axpy_rn(int64_t n, double *y, const double *x, double c, const double *v,
+                   hipStream_t s);                                                 // y = x + c v
+hipError_t vec_sub(int64_t n, double *f, const double *a, const double *b, hipStream_t s);
+
 hipError_t assemble_elements(int kind, int64_t p0, int64_t p1, int64_t ne_total,
                              const int *order, const int *conn, const int *cn,
                              const double *xyz, const double *mat_a, const double *mat_b,

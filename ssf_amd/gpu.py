@@ -23,6 +23,7 @@ PREC_NONE = 0
 PREC_DIAG = 1
 FP64 = 0
 MIXED = 1
+KIND_HEX8_HEAT_CAP = 3
 
 _P = ctypes.c_void_p
 _L = ctypes.c_int64
@@ -86,6 +87,10 @@ SIGNATURES = {
                                ctypes.POINTER(_L)]),
     "ssf_server_wait": (_I, [_P, _L, _P, ctypes.POINTER(SolveInfo), _P]),
     "ssf_server_stats": (_I, [_P, _P]),
+    "ssf_matrix_scale": (_I, [_P, _D]),
+    "ssf_matrix_add": (_I, [_P, _D, _P]),
+    "ssf_np_step_linear": (_I, [_P, _P, _P, _D, _D, ctypes.POINTER(SolveParams), _P, _P, _P,
+                                ctypes.POINTER(SolveInfo)]),
     "ssf_elements_create": (_I, [_P, _I, _L, _L, _P, _L, _P, _P, _P, _P, _P, ctypes.POINTER(_P)]),
     "ssf_elements_destroy": (_I, [_P]),
     "ssf_elements_sizes": (_I, [_P, _P, _P, _P]),
@@ -253,6 +258,26 @@ class Matrix:
                                          0 if pv is None else pv.shape[0], _p(out)))
         return out
 
+    def assemble_capacity(self, prob: mesh.Problem, c, rho=None):
+        """Heat capacity matrix of a hex8 heat mesh (specific heat c[ne], density
+        rho[ne] or None = 1) added into the values."""
+        cn = _i64(prob.elem_codes())
+        conn = _i64(prob.conn)
+        xyz = _f64(prob.coords)
+        c = _f64(c)
+        rho = None if rho is None else _f64(rho)
+        _check(lib().ssf_matrix_assemble(self.h, KIND_HEX8_HEAT_CAP, xyz.shape[0], _p(xyz),
+                                         conn.shape[0], _p(conn), _p(cn), _p(c), _p(rho), None,
+                                         None, 0, None))
+
+    def scale(self, c: float):
+        """gmatrix::scalgm: values *= c"""
+        _check(lib().ssf_matrix_scale(self.h, float(c)))
+
+    def add(self, c: float, other: "Matrix"):
+        """gmatrix::addgm: values += c * other.values (same pattern)"""
+        _check(lib().ssf_matrix_add(self.h, float(c), other.h))
+
     def gmxv(self, a):
         a = _f64(a)
         b = np.zeros(self.n)
@@ -297,6 +322,17 @@ class Matrix:
     def set_tune(self, precision, tune):
         t = np.ascontiguousarray(tune, dtype=np.int32)
         _check(lib().ssf_matrix_set_tune(self.h, precision, _p(t)))
+
+
+def np_step_linear(K: Matrix, C: Matrix, A: Matrix, alpha: float, dt: float, rhs_ptr, lhs_ptr,
+                   tdlhs_ptr, ni=100000, res=1e-20, precision=FP64, zero=1e-30, inner_ni=0):
+    """One generalized-trapezoidal step (TRFEL one_step_linear) on device vectors:
+    lhs (nodal values) and tdlhs (their rates) are updated; returns the solve info."""
+    p = SolveParams(SOLVER_CG, PREC_DIAG, ni, res, zero, 1, precision, 0.0, inner_ni, 0)
+    info = SolveInfo(-1, -1.0, 0, 0, 0, 0, 0.0)
+    _check(lib().ssf_np_step_linear(K.h, C.h, A.h, float(alpha), float(dt), ctypes.byref(p),
+                                    rhs_ptr, lhs_ptr, tdlhs_ptr, ctypes.byref(info)))
+    return info
 
 
 class Batch:
