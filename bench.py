@@ -16,7 +16,9 @@ format it streams per launch, divided by its mean duration, sampled inside the
 timed solves with the launch's own start/stop events (hipExtLaunchKernelGGL).
 The post-solve element pass (temperature gradients and fluxes at the 8 points of
 every element, assembled internal fluxes) runs on the mixed solution and is
-reported under "post" with its own bytes and kernel time; --only-assembly times
+reported under "post" with its own bytes and kernel time. The nonstationary leg
+("transient") repeats TRFEL's one_step_linear on the same cube with device-resident
+vectors; --only-assembly times
 the element kernels and the post pass of the 64x64x256 elasticity column.
 Prints one JSON line.
 """
@@ -292,6 +294,64 @@ def bench_post(ctx, args, gpu, mesh, torch, prob, x_dev, label):
     return out
 
 
+def bench_transient(ctx, args, gpu, mesh, torch, prob, K, rhs):
+    """Nonstationary heat on the heat cube: one_step_linear (generalized trapezoidal
+    rule, alpha 0.5, dt = --np-dt) repeated --np-steps times from T = v = 0 with
+    c = rho = 1 and the Dirichlet lift as the right-hand side. Everything stays in
+    HBM between steps; the system matrix C + alpha dt K is formed once. CPU
+    baseline: the oracle's one_step_linear on the same system, one step."""
+    cn = prob.elem_codes()
+    t0 = time.time()
+    C = gpu.Matrix.from_elements(ctx, gpu.STORAGE_SCR, prob.n, cn)
+    A = gpu.Matrix.from_elements(ctx, gpu.STORAGE_SCR, prob.n, cn)
+    C.assemble_capacity(prob, np.ones(prob.ne))
+    log(f"[np] capacity matrix and system matrix set up ({time.time() - t0:.1f}s)")
+    dev = torch.device("cuda:0")
+    f = torch.tensor(rhs, dtype=torch.float64, device=dev)
+    out = {"workload": f"heat hex8 {args.nel}^3 transient, n={prob.n}, alpha 0.5, "
+                       f"dt {args.np_dt}, c = rho = 1, {args.np_steps} steps from T = 0"}
+    for name, prec in (("fp64", gpu.FP64), ("mixed", gpu.MIXED)):
+        T = torch.zeros(prob.n, dtype=torch.float64, device=dev)
+        v = torch.zeros(prob.n, dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+        gpu.np_step_linear(K, C, A, 0.5, args.np_dt, f.data_ptr(), T.data_ptr(), v.data_ptr(),
+                           res=args.res, precision=prec)  # warm-up step (forms A)
+        torch.cuda.synchronize()
+        its = []
+        t = time.perf_counter()
+        for _ in range(args.np_steps):
+            info = gpu.np_step_linear(K, C, A, 0.5, args.np_dt, f.data_ptr(), T.data_ptr(),
+                                      v.data_ptr(), res=args.res, precision=prec)
+            its.append(int(info.iterations))
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        out[name] = {"steps_per_s": args.np_steps / dt, "ms_per_step": 1e3 * dt / args.np_steps,
+                     "iterations_per_step": float(np.mean(its)), "status": int(info.status),
+                     "T_max": float(T.max().item())}
+        if name == "fp64":
+            Tref = T.clone()
+        else:
+            out[name]["rel_diff_vs_fp64"] = (torch.linalg.norm(T - Tref) /
+                                             torch.linalg.norm(Tref)).item()
+        log(f"[np] {name}: {out[name]}")
+    if not args.no_cpu:
+        import oracle as O
+        adr, ci = K.pattern()
+        aK, aC = K.get_values(), C.get_values()
+        Tc, vc = np.zeros(prob.n), np.zeros(prob.n)
+        t = time.perf_counter()
+        st, ani, _, _ = O.np_step_linear(adr, ci, aK, aC, 0.5, args.np_dt, rhs, Tc, vc, 200000,
+                                         args.res)
+        dtc = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": 1.0 / dtc, "unit": "steps/s", "cores": 1, "kind": "port",
+                               "sample": f"one oracle one_step_linear (1 thread) on the same "
+                                         f"system from T = 0 ({ani + 1} iterations)"}
+        out["speedup_fp64_vs_cpu"] = out["fp64"]["steps_per_s"] * dtc
+    C.close()
+    A.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -319,6 +379,9 @@ def main():
     ap.add_argument("--asm-reps", type=int, default=3)
     ap.add_argument("--post-reps", type=int, default=10)
     ap.add_argument("--no-post", action="store_true", help="skip the post-solve element pass")
+    ap.add_argument("--no-np", action="store_true", help="skip the nonstationary heat leg")
+    ap.add_argument("--np-steps", type=int, default=10)
+    ap.add_argument("--np-dt", type=float, default=1e-3)
     ap.add_argument("--serve-count", type=int, default=24)
     ap.add_argument("--serve-res", type=float, default=1e-22)
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[20.0])
@@ -453,6 +516,9 @@ def main():
     post = None
     if not args.no_post:
         post = bench_post(ctx, args, gpu, mesh, torch, prob, x, f"heat hex8 {args.nel}^3")
+    transient = None
+    if not args.no_np:
+        transient = bench_transient(ctx, args, gpu, mesh, torch, prob, m, rhs)
 
     batched = None
     if not args.no_batch:
@@ -498,6 +564,7 @@ def main():
                      "usec_per_launch": us32, "samples": mixed["spmv_samples"]},
         "cpu_baseline": cpu,
         "post": post,
+        "transient": transient,
         "batched": batched,
         "served": served,
     }
