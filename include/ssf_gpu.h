@@ -105,6 +105,13 @@ int ssf_matrix_create_elem(ssf_ctx *ctx, int storage, int64_t n, int64_t ne,
  * symcomprow rows ascending with the diagonal last). */
 int ssf_matrix_create_csr(ssf_ctx *ctx, int storage, int64_t n, const int64_t *adr,
                           const int64_t *ci, ssf_matrix **out);
+/* symcomprow::initiate with the pattern built on the device: the same adr / ci,
+ * from a sort of the elements' DOF pairs (no host pattern work). The matrix keeps
+ * the SELL-64 layout only: it assembles, multiplies and solves in FP64; the
+ * mixed-precision solve, batches and the stream / row-vector SpMV need a matrix
+ * from ssf_matrix_create_elem. */
+int ssf_matrix_create_elem_device(ssf_ctx *ctx, int64_t n, int64_t ne, int64_t ndofe,
+                                  const int64_t *cn, ssf_matrix **out);
 int ssf_matrix_destroy(ssf_matrix *m);
 /* number of stored entries in the reference layout (negm) and n */
 int ssf_matrix_sizes(const ssf_matrix *m, int64_t *n, int64_t *negm, int64_t *nnz_full);

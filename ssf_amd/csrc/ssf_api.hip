@@ -51,35 +51,22 @@ static void parallel_tasks(int T, F f)
   for (auto &x : th) x.join();
 }
 
-// FNV-1a fingerprint of a reference-layout pattern: chunks of 2^20 entries hashed
-// in parallel and combined in order, so the value does not depend on the thread
-// count
-static uint64_t fnv(uint64_t h, const int64_t *v, int64_t n)
-{
-  const unsigned char *b = (const unsigned char *)v;
-  for (int64_t k = 0; k < n * 8; k++) {
-    h ^= b[k];
-    h *= 1099511628211ULL;
-  }
-  return h;
-}
-
+// checksum of a reference-layout pattern, equal to pattern_checksum_dev's:
+// sum of (value + 1) * mix64(index ^ salt) over adr and ci, wrapping
 static uint64_t pattern_fingerprint(const std::vector<int64_t> &adr, const std::vector<int64_t> &ci)
 {
-  constexpr int64_t CH = 1 << 20;
-  const int64_t nch = ((int64_t)ci.size() + CH - 1) / CH;
-  std::vector<uint64_t> part(nch);
-  parallel_for(nch, [&](int64_t b, int64_t e) {
-    for (int64_t c = b; c < e; c++)
-      part[c] = fnv(1469598103934665603ULL, ci.data() + c * CH,
-                    std::min<int64_t>(CH, (int64_t)ci.size() - c * CH));
-  });
-  uint64_t h = fnv(1469598103934665603ULL, adr.data(), (int64_t)adr.size());
-  for (uint64_t v : part) {
-    h ^= v;
-    h *= 1099511628211ULL;
-  }
-  return h;
+  std::atomic<unsigned long long> h{0};
+  auto sum = [&](const std::vector<int64_t> &v, unsigned long long salt) {
+    parallel_for((int64_t)v.size(), [&](int64_t b, int64_t e) {
+      unsigned long long s = 0;
+      for (int64_t k = b; k < e; k++)
+        s += (unsigned long long)(v[k] + 1) * mix64((unsigned long long)k ^ salt);
+      h += s;
+    });
+  };
+  sum(adr, kSaltAdr);
+  sum(ci, kSaltCi);
+  return h.load();
 }
 
 // Device memory of matrices and workspaces is stream-ordered (hipMallocAsync /
@@ -663,26 +650,10 @@ int ssf::matrix_create(ssf_ctx *ctx, int storage, int64_t n, const int64_t *adr,
   return SSF_OK;
 }
 
-int ssf::matrix_create_sell(ssf_ctx *ctx, int64_t n, std::vector<int64_t> &adr,
-                            std::vector<int64_t> &ci, ssf_matrix **out)
+int ssf::matrix_create_sell_dev(ssf_ctx *ctx, int64_t n, int *d_adr, int *d_ci, int64_t negm,
+                                ssf_matrix **out)
 {
-  SSF_CHECK(ctx && out && (int64_t)adr.size() == n + 1, SSF_EINVAL, "matrix_create_sell: bad argument");
-  SSF_CHECK(n > 0 && n < (1LL << 31) && adr[n] < (1LL << 31) && 2 * adr[n] < (1LL << 31),
-            SSF_EUNSUP, "matrix_create_sell: more than 2^31 entries");
-  if (const char *msg = check_pattern(n, adr.data(), ci.data(), true)) {
-    set_error(msg);
-    return SSF_EINVAL;
-  }
-  SSF_HIP(hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
-  const int64_t negm = adr[n];
-  std::vector<int> adr32(n + 1), ci32(negm);
-  parallel_for(n + 1, [&](int64_t b, int64_t e) {
-    for (int64_t i = b; i < e; i++) adr32[i] = (int)adr[i];
-  });
-  parallel_for(negm, [&](int64_t b, int64_t e) {
-    for (int64_t k = b; k < e; k++) ci32[k] = (int)ci[k];
-  });
   ssf_matrix *m = new ssf_matrix();
   m->ctx = ctx;
   ctx->live_matrices++;
@@ -691,15 +662,11 @@ int ssf::matrix_create_sell(ssf_ctx *ctx, int64_t n, std::vector<int64_t> &adr,
   m->negm = negm;
   m->nnz = 2 * negm - n;
   m->sell_only = true;
+  m->d_adr32 = d_adr;
+  m->d_ci32 = d_ci;
   hipError_t e = hipSuccess;
   auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
-  chk(dalloc(&m->d_adr32, n + 1, s));
-  chk(dalloc(&m->d_ci32, negm, s));
-  if (e == hipSuccess) {
-    chk(hipMemcpyAsync(m->d_adr32, adr32.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice, s));
-    chk(hipMemcpyAsync(m->d_ci32, ci32.data(), negm * sizeof(int), hipMemcpyHostToDevice, s));
-    chk(hipStreamSynchronize(s));
-  }
+  chk(pattern_checksum_dev(d_adr, n, d_ci, negm, s, &m->pattern_fp));
   SellLayout L;
   if (e == hipSuccess) chk(sell_layout_build((int)n, negm, m->d_adr32, m->d_ci32, s, &L));
   if (e == hipSuccess) {
@@ -726,15 +693,54 @@ int ssf::matrix_create_sell(ssf_ctx *ctx, int64_t n, std::vector<int64_t> &adr,
     set_error(msg);
     return e == hipErrorOutOfMemory ? SSF_ENOMEM : SSF_EHIP;
   }
-  m->adr.swap(adr);
-  m->ci.swap(ci);
-  m->pattern_fp = pattern_fingerprint(m->adr, m->ci);
   SpmvTune t;
   t.format = 2;
   t.blocks = (int)((std::max<int64_t>(8, std::min<int64_t>(2048, (m->nslice + 3) / 4)) + 7) / 8 * 8);
   t.idx16 = m->sell_col16 ? 1 : 0;
   m->tune64 = m->tune32 = t;
   *out = m;
+  return SSF_OK;
+}
+
+int ssf::matrix_create_sell(ssf_ctx *ctx, int64_t n, std::vector<int64_t> &adr,
+                            std::vector<int64_t> &ci, ssf_matrix **out)
+{
+  SSF_CHECK(ctx && out && (int64_t)adr.size() == n + 1, SSF_EINVAL, "matrix_create_sell: bad argument");
+  SSF_CHECK(n > 0 && n < (1LL << 31) && adr[n] < (1LL << 31) && 2 * adr[n] < (1LL << 31),
+            SSF_EUNSUP, "matrix_create_sell: more than 2^31 entries");
+  if (const char *msg = check_pattern(n, adr.data(), ci.data(), true)) {
+    set_error(msg);
+    return SSF_EINVAL;
+  }
+  SSF_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const int64_t negm = adr[n];
+  std::vector<int> adr32(n + 1), ci32(negm);
+  parallel_for(n + 1, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; i++) adr32[i] = (int)adr[i];
+  });
+  parallel_for(negm, [&](int64_t b, int64_t e) {
+    for (int64_t k = b; k < e; k++) ci32[k] = (int)ci[k];
+  });
+  int *d_adr = nullptr, *d_ci = nullptr;
+  hipError_t e = hipSuccess;
+  auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
+  chk(dalloc(&d_adr, n + 1, s));
+  chk(dalloc(&d_ci, negm, s));
+  if (e == hipSuccess) {
+    chk(hipMemcpyAsync(d_adr, adr32.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice, s));
+    chk(hipMemcpyAsync(d_ci, ci32.data(), negm * sizeof(int), hipMemcpyHostToDevice, s));
+    chk(hipStreamSynchronize(s));
+  }
+  if (e != hipSuccess) {
+    dfree(d_adr, s);
+    dfree(d_ci, s);
+    SSF_HIP(e);
+  }
+  int rc = matrix_create_sell_dev(ctx, n, d_adr, d_ci, negm, out);
+  if (rc) return rc;
+  (*out)->adr.swap(adr);
+  (*out)->ci.swap(ci);
   return SSF_OK;
 }
 
@@ -764,6 +770,47 @@ extern "C" int ssf_matrix_create_elem(ssf_ctx *ctx, int storage, int64_t n, int6
   return matrix_create(ctx, storage, n, adr.data(), ci.data(), &adr, &ci, true, out);
 }
 
+extern "C" int ssf_matrix_create_elem_device(ssf_ctx *ctx, int64_t n, int64_t ne, int64_t ndofe,
+                                             const int64_t *cn, ssf_matrix **out)
+{
+  SSF_ENTER();
+  SSF_CHECK(ctx && out && cn, SSF_EINVAL, "ssf_matrix_create_elem_device: NULL argument");
+  *out = nullptr;
+  SSF_CHECK(n > 0 && n < (1LL << 31) - 1 && ne > 0 && ndofe > 0 && ndofe <= 64 &&
+                ne * ndofe < (1LL << 31),
+            SSF_EINVAL, "ssf_matrix_create_elem_device: bad sizes");
+  std::vector<int> cn32(ne * ndofe);
+  std::atomic<int> bad{0};
+  parallel_for(ne * ndofe, [&](int64_t b, int64_t e) {
+    for (int64_t k = b; k < e; k++) {
+      if (cn[k] > n || cn[k] < -(1LL << 31) + 1) bad = 1;
+      cn32[k] = (int)cn[k];
+    }
+  });
+  SSF_CHECK(!bad.load(), SSF_EINVAL, "ssf_matrix_create_elem_device: code number out of range");
+  SSF_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  int *d_cn = nullptr;
+  SSF_HIP(dalloc(&d_cn, ne * ndofe, s));
+  hipError_t e = hipMemcpyAsync(d_cn, cn32.data(), ne * ndofe * sizeof(int), hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) {
+    dfree(d_cn, s);
+    SSF_HIP(e);
+  }
+  int *d_adr = nullptr, *d_ci = nullptr;
+  int64_t negm = 0;
+  int rc = pattern_build_dev(n, ne, (int)ndofe, d_cn, s, &d_adr, &d_ci, &negm);
+  dfree(d_cn, s);
+  if (rc) return rc;
+  if (negm >= (1LL << 30)) {
+    dfree(d_adr, s);
+    dfree(d_ci, s);
+    set_error("ssf_matrix_create_elem_device: more than 2^31 entries in the full matrix");
+    return SSF_EUNSUP;
+  }
+  return matrix_create_sell_dev(ctx, n, d_adr, d_ci, negm, out);
+}
+
 extern "C" int ssf_matrix_sizes(const ssf_matrix *m, int64_t *n, int64_t *negm, int64_t *nnz_full)
 {
   SSF_ENTER();
@@ -778,8 +825,23 @@ extern "C" int ssf_matrix_pattern(const ssf_matrix *m, int64_t *adr, int64_t *ci
 {
   SSF_ENTER();
   SSF_CHECK(m, SSF_EINVAL, "ssf_matrix_pattern: NULL matrix");
-  if (adr) std::memcpy(adr, m->adr.data(), (m->n + 1) * sizeof(int64_t));
-  if (ci && m->negm) std::memcpy(ci, m->ci.data(), m->negm * sizeof(int64_t));
+  if ((int64_t)m->adr.size() == m->n + 1) {
+    if (adr) std::memcpy(adr, m->adr.data(), (m->n + 1) * sizeof(int64_t));
+    if (ci && m->negm) std::memcpy(ci, m->ci.data(), m->negm * sizeof(int64_t));
+    return SSF_OK;
+  }
+  // pattern kept on the device only
+  SSF_CHECK(m->d_adr32 && m->d_ci32, SSF_ESTATE, "ssf_matrix_pattern: the pattern was released");
+  SSF_HIP(hipSetDevice(m->ctx->device));
+  std::vector<int> a32(m->n + 1), c32(m->negm);
+  SSF_HIP(hipMemcpyAsync(a32.data(), m->d_adr32, (m->n + 1) * sizeof(int), hipMemcpyDeviceToHost,
+                         m->ctx->stream));
+  if (m->negm)
+    SSF_HIP(hipMemcpyAsync(c32.data(), m->d_ci32, m->negm * sizeof(int), hipMemcpyDeviceToHost,
+                           m->ctx->stream));
+  SSF_HIP(hipStreamSynchronize(m->ctx->stream));
+  if (adr) std::copy(a32.begin(), a32.end(), adr);
+  if (ci) std::copy(c32.begin(), c32.end(), ci);
   return SSF_OK;
 }
 

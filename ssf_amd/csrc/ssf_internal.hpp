@@ -208,6 +208,29 @@ struct ssf_matrix {
 };
 
 namespace ssf {
+// pattern checksum: sum over adr and ci of (value + 1) * mix64(index ^ salt),
+// wrapping; the same on the host and the device (order-independent sums)
+__host__ __device__ inline unsigned long long mix64(unsigned long long x)
+{
+  x += 0x9E3779B97F4A7C15ULL;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+  return x ^ (x >> 31);
+}
+constexpr unsigned long long kSaltAdr = 0x5AD5AD5AD5ULL, kSaltCi = 0xC1C1C1C1C1ULL;
+hipError_t pattern_checksum_dev(const int *adr, int64_t n, const int *ci, int64_t negm,
+                                hipStream_t s, uint64_t *out);
+// symcomprow pattern built on the device from element code numbers cn[ne*nd]
+// (int32, device); adr[n+1], ci[negm] are allocated stream-ordered on s
+int pattern_build_dev(int64_t n, int64_t ne, int nd, const int *cn, hipStream_t s, int **adr,
+                      int **ci, int64_t *negm);
+int64_t pattern_build_bytes(int64_t ne, int nd);  // its transient device bytes
+// entries of the SELL-64 layout of the full matrix of a device symcomprow pattern
+hipError_t sell_entries_dev(int64_t n, int64_t negm, const int *adr, const int *ci, hipStream_t s,
+                            int64_t *tot);
+// matrix_create_sell from a device pattern (ownership of adr / ci passes to the matrix)
+int matrix_create_sell_dev(ssf_ctx *ctx, int64_t n, int *adr, int *ci, int64_t negm,
+                           ssf_matrix **out);
 // symbolic pattern of symcomprow / comprow from element code numbers (host)
 int pattern_from_elements(int storage, int64_t n, int64_t ne, int64_t ndofe, const int64_t *cn,
                           std::vector<int64_t> &adr, std::vector<int64_t> &ci);

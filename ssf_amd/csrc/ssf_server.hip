@@ -115,24 +115,6 @@ struct ssf_server {
 
 namespace {
 
-// entries of the SELL-64 layout of the full matrix of a symcomprow pattern
-int64_t sell_entries(int64_t n, const std::vector<int64_t> &adr, const std::vector<int64_t> &ci)
-{
-  std::vector<int> len(n, 0);
-  for (int64_t i = 0; i < n; i++)
-    for (int64_t k = adr[i]; k < adr[i + 1]; k++) {
-      len[i]++;
-      if (ci[k] != i) len[ci[k]]++;
-    }
-  int64_t tot = 0;
-  for (int64_t s0 = 0; s0 < n; s0 += 64) {
-    int w = 0;
-    for (int64_t r = s0; r < std::min<int64_t>(n, s0 + 64); r++) w = std::max(w, len[r]);
-    tot += (int64_t)(w + SELL_V - 1) / SELL_V * SELL_V * 64;
-  }
-  return tot;
-}
-
 // device bytes of a resident request: SELL slices (offsets, 16-bit columns or
 // 32-bit ones, FP64 values), diagonal, six solve vectors
 int64_t resident_bytes(int64_t n, int64_t tot, bool col16)
@@ -176,40 +158,94 @@ void prep_main(ssf_server *S, int w)
     }
     int rc = SSF_OK;
     std::string err;
-    std::vector<int64_t> adr, ci;
+    // FIFO over the budget: wait until `bytes` fit, then hold them
+    auto reserve = [&](int64_t bytes) {
+      std::unique_lock<std::mutex> lk(S->mu);
+      if (bytes > S->budget) {
+        err = "ssf_server: request larger than the whole HBM budget";
+        return SSF_ENOMEM;
+      }
+      S->cv_budget.wait(lk, [&] { return S->used + bytes <= S->budget; });
+      S->used += bytes;
+      S->peak = std::max(S->peak, S->used);
+      return SSF_OK;
+    };
+    auto release = [&](int64_t bytes) {
+      std::unique_lock<std::mutex> lk(S->mu);
+      S->used -= bytes;
+      S->cv_budget.notify_all();
+    };
+    // 1. the symcomprow pattern, on the device (its sort holds transient bytes)
     const auto tp0 = clk::now();
-    rc = pattern_from_elements(SSF_STORAGE_SYMM_COMP_ROWS, q->n, q->ne, q->ndofe, q->cn.data(),
-                               adr, ci);
+    int *d_adr = nullptr, *d_ci = nullptr;
+    int64_t negm = 0, tot = 0;
+    {
+      const int64_t pbytes = pattern_build_bytes(q->ne, (int)q->ndofe) + 4 * q->ne * q->ndofe;
+      rc = reserve(pbytes);
+      if (rc == SSF_OK) {
+        std::vector<int> cn32(q->ne * q->ndofe);
+        bool bad = false;
+        for (int64_t k = 0; k < q->ne * q->ndofe; k++) {
+          bad |= q->cn[k] > q->n;
+          cn32[k] = (int)q->cn[k];
+        }
+        int *d_cn = nullptr;
+        hipError_t e = bad ? hipErrorInvalidValue
+                           : hipMallocAsync((void **)&d_cn, cn32.size() * sizeof(int), ctx->stream);
+        if (e == hipSuccess)
+          e = hipMemcpyAsync(d_cn, cn32.data(), cn32.size() * sizeof(int), hipMemcpyHostToDevice,
+                             ctx->stream);
+        if (bad) {
+          rc = SSF_EINVAL;
+          err = "ssf_server: code number larger than the number of unknowns";
+        } else if (e != hipSuccess) {
+          rc = SSF_EHIP;
+          err = std::string("ssf_server: ") + hipGetErrorString(e);
+        } else {
+          rc = pattern_build_dev(q->n, q->ne, (int)q->ndofe, d_cn, ctx->stream, &d_adr, &d_ci,
+                                 &negm);
+          if (rc) err = ssf_last_error();
+        }
+        if (rc == SSF_OK && 2 * negm >= (1LL << 31)) {
+          rc = SSF_EUNSUP;
+          err = "ssf_server: more than 2^31 entries in the full matrix";
+        }
+        if (rc == SSF_OK) {
+          e = sell_entries_dev(q->n, negm, d_adr, d_ci, ctx->stream, &tot);
+          if (e != hipSuccess) {
+            rc = SSF_EHIP;
+            err = std::string("ssf_server: ") + hipGetErrorString(e);
+          }
+        }
+        if (d_cn) (void)hipFreeAsync(d_cn, ctx->stream);
+        (void)hipStreamSynchronize(ctx->stream);
+        release(pbytes);
+      }
+    }
     const double dt_pattern = std::chrono::duration<double>(clk::now() - tp0).count();
     double dt_create = 0, dt_assemble = 0;
-    if (rc) err = ssf_last_error();
+    // 2. the request's set-up bytes, then the matrix laid out from the pattern
     if (rc == SSF_OK) {
-      const int64_t negm = adr[q->n];
-      const int64_t nnz = 2 * negm - q->n;
-      q->bytes = prep_bytes(*q, negm, sell_entries(q->n, adr, ci));
-      std::unique_lock<std::mutex> lk(S->mu);
-      if (q->bytes > S->budget) {
-        rc = SSF_ENOMEM;
-        err = "ssf_server: request larger than the whole HBM budget";
-      } else {
-        // FIFO over the budget: a request waits until its bytes fit
-        S->cv_budget.wait(lk, [&] { return S->used + q->bytes <= S->budget; });
-        S->used += q->bytes;
-        S->peak = std::max(S->peak, S->used);
+      q->bytes = prep_bytes(*q, negm, tot);
+      rc = reserve(q->bytes);
+      if (rc == SSF_OK) {
+        std::unique_lock<std::mutex> lk(S->mu);
         q->t_admit = S->now();
       }
     }
+    if (rc != SSF_OK) {
+      if (d_adr) (void)hipFreeAsync(d_adr, ctx->stream);
+      if (d_ci) (void)hipFreeAsync(d_ci, ctx->stream);
+    }
     if (rc == SSF_OK) {
       const auto tc0 = clk::now();
-      rc = matrix_create_sell(ctx, q->n, adr, ci, &q->m);
+      rc = matrix_create_sell_dev(ctx, q->n, d_adr, d_ci, negm, &q->m);
       dt_create = std::chrono::duration<double>(clk::now() - tc0).count();
       const auto ta0 = clk::now();
       if (rc == SSF_OK && q->m->nslice == 0) {
         set_error("ssf_server: the matrix has no SELL layout");
         rc = SSF_EUNSUP;
       }
-      std::vector<int64_t>().swap(adr);
-      std::vector<int64_t>().swap(ci);
       std::vector<double> lift(q->n, 0.0);
       if (rc == SSF_OK)
         rc = ssf_matrix_assemble(q->m, q->kind, q->nn, q->xyz.data(), q->ne, q->conn.data(),

@@ -54,6 +54,7 @@ SIGNATURES = {
     "ssf_ctx_sync": (_I, [_P]),
     "ssf_ctx_stream": (_P, [_P]),
     "ssf_matrix_create_elem": (_I, [_P, _I, _L, _L, _L, _P, ctypes.POINTER(_P)]),
+    "ssf_matrix_create_elem_device": (_I, [_P, _L, _L, _L, _P, ctypes.POINTER(_P)]),
     "ssf_matrix_create_csr": (_I, [_P, _I, _L, _P, _P, ctypes.POINTER(_P)]),
     "ssf_matrix_destroy": (_I, [_P]),
     "ssf_matrix_sizes": (_I, [_P, _P, _P, _P]),
@@ -198,6 +199,15 @@ class Matrix:
         h = _P()
         _check(lib().ssf_matrix_create_elem(ctx.h, storage, n, cn.shape[0], cn.shape[1], _p(cn),
                                             ctypes.byref(h)))
+        return cls(ctx, h)
+
+    @classmethod
+    def from_elements_device(cls, ctx: Context, n: int, cn: np.ndarray) -> "Matrix":
+        """symcomprow pattern built on the device; SELL-only matrix (FP64 solves)."""
+        cn = _i64(cn)
+        h = _P()
+        _check(lib().ssf_matrix_create_elem_device(ctx.h, n, cn.shape[0], cn.shape[1], _p(cn),
+                                                   ctypes.byref(h)))
         return cls(ctx, h)
 
     @classmethod
