@@ -287,6 +287,18 @@ int ssf_elements_results_dev(ssf_elements *el, const double *lhs_dev, const doub
                              int64_t npv, double *strain_dev, double *stress_dev,
                              double *ifor_dev);
 
+/* Element volume loads of the set added into rhs[n] (on top of its contents,
+ * e.g. the nodal loads), per unknown in element order as loadcase::forcecontrib
+ * (MEFEL/SRC/loadcase.cpp:571-597) localizes them. Hex8 elasticity: the nodal
+ * force vector of a volume load, loadel::volumeload (MEFEL/SRC/loadel.cpp:438)
+ * with linhex::load_matrix (MEFEL/SRC/linhex.cpp:969); hex8 heat: the source
+ * vector of linhext::quantity_source_vector (TRFEL/SRC/linhext.cpp:1039).
+ * nodval[ne][ndofe] are the load's nodal values on each element. Host arrays,
+ * synchronous (SSF_EUNSUP for quad4). */
+int ssf_elements_volume_load(ssf_elements *el, const double *nodval, double *rhs);
+/* same on device pointers, asynchronous on the context's stream */
+int ssf_elements_volume_load_dev(ssf_elements *el, const double *nodval_dev, double *rhs_dev);
+
 /* ------------------------------------------------------------ diagnostics */
 /* Time `reps` calls of ssf_elements_results_dev: usec[0] = mean duration of the
  * element kernel (its own start and end events), usec[1] = mean duration of the

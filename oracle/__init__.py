@@ -61,6 +61,8 @@ def lib():
                                             _P, _P, _P, _P]),
             "cr_hex8_heat_cap": (None, [_P, _P, _P, _D, _P, _P]),
             "cr_hex8_heat_cap_mesh": (None, [_L, _P, _P, _P, _P, _P]),
+            "cr_hex8_volume_load": (None, [ctypes.c_int, _P, _P, _P, _P, _P]),
+            "cr_volume_load_mesh": (None, [ctypes.c_int, _L, _P, _P, _P, _P, _P]),
             "cr_np_step_linear": (ctypes.c_int, [_L, _P, _P, _P, _P, _P, _D, _D, _P, _P, _P, _P,
                                                  _P, _P, _L, _D, _D, _P, _P]),
         }
@@ -106,6 +108,7 @@ def ref():
             "ref_quad4_pstress_ip": (None, [_P, _P, _D, _D, _P, _P, _P, _P, _P]),
             "ref_hex8_heat_ip": (None, [_P, _P, _P, _D, _P, _P, _P, _P]),
             "ref_locglob": (None, [_P, _P, _P, _L]),
+            "ref_hex8_volume_load": (None, [ctypes.c_int, _P, _P, _P, _P, _P]),
             "ref_hex8_heat_cap": (None, [_P, _P, _P, _D, _P, _P]),
             "ref_np_step_linear": (None, [_P, _P, _P, _L, _D, _D, _P, _P, _P, _L, _D, _D, _P, _P]),
         }
@@ -320,3 +323,12 @@ def np_step_linear(adr, ci, aK, aC, alpha, dt, rhs, lhs, tdlhs, ni=20000, res=1e
                                  _ptr(p), _ptr(f), ni, res, zero, ctypes.byref(ani),
                                  ctypes.byref(ares))
     return st, ani.value, ares.value, aA
+
+
+def volume_load(prob, nodval, rhs=None):
+    """Element volume loads (elasticity) / heat sources of a hex8 mesh added into
+    rhs (a copy of rhs, or zeros) in element order; nodval (ne, ndofe)."""
+    out = np.zeros(prob.n) if rhs is None else f64(rhs).copy()
+    lib().cr_volume_load_mesh(prob.kind, prob.ne, _ptr(i64(prob.conn)), _ptr(f64(prob.coords)),
+                              _ptr(i64(prob.elem_codes())), _ptr(f64(nodval)), _ptr(out))
+    return out

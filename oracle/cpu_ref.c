@@ -1183,3 +1183,95 @@ int cr_np_step_linear(long n, const long *adr, const long *ci, const double *aK,
   for (long j = 0; j < n; j++) lhs[j] = d[j] + alpha * dt * tdlhs[j];
   return st;
 }
+
+/* ======================================== element volume loads / sources === */
+
+static void bf_hex(double *n, double xi, double eta, double zeta)
+{
+  n[0] = (1.0 + xi) * (1.0 + eta) * (1.0 + zeta) / 8.0;
+  n[1] = (1.0 - xi) * (1.0 + eta) * (1.0 + zeta) / 8.0;
+  n[2] = (1.0 - xi) * (1.0 - eta) * (1.0 + zeta) / 8.0;
+  n[3] = (1.0 + xi) * (1.0 - eta) * (1.0 + zeta) / 8.0;
+  n[4] = (1.0 + xi) * (1.0 + eta) * (1.0 - zeta) / 8.0;
+  n[5] = (1.0 - xi) * (1.0 + eta) * (1.0 - zeta) / 8.0;
+  n[6] = (1.0 - xi) * (1.0 - eta) * (1.0 - zeta) / 8.0;
+  n[7] = (1.0 + xi) * (1.0 - eta) * (1.0 - zeta) / 8.0;
+}
+
+/* nodal load vector of one hex8 element from the nodal values of a volume load:
+ *   kind 0 (elasticity): loadel::volumeload (MEFEL/SRC/loadel.cpp:438-450) with
+ *     linhex::load_matrix (MEFEL/SRC/linhex.cpp:969-1000): lm = sum N^T N jac
+ *     (N = 3x24 of linhex::bf_matrix :126-140, signed jac_3d), nf = lm nodval
+ *     by the array mxv (GEFEL/matrix.cpp:1749-1762);
+ *   kind 1 (heat): linhext::quantity_source_vector (TRFEL/SRC/linhext.cpp:1039-1072):
+ *     nm = sum N^T N |jac| (N = 1x8), sv = 0 + nm nodval (mxv :1711, addv).
+ * nnj (GEFEL/matrix.cpp:3149-3174) restated for the 3-row N: entry (i, j) gets
+ * s = sum_k n[k][i] n[k][j] in k order times jac. */
+void cr_hex8_volume_load(int kind, const double *x, const double *y, const double *z,
+                         const double *nodval, double *nf)
+{
+  const int nd = kind == 0 ? 24 : 8, mm = kind == 0 ? 3 : 1;
+  double lm[24 * 24], n[3 * 24], bf[8], dx[8], dy[8], dz[8];
+  memset(lm, 0, sizeof lm);
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++)
+      for (int k = 0; k < 2; k++) {
+        double xi = GP2[i], eta = GP2[j], zeta = GP2[k];
+        dx_hex(dx, eta, zeta);
+        dy_hex(dy, xi, zeta);
+        dz_hex(dz, xi, eta);
+        double jac = derivatives_3d(dx, dy, dz, x, y, z, xi, eta, zeta);
+        bf_hex(bf, xi, eta, zeta);
+        memset(n, 0, sizeof n);
+        if (kind == 0) {
+          for (int a = 0; a < 8; a++) {
+            n[0 * 24 + 3 * a] = bf[a];
+            n[1 * 24 + 3 * a + 1] = bf[a];
+            n[2 * 24 + 3 * a + 2] = bf[a];
+          }
+        } else {
+          for (int a = 0; a < 8; a++) n[a] = bf[a];
+          if (jac < 0.0) jac = fabs(jac);
+        }
+        jac *= GW2[i] * GW2[j] * GW2[k];
+        for (int ii = 0; ii < nd; ii++) {
+          double s = 0.0;
+          for (int q = 0; q < mm; q++) s += n[q * nd + ii] * n[q * nd + ii];
+          lm[ii * nd + ii] += s * jac;
+          for (int jj = ii + 1; jj < nd; jj++) {
+            s = 0.0;
+            for (int q = 0; q < mm; q++) s += n[q * nd + ii] * n[q * nd + jj];
+            lm[ii * nd + jj] += s * jac;
+            lm[jj * nd + ii] += s * jac;
+          }
+        }
+      }
+  for (int ii = 0; ii < nd; ii++) {
+    double s = 0.0;
+    for (int jj = 0; jj < nd; jj++) s += lm[ii * nd + jj] * nodval[jj];
+    nf[ii] = kind == 0 ? s : 0.0 + s;
+  }
+}
+
+/* the element loads of a mesh added into rhs[n] (on top of its contents) in
+ * element order by locglob: loadcase::forcecontrib's element loop
+ * (MEFEL/SRC/loadcase.cpp:571-597) / the source loop of trfel_right_hand_side.
+ * nodval[ne][nd]. */
+void cr_volume_load_mesh(int kind, long ne, const long *conn, const double *xyz, const long *cn,
+                         const double *nodval, double *rhs)
+{
+  const int nd = kind == 0 ? 24 : 8;
+  double x[8], y[8], z[8], nf[24];
+  for (long el = 0; el < ne; el++) {
+    for (int a = 0; a < 8; a++) {
+      long nd0 = conn[el * 8 + a];
+      x[a] = xyz[3 * nd0];
+      y[a] = xyz[3 * nd0 + 1];
+      z[a] = xyz[3 * nd0 + 2];
+    }
+    cr_hex8_volume_load(kind, x, y, z, nodval + el * nd, nf);
+    const long *c = cn + el * nd;
+    for (int i = 0; i < nd; i++)
+      if (c[i] >= 1) rhs[c[i] - 1] += nf[i];
+  }
+}

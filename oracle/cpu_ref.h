@@ -36,6 +36,10 @@
  *   cr_np_step_linear     bitwise vs oracle/_ref (the reference's own symcomprow
  *                         mxv_scr / scalmat_scr / addmat_scr / cg_prec); amplification
  *                         factor of the trapezoidal rule on one unknown; steady state
+ *   cr_hex8_volume_load, cr_volume_load_mesh
+ *                         bitwise vs oracle/_ref (GEFEL jac_3d, nnj, mxv in the loop
+ *                         order of load_matrix / quantity_source_vector); constant
+ *                         load on a parallelepiped: b V / 8 per node; totals = b V
  */
 #ifndef SSF_ORACLE_CPU_REF_H
 #define SSF_ORACLE_CPU_REF_H
@@ -165,6 +169,13 @@ int cr_np_step_linear(long n, const long *adr, const long *ci, const double *aK,
                       const double *aC, double *aA, double alpha, double dt,
                       const double *rhs, double *lhs, double *tdlhs, double *d, double *p,
                       double *f, long ni, double res, double zero, long *ani, double *ares);
+
+/* ---- element volume loads (MEFEL loadel::volumeload with linhex::load_matrix,
+ * kind 0) and heat sources (TRFEL linhext::quantity_source_vector, kind 1) */
+void cr_hex8_volume_load(int kind, const double *x, const double *y, const double *z,
+                         const double *nodval, double *nf);
+void cr_volume_load_mesh(int kind, long ne, const long *conn, const double *xyz, const long *cn,
+                         const double *nodval, double *rhs);
 
 #ifdef __cplusplus
 }

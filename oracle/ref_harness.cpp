@@ -477,4 +477,51 @@ void ref_np_step_linear(void *hK, void *hC, void *hA, long n, double alpha, doub
   delete[] f;
 }
 
+// ------------------------------------------------ element volume loads / sources
+// kind 0: loadel::volumeload with linhex::load_matrix (MEFEL/SRC/loadel.cpp:438,
+// MEFEL/SRC/linhex.cpp:969) and linhex::bf_matrix (:126); kind 1:
+// linhext::quantity_source_vector (TRFEL/SRC/linhext.cpp:1039); GEFEL's
+// gauss_points, jac_3d, bf_lin_hex_3d, nnj, mxv, addv
+void ref_hex8_volume_load(int kind, const double *xc, const double *yc, const double *zc,
+                          const double *nodval_in, double *nf_out)
+{
+  const long nne = 8, nd = kind == 0 ? 24 : 8, mm = kind == 0 ? 3 : 1;
+  vector x(nne), y(nne), z(nne), w(2), gp(2), bf(nne), nodval(nd), v(nd), sv(nd);
+  matrix n(mm, nd), lm(nd, nd);
+  for (long i = 0; i < nne; i++) { x[i] = xc[i]; y[i] = yc[i]; z[i] = zc[i]; }
+  for (long i = 0; i < nd; i++) nodval[i] = nodval_in[i];
+  gauss_points(gp.a, w.a, 2);
+  nullm(lm);
+  for (long i = 0; i < 2; i++)
+    for (long j = 0; j < 2; j++)
+      for (long k = 0; k < 2; k++) {
+        double xi = gp[i], eta = gp[j], zeta = gp[k], jac;
+        jac_3d(jac, x, y, z, xi, eta, zeta);
+        nullm(n);
+        if (kind == 0) {
+          bf_lin_hex_3d(bf.a, xi, eta, zeta);
+          long jj = 0, kk = 1, ll = 2;
+          for (long a = 0; a < nne; a++) {
+            n[0][jj] = bf[a]; jj += 3;
+            n[1][kk] = bf[a]; kk += 3;
+            n[2][ll] = bf[a]; ll += 3;
+          }
+        } else {
+          bf_lin_hex_3d(n.a, xi, eta, zeta);
+          if (jac < 0.0) jac = fabs(jac);
+        }
+        jac *= w[i] * w[j] * w[k];
+        nnj(lm.a, n.a, jac, n.m, n.n);
+      }
+  if (kind == 0) {
+    mxv(lm.a, nodval.a, v.a, nd, nd);
+    for (long i = 0; i < nd; i++) nf_out[i] = v[i];
+  } else {
+    nullv(sv);
+    mxv(lm, nodval, v);
+    addv(sv, v, sv);
+    for (long i = 0; i < nd; i++) nf_out[i] = sv[i];
+  }
+}
+
 }  // extern "C"

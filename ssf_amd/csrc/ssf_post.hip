@@ -414,18 +414,99 @@ __global__ __launch_bounds__(TPB) void k_post(int64_t ne, const int *__restrict_
   }
 }
 
+// Element volume loads (hex8 elasticity: loadel::volumeload MEFEL/SRC/loadel.cpp:438
+// with linhex::load_matrix MEFEL/SRC/linhex.cpp:969, signed Jacobian) and heat
+// sources (linhext::quantity_source_vector TRFEL/SRC/linhext.cpp:1039, |J|), one
+// thread per element: lm = sum over the points of N^T N jac (the scalar 8x8
+// block; the 3-row N of elasticity repeats it per component and adds exact zeros
+// elsewhere), nf = lm nodval in column order. Written to fe in the component-major
+// order of the force entries.
+template <int KIND>
+__global__ __launch_bounds__(TPB) void k_vload(int64_t ne, const int *__restrict__ conn,
+                                               const double *__restrict__ xyz,
+                                               const double *__restrict__ nodval,
+                                               double *__restrict__ fe)
+{
+#pragma clang fp contract(off)
+  constexpr int DPN = KIND == SSF_ELEM_HEX8_ELAST ? 3 : 1, ND = 8 * DPN;
+  const int64_t e = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (e >= ne) return;
+  double x[8], y[8], z[8];
+#pragma unroll
+  for (int a = 0; a < 8; a++) {
+    const int64_t nd = conn[e * 8 + a];
+    x[a] = xyz[3 * nd];
+    y[a] = xyz[3 * nd + 1];
+    z[a] = xyz[3 * nd + 2];
+  }
+  double lm[8][8];
+#pragma unroll
+  for (int a = 0; a < 8; a++)
+#pragma unroll
+    for (int b = 0; b < 8; b++) lm[a][b] = 0.0;
+  for (int g = 0; g < 8; g++) {
+    const double xi = (g & 4) ? kGP : -kGP, eta = (g & 2) ? kGP : -kGP, zeta = (g & 1) ? kGP : -kGP;
+    double nx[8], ny[8], nz[8];
+    hex_nat(g, nx, ny, nz);
+    double d1 = 0.0, d2 = 0.0, d3 = 0.0, d4 = 0.0, d5 = 0.0, d6 = 0.0, d7 = 0.0, d8 = 0.0,
+           d9 = 0.0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      d1 += nx[i] * x[i];
+      d2 += ny[i] * x[i];
+      d3 += nz[i] * x[i];
+      d4 += nx[i] * y[i];
+      d5 += ny[i] * y[i];
+      d6 += nz[i] * y[i];
+      d7 += nx[i] * z[i];
+      d8 += ny[i] * z[i];
+      d9 += nz[i] * z[i];
+    }
+    double jac = d1 * d5 * d9 + d4 * d8 * d3 + d7 * d2 * d6 - d7 * d5 * d3 - d8 * d6 * d1 -
+                 d9 * d4 * d2;
+    if (KIND == SSF_ELEM_HEX8_HEAT && jac < 0.0) jac = fabs(jac);
+    double bf[8];
+    bf[0] = (1.0 + xi) * (1.0 + eta) * (1.0 + zeta) / 8.0;
+    bf[1] = (1.0 - xi) * (1.0 + eta) * (1.0 + zeta) / 8.0;
+    bf[2] = (1.0 - xi) * (1.0 - eta) * (1.0 + zeta) / 8.0;
+    bf[3] = (1.0 + xi) * (1.0 - eta) * (1.0 + zeta) / 8.0;
+    bf[4] = (1.0 + xi) * (1.0 + eta) * (1.0 - zeta) / 8.0;
+    bf[5] = (1.0 - xi) * (1.0 + eta) * (1.0 - zeta) / 8.0;
+    bf[6] = (1.0 - xi) * (1.0 - eta) * (1.0 - zeta) / 8.0;
+    bf[7] = (1.0 + xi) * (1.0 - eta) * (1.0 - zeta) / 8.0;
+#pragma unroll
+    for (int a = 0; a < 8; a++)
+#pragma unroll
+      for (int b = a; b < 8; b++) {
+        const double v = (bf[a] * bf[b]) * jac;
+        lm[a][b] += v;
+        if (b != a) lm[b][a] += v;
+      }
+  }
+  const double *nv = nodval + e * ND;
+#pragma unroll
+  for (int c = 0; c < DPN; c++)
+#pragma unroll
+    for (int a = 0; a < 8; a++) {
+      double sacc = 0.0;
+#pragma unroll
+      for (int b = 0; b < 8; b++) sacc += lm[a][b] * nv[b * DPN + c];
+      fe[e * ND + c * 8 + a] = KIND == SSF_ELEM_HEX8_ELAST ? sacc : 0.0 + sacc;
+    }
+}
+
 // ifor[d] = sum over the element entries of unknown d, in element order
 // (locglob into a cleared vector). The entry indices and values are loaded eight
 // at a time so that their loads overlap; the sum keeps the entry order.
 __global__ __launch_bounds__(TPB) void k_post_gather(int n, const int *__restrict__ start,
                                                      const int *__restrict__ src,
                                                      const double *__restrict__ fe,
-                                                     double *__restrict__ ifor)
+                                                     double *__restrict__ ifor, int accumulate)
 {
   const int d = xcd_block(blockIdx.x, gridDim.x) * TPB + threadIdx.x;
   if (d >= n) return;
   const int k1 = start[d + 1];
-  double s = 0.0;
+  double s = accumulate ? ifor[d] : 0.0;
   for (int k = start[d]; k < k1; k += 8) {
     int idx[8];
     double v[8];
@@ -676,7 +757,7 @@ static hipError_t post_launch(ssf_elements *el, const double *lhs, const double 
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || !ifor || el->n == 0) return e;
   hipLaunchKernelGGL(k_post_gather, dim3(blocks_for(el->n, TPB)), dim3(TPB), 0, s, (int)el->n,
-                     el->start, el->src, el->fe, ifor);
+                     el->start, el->src, el->fe, ifor, 0);
   return hipGetLastError();
 }
 
@@ -771,5 +852,63 @@ extern "C" int ssf_elements_bench(ssf_elements *el, const double *lhs_dev, const
   SSF_HIP(e);
   usec[0] = 1e3 * tk / reps;
   usec[1] = 1e3 * tc / reps;
+  return SSF_OK;
+}
+
+static hipError_t vload_launch(ssf_elements *el, const double *nodval, double *rhs)
+{
+  hipStream_t s = el->ctx->stream;
+  const dim3 grid(blocks_for(el->ne, TPB)), block(TPB);
+  if (el->kind == SSF_ELEM_HEX8_ELAST)
+    hipLaunchKernelGGL(k_vload<SSF_ELEM_HEX8_ELAST>, grid, block, 0, s, el->ne, el->conn, el->xyz,
+                       nodval, el->fe);
+  else
+    hipLaunchKernelGGL(k_vload<SSF_ELEM_HEX8_HEAT>, grid, block, 0, s, el->ne, el->conn, el->xyz,
+                       nodval, el->fe);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || el->n == 0) return e;
+  hipLaunchKernelGGL(k_post_gather, dim3(blocks_for(el->n, TPB)), block, 0, s, (int)el->n,
+                     el->start, el->src, el->fe, rhs, 1);
+  return hipGetLastError();
+}
+
+extern "C" int ssf_elements_volume_load_dev(ssf_elements *el, const double *nodval_dev,
+                                            double *rhs_dev)
+{
+  SSF_ENTER();
+  SSF_CHECK(el && nodval_dev && (rhs_dev || el->n == 0), SSF_EINVAL,
+            "ssf_elements_volume_load_dev: NULL argument");
+  SSF_CHECK(el->kind != SSF_ELEM_QUAD4_PSTRESS, SSF_EUNSUP,
+            "ssf_elements_volume_load: hex8 elasticity and heat only");
+  SSF_HIP(hipSetDevice(el->ctx->device));
+  SSF_HIP(vload_launch(el, nodval_dev, rhs_dev));
+  return SSF_OK;
+}
+
+extern "C" int ssf_elements_volume_load(ssf_elements *el, const double *nodval, double *rhs)
+{
+  SSF_ENTER();
+  SSF_CHECK(el && nodval && (rhs || el->n == 0), SSF_EINVAL,
+            "ssf_elements_volume_load: NULL argument");
+  SSF_CHECK(el->kind != SSF_ELEM_QUAD4_PSTRESS, SSF_EUNSUP,
+            "ssf_elements_volume_load: hex8 elasticity and heat only");
+  SSF_HIP(hipSetDevice(el->ctx->device));
+  hipStream_t s = el->ctx->stream;
+  double *d_nv = nullptr, *d_rhs = nullptr;
+  hipError_t e = hipSuccess;
+  auto c = [&](hipError_t x) { if (e == hipSuccess) e = x; };
+  c(dalloc(&d_nv, el->ne * el->nd, s));
+  c(dalloc(&d_rhs, el->n, s));
+  if (e == hipSuccess)
+    c(hipMemcpyAsync(d_nv, nodval, el->ne * el->nd * sizeof(double), hipMemcpyHostToDevice, s));
+  if (e == hipSuccess && el->n)
+    c(hipMemcpyAsync(d_rhs, rhs, el->n * sizeof(double), hipMemcpyHostToDevice, s));
+  if (e == hipSuccess) c(vload_launch(el, d_nv, d_rhs));
+  if (e == hipSuccess && el->n)
+    c(hipMemcpyAsync(rhs, d_rhs, el->n * sizeof(double), hipMemcpyDeviceToHost, s));
+  c(hipStreamSynchronize(s));
+  dfree(d_nv, s);
+  dfree(d_rhs, s);
+  SSF_HIP(e);
   return SSF_OK;
 }

@@ -97,6 +97,8 @@ SIGNATURES = {
     "ssf_elements_sizes": (_I, [_P, _P, _P, _P]),
     "ssf_elements_results": (_I, [_P, _P, _P, _L, _P, _P, _P]),
     "ssf_elements_results_dev": (_I, [_P, _P, _P, _L, _P, _P, _P]),
+    "ssf_elements_volume_load": (_I, [_P, _P, _P]),
+    "ssf_elements_volume_load_dev": (_I, [_P, _P, _P]),
     "ssf_elements_bench": (_I, [_P, _P, _P, _L, _P, _P, _P, _I, _P]),
 }
 
@@ -514,6 +516,13 @@ class Elements:
                                           0 if pv is None else pv.shape[0], _p(eps), _p(sig),
                                           _p(f)))
         return eps, sig, f
+
+    def volume_load(self, nodval, rhs=None):
+        """Element volume loads (heat: sources) with nodal values nodval (ne, ndofe)
+        added into a copy of rhs (zeros if None) in element order."""
+        out = np.zeros(self.n) if rhs is None else _f64(rhs).copy()
+        _check(lib().ssf_elements_volume_load(self.h, _p(_f64(nodval)), _p(out)))
+        return out
 
     def results_dev(self, lhs_ptr, pv_ptr, npv, strain_ptr, stress_ptr, ifor_ptr):
         _check(lib().ssf_elements_results_dev(self.h, lhs_ptr, pv_ptr, npv, strain_ptr,

@@ -16,6 +16,10 @@ Nonstationary heat (heat_np_*): capacity matrices of the heat case (seeded c and
 rho) from the harness's ref_hex8_heat_cap, and three steps of one_step_linear on
 the reference's own symcomprow objects (alpha 0.5, dt 0.01, from T = v = 0 with
 the Dirichlet lift as the right-hand side): T, v, ani and ares after each step.
+Element volume loads (elast_vload_*, heat_vload_*): seeded nodal load values per
+element, their load vectors from the harness's ref_hex8_volume_load (load_matrix
+/ quantity_source_vector), added by locglob in element order onto a seeded
+starting vector (standing for the nodal loads added first).
 Post-solve element pass (post_*): for a seeded nodal solution post_x, the
 integration-point strains / stresses (gradients / fluxes for heat) and element
 forces of the harness's ref_*_ip, gathered by eldispl's code-number rule and added
@@ -163,6 +167,24 @@ def main():
         out[f"{name}_post_eps"], out[f"{name}_post_sig"], out[f"{name}_post_ifor"] = \
             post_pass(R, prob, xp)
         print(name, prob.n, "unknowns", flush=True)
+    # element volume loads / heat sources
+    for name in ("elast", "heat"):
+        prob = CASES[name]()
+        vr = np.random.default_rng(3000 + list(CASES).index(name))
+        nodval = vr.standard_normal((prob.ne, prob.ndofe))
+        rhs0 = vr.standard_normal(prob.n)
+        rhs = rhs0.copy()
+        cn = i64(prob.elem_codes())
+        kind = 0 if prob.kind == mesh.KIND_HEX8_ELAST else 1
+        for e in range(prob.ne):
+            nodes = prob.conn[e]
+            xx, yy, zz = (f64(prob.coords[nodes, k]) for k in range(3))
+            nf = np.zeros(prob.ndofe)
+            R.ref_hex8_volume_load(kind, _ptr(xx), _ptr(yy), _ptr(zz), _ptr(f64(nodval[e])),
+                                   _ptr(nf))
+            R.ref_locglob(_ptr(rhs), _ptr(nf), _ptr(cn[e].copy()), prob.ndofe)
+        out[f"{name}_vload_nodval"], out[f"{name}_vload_rhs0"] = nodval, rhs0
+        out[f"{name}_vload_rhs"] = rhs
     # nonstationary heat on the heat case
     prob = CASES["heat"]()
     grng = np.random.default_rng(2000)
