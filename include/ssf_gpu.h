@@ -35,6 +35,7 @@ extern "C" {
 #define SSF_STORAGE_COMPRESSED_ROWS 10 /* comprow            */
 #define SSF_STORAGE_SYMM_COMP_ROWS 11  /* symcomprow        */
 #define SSF_SOLVER_CG 20               /* linsolvertype cg   */
+#define SSF_SOLVER_BICG 30             /* linsolvertype bicg */
 #define SSF_PREC_NOPRECOND 0           /* precondtype        */
 #define SSF_PREC_DIAG 1                /* diagprec (Jacobi)  */
 
@@ -184,6 +185,19 @@ int ssf_solve_dev(ssf_matrix *m, const ssf_solve_params *p, const double *rhs_de
 int ssf_np_step_linear(ssf_matrix *K, ssf_matrix *C, ssf_matrix *A, double alpha, double dt,
                        const ssf_solve_params *p, const double *rhs_dev, double *lhs_dev,
                        double *tdlhs_dev, ssf_solve_info *info);
+
+/* One step of linear_nonstat_solv_dform (TRFEL/SRC/npsolvert.cpp:1071-1227, the
+ * loop body :1152-1185): the generalized trapezoidal rule in the d-form, the
+ * scheme the reference selects for tnpsolver = trapezoidd and for discontinuous
+ * problems (:19-31):
+ *   d = lhs + (1 - alpha) dt tdlhs;  p = C d;  A = K * (dt alpha) + 1.0 * C;
+ *   f = rhs alpha dt + p;  A lhs = f by ssf_solve_dev from the current lhs (the
+ *   iv field of p is ignored);  tdlhs = (lhs - d) / dt / alpha.
+ * Arguments, A's caching and the errors are those of ssf_np_step_linear; alpha
+ * and dt must be nonzero. Synchronous. */
+int ssf_np_step_dform(ssf_matrix *K, ssf_matrix *C, ssf_matrix *A, double alpha, double dt,
+                      const ssf_solve_params *p, const double *rhs_dev, double *lhs_dev,
+                      double *tdlhs_dev, ssf_solve_info *info);
 
 /* ------------------------------------------------------------------ batches */
 /* Many assembled systems solved together, each with nrhs right-hand sides

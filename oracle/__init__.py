@@ -65,6 +65,8 @@ def lib():
             "cr_volume_load_mesh": (None, [ctypes.c_int, _L, _P, _P, _P, _P, _P]),
             "cr_np_step_linear": (ctypes.c_int, [_L, _P, _P, _P, _P, _P, _D, _D, _P, _P, _P, _P,
                                                  _P, _P, _L, _D, _D, _P, _P]),
+            "cr_np_step_dform": (ctypes.c_int, [_L, _P, _P, _P, _P, _P, _D, _D, _P, _P, _P, _P,
+                                                _P, _P, _L, _D, _D, _P, _P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(l, name)
@@ -111,6 +113,7 @@ def ref():
             "ref_hex8_volume_load": (None, [ctypes.c_int, _P, _P, _P, _P, _P]),
             "ref_hex8_heat_cap": (None, [_P, _P, _P, _D, _P, _P]),
             "ref_np_step_linear": (None, [_P, _P, _P, _L, _D, _D, _P, _P, _P, _L, _D, _D, _P, _P]),
+            "ref_np_step_dform": (None, [_P, _P, _P, _L, _D, _D, _P, _P, _P, _L, _D, _D, _P, _P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(l, name)
@@ -310,18 +313,18 @@ def capacity_matrices(prob, c, rho=None):
 
 
 def np_step_linear(adr, ci, aK, aC, alpha, dt, rhs, lhs, tdlhs, ni=20000, res=1e-20,
-                   zero=1e-30):
+                   zero=1e-30, form="v"):
     """One generalized-trapezoidal step; lhs and tdlhs are updated in place.
-    Returns (status, ani, ares, aA)."""
+    form "v": one_step_linear, "d": the d-form step. Returns (status, ani, ares, aA)."""
     n = adr.shape[0] - 1
     aA = np.zeros(aK.shape[0])
     d, p, f = np.zeros(n), np.zeros(n), np.zeros(n)
     ani = ctypes.c_long(-1)
     ares = ctypes.c_double(-1.0)
-    st = lib().cr_np_step_linear(n, _ptr(adr), _ptr(ci), _ptr(f64(aK)), _ptr(f64(aC)), _ptr(aA),
-                                 alpha, dt, _ptr(f64(rhs)), _ptr(lhs), _ptr(tdlhs), _ptr(d),
-                                 _ptr(p), _ptr(f), ni, res, zero, ctypes.byref(ani),
-                                 ctypes.byref(ares))
+    fn = {"v": lib().cr_np_step_linear, "d": lib().cr_np_step_dform}[form]
+    st = fn(n, _ptr(adr), _ptr(ci), _ptr(f64(aK)), _ptr(f64(aC)), _ptr(aA), alpha, dt,
+            _ptr(f64(rhs)), _ptr(lhs), _ptr(tdlhs), _ptr(d), _ptr(p), _ptr(f), ni, res, zero,
+            ctypes.byref(ani), ctypes.byref(ares))
     return st, ani.value, ares.value, aA
 
 

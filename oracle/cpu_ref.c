@@ -1184,6 +1184,33 @@ int cr_np_step_linear(long n, const long *adr, const long *ci, const double *aK,
   return st;
 }
 
+/* TRFEL/SRC/npsolvert.cpp:1071-1227 linear_nonstat_solv_dform, one pass of its
+ * time loop (:1152-1185), generalized trapezoidal rule in the d-form:
+ *   d = lhs + (1 - alpha) dt v           (:1152-1154)
+ *   p = C d                              (:1156, gmxv -> mxv_scr GEFEL/scr.cpp:655)
+ *   A = K * (dt alpha) + 1.0 * C         (:1158-1159)
+ *   f = rhs alpha dt + p                 (:1166-1168)
+ *   A lhs = f by cg_prec from lhs (iv=1) (:1181)
+ *   v = (lhs - d) / dt / alpha           (:1183-1185)
+ * Work vectors and the return value as cr_np_step_linear. */
+int cr_np_step_dform(long n, const long *adr, const long *ci, const double *aK,
+                     const double *aC, double *aA, double alpha, double dt,
+                     const double *rhs, double *lhs, double *tdlhs, double *d, double *p,
+                     double *f, long ni, double res, double zero, long *ani, double *ares)
+{
+  long negm = adr[n];
+  for (long j = 0; j < n; j++) d[j] = lhs[j] + (1.0 - alpha) * dt * tdlhs[j];
+  cr_mxv_scr(n, adr, ci, aC, d, p);
+  for (long i = 0; i < negm; i++) aA[i] = aK[i];
+  double s = dt * alpha;
+  for (long i = 0; i < negm; i++) aA[i] *= s;
+  for (long i = 0; i < negm; i++) aA[i] += 1.0 * aC[i];
+  for (long j = 0; j < n; j++) f[j] = rhs[j] * alpha * dt + p[j];
+  int st = cr_scr_cg_prec(n, adr, ci, aA, lhs, f, ni, res, ani, ares, zero, 1);
+  for (long j = 0; j < n; j++) tdlhs[j] = (lhs[j] - d[j]) / dt / alpha;
+  return st;
+}
+
 /* ======================================== element volume loads / sources === */
 
 static void bf_hex(double *n, double xi, double eta, double zeta)

@@ -33,9 +33,11 @@
  *                         assembled forces = A x - lift, D / -k closed forms
  *   cr_hex8_heat_cap      bitwise vs oracle/_ref; unit-cube closed form (exact with
  *                         2x2x2 points); sum of entries = rho c V
- *   cr_np_step_linear     bitwise vs oracle/_ref (the reference's own symcomprow
+ *   cr_np_step_linear, cr_np_step_dform
+ *                         bitwise vs oracle/_ref (the reference's own symcomprow
  *                         mxv_scr / scalmat_scr / addmat_scr / cg_prec); amplification
- *                         factor of the trapezoidal rule on one unknown; steady state
+ *                         factor of the trapezoidal rule on one unknown; steady state;
+ *                         the two forms agree to rounding on the same problem
  *   cr_hex8_volume_load, cr_volume_load_mesh
  *                         bitwise vs oracle/_ref (GEFEL jac_3d, nnj, mxv in the loop
  *                         order of load_matrix / quantity_source_vector); constant
@@ -169,6 +171,10 @@ int cr_np_step_linear(long n, const long *adr, const long *ci, const double *aK,
                       const double *aC, double *aA, double alpha, double dt,
                       const double *rhs, double *lhs, double *tdlhs, double *d, double *p,
                       double *f, long ni, double res, double zero, long *ani, double *ares);
+int cr_np_step_dform(long n, const long *adr, const long *ci, const double *aK,
+                     const double *aC, double *aA, double alpha, double dt,
+                     const double *rhs, double *lhs, double *tdlhs, double *d, double *p,
+                     double *f, long ni, double res, double zero, long *ani, double *ares);
 
 /* ---- element volume loads (MEFEL loadel::volumeload with linhex::load_matrix,
  * kind 0) and heat sources (TRFEL linhext::quantity_source_vector, kind 1) */

@@ -477,6 +477,32 @@ void ref_np_step_linear(void *hK, void *hC, void *hA, long n, double alpha, doub
   delete[] f;
 }
 
+// one pass of linear_nonstat_solv_dform's time loop (TRFEL/SRC/npsolvert.cpp:
+// 1152-1185) on the reference's own symcomprow objects: C->mxv_scr,
+// A->scalmat_scr, A->addmat_scr, A->cg_prec; the vector loops are restated
+void ref_np_step_dform(void *hK, void *hC, void *hA, long n, double alpha, double dt,
+                       const double *rhs, double *lhs, double *tdlhs, long ni, double res,
+                       double zero, long *ani, double *ares)
+{
+  symcomprow *K = (symcomprow *)hK, *C = (symcomprow *)hC, *A = (symcomprow *)hA;
+  double *d = new double[n], *p = new double[n], *f = new double[n];
+  for (long j = 0; j < n; j++) d[j] = lhs[j] + (1.0 - alpha) * dt * tdlhs[j];
+  C->mxv_scr(d, p);
+  memcpy(A->a, K->a, K->negm * sizeof(double));
+  A->scalmat_scr(dt * alpha);
+  A->addmat_scr(1.0, *C);
+  for (long j = 0; j < n; j++) f[j] = rhs[j] * alpha * dt + p[j];
+  long a = *ani;
+  double r = *ares;
+  A->cg_prec(lhs, f, ni, res, a, r, zero, 1, 1, 0.0, NULL);
+  *ani = a;
+  *ares = r;
+  for (long j = 0; j < n; j++) tdlhs[j] = (lhs[j] - d[j]) / dt / alpha;
+  delete[] d;
+  delete[] p;
+  delete[] f;
+}
+
 // ------------------------------------------------ element volume loads / sources
 // kind 0: loadel::volumeload with linhex::load_matrix (MEFEL/SRC/loadel.cpp:438,
 // MEFEL/SRC/linhex.cpp:969) and linhex::bf_matrix (:126); kind 1:

@@ -15,29 +15,6 @@ namespace ssf {
 static thread_local std::string g_err;
 void set_error(const std::string &msg) { g_err = msg; }
 
-static int host_threads()
-{
-  unsigned h = std::thread::hardware_concurrency();
-  if (h == 0) h = 4;
-  return (int)std::min(16u, h);
-}
-
-template <typename F>
-static void parallel_for(int64_t n, F f)
-{
-  const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(host_threads(), n / 4096 + 1));
-  if (nt == 1) {
-    f(0, n);
-    return;
-  }
-  std::vector<std::thread> th;
-  for (int t = 0; t < nt; t++) {
-    const int64_t b = n * t / nt, e = n * (t + 1) / nt;
-    th.emplace_back([=]() { f(b, e); });
-  }
-  for (auto &x : th) x.join();
-}
-
 // f(t) for t in [0, T), one host thread each
 template <typename F>
 static void parallel_tasks(int T, F f)
@@ -67,25 +44,6 @@ static uint64_t pattern_fingerprint(const std::vector<int64_t> &adr, const std::
   sum(adr, kSaltAdr);
   sum(ci, kSaltCi);
   return h.load();
-}
-
-// Device memory of matrices and workspaces is stream-ordered (hipMallocAsync /
-// hipFreeAsync on the owning context's stream): releasing it never waits for
-// work on other streams, e.g. a served request's solve while another request
-// is being assembled.
-template <typename T>
-static hipError_t dalloc(T **p, size_t count, hipStream_t s)
-{
-  *p = nullptr;
-  if (count == 0) count = 1;
-  return hipMallocAsync((void **)p, count * sizeof(T), s);
-}
-
-template <typename T>
-static void dfree(T *&p, hipStream_t s)
-{
-  if (p) (void)hipFreeAsync(p, s);
-  p = nullptr;
 }
 
 // ------------------------------------------------------------ patterns (host)
@@ -225,9 +183,9 @@ static bool build_tiles(int64_t n, const std::vector<int> &rowptr, const std::ve
 // SELL-64 layout of a full CSR: slices of 64 rows, each row's entries in
 // column order, padded to the slice's longest row (rounded up to SELL_V) with
 // value 0 at the row's own column; src maps a SELL entry to its CSR entry.
-static bool build_sell(int64_t n, const std::vector<int> &rowptr, const std::vector<int> &col,
-                       std::vector<int> &off, std::vector<int> &scol, std::vector<int> &src,
-                       std::vector<short> &scol16)
+bool build_sell(int64_t n, const std::vector<int> &rowptr, const std::vector<int> &col,
+                std::vector<int> &off, std::vector<int> &scol, std::vector<int> &src,
+                std::vector<short> &scol16)
 {
   const int64_t nslice = (n + 63) / 64;
   off.assign(nslice + 1, 0);
@@ -373,6 +331,7 @@ extern "C" int ssf_matrix_destroy(ssf_matrix *m)
   dfree(m->sell_lo, m->ctx->stream); dfree(m->sell_up, m->ctx->stream);
   dfree(m->d_adr32, m->ctx->stream); dfree(m->d_ci32, m->ctx->stream);
   free_workspace(m->ws, m->ctx->stream);
+  dual_free(m->dual, m->ctx->stream);
   m->ctx->live_matrices--;
   delete m;
   return SSF_OK;
@@ -1446,8 +1405,29 @@ extern "C" int ssf_solve_dev(ssf_matrix *m, const ssf_solve_params *p, const dou
   SSF_ENTER();
   SSF_CHECK(m && p && info && rhs_dev && lhs_dev, SSF_EINVAL, "ssf_solve_dev: NULL argument");
   SSF_CHECK(m->values_set, SSF_ESTATE, "ssf_solve_dev: matrix values were never set");
-  SSF_CHECK(p->linsolver == SSF_SOLVER_CG, SSF_EUNSUP, "ssf_solve_dev: only the cg solver (20) is on this path");
+  SSF_CHECK(p->linsolver == SSF_SOLVER_CG || p->linsolver == SSF_SOLVER_BICG, SSF_EUNSUP,
+            "ssf_solve_dev: the cg (20) and bicg (30) solvers are on this path");
   SSF_CHECK(p->ni >= 0, SSF_EINVAL, "ssf_solve_dev: negative ni");
+  if (p->linsolver == SSF_SOLVER_BICG) {
+    // gmatrix::solve_system, GEFEL/gmatrix.cpp:1273-1289: comprow::bicg, no
+    // preconditioner; symcomprow has no bicg in the reference
+    SSF_CHECK(m->storage == SSF_STORAGE_COMPRESSED_ROWS, SSF_EUNSUP,
+              "ssf_solve_dev: bicg needs compressed_rows (10) storage");
+    SSF_CHECK(p->precision == SSF_PRECISION_FP64, SSF_EUNSUP,
+              "ssf_solve_dev: bicg runs in FP64 only");
+    SSF_HIP(hipSetDevice(m->ctx->device));
+    int rc = ensure_ws(m, false);
+    if (rc) return rc;
+    if (m->n == 0) {
+      info->status = 1;
+      info->iterations = 0;
+      return SSF_OK;
+    }
+    info->rel_residual = 0.0;
+    rc = solve_bicg(m, p, rhs_dev, lhs_dev, info);
+    if (rc) return rc;
+    return true_residual(m, lhs_dev, rhs_dev, &info->rel_residual);
+  }
   int mode;
   bool jac;
   if (m->storage == SSF_STORAGE_SYMM_COMP_ROWS) {
@@ -1739,26 +1719,23 @@ extern "C" int ssf_matrix_add(ssf_matrix *m, double c, const ssf_matrix *other)
   return SSF_OK;
 }
 
-extern "C" int ssf_np_step_linear(ssf_matrix *K, ssf_matrix *C, ssf_matrix *A, double alpha,
-                                  double dt, const ssf_solve_params *p, const double *rhs_dev,
-                                  double *lhs_dev, double *tdlhs_dev, ssf_solve_info *info)
+// A = K * (dt alpha) + 1.0 * C, unless A already holds it for these values of K
+// and C and this dt * alpha; both time-stepping forms form it this way
+// (npsolvert.cpp:432-433 and :1158-1159)
+static int np_form_system(ssf_matrix *K, ssf_matrix *C, ssf_matrix *A, double alpha, double dt,
+                          const char *who)
 {
-  SSF_ENTER();
-  SSF_CHECK(K && C && A && p && rhs_dev && lhs_dev && tdlhs_dev, SSF_EINVAL,
-            "ssf_np_step_linear: NULL argument");
-  SSF_CHECK(K != A && C != A, SSF_EINVAL, "ssf_np_step_linear: A must be a third matrix");
+  const std::string w(who);
+  SSF_CHECK(K != A && C != A, SSF_EINVAL, w + ": A must be a third matrix");
   SSF_CHECK(K->a_ref && C->a_ref && A->a_ref && K->values_set && C->values_set, SSF_ESTATE,
-            "ssf_np_step_linear: K and C need their values");
+            w + ": K and C need their values");
   SSF_CHECK(same_pattern(K, C) && same_pattern(K, A), SSF_EINVAL,
-            "ssf_np_step_linear: K, C and A differ in context, storage or pattern");
+            w + ": K, C and A differ in context, storage or pattern");
   SSF_HIP(hipSetDevice(K->ctx->device));
-  hipStream_t s = K->ctx->stream;
-  const int64_t n = K->n;
-  // A = K * (dt alpha) + 1.0 * C (npsolvert.cpp:432-433), unless already formed
   const double sc = dt * alpha;
   if (A->np_k != K || A->np_c != C || A->np_kv != K->values_version ||
       A->np_cv != C->values_version || A->np_s != sc || !A->values_set) {
-    SSF_HIP(vals_combine(A->negm, A->a_ref, K->a_ref, sc, C->a_ref, s));
+    SSF_HIP(vals_combine(A->negm, A->a_ref, K->a_ref, sc, C->a_ref, K->ctx->stream));
     int rc = values_changed(A);
     if (rc) return rc;
     A->np_k = K;
@@ -1767,6 +1744,22 @@ extern "C" int ssf_np_step_linear(ssf_matrix *K, ssf_matrix *C, ssf_matrix *A, d
     A->np_cv = C->values_version;
     A->np_s = sc;
   }
+  return SSF_OK;
+}
+
+extern "C" int ssf_np_step_linear(ssf_matrix *K, ssf_matrix *C, ssf_matrix *A, double alpha,
+                                  double dt, const ssf_solve_params *p, const double *rhs_dev,
+                                  double *lhs_dev, double *tdlhs_dev, ssf_solve_info *info)
+{
+  SSF_ENTER();
+  SSF_CHECK(K && C && A && p && rhs_dev && lhs_dev && tdlhs_dev, SSF_EINVAL,
+            "ssf_np_step_linear: NULL argument");
+  {
+    int rc = np_form_system(K, C, A, alpha, dt, "ssf_np_step_linear");
+    if (rc) return rc;
+  }
+  hipStream_t s = K->ctx->stream;
+  const int64_t n = K->n;
   double *d = nullptr, *pv = nullptr, *f = nullptr;
   hipError_t e = hipSuccess;
   auto c = [&](hipError_t x) { if (e == hipSuccess) e = x; };
@@ -1788,6 +1781,51 @@ extern "C" int ssf_np_step_linear(ssf_matrix *K, ssf_matrix *C, ssf_matrix *A, d
   }
   // lhs = d + alpha dt v (:454-456)
   if (e == hipSuccess && rc == SSF_OK) c(axpy_rn(n, lhs_dev, d, alpha * dt, tdlhs_dev, s));
+  c(hipStreamSynchronize(s));
+  dfree(d, s);
+  dfree(pv, s);
+  dfree(f, s);
+  if (rc) return rc;
+  SSF_HIP(e);
+  return SSF_OK;
+}
+
+extern "C" int ssf_np_step_dform(ssf_matrix *K, ssf_matrix *C, ssf_matrix *A, double alpha,
+                                 double dt, const ssf_solve_params *p, const double *rhs_dev,
+                                 double *lhs_dev, double *tdlhs_dev, ssf_solve_info *info)
+{
+  SSF_ENTER();
+  SSF_CHECK(K && C && A && p && rhs_dev && lhs_dev && tdlhs_dev, SSF_EINVAL,
+            "ssf_np_step_dform: NULL argument");
+  SSF_CHECK(alpha != 0.0 && dt != 0.0, SSF_EINVAL,
+            "ssf_np_step_dform: alpha and dt must be nonzero (the rate divides by both)");
+  {
+    int rc = np_form_system(K, C, A, alpha, dt, "ssf_np_step_dform");
+    if (rc) return rc;
+  }
+  hipStream_t s = K->ctx->stream;
+  const int64_t n = K->n;
+  double *d = nullptr, *pv = nullptr, *f = nullptr;
+  hipError_t e = hipSuccess;
+  auto c = [&](hipError_t x) { if (e == hipSuccess) e = x; };
+  c(dalloc(&d, n, s));
+  c(dalloc(&pv, n, s));
+  c(dalloc(&f, n, s));
+  // d = lhs + (1 - alpha) dt v (npsolvert.cpp:1152-1154)
+  if (e == hipSuccess) c(axpy_rn(n, d, lhs_dev, (1.0 - alpha) * dt, tdlhs_dev, s));
+  int rc = SSF_OK;
+  // p = C d (:1156)
+  if (e == hipSuccess) rc = ssf_gmxv_dev(C, d, pv);
+  // f = rhs alpha dt + p (:1166-1168)
+  if (e == hipSuccess && rc == SSF_OK) c(vec_sc2_add(n, f, rhs_dev, alpha, dt, pv, s));
+  // A lhs = f from the current lhs (:1181)
+  if (e == hipSuccess && rc == SSF_OK) {
+    ssf_solve_params q = *p;
+    q.iv = 1;
+    rc = ssf_solve_dev(A, &q, f, lhs_dev, info);
+  }
+  // v = (lhs - d) / dt / alpha (:1183-1185)
+  if (e == hipSuccess && rc == SSF_OK) c(vec_rate(n, tdlhs_dev, lhs_dev, d, dt, alpha, s));
   c(hipStreamSynchronize(s));
   dfree(d, s);
   dfree(pv, s);

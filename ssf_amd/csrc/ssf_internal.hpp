@@ -3,9 +3,11 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cstdint>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "ssf_gpu.h"
@@ -13,6 +15,49 @@
 namespace ssf {
 
 void set_error(const std::string &msg);
+
+inline int host_threads()
+{
+  unsigned h = std::thread::hardware_concurrency();
+  if (h == 0) h = 4;
+  return (int)std::min(16u, h);
+}
+
+// f(b, e) over contiguous ranges of [0, n) on up to 16 host threads
+template <typename F>
+void parallel_for(int64_t n, F f)
+{
+  const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(host_threads(), n / 4096 + 1));
+  if (nt == 1) {
+    f(0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; t++) {
+    const int64_t b = n * t / nt, e = n * (t + 1) / nt;
+    th.emplace_back([=]() { f(b, e); });
+  }
+  for (auto &x : th) x.join();
+}
+
+// Device memory of matrices and workspaces is stream-ordered (hipMallocAsync /
+// hipFreeAsync on the owning context's stream): releasing it never waits for
+// work on other streams, e.g. a served request's solve while another request
+// is being assembled.
+template <typename T>
+hipError_t dalloc(T **p, size_t count, hipStream_t s)
+{
+  *p = nullptr;
+  if (count == 0) count = 1;
+  return hipMallocAsync((void **)p, count * sizeof(T), s);
+}
+
+template <typename T>
+void dfree(T *&p, hipStream_t s)
+{
+  if (p) (void)hipFreeAsync(p, s);
+  p = nullptr;
+}
 
 #define SSF_HIP(call)                                                              \
   do {                                                                             \
@@ -118,6 +163,8 @@ struct Workspace {  // vectors and reduction scratch of one solve shape
   MixState *mix_host = nullptr;         // pinned, 2 entries
 };
 
+struct DualSell;  // union layout of A and A^T of a comprow matrix (ssf_bicg.hip)
+
 }  // namespace ssf
 
 struct ssf_ctx {
@@ -183,6 +230,9 @@ struct ssf_matrix {
   // combine two matrices' values; counts every change of the values
   uint64_t pattern_fp = 0;
   uint64_t values_version = 0;
+  // biconjugate gradients: union layout of A and A^T with both values (built at
+  // the first bicg solve)
+  ssf::DualSell *dual = nullptr;
   // as the system matrix of a nonstationary step: the inputs it was formed from
   const ssf_matrix *np_k = nullptr, *np_c = nullptr;
   uint64_t np_kv = 0, np_cv = 0;
@@ -249,4 +299,15 @@ void matrix_drop_assembly(ssf_matrix *m, bool host_pattern);
 // keeps only what the SELL-64 solve reads (slice offsets, columns of the
 // narrowest width, SELL values, diagonal): the CSR and the FP32 copies go
 void matrix_keep_sell_only(ssf_matrix *m);
+// SELL-64 layout of a full CSR (rows of 64-row slices padded to the slice's
+// longest row, rounded up to SELL_V; padding: the row's own column, value 0);
+// src maps a SELL entry to its CSR entry or -1. false beyond 2^31 entries.
+bool build_sell(int64_t n, const std::vector<int> &rowptr, const std::vector<int> &col,
+                std::vector<int> &off, std::vector<int> &scol, std::vector<int> &src,
+                std::vector<short> &scol16);
+// comprow::bicg (GEFEL/cr.cpp:1353) of a compressed_rows matrix on device vectors;
+// fills info except rel_residual
+int solve_bicg(ssf_matrix *m, const ssf_solve_params *p, const double *y, double *x,
+               ssf_solve_info *info);
+void dual_free(DualSell *L, hipStream_t s);
 }  // namespace ssf

@@ -548,6 +548,24 @@ __global__ void k_sub(int64_t n, double *__restrict__ f, const double *__restric
   if (i < n) f[i] = a[i] - b[i];
 }
 
+// f = rhs alpha dt + p, the products left to right and unfused
+__global__ void k_sc2_add(int64_t n, double *__restrict__ f, const double *__restrict__ rhs,
+                          double alpha, double dt, const double *__restrict__ p)
+{
+#pragma clang fp contract(off)
+  const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (i < n) f[i] = rhs[i] * alpha * dt + p[i];
+}
+
+// v = (x - d) / dt / alpha, two correctly rounded divisions
+__global__ void k_rate(int64_t n, double *__restrict__ v, const double *__restrict__ x,
+                       const double *__restrict__ d, double dt, double alpha)
+{
+#pragma clang fp contract(off)
+  const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (i < n) v[i] = (x[i] - d[i]) / dt / alpha;
+}
+
 static unsigned grid_for(int64_t n) { return (unsigned)((std::max<int64_t>(n, 1) + TPB - 1) / TPB); }
 
 hipError_t vals_scale(int64_t n, double *a, double c, hipStream_t s)
@@ -574,6 +592,18 @@ hipError_t axpy_rn(int64_t n, double *y, const double *x, double c, const double
 hipError_t vec_sub(int64_t n, double *f, const double *a, const double *b, hipStream_t s)
 {
   hipLaunchKernelGGL(k_sub, dim3(grid_for(n)), dim3(TPB), 0, s, n, f, a, b);
+  return hipGetLastError();
+}
+hipError_t vec_sc2_add(int64_t n, double *f, const double *rhs, double alpha, double dt,
+                       const double *p, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_sc2_add, dim3(grid_for(n)), dim3(TPB), 0, s, n, f, rhs, alpha, dt, p);
+  return hipGetLastError();
+}
+hipError_t vec_rate(int64_t n, double *v, const double *x, const double *d, double dt,
+                    double alpha, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_rate, dim3(grid_for(n)), dim3(TPB), 0, s, n, v, x, d, dt, alpha);
   return hipGetLastError();
 }
 

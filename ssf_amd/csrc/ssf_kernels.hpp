@@ -86,6 +86,10 @@ hipError_t vals_combine(int64_t n, double *out, const double *k, double sc, cons
 hipError_t axpy_rn(int64_t n, double *y, const double *x, double c, const double *v,
                    hipStream_t s);                                                 // y = x + c v
 hipError_t vec_sub(int64_t n, double *f, const double *a, const double *b, hipStream_t s);
+hipError_t vec_sc2_add(int64_t n, double *f, const double *rhs, double alpha, double dt,
+                       const double *p, hipStream_t s);                           // f = rhs alpha dt + p
+hipError_t vec_rate(int64_t n, double *v, const double *x, const double *d, double dt,
+                    double alpha, hipStream_t s);                                 // v = (x - d) / dt / alpha
 
 hipError_t assemble_elements(int kind, int64_t p0, int64_t p1, int64_t ne_total,
                              const int *order, const int *conn, const int *cn,

@@ -556,21 +556,6 @@ __global__ void k_post_fill_int(int64_t m, int v, int *__restrict__ a)
   if (k < m) a[k] = v;
 }
 
-template <typename T>
-hipError_t dalloc(T **p, size_t count, hipStream_t s)
-{
-  *p = nullptr;
-  if (count == 0) count = 1;
-  return hipMallocAsync((void **)p, count * sizeof(T), s);
-}
-
-template <typename T>
-void dfree(T *&p, hipStream_t s)
-{
-  if (p) (void)hipFreeAsync(p, s);
-  p = nullptr;
-}
-
 inline unsigned blocks_for(int64_t m, int64_t per) { return (unsigned)((m + per - 1) / per); }
 
 }  // namespace

@@ -213,8 +213,21 @@ def main():
         R.ref_np_step_linear(hK, hC, hA, prob.n, 0.5, 0.01, _ptr(f64(y).copy()), _ptr(T), _ptr(v),
                              5000, 1e-20, 1e-30, ctypes.byref(ani), ctypes.byref(ares))
         traj.append((T.copy(), v.copy(), ani.value, ares.value))
+    # the d-form step (linear_nonstat_solv_dform) on the same matrices
+    Td, vd = np.zeros(prob.n), np.zeros(prob.n)
+    dtraj = []
+    for alpha, dt in ((0.5, 0.01), (0.5, 0.01), (1.0, 0.02)):
+        ani = ctypes.c_long(-7)
+        ares = ctypes.c_double(-7.0)
+        R.ref_np_step_dform(hK, hC, hA, prob.n, alpha, dt, _ptr(f64(y).copy()), _ptr(Td),
+                            _ptr(vd), 5000, 1e-20, 1e-30, ctypes.byref(ani), ctypes.byref(ares))
+        dtraj.append((Td.copy(), vd.copy(), ani.value, ares.value))
     for h in (hK, hC, hA):
         free(h)
+    out["heat_npd_T"] = np.array([t[0] for t in dtraj])
+    out["heat_npd_v"] = np.array([t[1] for t in dtraj])
+    out["heat_npd_ani"] = np.array([t[2] for t in dtraj])
+    out["heat_npd_ares"] = np.array([t[3] for t in dtraj])
     out["heat_np_T"] = np.array([t[0] for t in traj])
     out["heat_np_v"] = np.array([t[1] for t in traj])
     out["heat_np_ani"] = np.array([t[2] for t in traj])

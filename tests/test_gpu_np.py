@@ -1,6 +1,6 @@
 """GPU nonstationary heat (capacity assembly, gmatrix::scalgm / addgm on the
-device values, one_step_linear on device vectors) against the oracle, through
-the C ABI."""
+device values, one_step_linear and the d-form step of linear_nonstat_solv_dform
+on device vectors) against the oracle, through the C ABI."""
 import numpy as np
 import pytest
 
@@ -88,6 +88,78 @@ def test_np_steps_vs_oracle(ctx, alpha, dt):
         assert np.linalg.norm(vg - v) <= 1e-9 * np.linalg.norm(v), step
         if step == 0:
             assert np.array_equal(A.get_values(), aA)  # scalmat + addmat, bit for bit
+    for m in (K, C, A):
+        m.close()
+
+
+@pytest.mark.parametrize("alpha,dt", [(0.5, 0.01), (1.0, 0.05), (0.75, 0.02)])
+def test_np_dform_steps_vs_oracle(ctx, alpha, dt):
+    torch = pytest.importorskip("torch")
+    prob = mesh.heat_cube(9, seed=46)
+    rng = np.random.default_rng(46)
+    c, rho = rng.uniform(0.5, 2.0, prob.ne), rng.uniform(0.5, 2.0, prob.ne)
+    K, C, A, adr, ci, aK, aC, rhs = _mats(ctx, prob, c, rho, from_oracle=True)
+    T, v = np.zeros(prob.n), np.zeros(prob.n)
+    Td, vd, fd = _dev(T), _dev(v), _dev(rhs[0])
+    torch.cuda.synchronize()
+    for step in range(4):
+        st, ani, _, aA = O.np_step_linear(adr, ci, aK, aC, alpha, dt, rhs[0], T, v, 20000, 1e-22,
+                                          form="d")
+        info = gpu.np_step_linear(K, C, A, alpha, dt, fd.data_ptr(), Td.data_ptr(), vd.data_ptr(),
+                                  ni=20000, res=1e-22, form="d")
+        assert info.status == st == 1
+        assert abs(info.ani - ani) <= 2
+        Tg, vg = Td.cpu().numpy(), vd.cpu().numpy()
+        assert np.linalg.norm(Tg - T) <= 1e-10 * np.linalg.norm(T), step
+        assert np.linalg.norm(vg - v) <= 1e-8 * np.linalg.norm(v), step
+        if step == 0:
+            assert np.array_equal(A.get_values(), aA)
+    for m in (K, C, A):
+        m.close()
+
+
+def test_np_dform_one_unknown_and_errors(ctx):
+    torch = pytest.importorskip("torch")
+    adr, ci = np.array([0, 1]), np.array([0])
+    k, m, alpha, dt, T0, N = 3.0, 2.0, 0.75, 0.4, 5.0, 12
+    mats = [gpu.Matrix.from_csr(ctx, gpu.STORAGE_SCR, adr, ci) for _ in range(3)]
+    mats[0].set_values(np.array([k]))
+    mats[1].set_values(np.array([m]))
+    Td, vd, fd = _dev([T0]), _dev([-k / m * T0]), _dev([0.0])
+    torch.cuda.synchronize()
+    for _ in range(N):
+        gpu.np_step_linear(mats[0], mats[1], mats[2], alpha, dt, fd.data_ptr(), Td.data_ptr(),
+                           vd.data_ptr(), ni=10, res=1e-30, form="d")
+    lam = k / m
+    r = (1 - (1 - alpha) * dt * lam) / (1 + alpha * dt * lam)
+    assert abs(Td.cpu().numpy()[0] - T0 * r ** N) <= 1e-13 * T0
+    assert abs(vd.cpu().numpy()[0] + lam * Td.cpu().numpy()[0]) <= 1e-12 * T0
+    with pytest.raises(gpu.SsfError):
+        gpu.np_step_linear(mats[0], mats[1], mats[2], 0.0, dt, fd.data_ptr(), Td.data_ptr(),
+                           vd.data_ptr(), form="d")
+    with pytest.raises(gpu.SsfError):  # A must be a third matrix
+        gpu.np_step_linear(mats[0], mats[1], mats[0], alpha, dt, fd.data_ptr(), Td.data_ptr(),
+                           vd.data_ptr(), form="d")
+    for mm in mats:
+        mm.close()
+
+
+def test_np_forms_agree_mixed_precision(ctx):
+    torch = pytest.importorskip("torch")
+    prob = mesh.heat_cube(12, seed=47)
+    c = np.full(prob.ne, 1.0)
+    K, C, A, adr, ci, aK, aC, rhs = _mats(ctx, prob, c, None, from_oracle=False)
+    Tv, vv, fd = _dev(np.zeros(prob.n)), _dev(np.zeros(prob.n)), _dev(rhs[0])
+    Td, vd = _dev(np.zeros(prob.n)), _dev(np.zeros(prob.n))
+    torch.cuda.synchronize()
+    for _ in range(3):
+        gpu.np_step_linear(K, C, A, 0.5, 0.02, fd.data_ptr(), Tv.data_ptr(), vv.data_ptr(),
+                           res=1e-24)
+        info = gpu.np_step_linear(K, C, A, 0.5, 0.02, fd.data_ptr(), Td.data_ptr(), vd.data_ptr(),
+                                  res=1e-22, precision=gpu.MIXED, form="d")
+        assert info.status == 1
+    a, b = Tv.cpu().numpy(), Td.cpu().numpy()
+    assert np.linalg.norm(a - b) <= 1e-8 * np.linalg.norm(a)
     for m in (K, C, A):
         m.close()
 

@@ -1,16 +1,19 @@
-"""CPU checks that pin the oracle's nonstationary heat step (cr_hex8_heat_cap,
-cr_np_step_linear) before it is trusted:
+"""CPU checks that pin the oracle's nonstationary heat steps (cr_hex8_heat_cap,
+cr_np_step_linear for the v-form, cr_np_step_dform for the d-form) before they
+are trusted:
 
 * bitwise against the reference: the capacity matrices of the harness (GEFEL's
   jac_3d, bf_lin_hex_3d, scprd, nnj in the loop order of
-  linhext::capacity_matrix) and three steps of one_step_linear run on the
-  reference's own symcomprow objects (mxv_scr, scalmat_scr, addmat_scr, cg_prec),
-  from the committed fixture and, where oracle/_ref was built, live;
+  linhext::capacity_matrix) and three steps of one_step_linear and of the
+  linear_nonstat_solv_dform loop run on the reference's own symcomprow objects
+  (mxv_scr, scalmat_scr, addmat_scr, cg_prec), from the committed fixture and,
+  where oracle/_ref was built, live;
 * mathematics: the exact unit-cube mass matrix (2x2x2 points integrate N_i N_j
   exactly), sum of entries = rho c V on a distorted element, the amplification
   factor (1 - (1-alpha) dt lambda) / (1 + alpha dt lambda) of the generalized
-  trapezoidal rule on one unknown, and the stationary solution reached by
-  backward Euler with a long step.
+  trapezoidal rule on one unknown (both forms), the two forms agreeing to the
+  solver tolerance, and the stationary solution reached by backward Euler with a
+  long step.
 """
 import ctypes
 import os
@@ -60,6 +63,21 @@ def test_np_steps_bitwise_fixture():
         assert ani == g["heat_np_ani"][step] and ares == g["heat_np_ares"][step]
 
 
+def test_np_dform_steps_bitwise_fixture():
+    g = np.load(GOLD)
+    prob = _heat_case()
+    adr, ci, aK, rhs = O.assemble_problem(prob, "scr")
+    cm = O.capacity_matrices(prob, g["heat_np_c"], g["heat_np_rho"])
+    aC = O.scr_assemble(prob.n, adr, ci, prob.elem_codes(), cm)
+    T, v = np.zeros(prob.n), np.zeros(prob.n)
+    for step, (alpha, dt) in enumerate(((0.5, 0.01), (0.5, 0.01), (1.0, 0.02))):
+        st, ani, ares, _ = O.np_step_linear(adr, ci, aK, aC, alpha, dt, rhs[0], T, v, 5000,
+                                            1e-20, form="d")
+        assert st == 1
+        assert np.array_equal(T, g["heat_npd_T"][step]) and np.array_equal(v, g["heat_npd_v"][step])
+        assert ani == g["heat_npd_ani"][step] and ares == g["heat_npd_ares"][step]
+
+
 def test_np_live_reference():
     if not O.ref_available():
         pytest.skip("oracle/_ref not built (needs the reference sources)")
@@ -93,6 +111,17 @@ def test_np_live_reference():
                              ctypes.byref(r2))
         assert np.array_equal(T, T2) and np.array_equal(v, v2)
         assert ani == a2.value and ares == r2.value
+    T, v = np.zeros(prob.n), np.zeros(prob.n)
+    T2, v2 = T.copy(), v.copy()
+    for alpha, dt in ((1.0, 0.05), (0.5, 0.02), (0.75, 0.1)):
+        st, ani, ares, _ = O.np_step_linear(adr, ci, aK, aC, alpha, dt, rhs[0], T, v, 5000, 1e-22,
+                                            form="d")
+        a2, r2 = ctypes.c_long(-1), ctypes.c_double(-1.0)
+        R.ref_np_step_dform(hs[0], hs[1], hs[2], prob.n, alpha, dt, _ptr(f64(rhs[0]).copy()),
+                            _ptr(T2), _ptr(v2), 5000, 1e-22, 1e-30, ctypes.byref(a2),
+                            ctypes.byref(r2))
+        assert np.array_equal(T, T2) and np.array_equal(v, v2)
+        assert ani == a2.value and ares == r2.value
     for h in hs:
         R.ref_scr_free(h)
 
@@ -121,22 +150,52 @@ def test_capacity_unit_cube_closed_form():
     assert np.allclose(M, M.T, rtol=0, atol=1e-16) and np.linalg.eigvalsh(M).min() > 0
 
 
-def _one_unknown(k, m, alpha, dt, T0, steps):
+def _one_unknown(k, m, alpha, dt, T0, steps, form="v", src=0.0):
     adr = i64([0, 1])
     ci = i64([0])
     T = np.array([T0])
-    v = np.array([-k / m * T0])  # consistent initial rate: m v = -k T
+    v = np.array([(src - k * T0) / m])  # consistent initial rate: m v = f - k T
     for _ in range(steps):
-        O.np_step_linear(adr, ci, f64([k]), f64([m]), alpha, dt, f64([0.0]), T, v, 10, 1e-30)
-    return T[0]
+        O.np_step_linear(adr, ci, f64([k]), f64([m]), alpha, dt, f64([src]), T, v, 10, 1e-30,
+                         form=form)
+    return T[0], v[0]
 
 
+@pytest.mark.parametrize("form", ["v", "d"])
 @pytest.mark.parametrize("alpha", [0.5, 0.75, 1.0])
-def test_np_amplification_factor(alpha):
+def test_np_amplification_factor(alpha, form):
     k, m, dt, T0, N = 3.0, 2.0, 0.4, 5.0, 12
     lam = k / m
     r = (1 - (1 - alpha) * dt * lam) / (1 + alpha * dt * lam)
-    assert abs(_one_unknown(k, m, alpha, dt, T0, N) - T0 * r ** N) <= 1e-13 * T0
+    T, v = _one_unknown(k, m, alpha, dt, T0, N, form)
+    assert abs(T - T0 * r ** N) <= 1e-13 * T0
+    assert abs(v + lam * T) <= 1e-12 * T0  # the rate stays consistent
+
+
+@pytest.mark.parametrize("alpha", [0.5, 1.0])
+def test_np_forms_agree_with_a_source(alpha):
+    """With a constant source f the exact discrete solution relaxes to f / k by
+    the same factor in both forms: T_N - f/k = (T_0 - f/k) r^N."""
+    k, m, dt, T0, N, src = 3.0, 2.0, 0.4, 5.0, 9, 4.5
+    lam = k / m
+    r = (1 - (1 - alpha) * dt * lam) / (1 + alpha * dt * lam)
+    for form in ("v", "d"):
+        T, _ = _one_unknown(k, m, alpha, dt, T0, N, form, src)
+        assert abs(T - (src / k + (T0 - src / k) * r ** N)) <= 1e-13 * T0
+
+
+def test_np_forms_agree_on_a_mesh():
+    prob = _heat_case()
+    adr, ci, aK, rhs = O.assemble_problem(prob, "scr")
+    aC = O.scr_assemble(prob.n, adr, ci, prob.elem_codes(),
+                        O.capacity_matrices(prob, np.full(prob.ne, 1.3)))
+    Tv, vv = np.zeros(prob.n), np.zeros(prob.n)
+    Td, vd = np.zeros(prob.n), np.zeros(prob.n)
+    for alpha, dt in ((0.5, 0.01), (0.5, 0.01), (1.0, 0.05), (0.75, 0.02)):
+        O.np_step_linear(adr, ci, aK, aC, alpha, dt, rhs[0], Tv, vv, 5000, 1e-28)
+        O.np_step_linear(adr, ci, aK, aC, alpha, dt, rhs[0], Td, vd, 5000, 1e-28, form="d")
+        assert np.abs(Td - Tv).max() <= 1e-11 * np.abs(Tv).max()
+        assert np.abs(vd - vv).max() <= 1e-9 * np.abs(vv).max()
 
 
 def test_np_backward_euler_reaches_the_stationary_solution():
@@ -151,13 +210,35 @@ def test_np_backward_euler_reaches_the_stationary_solution():
     assert np.abs(T - xs).max() <= 1e-7 * np.abs(xs).max()
 
 
+def test_np_dform_backward_euler_reaches_the_stationary_solution():
+    prob = _heat_case()
+    adr, ci, aK, rhs = O.assemble_problem(prob, "scr")
+    aC = O.scr_assemble(prob.n, adr, ci, prob.elem_codes(),
+                        O.capacity_matrices(prob, np.ones(prob.ne)))
+    xs, _, _, _ = O.scr_cg_prec(adr, ci, aK, rhs[0], 5000, 1e-26)
+    T, v = np.zeros(prob.n), np.zeros(prob.n)
+    O.np_step_linear(adr, ci, aK, aC, 1.0, 1e9, rhs[0], T, v, 5000, 1e-26, form="d")
+    assert np.abs(T - xs).max() <= 1e-7 * np.abs(xs).max()
+
+
 def test_np_checks_catch_planted_mistakes():
     k, m, dt, T0, N, alpha = 3.0, 2.0, 0.4, 5.0, 12, 0.75
     lam = k / m
-    good = _one_unknown(k, m, alpha, dt, T0, N)
+    good, _ = _one_unknown(k, m, alpha, dt, T0, N)
     # predictor with alpha and 1 - alpha swapped
     r_swapped = (1 - alpha * dt * lam) / (1 + alpha * dt * lam)
     assert abs(good - T0 * r_swapped ** N) > 1e-3 * T0
     # capacity without the density
     M1 = _cap(HEX, 2.0, np.full(8, 3.0))
     assert abs(M1.sum() - 2.0 * 3.0) <= 1e-14 and abs(_cap(HEX, 2.0).sum() - 6.0) > 1.0
+    # d-form right-hand side without its alpha dt factor: the source enters with
+    # weight 1 / (alpha dt) too much and the relaxation limit moves off f / k
+    src, T, N = 4.5, 5.0, 40
+    d, v = T, (src - k * T) / m
+    for _ in range(N):
+        dd = d + (1 - alpha) * dt * v
+        T = (src + m * dd) / (k * dt * alpha + m)
+        v = (T - dd) / dt / alpha
+        d = T
+    assert abs(T - src / k) > 1e-2
+    assert abs(_one_unknown(k, m, alpha, dt, 5.0, N, "d", src)[0] - src / k) <= 1e-6
