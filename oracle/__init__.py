@@ -43,6 +43,8 @@ def lib():
             "cr_cr_assemble": (None, [_L, _P, _P, _P, _L, _L, _P, _P]),
             "cr_mxv_scr": (None, [_L, _P, _P, _P, _P, _P]),
             "cr_mxv_cr": (None, [_L, _P, _P, _P, _P, _P]),
+            "cr_mtxv_cr": (None, [_L, _P, _P, _P, _P, _P]),
+            "cr_cr_bicg": (ctypes.c_int, [_L, _P, _P, _P, _P, _P, _L, _D, _P, _P, _D, _L]),
             "cr_scr_cg_prec": (ctypes.c_int, [_L, _P, _P, _P, _P, _P, _L, _D, _P, _P, _D, _L]),
             "cr_scr_cg": (ctypes.c_int, [_L, _P, _P, _P, _P, _P, _L, _D, _P, _P, _D, _L]),
             "cr_cr_cg_prec_noprec": (ctypes.c_int, [_L, _P, _P, _P, _P, _P, _L, _D, _P, _P, _D, _L]),
@@ -101,6 +103,9 @@ def ref():
             "ref_cr_get": (None, [_P, _P, _P, _P]),
             "ref_cr_localized": (None, [_P, _P, _P, _L]),
             "ref_cr_mxv": (None, [_P, _P, _P]),
+            "ref_cr_mtxv": (None, [_P, _P, _P]),
+            "ref_cr_set_a": (None, [_P, _P]),
+            "ref_cr_bicg": (None, [_P, _P, _P, _L, _D, _P, _P, _D, _L]),
             "ref_cr_cg_prec_noprec": (None, [_P, _P, _P, _L, _D, _P, _P, _D, _L]),
             "ref_cr_free": (None, [_P]),
             "ref_hex8_elast_ke": (None, [_P, _P, _P, _D, _D, _P]),
@@ -242,6 +247,37 @@ def cr_cg_prec_noprec(adr, ci, a, y, ni, res, zero=1e-30):
                                     _ptr(f64(y)), ni, res, ctypes.byref(ani),
                                     ctypes.byref(ares), zero, 0)
     return x, ani.value, ares.value, st
+
+
+def cr_bicg(adr, ci, a, y, ni, res, zero=1e-30, x0=None):
+    """comprow::bicg; x0 given: start from it (iv = 1). Returns (x, ani, ares, status)."""
+    n = adr.shape[0] - 1
+    x = np.zeros(n) if x0 is None else f64(x0).copy()
+    ani = ctypes.c_long(-1)
+    ares = ctypes.c_double(-1.0)
+    st = lib().cr_cr_bicg(n, _ptr(adr), _ptr(ci), _ptr(f64(a)), _ptr(x), _ptr(f64(y)), ni, res,
+                          ctypes.byref(ani), ctypes.byref(ares), zero,
+                          0 if x0 is None else 1)
+    return x, ani.value, ares.value, st
+
+
+def mtxv_cr(adr, ci, a, b):
+    n = adr.shape[0] - 1
+    c = np.zeros(n)
+    lib().cr_mtxv_cr(n, _ptr(adr), _ptr(ci), _ptr(f64(a)), _ptr(f64(b)), _ptr(c))
+    return c
+
+
+def nonsymmetric_values(adr, ci, a, gamma, seed):
+    """Values of a nonsymmetric system on a comprow pattern: every off-diagonal
+    a_ij scaled by 1 + gamma t_ij with t_ij uniform in [-1, 1] drawn per entry,
+    so a_ij and a_ji are scaled independently (test data, not a reference
+    routine)."""
+    rng = np.random.default_rng(seed)
+    n = adr.shape[0] - 1
+    rows = np.repeat(np.arange(n), np.diff(adr))
+    t = rng.uniform(-1.0, 1.0, a.shape[0])
+    return np.where(ci == rows, a, a * (1.0 + gamma * t))
 
 
 def assemble_problem(prob, storage="scr"):

@@ -242,6 +242,17 @@ void cr_mxv_cr(long n, const long *adr, const long *ci, const double *a,
   }
 }
 
+/* GEFEL/cr.cpp:843-858: c = A^T b, row by row, c[ci[j]] += a[j] b[i] */
+void cr_mtxv_cr(long n, const long *adr, const long *ci, const double *a,
+                const double *b, double *c)
+{
+  for (long i = 0; i < n; i++) c[i] = 0.0;
+  for (long i = 0; i < n; i++) {
+    double s = b[i];
+    for (long j = adr[i]; j < adr[i + 1]; j++) c[ci[j]] += a[j] * s;
+  }
+}
+
 /* GEFEL/vector.cpp:2052 ss */
 static double ss(const double *a, const double *b, long n)
 {
@@ -1154,6 +1165,77 @@ void cr_hex8_heat_cap_mesh(long ne, const long *conn, const double *xyz, const d
     }
     cr_hex8_heat_cap(x, y, z, c[el], d, cm + el * 64);
   }
+}
+
+/* GEFEL/cr.cpp:1353-1419 comprow::bicg: biconjugate gradients, no
+ * preconditioner. Returns 1 when the test of :1400 stopped the loop, 3 for a
+ * right-hand side with y.y < zero (x untouched, ares = y.y, ani = 0), 0 when ni
+ * iterations ran (ani = ni, ares = the last r.r). Like the reference it does not
+ * guard the divisions. */
+int cr_cr_bicg(long n, const long *adr, const long *ci, const double *a, double *x,
+               const double *y, long ni, double res, long *ani, double *ares, double zero,
+               long iv)
+{
+  long i, j;
+  double normy, normr = 0.0, normrr, nom, denom, alpha, beta;
+  int status = 0;
+  double *r = (double *)malloc((size_t)n * sizeof(double));
+  double *rr = (double *)malloc((size_t)n * sizeof(double));
+  double *d = (double *)malloc((size_t)n * sizeof(double));
+  double *dd = (double *)malloc((size_t)n * sizeof(double));
+  double *p = (double *)malloc((size_t)n * sizeof(double));
+  double *pp = (double *)malloc((size_t)n * sizeof(double));
+  normy = ss(y, y, n);
+  if (normy < zero) {
+    *ares = normy;
+    *ani = 0;
+    status = 3;
+    goto out;
+  }
+  if (iv == 0)
+    for (i = 0; i < n; i++) x[i] = 0.0;
+  cr_mxv_cr(n, adr, ci, a, x, p);
+  for (i = 0; i < n; i++) {
+    r[i] = y[i] - p[i];
+    d[i] = r[i];
+    rr[i] = r[i];
+    dd[i] = rr[i];
+  }
+  nom = ss(r, rr, n);
+  for (i = 0; i < ni; i++) {
+    cr_mxv_cr(n, adr, ci, a, d, p);
+    cr_mtxv_cr(n, adr, ci, a, dd, pp);
+    denom = ss(dd, p, n);
+    alpha = nom / denom;
+    for (j = 0; j < n; j++) {
+      x[j] += alpha * d[j];
+      r[j] -= alpha * p[j];
+      rr[j] -= alpha * pp[j];
+    }
+    normr = ss(r, r, n);
+    normrr = ss(rr, rr, n);
+    if (normr / normy < res && normrr / normy < res) {
+      status = 1;
+      break;
+    }
+    denom = nom;
+    nom = ss(r, rr, n);
+    beta = nom / denom;
+    for (j = 0; j < n; j++) {
+      d[j] = r[j] + beta * d[j];
+      dd[j] = rr[j] + beta * dd[j];
+    }
+  }
+  *ani = i;
+  *ares = normr;
+out:
+  free(pp);
+  free(p);
+  free(dd);
+  free(d);
+  free(rr);
+  free(r);
+  return status;
 }
 
 /* TRFEL/SRC/npsolvert.cpp:367-482 one_step_linear (generalized trapezoidal rule,

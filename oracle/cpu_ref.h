@@ -33,6 +33,11 @@
  *                         assembled forces = A x - lift, D / -k closed forms
  *   cr_hex8_heat_cap      bitwise vs oracle/_ref; unit-cube closed form (exact with
  *                         2x2x2 points); sum of entries = rho c V
+ *   cr_mtxv_cr, cr_cr_bicg
+ *                         bitwise vs oracle/_ref (comprow::mtxv_cr, comprow::bicg);
+ *                         transpose identity b.(A^T c) = (A b).c; on a symmetric
+ *                         matrix BiCG reaches the CG solution; on small dense
+ *                         nonsymmetric systems the solution of a direct solve
  *   cr_np_step_linear, cr_np_step_dform
  *                         bitwise vs oracle/_ref (the reference's own symcomprow
  *                         mxv_scr / scalmat_scr / addmat_scr / cg_prec); amplification
@@ -167,6 +172,11 @@ void cr_hex8_heat_cap_mesh(long ne, const long *conn, const double *xyz, const d
                            const double *rho, double *cm);
 /* TRFEL/SRC/npsolvert.cpp:367-482 one_step_linear on symcomprow K and C (same
  * pattern); returns the cg_prec status (see the definition for the steps). */
+void cr_mtxv_cr(long n, const long *adr, const long *ci, const double *a,
+                const double *b, double *c);
+int cr_cr_bicg(long n, const long *adr, const long *ci, const double *a, double *x,
+               const double *y, long ni, double res, long *ani, double *ares, double zero,
+               long iv);
 int cr_np_step_linear(long n, const long *adr, const long *ci, const double *aK,
                       const double *aC, double *aA, double alpha, double dt,
                       const double *rhs, double *lhs, double *tdlhs, double *d, double *p,

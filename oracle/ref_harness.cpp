@@ -126,6 +126,22 @@ void ref_cr_cg_prec_noprec(void *h, double *x, double *y, long ni, double res, l
   *ani = a;
   *ares = r;
 }
+void ref_cr_set_a(void *h, const double *a)
+{
+  comprow *s = (comprow *)h;
+  memcpy(s->a, a, s->negm * sizeof(double));
+}
+void ref_cr_mtxv(void *h, double *b, double *c) { ((comprow *)h)->mtxv_cr(b, c); }
+// comprow::bicg (GEFEL/cr.cpp:1353-1419) itself
+void ref_cr_bicg(void *h, double *x, double *y, long ni, double res, long *ani, double *ares,
+                 double zero, long iv)
+{
+  long a = *ani;
+  double r = *ares;
+  ((comprow *)h)->bicg(x, y, ni, res, a, r, zero, iv);
+  *ani = a;
+  *ares = r;
+}
 void ref_cr_free(void *h) { delete (comprow *)h; }
 
 // ------------------------------------------------------------ element routines

@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(HERE, "libssf_gpu.so")
 STORAGE_CR = 10
 STORAGE_SCR = 11
 SOLVER_CG = 20
+SOLVER_BICG = 30
 PREC_NONE = 0
 PREC_DIAG = 1
 FP64 = 0
@@ -299,18 +300,21 @@ class Matrix:
         return b
 
     def solve(self, rhs, ni=100000, res=1e-20, prec=PREC_DIAG, precision=FP64, zero=1e-30,
-              x0=None, inner_res=1e-4, inner_ni=0, max_outer=0):
+              x0=None, inner_res=1e-4, inner_ni=0, max_outer=0, linsolver=SOLVER_CG):
+        """gmatrix::solve_system on host vectors: cg (symcomprow / comprow) or bicg
+        (comprow, no preconditioner)."""
         rhs = _f64(rhs)
         x = np.zeros(self.n) if x0 is None else _f64(x0).copy()
-        p = SolveParams(SOLVER_CG, prec, ni, res, zero, 0 if x0 is None else 1, precision,
+        p = SolveParams(linsolver, prec, ni, res, zero, 0 if x0 is None else 1, precision,
                         inner_res, inner_ni, max_outer)
         info = SolveInfo(-1, -1.0, 0, 0, 0, 0, 0.0)
         _check(lib().ssf_solve(self.h, ctypes.byref(p), _p(rhs), _p(x), ctypes.byref(info)))
         return x, info
 
     def solve_dev(self, rhs_ptr, x_ptr, ni=100000, res=1e-20, prec=PREC_DIAG, precision=FP64,
-                  zero=1e-30, iv=0, inner_res=1e-4, inner_ni=0, max_outer=0):
-        p = SolveParams(SOLVER_CG, prec, ni, res, zero, iv, precision, inner_res, inner_ni,
+                  zero=1e-30, iv=0, inner_res=1e-4, inner_ni=0, max_outer=0,
+                  linsolver=SOLVER_CG):
+        p = SolveParams(linsolver, prec, ni, res, zero, iv, precision, inner_res, inner_ni,
                         max_outer)
         info = SolveInfo(-1, -1.0, 0, 0, 0, 0, 0.0)
         _check(lib().ssf_solve_dev(self.h, ctypes.byref(p), ctypes.c_void_p(rhs_ptr),

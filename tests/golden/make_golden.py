@@ -20,6 +20,10 @@ Element volume loads (elast_vload_*, heat_vload_*): seeded nodal load values per
 element, their load vectors from the harness's ref_hex8_volume_load (load_matrix
 / quantity_source_vector), added by locglob in element order onto a seeded
 starting vector (standing for the nodal loads added first).
+Biconjugate gradients ({heat,elast}_bicg_*): nonsymmetric values on the comprow
+pattern (every off-diagonal entry scaled by 1 + gamma t, t seeded uniform), the
+reference's comprow::mtxv_cr of a seeded vector, comprow::bicg from x = 0 to
+res 1e-24, and 5 iterations from a seeded start vector (iv = 1).
 Post-solve element pass (post_*): for a seeded nodal solution post_x, the
 integration-point strains / stresses (gradients / fluxes for heat) and element
 forces of the harness's ref_*_ip, gathered by eldispl's code-number rule and added
@@ -233,6 +237,31 @@ def main():
     out["heat_np_ani"] = np.array([t[2] for t in traj])
     out["heat_np_ares"] = np.array([t[3] for t in traj])
     print("heat np", prob.n, "unknowns", flush=True)
+    # comprow::bicg on nonsymmetric values of the heat and elasticity patterns
+    for name, gamma in (("heat", 0.4), ("elast", 0.3)):
+        prob = CASES[name]()
+        ke = ref_ke(R, prob)
+        h, adr, ci, a, free = storage(R, prob, ke, "cr")
+        an = O.nonsymmetric_values(adr, ci, a, gamma, 4000 + list(CASES).index(name))
+        R.ref_cr_set_a(h, _ptr(f64(an)))
+        y = prob.rhs[0] + O.dirichlet_lift(prob, i64(prob.elem_codes()), ke)
+        vr = np.random.default_rng(5000 + list(CASES).index(name))
+        b = vr.standard_normal(prob.n)
+        tv = np.zeros(prob.n)
+        R.ref_cr_mtxv(h, _ptr(b.copy()), _ptr(tv))
+        x, ani, ares = solve(R.ref_cr_bicg, h, prob.n, y, 2000, 1e-24)
+        x0 = vr.standard_normal(prob.n) * np.abs(x).max()
+        x5 = x0.copy()
+        ani5, ares5 = ctypes.c_long(-7), ctypes.c_double(-7.0)
+        R.ref_cr_bicg(h, _ptr(x5), _ptr(f64(y).copy()), 5, 1e-24, ctypes.byref(ani5),
+                      ctypes.byref(ares5), 1e-30, 1)
+        free(h)
+        out[f"{name}_bicg_a"], out[f"{name}_bicg_y"] = an, f64(y)
+        out[f"{name}_bicg_b"], out[f"{name}_bicg_mtxv"] = b, tv
+        out[f"{name}_bicg_x"], out[f"{name}_bicg_ani"], out[f"{name}_bicg_ares"] = x, ani, ares
+        out[f"{name}_bicg_x0"], out[f"{name}_bicg_x5"] = x0, x5
+        out[f"{name}_bicg_ani5"], out[f"{name}_bicg_ares5"] = ani5.value, ares5.value
+        print("bicg", name, prob.n, "unknowns, ani", ani, flush=True)
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ref_vectors.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, os.path.getsize(path), "bytes")
