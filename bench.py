@@ -352,6 +352,85 @@ def bench_transient(ctx, args, gpu, mesh, torch, prob, K, rhs):
     return out
 
 
+def skew_values(adr, ci, a, gamma, seed):
+    """Nonsymmetric values on a comprow pattern: each off-diagonal a_ij scaled by
+    1 + gamma t_ij, t_ij uniform in [-1, 1] per entry (a_ij and a_ji independently)."""
+    rng = np.random.default_rng(seed)
+    rows = np.repeat(np.arange(adr.shape[0] - 1, dtype=np.int64), np.diff(adr))
+    t = rng.uniform(-1.0, 1.0, a.shape[0])
+    return np.where(ci == rows, a, a * (1.0 + gamma * t))
+
+
+def bench_bicg(ctx, args, gpu, mesh, torch, prob):
+    """comprow::bicg on the heat cube stored as comprow with nonsymmetric values
+    (off-diagonals scaled by 1 + 0.3 t, seeded), from x = 0 to --res, FP64 (the
+    reference's bicg has no preconditioner). The dual SELL product A d, A^T dd is
+    the dominant kernel; its launches are timed live (second product of every
+    iteration chunk). CPU baseline: --cpu-iters iterations of the oracle's bicg
+    on the same system, scaled to the GPU's iteration count."""
+    cn = prob.elem_codes()
+    t0 = time.time()
+    m = gpu.Matrix.from_elements(ctx, gpu.STORAGE_CR, prob.n, cn)
+    lift = m.assemble(prob)
+    rhs = prob.rhs[0] + lift
+    adr, ci = m.pattern()
+    m.set_values(skew_values(adr, ci, m.get_values(), 0.3, 11))
+    nnz = int(adr[-1])
+    log(f"[bicg] comprow matrix nnz={nnz} set up ({time.time() - t0:.1f}s)")
+    dev = torch.device("cuda:0")
+    b = torch.tensor(rhs, dtype=torch.float64, device=dev)
+    x = torch.zeros(prob.n, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+
+    def solve():
+        return m.solve_dev(b.data_ptr(), x.data_ptr(), ni=200000, res=args.res,
+                           prec=gpu.PREC_NONE, linsolver=gpu.SOLVER_BICG)
+
+    t0 = time.time()
+    info = solve()  # builds the union layout of A and A^T, gathers both values
+    torch.cuda.synchronize()
+    log(f"[bicg] first solve incl. layout {time.time() - t0:.1f}s: status {info.status} "
+        f"iterations {info.iterations}")
+    ctx.set_profiling(True)
+    t = time.perf_counter()
+    infos = [solve() for _ in range(args.bicg_steps)]
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t
+    us, samples = ctx.kernel_time(0)
+    ctx.set_profiling(False)
+    its = int(infos[-1].iterations)
+    # algorithmic bytes of one dual product: a_ij and a_ji (8 + 8) and a 16-bit
+    # column per entry of the (structurally symmetric) pattern, d and dd read
+    # once, p and pp written once
+    alg = 18 * nnz + 32 * prob.n
+    gbs = alg / (us * 1e-6) / 1e9 if us else 0.0
+    out = {"workload": f"heat hex8 {args.nel}^3 as comprow, nonsymmetric values "
+                       f"(off-diagonals x (1 + 0.3 t)), n={prob.n}, nnz={nnz}, res {args.res}",
+           "solves_per_s": args.bicg_steps / dt, "ms_per_solve": 1e3 * dt / args.bicg_steps,
+           "iterations": its, "status": int(infos[-1].status),
+           "rel_residual": float(infos[-1].rel_residual),
+           "dual_spmv": {"kernel": "k_dual_spmv<true, true> (A d and A^T dd, one SELL sweep)",
+                         "usec_per_launch": us, "samples": samples,
+                         "algorithmic_bytes_per_launch": alg, "achieved_gbs": gbs,
+                         "frac_of_hbm_peak": gbs * 1e9 / PEAK_HBM}}
+    log(f"[bicg] {out['solves_per_s']:.2f} solves/s, {its} iterations, dual spmv {us:.1f} us "
+        f"{gbs:.0f} GB/s")
+    if not args.no_cpu:
+        import oracle as O
+        a = m.get_values()
+        k = args.cpu_iters
+        t = time.perf_counter()
+        O.cr_bicg(adr, ci, a, rhs, k, 0.0)
+        per_it = (time.perf_counter() - t) / k
+        out["cpu_baseline"] = {"value": 1.0 / (per_it * its), "unit": "solves/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"{k} iterations of the oracle's comprow bicg (1 thread) "
+                                         f"on the same system, scaled to the GPU's {its}"}
+        out["speedup_vs_cpu"] = out["solves_per_s"] / out["cpu_baseline"]["value"]
+    m.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -382,6 +461,8 @@ def main():
     ap.add_argument("--no-np", action="store_true", help="skip the nonstationary heat leg")
     ap.add_argument("--np-steps", type=int, default=10)
     ap.add_argument("--np-dt", type=float, default=1e-3)
+    ap.add_argument("--no-bicg", action="store_true", help="skip the comprow bicg leg")
+    ap.add_argument("--bicg-steps", type=int, default=2)
     ap.add_argument("--serve-count", type=int, default=24)
     ap.add_argument("--serve-res", type=float, default=1e-22)
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[20.0])
@@ -519,6 +600,9 @@ def main():
     transient = None
     if not args.no_np:
         transient = bench_transient(ctx, args, gpu, mesh, torch, prob, m, rhs)
+    bicg = None
+    if not args.no_bicg:
+        bicg = bench_bicg(ctx, args, gpu, mesh, torch, prob)
 
     batched = None
     if not args.no_batch:
@@ -565,6 +649,7 @@ def main():
         "cpu_baseline": cpu,
         "post": post,
         "transient": transient,
+        "bicg": bicg,
         "batched": batched,
         "served": served,
     }

@@ -55,7 +55,7 @@ typedef struct ssf_ctx ssf_ctx;
 typedef struct ssf_matrix ssf_matrix;
 
 typedef struct ssf_solve_params {
-  int32_t linsolver;   /* SSF_SOLVER_CG                                          */
+  int32_t linsolver;   /* SSF_SOLVER_CG, SSF_SOLVER_BICG (compressed_rows)        */
   int32_t prec;        /* SSF_PREC_NOPRECOND / SSF_PREC_DIAG                      */
   int64_t ni;          /* maximum number of iterations (gmatrix::ni)              */
   double res;          /* required residual norm (gmatrix::res); squared test for
@@ -160,11 +160,21 @@ int ssf_matrix_add(ssf_matrix *m, double c, const ssf_matrix *other);
 int ssf_gmxv(ssf_matrix *m, const double *a, double *b);
 /* same on device pointers, asynchronous */
 int ssf_gmxv_dev(ssf_matrix *m, const double *a_dev, double *b_dev);
-/* gmatrix::solve_system (GEFEL/gmatrix.cpp:1049), tlinsol = cg: dispatch on the
- * storage and preconditioner as the reference does (symcomprow + diagprec ->
- * symcomprow::cg_prec GEFEL/scr.cpp:1093; symcomprow + noprecond ->
- * symcomprow::cg :786; comprow -> comprow::cg_prec GEFEL/cr.cpp:1124 with the
- * noprecond preconditioner). lhs/rhs host vectors of length n. */
+/* gmatrix::solve_system (GEFEL/gmatrix.cpp:1049), dispatched on the solver,
+ * storage and preconditioner as the reference does:
+ *   cg (20): symcomprow + diagprec -> symcomprow::cg_prec GEFEL/scr.cpp:1093;
+ *     symcomprow + noprecond -> symcomprow::cg :786; comprow ->
+ *     comprow::cg_prec GEFEL/cr.cpp:1124 with the noprecond preconditioner.
+ *   bicg (30): comprow only -> comprow::bicg GEFEL/cr.cpp:1353 (gmatrix.cpp:
+ *     1273-1289), FP64, no preconditioner (prec is ignored, as there). Stops when
+ *     r.r / y.y < res and rr.rr / y.y < res (:1400): status 1, ani = the
+ *     iteration index, ares = r.r. y.y < zero: status 3, lhs untouched, ani 0,
+ *     ares = y.y. ni spent: status 0, ani = ni, ares = the last r.r. Where the
+ *     reference would divide by a zero dd.p or r.rr and carry NaN to ni, the
+ *     device stops with status 2. The first bicg solve of a matrix lays out the
+ *     union pattern of A and A^T on the device (kept with the matrix); its values
+ *     are re-taken after every change of the matrix's values.
+ * lhs/rhs host vectors of length n. */
 int ssf_solve(ssf_matrix *m, const ssf_solve_params *p, const double *rhs, double *lhs,
               ssf_solve_info *info);
 /* same with device vectors (rhs_dev read-only, lhs_dev in/out), synchronous */

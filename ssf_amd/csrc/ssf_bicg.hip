@@ -9,7 +9,9 @@
 // accumulates c[i]; row i of A lists a_ij in the order of mxv_cr (:653-700).
 // Slots of the union that A (or A^T) lacks, and the padding of a slice, hold 0.
 //
-// Iteration i (cr.cpp:1386-1411), three launches:
+// Iteration i (cr.cpp:1386-1411), three launches; the vectors of the recurrence
+// are held in pairs (r, rr), (d, dd), (p, pp), so each row of a pair is one
+// 16-byte access and the SpMV gathers d_j and dd_j with one load:
 //   K1  p = A d, pp = A^T dd, partials dd.p               (k_dual_spmv, iter++)
 //   K2  alpha = nom / dd.p; x += alpha d, r -= alpha p, rr -= alpha pp;
 //       partials r.r, rr.rr, r.rr                          (k_bicg_update)
@@ -45,8 +47,10 @@ struct DualSell {
   double *va = nullptr, *vt = nullptr;
   uint64_t version = ~0ULL;            // values_version the values were taken at
   int nbs = 8, nbv = 8;                // blocks of K1 / of the vector kernels
-  double *r = nullptr, *rr = nullptr, *d = nullptr, *dd = nullptr, *p = nullptr,
-         *pp = nullptr, *part = nullptr;
+  // iteration vectors in pairs, one 16-byte access per row: (r, rr), (d, dd),
+  // (p, pp); the shadow quantities of BiCG sit beside their primal ones
+  double2 *rv = nullptr, *dv = nullptr, *pv = nullptr;
+  double *part = nullptr;
   BicgState *st = nullptr, *st_host = nullptr;  // st_host: 2 pinned copies
 };
 
@@ -59,12 +63,13 @@ struct DualDev {
 
 namespace {
 
-// DUAL: p = A d, pp = A^T dd and the block partial of dd.p; otherwise p = A d only
+// DUAL: (p, pp) = (A d, A^T dd) from the pairs dv = (d, dd), one 16-byte gather
+// per entry, and the block partial of dd.p; otherwise q = A x (x, q plain)
 template <bool I16, bool DUAL>
-__global__ __launch_bounds__(TPB) void k_dual_spmv(DualDev L, const double *__restrict__ d,
-                                                   const double *__restrict__ dd,
-                                                   double *__restrict__ p,
-                                                   double *__restrict__ pp,
+__global__ __launch_bounds__(TPB) void k_dual_spmv(DualDev L, const double2 *__restrict__ dv,
+                                                   double2 *__restrict__ pv,
+                                                   const double *__restrict__ x,
+                                                   double *__restrict__ q,
                                                    double *__restrict__ part, const int *done,
                                                    long long *iter)
 {
@@ -100,16 +105,22 @@ __global__ __launch_bounds__(TPB) void k_dual_spmv(DualDev L, const double *__re
       }
 #pragma unroll
       for (int u = 0; u < SELL_V; u++) {
-        sa += va[u] * d[c[u]];
-        if (DUAL) st += vt[u] * dd[c[u]];
+        if (DUAL) {
+          const double2 g = dv[c[u]];
+          sa += va[u] * g.x;
+          st += vt[u] * g.y;
+        } else {
+          sa += va[u] * x[c[u]];
+        }
       }
     }
     const int row = sl * 64 + lane;
     if (row < L.n) {
-      p[row] = sa;
       if (DUAL) {
-        pp[row] = st;
-        acc += dd[row] * sa;
+        pv[row] = make_double2(sa, st);
+        acc += dv[row].y * sa;
+      } else {
+        q[row] = sa;
       }
     }
   }
@@ -119,13 +130,12 @@ __global__ __launch_bounds__(TPB) void k_dual_spmv(DualDev L, const double *__re
   }
 }
 
-// r = y - q (q = A x already in r) or y; rr = d = dd = r; partials y.y, r.rr
+// r = y - q (q = A x) or y; rr = d = dd = r; partials y.y, r.rr
 template <bool HAVEX>
 __global__ __launch_bounds__(TPB) void k_bicg_init(int n, const double *__restrict__ y,
-                                                   double *__restrict__ r,
-                                                   double *__restrict__ rr,
-                                                   double *__restrict__ d,
-                                                   double *__restrict__ dd,
+                                                   const double *__restrict__ q,
+                                                   double2 *__restrict__ rv,
+                                                   double2 *__restrict__ dv,
                                                    double *__restrict__ part)
 {
   int r0, r1;
@@ -133,11 +143,9 @@ __global__ __launch_bounds__(TPB) void k_bicg_init(int n, const double *__restri
   double ayy = 0.0, arr = 0.0;
   for (int i = r0 + threadIdx.x; i < r1; i += TPB) {
     const double yi = y[i];
-    const double ri = HAVEX ? yi - r[i] : yi;
-    r[i] = ri;
-    rr[i] = ri;
-    d[i] = ri;
-    dd[i] = ri;
+    const double ri = HAVEX ? yi - q[i] : yi;
+    rv[i] = make_double2(ri, ri);
+    dv[i] = make_double2(ri, ri);
     ayy += yi * yi;
     arr += ri * ri;
   }
@@ -184,11 +192,9 @@ __global__ __launch_bounds__(TPB) void k_bicg_zero_x(int n, const BicgState *st,
 __global__ __launch_bounds__(TPB) void k_bicg_update(int n, BicgState *st,
                                                      const double *__restrict__ pden, int npden,
                                                      double *__restrict__ x,
-                                                     double *__restrict__ r,
-                                                     double *__restrict__ rr,
-                                                     const double *__restrict__ d,
-                                                     const double *__restrict__ p,
-                                                     const double *__restrict__ pp,
+                                                     double2 *__restrict__ rv,
+                                                     const double2 *__restrict__ dv,
+                                                     const double2 *__restrict__ pv,
                                                      double *__restrict__ prr)
 {
   if (st->done) return;
@@ -209,14 +215,16 @@ __global__ __launch_bounds__(TPB) void k_bicg_update(int n, BicgState *st,
   chunk(n, r0, r1);
   double a1 = 0.0, a2 = 0.0, a3 = 0.0;
   for (int i = r0 + threadIdx.x; i < r1; i += TPB) {
-    x[i] += alpha * d[i];
-    const double ri = r[i] - alpha * p[i];
-    const double si = rr[i] - alpha * pp[i];
-    r[i] = ri;
-    rr[i] = si;
-    a1 += ri * ri;
+    const double2 di = dv[i], pi = pv[i];
+    double2 ri = rv[i];
+    x[i] += alpha * di.x;
+    ri.x = ri.x - alpha * pi.x;
+    ri.y = ri.y - alpha * pi.y;
+    rv[i] = ri;
+    const double si = ri.y;
+    a1 += ri.x * ri.x;
     a2 += si * si;
-    a3 += ri * si;
+    a3 += ri.x * si;
   }
   const double b1 = block_sum(a1);
   const double b2 = block_sum(a2);
@@ -230,10 +238,8 @@ __global__ __launch_bounds__(TPB) void k_bicg_update(int n, BicgState *st,
 
 __global__ __launch_bounds__(TPB) void k_bicg_dir(int n, BicgState *st,
                                                   const double *__restrict__ prr, double res,
-                                                  double *__restrict__ d,
-                                                  double *__restrict__ dd,
-                                                  const double *__restrict__ r,
-                                                  const double *__restrict__ rr)
+                                                  double2 *__restrict__ dv,
+                                                  const double2 *__restrict__ rv)
 {
   if (st->done) return;
   const int nb = gridDim.x;
@@ -266,8 +272,11 @@ __global__ __launch_bounds__(TPB) void k_bicg_dir(int n, BicgState *st,
   int r0, r1;
   chunk(n, r0, r1);
   for (int i = r0 + threadIdx.x; i < r1; i += TPB) {
-    d[i] = r[i] + beta * d[i];
-    dd[i] = rr[i] + beta * dd[i];
+    const double2 ri = rv[i];
+    double2 di = dv[i];
+    di.x = ri.x + beta * di.x;
+    di.y = ri.y + beta * di.y;
+    dv[i] = di;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->nom[(it + 1) & 1] = nomn;
@@ -343,8 +352,7 @@ void dual_free(DualSell *L, hipStream_t s)
   if (!L) return;
   dfree(L->off, s); dfree(L->col, s); dfree(L->col16, s);
   dfree(L->src_a, s); dfree(L->src_t, s); dfree(L->va, s); dfree(L->vt, s);
-  dfree(L->r, s); dfree(L->rr, s); dfree(L->d, s); dfree(L->dd, s);
-  dfree(L->p, s); dfree(L->pp, s); dfree(L->part, s); dfree(L->st, s);
+  dfree(L->rv, s); dfree(L->dv, s); dfree(L->pv, s); dfree(L->part, s); dfree(L->st, s);
   if (L->st_host) (void)hipHostFree(L->st_host);
   delete L;
 }
@@ -395,7 +403,7 @@ static int dual_prepare(ssf_matrix *m)
     up(&L->src_t, st);
     c(dalloc(&L->va, tot, s));
     c(dalloc(&L->vt, tot, s));
-    for (double **v : {&L->r, &L->rr, &L->d, &L->dd, &L->p, &L->pp}) c(dalloc(v, n, s));
+    for (double2 **v : {&L->rv, &L->dv, &L->pv}) c(dalloc(v, n, s));
     c(dalloc(&L->part, (size_t)L->nbs + 3 * (size_t)L->nbv, s));
     c(dalloc(&L->st, 1, s));
     if (e == hipSuccess) c(hipHostMalloc((void **)&L->st_host, 2 * sizeof(BicgState)));
@@ -417,19 +425,19 @@ static int dual_prepare(ssf_matrix *m)
   return SSF_OK;
 }
 
+// DUAL: (p, pp) into L->pv from L->dv; otherwise q = A x
 template <bool DUAL>
-static hipError_t dual_launch(const DualSell *L, const double *d, const double *dd, double *p,
-                              double *pp, double *part, const int *done, long long *iter,
-                              hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
+static hipError_t dual_launch(const DualSell *L, const double *x, double *q, const int *done,
+                              long long *iter, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
 {
-  const DualDev dv{L->n, L->nslice, L->off, L->col, L->col16, L->va, L->vt};
+  const DualDev dd{L->n, L->nslice, L->off, L->col, L->col16, L->va, L->vt};
   dim3 grid(L->nbs), block(TPB);
   if (L->col16) {
-    if (ev0) hipExtLaunchKernelGGL((k_dual_spmv<true, DUAL>), grid, block, 0, s, ev0, ev1, 0, dv, d, dd, p, pp, part, done, iter);
-    else hipLaunchKernelGGL((k_dual_spmv<true, DUAL>), grid, block, 0, s, dv, d, dd, p, pp, part, done, iter);
+    if (ev0) hipExtLaunchKernelGGL((k_dual_spmv<true, DUAL>), grid, block, 0, s, ev0, ev1, 0, dd, L->dv, L->pv, x, q, L->part, done, iter);
+    else hipLaunchKernelGGL((k_dual_spmv<true, DUAL>), grid, block, 0, s, dd, L->dv, L->pv, x, q, L->part, done, iter);
   } else {
-    if (ev0) hipExtLaunchKernelGGL((k_dual_spmv<false, DUAL>), grid, block, 0, s, ev0, ev1, 0, dv, d, dd, p, pp, part, done, iter);
-    else hipLaunchKernelGGL((k_dual_spmv<false, DUAL>), grid, block, 0, s, dv, d, dd, p, pp, part, done, iter);
+    if (ev0) hipExtLaunchKernelGGL((k_dual_spmv<false, DUAL>), grid, block, 0, s, ev0, ev1, 0, dd, L->dv, L->pv, x, q, L->part, done, iter);
+    else hipLaunchKernelGGL((k_dual_spmv<false, DUAL>), grid, block, 0, s, dd, L->dv, L->pv, x, q, L->part, done, iter);
   }
   return hipGetLastError();
 }
@@ -437,14 +445,13 @@ static hipError_t dual_launch(const DualSell *L, const double *d, const double *
 static hipError_t bicg_iteration(const DualSell *L, double *x, double res, hipStream_t s,
                                  hipEvent_t ev0, hipEvent_t ev1)
 {
-  hipError_t e = dual_launch<true>(L, L->d, L->dd, L->p, L->pp, L->part, &L->st->done,
-                                   &L->st->iter, s, ev0, ev1);
+  hipError_t e = dual_launch<true>(L, nullptr, nullptr, &L->st->done, &L->st->iter, s, ev0, ev1);
   if (e != hipSuccess) return e;
   double *prr = L->part + L->nbs;
   hipLaunchKernelGGL(k_bicg_update, dim3(L->nbv), dim3(TPB), 0, s, L->n, L->st, L->part, L->nbs,
-                     x, L->r, L->rr, L->d, L->p, L->pp, prr);
-  hipLaunchKernelGGL(k_bicg_dir, dim3(L->nbv), dim3(TPB), 0, s, L->n, L->st, prr, res, L->d,
-                     L->dd, L->r, L->rr);
+                     x, L->rv, L->dv, L->pv, prr);
+  hipLaunchKernelGGL(k_bicg_dir, dim3(L->nbv), dim3(TPB), 0, s, L->n, L->st, prr, res, L->dv,
+                     L->rv);
   return hipGetLastError();
 }
 
@@ -458,14 +465,15 @@ int solve_bicg(ssf_matrix *m, const ssf_solve_params *p, const double *y, double
   ssf_ctx *ctx = m->ctx;
   const int n = L->n;
   const bool havex = p->iv != 0;
-  if (havex) SSF_HIP(dual_launch<false>(L, x, nullptr, L->r, nullptr, nullptr, nullptr, nullptr, s,
-                                        nullptr, nullptr));
+  // q = A x goes to the (p, pp) pairs' storage, read as n plain doubles
+  double *q = reinterpret_cast<double *>(L->pv);
+  if (havex) SSF_HIP(dual_launch<false>(L, x, q, nullptr, nullptr, s, nullptr, nullptr));
   if (havex)
-    hipLaunchKernelGGL((k_bicg_init<true>), dim3(L->nbv), dim3(TPB), 0, s, n, y, L->r, L->rr,
-                       L->d, L->dd, L->part);
+    hipLaunchKernelGGL((k_bicg_init<true>), dim3(L->nbv), dim3(TPB), 0, s, n, y, q, L->rv, L->dv,
+                       L->part);
   else
-    hipLaunchKernelGGL((k_bicg_init<false>), dim3(L->nbv), dim3(TPB), 0, s, n, y, L->r, L->rr,
-                       L->d, L->dd, L->part);
+    hipLaunchKernelGGL((k_bicg_init<false>), dim3(L->nbv), dim3(TPB), 0, s, n, y, q, L->rv, L->dv,
+                       L->part);
   hipLaunchKernelGGL(k_bicg_init2, dim3(1), dim3(TPB), 0, s, L->part, L->nbv, L->st, p->zero);
   if (!havex)
     hipLaunchKernelGGL(k_bicg_zero_x, dim3((n + TPB - 1) / TPB), dim3(TPB), 0, s, n, L->st, x);
