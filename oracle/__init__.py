@@ -44,6 +44,10 @@ def lib():
             "cr_mxv_scr": (None, [_L, _P, _P, _P, _P, _P]),
             "cr_mxv_cr": (None, [_L, _P, _P, _P, _P, _P]),
             "cr_mtxv_cr": (None, [_L, _P, _P, _P, _P, _P]),
+            "cr_scr_incomplete_ldl": (None, [_L, _P, _P, _P, _D, _P]),
+            "cr_scr_prec_ildl": (None, [_L, _P, _P, _P, _P, _P]),
+            "cr_scr_cg_prec_ildl": (ctypes.c_int, [_L, _P, _P, _P, _P, _P, _P, _L, _D, _P, _P,
+                                                   _D, _L]),
             "cr_cr_bicg": (ctypes.c_int, [_L, _P, _P, _P, _P, _P, _L, _D, _P, _P, _D, _L]),
             "cr_scr_cg_prec": (ctypes.c_int, [_L, _P, _P, _P, _P, _P, _L, _D, _P, _P, _D, _L]),
             "cr_scr_cg": (ctypes.c_int, [_L, _P, _P, _P, _P, _P, _L, _D, _P, _P, _D, _L]),
@@ -104,6 +108,8 @@ def ref():
             "ref_cr_localized": (None, [_P, _P, _P, _L]),
             "ref_cr_mxv": (None, [_P, _P, _P]),
             "ref_cr_mtxv": (None, [_P, _P, _P]),
+            "ref_scr_incomplete_ldl": (None, [_P, _D, _P]),
+            "ref_scr_prec_ildl": (None, [_P, _P, _P]),
             "ref_cr_set_a": (None, [_P, _P]),
             "ref_cr_bicg": (None, [_P, _P, _P, _L, _D, _P, _P, _D, _L]),
             "ref_cr_cg_prec_noprec": (None, [_P, _P, _P, _L, _D, _P, _P, _D, _L]),
@@ -246,6 +252,36 @@ def cr_cg_prec_noprec(adr, ci, a, y, ni, res, zero=1e-30):
     st = lib().cr_cr_cg_prec_noprec(n, _ptr(adr), _ptr(ci), _ptr(f64(a)), _ptr(x),
                                     _ptr(f64(y)), ni, res, ctypes.byref(ani),
                                     ctypes.byref(ares), zero, 0)
+    return x, ani.value, ares.value, st
+
+
+def incomplete_ldl(adr, ci, a, gamma=0.0):
+    """symcomprow::incomplete_ldl: the factor incdec[negm] (L off the diagonal, D on it)."""
+    n = adr.shape[0] - 1
+    out = np.zeros(a.shape[0])
+    lib().cr_scr_incomplete_ldl(n, _ptr(adr), _ptr(ci), _ptr(f64(a)), gamma, _ptr(out))
+    return out
+
+
+def prec_ildl(adr, ci, incdec, y):
+    """symcomprow::prec_ildl_scr: (L D L^T)^-1 y."""
+    n = adr.shape[0] - 1
+    x = np.zeros(n)
+    lib().cr_scr_prec_ildl(n, _ptr(adr), _ptr(ci), _ptr(f64(incdec)), _ptr(x), _ptr(f64(y)))
+    return x
+
+
+def scr_cg_prec_ildl(adr, ci, a, y, ni, res, zero=1e-30, incdec=None, x0=None):
+    """cg_prec with tprec = 10 (incomplete LDL^T, factored here when incdec is None).
+    Returns (x, ani, ares, status)."""
+    n = adr.shape[0] - 1
+    inc = incomplete_ldl(adr, ci, a) if incdec is None else f64(incdec)
+    x = np.zeros(n) if x0 is None else f64(x0).copy()
+    ani = ctypes.c_long(-1)
+    ares = ctypes.c_double(-1.0)
+    st = lib().cr_scr_cg_prec_ildl(n, _ptr(adr), _ptr(ci), _ptr(f64(a)), _ptr(inc), _ptr(x),
+                                   _ptr(f64(y)), ni, res, ctypes.byref(ani), ctypes.byref(ares),
+                                   zero, 0 if x0 is None else 1)
     return x, ani.value, ares.value, st
 
 

@@ -263,10 +263,78 @@ static double ss(const double *a, const double *b, long n)
 
 /* ------------------------------------------------------------------- solvers */
 
-/* GEFEL/scr.cpp:1093-1203, tprec = 1 */
-int cr_scr_cg_prec(long n, const long *adr, const long *ci, const double *a, double *x,
-                   const double *y, long ni, double res, long *ani, double *ares,
-                   double zero, long iv)
+/* GEFEL/scr.cpp:958-975 rows_mult_ldl_scr: sum over the common columns k < o of
+ * rows m and o of incdec_mk incdec_ok D_k, merged in column order */
+static double rows_mult_ldl(const long *adr, const long *ci, const double *incdec, long m,
+                            long o)
+{
+  double s = 0.0;
+  long ii = adr[m], jj = adr[o];
+  long i = ci[ii], j = ci[jj];
+  while (j < o) {
+    if (i == j) {
+      s += incdec[ii] * incdec[jj] * incdec[adr[j + 1] - 1];
+      ii++;
+      jj++;
+    }
+    if (i < j) ii++;
+    if (i > j) jj++;
+    i = ci[ii];
+    j = ci[jj];
+  }
+  return s;
+}
+
+/* GEFEL/scr.cpp:990-1022 symcomprow::incomplete_ldl: incomplete LDL^T on the
+ * pattern of a; off-diagonals L_ij, diagonal D_i, in incdec[negm]. Products of
+ * rows with columns outside the pattern are lumped into the diagonal with
+ * weight gamma (gmatrix fixes gamma = 0, GEFEL/gmatrix.cpp:24). */
+void cr_scr_incomplete_ldl(long n, const long *adr, const long *ci, const double *a,
+                           double gamma, double *incdec)
+{
+  for (long i = 0; i < n; i++) {
+    long ii = adr[i], lj = ci[ii], uj = adr[i + 1] - 1;
+    incdec[uj] = a[uj];
+    for (long j = lj; j < i; j++) {
+      double s = rows_mult_ldl(adr, ci, incdec, i, j);
+      if (j == ci[ii]) {
+        incdec[ii] = (a[ii] - s) / incdec[adr[j + 1] - 1];
+        ii++;
+      } else {
+        incdec[uj] -= gamma * s;
+      }
+    }
+    double s = 0.0;
+    for (long j = adr[i]; j < uj; j++) s += incdec[j] * incdec[j] * incdec[adr[ci[j] + 1] - 1];
+    incdec[uj] -= s;
+  }
+}
+
+/* GEFEL/scr.cpp:1031-1060 symcomprow::prec_ildl_scr: x = (L D L^T)^-1 y; the
+ * back substitution scatters row by row from the last row down */
+void cr_scr_prec_ildl(long n, const long *adr, const long *ci, const double *incdec,
+                      double *x, const double *y)
+{
+  double *p = (double *)malloc((size_t)n * sizeof(double));
+  for (long i = 0; i < n; i++) {
+    double s = 0.0;
+    for (long j = adr[i]; j < adr[i + 1] - 1; j++) s += incdec[j] * p[ci[j]];
+    p[i] = y[i] - s;
+  }
+  for (long i = 0; i < n; i++) p[i] /= incdec[adr[i + 1] - 1];
+  for (long i = n - 1; i > -1; i--) {
+    x[i] = p[i];
+    double s = x[i];
+    for (long j = adr[i]; j < adr[i + 1] - 1; j++) p[ci[j]] -= incdec[j] * s;
+  }
+  free(p);
+}
+
+/* GEFEL/scr.cpp:1093-1203: tprec = 1 (prec_diag_scr) or 10 (prec_ildl_scr with
+ * the factor incdec) */
+static int scr_cg_prec_t(long n, const long *adr, const long *ci, const double *a, double *x,
+                         const double *y, long ni, double res, long *ani, double *ares,
+                         double zero, long iv, long tprec, const double *incdec)
 {
   long i, j;
   double nom, denom, nory, alpha, beta, rr;
@@ -295,7 +363,9 @@ int cr_scr_cg_prec(long n, const long *adr, const long *ci, const double *a, dou
     *ares = nory;
     *ani = 0;
   }
-  for (i = 0; i < n; i++) h[i] = r[i] / a[adr[i + 1] - 1];
+  if (tprec == 10) cr_scr_prec_ildl(n, adr, ci, incdec, h, r);
+  else
+    for (i = 0; i < n; i++) h[i] = r[i] / a[adr[i + 1] - 1];
   nom = ss(r, h, n);
   for (i = 0; i < n; i++) d[i] = h[i];
   for (i = 0; i < ni; i++) {
@@ -314,7 +384,9 @@ int cr_scr_cg_prec(long n, const long *adr, const long *ci, const double *a, dou
       r[j] -= alpha * p[j];
       rr += r[j] * r[j];
     }
-    for (j = 0; j < n; j++) h[j] = r[j] / a[adr[j + 1] - 1];
+    if (tprec == 10) cr_scr_prec_ildl(n, adr, ci, incdec, h, r);
+    else
+      for (j = 0; j < n; j++) h[j] = r[j] / a[adr[j + 1] - 1];
     denom = nom;
     nom = ss(r, h, n);
     beta = nom / denom;
@@ -331,6 +403,22 @@ int cr_scr_cg_prec(long n, const long *adr, const long *ci, const double *a, dou
   free(r);
   free(d);
   return status;
+}
+
+int cr_scr_cg_prec(long n, const long *adr, const long *ci, const double *a, double *x,
+                   const double *y, long ni, double res, long *ani, double *ares,
+                   double zero, long iv)
+{
+  return scr_cg_prec_t(n, adr, ci, a, x, y, ni, res, ani, ares, zero, iv, 1, NULL);
+}
+
+/* cg_prec with tprec = 10 on the factor of cr_scr_incomplete_ldl, as
+ * gmatrix::solve_system runs it for incomdec (GEFEL/gmatrix.cpp:1234-1240) */
+int cr_scr_cg_prec_ildl(long n, const long *adr, const long *ci, const double *a,
+                        const double *incdec, double *x, const double *y, long ni, double res,
+                        long *ani, double *ares, double zero, long iv)
+{
+  return scr_cg_prec_t(n, adr, ci, a, x, y, ni, res, ani, ares, zero, iv, 10, incdec);
 }
 
 /* GEFEL/scr.cpp:786-880 */

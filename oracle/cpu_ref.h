@@ -33,6 +33,13 @@
  *                         assembled forces = A x - lift, D / -k closed forms
  *   cr_hex8_heat_cap      bitwise vs oracle/_ref; unit-cube closed form (exact with
  *                         2x2x2 points); sum of entries = rho c V
+ *   cr_scr_incomplete_ldl, cr_scr_prec_ildl, cr_scr_cg_prec_ildl
+ *                         bitwise vs oracle/_ref (symcomprow::incomplete_ldl,
+ *                         prec_ildl_scr, cg_prec with tprec = 10; gamma 0 and not);
+ *                         a pattern without fill gives the exact LDL^T (dense and
+ *                         tridiagonal: L D L^T = A, the preconditioned solve is
+ *                         exact, cg_prec converges in one iteration); the product
+ *                         L D L^T equals A on the pattern
  *   cr_mtxv_cr, cr_cr_bicg
  *                         bitwise vs oracle/_ref (comprow::mtxv_cr, comprow::bicg);
  *                         transpose identity b.(A^T c) = (A b).c; on a symmetric
@@ -99,6 +106,13 @@ void cr_mxv_cr(long n, const long *adr, const long *ci, const double *a,
 int cr_scr_cg_prec(long n, const long *adr, const long *ci, const double *a, double *x,
                    const double *y, long ni, double res, long *ani, double *ares,
                    double zero, long iv);
+void cr_scr_incomplete_ldl(long n, const long *adr, const long *ci, const double *a,
+                           double gamma, double *incdec);
+void cr_scr_prec_ildl(long n, const long *adr, const long *ci, const double *incdec,
+                      double *x, const double *y);
+int cr_scr_cg_prec_ildl(long n, const long *adr, const long *ci, const double *a,
+                        const double *incdec, double *x, const double *y, long ni, double res,
+                        long *ani, double *ares, double zero, long iv);
 /* GEFEL/scr.cpp:786-880 symcomprow::cg (no preconditioner, sqrt test). */
 int cr_scr_cg(long n, const long *adr, const long *ci, const double *a, double *x,
               const double *y, long ni, double res, long *ani, double *ares,

@@ -79,6 +79,19 @@ void ref_scr_cg(void *h, double *x, double *y, long ni, double res, long *ani,
   *ani = a;
   *ares = r;
 }
+// symcomprow::incomplete_ldl (GEFEL/scr.cpp:990) on the object's values; the
+// factor incdec[negm] is copied out
+void ref_scr_incomplete_ldl(void *h, double gamma, double *out)
+{
+  symcomprow *s = (symcomprow *)h;
+  delete[] s->incdec;
+  s->incdec = NULL;
+  s->incomplete_ldl(gamma);
+  memcpy(out, s->incdec, s->negm * sizeof(double));
+}
+// symcomprow::prec_ildl_scr (GEFEL/scr.cpp:1031) with the factor of the last
+// ref_scr_incomplete_ldl
+void ref_scr_prec_ildl(void *h, double *x, double *y) { ((symcomprow *)h)->prec_ildl_scr(x, y); }
 void ref_scr_free(void *h) { delete (symcomprow *)h; }
 
 // ------------------------------------------------------------------ comprow

@@ -20,6 +20,9 @@ Element volume loads (elast_vload_*, heat_vload_*): seeded nodal load values per
 element, their load vectors from the harness's ref_hex8_volume_load (load_matrix
 / quantity_source_vector), added by locglob in element order onto a seeded
 starting vector (standing for the nodal loads added first).
+Incomplete LDL^T ({heat,elast}_ildl_*): the reference's symcomprow::incomplete_ldl
+(gamma 0, as gmatrix fixes it) of the scr values, prec_ildl_scr of a seeded
+vector, and cg_prec with tprec = 10 from x = 0 to res 1e-24.
 Biconjugate gradients ({heat,elast}_bicg_*): nonsymmetric values on the comprow
 pattern (every off-diagonal entry scaled by 1 + gamma t, t seeded uniform), the
 reference's comprow::mtxv_cr of a seeded vector, comprow::bicg from x = 0 to
@@ -237,6 +240,23 @@ def main():
     out["heat_np_ani"] = np.array([t[2] for t in traj])
     out["heat_np_ares"] = np.array([t[3] for t in traj])
     print("heat np", prob.n, "unknowns", flush=True)
+    # symcomprow::incomplete_ldl, prec_ildl_scr and cg_prec with tprec = 10
+    for name in ("heat", "elast"):
+        prob = CASES[name]()
+        ke = ref_ke(R, prob)
+        h, adr, ci, a, free = storage(R, prob, ke, "scr")
+        inc = np.zeros(a.shape[0])
+        R.ref_scr_incomplete_ldl(h, 0.0, _ptr(inc))
+        vr = np.random.default_rng(6000 + list(CASES).index(name))
+        yv = vr.standard_normal(prob.n)
+        pv = np.zeros(prob.n)
+        R.ref_scr_prec_ildl(h, _ptr(pv), _ptr(yv.copy()))
+        y = prob.rhs[0] + O.dirichlet_lift(prob, i64(prob.elem_codes()), ke)
+        x, ani, ares = solve(R.ref_scr_cg_prec, h, prob.n, y, 2000, 1e-24, 10)
+        free(h)
+        out[f"{name}_ildl_inc"], out[f"{name}_ildl_yv"], out[f"{name}_ildl_pv"] = inc, yv, pv
+        out[f"{name}_ildl_x"], out[f"{name}_ildl_ani"], out[f"{name}_ildl_ares"] = x, ani, ares
+        print("ildl", name, prob.n, "unknowns, ani", ani, flush=True)
     # comprow::bicg on nonsymmetric values of the heat and elasticity patterns
     for name, gamma in (("heat", 0.4), ("elast", 0.3)):
         prob = CASES[name]()
