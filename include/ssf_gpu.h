@@ -38,6 +38,7 @@ extern "C" {
 #define SSF_SOLVER_BICG 30             /* linsolvertype bicg */
 #define SSF_PREC_NOPRECOND 0           /* precondtype        */
 #define SSF_PREC_DIAG 1                /* diagprec (Jacobi)  */
+#define SSF_PREC_ILDL 10               /* incomdec (incomplete LDL^T, symcomprow) */
 
 /* element kinds of the batched element kernels */
 #define SSF_ELEM_HEX8_ELAST 0  /* MEFEL linhex, 24x24, 3 DOF/node            */
@@ -56,7 +57,7 @@ typedef struct ssf_matrix ssf_matrix;
 
 typedef struct ssf_solve_params {
   int32_t linsolver;   /* SSF_SOLVER_CG, SSF_SOLVER_BICG (compressed_rows)        */
-  int32_t prec;        /* SSF_PREC_NOPRECOND / SSF_PREC_DIAG                      */
+  int32_t prec;        /* SSF_PREC_NOPRECOND / SSF_PREC_DIAG / SSF_PREC_ILDL       */
   int64_t ni;          /* maximum number of iterations (gmatrix::ni)              */
   double res;          /* required residual norm (gmatrix::res); squared test for
                           symcomprow + diagprec as GEFEL/scr.cpp:1188             */
@@ -165,6 +166,11 @@ int ssf_gmxv_dev(ssf_matrix *m, const double *a_dev, double *b_dev);
  *   cg (20): symcomprow + diagprec -> symcomprow::cg_prec GEFEL/scr.cpp:1093;
  *     symcomprow + noprecond -> symcomprow::cg :786; comprow ->
  *     comprow::cg_prec GEFEL/cr.cpp:1124 with the noprecond preconditioner.
+ *     symcomprow + incomdec (10) -> symcomprow::incomplete_ldl GEFEL/scr.cpp:990
+ *     with weight 0 (gmatrix.cpp:24,1234-1240), then cg_prec :1093 with
+ *     prec_ildl_scr :1031; FP64. The factor is formed at the first such solve and
+ *     again after every change of the values (the reference factors once); its
+ *     rows run by dependency levels, each row in the reference's operation order.
  *   bicg (30): comprow only -> comprow::bicg GEFEL/cr.cpp:1353 (gmatrix.cpp:
  *     1273-1289), FP64, no preconditioner (prec is ignored, as there). Stops when
  *     r.r / y.y < res and rr.rr / y.y < res (:1400): status 1, ani = the
@@ -180,6 +186,14 @@ int ssf_solve(ssf_matrix *m, const ssf_solve_params *p, const double *rhs, doubl
 /* same with device vectors (rhs_dev read-only, lhs_dev in/out), synchronous */
 int ssf_solve_dev(ssf_matrix *m, const ssf_solve_params *p, const double *rhs_dev,
                   double *lhs_dev, ssf_solve_info *info);
+/* symcomprow::incomplete_ldl (GEFEL/scr.cpp:990-1022) with weight 0, as gmatrix
+ * runs it (GEFEL/gmatrix.cpp:24,1237): the factor incdec[negm] of the current
+ * values in the reference layout (L off the diagonal, D on it), copied to the
+ * host. symm_comp_rows only. */
+int ssf_matrix_ildl_factor(ssf_matrix *m, double *incdec);
+/* symcomprow::prec_ildl_scr (GEFEL/scr.cpp:1031-1060): x_dev = (L D L^T)^-1 y_dev
+ * with the factor of the current values (formed if needed); asynchronous. */
+int ssf_ildl_apply_dev(ssf_matrix *m, const double *y_dev, double *x_dev);
 
 /* one_step_linear (TRFEL/SRC/npsolvert.cpp:367-482): one step of the generalized
  * trapezoidal rule in the v-form for C dT/dt + K T = f, on device vectors:

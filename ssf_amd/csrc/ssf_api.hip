@@ -332,6 +332,7 @@ extern "C" int ssf_matrix_destroy(ssf_matrix *m)
   dfree(m->d_adr32, m->ctx->stream); dfree(m->d_ci32, m->ctx->stream);
   free_workspace(m->ws, m->ctx->stream);
   dual_free(m->dual, m->ctx->stream);
+  ildl_free(m->ildl, m->ctx->stream);
   m->ctx->live_matrices--;
   delete m;
   return SSF_OK;
@@ -1428,12 +1429,29 @@ extern "C" int ssf_solve_dev(ssf_matrix *m, const ssf_solve_params *p, const dou
     if (rc) return rc;
     return true_residual(m, lhs_dev, rhs_dev, &info->rel_residual);
   }
+  if (m->storage == SSF_STORAGE_SYMM_COMP_ROWS && p->prec == SSF_PREC_ILDL) {
+    // gmatrix::solve_system incomdec branch (GEFEL/gmatrix.cpp:1234-1240)
+    SSF_CHECK(p->precision == SSF_PRECISION_FP64, SSF_EUNSUP,
+              "ssf_solve_dev: the incomplete LDL^T preconditioner runs in FP64 only");
+    SSF_HIP(hipSetDevice(m->ctx->device));
+    int rc = ensure_ws(m, false);
+    if (rc) return rc;
+    if (m->n == 0) {
+      info->status = 1;
+      info->iterations = 0;
+      return SSF_OK;
+    }
+    info->rel_residual = 0.0;
+    rc = solve_ildl(m, p, rhs_dev, lhs_dev, info);
+    if (rc) return rc;
+    return true_residual(m, lhs_dev, rhs_dev, &info->rel_residual);
+  }
   int mode;
   bool jac;
   if (m->storage == SSF_STORAGE_SYMM_COMP_ROWS) {
     if (p->prec == SSF_PREC_DIAG) { mode = MODE_SCR_CG_PREC; jac = true; }
     else if (p->prec == SSF_PREC_NOPRECOND) { mode = MODE_SCR_CG; jac = false; }
-    else { set_error("ssf_solve_dev: symm_comp_rows supports noprecond (0) and diagprec (1) here"); return SSF_EUNSUP; }
+    else { set_error("ssf_solve_dev: symm_comp_rows supports noprecond (0), diagprec (1) and incomdec (10) here"); return SSF_EUNSUP; }
   } else {
     if (p->prec == SSF_PREC_NOPRECOND) { mode = MODE_CR_CG_PREC; jac = false; }
     else if (p->prec == SSF_PREC_DIAG && p->precision == SSF_PRECISION_MIXED) { mode = MODE_SCR_CG_PREC; jac = true; }
@@ -1456,6 +1474,31 @@ extern "C" int ssf_solve_dev(ssf_matrix *m, const ssf_solve_params *p, const dou
   info->rel_residual = 0.0;
   return mixed ? solve_mixed(m, p, rhs_dev, lhs_dev, info)
                : solve_fp64(m, mode, jac, p, rhs_dev, lhs_dev, info);
+}
+
+extern "C" int ssf_matrix_ildl_factor(ssf_matrix *m, double *incdec)
+{
+  SSF_ENTER();
+  SSF_CHECK(m && incdec, SSF_EINVAL, "ssf_matrix_ildl_factor: NULL argument");
+  SSF_CHECK(m->storage == SSF_STORAGE_SYMM_COMP_ROWS, SSF_EUNSUP,
+            "ssf_matrix_ildl_factor: symm_comp_rows storage only");
+  SSF_CHECK(m->values_set, SSF_ESTATE, "ssf_matrix_ildl_factor: matrix values were never set");
+  SSF_HIP(hipSetDevice(m->ctx->device));
+  if (m->n == 0) return SSF_OK;
+  return ildl_factor_copy(m, incdec);
+}
+
+extern "C" int ssf_ildl_apply_dev(ssf_matrix *m, const double *y_dev, double *x_dev)
+{
+  SSF_ENTER();
+  SSF_CHECK(m && y_dev && x_dev, SSF_EINVAL, "ssf_ildl_apply_dev: NULL argument");
+  SSF_CHECK(m->storage == SSF_STORAGE_SYMM_COMP_ROWS, SSF_EUNSUP,
+            "ssf_ildl_apply_dev: symm_comp_rows storage only");
+  SSF_CHECK(m->values_set, SSF_ESTATE, "ssf_ildl_apply_dev: matrix values were never set");
+  SSF_CHECK(y_dev != x_dev, SSF_EINVAL, "ssf_ildl_apply_dev: x and y must not alias");
+  SSF_HIP(hipSetDevice(m->ctx->device));
+  if (m->n == 0) return SSF_OK;
+  return ildl_apply_dev(m, y_dev, x_dev);
 }
 
 extern "C" int ssf_solve(ssf_matrix *m, const ssf_solve_params *p, const double *rhs, double *lhs,

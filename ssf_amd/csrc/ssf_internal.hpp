@@ -164,6 +164,7 @@ struct Workspace {  // vectors and reduction scratch of one solve shape
 };
 
 struct DualSell;  // union layout of A and A^T of a comprow matrix (ssf_bicg.hip)
+struct IldlPlan;  // incomplete LDL^T levels and factor of a symcomprow matrix (ssf_ildl.hip)
 
 }  // namespace ssf
 
@@ -233,6 +234,9 @@ struct ssf_matrix {
   // biconjugate gradients: union layout of A and A^T with both values (built at
   // the first bicg solve)
   ssf::DualSell *dual = nullptr;
+  // incomplete LDL^T: dependency levels of the pattern, the factor of the values
+  // (built at the first incomdec solve, factored again after a values change)
+  ssf::IldlPlan *ildl = nullptr;
   // as the system matrix of a nonstationary step: the inputs it was formed from
   const ssf_matrix *np_k = nullptr, *np_c = nullptr;
   uint64_t np_kv = 0, np_cv = 0;
@@ -310,4 +314,11 @@ bool build_sell(int64_t n, const std::vector<int> &rowptr, const std::vector<int
 int solve_bicg(ssf_matrix *m, const ssf_solve_params *p, const double *y, double *x,
                ssf_solve_info *info);
 void dual_free(DualSell *L, hipStream_t s);
+// cg_prec with tprec = incomdec (GEFEL/scr.cpp:1093, :990, :1031) of a symcomprow
+// matrix on device vectors; fills info except rel_residual
+int solve_ildl(ssf_matrix *m, const ssf_solve_params *p, const double *y, double *x,
+               ssf_solve_info *info);
+int ildl_factor_copy(ssf_matrix *m, double *out);                 // factor -> host
+int ildl_apply_dev(ssf_matrix *m, const double *y, double *h);    // h = (LDL^T)^-1 y
+void ildl_free(IldlPlan *P, hipStream_t s);
 }  // namespace ssf

@@ -1,6 +1,8 @@
 // Launchers of the gfx950 kernels (defined in ssf_kernels.hip / ssf_elast.hip).
 #pragma once
 
+#include <functional>
+
 #include "ssf_internal.hpp"
 
 namespace ssf {
@@ -28,6 +30,20 @@ hipError_t pcg_iteration(const SpmvTune &t, int mode, bool jac, const CsrDev &A,
                          const T *diag, T *x, T *r, T *d, T *p, T *h, T *part,
                          PcgState<T> *st, T res, T zero, hipStream_t s,
                          hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+
+// FP64 PCG with a preconditioner enqueued by prec (h = M^-1 r, on stream s, reading
+// r and writing h; it may skip its work once st->done is set), in the cg_prec
+// recurrence and stopping test (GEFEL/scr.cpp:1093-1203). Partial layout as
+// pcg_iteration's, r.h in [P1 + nv, P1 + 2 nv).
+hipError_t pcg_init_prec(const SpmvTune &t, bool havex, const CsrDev &A, const double *val,
+                         const double *x, const double *y, double *r, double *h, double *d,
+                         double *part, PcgState<double> *st, double zero, hipStream_t s,
+                         const std::function<hipError_t()> &prec);
+hipError_t pcg_iteration_prec(const SpmvTune &t, const CsrDev &A, const double *val, double *x,
+                              double *r, double *d, double *p, double *h, double *part,
+                              PcgState<double> *st, double res, double zero, hipStream_t s,
+                              hipEvent_t ev0, hipEvent_t ev1,
+                              const std::function<hipError_t()> &prec);
 
 // mixed precision with reliable updates (ssf_solver.hip)
 hipError_t mixed_init(MixState *ms, PcgState<float> *st, const double *slots,

@@ -480,6 +480,90 @@ template hipError_t pcg_iteration<float>(const SpmvTune &, int, bool, const CsrD
                                          float *, float *, float *, PcgState<float> *, float,
                                          float, hipStream_t, hipEvent_t, hipEvent_t);
 
+// ------------------------------------------- PCG with an external preconditioner
+
+// partials of a.b in the element-wise kernels' row chunks; skipped once stopped
+__global__ __launch_bounds__(TPB) void k_prec_dot(int n, const PcgState<double> *st,
+                                                  const double *__restrict__ a,
+                                                  const double *__restrict__ b,
+                                                  double *__restrict__ part)
+{
+  if (st && st->done) return;
+  int r0, r1;
+  chunk(n, r0, r1);
+  double acc = 0.0;
+  for (int i = r0 + threadIdx.x; i < r1; i += TPB) acc += a[i] * b[i];
+  const double t = block_sum(acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// d = h and the partials of r.h
+__global__ __launch_bounds__(TPB) void k_prec_start(int n, const double *__restrict__ r,
+                                                    const double *__restrict__ h,
+                                                    double *__restrict__ d,
+                                                    double *__restrict__ part)
+{
+  int r0, r1;
+  chunk(n, r0, r1);
+  double acc = 0.0;
+  for (int i = r0 + threadIdx.x; i < r1; i += TPB) {
+    const double hi = h[i];
+    d[i] = hi;
+    acc += r[i] * hi;
+  }
+  const double t = block_sum(acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+hipError_t pcg_init_prec(const SpmvTune &t, bool havex, const CsrDev &A, const double *val,
+                         const double *x, const double *y, double *r, double *h, double *d,
+                         double *part, PcgState<double> *st, double zero, hipStream_t s,
+                         const std::function<hipError_t()> &prec)
+{
+  if (havex) {
+    hipError_t e = spmv<double>(t, 0, A, val, x, r, nullptr, nullptr, nullptr, nullptr, s);
+    if (e != hipSuccess) return e;
+  }
+  const int nb = vec_blocks(t, A.n);
+  dim3 grid(nb), block(TPB);
+  // r = y - A x (or y), partials y.y into part[0, nb)
+  if (havex) hipLaunchKernelGGL((k_pcg_init<double, false, true>), grid, block, 0, s, A.n, nullptr, y, r, h, d, part);
+  else hipLaunchKernelGGL((k_pcg_init<double, false, false>), grid, block, 0, s, A.n, nullptr, y, r, h, d, part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  e = prec();  // h = M^-1 r
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_prec_start, grid, block, 0, s, A.n, r, h, d, part + 2 * nb);
+  hipLaunchKernelGGL((k_pcg_init2<double>), dim3(1), dim3(TPB), 0, s, part, nb, st, zero);
+  return hipGetLastError();
+}
+
+hipError_t pcg_iteration_prec(const SpmvTune &t, const CsrDev &A, const double *val, double *x,
+                              double *r, double *d, double *p, double *h, double *part,
+                              PcgState<double> *st, double res, double zero, hipStream_t s,
+                              hipEvent_t ev0, hipEvent_t ev1,
+                              const std::function<hipError_t()> &prec)
+{
+  hipError_t e = spmv<double>(t, 1, A, val, d, p, nullptr, part, &st->done, &st->iter, s, ev0, ev1);
+  if (e != hipSuccess) return e;
+  const int nv = vec_blocks(t, A.n);
+  dim3 grid(nv), block(TPB);
+  double *prr = part + t.blocks;
+  const int n = A.n;
+  // x += a d, r -= a p, partials r.r into prr[0, nv)
+  hipLaunchKernelGGL((k_pcg_update<double, false>), grid, block, 0, s, n, st, part, t.blocks, zero,
+                     x, r, d, p, nullptr, nullptr, prr);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  e = prec();  // h = M^-1 r
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_prec_dot, grid, block, 0, s, n, st, r, h, prr + nv);
+  // the cg_prec test on r.r, nom = r.h, d = b d + h
+  hipLaunchKernelGGL((k_pcg_dir<double, true, MODE_SCR_CG_PREC>), grid, block, 0, s, n, st, prr,
+                     res, zero, d, h);
+  return hipGetLastError();
+}
+
 }  // namespace ssf
 
 // ------------------------------------------- mixed precision: reliable updates

@@ -22,6 +22,7 @@ SOLVER_CG = 20
 SOLVER_BICG = 30
 PREC_NONE = 0
 PREC_DIAG = 1
+PREC_ILDL = 10
 FP64 = 0
 MIXED = 1
 KIND_HEX8_HEAT_CAP = 3
@@ -93,6 +94,8 @@ SIGNATURES = {
     "ssf_matrix_add": (_I, [_P, _D, _P]),
     "ssf_np_step_linear": (_I, [_P, _P, _P, _D, _D, ctypes.POINTER(SolveParams), _P, _P, _P,
                                 ctypes.POINTER(SolveInfo)]),
+    "ssf_matrix_ildl_factor": (_I, [_P, _P]),
+    "ssf_ildl_apply_dev": (_I, [_P, _P, _P]),
     "ssf_np_step_dform": (_I, [_P, _P, _P, _D, _D, ctypes.POINTER(SolveParams), _P, _P, _P,
                                ctypes.POINTER(SolveInfo)]),
     "ssf_elements_create": (_I, [_P, _I, _L, _L, _P, _L, _P, _P, _P, _P, _P, ctypes.POINTER(_P)]),
@@ -311,6 +314,16 @@ class Matrix:
         _check(lib().ssf_solve(self.h, ctypes.byref(p), _p(rhs), _p(x), ctypes.byref(info)))
         return x, info
 
+    def ildl_factor(self):
+        """symcomprow::incomplete_ldl of the current values (weight 0), to the host."""
+        out = np.zeros(self.negm)
+        _check(lib().ssf_matrix_ildl_factor(self.h, _p(out)))
+        return out
+
+    def ildl_apply_dev(self, y_ptr, x_ptr):
+        """prec_ildl_scr on device vectors: x = (L D L^T)^-1 y."""
+        _check(lib().ssf_ildl_apply_dev(self.h, ctypes.c_void_p(y_ptr), ctypes.c_void_p(x_ptr)))
+
     def solve_dev(self, rhs_ptr, x_ptr, ni=100000, res=1e-20, prec=PREC_DIAG, precision=FP64,
                   zero=1e-30, iv=0, inner_res=1e-4, inner_ni=0, max_outer=0,
                   linsolver=SOLVER_CG):
@@ -416,6 +429,16 @@ class Batch:
         infos = self._infos()
         _check(lib().ssf_batch_solve(self.h, ctypes.byref(p), _p(y), _p(x), infos))
         return self.unpack(x), list(infos)
+
+    def ildl_factor(self):
+        """symcomprow::incomplete_ldl of the current values (weight 0), to the host."""
+        out = np.zeros(self.negm)
+        _check(lib().ssf_matrix_ildl_factor(self.h, _p(out)))
+        return out
+
+    def ildl_apply_dev(self, y_ptr, x_ptr):
+        """prec_ildl_scr on device vectors: x = (L D L^T)^-1 y."""
+        _check(lib().ssf_ildl_apply_dev(self.h, ctypes.c_void_p(y_ptr), ctypes.c_void_p(x_ptr)))
 
     def solve_dev(self, rhs_ptr, x_ptr, ni=100000, res=1e-20, precision=FP64, zero=1e-30, iv=0,
                   inner_res=1e-4, inner_ni=0):
