@@ -431,6 +431,63 @@ def bench_bicg(ctx, args, gpu, mesh, torch, prob):
     return out
 
 
+def bench_ildl(ctx, args, gpu, mesh, torch, prob, m, rhs):
+    """cg_prec with the incomplete LDL^T preconditioner (incomdec) on the heat cube's
+    symcomprow matrix, FP64 from x = 0 to --res, against the Jacobi cg_prec on the
+    same matrix. The factor and the sweeps run by dependency levels, one launch per
+    level. CPU baseline: --cpu-iters iterations of the oracle's incomdec cg_prec on
+    the GPU's factor, scaled to the GPU's iteration count; the factorization is
+    left out of it (the oracle forms every dropped-fill product, as the reference
+    does, which takes minutes at this size)."""
+    t0 = time.time()
+    nf, nb, us_apply = m.ildl_bench(reps=5)  # plan, factor and the apply graph
+    setup_s = time.time() - t0
+    _, _, us_direct = m.ildl_bench(reps=5, graph=False)
+    dev = torch.device("cuda:0")
+    b = torch.tensor(rhs, dtype=torch.float64, device=dev)
+    x = torch.zeros(prob.n, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+
+    def solve():
+        return m.solve_dev(b.data_ptr(), x.data_ptr(), ni=200000, res=args.res,
+                           prec=gpu.PREC_ILDL)
+
+    solve()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    infos = [solve() for _ in range(args.ildl_steps)]
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t
+    its = int(infos[-1].iterations)
+    out = {"workload": f"heat hex8 {args.nel}^3 symcomprow, n={prob.n}, res {args.res}, "
+                       f"incomplete LDL^T (weight 0) from x = 0",
+           "solves_per_s": args.ildl_steps / dt, "ms_per_solve": 1e3 * dt / args.ildl_steps,
+           "iterations": its, "status": int(infos[-1].status),
+           "rel_residual": float(infos[-1].rel_residual),
+           "levels": {"forward": nf, "backward": nb},
+           "usec_per_application": us_apply,
+           "usec_per_application_stream_launches": us_direct,
+           "setup_s_levels_and_factor": setup_s}
+    log(f"[ildl] {out['solves_per_s']:.3f} solves/s, {its} iterations, levels {nf}/{nb}, "
+        f"apply {us_apply:.0f} us (graph) / {us_direct:.0f} us (stream), set-up {setup_s:.1f}s")
+    if not args.no_cpu:
+        import oracle as O
+        adr, ci = m.pattern()
+        a = m.get_values()
+        inc = m.ildl_factor()
+        k = args.cpu_iters
+        t = time.perf_counter()
+        O.scr_cg_prec_ildl(adr, ci, a, rhs, k, 0.0, incdec=inc)
+        per_it = (time.perf_counter() - t) / k
+        out["cpu_baseline"] = {"value": 1.0 / (per_it * its), "unit": "solves/s",
+                               "cores": 1, "kind": "port",
+                               "sample": f"{k} iterations of the oracle's incomdec cg_prec "
+                                         f"(1 thread) on the same system and factor, scaled to "
+                                         f"the GPU's {its}; factorization not included"}
+        out["speedup_vs_cpu"] = out["solves_per_s"] / out["cpu_baseline"]["value"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -462,6 +519,8 @@ def main():
     ap.add_argument("--np-steps", type=int, default=10)
     ap.add_argument("--np-dt", type=float, default=1e-3)
     ap.add_argument("--no-bicg", action="store_true", help="skip the comprow bicg leg")
+    ap.add_argument("--no-ildl", action="store_true", help="skip the incomplete LDL^T leg")
+    ap.add_argument("--ildl-steps", type=int, default=1)
     ap.add_argument("--bicg-steps", type=int, default=2)
     ap.add_argument("--serve-count", type=int, default=24)
     ap.add_argument("--serve-res", type=float, default=1e-22)
@@ -600,6 +659,9 @@ def main():
     transient = None
     if not args.no_np:
         transient = bench_transient(ctx, args, gpu, mesh, torch, prob, m, rhs)
+    ildl = None
+    if not args.no_ildl:
+        ildl = bench_ildl(ctx, args, gpu, mesh, torch, prob, m, rhs)
     bicg = None
     if not args.no_bicg:
         bicg = bench_bicg(ctx, args, gpu, mesh, torch, prob)
@@ -649,6 +711,7 @@ def main():
         "cpu_baseline": cpu,
         "post": post,
         "transient": transient,
+        "ildl": ildl,
         "bicg": bicg,
         "batched": batched,
         "served": served,

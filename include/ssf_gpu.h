@@ -363,6 +363,13 @@ int ssf_copy_bench(ssf_ctx *ctx, int64_t bytes, int reps, double *usec);
 /* bytes the SpMV of the given precision streams by the canonical count
  * nnz*(s_v+4) + 4(n+1) + 2 s_v n, and the bytes of the format actually used */
 int ssf_spmv_bytes(const ssf_matrix *m, int precision, double *canonical, double *actual);
+/* incomplete LDL^T of a symcomprow matrix (forms its plan and factor if needed):
+ * the number of dependency levels of the forward and backward sweeps, each one
+ * launch per application, and usec = mean time of `reps` applications
+ * (prec_ildl_scr) on device vectors, HIP events on the context's stream;
+ * graph = 1 replays the captured sweeps, 0 launches them on the stream */
+int ssf_ildl_bench(ssf_matrix *m, int reps, int graph, int64_t *nfwd, int64_t *nbwd,
+                   double *usec);
 
 #ifdef __cplusplus
 }

@@ -1488,6 +1488,18 @@ extern "C" int ssf_matrix_ildl_factor(ssf_matrix *m, double *incdec)
   return ildl_factor_copy(m, incdec);
 }
 
+extern "C" int ssf_ildl_bench(ssf_matrix *m, int reps, int graph, int64_t *nfwd, int64_t *nbwd,
+                              double *usec)
+{
+  SSF_ENTER();
+  SSF_CHECK(m && nfwd && nbwd && usec && reps >= 0, SSF_EINVAL, "ssf_ildl_bench: bad argument");
+  SSF_CHECK(m->storage == SSF_STORAGE_SYMM_COMP_ROWS, SSF_EUNSUP,
+            "ssf_ildl_bench: symm_comp_rows storage only");
+  SSF_CHECK(m->values_set && m->n > 0, SSF_ESTATE, "ssf_ildl_bench: matrix values were never set");
+  SSF_HIP(hipSetDevice(m->ctx->device));
+  return ildl_bench(m, reps, graph, nfwd, nbwd, usec);
+}
+
 extern "C" int ssf_ildl_apply_dev(ssf_matrix *m, const double *y_dev, double *x_dev)
 {
   SSF_ENTER();

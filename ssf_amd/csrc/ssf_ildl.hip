@@ -34,6 +34,7 @@ struct IldlPlan {
   int *frows = nullptr, *brows = nullptr; // rows grouped by forward / backward level
   std::vector<int> flev, blev;            // level offsets (host)
   int *tptr = nullptr, *trow = nullptr, *tslot = nullptr;  // per k: (i, slot) of L_ik, i desc.
+  double *tval = nullptr;                 // L_ik in that order (gathered after each factor)
   double *inc = nullptr;                  // the factor (reference layout)
   double *z = nullptr, *w = nullptr;      // forward sweep, before / after D^-1
   uint64_t version = ~0ULL;               // values_version of the factor
@@ -83,6 +84,11 @@ __global__ __launch_bounds__(TPB) void k_ildl_factor(int cnt, const int *__restr
   inc[uj] = a[uj] - s;
 }
 
+// a row's entries are loaded ILDL_U at a time (indices, then values and the
+// gathered vector entries, all in flight together) and summed in order: a level
+// costs a couple of memory round trips instead of one per entry
+constexpr int ILDL_U = 16;
+
 // z_i = y_i - sum_j L_ij z_j, w_i = z_i / D_i (scr.cpp:1038-1049)
 __global__ __launch_bounds__(TPB) void k_ildl_fwd(int cnt, const int *__restrict__ rows,
                                                   const int *__restrict__ adr,
@@ -99,18 +105,31 @@ __global__ __launch_bounds__(TPB) void k_ildl_fwd(int cnt, const int *__restrict
   const int i = rows[t];
   const int b = adr[i], uj = adr[i + 1] - 1;
   double s = 0.0;
-  for (int j = b; j < uj; j++) s += inc[j] * z[ci[j]];
+  for (int j0 = b; j0 < uj; j0 += ILDL_U) {
+    int c[ILDL_U];
+    double l[ILDL_U], v[ILDL_U];
+#pragma unroll
+    for (int u = 0; u < ILDL_U; u++) c[u] = j0 + u < uj ? ci[j0 + u] : i;
+#pragma unroll
+    for (int u = 0; u < ILDL_U; u++) {
+      l[u] = j0 + u < uj ? inc[j0 + u] : 0.0;
+      v[u] = j0 + u < uj ? z[c[u]] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < ILDL_U; u++)
+      if (j0 + u < uj) s += l[u] * v[u];
+  }
   const double zi = y[i] - s;
   z[i] = zi;
   w[i] = zi / inc[uj];
 }
 
-// h_k = w_k - L_ik h_i over i > k in descending i (scr.cpp:1051-1058)
+// h_k = w_k - L_ik h_i over i > k in descending i (scr.cpp:1051-1058); tval holds
+// L_ik in that order
 __global__ __launch_bounds__(TPB) void k_ildl_bwd(int cnt, const int *__restrict__ rows,
                                                   const int *__restrict__ tptr,
                                                   const int *__restrict__ trow,
-                                                  const int *__restrict__ tslot,
-                                                  const double *__restrict__ inc,
+                                                  const double *__restrict__ tval,
                                                   const double *__restrict__ w,
                                                   double *__restrict__ h, const int *done)
 {
@@ -119,9 +138,31 @@ __global__ __launch_bounds__(TPB) void k_ildl_bwd(int cnt, const int *__restrict
   const int t = blockIdx.x * TPB + threadIdx.x;
   if (t >= cnt) return;
   const int k = rows[t];
+  const int e0 = tptr[k], e1 = tptr[k + 1];
   double v = w[k];
-  for (int e = tptr[k]; e < tptr[k + 1]; e++) v -= inc[tslot[e]] * h[trow[e]];
+  for (int j0 = e0; j0 < e1; j0 += ILDL_U) {
+    int c[ILDL_U];
+    double l[ILDL_U], x[ILDL_U];
+#pragma unroll
+    for (int u = 0; u < ILDL_U; u++) c[u] = j0 + u < e1 ? trow[j0 + u] : k;
+#pragma unroll
+    for (int u = 0; u < ILDL_U; u++) {
+      l[u] = j0 + u < e1 ? tval[j0 + u] : 0.0;
+      x[u] = j0 + u < e1 ? h[c[u]] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < ILDL_U; u++)
+      if (j0 + u < e1) v -= l[u] * x[u];
+  }
   h[k] = v;
+}
+
+__global__ __launch_bounds__(TPB) void k_ildl_tval(int64_t m, const int *__restrict__ tslot,
+                                                   const double *__restrict__ inc,
+                                                   double *__restrict__ tval)
+{
+  const int64_t e = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (e < m) tval[e] = inc[tslot[e]];
 }
 
 unsigned grid_of(int cnt) { return (unsigned)((cnt + TPB - 1) / TPB); }
@@ -133,7 +174,7 @@ void ildl_free(IldlPlan *P, hipStream_t s)
   if (!P) return;
   if (P->graph) (void)hipGraphExecDestroy(P->graph);
   dfree(P->adr, s); dfree(P->ci, s); dfree(P->frows, s); dfree(P->brows, s);
-  dfree(P->tptr, s); dfree(P->trow, s); dfree(P->tslot, s);
+  dfree(P->tptr, s); dfree(P->trow, s); dfree(P->tslot, s); dfree(P->tval, s);
   dfree(P->inc, s); dfree(P->z, s); dfree(P->w, s);
   delete P;
 }
@@ -212,6 +253,7 @@ static int ildl_prepare(ssf_matrix *m)
     up(&P->trow, trow);
     up(&P->tslot, tslot);
     c(dalloc(&P->inc, negm, s));
+    c(dalloc(&P->tval, std::max<int64_t>(1, negm - n), s));
     c(dalloc(&P->z, n, s));
     c(dalloc(&P->w, n, s));
     if (e == hipSuccess) c(hipStreamSynchronize(s));  // the host vectors go out of scope
@@ -230,6 +272,10 @@ static int ildl_prepare(ssf_matrix *m)
       hipLaunchKernelGGL(k_ildl_factor, dim3(grid_of(cnt)), dim3(TPB), 0, s, cnt,
                          P->frows + P->flev[l], P->adr, P->ci, m->a_ref, P->inc);
     }
+    const int64_t nl = P->negm - P->n;  // entries below the diagonal
+    if (nl > 0)
+      hipLaunchKernelGGL(k_ildl_tval, dim3((unsigned)((nl + TPB - 1) / TPB)), dim3(TPB), 0, s, nl,
+                         P->tslot, P->inc, P->tval);
     SSF_HIP(hipGetLastError());
     P->version = m->values_version;
   }
@@ -248,7 +294,7 @@ static hipError_t ildl_sweeps(const IldlPlan *P, const double *y, double *h, con
   for (size_t l = 0; l + 1 < P->blev.size(); l++) {
     const int cnt = P->blev[l + 1] - P->blev[l];
     hipLaunchKernelGGL(k_ildl_bwd, dim3(grid_of(cnt)), dim3(TPB), 0, s, cnt,
-                       P->brows + P->blev[l], P->tptr, P->trow, P->tslot, P->inc, P->w, h, done);
+                       P->brows + P->blev[l], P->tptr, P->trow, P->tval, P->w, h, done);
   }
   return hipGetLastError();
 }
@@ -294,6 +340,43 @@ int ildl_apply_dev(ssf_matrix *m, const double *y, double *h)
   int rc = ildl_prepare(m);
   if (rc) return rc;
   SSF_HIP(ildl_sweeps(m->ildl, y, h, nullptr, m->ctx->stream));
+  return SSF_OK;
+}
+
+int ildl_bench(ssf_matrix *m, int reps, int graph, int64_t *nfwd, int64_t *nbwd, double *usec)
+{
+  int rc = ildl_prepare(m);
+  if (rc) return rc;
+  IldlPlan *P = m->ildl;
+  hipStream_t s = m->ctx->stream;
+  *nfwd = (int64_t)P->flev.size() - 1;
+  *nbwd = (int64_t)P->blev.size() - 1;
+  double *y = nullptr, *h = nullptr;
+  hipEvent_t a = nullptr, b = nullptr;
+  hipError_t e = dalloc(&y, P->n, s);
+  if (e == hipSuccess) e = dalloc(&h, P->n, s);
+  if (e == hipSuccess) e = hipMemsetAsync(y, 0, P->n * sizeof(double), s);
+  if (e == hipSuccess) e = hipEventCreate(&a);
+  if (e == hipSuccess) e = hipEventCreate(&b);
+  auto once = [&]() { return graph ? ildl_apply(P, y, h, nullptr, s) : ildl_sweeps(P, y, h, nullptr, s); };
+  if (e == hipSuccess) e = once();  // captures the graph
+  if (e == hipSuccess) e = hipEventRecord(a, s);
+  for (int r = 0; r < reps && e == hipSuccess; r++) e = once();
+  if (e == hipSuccess) e = hipEventRecord(b, s);
+  if (e == hipSuccess) e = hipEventSynchronize(b);
+  float ms = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+  *usec = reps > 0 ? 1000.0 * ms / reps : 0.0;
+  // the graph holds these pointers: drop it with them
+  if (P->graph) (void)hipGraphExecDestroy(P->graph);
+  P->graph = nullptr;
+  P->gy = nullptr;
+  P->gh = nullptr;
+  dfree(y, s);
+  dfree(h, s);
+  if (a) (void)hipEventDestroy(a);
+  if (b) (void)hipEventDestroy(b);
+  SSF_HIP(e);
   return SSF_OK;
 }
 

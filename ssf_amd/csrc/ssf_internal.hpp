@@ -320,5 +320,6 @@ int solve_ildl(ssf_matrix *m, const ssf_solve_params *p, const double *y, double
                ssf_solve_info *info);
 int ildl_factor_copy(ssf_matrix *m, double *out);                 // factor -> host
 int ildl_apply_dev(ssf_matrix *m, const double *y, double *h);    // h = (LDL^T)^-1 y
+int ildl_bench(ssf_matrix *m, int reps, int graph, int64_t *nfwd, int64_t *nbwd, double *usec);
 void ildl_free(IldlPlan *P, hipStream_t s);
 }  // namespace ssf

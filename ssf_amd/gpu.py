@@ -96,6 +96,8 @@ SIGNATURES = {
                                 ctypes.POINTER(SolveInfo)]),
     "ssf_matrix_ildl_factor": (_I, [_P, _P]),
     "ssf_ildl_apply_dev": (_I, [_P, _P, _P]),
+    "ssf_ildl_bench": (_I, [_P, _I, _I, ctypes.POINTER(_L), ctypes.POINTER(_L),
+                            ctypes.POINTER(_D)]),
     "ssf_np_step_dform": (_I, [_P, _P, _P, _D, _D, ctypes.POINTER(SolveParams), _P, _P, _P,
                                ctypes.POINTER(SolveInfo)]),
     "ssf_elements_create": (_I, [_P, _I, _L, _L, _P, _L, _P, _P, _P, _P, _P, ctypes.POINTER(_P)]),
@@ -324,6 +326,13 @@ class Matrix:
         """prec_ildl_scr on device vectors: x = (L D L^T)^-1 y."""
         _check(lib().ssf_ildl_apply_dev(self.h, ctypes.c_void_p(y_ptr), ctypes.c_void_p(x_ptr)))
 
+    def ildl_bench(self, reps=5, graph=True):
+        """(forward levels, backward levels, mean usec per application)."""
+        nf, nb, us = _L(), _L(), _D()
+        _check(lib().ssf_ildl_bench(self.h, reps, 1 if graph else 0, ctypes.byref(nf),
+                                    ctypes.byref(nb), ctypes.byref(us)))
+        return nf.value, nb.value, us.value
+
     def solve_dev(self, rhs_ptr, x_ptr, ni=100000, res=1e-20, prec=PREC_DIAG, precision=FP64,
                   zero=1e-30, iv=0, inner_res=1e-4, inner_ni=0, max_outer=0,
                   linsolver=SOLVER_CG):
@@ -439,6 +448,13 @@ class Batch:
     def ildl_apply_dev(self, y_ptr, x_ptr):
         """prec_ildl_scr on device vectors: x = (L D L^T)^-1 y."""
         _check(lib().ssf_ildl_apply_dev(self.h, ctypes.c_void_p(y_ptr), ctypes.c_void_p(x_ptr)))
+
+    def ildl_bench(self, reps=5, graph=True):
+        """(forward levels, backward levels, mean usec per application)."""
+        nf, nb, us = _L(), _L(), _D()
+        _check(lib().ssf_ildl_bench(self.h, reps, 1 if graph else 0, ctypes.byref(nf),
+                                    ctypes.byref(nb), ctypes.byref(us)))
+        return nf.value, nb.value, us.value
 
     def solve_dev(self, rhs_ptr, x_ptr, ni=100000, res=1e-20, precision=FP64, zero=1e-30, iv=0,
                   inner_res=1e-4, inner_ni=0):
