@@ -337,6 +337,58 @@ int ssf_elements_volume_load(ssf_elements *el, const double *nodval, double *rhs
 /* same on device pointers, asynchronous on the context's stream */
 int ssf_elements_volume_load_dev(ssf_elements *el, const double *nodval_dev, double *rhs_dev);
 
+/* ----------------------------------------------------------- Newton-Raphson */
+/* nonlinman fields of the Newton-Raphson load control (MEFEL/SRC/nonlinman.h) */
+typedef struct ssf_nr_params {
+  int64_t ninr;          /* maximum number of increments                         */
+  int64_t niilnr;        /* maximum number of inner iterations per increment     */
+  int64_t li;            /* index of the first increment (restart), 0 normally   */
+  double errnr;          /* required norm of the unbalanced forces               */
+  double incrnr;         /* initial load increment dlambda                       */
+  double minincrnr;      /* minimum dlambda                                      */
+  double maxincrnr;      /* maximum dlambda                                      */
+  double ilambda;        /* initial load coefficient                             */
+  double lambdar;        /* required load coefficient when check_lambdar = 1     */
+  int32_t rnormtnr;      /* resnormt: 1 rel_load_norm, 4 absol_norm               */
+  int32_t check_lambdar; /* flagsw: 1 stop at lambdar                            */
+  int32_t check_div;     /* must be 0: the divergence check is not on this path  */
+  int32_t pad;
+} ssf_nr_params;
+
+typedef struct ssf_nr_info {
+  double lambda;         /* attained load coefficient (gnewton_raphson's result) */
+  double norf;           /* last norm of the unbalanced forces                   */
+  int64_t increments;    /* increments run                                       */
+  int64_t solves;        /* linear solves                                        */
+  int64_t rejected;      /* increments rejected (dlambda halved)                 */
+  int64_t inner;         /* inner iterations                                     */
+  int32_t status;        /* 1 ended (ninr or lambdar), 2 stopped: no equilibrium
+                            at the minimum increment                            */
+  int32_t pad;
+} ssf_nr_info;
+
+/* gnewton_raphson (MEFEL/SRC/newtonraph.cpp:51-268) for linear elastic element
+ * sets (hex8 elasticity, quad4 plane stress) on device vectors. Per increment:
+ * fa += dlambda fp; fi = the internal forces of ra (ssf_elements, as
+ * loc_internal_forces); fb = fa - fi; K dr = fb by ssf_solve_dev with sp;
+ * ra += dr; the residual norm (1: |fb| / |fa| when |fa| > 1e-9, 4: |fb|);
+ * accepted below errnr, else up to niilnr inner corrections; a rejected
+ * increment restores fa and ra and halves dlambda (not below minincrnr; at
+ * minincrnr the loop stops); dlambda doubles after every second increment
+ * accepted without inner iterations, capped at maxincrnr; with check_lambdar
+ * the last increment is cut to reach lambdar and the loop ends there. With
+ * ilambda = 0, fa = fc + 0 fp first (assemble_attained_load_vector).
+ * K is the stiffness matrix of el's mesh (the linear elastic tangent at every
+ * state); pv_dev[npv] are the prescribed values as in ssf_elements_results_dev.
+ * fa_dev and ra_dev are updated. Histories (may be NULL, ninr - li entries):
+ * lambda after each increment, its dlambda, its linear solves, accepted (1) or
+ * rejected (0). Synchronous. */
+int ssf_newton_raphson(ssf_matrix *K, ssf_elements *el, const ssf_nr_params *nr,
+                       const ssf_solve_params *sp, const double *pv_dev, int64_t npv,
+                       const double *fc_dev, const double *fp_dev, double *fa_dev,
+                       double *ra_dev, double *lam_hist, double *dlam_hist, int64_t *nsol_hist,
+                       int32_t *acc_hist, ssf_nr_info *info);
+
 /* ------------------------------------------------------------ diagnostics */
 /* Time `reps` calls of ssf_elements_results_dev: usec[0] = mean duration of the
  * element kernel (its own start and end events), usec[1] = mean duration of the

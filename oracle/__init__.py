@@ -44,6 +44,10 @@ def lib():
             "cr_mxv_scr": (None, [_L, _P, _P, _P, _P, _P]),
             "cr_mxv_cr": (None, [_L, _P, _P, _P, _P, _P]),
             "cr_mtxv_cr": (None, [_L, _P, _P, _P, _P, _P]),
+            "cr_newton_raphson": (ctypes.c_double, [_L, _P, _P, _P, ctypes.c_int, _L, _P, _P, _P,
+                                                    _P, _P, _P, _P, _P, _P, _P, _P, _L, _D, _L,
+                                                    _L, _D, _D, _D, _D, ctypes.c_int, ctypes.c_int,
+                                                    _D, _L, _D, _D, _P, _P, _P, _P, _P]),
             "cr_scr_incomplete_ldl": (None, [_L, _P, _P, _P, _D, _P]),
             "cr_scr_prec_ildl": (None, [_L, _P, _P, _P, _P, _P]),
             "cr_scr_cg_prec_ildl": (ctypes.c_int, [_L, _P, _P, _P, _P, _P, _P, _L, _D, _P, _P,
@@ -283,6 +287,35 @@ def scr_cg_prec_ildl(adr, ci, a, y, ni, res, zero=1e-30, incdec=None, x0=None):
                                    _ptr(f64(y)), ni, res, ctypes.byref(ani), ctypes.byref(ares),
                                    zero, 0 if x0 is None else 1)
     return x, ani.value, ares.value, st
+
+
+def newton_raphson(prob, adr, ci, a, fp, fc=None, ilambda=0.0, li=0, ninr=50, niilnr=10,
+                   errnr=1e-6, incrnr=0.1, minincrnr=1e-3, maxincrnr=1.0, normt=1,
+                   check_lambdar=False, lambdar=1.0, sni=20000, sres=1e-24, szero=1e-30,
+                   ra=None, fa=None):
+    """gnewton_raphson with linear elastic internal forces and Jacobi cg_prec solves.
+    Returns (lambda, ra, fa, hist) with hist = dict(lam, dlam, nsol, acc) over the
+    increments run."""
+    n = prob.n
+    fc = np.zeros(n) if fc is None else f64(fc)
+    ra = np.zeros(n) if ra is None else f64(ra).copy()
+    fa = np.zeros(n) if fa is None else f64(fa).copy()
+    m = ninr - li
+    lam, dlam = np.zeros(m), np.zeros(m)
+    nsol, acc = np.zeros(m, np.int64), np.zeros(m, np.int64)
+    counts = np.zeros(2, np.int64)
+    pv = f64(prob.pv) if prob.pv.size else np.zeros(1)
+    lam_end = lib().cr_newton_raphson(
+        n, _ptr(adr), _ptr(ci), _ptr(f64(a)), prob.kind, prob.ne, _ptr(i64(prob.conn)),
+        _ptr(f64(prob.coords)), _ptr(i64(prob.elem_codes())), _ptr(f64(prob.e)),
+        _ptr(f64(prob.nu)), _ptr(f64(prob.thick)), _ptr(pv), _ptr(fc), _ptr(f64(fp)), _ptr(fa),
+        _ptr(ra), li, ilambda, ninr, niilnr, errnr, incrnr, minincrnr, maxincrnr, normt,
+        1 if check_lambdar else 0, lambdar, sni, sres, szero, _ptr(lam), _ptr(dlam), _ptr(nsol),
+        _ptr(acc), _ptr(counts))
+    k = int(counts[0])
+    hist = {"lam": lam[:k], "dlam": dlam[:k], "nsol": nsol[:k], "acc": acc[:k],
+            "solves": int(counts[1])}
+    return lam_end, ra, fa, hist
 
 
 def cr_bicg(adr, ci, a, y, ni, res, zero=1e-30, x0=None):

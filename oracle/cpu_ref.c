@@ -1189,6 +1189,141 @@ void cr_elem_results_mesh(int kind, long ne, const long *conn, const double *xyz
   }
 }
 
+/* ================================================ Newton-Raphson (load control) === */
+
+static double normv(const double *a, long n)  /* GEFEL/vector.cpp:2282 */
+{
+  double norm = 0.0;
+  for (long i = 0; i < n; i++) norm += a[i] * a[i];
+  return sqrt(norm);
+}
+
+/* MEFEL/SRC/newtonraph.cpp:866-905 compute_res_norm_nr, norms 1 (rel_load_norm)
+ * and 4 (absol_norm) */
+static double res_norm_nr(int normt, const double *fb, const double *fa, long n)
+{
+  double norf = normv(fb, n);
+  if (normt == 1) {
+    double norfa = normv(fa, n);
+    if (norfa > 1.0e-9) norf /= norfa;
+  }
+  return norf;
+}
+
+/* MEFEL/SRC/newtonraph.cpp:51-268 gnewton_raphson for linear elastic elements:
+ *   internal_forces = cr_elem_results_mesh (loc_internal_forces, element order),
+ *   the stiffness matrix a (symcomprow) is the same at every assembly (linear
+ *   elastic tangent), solve_system = cr_scr_cg_prec (Jacobi, iv = 0),
+ *   assemble_attained_load_vector (MEFEL/SRC/nssolver.cpp:312) for lambda = 0,
+ *   the divergence check off (nonlinman default), norms 1 or 4.
+ * Per increment k = i - li: lam[k] = lambda after it, dlam[k] = the dlambda it
+ * tried, nsol[k] = linear solves, acc[k] = 1 accepted / 0 rejected (ninr - li
+ * entries each). counts = {increments run, total solves}. Returns lambda. */
+double cr_newton_raphson(long n, const long *adr, const long *ci, const double *a, int kind,
+                         long ne, const long *conn, const double *xyz, const long *cn,
+                         const double *ma, const double *mb, const double *th,
+                         const double *pv, const double *fc, const double *fp, double *fa,
+                         double *ra, long li, double ilambda, long ninr, long niilnr,
+                         double errnr, double incrnr, double minincrnr, double maxincrnr,
+                         int normt, int check_lambdar, double lambdar, long sni, double sres,
+                         double szero, double *lam, double *dlam, long *nsol, long *acc,
+                         long *counts)
+{
+  long i, j, modif = 0, solves = 0, ani;
+  double lambda = ilambda, blambda, dlambda = incrnr, norf = 0.0, ares;
+  int check_rv_fp = check_lambdar;
+  double *rb = (double *)calloc((size_t)n, sizeof(double));
+  double *fb = (double *)calloc((size_t)n, sizeof(double));
+  double *fi = (double *)calloc((size_t)n, sizeof(double));
+  double *dr = (double *)calloc((size_t)n, sizeof(double));
+  if (lambda == 0.0) {
+    double aux = lambda;
+    if (check_lambdar && lambda > lambdar) aux = lambdar;
+    for (long k = 0; k < n; k++) fa[k] = fc[k] + fp[k] * aux;
+  }
+#define NR_FI() cr_elem_results_mesh(kind, ne, conn, xyz, cn, ma, mb, th, n, ra, pv, NULL, NULL, fi)
+#define NR_SOLVE()                                                              \
+  do {                                                                          \
+    cr_scr_cg_prec(n, adr, ci, a, dr, fb, sni, sres, &ani, &ares, szero, 0);    \
+    solves++;                                                                   \
+  } while (0)
+  for (i = li; i < ninr; i++) {
+    long k = i - li, s0 = solves;
+    memcpy(rb, ra, (size_t)n * sizeof(double));
+    blambda = lambda;
+    if (check_rv_fp && dlambda + lambda > lambdar) {
+      check_rv_fp = 0;
+      dlambda = lambdar - lambda;
+    }
+    dlam[k] = dlambda;
+    for (long q = 0; q < n; q++) fa[q] += fp[q] * dlambda;
+    NR_FI();
+    for (long q = 0; q < n; q++) fb[q] = fa[q] - fi[q];
+    NR_SOLVE();
+    for (long q = 0; q < n; q++) ra[q] += dr[q];
+    NR_FI();
+    for (long q = 0; q < n; q++) fb[q] = fa[q] - fi[q];
+    norf = res_norm_nr(normt, fb, fa, n);
+    if (norf < errnr) {
+      lambda += dlambda;
+      lam[k] = lambda;
+      nsol[k] = solves - s0;
+      acc[k] = 1;
+      modif++;
+      if (modif > 1) {
+        dlambda *= 2.0;
+        if (dlambda > maxincrnr) dlambda = maxincrnr;
+        modif = 0;
+      }
+      if (check_lambdar && !check_rv_fp) {
+        i++;
+        break;
+      }
+      continue;
+    }
+    for (j = 0; j < niilnr; j++) {
+      NR_SOLVE();
+      for (long q = 0; q < n; q++) ra[q] += dr[q];
+      NR_FI();
+      for (long q = 0; q < n; q++) fb[q] = fa[q] - fi[q];
+      norf = res_norm_nr(normt, fb, fa, n);
+      if (norf < errnr) break;
+    }
+    modif = 0;
+    nsol[k] = solves - s0;
+    if (j == niilnr || norf > errnr) {
+      for (long q = 0; q < n; q++) fa[q] += fp[q] * -dlambda;
+      memcpy(ra, rb, (size_t)n * sizeof(double));
+      lambda = blambda;
+      lam[k] = lambda;
+      acc[k] = 0;
+      if (dlambda == minincrnr) {
+        i++;
+        break;
+      }
+      dlambda /= 2.0;
+      if (dlambda < minincrnr) dlambda = minincrnr;
+    } else {
+      lambda += dlambda;
+      lam[k] = lambda;
+      acc[k] = 1;
+      if (check_lambdar && !check_rv_fp) {
+        i++;
+        break;
+      }
+    }
+  }
+#undef NR_FI
+#undef NR_SOLVE
+  counts[0] = i - li;
+  counts[1] = solves;
+  free(rb);
+  free(fb);
+  free(fi);
+  free(dr);
+  return lambda;
+}
+
 /* ====================================== nonstationary heat (one linear step) === */
 
 /* TRFEL/SRC/linhext.cpp:979-1029 capacity_matrix: cm = sum over the 2x2x2 points of

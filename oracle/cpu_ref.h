@@ -45,6 +45,15 @@
  *                         transpose identity b.(A^T c) = (A b).c; on a symmetric
  *                         matrix BiCG reaches the CG solution; on small dense
  *                         nonsymmetric systems the solution of a direct solve
+ *   cr_newton_raphson     parity unpinned against the reference itself (gnewton_raphson
+ *                         needs MEFEL's global problem objects, which the GEFEL-only
+ *                         harness does not build); pinned by closed forms: the
+ *                         attained displacements equal the direct solution at lambda
+ *                         (linear elasticity, superposition of fc and fp), the
+ *                         increment sequence (dlambda doubled after every second
+ *                         accepted increment, capped at maxincrnr, cut at lambdar),
+ *                         rejection and halving down to minincrnr when errnr is
+ *                         below the solver's reach, the rel_load / absolute norms
  *   cr_np_step_linear, cr_np_step_dform
  *                         bitwise vs oracle/_ref (the reference's own symcomprow
  *                         mxv_scr / scalmat_scr / addmat_scr / cg_prec); amplification
@@ -191,6 +200,15 @@ void cr_mtxv_cr(long n, const long *adr, const long *ci, const double *a,
 int cr_cr_bicg(long n, const long *adr, const long *ci, const double *a, double *x,
                const double *y, long ni, double res, long *ani, double *ares, double zero,
                long iv);
+double cr_newton_raphson(long n, const long *adr, const long *ci, const double *a, int kind,
+                         long ne, const long *conn, const double *xyz, const long *cn,
+                         const double *ma, const double *mb, const double *th,
+                         const double *pv, const double *fc, const double *fp, double *fa,
+                         double *ra, long li, double ilambda, long ninr, long niilnr,
+                         double errnr, double incrnr, double minincrnr, double maxincrnr,
+                         int normt, int check_lambdar, double lambdar, long sni, double sres,
+                         double szero, double *lam, double *dlam, long *nsol, long *acc,
+                         long *counts);
 int cr_np_step_linear(long n, const long *adr, const long *ci, const double *aK,
                       const double *aC, double *aA, double alpha, double dt,
                       const double *rhs, double *lhs, double *tdlhs, double *d, double *p,

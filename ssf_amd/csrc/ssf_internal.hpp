@@ -322,4 +322,6 @@ int ildl_factor_copy(ssf_matrix *m, double *out);                 // factor -> h
 int ildl_apply_dev(ssf_matrix *m, const double *y, double *h);    // h = (LDL^T)^-1 y
 int ildl_bench(ssf_matrix *m, int reps, int graph, int64_t *nfwd, int64_t *nbwd, double *usec);
 void ildl_free(IldlPlan *P, hipStream_t s);
+// context, element kind and number of unknowns of an element set (ssf_post.hip)
+void elements_shape(const ssf_elements *el, ssf_ctx **ctx, int *kind, int64_t *n);
 }  // namespace ssf

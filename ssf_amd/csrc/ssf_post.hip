@@ -575,6 +575,13 @@ struct ssf_elements {
 
 using namespace ssf;
 
+void ssf::elements_shape(const ssf_elements *el, ssf_ctx **ctx, int *kind, int64_t *n)
+{
+  *ctx = el->ctx;
+  *kind = el->kind;
+  *n = el->n;
+}
+
 static void post_shape(int kind, int &nne, int &nd, int &ngp, int &nc)
 {
   nne = kind == SSF_ELEM_QUAD4_PSTRESS ? 4 : 8;

@@ -49,6 +49,22 @@ class SolveInfo(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
 
 
+class NrParams(ctypes.Structure):
+    _fields_ = [("ninr", ctypes.c_int64), ("niilnr", ctypes.c_int64), ("li", ctypes.c_int64),
+                ("errnr", ctypes.c_double), ("incrnr", ctypes.c_double),
+                ("minincrnr", ctypes.c_double), ("maxincrnr", ctypes.c_double),
+                ("ilambda", ctypes.c_double), ("lambdar", ctypes.c_double),
+                ("rnormtnr", ctypes.c_int32), ("check_lambdar", ctypes.c_int32),
+                ("check_div", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+
+class NrInfo(ctypes.Structure):
+    _fields_ = [("lambda_", ctypes.c_double), ("norf", ctypes.c_double),
+                ("increments", ctypes.c_int64), ("solves", ctypes.c_int64),
+                ("rejected", ctypes.c_int64), ("inner", ctypes.c_int64),
+                ("status", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+
 SIGNATURES = {
     "ssf_last_error": (ctypes.c_char_p, []),
     "ssf_ctx_create": (_I, [_I, ctypes.POINTER(_P)]),
@@ -94,6 +110,8 @@ SIGNATURES = {
     "ssf_matrix_add": (_I, [_P, _D, _P]),
     "ssf_np_step_linear": (_I, [_P, _P, _P, _D, _D, ctypes.POINTER(SolveParams), _P, _P, _P,
                                 ctypes.POINTER(SolveInfo)]),
+    "ssf_newton_raphson": (_I, [_P, _P, ctypes.POINTER(NrParams), ctypes.POINTER(SolveParams),
+                                _P, _L, _P, _P, _P, _P, _P, _P, _P, _P, ctypes.POINTER(NrInfo)]),
     "ssf_matrix_ildl_factor": (_I, [_P, _P]),
     "ssf_ildl_apply_dev": (_I, [_P, _P, _P]),
     "ssf_ildl_bench": (_I, [_P, _I, _I, ctypes.POINTER(_L), ctypes.POINTER(_L),
@@ -377,6 +395,28 @@ def np_step_linear(K: Matrix, C: Matrix, A: Matrix, alpha: float, dt: float, rhs
     _check(getattr(lib(), fn)(K.h, C.h, A.h, float(alpha), float(dt), ctypes.byref(p),
                               rhs_ptr, lhs_ptr, tdlhs_ptr, ctypes.byref(info)))
     return info
+
+
+def newton_raphson(K: Matrix, el: "Elements", fp_ptr, fa_ptr, ra_ptr, fc_ptr, ninr=50, niilnr=10,
+                   errnr=1e-6, incrnr=0.1, minincrnr=1e-3, maxincrnr=1.0, ilambda=0.0, li=0,
+                   normt=1, check_lambdar=False, lambdar=1.0, ni=20000, res=1e-24,
+                   prec=PREC_DIAG, pv_ptr=None, npv=0):
+    """MEFEL's gnewton_raphson on device vectors (fp, fc read; fa, ra updated).
+    Returns (info, hist) with hist = dict(lam, dlam, nsol, acc) over the increments run."""
+    p = NrParams(ninr, niilnr, li, errnr, incrnr, minincrnr, maxincrnr, ilambda, lambdar, normt,
+                 1 if check_lambdar else 0, 0, 0)
+    sp = SolveParams(SOLVER_CG, prec, ni, res, 1e-30, 0, FP64, 0.0, 0, 0)
+    m = ninr - li
+    lam, dlam = np.zeros(m), np.zeros(m)
+    nsol, acc = np.zeros(m, np.int64), np.zeros(m, np.int32)
+    info = NrInfo()
+    _check(lib().ssf_newton_raphson(K.h, el.h, ctypes.byref(p), ctypes.byref(sp),
+                                    ctypes.c_void_p(pv_ptr), npv, ctypes.c_void_p(fc_ptr),
+                                    ctypes.c_void_p(fp_ptr), ctypes.c_void_p(fa_ptr),
+                                    ctypes.c_void_p(ra_ptr), _p(lam), _p(dlam), _p(nsol), _p(acc),
+                                    ctypes.byref(info)))
+    k = int(info.increments)
+    return info, {"lam": lam[:k], "dlam": dlam[:k], "nsol": nsol[:k], "acc": acc[:k]}
 
 
 class Batch:
