@@ -45,7 +45,7 @@
  *                         transpose identity b.(A^T c) = (A b).c; on a symmetric
  *                         matrix BiCG reaches the CG solution; on small dense
  *                         nonsymmetric systems the solution of a direct solve
- *   cr_newton_raphson     parity unpinned against the reference itself (gnewton_raphson
+ *   cr_newton_raphson     not run against the reference itself (gnewton_raphson
  *                         needs MEFEL's global problem objects, which the GEFEL-only
  *                         harness does not build); pinned by closed forms: the
  *                         attained displacements equal the direct solution at lambda
