@@ -336,6 +336,21 @@ int ssf_elements_results_dev(ssf_elements *el, const double *lhs_dev, const doub
 int ssf_elements_volume_load(ssf_elements *el, const double *nodval, double *rhs);
 /* same on device pointers, asynchronous on the context's stream */
 int ssf_elements_volume_load_dev(ssf_elements *el, const double *nodval_dev, double *rhs_dev);
+/* Surface loads of hex8 elasticity element records (loadel::surfaceload ->
+ * linhex::node_forces_surf, MEFEL/SRC/loadel.cpp:549, MEFEL/SRC/linhex.cpp:
+ * 2543-3047): record r loads element eid[r] (0-based) on the faces f with
+ * ls[6r + f] > 0 (1: nodal values in global components, 2: in the face's local
+ * axes of linhex::locglob_nodeval), nodvals[72r + 12f + 3m + c] = component c at
+ * the face's m-th node. Faces in the reference's order: xi = 1, eta = 1, xi = -1,
+ * eta = -1, zeta = 1, zeta = -1. The nodal forces are added to rhs by locglob
+ * record after record (loadcase::assemble, MEFEL/SRC/loadcase.cpp:572-597), bit
+ * for bit the reference's arithmetic. Host arrays, rhs in / out, synchronous; the
+ * _dev form takes device arrays (int32 ids and flags). */
+int ssf_elements_surface_load(ssf_elements *el, int64_t nrec, const int64_t *eid,
+                              const int64_t *ls, const double *nodvals, double *rhs);
+int ssf_elements_surface_load_dev(ssf_elements *el, int64_t nrec, const int32_t *eid_dev,
+                                  const int32_t *ls_dev, const double *nodvals_dev,
+                                  double *rhs_dev);
 
 /* ----------------------------------------------------------- Newton-Raphson */
 /* nonlinman fields of the Newton-Raphson load control (MEFEL/SRC/nonlinman.h) */

@@ -44,6 +44,8 @@ def lib():
             "cr_mxv_scr": (None, [_L, _P, _P, _P, _P, _P]),
             "cr_mxv_cr": (None, [_L, _P, _P, _P, _P, _P]),
             "cr_mtxv_cr": (None, [_L, _P, _P, _P, _P, _P]),
+            "cr_hex8_surface_load": (None, [_P, _P, _P, _P, _P, _P]),
+            "cr_surface_load_mesh": (None, [_L, _P, _P, _P, _P, _P, _P, _P]),
             "cr_newton_raphson": (ctypes.c_double, [_L, _P, _P, _P, ctypes.c_int, _L, _P, _P, _P,
                                                     _P, _P, _P, _P, _P, _P, _P, _P, _L, _D, _L,
                                                     _L, _D, _D, _D, _D, ctypes.c_int, ctypes.c_int,
@@ -112,6 +114,7 @@ def ref():
             "ref_cr_localized": (None, [_P, _P, _P, _L]),
             "ref_cr_mxv": (None, [_P, _P, _P]),
             "ref_cr_mtxv": (None, [_P, _P, _P]),
+            "ref_hex8_surface_load": (None, [_P, _P, _P, _P, _P, _P]),
             "ref_scr_incomplete_ldl": (None, [_P, _D, _P]),
             "ref_scr_prec_ildl": (None, [_P, _P, _P]),
             "ref_cr_set_a": (None, [_P, _P]),
@@ -431,6 +434,26 @@ def np_step_linear(adr, ci, aK, aC, alpha, dt, rhs, lhs, tdlhs, ni=20000, res=1e
             _ptr(f64(rhs)), _ptr(lhs), _ptr(tdlhs), _ptr(d), _ptr(p), _ptr(f), ni, res, zero,
             ctypes.byref(ani), ctypes.byref(ares))
     return st, ani.value, ares.value, aA
+
+
+def surface_load(prob, eid, ls, nodvals, rhs=None):
+    """Surface loads of hex8 elasticity records (element eid[r], face flags
+    ls[r, 6], nodal values nodvals[r, 72]) added into a copy of rhs (zeros if
+    None) by locglob in record order."""
+    out = np.zeros(prob.n) if rhs is None else f64(rhs).copy()
+    eid, ls, nodvals = i64(eid), i64(ls), f64(nodvals)
+    lib().cr_surface_load_mesh(eid.shape[0], _ptr(eid), _ptr(ls), _ptr(nodvals),
+                               _ptr(i64(prob.conn)), _ptr(f64(prob.coords)),
+                               _ptr(i64(prob.elem_codes())), _ptr(out))
+    return out
+
+
+def hex8_surface_load(x, y, z, ls, nodvals):
+    """nf[24] of one hex8 element (node_forces_surf)."""
+    nf = np.zeros(24)
+    lib().cr_hex8_surface_load(_ptr(f64(x)), _ptr(f64(y)), _ptr(f64(z)), _ptr(i64(ls)),
+                               _ptr(f64(nodvals)), _ptr(nf))
+    return nf
 
 
 def volume_load(prob, nodval, rhs=None):

@@ -1530,6 +1530,172 @@ static void bf_hex(double *n, double xi, double eta, double zeta)
   n[7] = (1.0 + xi) * (1.0 - eta) * (1.0 - zeta) / 8.0;
 }
 
+/* the six faces of the hex8 as linhex::node_forces_surf visits them
+ * (MEFEL/SRC/linhex.cpp:2543-3047): the fixed natural coordinate (0 xi, 1 eta,
+ * 2 zeta) and its value, the coordinate of the outer and of the inner Gauss loop,
+ * and the face's nodes in the order of jac2d_3d (GEFEL/difcalc.cpp:1060-1097),
+ * which is also the order of the face's nodal values; the node pairs spanning the
+ * face's local axes g1 = X[q1] - X[q0], g2 = X[r1] - X[r0] (linhex::locglob_nodeval,
+ * MEFEL/SRC/linhex.cpp:3049-3143) */
+static const int SURF_FIX[6] = {0, 1, 0, 1, 2, 2};
+static const double SURF_VAL[6] = {1.0, 1.0, -1.0, -1.0, 1.0, -1.0};
+static const int SURF_OUT[6] = {1, 0, 1, 0, 0, 0};
+static const int SURF_IN[6] = {2, 2, 2, 2, 1, 1};
+static const int SURF_NODES[6][4] = {{0, 3, 7, 4}, {1, 0, 4, 5}, {2, 1, 5, 6},
+                                     {3, 2, 6, 7}, {0, 1, 2, 3}, {4, 7, 6, 5}};
+static const int SURF_G[6][4] = {{7, 4, 7, 3}, {4, 5, 4, 0}, {5, 6, 5, 1},
+                                 {6, 7, 6, 2}, {2, 3, 2, 1}, {6, 5, 6, 7}};
+
+/* GEFEL/difcalc.cpp:819-866 jac2d3d for a bilinear face: sqrt(g11 g22 - g12^2)
+ * with the derivatives of GEFEL/basefun.cpp:626-649 */
+static double jac2d3d_4(const double *lx, const double *ly, const double *lz, double xi,
+                        double eta)
+{
+  double nu[4], nv[4];
+  nu[0] = 0.25 * (1.0 + eta);
+  nu[1] = 0.25 * (-1.0 - eta);
+  nu[2] = 0.25 * (-1.0 + eta);
+  nu[3] = 0.25 * (1.0 - eta);
+  nv[0] = 0.25 * (1.0 + xi);
+  nv[1] = 0.25 * (1.0 - xi);
+  nv[2] = 0.25 * (-1.0 + xi);
+  nv[3] = 0.25 * (-1.0 - xi);
+  double dxu = 0.0, dyu = 0.0, dzu = 0.0, dxv = 0.0, dyv = 0.0, dzv = 0.0;
+  for (int i = 0; i < 4; i++) {
+    dxu += nu[i] * lx[i];
+    dyu += nu[i] * ly[i];
+    dzu += nu[i] * lz[i];
+    dxv += nv[i] * lx[i];
+    dyv += nv[i] * ly[i];
+    dzv += nv[i] * lz[i];
+  }
+  double g11 = dxu * dxu + dyu * dyu + dzu * dzu;
+  double g22 = dxv * dxv + dyv * dyv + dzv * dzv;
+  double g12 = dxu * dxv + dyu * dyv + dzu * dzv;
+  return sqrt(g11 * g22 - g12 * g12);
+}
+
+/* linhex::locglob_nodeval (MEFEL/SRC/linhex.cpp:3049-3143): face-local nodal
+ * values nv[12] (4 nodes x 3) to global ones tnv[12], t = [g1 g2 g3] */
+static void surf_locglob(int f, const double *x, const double *y, const double *z,
+                         const double *nv, double *tnv)
+{
+  const int *g = SURF_G[f];
+  double g1[3] = {x[g[1]] - x[g[0]], y[g[1]] - y[g[0]], z[g[1]] - z[g[0]]};
+  double g2[3] = {x[g[3]] - x[g[2]], y[g[3]] - y[g[2]], z[g[3]] - z[g[2]]};
+  double g3[3], norm = 0.0, t[3][3];
+  for (int i = 0; i < 3; i++) norm += g1[i] * g1[i];
+  norm = 1.0 / sqrt(norm);
+  for (int i = 0; i < 3; i++) g1[i] = norm * g1[i];
+  norm = 0.0;
+  for (int i = 0; i < 3; i++) norm += g1[i] * g2[i];
+  g2[0] = g2[0] - norm * g1[0];
+  g2[1] = g2[1] - norm * g1[1];
+  g2[2] = g2[2] - norm * g1[2];
+  norm = 0.0;
+  for (int i = 0; i < 3; i++) norm += g2[i] * g2[i];
+  norm = 1.0 / sqrt(norm);
+  for (int i = 0; i < 3; i++) g2[i] = norm * g2[i];
+  g3[0] = g1[1] * g2[2] - g1[2] * g2[1];
+  g3[1] = g1[2] * g2[0] - g1[0] * g2[2];
+  g3[2] = g1[0] * g2[1] - g1[1] * g2[0];
+  for (int r = 0; r < 3; r++) {
+    t[r][0] = g1[r];
+    t[r][1] = g2[r];
+    t[r][2] = g3[r];
+  }
+  for (int m = 0; m < 4; m++)
+    for (int r = 0; r < 3; r++) {
+      double sacc = 0.0;
+      for (int c = 0; c < 3; c++) sacc += t[r][c] * nv[3 * m + c];
+      tnv[3 * m + r] = sacc;
+    }
+}
+
+/* linhex::node_forces_surf (MEFEL/SRC/linhex.cpp:2543-3047): nodal forces nf[24]
+ * (added to) of surface loads on the faces with ls[f] > 0: ls 1 nodal values in
+ * global components, ls 2 in the face's local axes; nodvals[72] = per face 4
+ * nodes x 3 components. am = sum over 2x2 points of N^T N |J_face| (nnj, GEFEL/
+ * matrix.cpp:3149), v = am av (the 24x24 mxv), nf = v + nf (addv); av keeps the
+ * values of earlier faces, as there (they meet zero rows of am). */
+void cr_hex8_surface_load(const double *x, const double *y, const double *z, const long *ls,
+                          const double *nodvals, double *nf)
+{
+  double am[24 * 24], av[24], v[24], n[3 * 24], bf[8], lx[4], ly[4], lz[4], tnv[12];
+  memset(av, 0, sizeof av);
+  for (int f = 0; f < 6; f++) {
+    if (ls[f] <= 0) continue;
+    const int *fn = SURF_NODES[f];
+    for (int m = 0; m < 4; m++) {
+      lx[m] = x[fn[m]];
+      ly[m] = y[fn[m]];
+      lz[m] = z[fn[m]];
+    }
+    memset(am, 0, sizeof am);
+    for (int i = 0; i < 2; i++)
+      for (int j = 0; j < 2; j++) {
+        double c[3];
+        c[SURF_FIX[f]] = SURF_VAL[f];
+        c[SURF_OUT[f]] = GP2[i];
+        c[SURF_IN[f]] = GP2[j];
+        double jac = jac2d3d_4(lx, ly, lz, c[SURF_OUT[f]], c[SURF_IN[f]]);
+        bf_hex(bf, c[0], c[1], c[2]);
+        memset(n, 0, sizeof n);
+        for (int a = 0; a < 8; a++) {
+          n[0 * 24 + 3 * a] = bf[a];
+          n[1 * 24 + 3 * a + 1] = bf[a];
+          n[2 * 24 + 3 * a + 2] = bf[a];
+        }
+        jac = jac * 1.0 * 1.0; /* w[i] w[j] */
+        for (int p = 0; p < 24; p++)
+          for (int q = p; q < 24; q++) {
+            double sacc = 0.0;
+            for (int k = 0; k < 3; k++) sacc += n[k * 24 + p] * n[k * 24 + q];
+            am[p * 24 + q] += sacc * jac;
+            if (q != p) am[q * 24 + p] += sacc * jac;
+          }
+      }
+    const double *fv = nodvals + 12 * f;
+    const double *src = fv;
+    if (ls[f] == 2) {
+      surf_locglob(f, x, y, z, fv, tnv);
+      memset(av, 0, sizeof av);
+      src = tnv;
+    }
+    for (int m = 0; m < 4; m++)
+      for (int c = 0; c < 3; c++) av[3 * fn[m] + c] = src[3 * m + c];
+    for (int p = 0; p < 24; p++) {
+      double sacc = 0.0;
+      for (int q = 0; q < 24; q++) sacc += am[p * 24 + q] * av[q];
+      v[p] = sacc;
+    }
+    for (int p = 0; p < 24; p++) nf[p] = v[p] + nf[p];
+  }
+}
+
+/* loadcase::assemble element loads (MEFEL/SRC/loadcase.cpp:572-597): the surface
+ * loads of nrec records (element eid, ls[6], nodvals[72]) added to rhs by
+ * locglob in record order */
+void cr_surface_load_mesh(long nrec, const long *eid, const long *ls, const double *nodvals,
+                          const long *conn, const double *xyz, const long *cn, double *rhs)
+{
+  double x[8], y[8], z[8], nf[24];
+  for (long r = 0; r < nrec; r++) {
+    const long el = eid[r];
+    for (int a = 0; a < 8; a++) {
+      long nd0 = conn[el * 8 + a];
+      x[a] = xyz[3 * nd0];
+      y[a] = xyz[3 * nd0 + 1];
+      z[a] = xyz[3 * nd0 + 2];
+    }
+    memset(nf, 0, sizeof nf);
+    cr_hex8_surface_load(x, y, z, ls + 6 * r, nodvals + 72 * r, nf);
+    const long *c = cn + el * 24;
+    for (int i = 0; i < 24; i++)
+      if (c[i] >= 1) rhs[c[i] - 1] += nf[i];
+  }
+}
+
 /* nodal load vector of one hex8 element from the nodal values of a volume load:
  *   kind 0 (elasticity): loadel::volumeload (MEFEL/SRC/loadel.cpp:438-450) with
  *     linhex::load_matrix (MEFEL/SRC/linhex.cpp:969-1000): lm = sum N^T N jac

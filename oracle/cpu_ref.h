@@ -45,6 +45,13 @@
  *                         transpose identity b.(A^T c) = (A b).c; on a symmetric
  *                         matrix BiCG reaches the CG solution; on small dense
  *                         nonsymmetric systems the solution of a direct solve
+ *   cr_hex8_surface_load, cr_surface_load_mesh
+ *                         bitwise vs oracle/_ref (GEFEL jac2d_3d, bf_lin_hex_3d, nnj,
+ *                         mxv, addv, scprd, cmulv in the loop order of
+ *                         linhex::node_forces_surf / locglob_nodeval); a uniform
+ *                         pressure on a rectangular face gives pA/4 per node; the
+ *                         local-axes form equals the global form with the rotated
+ *                         values; the loads integrate the traction (total = int t dA)
  *   cr_newton_raphson     not run against the reference itself (gnewton_raphson
  *                         needs MEFEL's global problem objects, which the GEFEL-only
  *                         harness does not build); pinned by closed forms: the
@@ -209,6 +216,10 @@ double cr_newton_raphson(long n, const long *adr, const long *ci, const double *
                          int normt, int check_lambdar, double lambdar, long sni, double sres,
                          double szero, double *lam, double *dlam, long *nsol, long *acc,
                          long *counts);
+void cr_hex8_surface_load(const double *x, const double *y, const double *z, const long *ls,
+                          const double *nodvals, double *nf);
+void cr_surface_load_mesh(long nrec, const long *eid, const long *ls, const double *nodvals,
+                          const long *conn, const double *xyz, const long *cn, double *rhs);
 int cr_np_step_linear(long n, const long *adr, const long *ci, const double *aK,
                       const double *aC, double *aA, double alpha, double dt,
                       const double *rhs, double *lhs, double *tdlhs, double *d, double *p,

@@ -579,4 +579,93 @@ void ref_hex8_volume_load(int kind, const double *xc, const double *yc, const do
   }
 }
 
+// linhex::node_forces_surf (MEFEL/SRC/linhex.cpp:2543-3047) composed from GEFEL's
+// gauss_points, jac2d_3d (with the face index), bf_lin_hex_3d, nnj, mxv and addv in
+// the member's loop order, with linhex::locglob_nodeval (:3049-3143) composed from
+// scprd, cmulv and mxv; the per-face DOF positions are those the member writes
+void ref_hex8_surface_load(const double *xc, const double *yc, const double *zc,
+                           const long *is, const double *nv, double *nf_out)
+{
+  // for face f: the fixed coordinate (0 xi, 1 eta, 2 zeta) and value, the
+  // coordinates of the outer / inner loop, the DOFs of the 4 nodal values, the
+  // node pairs of locglob_nodeval's g1 and g2
+  static const int fix[6] = {0, 1, 0, 1, 2, 2};
+  static const double val[6] = {1.0, 1.0, -1.0, -1.0, 1.0, -1.0};
+  static const int outer[6] = {1, 0, 1, 0, 0, 0}, inner[6] = {2, 2, 2, 2, 1, 1};
+  static const int dof[6][4] = {{0, 9, 21, 12}, {3, 0, 12, 15}, {6, 3, 15, 18},
+                                {9, 6, 18, 21}, {0, 3, 6, 9}, {12, 21, 18, 15}};
+  static const int gn[6][4] = {{4, 7, 3, 7}, {5, 4, 0, 4}, {6, 5, 1, 5},
+                               {7, 6, 2, 6}, {3, 2, 1, 2}, {5, 6, 7, 6}};
+  const long nne = 8, ndofe = 24;
+  vector x(nne), y(nne), z(nne), gp(2), w(2), av(ndofe), v(ndofe), nf(ndofe), bf(nne);
+  matrix n(3, ndofe), am(ndofe, ndofe);
+  double tnv[12];
+  for (long i = 0; i < nne; i++) { x[i] = xc[i]; y[i] = yc[i]; z[i] = zc[i]; }
+  gauss_points(gp.a, w.a, 2);
+  for (int f = 0; f < 6; f++) {
+    if (is[f] <= 0) continue;
+    nullm(am);
+    for (long i = 0; i < 2; i++)
+      for (long j = 0; j < 2; j++) {
+        double c[3], jac;
+        c[fix[f]] = val[f];
+        c[outer[f]] = gp[i];
+        c[inner[f]] = gp[j];
+        jac2d_3d(jac, x, y, z, c[outer[f]], c[inner[f]], f);
+        nullm(n);
+        bf_lin_hex_3d(bf.a, c[0], c[1], c[2]);
+        long jj = 0, kk = 1, ll = 2;
+        for (long a = 0; a < nne; a++) {
+          n[0][jj] = bf[a]; jj += 3;
+          n[1][kk] = bf[a]; kk += 3;
+          n[2][ll] = bf[a]; ll += 3;
+        }
+        jac = jac * w[i] * w[j];
+        nnj(am.a, n.a, jac, n.m, n.n);
+      }
+    const double *fv = nv + 12 * f;
+    if (is[f] == 1) {
+      for (int m = 0; m < 4; m++)
+        for (int k = 0; k < 3; k++) av[dof[f][m] + k] = fv[3 * m + k];
+    }
+    if (is[f] == 2) {
+      for (int q = 0; q < 12; q++) av[q] = fv[q];
+      vector g1(3), g2(3), g3(3);
+      matrix t(3, 3);
+      double norm;
+      g1[0] = x[gn[f][0]] - x[gn[f][1]];
+      g1[1] = y[gn[f][0]] - y[gn[f][1]];
+      g1[2] = z[gn[f][0]] - z[gn[f][1]];
+      g2[0] = x[gn[f][2]] - x[gn[f][3]];
+      g2[1] = y[gn[f][2]] - y[gn[f][3]];
+      g2[2] = z[gn[f][2]] - z[gn[f][3]];
+      scprd(g1, g1, norm);
+      norm = 1.0 / sqrt(norm);
+      cmulv(norm, g1, g1);
+      scprd(g1, g2, norm);
+      g2[0] = g2[0] - norm * g1[0];
+      g2[1] = g2[1] - norm * g1[1];
+      g2[2] = g2[2] - norm * g1[2];
+      scprd(g2, g2, norm);
+      norm = 1.0 / sqrt(norm);
+      cmulv(norm, g2, g2);
+      g3[0] = g1[1] * g2[2] - g1[2] * g2[1];
+      g3[1] = g1[2] * g2[0] - g1[0] * g2[2];
+      g3[2] = g1[0] * g2[1] - g1[1] * g2[0];
+      for (int r = 0; r < 3; r++) {
+        t[r][0] = g1[r];
+        t[r][1] = g2[r];
+        t[r][2] = g3[r];
+      }
+      for (int q = 0; q < 4; q++) mxv(t.a, av.a + 3 * q, tnv + 3 * q, 3, 3);
+      nullv(av);
+      for (int m = 0; m < 4; m++)
+        for (int k = 0; k < 3; k++) av[dof[f][m] + k] = tnv[3 * m + k];
+    }
+    mxv(am, av, v);
+    addv(v, nf, nf);
+  }
+  for (long i = 0; i < ndofe; i++) nf_out[i] = nf[i];
+}
+
 }  // extern "C"

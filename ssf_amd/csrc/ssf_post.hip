@@ -495,6 +495,165 @@ __global__ __launch_bounds__(TPB) void k_vload(int64_t ne, const int *__restrict
     }
 }
 
+// ------------------------------------------------------------ surface loads
+// linhex::node_forces_surf (MEFEL/SRC/linhex.cpp:2543-3047), one thread per load
+// record (element, face flags ls[6], nodal values nv[72] = 6 faces x 4 nodes x 3).
+// Per loaded face: M_ab = sum over 2x2 points of (N_a N_b) |J_face| for the face's
+// four nodes (the nonzero entries of nnj's 24x24 matrix, the same for the three
+// components), the nodal values in global components (ls 1) or rotated from the
+// face's axes (ls 2, locglob_nodeval), v = M av summed in ascending DOF order (the
+// order of mxv's nonzero terms) and nf = v + nf. Other entries of the reference's
+// products are exact zeros and change no sum.
+struct SurfFace {
+  int fix, outer, inner;
+  double val;
+  int node[4];  // jac2d_3d's face order, also the order of the nodal values
+  int asc[4];   // positions in node[] by ascending element node
+  int g[4];     // g1 = X[g[1]] - X[g[0]], g2 = X[g[3]] - X[g[2]]
+};
+__constant__ SurfFace kSurf[6] = {
+    {0, 1, 2, 1.0, {0, 3, 7, 4}, {0, 1, 3, 2}, {7, 4, 7, 3}},
+    {1, 0, 2, 1.0, {1, 0, 4, 5}, {1, 0, 2, 3}, {4, 5, 4, 0}},
+    {0, 1, 2, -1.0, {2, 1, 5, 6}, {1, 0, 2, 3}, {5, 6, 5, 1}},
+    {1, 0, 2, -1.0, {3, 2, 6, 7}, {1, 0, 2, 3}, {6, 7, 6, 2}},
+    {2, 0, 1, 1.0, {0, 1, 2, 3}, {0, 1, 2, 3}, {2, 3, 2, 1}},
+    {2, 0, 1, -1.0, {4, 7, 6, 5}, {0, 3, 2, 1}, {6, 5, 6, 7}},
+};
+
+__device__ double surf_bf(int a, double xi, double eta, double zeta)
+{
+#pragma clang fp contract(off)
+  switch (a) {  // bf_lin_hex_3d (GEFEL/basefun.cpp)
+    case 0: return (1.0 + xi) * (1.0 + eta) * (1.0 + zeta) / 8.0;
+    case 1: return (1.0 - xi) * (1.0 + eta) * (1.0 + zeta) / 8.0;
+    case 2: return (1.0 - xi) * (1.0 - eta) * (1.0 + zeta) / 8.0;
+    case 3: return (1.0 + xi) * (1.0 - eta) * (1.0 + zeta) / 8.0;
+    case 4: return (1.0 + xi) * (1.0 + eta) * (1.0 - zeta) / 8.0;
+    case 5: return (1.0 - xi) * (1.0 + eta) * (1.0 - zeta) / 8.0;
+    case 6: return (1.0 - xi) * (1.0 - eta) * (1.0 - zeta) / 8.0;
+    default: return (1.0 + xi) * (1.0 - eta) * (1.0 - zeta) / 8.0;
+  }
+}
+
+__global__ __launch_bounds__(TPB) void k_sload(int64_t nrec, const int *__restrict__ eid,
+                                               const int *__restrict__ ls,
+                                               const double *__restrict__ nodvals,
+                                               const int *__restrict__ conn,
+                                               const double *__restrict__ xyz,
+                                               double *__restrict__ fe)
+{
+#pragma clang fp contract(off)
+  const int64_t r = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (r >= nrec) return;
+  const int e = eid[r];
+  double x[8], y[8], z[8], nf[8][3];
+#pragma unroll
+  for (int a = 0; a < 8; a++) {
+    const int nd = conn[e * 8 + a];
+    x[a] = xyz[3 * nd];
+    y[a] = xyz[3 * nd + 1];
+    z[a] = xyz[3 * nd + 2];
+    nf[a][0] = nf[a][1] = nf[a][2] = 0.0;
+  }
+  for (int f = 0; f < 6; f++) {
+    const int lsf = ls[r * 6 + f];
+    if (lsf <= 0) continue;
+    const SurfFace &F = kSurf[f];
+    double lx[4], ly[4], lz[4], M[4][4];
+    for (int m = 0; m < 4; m++) {
+      lx[m] = x[F.node[m]];
+      ly[m] = y[F.node[m]];
+      lz[m] = z[F.node[m]];
+      for (int q = 0; q < 4; q++) M[m][q] = 0.0;
+    }
+    for (int i = 0; i < 2; i++)
+      for (int j = 0; j < 2; j++) {
+        double c[3];
+        c[F.fix] = F.val;
+        c[F.outer] = i ? kGP : -kGP;
+        c[F.inner] = j ? kGP : -kGP;
+        // jac2d3d (GEFEL/difcalc.cpp:819-866), derivatives of basefun.cpp:626-649
+        const double u = c[F.outer], w = c[F.inner];
+        const double nu[4] = {0.25 * (1.0 + w), 0.25 * (-1.0 - w), 0.25 * (-1.0 + w),
+                              0.25 * (1.0 - w)};
+        const double nv[4] = {0.25 * (1.0 + u), 0.25 * (1.0 - u), 0.25 * (-1.0 + u),
+                              0.25 * (-1.0 - u)};
+        double dxu = 0.0, dyu = 0.0, dzu = 0.0, dxv = 0.0, dyv = 0.0, dzv = 0.0;
+        for (int m = 0; m < 4; m++) {
+          dxu += nu[m] * lx[m];
+          dyu += nu[m] * ly[m];
+          dzu += nu[m] * lz[m];
+          dxv += nv[m] * lx[m];
+          dyv += nv[m] * ly[m];
+          dzv += nv[m] * lz[m];
+        }
+        const double g11 = dxu * dxu + dyu * dyu + dzu * dzu;
+        const double g22 = dxv * dxv + dyv * dyv + dzv * dzv;
+        const double g12 = dxu * dxv + dyu * dyv + dzu * dzv;
+        const double jac = sqrt(g11 * g22 - g12 * g12) * 1.0 * 1.0;  // * w[i] * w[j]
+        double N[4];
+        for (int m = 0; m < 4; m++) N[m] = surf_bf(F.node[m], c[0], c[1], c[2]);
+        for (int m = 0; m < 4; m++)
+          for (int q = 0; q < 4; q++) M[m][q] += (N[m] * N[q]) * jac;
+      }
+    const double *fv = nodvals + r * 72 + 12 * f;
+    double av[4][3];
+    if (lsf == 2) {
+      // locglob_nodeval (MEFEL/SRC/linhex.cpp:3049-3143)
+      double g1[3] = {x[F.g[1]] - x[F.g[0]], y[F.g[1]] - y[F.g[0]], z[F.g[1]] - z[F.g[0]]};
+      double g2[3] = {x[F.g[3]] - x[F.g[2]], y[F.g[3]] - y[F.g[2]], z[F.g[3]] - z[F.g[2]]};
+      double norm = 0.0;
+      for (int k = 0; k < 3; k++) norm += g1[k] * g1[k];
+      norm = 1.0 / sqrt(norm);
+      for (int k = 0; k < 3; k++) g1[k] = norm * g1[k];
+      norm = 0.0;
+      for (int k = 0; k < 3; k++) norm += g1[k] * g2[k];
+      for (int k = 0; k < 3; k++) g2[k] = g2[k] - norm * g1[k];
+      norm = 0.0;
+      for (int k = 0; k < 3; k++) norm += g2[k] * g2[k];
+      norm = 1.0 / sqrt(norm);
+      for (int k = 0; k < 3; k++) g2[k] = norm * g2[k];
+      const double g3[3] = {g1[1] * g2[2] - g1[2] * g2[1], g1[2] * g2[0] - g1[0] * g2[2],
+                            g1[0] * g2[1] - g1[1] * g2[0]};
+      for (int m = 0; m < 4; m++)
+        for (int k = 0; k < 3; k++) {
+          double sacc = 0.0;
+          sacc += g1[k] * fv[3 * m];
+          sacc += g2[k] * fv[3 * m + 1];
+          sacc += g3[k] * fv[3 * m + 2];
+          av[m][k] = sacc;
+        }
+    } else {
+      for (int m = 0; m < 4; m++)
+        for (int k = 0; k < 3; k++) av[m][k] = fv[3 * m + k];
+    }
+    for (int m = 0; m < 4; m++)
+      for (int k = 0; k < 3; k++) {
+        double v = 0.0;
+        for (int q = 0; q < 4; q++) v += M[m][F.asc[q]] * av[F.asc[q]][k];
+        nf[F.node[m]][k] = v + nf[F.node[m]][k];
+      }
+  }
+#pragma unroll
+  for (int a = 0; a < 8; a++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) fe[r * 24 + k * 8 + a] = nf[a][k];
+}
+
+// code numbers of each record's element; records with an element id outside
+// [0, ne) get no DOFs and raise *bad
+__global__ void k_rec_cn(int64_t nrec, int64_t ne, const int *__restrict__ eid,
+                         const int *__restrict__ cn, int *__restrict__ cnr, int *bad)
+{
+  const int64_t k = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (k >= nrec * 24) return;
+  const int64_t r = k / 24;
+  const int e = eid[r];
+  const bool ok = e >= 0 && e < ne;
+  if (!ok && k % 24 == 0) *bad = 1;
+  cnr[k] = ok ? cn[(int64_t)e * 24 + k % 24] : 0;
+}
+
 // ifor[d] = sum over the element entries of unknown d, in element order
 // (locglob into a cleared vector). The entry indices and values are loaded eight
 // at a time so that their loads overlap; the sum keeps the entry order.
@@ -862,6 +1021,133 @@ static hipError_t vload_launch(ssf_elements *el, const double *nodval, double *r
   hipLaunchKernelGGL(k_post_gather, dim3(blocks_for(el->n, TPB)), block, 0, s, (int)el->n,
                      el->start, el->src, el->fe, rhs, 1);
   return hipGetLastError();
+}
+
+// surface loads of nrec records added into rhs in record order: per-record forces,
+// then a table of the records' (record, local DOF) entries sorted stably by code
+// number, summed per unknown (locglob record by record, loadcase.cpp:572-597)
+static int sload_launch(ssf_elements *el, int64_t nrec, const int *eid, const int *ls,
+                        const double *nodvals, double *rhs)
+{
+  hipStream_t s = el->ctx->stream;
+  const int64_t nent = nrec * 24, n = el->n;
+  double *fe = nullptr;
+  int *cnr = nullptr, *key = nullptr, *skey = nullptr, *val = nullptr, *src = nullptr,
+      *start = nullptr, *bad = nullptr;
+  void *tmp = nullptr;
+  hipError_t e = hipSuccess;
+  auto c = [&](hipError_t x) { if (e == hipSuccess) e = x; };
+  c(dalloc(&fe, nent, s));
+  c(dalloc(&cnr, nent, s));
+  c(dalloc(&key, nent, s));
+  c(dalloc(&skey, nent, s));
+  c(dalloc(&val, nent, s));
+  c(dalloc(&src, nent, s));
+  c(dalloc(&start, n + 1, s));
+  c(dalloc(&bad, 1, s));
+  int hbad = 0;
+  if (e == hipSuccess) c(hipMemsetAsync(bad, 0, sizeof(int), s));
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_rec_cn, dim3(blocks_for(nent, TPB)), dim3(TPB), 0, s, nrec, el->ne, eid,
+                       el->cn, cnr, bad);
+    c(hipGetLastError());
+  }
+  if (e == hipSuccess) c(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+  if (e == hipSuccess) c(hipStreamSynchronize(s));
+  int bits = 1;
+  while (bits < 31 && (1LL << bits) <= n) bits++;
+  size_t tb = 0;
+  if (e == hipSuccess && !hbad) {
+    hipLaunchKernelGGL(k_sload, dim3(blocks_for(nrec, TPB)), dim3(TPB), 0, s, nrec, eid, ls,
+                       nodvals, el->conn, el->xyz, fe);
+    hipLaunchKernelGGL(k_post_keys, dim3(blocks_for(nent, TPB)), dim3(TPB), 0, s, nent, (int)n,
+                       8, 3, cnr, key, val);
+    c(hipGetLastError());
+    if (e == hipSuccess)
+      c(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, key, skey, val, src, (int)nent, 0, bits,
+                                           s));
+    if (e == hipSuccess) c(hipMallocAsync(&tmp, std::max<size_t>(tb, 1), s));
+    if (e == hipSuccess)
+      c(hipcub::DeviceRadixSort::SortPairs(tmp, tb, key, skey, val, src, (int)nent, 0, bits, s));
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_post_fill_int, dim3(blocks_for(n + 1, TPB)), dim3(TPB), 0, s, n + 1,
+                         (int)nent, start);
+      hipLaunchKernelGGL(k_post_start, dim3(blocks_for(nent, TPB)), dim3(TPB), 0, s, nent,
+                         (int)n, skey, start);
+      hipLaunchKernelGGL(k_post_gather, dim3(blocks_for(n, TPB)), dim3(TPB), 0, s, (int)n, start,
+                         src, fe, rhs, 1);
+      c(hipGetLastError());
+    }
+  }
+  c(hipStreamSynchronize(s));
+  if (tmp) (void)hipFreeAsync(tmp, s);
+  dfree(fe, s); dfree(cnr, s); dfree(key, s); dfree(skey, s); dfree(val, s); dfree(src, s);
+  dfree(start, s); dfree(bad, s);
+  SSF_HIP(e);
+  SSF_CHECK(!hbad, SSF_EINVAL, "ssf_elements_surface_load: element id out of range");
+  return SSF_OK;
+}
+
+extern "C" int ssf_elements_surface_load_dev(ssf_elements *el, int64_t nrec, const int32_t *eid_dev,
+                                             const int32_t *ls_dev, const double *nodvals_dev,
+                                             double *rhs_dev)
+{
+  SSF_ENTER();
+  SSF_CHECK(el && nrec >= 0 && (nrec == 0 || (eid_dev && ls_dev && nodvals_dev)) &&
+                (rhs_dev || el->n == 0),
+            SSF_EINVAL, "ssf_elements_surface_load_dev: NULL argument");
+  SSF_CHECK(el->kind == SSF_ELEM_HEX8_ELAST, SSF_EUNSUP,
+            "ssf_elements_surface_load: hex8 elasticity only (linhex::node_forces_surf)");
+  SSF_CHECK(nrec * 24 < (1LL << 31), SSF_EINVAL, "ssf_elements_surface_load: too many records");
+  SSF_HIP(hipSetDevice(el->ctx->device));
+  if (nrec == 0 || el->n == 0) return SSF_OK;
+  return sload_launch(el, nrec, eid_dev, ls_dev, nodvals_dev, rhs_dev);
+}
+
+extern "C" int ssf_elements_surface_load(ssf_elements *el, int64_t nrec, const int64_t *eid,
+                                         const int64_t *ls, const double *nodvals, double *rhs)
+{
+  SSF_ENTER();
+  SSF_CHECK(el && nrec >= 0 && (nrec == 0 || (eid && ls && nodvals)) && (rhs || el->n == 0),
+            SSF_EINVAL, "ssf_elements_surface_load: NULL argument");
+  SSF_CHECK(el->kind == SSF_ELEM_HEX8_ELAST, SSF_EUNSUP,
+            "ssf_elements_surface_load: hex8 elasticity only (linhex::node_forces_surf)");
+  SSF_CHECK(nrec * 24 < (1LL << 31), SSF_EINVAL, "ssf_elements_surface_load: too many records");
+  for (int64_t r = 0; r < nrec; r++)
+    SSF_CHECK(eid[r] >= 0 && eid[r] < el->ne, SSF_EINVAL,
+              "ssf_elements_surface_load: element id out of range");
+  SSF_HIP(hipSetDevice(el->ctx->device));
+  if (nrec == 0 || el->n == 0) return SSF_OK;
+  hipStream_t s = el->ctx->stream;
+  std::vector<int> e32(nrec), l32(nrec * 6);
+  for (int64_t r = 0; r < nrec; r++) e32[r] = (int)eid[r];
+  for (int64_t k = 0; k < nrec * 6; k++) l32[k] = (int)ls[k];
+  int *d_eid = nullptr, *d_ls = nullptr;
+  double *d_nv = nullptr, *d_rhs = nullptr;
+  hipError_t e = hipSuccess;
+  auto c = [&](hipError_t x) { if (e == hipSuccess) e = x; };
+  c(dalloc(&d_eid, nrec, s));
+  c(dalloc(&d_ls, nrec * 6, s));
+  c(dalloc(&d_nv, nrec * 72, s));
+  c(dalloc(&d_rhs, el->n, s));
+  if (e == hipSuccess) {
+    c(hipMemcpyAsync(d_eid, e32.data(), nrec * sizeof(int), hipMemcpyHostToDevice, s));
+    c(hipMemcpyAsync(d_ls, l32.data(), nrec * 6 * sizeof(int), hipMemcpyHostToDevice, s));
+    c(hipMemcpyAsync(d_nv, nodvals, nrec * 72 * sizeof(double), hipMemcpyHostToDevice, s));
+    c(hipMemcpyAsync(d_rhs, rhs, el->n * sizeof(double), hipMemcpyHostToDevice, s));
+  }
+  int rc = SSF_OK;
+  if (e == hipSuccess) rc = sload_launch(el, nrec, d_eid, d_ls, d_nv, d_rhs);
+  if (e == hipSuccess && rc == SSF_OK)
+    c(hipMemcpyAsync(rhs, d_rhs, el->n * sizeof(double), hipMemcpyDeviceToHost, s));
+  c(hipStreamSynchronize(s));
+  dfree(d_eid, s);
+  dfree(d_ls, s);
+  dfree(d_nv, s);
+  dfree(d_rhs, s);
+  if (rc) return rc;
+  SSF_HIP(e);
+  return SSF_OK;
 }
 
 extern "C" int ssf_elements_volume_load_dev(ssf_elements *el, const double *nodval_dev,

@@ -124,6 +124,8 @@ SIGNATURES = {
     "ssf_elements_results": (_I, [_P, _P, _P, _L, _P, _P, _P]),
     "ssf_elements_results_dev": (_I, [_P, _P, _P, _L, _P, _P, _P]),
     "ssf_elements_volume_load": (_I, [_P, _P, _P]),
+    "ssf_elements_surface_load": (_I, [_P, _L, _P, _P, _P, _P]),
+    "ssf_elements_surface_load_dev": (_I, [_P, _L, _P, _P, _P, _P]),
     "ssf_elements_volume_load_dev": (_I, [_P, _P, _P]),
     "ssf_elements_bench": (_I, [_P, _P, _P, _L, _P, _P, _P, _I, _P]),
 }
@@ -605,6 +607,21 @@ class Elements:
                                           0 if pv is None else pv.shape[0], _p(eps), _p(sig),
                                           _p(f)))
         return eps, sig, f
+
+    def surface_load(self, eid, ls, nodvals, rhs=None):
+        """Surface loads of records (element eid[r], face flags ls[r, 6], nodal values
+        nodvals[r, 72]) added into a copy of rhs (zeros if None) in record order."""
+        out = np.zeros(self.n) if rhs is None else _f64(rhs).copy()
+        eid, ls, nodvals = _i64(eid), _i64(ls), _f64(nodvals)
+        _check(lib().ssf_elements_surface_load(self.h, eid.shape[0], _p(eid), _p(ls),
+                                               _p(nodvals), _p(out)))
+        return out
+
+    def surface_load_dev(self, nrec, eid_ptr, ls_ptr, nodvals_ptr, rhs_ptr):
+        _check(lib().ssf_elements_surface_load_dev(self.h, nrec, ctypes.c_void_p(eid_ptr),
+                                                   ctypes.c_void_p(ls_ptr),
+                                                   ctypes.c_void_p(nodvals_ptr),
+                                                   ctypes.c_void_p(rhs_ptr)))
 
     def volume_load(self, nodval, rhs=None):
         """Element volume loads (heat: sources) with nodal values nodval (ne, ndofe)

@@ -20,6 +20,10 @@ Element volume loads (elast_vload_*, heat_vload_*): seeded nodal load values per
 element, their load vectors from the harness's ref_hex8_volume_load (load_matrix
 / quantity_source_vector), added by locglob in element order onto a seeded
 starting vector (standing for the nodal loads added first).
+Surface loads ({elast,cant}_sload_*): seeded records (element, face flags ls in
+{0, 1, 2}, 72 nodal values) through the harness's ref_hex8_surface_load
+(linhex::node_forces_surf composed from GEFEL's jac2d_3d, nnj, mxv, addv) and the
+reference's locglob in record order onto a seeded starting vector.
 Incomplete LDL^T ({heat,elast}_ildl_*): the reference's symcomprow::incomplete_ldl
 (gamma 0, as gmatrix fixes it) of the scr values, prec_ildl_scr of a seeded
 vector, and cg_prec with tprec = 10 from x = 0 to res 1e-24.
@@ -240,6 +244,27 @@ def main():
     out["heat_np_ani"] = np.array([t[2] for t in traj])
     out["heat_np_ares"] = np.array([t[3] for t in traj])
     print("heat np", prob.n, "unknowns", flush=True)
+    # linhex::node_forces_surf on records of the elasticity cases, added by locglob
+    for name in ("elast", "cant"):
+        prob = CASES[name]()
+        sr = np.random.default_rng(7000 + list(CASES).index(name))
+        nrec = min(prob.ne, 10)
+        eid = sr.choice(prob.ne, nrec, replace=True)  # an element may carry two records
+        ls = sr.integers(0, 3, (nrec, 6))
+        nodvals = sr.standard_normal((nrec, 72)) * 1e3
+        rhs = sr.standard_normal(prob.n)
+        out_rhs = rhs.copy()
+        cn = i64(prob.elem_codes())
+        for r in range(nrec):
+            nodes = prob.conn[eid[r]]
+            xx, yy, zz = (f64(prob.coords[nodes, k]) for k in range(3))
+            nf = np.zeros(24)
+            R.ref_hex8_surface_load(_ptr(xx), _ptr(yy), _ptr(zz), _ptr(i64(ls[r])),
+                                    _ptr(f64(nodvals[r])), _ptr(nf))
+            R.ref_locglob(_ptr(out_rhs), _ptr(nf), _ptr(i64(cn[eid[r]]).copy()), 24)
+        out[f"{name}_sload_eid"], out[f"{name}_sload_ls"] = i64(eid), i64(ls)
+        out[f"{name}_sload_nv"], out[f"{name}_sload_rhs0"] = nodvals, rhs
+        out[f"{name}_sload_rhs"] = out_rhs
     # symcomprow::incomplete_ldl, prec_ildl_scr and cg_prec with tprec = 10
     for name in ("heat", "elast"):
         prob = CASES[name]()
