@@ -97,6 +97,17 @@ void *ssf_ctx_stream(ssf_ctx *ctx);
 int ssf_ctx_set_profiling(ssf_ctx *ctx, int on);
 int ssf_ctx_kernel_time(ssf_ctx *ctx, int which, double *avg_us, int64_t *count);
 
+/* ------------------------------------------------------ node renumbering */
+/* gtopology::cuthill_mckee_renumb (GEFEL/gtopology.cpp:6242-6355) on the edge graph
+ * of adjacnodes_edge (:2632) of nn nodes and ne hex8 (nne 8) or quad4 (nne 4)
+ * elements conn[ne*nne] (0-based): ordering[k] = the node numbered k (reverse =
+ * 1: rev_cuthill_mckee, the order reversed), equal to the reference's ordering.
+ * gencodnum then numbers the DOFs by visiting the nodes in this order. The graph
+ * must be connected (SSF_EINVAL otherwise). Built on the device level by level;
+ * synchronous. */
+int ssf_cuthill_mckee(ssf_ctx *ctx, int64_t nn, int64_t ne, int64_t nne, const int64_t *conn,
+                      int reverse, int64_t *ordering);
+
 /* ------------------------------------------------------------ storage setup */
 /* symcomprow::initiate / comprow::initiate (GEFEL/scr.cpp:556, GEFEL/cr.cpp:454,
  * reached through gmatrix::initiate GEFEL/gmatrix.cpp:268): symbolic pattern

@@ -44,6 +44,8 @@ def lib():
             "cr_mxv_scr": (None, [_L, _P, _P, _P, _P, _P]),
             "cr_mxv_cr": (None, [_L, _P, _P, _P, _P, _P]),
             "cr_mtxv_cr": (None, [_L, _P, _P, _P, _P, _P]),
+            "cr_edge_adjacency": (None, [_L, _L, _L, _P, _P, _P]),
+            "cr_cuthill_mckee": (ctypes.c_int, [_L, _P, _P, ctypes.c_int, _P]),
             "cr_hex8_surface_load": (None, [_P, _P, _P, _P, _P, _P]),
             "cr_surface_load_mesh": (None, [_L, _P, _P, _P, _P, _P, _P, _P]),
             "cr_newton_raphson": (ctypes.c_double, [_L, _P, _P, _P, ctypes.c_int, _L, _P, _P, _P,
@@ -114,6 +116,7 @@ def ref():
             "ref_cr_localized": (None, [_P, _P, _P, _L]),
             "ref_cr_mxv": (None, [_P, _P, _P]),
             "ref_cr_mtxv": (None, [_P, _P, _P]),
+            "ref_cuthill_mckee": (None, [_L, _L, _L, _P, ctypes.c_int, _P]),
             "ref_hex8_surface_load": (None, [_P, _P, _P, _P, _P, _P]),
             "ref_scr_incomplete_ldl": (None, [_P, _D, _P]),
             "ref_scr_prec_ildl": (None, [_P, _P, _P]),
@@ -319,6 +322,25 @@ def newton_raphson(prob, adr, ci, a, fp, fc=None, ilambda=0.0, li=0, ninr=50, ni
     hist = {"lam": lam[:k], "dlam": dlam[:k], "nsol": nsol[:k], "acc": acc[:k],
             "solves": int(counts[1])}
     return lam_end, ra, fa, hist
+
+
+def edge_adjacency(nn, conn):
+    """adjacnodes_edge: CSR (ptr, adj) of the element-edge graph, neighbours ascending."""
+    conn = i64(conn)
+    ne, nne = conn.shape
+    ptr = np.zeros(nn + 1, np.int64)
+    adj = np.zeros(max(1, 2 * ne * (12 if nne == 8 else 4)), np.int64)
+    lib().cr_edge_adjacency(nn, ne, nne, _ptr(conn), _ptr(ptr), _ptr(adj))
+    return ptr, adj[:ptr[-1]].copy()
+
+
+def cuthill_mckee(nn, conn, reverse=False):
+    """cuthill_mckee_renumb: ordering[k] = the node numbered k; None if the graph is
+    not connected."""
+    ptr, adj = edge_adjacency(nn, conn)
+    out = np.zeros(nn, np.int64)
+    rc = lib().cr_cuthill_mckee(nn, _ptr(ptr), _ptr(adj), 1 if reverse else 0, _ptr(out))
+    return out if rc == 0 else None
 
 
 def cr_bicg(adr, ci, a, y, ni, res, zero=1e-30, x0=None):

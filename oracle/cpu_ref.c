@@ -8,6 +8,7 @@
  */
 #include "cpu_ref.h"
 
+#include <limits.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -262,6 +263,115 @@ static double ss(const double *a, const double *b, long n)
 }
 
 /* ------------------------------------------------------------------- solvers */
+
+/* ============================================== node renumbering (Cuthill-McKee) === */
+
+/* the edges of hex8 (nne 8) and quad4 (nne 4) elements in SIFEL's node order */
+static const int HEX_EDGES[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
+                                     {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+static const int QUAD_EDGES[4][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}};
+
+/* GEFEL/gtopology.cpp:2632-2840 adjacnodes_edge for two-node edges: node i's
+ * neighbours are the nodes it shares an element edge with, each edge counted once,
+ * in ascending order. CSR in ptr[nn+1], adj[ptr[nn]]; adj has room for
+ * 2 * ne * (edges per element) entries. */
+void cr_edge_adjacency(long nn, long ne, long nne, const long *conn, long *ptr, long *adj)
+{
+  const int ned = nne == 8 ? 12 : 4;
+  const int(*ed)[2] = nne == 8 ? HEX_EDGES : QUAD_EDGES;
+  long *cnt = (long *)calloc((size_t)nn + 1, sizeof(long));
+  long *tmp = (long *)malloc((size_t)(2 * ne * ned > 0 ? 2 * ne * ned : 1) * sizeof(long));
+  long *tp = (long *)calloc((size_t)nn + 1, sizeof(long));
+  for (long e = 0; e < ne; e++)
+    for (int k = 0; k < ned; k++) {
+      cnt[conn[e * nne + ed[k][0]] + 1]++;
+      cnt[conn[e * nne + ed[k][1]] + 1]++;
+    }
+  for (long i = 0; i < nn; i++) cnt[i + 1] += cnt[i];
+  memcpy(tp, cnt, ((size_t)nn + 1) * sizeof(long));
+  for (long e = 0; e < ne; e++)
+    for (int k = 0; k < ned; k++) {
+      long a = conn[e * nne + ed[k][0]], b = conn[e * nne + ed[k][1]];
+      tmp[tp[a]++] = b;
+      tmp[tp[b]++] = a;
+    }
+  ptr[0] = 0;
+  for (long i = 0; i < nn; i++) {
+    long b0 = cnt[i], b1 = cnt[i + 1], m = 0;
+    qsort(tmp + b0, (size_t)(b1 - b0), sizeof(long), cmp_long);
+    for (long q = b0; q < b1; q++)
+      if (q == b0 || tmp[q] != tmp[q - 1]) adj[ptr[i] + m++] = tmp[q];
+    ptr[i + 1] = ptr[i] + m;
+  }
+  free(cnt);
+  free(tmp);
+  free(tp);
+}
+
+/* GEFEL/gtopology.cpp:6242-6355 cuthill_mckee_renumb on the graph (ptr, adj):
+ * the start is the last node of least degree; every neighbour list is bubble-
+ * sorted by degree (stable, so ties stay in node order); the nodes numbered so
+ * far are visited in number order and number their unnumbered neighbours in list
+ * order. ordering[k] = the node numbered k (reverse: the order reversed). The
+ * neighbour lists in adj are sorted in place. Returns 0, or -1 if the graph is
+ * not connected (the reference leaves such nodes unnumbered). */
+int cr_cuthill_mckee(long nn, const long *ptr, long *adj, int reverse, long *ordering)
+{
+  long i, j, k, m, minnode = 0, min = LONG_MAX, startnode;
+  long *cmk = (long *)malloc((size_t)(nn > 0 ? nn : 1) * sizeof(long));
+  for (i = 0; i < nn; i++) cmk[i] = -1;
+  for (i = 0; i < nn; i++) {
+    long deg = ptr[i + 1] - ptr[i];
+    if (min >= deg) {
+      min = deg;
+      minnode = i;
+    }
+  }
+  if (nn > 0) cmk[minnode] = 0;
+  for (i = 0; i < nn; i++) {
+    long *a = adj + ptr[i], na = ptr[i + 1] - ptr[i];
+    for (m = na - 1; m >= 0; m--)
+      for (j = 1; j <= m; j++)
+        if (ptr[a[j - 1] + 1] - ptr[a[j - 1]] > ptr[a[j] + 1] - ptr[a[j]]) {
+          long t = a[j - 1];
+          a[j - 1] = a[j];
+          a[j] = t;
+        }
+  }
+  startnode = 0;
+  m = 1;
+  int stop = nn <= 1;
+  for (k = 0; k < nn && !stop; k++) {
+    for (i = 0; i < nn; i++) {
+      if (cmk[i] == startnode) {
+        for (j = ptr[i]; j < ptr[i + 1]; j++) {
+          if (cmk[adj[j]] == -1) {
+            cmk[adj[j]] = m;
+            m++;
+          }
+          if (m == nn) {
+            stop = 1;
+            break;
+          }
+        }
+        break;
+      }
+    }
+    startnode++;
+  }
+  int rc = 0;
+  for (i = 0; i < nn; i++)
+    if (cmk[i] < 0) rc = -1;
+  if (rc == 0) {
+    for (i = 0; i < nn; i++) ordering[cmk[i]] = i;
+    if (reverse) {
+      for (i = 0; i < nn; i++) cmk[nn - 1 - i] = ordering[i];
+      for (i = 0; i < nn; i++) ordering[i] = cmk[i];
+    }
+  }
+  free(cmk);
+  return rc;
+}
 
 /* GEFEL/scr.cpp:958-975 rows_mult_ldl_scr: sum over the common columns k < o of
  * rows m and o of incdec_mk incdec_ok D_k, merged in column order */

@@ -52,6 +52,13 @@
  *                         pressure on a rectangular face gives pA/4 per node; the
  *                         local-axes form equals the global form with the rotated
  *                         values; the loads integrate the traction (total = int t dA)
+ *   cr_edge_adjacency, cr_cuthill_mckee
+ *                         the ordering equals oracle/_ref's (the reference's own
+ *                         gtopology::adjacnodes_edge + cuthill_mckee_renumb, both
+ *                         forms); a path graph gives the path order; the ordering
+ *                         is a permutation and a BFS (levels never decrease); the
+ *                         profile of the renumbered matrix does not exceed a
+ *                         shuffled numbering's on a mesh
  *   cr_newton_raphson     not run against the reference itself (gnewton_raphson
  *                         needs MEFEL's global problem objects, which the GEFEL-only
  *                         harness does not build); pinned by closed forms: the
@@ -220,6 +227,8 @@ void cr_hex8_surface_load(const double *x, const double *y, const double *z, con
                           const double *nodvals, double *nf);
 void cr_surface_load_mesh(long nrec, const long *eid, const long *ls, const double *nodvals,
                           const long *conn, const double *xyz, const long *cn, double *rhs);
+void cr_edge_adjacency(long nn, long ne, long nne, const long *conn, long *ptr, long *adj);
+int cr_cuthill_mckee(long nn, const long *ptr, long *adj, int reverse, long *ordering);
 int cr_np_step_linear(long n, const long *adr, const long *ci, const double *aK,
                       const double *aC, double *aA, double alpha, double dt,
                       const double *rhs, double *lhs, double *tdlhs, double *d, double *p,

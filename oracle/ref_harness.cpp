@@ -21,6 +21,8 @@
 #include "basefun.h"
 #include "difcalc.h"
 #include "intp.h"
+#include "gtopology.h"
+#include "gelement.h"
 
 extern "C" {
 
@@ -666,6 +668,31 @@ void ref_hex8_surface_load(const double *xc, const double *yc, const double *zc,
     addv(v, nf, nf);
   }
   for (long i = 0; i < ndofe; i++) nf_out[i] = nf[i];
+}
+
+// gtopology::cuthill_mckee_renumb (GEFEL/gtopology.cpp:6242) itself, on a gtopology
+// holding only the elements (linhexa for nne 8, linquad for nne 4); it builds its
+// graph with adjacnodes_edge (:2632). ordering_out[k] = the node numbered k.
+void ref_cuthill_mckee(long nn, long ne, long nne, const long *conn, int reverse,
+                       long *ordering_out)
+{
+  gtopology *gt = new gtopology();
+  gt->nn = nn;
+  gt->ne = ne;
+  gt->gelements = new gelement[ne];
+  for (long e = 0; e < ne; e++) {
+    gt->gelements[e].get = nne == 8 ? linhexa : linquad;
+    gt->gelements[e].nne = nne;
+    gt->gelements[e].nodes = new long[nne];
+    for (long a = 0; a < nne; a++) gt->gelements[e].nodes[a] = conn[e * nne + a];
+  }
+  gt->nodren = reverse ? rev_cuthill_mckee : cuthill_mckee;
+  gt->ordering = new long[nn];
+  gt->cuthill_mckee_renumb(stdout);
+  for (long i = 0; i < nn; i++) ordering_out[i] = gt->ordering[i];
+  delete[] gt->ordering;
+  gt->ordering = NULL;
+  delete gt;
 }
 
 }  // extern "C"

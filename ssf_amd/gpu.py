@@ -110,6 +110,7 @@ SIGNATURES = {
     "ssf_matrix_add": (_I, [_P, _D, _P]),
     "ssf_np_step_linear": (_I, [_P, _P, _P, _D, _D, ctypes.POINTER(SolveParams), _P, _P, _P,
                                 ctypes.POINTER(SolveInfo)]),
+    "ssf_cuthill_mckee": (_I, [_P, _L, _L, _L, _P, _I, _P]),
     "ssf_newton_raphson": (_I, [_P, _P, ctypes.POINTER(NrParams), ctypes.POINTER(SolveParams),
                                 _P, _L, _P, _P, _P, _P, _P, _P, _P, _P, ctypes.POINTER(NrInfo)]),
     "ssf_matrix_ildl_factor": (_I, [_P, _P]),
@@ -397,6 +398,15 @@ def np_step_linear(K: Matrix, C: Matrix, A: Matrix, alpha: float, dt: float, rhs
     _check(getattr(lib(), fn)(K.h, C.h, A.h, float(alpha), float(dt), ctypes.byref(p),
                               rhs_ptr, lhs_ptr, tdlhs_ptr, ctypes.byref(info)))
     return info
+
+
+def cuthill_mckee(ctx: Context, nn: int, conn: np.ndarray, reverse=False) -> np.ndarray:
+    """gtopology::cuthill_mckee_renumb on the device: ordering[k] = the node numbered k."""
+    conn = _i64(conn)
+    out = np.zeros(nn, np.int64)
+    _check(lib().ssf_cuthill_mckee(ctx.h, nn, conn.shape[0], conn.shape[1], _p(conn),
+                                   1 if reverse else 0, _p(out)))
+    return out
 
 
 def newton_raphson(K: Matrix, el: "Elements", fp_ptr, fa_ptr, ra_ptr, fc_ptr, ninr=50, niilnr=10,

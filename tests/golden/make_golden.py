@@ -20,6 +20,9 @@ Element volume loads (elast_vload_*, heat_vload_*): seeded nodal load values per
 element, their load vectors from the harness's ref_hex8_volume_load (load_matrix
 / quantity_source_vector), added by locglob in element order onto a seeded
 starting vector (standing for the nodal loads added first).
+Node renumbering ({case}_cm_ordering, {case}_cmr_ordering): the reference's own
+gtopology::cuthill_mckee_renumb (edge graph of adjacnodes_edge) on each case's
+mesh, forward and reversed: ordering[k] = the node numbered k.
 Surface loads ({elast,cant}_sload_*): seeded records (element, face flags ls in
 {0, 1, 2}, 72 nodal values) through the harness's ref_hex8_surface_load
 (linhex::node_forces_surf composed from GEFEL's jac2d_3d, nnj, mxv, addv) and the
@@ -244,6 +247,23 @@ def main():
     out["heat_np_ani"] = np.array([t[2] for t in traj])
     out["heat_np_ares"] = np.array([t[3] for t in traj])
     print("heat np", prob.n, "unknowns", flush=True)
+    # gtopology::cuthill_mckee_renumb (both forms) on every case's mesh
+    for name, make in CASES.items():
+        prob = make()
+        nn = prob.coords.shape[0]
+        conn = i64(prob.conn)
+        for rev in (0, 1):
+            o = np.zeros(nn, np.int64)
+            sys.stdout.flush()
+            saved = os.dup(1)
+            dn = os.open(os.devnull, os.O_WRONLY)
+            os.dup2(dn, 1)  # the reference prints its graph on the C stdout
+            R.ref_cuthill_mckee(nn, conn.shape[0], conn.shape[1], _ptr(conn), rev, _ptr(o))
+            ctypes.CDLL(None).fflush(None)
+            os.dup2(saved, 1)
+            os.close(dn)
+            os.close(saved)
+            out[f"{name}_cm{'r' if rev else ''}_ordering"] = o
     # linhex::node_forces_surf on records of the elasticity cases, added by locglob
     for name in ("elast", "cant"):
         prob = CASES[name]()
