@@ -488,6 +488,32 @@ def bench_ildl(ctx, args, gpu, mesh, torch, prob, m, rhs):
     return out
 
 
+def bench_renumber(ctx, args, gpu, mesh, prob):
+    """Reverse Cuthill-McKee ordering of the heat cube's mesh (edge graph of its hex8
+    elements) on the device. CPU baseline: the oracle's restatement of the
+    reference's routine, whose search for the next parent scans every node per
+    number, on a 40^3 cube."""
+    nn = prob.coords.shape[0]
+    t = time.perf_counter()
+    o = gpu.cuthill_mckee(ctx, nn, prob.conn, reverse=True)
+    dt = time.perf_counter() - t
+    out = {"workload": f"reverse Cuthill-McKee of the heat hex8 {args.nel}^3 mesh, nn={nn}",
+           "seconds": dt, "nodes_per_s": nn / dt, "first": int(o[0]), "last": int(o[-1])}
+    log(f"[renumber] {nn} nodes in {dt * 1e3:.0f} ms")
+    if not args.no_cpu:
+        import oracle as O
+        small = mesh.heat_cube(40, seed=2)
+        ns = small.coords.shape[0]
+        t = time.perf_counter()
+        O.cuthill_mckee(ns, small.conn, True)
+        dts = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": ns / dts, "unit": "nodes/s", "cores": 1, "kind": "port",
+                               "sample": f"the oracle's cuthill_mckee_renumb (nn^2 parent scan, "
+                                         f"1 thread) on the 40^3 cube ({ns} nodes, {dts:.2f} s); "
+                                         f"its cost grows as nn^2"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -520,6 +546,7 @@ def main():
     ap.add_argument("--np-dt", type=float, default=1e-3)
     ap.add_argument("--no-bicg", action="store_true", help="skip the comprow bicg leg")
     ap.add_argument("--no-ildl", action="store_true", help="skip the incomplete LDL^T leg")
+    ap.add_argument("--no-renumber", action="store_true", help="skip the Cuthill-McKee leg")
     ap.add_argument("--ildl-steps", type=int, default=1)
     ap.add_argument("--bicg-steps", type=int, default=2)
     ap.add_argument("--serve-count", type=int, default=24)
@@ -659,6 +686,9 @@ def main():
     transient = None
     if not args.no_np:
         transient = bench_transient(ctx, args, gpu, mesh, torch, prob, m, rhs)
+    renumber = None
+    if not args.no_renumber:
+        renumber = bench_renumber(ctx, args, gpu, mesh, prob)
     ildl = None
     if not args.no_ildl:
         ildl = bench_ildl(ctx, args, gpu, mesh, torch, prob, m, rhs)
@@ -711,6 +741,7 @@ def main():
         "cpu_baseline": cpu,
         "post": post,
         "transient": transient,
+        "renumber": renumber,
         "ildl": ildl,
         "bicg": bicg,
         "batched": batched,
