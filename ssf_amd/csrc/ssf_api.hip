@@ -8,6 +8,8 @@
 #include <cstring>
 #include <thread>
 
+#include <unordered_map>
+
 #include "ssf_kernels.hpp"
 
 namespace ssf {
@@ -180,6 +182,63 @@ static bool build_tiles(int64_t n, const std::vector<int> &rowptr, const std::ve
   return true;
 }
 
+bool build_sell_patterns(int64_t n, const std::vector<int> &off, const std::vector<int> &scol,
+                         std::vector<unsigned short> &pid, std::vector<int> &dict,
+                         std::vector<int> &dict_off, int64_t max_dict)
+{
+  const int64_t nslice = (int64_t)off.size() - 1;
+  const int64_t nrow = nslice * 64;
+  // a hash of every layout row's (width, offsets), in parallel
+  std::vector<uint64_t> h(nrow);
+  auto offset_at = [&](int64_t sl, int64_t l, int64_t q) {
+    const int64_t k = off[sl] + (q / SELL_V) * 64 * SELL_V + l * SELL_V + q % SELL_V;
+    const int64_t rb = std::min<int64_t>(sl * 64 + l, n - 1);
+    return (int)(scol[k] - rb);
+  };
+  parallel_for(nslice, [&](int64_t b, int64_t e) {
+    for (int64_t sl = b; sl < e; sl++) {
+      const int64_t w = (off[sl + 1] - off[sl]) / 64;
+      for (int64_t l = 0; l < 64; l++) {
+        uint64_t x = mix64((uint64_t)w);
+        for (int64_t q = 0; q < w; q++) x = mix64(x ^ (uint64_t)(uint32_t)offset_at(sl, l, q));
+        h[sl * 64 + l] = x;
+      }
+    }
+  });
+  // first occurrence of each pattern, compared entry by entry on equal hashes
+  pid.assign(nrow, 0);
+  dict.clear();
+  dict_off.clear();
+  std::vector<int> width;
+  std::unordered_multimap<uint64_t, int> seen;
+  for (int64_t sl = 0; sl < nslice; sl++) {
+    const int w = (off[sl + 1] - off[sl]) / 64;
+    for (int64_t l = 0; l < 64; l++) {
+      const uint64_t x = h[sl * 64 + l];
+      int found = -1;
+      auto range = seen.equal_range(x);
+      for (auto it = range.first; it != range.second && found < 0; ++it) {
+        const int p = it->second;
+        if (width[p] != w) continue;
+        bool same = true;
+        for (int q = 0; q < w && same; q++) same = dict[dict_off[p] + q] == offset_at(sl, l, q);
+        if (same) found = p;
+      }
+      if (found < 0) {
+        if ((int64_t)dict_off.size() >= 65535 || (int64_t)dict.size() + w > max_dict) return false;
+        found = (int)dict_off.size();
+        dict_off.push_back((int)dict.size());
+        width.push_back(w);
+        for (int q = 0; q < w; q++) dict.push_back(offset_at(sl, l, q));
+        seen.emplace(x, found);
+      }
+      pid[sl * 64 + l] = (unsigned short)found;
+    }
+  }
+  if (dict.empty()) dict.push_back(0);
+  return true;
+}
+
 // SELL-64 layout of a full CSR: slices of 64 rows, each row's entries in
 // column order, padded to the slice's longest row (rounded up to SELL_V) with
 // value 0 at the row's own column; src maps a SELL entry to its CSR entry.
@@ -328,6 +387,8 @@ extern "C" int ssf_matrix_destroy(ssf_matrix *m)
   dfree(m->tiles, m->ctx->stream); dfree(m->col16, m->ctx->stream);
   dfree(m->sell_off, m->ctx->stream); dfree(m->sell_col, m->ctx->stream); dfree(m->sell_src, m->ctx->stream); dfree(m->sell_v64, m->ctx->stream); dfree(m->sell_v32, m->ctx->stream);
   dfree(m->sell_col16, m->ctx->stream);
+  dfree(m->sell_pid, m->ctx->stream); dfree(m->sell_dict, m->ctx->stream);
+  dfree(m->sell_dict_off, m->ctx->stream);
   dfree(m->sell_lo, m->ctx->stream); dfree(m->sell_up, m->ctx->stream);
   dfree(m->d_adr32, m->ctx->stream); dfree(m->d_ci32, m->ctx->stream);
   free_workspace(m->ws, m->ctx->stream);
@@ -575,6 +636,24 @@ int ssf::matrix_create(ssf_ctx *ctx, int storage, int64_t n, const int64_t *adr,
       }
       chk(hipMemsetAsync(m->sell_v64, 0, std::max<int64_t>(1, m->sell_tot) * sizeof(double), ctx->stream));
     }
+    // row patterns: the column stream of a mesh matrix repeats a few offset lists
+    std::vector<unsigned short> pid;
+    std::vector<int> dict, dict_off;
+    if (e == hipSuccess && build_sell_patterns(n, soff, scol, pid, dict, dict_off, 1 << 22)) {
+      m->sell_npat = (int)dict_off.size();
+      chk(dalloc(&m->sell_pid, pid.size(), ctx->stream));
+      chk(dalloc(&m->sell_dict, dict.size(), ctx->stream));
+      chk(dalloc(&m->sell_dict_off, dict_off.size(), ctx->stream));
+      if (e == hipSuccess) {
+        chk(hipMemcpyAsync(m->sell_pid, pid.data(), pid.size() * sizeof(unsigned short),
+                           hipMemcpyHostToDevice, ctx->stream));
+        chk(hipMemcpyAsync(m->sell_dict, dict.data(), dict.size() * sizeof(int),
+                           hipMemcpyHostToDevice, ctx->stream));
+        chk(hipMemcpyAsync(m->sell_dict_off, dict_off.data(), dict_off.size() * sizeof(int),
+                           hipMemcpyHostToDevice, ctx->stream));
+        chk(hipStreamSynchronize(ctx->stream));  // the host vectors go out of scope
+      }
+    }
   }
   if (sym) {
     chk(dalloc(&m->a_ref, negm, ctx->stream));
@@ -606,6 +685,7 @@ int ssf::matrix_create(ssf_ctx *ctx, int storage, int64_t n, const int64_t *adr,
   m->tune64 = default_tune(n, m->nnz, m->ntiles, m->nslice, false);
   m->tune32 = default_tune(n, m->nnz, m->ntiles, m->nslice, m->col16 != nullptr);
   if (m->sell_col16) m->tune64.idx16 = m->tune32.idx16 = (m->tune64.format == 2) ? 1 : 0;
+  if (m->sell_pid && m->tune64.format == 2) m->tune64.idx16 = m->tune32.idx16 = 2;
   *out = m;
   return SSF_OK;
 }
@@ -1620,8 +1700,9 @@ extern "C" int ssf_matrix_autotune(ssf_matrix *m, int precision, int reps, int32
         }
   if (m->nslice > 0)
     for (int b : {512, 1024, 2048, 4096})
-      for (int i16 = 0; i16 <= (m->sell_col16 ? 1 : 0); i16++) {
+      for (int i16 = 0; i16 <= (m->sell_pid ? 2 : (m->sell_col16 ? 1 : 0)); i16++) {
         if (b > std::max(8, (m->nslice + 3) / 4)) continue;
+        if (i16 == 1 && !m->sell_col16) continue;
         SpmvTune t;
         t.format = 2;
         t.blocks = b;
@@ -1671,11 +1752,13 @@ extern "C" int ssf_matrix_set_tune(ssf_matrix *m, int precision, const int32_t *
     SSF_CHECK(t.lanes == 4 || t.lanes == 8 || t.lanes == 16, SSF_EINVAL,
               "ssf_matrix_set_tune: stream lanes must be 4, 8 or 16");
   SSF_CHECK(t.blocks >= 1 && t.blocks <= 4096, SSF_EINVAL, "ssf_matrix_set_tune: blocks out of [1, 4096]");
-  SSF_CHECK(t.idx16 == 0 || (precision && t.format == 1 && m->col16) ||
-                (t.format == 2 && m->sell_col16),
+  SSF_CHECK(t.idx16 == 0 || (t.idx16 == 1 && precision && t.format == 1 && m->col16) ||
+                (t.idx16 == 1 && t.format == 2 && m->sell_col16) ||
+                (t.idx16 == 2 && t.format == 2 && m->sell_pid),
             SSF_EINVAL,
-            "ssf_matrix_set_tune: 16-bit columns for the FP32 stream operator or the SELL operator "
-            "of a matrix that has them");
+            "ssf_matrix_set_tune: index form 1 (16-bit columns) for the FP32 stream operator or "
+            "the SELL operator of a matrix that has them, 2 (row patterns) for a SELL operator "
+            "that has them");
   if (precision) m->tune32 = t;
   else m->tune64 = t;
   return SSF_OK;
@@ -1729,8 +1812,13 @@ extern "C" int ssf_spmv_bytes(const ssf_matrix *m, int precision, double *canoni
                      (t.format == 2 && t.idx16 && m->sell_col16)) ? 2.0 : 4.0;
   double a = (double)m->nnz * (sv + ib) + 4.0 * (double)(m->n + 1) + 2.0 * sv * (double)m->n;
   if (t.format == 1) a += 4.0 * (double)(m->ntiles + 1);
-  if (t.format == 2 && m->nslice > 0)  // padded slices, slice offsets instead of row pointers
-    a = (double)m->sell_tot * (sv + ib) + 4.0 * (double)(m->nslice + 1) + 2.0 * sv * (double)m->n;
+  if (t.format == 2 && m->nslice > 0) {  // padded slices, slice offsets instead of row pointers
+    if (t.idx16 == 2 && m->sell_pid)  // a 2-byte pattern id per layout row instead of columns
+      a = (double)m->sell_tot * sv + 2.0 * 64.0 * (double)m->nslice +
+          4.0 * (double)(m->nslice + 1) + 2.0 * sv * (double)m->n;
+    else
+      a = (double)m->sell_tot * (sv + ib) + 4.0 * (double)(m->nslice + 1) + 2.0 * sv * (double)m->n;
+  }
   if (canonical) *canonical = b;
   if (actual) *actual = a;
   return SSF_OK;

@@ -123,7 +123,9 @@ struct SpmvTune {
   int format = 0;    // 0: row-vector kernel, 1: LDS-tiled stream kernel, 2: SELL-64 slices
   int lanes = 8;     // lanes (of a 64-wide wavefront) per row
   int blocks = 1024; // blocks of 256 threads (rows / tiles split in contiguous chunks)
-  int idx16 = 0;     // stream kernel reads 16-bit tile-relative column offsets
+  int idx16 = 0;     // column index form: 0 int32; 1 16-bit (stream tile- / SELL
+                     // slice-relative); 2 SELL row patterns (a pattern id per row
+                     // and a shared dictionary of column offsets)
 };
 
 // Device view of a full CSR matrix and its stream tiling.
@@ -141,6 +143,11 @@ struct CsrDev {
   const int *sell_off = nullptr;  // nslice + 1, multiples of 64 * SELL_V
   const int *sell_col = nullptr;  // padding: column of a row of the slice, value 0
   const short *sell_col16 = nullptr;  // sell_col - 64 s (null if an offset does not fit)
+  // row patterns: row r of the layout (r < 64 nslice) reads its entries' columns
+  // as min(r, n-1) + sell_dict[sell_dict_off[sell_pid[r]] + q]
+  const unsigned short *sell_pid = nullptr;
+  const int *sell_dict = nullptr;
+  const int *sell_dict_off = nullptr;
   const double *sell_v64 = nullptr;
   const float *sell_v32 = nullptr;
 };
@@ -219,6 +226,9 @@ struct ssf_matrix {
   int64_t sell_tot = 0;
   int *sell_off = nullptr, *sell_col = nullptr, *sell_src = nullptr;
   short *sell_col16 = nullptr;
+  unsigned short *sell_pid = nullptr;  // row patterns (null when not built)
+  int *sell_dict = nullptr, *sell_dict_off = nullptr;
+  int sell_npat = 0;
   double *sell_v64 = nullptr;
   float *sell_v32 = nullptr;
   // SELL-only matrices (laid out on the device, no full CSR): SELL positions of
@@ -254,6 +264,9 @@ struct ssf_matrix {
     c.sell_off = sell_off;
     c.sell_col = sell_col;
     c.sell_col16 = sell_col16;
+    c.sell_pid = sell_pid;
+    c.sell_dict = sell_dict;
+    c.sell_dict_off = sell_dict_off;
     c.sell_v64 = sell_v64;
     c.sell_v32 = sell_v32;
     return c;
@@ -309,6 +322,14 @@ void matrix_keep_sell_only(ssf_matrix *m);
 bool build_sell(int64_t n, const std::vector<int> &rowptr, const std::vector<int> &col,
                 std::vector<int> &off, std::vector<int> &scol, std::vector<int> &src,
                 std::vector<short> &scol16);
+// row patterns of a SELL-64 layout (off, scol from build_sell): per row of the
+// layout its column offsets from min(row, n-1), deduplicated into dict (each
+// pattern's entries start at a multiple of SELL_V, dict_off[p]); pid[r] = the
+// pattern of layout row r. false when there are more than 65535 patterns or
+// max_dict dictionary entries (the layout keeps its columns then).
+bool build_sell_patterns(int64_t n, const std::vector<int> &off, const std::vector<int> &scol,
+                         std::vector<unsigned short> &pid, std::vector<int> &dict,
+                         std::vector<int> &dict_off, int64_t max_dict);
 // comprow::bicg (GEFEL/cr.cpp:1353) of a compressed_rows matrix on device vectors;
 // fills info except rel_residual
 int solve_bicg(ssf_matrix *m, const ssf_solve_params *p, const double *y, double *x,

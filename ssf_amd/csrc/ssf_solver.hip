@@ -148,10 +148,13 @@ __device__ __forceinline__ const float *sell_values<float>(const CsrDev &A)
 }
 
 // One wavefront per 64-row slice, one lane per row: the lane streams its row's
-// entries SELL_V at a time (one 16/32-byte value load and one 16-byte column
-// load), gathers x and sums in column order. No LDS, no barrier. A block's waves
-// take the slices of its contiguous slice range round-robin.
-template <typename T, bool I16, int KIND>
+// entries SELL_V at a time (one 16/32-byte value load and the columns), gathers x
+// and sums in column order. No LDS, no barrier. A block's waves take the slices
+// of its contiguous slice range round-robin. Column forms (IDX): 0 int32, 1 16-bit
+// offsets from the slice's first row, 2 row patterns: the columns are the row
+// plus offsets shared by every row of the same pattern (read through the caches,
+// so the HBM stream is the values alone).
+template <typename T, int IDX, int KIND>
 __global__ __launch_bounds__(TPB) void k_spmv_sell(CsrDev A, const T *__restrict__ x,
                                                    T *__restrict__ y, const T *__restrict__ b,
                                                    T *__restrict__ partial, const int *done,
@@ -173,10 +176,40 @@ __global__ __launch_bounds__(TPB) void k_spmv_sell(CsrDev A, const T *__restrict
     const int off = A.sell_off[sl];
     const int w = (A.sell_off[sl + 1] - off) / 64;  // a multiple of SELL_V
     T sum = T(0);
-    for (int q = 0; q < w; q += SELL_V) {
+    const int *__restrict__ po = nullptr;
+    int rb = 0;
+    if (IDX == 2) {
+      rb = min(sl * 64 + lane, A.n - 1);
+      po = A.sell_dict + A.sell_dict_off[A.sell_pid[sl * 64 + lane]];
+    }
+    int q0 = 0;
+    if (IDX == 2) {  // two groups per step: both value loads in flight together
+      for (; q0 + 2 * SELL_V <= w; q0 += 2 * SELL_V) {
+        const int k = off + q0 * 64 + lane * SELL_V;
+        const tv v0 = ldnt((const tv *)(sval + k));
+        const tv v1 = ldnt((const tv *)(sval + k + 64 * SELL_V));
+        const iv o0 = *(const iv *)(po + q0);
+        const iv o1 = *(const iv *)(po + q0 + SELL_V);
+        T g0[SELL_V], g1[SELL_V];
+#pragma unroll
+        for (int u = 0; u < SELL_V; u++) {
+          g0[u] = x[rb + o0[u]];
+          g1[u] = x[rb + o1[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < SELL_V; u++) sum += v0[u] * g0[u];
+#pragma unroll
+        for (int u = 0; u < SELL_V; u++) sum += v1[u] * g1[u];
+      }
+    }
+    for (int q = q0; q < w; q += SELL_V) {
       const int k = off + q * 64 + lane * SELL_V;
       const tv vv = ldnt((const tv *)(sval + k));
-      if (I16) {  // 16-bit offsets from the slice's first row
+      if (IDX == 2) {
+        const iv oo = *(const iv *)(po + q);
+#pragma unroll
+        for (int u = 0; u < SELL_V; u++) sum += vv[u] * x[rb + oo[u]];
+      } else if (IDX == 1) {  // 16-bit offsets from the slice's first row
         const sv cc = ldnt((const sv *)(A.sell_col16 + k));
         const T *__restrict__ xs = x + sl * 64;
 #pragma unroll
@@ -212,10 +245,12 @@ static hipError_t spmv_k(const SpmvTune &t, const CsrDev &A, const T *val, const
 #define SSF_VEC(G) SSF_LAUNCH((k_spmv_vec<T, G, KIND>), A, val, x, y, b, partial, done, iter)
 #define SSF_STR(I, G) SSF_LAUNCH((k_spmv_stream<T, I, G, KIND>), A, val, x, y, b, partial, done, iter)
   if (t.format == 2 && A.nslice > 0) {
-    if (t.idx16 && A.sell_col16)
-      SSF_LAUNCH((k_spmv_sell<T, true, KIND>), A, x, y, b, partial, done, iter);
+    if (t.idx16 == 2 && A.sell_pid)
+      SSF_LAUNCH((k_spmv_sell<T, 2, KIND>), A, x, y, b, partial, done, iter);
+    else if (t.idx16 && A.sell_col16)
+      SSF_LAUNCH((k_spmv_sell<T, 1, KIND>), A, x, y, b, partial, done, iter);
     else
-      SSF_LAUNCH((k_spmv_sell<T, false, KIND>), A, x, y, b, partial, done, iter);
+      SSF_LAUNCH((k_spmv_sell<T, 0, KIND>), A, x, y, b, partial, done, iter);
   } else if (t.format == 1 && A.ntiles > 0) {
     const bool i16 = t.idx16 && A.col16;
     switch (t.lanes) {

@@ -171,9 +171,9 @@ def test_cantilever_full_config(ctx):
 
 
 TILINGS64 = ([(0, l, 64, 0) for l in (4, 8, 16, 32, 64)] + [(1, l, 64, 0) for l in (4, 8, 16)]
-             + [(2, 0, b, i) for b in (1, 8, 64) for i in (0, 1)])
+             + [(2, 0, b, i) for b in (1, 8, 64) for i in (0, 1, 2)])
 TILINGS32 = [(0, 8, 64, 0), (1, 4, 64, 0), (1, 8, 64, 1), (1, 16, 16, 1), (2, 0, 8, 0),
-             (2, 0, 64, 1)]
+             (2, 0, 64, 1), (2, 0, 8, 2), (2, 0, 64, 2)]
 
 
 @pytest.mark.parametrize("name", ["hex8_elast", "hex8_heat", "quad4"])
@@ -195,6 +195,36 @@ def test_spmv_tilings(ctx, name):
         m.set_tune(gpu.MIXED, t)
         x, info = m.solve(rhs[0], ni=50000, res=1e-26, precision=gpu.MIXED)
         assert info.status in (1, 4) and rel(x, xo) <= 1e-8, (t, rel(x, xo))
+
+
+def test_sell_row_patterns(ctx):
+    """Row patterns (index form 2): a mesh matrix repeats few column-offset lists and
+    its products equal the 32-bit form's; a pattern without repetition beyond the
+    dictionary's limit keeps the column forms only."""
+    prob = mesh.heat_cube(17, seed=9)  # n not a multiple of 64: rows past n in the last slice
+    adr, ci, a, rhs = O.assemble_problem(prob, "scr")
+    m = gpu.Matrix.from_csr(ctx, gpu.STORAGE_SCR, adr, ci)
+    m.set_values(a)
+    b = np.random.default_rng(3).standard_normal(prob.n)
+    m.set_tune(gpu.FP64, (2, 0, 64, 0))
+    y0 = m.gmxv(b)
+    m.set_tune(gpu.FP64, (2, 0, 64, 2))
+    assert np.array_equal(m.gmxv(b), y0)  # same entries, same order
+    can, act = m.spmv_bytes(gpu.FP64)
+    assert act < can
+    m.close()
+    # a random pattern: every row its own offsets, more than 65535 of them
+    n = 70000
+    rng = np.random.default_rng(4)
+    rows = [sorted(set(rng.integers(0, n, 6).tolist()) | {i}) for i in range(n)]
+    adr2 = np.zeros(n + 1, np.int64)
+    adr2[1:] = np.cumsum([len(r) for r in rows])
+    ci2 = np.concatenate([np.array(r, np.int64) for r in rows])
+    c = gpu.Matrix.from_csr(ctx, gpu.STORAGE_CR, adr2, ci2)
+    c.set_values(rng.standard_normal(ci2.shape[0]))
+    with pytest.raises(gpu.SsfError):
+        c.set_tune(gpu.FP64, (2, 0, 64, 2))
+    c.close()
 
 
 def test_autotune_keeps_parity(ctx):
