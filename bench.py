@@ -732,9 +732,11 @@ def main():
         "spmv_gbs": {"fp32": gbs32, "fp64": gbs64, "copy_kernel": copy_gbs},
         "roofline": {"bound": "hbm", "achieved": gbs32, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                      "frac": gbs32 * 1e9 / PEAK_HBM, "traffic": traffic,
-                     "kernel": {2: "k_spmv_sell<float, %d, 1> (FP32 SELL-64 SpMV of the mixed solve, %s)"
-                                % (tune32[3], {0: "32-bit columns", 1: "16-bit columns",
-                                               2: "row patterns"}.get(tune32[3], "?")),
+                     "kernel": {2: ("k_spmv_sell_pat<float, 1, %d> (FP32 SELL-64 SpMV of the mixed "
+                                    "solve, row patterns)" % (8 if tune32[1] >= 8 else 4 if tune32[1] >= 4 else 2)
+                                    if tune32[3] == 2 else
+                                    "k_spmv_sell<float, %d, 1> (FP32 SELL-64 SpMV of the mixed solve, %s)"
+                                    % (tune32[3], "16-bit columns" if tune32[3] else "32-bit columns")),
                                 1: "k_spmv_stream<float> (FP32 stream SpMV of the mixed solve)"}.get(
                                     tune32[0], "k_spmv_vec<float> (FP32 SpMV of the mixed solve)"),
                      "bytes_per_launch": b32_act, "bytes_per_launch_canonical_csr": b32_can,

@@ -429,7 +429,9 @@ int ssf_spmv_bench(ssf_matrix *m, int precision, int reps, double *usec);
 /* SpMV autotuner: time every tiling of the FP64 (precision 0) or FP32 (1) SpMV
  * on this matrix and keep the fastest for later solves. tune_out (may be NULL)
  * receives {format (0 row-vector, 1 stream tiles, 2 SELL-64 slices), lanes per
- * row (formats 0, 1), blocks, idx16 (16-bit tile columns, format 1 FP32)}. */
+ * row (formats 0, 1) or value loads in flight per lane (format 2 with row
+ * patterns: 2, 4 or 8), blocks, idx16 (column form: 0 int32, 1 16-bit tile or
+ * slice offsets, 2 SELL row patterns)}. */
 int ssf_matrix_autotune(ssf_matrix *m, int precision, int reps, int32_t *tune_out);
 /* pin the SpMV tiling {format, lanes, blocks, idx16}; the result of a solve
  * depends only on the matrix, the right-hand side and the two tilings */

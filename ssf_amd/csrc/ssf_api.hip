@@ -1700,15 +1700,18 @@ extern "C" int ssf_matrix_autotune(ssf_matrix *m, int precision, int reps, int32
         }
   if (m->nslice > 0)
     for (int b : {512, 1024, 2048, 4096})
-      for (int i16 = 0; i16 <= (m->sell_pid ? 2 : (m->sell_col16 ? 1 : 0)); i16++) {
-        if (b > std::max(8, (m->nslice + 3) / 4)) continue;
-        if (i16 == 1 && !m->sell_col16) continue;
-        SpmvTune t;
-        t.format = 2;
-        t.blocks = b;
-        t.idx16 = i16;
-        cand.push_back(t);
-      }
+      for (int i16 = 0; i16 <= (m->sell_pid ? 2 : (m->sell_col16 ? 1 : 0)); i16++)
+        for (int dep : {2, 4, 8}) {  // row patterns: value loads in flight per lane
+          if (b > std::max(8, (m->nslice + 3) / 4)) continue;
+          if (i16 == 1 && !m->sell_col16) continue;
+          if (i16 != 2 && dep != 2) continue;
+          SpmvTune t;
+          t.format = 2;
+          t.lanes = dep;
+          t.blocks = b;
+          t.idx16 = i16;
+          cand.push_back(t);
+        }
   SpmvTune best = precision ? m->tune32 : m->tune64;
   double best_t = 1e300;
   for (const SpmvTune &t : cand) {
@@ -1751,6 +1754,10 @@ extern "C" int ssf_matrix_set_tune(ssf_matrix *m, int precision, const int32_t *
   else if (t.format == 1)
     SSF_CHECK(t.lanes == 4 || t.lanes == 8 || t.lanes == 16, SSF_EINVAL,
               "ssf_matrix_set_tune: stream lanes must be 4, 8 or 16");
+  else
+    SSF_CHECK(t.lanes >= 0 && t.lanes <= 8, SSF_EINVAL,
+              "ssf_matrix_set_tune: SELL depth must be in [0, 8] (value loads in flight per "
+              "lane of the row-pattern form: 8 or more, 4 or more, else 2)");
   SSF_CHECK(t.blocks >= 1 && t.blocks <= 4096, SSF_EINVAL, "ssf_matrix_set_tune: blocks out of [1, 4096]");
   SSF_CHECK(t.idx16 == 0 || (t.idx16 == 1 && precision && t.format == 1 && m->col16) ||
                 (t.idx16 == 1 && t.format == 2 && m->sell_col16) ||

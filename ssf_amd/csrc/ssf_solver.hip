@@ -151,9 +151,7 @@ __device__ __forceinline__ const float *sell_values<float>(const CsrDev &A)
 // entries SELL_V at a time (one 16/32-byte value load and the columns), gathers x
 // and sums in column order. No LDS, no barrier. A block's waves take the slices
 // of its contiguous slice range round-robin. Column forms (IDX): 0 int32, 1 16-bit
-// offsets from the slice's first row, 2 row patterns: the columns are the row
-// plus offsets shared by every row of the same pattern (read through the caches,
-// so the HBM stream is the values alone).
+// offsets from the slice's first row.
 template <typename T, int IDX, int KIND>
 __global__ __launch_bounds__(TPB) void k_spmv_sell(CsrDev A, const T *__restrict__ x,
                                                    T *__restrict__ y, const T *__restrict__ b,
@@ -176,40 +174,10 @@ __global__ __launch_bounds__(TPB) void k_spmv_sell(CsrDev A, const T *__restrict
     const int off = A.sell_off[sl];
     const int w = (A.sell_off[sl + 1] - off) / 64;  // a multiple of SELL_V
     T sum = T(0);
-    const int *__restrict__ po = nullptr;
-    int rb = 0;
-    if (IDX == 2) {
-      rb = min(sl * 64 + lane, A.n - 1);
-      po = A.sell_dict + A.sell_dict_off[A.sell_pid[sl * 64 + lane]];
-    }
-    int q0 = 0;
-    if (IDX == 2) {  // two groups per step: both value loads in flight together
-      for (; q0 + 2 * SELL_V <= w; q0 += 2 * SELL_V) {
-        const int k = off + q0 * 64 + lane * SELL_V;
-        const tv v0 = ldnt((const tv *)(sval + k));
-        const tv v1 = ldnt((const tv *)(sval + k + 64 * SELL_V));
-        const iv o0 = *(const iv *)(po + q0);
-        const iv o1 = *(const iv *)(po + q0 + SELL_V);
-        T g0[SELL_V], g1[SELL_V];
-#pragma unroll
-        for (int u = 0; u < SELL_V; u++) {
-          g0[u] = x[rb + o0[u]];
-          g1[u] = x[rb + o1[u]];
-        }
-#pragma unroll
-        for (int u = 0; u < SELL_V; u++) sum += v0[u] * g0[u];
-#pragma unroll
-        for (int u = 0; u < SELL_V; u++) sum += v1[u] * g1[u];
-      }
-    }
-    for (int q = q0; q < w; q += SELL_V) {
+    for (int q = 0; q < w; q += SELL_V) {
       const int k = off + q * 64 + lane * SELL_V;
       const tv vv = ldnt((const tv *)(sval + k));
-      if (IDX == 2) {
-        const iv oo = *(const iv *)(po + q);
-#pragma unroll
-        for (int u = 0; u < SELL_V; u++) sum += vv[u] * x[rb + oo[u]];
-      } else if (IDX == 1) {  // 16-bit offsets from the slice's first row
+      if (IDX == 1) {  // 16-bit offsets from the slice's first row
         const sv cc = ldnt((const sv *)(A.sell_col16 + k));
         const T *__restrict__ xs = x + sl * 64;
 #pragma unroll
@@ -220,6 +188,106 @@ __global__ __launch_bounds__(TPB) void k_spmv_sell(CsrDev A, const T *__restrict
         for (int u = 0; u < SELL_V; u++) sum += vv[u] * x[cc[u]];
       }
     }
+    const int row = sl * 64 + lane;
+    if (row < A.n) finish_row<T, 1, KIND>(row, sum, x, y, b, acc);
+  }
+  if (KIND != 0) {
+    const T t = block_sum(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+  }
+}
+
+// NG consecutive groups of SELL_V entries of one row in the row-pattern form: all
+// value loads, then all pattern-offset loads, then all gathers are issued before
+// the first product, so the lane waits once per call; the row sum grows in
+// column order.
+template <typename T, int NG>
+__device__ __forceinline__ T sell_pat_groups(const T *__restrict__ sv, const int *__restrict__ po,
+                                             const T *__restrict__ x, int rb, T sum)
+{
+  typedef T tv __attribute__((ext_vector_type(SELL_V)));
+  typedef int iv __attribute__((ext_vector_type(SELL_V)));
+  tv v[NG];
+  iv o[NG];
+  T g[NG][SELL_V];
+#pragma unroll
+  for (int k = 0; k < NG; k++) v[k] = ldnt((const tv *)(sv + k * 64 * SELL_V));
+#pragma unroll
+  for (int k = 0; k < NG; k++) o[k] = *(const iv *)(po + k * SELL_V);
+#pragma unroll
+  for (int k = 0; k < NG; k++)
+#pragma unroll
+    for (int u = 0; u < SELL_V; u++) g[k][u] = x[rb + o[k][u]];
+#pragma unroll
+  for (int k = 0; k < NG; k++)
+#pragma unroll
+    for (int u = 0; u < SELL_V; u++) sum += v[k][u] * g[k][u];
+  return sum;
+}
+
+// the last ng < D groups of a row (ng is the same for the whole wavefront)
+template <typename T, int D>
+__device__ __forceinline__ T sell_pat_tail(int ng, const T *__restrict__ sv,
+                                           const int *__restrict__ po, const T *__restrict__ x,
+                                           int rb, T sum)
+{
+  if constexpr (D > 1) {
+    if (ng == D - 1) return sell_pat_groups<T, D - 1>(sv, po, x, rb, sum);
+    return sell_pat_tail<T, D - 1>(ng, sv, po, x, rb, sum);
+  }
+  return sum;
+}
+
+// SELL-64 with row patterns: the columns of a row are the row plus the offsets of
+// its pattern, kept once in a dictionary read through the caches, so the HBM
+// stream is the values and a 2-byte pattern id per row. The slice bounds and the
+// pattern offsets of a wave's next slice are loaded while the current slice
+// streams, and a row is taken D groups (D * SELL_V entries) per round, each round
+// with all of its loads in flight before the first product. The row is summed in
+// column order, as in the column forms, so the products are equal bit for bit.
+template <typename T, int KIND, int D>
+__global__ __launch_bounds__(TPB) void k_spmv_sell_pat(CsrDev A, const T *__restrict__ x,
+                                                       T *__restrict__ y, const T *__restrict__ b,
+                                                       T *__restrict__ partial, const int *done,
+                                                       long long *iter)
+{
+  if (done && *done) return;
+  if (iter && blockIdx.x == 0 && threadIdx.x == 0) *iter += 1;
+  const T *__restrict__ sval = sell_values<T>(A);
+  const int nb = gridDim.x;
+  const int spb = (A.nslice + nb - 1) / nb;
+  const int lb = xcd_block(blockIdx.x, nb);
+  const int s0 = lb * spb, s1 = min(A.nslice, s0 + spb);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), lane = threadIdx.x % 64;
+  T acc = T(0);
+  // slice bounds one slice ahead, pattern offsets one and pattern ids two
+  // slices ahead of the one being summed
+  const int ST = TPB / 64;
+  int sl = s0 + wave;
+  int offc = 0, endc = 0, dofc = 0, pid1 = 0;
+  if (sl < s1) {
+    offc = A.sell_off[sl];
+    endc = A.sell_off[sl + 1];
+    dofc = A.sell_dict_off[A.sell_pid[sl * 64 + lane]];
+    if (sl + ST < s1) pid1 = A.sell_pid[(sl + ST) * 64 + lane];
+  }
+  for (; sl < s1; sl += ST) {
+    const int off = __builtin_amdgcn_readfirstlane(offc);
+    const int w = __builtin_amdgcn_readfirstlane(endc - offc) / 64;  // a multiple of SELL_V
+    const int *__restrict__ po = A.sell_dict + dofc;
+    const int rb = min(sl * 64 + lane, A.n - 1);
+    if (sl + ST < s1) {
+      offc = A.sell_off[sl + ST];
+      endc = A.sell_off[sl + ST + 1];
+      dofc = A.sell_dict_off[pid1];
+      if (sl + 2 * ST < s1) pid1 = A.sell_pid[(sl + 2 * ST) * 64 + lane];
+    }
+    const T *__restrict__ sv = sval + off + lane * SELL_V;
+    T sum = T(0);
+    int q0 = 0;
+    for (; q0 + D * SELL_V <= w; q0 += D * SELL_V)
+      sum = sell_pat_groups<T, D>(sv + q0 * 64, po + q0, x, rb, sum);
+    if (q0 < w) sum = sell_pat_tail<T, D>((w - q0) / SELL_V, sv + q0 * 64, po + q0, x, rb, sum);
     const int row = sl * 64 + lane;
     if (row < A.n) finish_row<T, 1, KIND>(row, sum, x, y, b, acc);
   }
@@ -245,9 +313,11 @@ static hipError_t spmv_k(const SpmvTune &t, const CsrDev &A, const T *val, const
 #define SSF_VEC(G) SSF_LAUNCH((k_spmv_vec<T, G, KIND>), A, val, x, y, b, partial, done, iter)
 #define SSF_STR(I, G) SSF_LAUNCH((k_spmv_stream<T, I, G, KIND>), A, val, x, y, b, partial, done, iter)
   if (t.format == 2 && A.nslice > 0) {
-    if (t.idx16 == 2 && A.sell_pid)
-      SSF_LAUNCH((k_spmv_sell<T, 2, KIND>), A, x, y, b, partial, done, iter);
-    else if (t.idx16 && A.sell_col16)
+    if (t.idx16 == 2 && A.sell_pid) {
+      if (t.lanes >= 8) SSF_LAUNCH((k_spmv_sell_pat<T, KIND, 8>), A, x, y, b, partial, done, iter);
+      else if (t.lanes >= 4) SSF_LAUNCH((k_spmv_sell_pat<T, KIND, 4>), A, x, y, b, partial, done, iter);
+      else SSF_LAUNCH((k_spmv_sell_pat<T, KIND, 2>), A, x, y, b, partial, done, iter);
+    } else if (t.idx16 && A.sell_col16)
       SSF_LAUNCH((k_spmv_sell<T, 1, KIND>), A, x, y, b, partial, done, iter);
     else
       SSF_LAUNCH((k_spmv_sell<T, 0, KIND>), A, x, y, b, partial, done, iter);

@@ -171,9 +171,10 @@ def test_cantilever_full_config(ctx):
 
 
 TILINGS64 = ([(0, l, 64, 0) for l in (4, 8, 16, 32, 64)] + [(1, l, 64, 0) for l in (4, 8, 16)]
-             + [(2, 0, b, i) for b in (1, 8, 64) for i in (0, 1, 2)])
+             + [(2, 0, b, i) for b in (1, 8, 64) for i in (0, 1, 2)]
+             + [(2, d, b, 2) for d in (4, 8) for b in (1, 64)])
 TILINGS32 = [(0, 8, 64, 0), (1, 4, 64, 0), (1, 8, 64, 1), (1, 16, 16, 1), (2, 0, 8, 0),
-             (2, 0, 64, 1), (2, 0, 8, 2), (2, 0, 64, 2)]
+             (2, 0, 64, 1), (2, 0, 8, 2), (2, 0, 64, 2), (2, 4, 8, 2), (2, 8, 64, 2)]
 
 
 @pytest.mark.parametrize("name", ["hex8_elast", "hex8_heat", "quad4"])
@@ -213,6 +214,20 @@ def test_sell_row_patterns(ctx):
     can, act = m.spmv_bytes(gpu.FP64)
     assert act < can
     m.close()
+    # every depth (value loads in flight) and block count, on heat rows (width 28)
+    # and elasticity rows (width 84: several load rounds per slice, a clamped tail)
+    for prob in (mesh.heat_cube(17, seed=9), mesh.elastic_column(5, 6, 7, seed=10)):
+        adr, ci, a, rhs = O.assemble_problem(prob, "scr")
+        m = gpu.Matrix.from_csr(ctx, gpu.STORAGE_SCR, adr, ci)
+        m.set_values(a)
+        b = np.random.default_rng(5).standard_normal(prob.n)
+        m.set_tune(gpu.FP64, (2, 0, 64, 0))
+        y0 = m.gmxv(b)
+        for d in (2, 4, 8):
+            for nb in (1, 3, 64):
+                m.set_tune(gpu.FP64, (2, d, nb, 2))
+                assert np.array_equal(m.gmxv(b), y0), (prob.name, d, nb)
+        m.close()
     # a random pattern: every row its own offsets, more than 65535 of them
     n = 70000
     rng = np.random.default_rng(4)
