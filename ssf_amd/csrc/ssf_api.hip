@@ -639,7 +639,9 @@ int ssf::matrix_create(ssf_ctx *ctx, int storage, int64_t n, const int64_t *adr,
     // row patterns: the column stream of a mesh matrix repeats a few offset lists
     std::vector<unsigned short> pid;
     std::vector<int> dict, dict_off;
-    if (e == hipSuccess && build_sell_patterns(n, soff, scol, pid, dict, dict_off, 1 << 22)) {
+    // (the pattern kernel addresses values and x by 32-bit byte offsets)
+    const bool fits32 = std::max<int64_t>(m->sell_tot, n) * (int64_t)sizeof(double) < (int64_t(1) << 32);
+    if (e == hipSuccess && fits32 && build_sell_patterns(n, soff, scol, pid, dict, dict_off, 1 << 22)) {
       m->sell_npat = (int)dict_off.size();
       chk(dalloc(&m->sell_pid, pid.size(), ctx->stream));
       chk(dalloc(&m->sell_dict, dict.size(), ctx->stream));

@@ -197,12 +197,22 @@ __global__ __launch_bounds__(TPB) void k_spmv_sell(CsrDev A, const T *__restrict
   }
 }
 
+// an element at a 32-bit byte offset from a base shared by the wavefront (the
+// base stays in scalar registers, the offset takes one vector register)
+template <typename U>
+__device__ __forceinline__ const U *at_byte(const void *base, unsigned off)
+{
+  return (const U *)((const char *)base + off);
+}
+
 // NG consecutive groups of SELL_V entries of one row in the row-pattern form: all
 // value loads, then all pattern-offset loads, then all gathers are issued before
 // the first product, so the lane waits once per call; the row sum grows in
-// column order.
+// column order. vb: byte offset of the first group's values in sval; ob: byte
+// offset of its pattern offsets in dict; rb: the row.
 template <typename T, int NG>
-__device__ __forceinline__ T sell_pat_groups(const T *__restrict__ sv, const int *__restrict__ po,
+__device__ __forceinline__ T sell_pat_groups(const T *__restrict__ sval, unsigned vb,
+                                             const int *__restrict__ dict, unsigned ob,
                                              const T *__restrict__ x, int rb, T sum)
 {
   typedef T tv __attribute__((ext_vector_type(SELL_V)));
@@ -211,13 +221,13 @@ __device__ __forceinline__ T sell_pat_groups(const T *__restrict__ sv, const int
   iv o[NG];
   T g[NG][SELL_V];
 #pragma unroll
-  for (int k = 0; k < NG; k++) v[k] = ldnt((const tv *)(sv + k * 64 * SELL_V));
+  for (int k = 0; k < NG; k++) v[k] = ldnt(at_byte<tv>(sval, vb + k * 64 * SELL_V * sizeof(T)));
 #pragma unroll
-  for (int k = 0; k < NG; k++) o[k] = *(const iv *)(po + k * SELL_V);
+  for (int k = 0; k < NG; k++) o[k] = *at_byte<iv>(dict, ob + k * SELL_V * sizeof(int));
 #pragma unroll
   for (int k = 0; k < NG; k++)
 #pragma unroll
-    for (int u = 0; u < SELL_V; u++) g[k][u] = x[rb + o[k][u]];
+    for (int u = 0; u < SELL_V; u++) g[k][u] = *at_byte<T>(x, (unsigned)(rb + o[k][u]) * sizeof(T));
 #pragma unroll
   for (int k = 0; k < NG; k++)
 #pragma unroll
@@ -227,13 +237,13 @@ __device__ __forceinline__ T sell_pat_groups(const T *__restrict__ sv, const int
 
 // the last ng < D groups of a row (ng is the same for the whole wavefront)
 template <typename T, int D>
-__device__ __forceinline__ T sell_pat_tail(int ng, const T *__restrict__ sv,
-                                           const int *__restrict__ po, const T *__restrict__ x,
-                                           int rb, T sum)
+__device__ __forceinline__ T sell_pat_tail(int ng, const T *__restrict__ sval, unsigned vb,
+                                           const int *__restrict__ dict, unsigned ob,
+                                           const T *__restrict__ x, int rb, T sum)
 {
   if constexpr (D > 1) {
-    if (ng == D - 1) return sell_pat_groups<T, D - 1>(sv, po, x, rb, sum);
-    return sell_pat_tail<T, D - 1>(ng, sv, po, x, rb, sum);
+    if (ng == D - 1) return sell_pat_groups<T, D - 1>(sval, vb, dict, ob, x, rb, sum);
+    return sell_pat_tail<T, D - 1>(ng, sval, vb, dict, ob, x, rb, sum);
   }
   return sum;
 }
@@ -245,6 +255,9 @@ __device__ __forceinline__ T sell_pat_tail(int ng, const T *__restrict__ sv,
 // streams, and a row is taken D groups (D * SELL_V entries) per round, each round
 // with all of its loads in flight before the first product. The row is summed in
 // column order, as in the column forms, so the products are equal bit for bit.
+// Values, dictionary and x are addressed by 32-bit byte offsets: the layout is
+// built with row patterns only when the values and x of either precision fit in
+// 4 GiB.
 template <typename T, int KIND, int D>
 __global__ __launch_bounds__(TPB) void k_spmv_sell_pat(CsrDev A, const T *__restrict__ x,
                                                        T *__restrict__ y, const T *__restrict__ b,
@@ -274,7 +287,7 @@ __global__ __launch_bounds__(TPB) void k_spmv_sell_pat(CsrDev A, const T *__rest
   for (; sl < s1; sl += ST) {
     const int off = __builtin_amdgcn_readfirstlane(offc);
     const int w = __builtin_amdgcn_readfirstlane(endc - offc) / 64;  // a multiple of SELL_V
-    const int *__restrict__ po = A.sell_dict + dofc;
+    const unsigned ob = (unsigned)dofc * sizeof(int);
     const int rb = min(sl * 64 + lane, A.n - 1);
     if (sl + ST < s1) {
       offc = A.sell_off[sl + ST];
@@ -282,12 +295,15 @@ __global__ __launch_bounds__(TPB) void k_spmv_sell_pat(CsrDev A, const T *__rest
       dofc = A.sell_dict_off[pid1];
       if (sl + 2 * ST < s1) pid1 = A.sell_pid[(sl + 2 * ST) * 64 + lane];
     }
-    const T *__restrict__ sv = sval + off + lane * SELL_V;
+    const unsigned vb = (unsigned)(off + lane * SELL_V) * sizeof(T);
     T sum = T(0);
     int q0 = 0;
     for (; q0 + D * SELL_V <= w; q0 += D * SELL_V)
-      sum = sell_pat_groups<T, D>(sv + q0 * 64, po + q0, x, rb, sum);
-    if (q0 < w) sum = sell_pat_tail<T, D>((w - q0) / SELL_V, sv + q0 * 64, po + q0, x, rb, sum);
+      sum = sell_pat_groups<T, D>(sval, vb + q0 * 64 * sizeof(T), A.sell_dict, ob + q0 * sizeof(int),
+                                  x, rb, sum);
+    if (q0 < w)
+      sum = sell_pat_tail<T, D>((w - q0) / SELL_V, sval, vb + q0 * 64 * sizeof(T), A.sell_dict,
+                                ob + q0 * sizeof(int), x, rb, sum);
     const int row = sl * 64 + lane;
     if (row < A.n) finish_row<T, 1, KIND>(row, sum, x, y, b, acc);
   }
