@@ -66,6 +66,17 @@ __device__ __forceinline__ T row_dot(const int *__restrict__ rowptr, const int *
   return s;
 }
 
+// the block's rows [r0, r1) with r0 a multiple of A (for A-wide vector accesses)
+template <int A>
+__device__ __forceinline__ void chunk_al(int n, int &r0, int &r1)
+{
+  const int nb = gridDim.x;
+  const int rpb = ((n + nb - 1) / nb + A - 1) / A * A;
+  const int lb = xcd_block(blockIdx.x, nb);
+  r0 = min(n, lb * rpb);
+  r1 = min(n, r0 + rpb);
+}
+
 __device__ __forceinline__ void chunk(int n, int &r0, int &r1)
 {
   const int nb = gridDim.x;
