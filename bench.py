@@ -361,6 +361,83 @@ def skew_values(adr, ci, a, gamma, seed):
     return np.where(ci == rows, a, a * (1.0 + gamma * t))
 
 
+def bench_elastic(ctx, args, gpu, mesh, torch):
+    """Linear elasticity on the hex8 column nel x nel x 4 nel (64 x 64 x 256 by
+    default: 3.2M unknowns, 257M stored entries of the full matrix; lognormal E,
+    uniform nu, clamped base, nodal body forces; seed 3). Host pattern with row
+    patterns, device assembly, then FP64 Jacobi-PCG (the drop-in) and the mixed
+    solve to --res from x = 0; errors against an FP64 solve to 1e-30 (which ends on
+    the zero-denominator test once the residual is at rounding level). The SpMV
+    launches are timed live as in the heat leg. CPU baseline: --cpu-iters
+    iterations of the oracle's cg_prec on the same system, scaled to the FP64
+    iteration count."""
+    nel = args.elast_nel
+    t0 = time.time()
+    prob = mesh.elastic_column(nel, nel, 4 * nel, seed=3)
+    m = gpu.Matrix.from_elements(ctx, gpu.STORAGE_SCR, prob.n, prob.elem_codes())
+    t_pat = time.time() - t0
+    t0 = time.time()
+    lift = m.assemble(prob)
+    rhs = prob.rhs[0] + (lift if lift is not None else 0.0)
+    t_asm = time.time() - t0
+    tune64 = m.autotune(gpu.FP64)
+    tune32 = m.autotune(gpu.MIXED)
+    log(f"[elast] n={prob.n} nnz={m.nnz}: pattern {t_pat:.1f}s, assembly {t_asm:.2f}s, "
+        f"tilings {tune64} {tune32}")
+    dev = torch.device("cuda:0")
+    b = torch.tensor(rhs, dtype=torch.float64, device=dev)
+    x = torch.zeros(prob.n, dtype=torch.float64, device=dev)
+    xref = torch.zeros(prob.n, dtype=torch.float64, device=dev)
+    info_ref = m.solve_dev(b.data_ptr(), xref.data_ptr(), ni=400000, res=1e-30)
+    torch.cuda.synchronize()
+    xref_norm = torch.linalg.norm(xref).item()
+    log(f"[elast] x_ref: status {info_ref.status} ani {info_ref.ani}")
+    out = {"workload": f"hex8 elasticity column {nel}x{nel}x{4 * nel}, n={prob.n}, nnz={m.nnz}",
+           "res": args.res, "refinement_interval": args.interval,
+           "spmv_tiling": {"fp64": list(tune64), "fp32": list(tune32)},
+           "pattern_s": t_pat, "assembly_s": t_asm}
+    for name, prec, which in (("fp64", gpu.FP64, 0), ("mixed", gpu.MIXED, 1)):
+        def solve():
+            if prec == gpu.FP64:
+                return m.solve_dev(b.data_ptr(), x.data_ptr(), ni=400000, res=args.res)
+            return m.solve_dev(b.data_ptr(), x.data_ptr(), ni=400000, res=args.res,
+                               precision=gpu.MIXED, inner_ni=args.interval)
+        solve()  # warm-up
+        torch.cuda.synchronize()
+        ctx.set_profiling(True)
+        t = time.perf_counter()
+        info = solve()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        spmv_us, samples = ctx.kernel_time(which)
+        ctx.set_profiling(False)
+        can, act = m.spmv_bytes(prec)
+        out[name] = {"solves_per_s": 1.0 / dt, "ms_per_solve": 1e3 * dt,
+                     "rel_err": torch.linalg.norm(x - xref).item() / xref_norm,
+                     "iterations": int(info.iterations), "refinements": int(info.outer),
+                     "status": int(info.status), "spmv_us_live": spmv_us, "spmv_samples": samples,
+                     "spmv_bytes": act, "spmv_gbs": act / (spmv_us * 1e-6) / 1e9 if spmv_us else None}
+        log(f"[elast] {name}: {out[name]}")
+    out["speedup_mixed_vs_fp64"] = out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"]
+    if not args.no_cpu:
+        import oracle as O
+        adr, ci = m.pattern()
+        a = m.get_values()
+        k = max(1, args.cpu_iters // 10)
+        t = time.perf_counter()
+        O.scr_cg_prec(adr, ci, a, rhs, k, 0.0)
+        per_it = (time.perf_counter() - t) / k
+        its = out["fp64"]["iterations"]
+        out["cpu_baseline"] = {"value": 1.0 / (per_it * its), "unit": "solves/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"{k} iterations of the oracle cg_prec (scr, Jacobi, 1 "
+                                         f"thread) on the same system, scaled to the {its} "
+                                         f"iterations of the FP64 recurrence"}
+        out["speedup_fp64_vs_cpu"] = out["fp64"]["solves_per_s"] / out["cpu_baseline"]["value"]
+    m.close()
+    return out
+
+
 def bench_bicg(ctx, args, gpu, mesh, torch, prob):
     """comprow::bicg on the heat cube stored as comprow with nonsymmetric values
     (off-diagonals scaled by 1 + 0.3 t, seeded), from x = 0 to --res, FP64 (the
@@ -547,6 +624,9 @@ def main():
     ap.add_argument("--no-bicg", action="store_true", help="skip the comprow bicg leg")
     ap.add_argument("--no-ildl", action="store_true", help="skip the incomplete LDL^T leg")
     ap.add_argument("--no-renumber", action="store_true", help="skip the Cuthill-McKee leg")
+    ap.add_argument("--no-elast", action="store_true", help="skip the elasticity column leg")
+    ap.add_argument("--elast-nel", type=int, default=64,
+                    help="elasticity column nel x nel x 4 nel elements")
     ap.add_argument("--ildl-steps", type=int, default=1)
     ap.add_argument("--bicg-steps", type=int, default=2)
     ap.add_argument("--serve-count", type=int, default=24)
@@ -695,6 +775,9 @@ def main():
     bicg = None
     if not args.no_bicg:
         bicg = bench_bicg(ctx, args, gpu, mesh, torch, prob)
+    elast = None
+    if not args.no_elast:
+        elast = bench_elastic(ctx, args, gpu, mesh, torch)
 
     batched = None
     if not args.no_batch:
@@ -745,6 +828,7 @@ def main():
         "post": post,
         "transient": transient,
         "renumber": renumber,
+        "elasticity": elast,
         "ildl": ildl,
         "bicg": bicg,
         "batched": batched,

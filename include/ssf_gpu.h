@@ -120,9 +120,9 @@ int ssf_matrix_create_csr(ssf_ctx *ctx, int storage, int64_t n, const int64_t *a
                           const int64_t *ci, ssf_matrix **out);
 /* symcomprow::initiate with the pattern built on the device: the same adr / ci,
  * from a sort of the elements' DOF pairs (no host pattern work). The matrix keeps
- * the SELL-64 layout only: it assembles, multiplies and solves in FP64; the
- * mixed-precision solve, batches and the stream / row-vector SpMV need a matrix
- * from ssf_matrix_create_elem. */
+ * the SELL-64 layout only (16/32-bit columns, no row patterns): it assembles,
+ * multiplies and solves in FP64 and in mixed precision; batches and the stream /
+ * row-vector SpMV need a matrix from ssf_matrix_create_elem. */
 int ssf_matrix_create_elem_device(ssf_ctx *ctx, int64_t n, int64_t ne, int64_t ndofe,
                                   const int64_t *cn, ssf_matrix **out);
 int ssf_matrix_destroy(ssf_matrix *m);

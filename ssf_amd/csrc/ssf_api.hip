@@ -1147,8 +1147,6 @@ extern "C" int ssf_matrix_localize_batch(ssf_matrix *m, int64_t ne, int64_t nc, 
 
 static int ensure_ws(ssf_matrix *m, bool mixed)
 {
-  SSF_CHECK(!(mixed && m->sell_only), SSF_EUNSUP,
-            "the mixed-precision solve needs the full-CSR copy this matrix does not keep");
   Workspace &w = m->ws;
   const int64_t n = std::max<int64_t>(1, m->n);
   const int nb = std::max(m->tune64.blocks, m->tune32.blocks);
@@ -1172,16 +1170,24 @@ static int ensure_ws(ssf_matrix *m, bool mixed)
     SSF_HIP(hipHostMalloc((void **)&w.mix_host, 2 * sizeof(MixState), hipHostMallocDefault));
   }
   if (mixed && !m->val32_valid) {
-    if (!m->val32) SSF_HIP(dalloc(&m->val32, m->nnz, m->ctx->stream));
+    // a SELL-only matrix (device-built pattern) has no full CSR: its FP32 copy
+    // and its norm come from the SELL values
+    if (!m->sell_only) {
+      if (!m->val32) SSF_HIP(dalloc(&m->val32, m->nnz, m->ctx->stream));
+      SSF_HIP(to_f32(m->nnz, m->val, m->val32, 1.0, m->ctx->stream));
+    }
     if (!m->diag32) SSF_HIP(dalloc(&m->diag32, m->n, m->ctx->stream));
-    SSF_HIP(to_f32(m->nnz, m->val, m->val32, 1.0, m->ctx->stream));
     SSF_HIP(to_f32(m->n, m->diag, m->diag32, 1.0, m->ctx->stream));
     if (m->sell_v64) {
       if (!m->sell_v32) SSF_HIP(dalloc(&m->sell_v32, m->sell_tot, m->ctx->stream));
       SSF_HIP(to_f32(m->sell_tot, m->sell_v64, m->sell_v32, 1.0, m->ctx->stream));
     }
     if (!m->norm_a) SSF_HIP(dalloc(&m->norm_a, 1, m->ctx->stream));
-    SSF_HIP(row_absmax((int)m->n, m->rowptr, m->val, w.part, 1024, m->norm_a, m->ctx->stream));
+    if (m->sell_only)
+      SSF_HIP(sell_row_absmax((int)m->n, m->sell_off, m->sell_v64, w.part, 1024, m->norm_a,
+                              m->ctx->stream));
+    else
+      SSF_HIP(row_absmax((int)m->n, m->rowptr, m->val, w.part, 1024, m->norm_a, m->ctx->stream));
     m->val32_valid = true;
   }
   return SSF_OK;

@@ -57,6 +57,9 @@ hipError_t mixed_update(const SpmvTune &t64, const SpmvTune &t32, const CsrDev &
 // ||A||_inf into *out (device); partial: nb doubles of scratch
 hipError_t row_absmax(int n, const int *rowptr, const double *val, double *partial, int nb,
                       double *out, hipStream_t s);
+// the same from the SELL-64 layout (sell_off: nslice + 1 offsets, v: its values)
+hipError_t sell_row_absmax(int n, const int *sell_off, const double *v, double *partial, int nb,
+                           double *out, hipStream_t s);
 hipError_t copy_kernel(int64_t n, const double *a, double *b, hipStream_t s);
 // SELL values from the full CSR values: out[k] = src[k] >= 0 ? val[src[k]] : 0
 hipError_t sell_gather(int64_t tot, const int *src, const double *val, double *out, hipStream_t s);

@@ -956,4 +956,39 @@ hipError_t row_absmax(int n, const int *rowptr, const double *val, double *parti
   return hipGetLastError();
 }
 
+// the same norm from the SELL-64 layout: a row's entries in column order (its
+// padding adds zeros), so the value equals the CSR one bit for bit
+__global__ __launch_bounds__(TPB) void k_sell_row_absmax(int n, const int *__restrict__ sell_off,
+                                                         const double *__restrict__ v,
+                                                         double *__restrict__ partial)
+{
+  int r0, r1;
+  chunk(n, r0, r1);
+  double m = 0.0;
+  for (int i = r0 + threadIdx.x; i < r1; i += TPB) {
+    const int off = sell_off[i / 64], lane = i % 64;
+    const int w = (sell_off[i / 64 + 1] - off) / 64;
+    double s = 0.0;
+    for (int q = 0; q < w; q++) s += fabs(v[off + (q / SELL_V) * 64 * SELL_V + lane * SELL_V + q % SELL_V]);
+    m = fmax(m, s);
+  }
+  __shared__ double lds[TPB];
+  lds[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = TPB / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) lds[threadIdx.x] = fmax(lds[threadIdx.x], lds[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = lds[0];
+}
+
+hipError_t sell_row_absmax(int n, const int *sell_off, const double *v, double *partial, int nb,
+                           double *out, hipStream_t s)
+{
+  if (n <= 0) return hipMemsetAsync(out, 0, sizeof(double), s);
+  hipLaunchKernelGGL(k_sell_row_absmax, dim3(nb), dim3(TPB), 0, s, n, sell_off, v, partial);
+  hipLaunchKernelGGL(k_max_partials, dim3(1), dim3(TPB), 0, s, partial, nb, out);
+  return hipGetLastError();
+}
+
 }  // namespace ssf

@@ -69,6 +69,14 @@ def test_device_pattern_matrix_assembles_and_solves_as_the_host_one(ctx, name):
     xd, idv = md.solve(rhs, ni=20000, res=1e-20)
     assert ih.status == idv.status == 1 and ih.iterations == idv.iterations
     assert np.array_equal(xh, xd)
+    # mixed precision: the SELL-only matrix takes its FP32 copy and its norm from
+    # the SELL values, equal to the host matrix's from the full CSR
+    mh.set_tune(gpu.MIXED, tune)
+    md.set_tune(gpu.MIXED, tune)
+    xh, ih = mh.solve(rhs, ni=20000, res=1e-20, precision=gpu.MIXED)
+    xd, idv = md.solve(rhs, ni=20000, res=1e-20, precision=gpu.MIXED)
+    assert ih.status == idv.status == 1 and ih.iterations == idv.iterations
+    assert np.array_equal(xh, xd)
     # same pattern: the matrices combine (gmatrix::addgm)
     md.add(1.0, mh)
     assert np.array_equal(md.get_values(), 2.0 * mh.get_values())
