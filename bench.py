@@ -18,9 +18,13 @@ The post-solve element pass (temperature gradients and fluxes at the 8 points of
 every element, assembled internal fluxes) runs on the mixed solution and is
 reported under "post" with its own bytes and kernel time. The nonstationary leg
 ("transient") repeats TRFEL's one_step_linear on the same cube with device-resident
-vectors; --only-assembly times
-the element kernels and the post pass of the 64x64x256 elasticity column.
-Prints one JSON line.
+vectors. Further legs, each with its CPU baseline: reverse Cuthill-McKee of the
+cube's mesh ("renumber"), cg_prec with the incomplete LDL^T preconditioner
+("ildl"), comprow bicg with nonsymmetric values ("bicg"), the FP64 and mixed
+solves of the 64x64x256 elasticity column ("elasticity"), 512 quad4 plates with
+8 load cases ("batched") and 24 served requests ("served"); every leg has a
+--no-* switch. --only-assembly times the element kernels and the post pass of
+the 64x64x256 elasticity column. Prints one JSON line.
 """
 from __future__ import annotations
 
