@@ -242,9 +242,30 @@ bool build_sell_patterns(int64_t n, const std::vector<int> &off, const std::vect
 // SELL-64 layout of a full CSR: slices of 64 rows, each row's entries in
 // column order, padded to the slice's longest row (rounded up to SELL_V) with
 // value 0 at the row's own column; src maps a SELL entry to its CSR entry.
+// Column offsets (col - row) that occur in the matrix, ascending, when there are
+// at most SELL_STEN_MAX of them and a layout with one slot per offset in every
+// row holds at most 5 % more entries than the compact one (tot); else empty.
+static std::vector<int> stencil_offsets(int64_t n, const std::vector<int> &rowptr,
+                                        const std::vector<int> &col, int64_t tot)
+{
+  std::vector<int> u;
+  for (int64_t r = 0; r < n; r++)
+    for (int k = rowptr[r]; k < rowptr[r + 1]; k++) {
+      const int o = col[k] - (int)r;
+      if (std::find(u.begin(), u.end(), o) == u.end()) {
+        if ((int)u.size() == SELL_STEN_MAX) return {};
+        u.push_back(o);
+      }
+    }
+  std::sort(u.begin(), u.end());
+  const int64_t w = ((int64_t)u.size() + SELL_V - 1) / SELL_V * SELL_V;
+  if (u.empty() || w * 64 * ((n + 63) / 64) > tot + tot / 20) return {};
+  return u;
+}
+
 bool build_sell(int64_t n, const std::vector<int> &rowptr, const std::vector<int> &col,
                 std::vector<int> &off, std::vector<int> &scol, std::vector<int> &src,
-                std::vector<short> &scol16)
+                std::vector<short> &scol16, std::vector<int> *sten)
 {
   const int64_t nslice = (n + 63) / 64;
   off.assign(nslice + 1, 0);
@@ -259,6 +280,43 @@ bool build_sell(int64_t n, const std::vector<int> &rowptr, const std::vector<int
     if (tot >= (1LL << 31)) return false;
   }
   off[nslice] = (int)tot;
+  std::vector<int> u;
+  if (sten) {
+    u = stencil_offsets(n, rowptr, col, tot);
+    *sten = u;
+  }
+  if (!u.empty()) {
+    // stencil form: every slice as wide as the offset list, slot k <-> u[k]
+    const int64_t w = ((int64_t)u.size() + SELL_V - 1) / SELL_V * SELL_V;
+    if (w * 64 * nslice >= (1LL << 31)) {
+      sten->clear();
+      u.clear();
+    } else {
+      for (int64_t sl = 0; sl <= nslice; sl++) off[sl] = (int)(sl * 64 * w);
+      tot = off[nslice];
+      scol.assign(tot, 0);
+      src.assign(tot, -1);
+      parallel_for(nslice, [&](int64_t b, int64_t e) {
+        for (int64_t sl = b; sl < e; sl++)
+          for (int64_t l = 0; l < 64; l++) {
+            const int64_t r = sl * 64 + l;
+            int k = r < n ? rowptr[r] : 0;
+            const int kend = r < n ? rowptr[r + 1] : 0;
+            for (int64_t q = 0; q < w; q++) {
+              const int64_t s = off[sl] + (q / SELL_V) * 64 * SELL_V + l * SELL_V + q % SELL_V;
+              if (q < (int64_t)u.size() && k < kend && col[k] - (int)r == u[q]) {
+                scol[s] = col[k];
+                src[s] = k;
+                k++;
+              } else {
+                scol[s] = (int)std::min<int64_t>(r, n - 1);  // absent: value 0
+              }
+            }
+          }
+      });
+    }
+  }
+  if (u.empty()) {
   scol.assign(tot, 0);
   src.assign(tot, -1);
   parallel_for(nslice, [&](int64_t b, int64_t e) {
@@ -279,6 +337,7 @@ bool build_sell(int64_t n, const std::vector<int> &rowptr, const std::vector<int
       }
     }
   });
+  }
   // 16-bit column offsets from each slice's first row, when every offset fits
   std::vector<char> fits(nslice, 1);
   scol16.assign(tot, 0);
@@ -599,7 +658,7 @@ int ssf::matrix_create(ssf_ctx *ctx, int storage, int64_t n, const int64_t *adr,
   const bool have_tiles = stream_tiles && build_tiles(n, rowptr, col, tiles, col16);
   std::vector<int> soff, scol, ssrc;
   std::vector<short> scol16;
-  const bool have_sell = n > 0 && build_sell(n, rowptr, col, soff, scol, ssrc, scol16);
+  const bool have_sell = n > 0 && build_sell(n, rowptr, col, soff, scol, ssrc, scol16, &m->sten);
   hipError_t e = hipSuccess;
   auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
   chk(dalloc(&m->rowptr, n + 1, ctx->stream));
@@ -688,6 +747,7 @@ int ssf::matrix_create(ssf_ctx *ctx, int storage, int64_t n, const int64_t *adr,
   m->tune32 = default_tune(n, m->nnz, m->ntiles, m->nslice, m->col16 != nullptr);
   if (m->sell_col16) m->tune64.idx16 = m->tune32.idx16 = (m->tune64.format == 2) ? 1 : 0;
   if (m->sell_pid && m->tune64.format == 2) m->tune64.idx16 = m->tune32.idx16 = 2;
+  if (!m->sten.empty() && m->tune64.format == 2) m->tune64.idx16 = m->tune32.idx16 = 3;
   *out = m;
   return SSF_OK;
 }
@@ -1708,10 +1768,12 @@ extern "C" int ssf_matrix_autotune(ssf_matrix *m, int precision, int reps, int32
         }
   if (m->nslice > 0)
     for (int b : {512, 1024, 2048, 4096})
-      for (int i16 = 0; i16 <= (m->sell_pid ? 2 : (m->sell_col16 ? 1 : 0)); i16++)
+      for (int i16 = 0; i16 <= 3; i16++)
         for (int dep : {2, 4, 8}) {  // row patterns: value loads in flight per lane
           if (b > std::max(8, (m->nslice + 3) / 4)) continue;
           if (i16 == 1 && !m->sell_col16) continue;
+          if (i16 == 2 && !m->sell_pid) continue;
+          if (i16 == 3 && m->sten.empty()) continue;
           if (i16 != 2 && dep != 2) continue;
           SpmvTune t;
           t.format = 2;
@@ -1769,11 +1831,12 @@ extern "C" int ssf_matrix_set_tune(ssf_matrix *m, int precision, const int32_t *
   SSF_CHECK(t.blocks >= 1 && t.blocks <= 4096, SSF_EINVAL, "ssf_matrix_set_tune: blocks out of [1, 4096]");
   SSF_CHECK(t.idx16 == 0 || (t.idx16 == 1 && precision && t.format == 1 && m->col16) ||
                 (t.idx16 == 1 && t.format == 2 && m->sell_col16) ||
-                (t.idx16 == 2 && t.format == 2 && m->sell_pid),
+                (t.idx16 == 2 && t.format == 2 && m->sell_pid) ||
+                (t.idx16 == 3 && t.format == 2 && !m->sten.empty()),
             SSF_EINVAL,
             "ssf_matrix_set_tune: index form 1 (16-bit columns) for the FP32 stream operator or "
-            "the SELL operator of a matrix that has them, 2 (row patterns) for a SELL operator "
-            "that has them");
+            "the SELL operator of a matrix that has them, 2 (row patterns) or 3 (stencil form) "
+            "for a SELL operator that has them");
   if (precision) m->tune32 = t;
   else m->tune64 = t;
   return SSF_OK;
@@ -1828,7 +1891,9 @@ extern "C" int ssf_spmv_bytes(const ssf_matrix *m, int precision, double *canoni
   double a = (double)m->nnz * (sv + ib) + 4.0 * (double)(m->n + 1) + 2.0 * sv * (double)m->n;
   if (t.format == 1) a += 4.0 * (double)(m->ntiles + 1);
   if (t.format == 2 && m->nslice > 0) {  // padded slices, slice offsets instead of row pointers
-    if (t.idx16 == 2 && m->sell_pid)  // a 2-byte pattern id per layout row instead of columns
+    if (t.idx16 == 3 && !m->sten.empty())  // stencil form: the values alone
+      a = (double)m->sell_tot * sv + 2.0 * sv * (double)m->n;
+    else if (t.idx16 == 2 && m->sell_pid)  // a 2-byte pattern id per layout row instead of columns
       a = (double)m->sell_tot * sv + 2.0 * 64.0 * (double)m->nslice +
           4.0 * (double)(m->nslice + 1) + 2.0 * sv * (double)m->n;
     else

@@ -129,6 +129,8 @@ struct SpmvTune {
 };
 
 // Device view of a full CSR matrix and its stream tiling.
+constexpr int SELL_STEN_MAX = 32;  // slots of the stencil form of the SELL layout
+
 struct CsrDev {
   int n = 0;
   const int *rowptr = nullptr;
@@ -148,6 +150,11 @@ struct CsrDev {
   const unsigned short *sell_pid = nullptr;
   const int *sell_dict = nullptr;
   const int *sell_dict_off = nullptr;
+  // stencil form: every row of the layout has the same slots, slot k at column
+  // row + sten[k] (sten ascending; a slot the row lacks holds value 0 at the
+  // row's own column), so the columns need no stream and no dictionary
+  int sten_n = 0;
+  int sten[SELL_STEN_MAX] = {};
   const double *sell_v64 = nullptr;
   const float *sell_v32 = nullptr;
 };
@@ -229,6 +236,7 @@ struct ssf_matrix {
   unsigned short *sell_pid = nullptr;  // row patterns (null when not built)
   int *sell_dict = nullptr, *sell_dict_off = nullptr;
   int sell_npat = 0;
+  std::vector<int> sten;  // stencil form of the SELL layout (empty: not stencil-aligned)
   double *sell_v64 = nullptr;
   float *sell_v32 = nullptr;
   // SELL-only matrices (laid out on the device, no full CSR): SELL positions of
@@ -267,6 +275,8 @@ struct ssf_matrix {
     c.sell_pid = sell_pid;
     c.sell_dict = sell_dict;
     c.sell_dict_off = sell_dict_off;
+    c.sten_n = (int)sten.size();
+    for (size_t k = 0; k < sten.size() && k < (size_t)ssf::SELL_STEN_MAX; k++) c.sten[k] = sten[k];
     c.sell_v64 = sell_v64;
     c.sell_v32 = sell_v32;
     return c;
@@ -321,7 +331,8 @@ void matrix_keep_sell_only(ssf_matrix *m);
 // src maps a SELL entry to its CSR entry or -1. false beyond 2^31 entries.
 bool build_sell(int64_t n, const std::vector<int> &rowptr, const std::vector<int> &col,
                 std::vector<int> &off, std::vector<int> &scol, std::vector<int> &src,
-                std::vector<short> &scol16);
+                std::vector<short> &scol16,
+                std::vector<int> *sten = nullptr);
 // row patterns of a SELL-64 layout (off, scol from build_sell): per row of the
 // layout its column offsets from min(row, n-1), deduplicated into dict (each
 // pattern's entries start at a multiple of SELL_V, dict_off[p]); pid[r] = the

@@ -315,6 +315,56 @@ __global__ __launch_bounds__(TPB) void k_spmv_sell_pat(CsrDev A, const T *__rest
   }
 }
 
+// SELL-64 in the stencil form: every row has the same NGRP * SELL_V slots and slot
+// k sits at column row + A.sten[k] (a slot the row lacks holds value 0 and reads
+// an x entry clamped into range), so the value stream is the only per-entry
+// traffic and the offsets are the same for every lane. A row is one round: all
+// value loads, then all gathers, then the products in slot (= column) order, so
+// the sums equal the column forms' bit for bit, absent slots adding exact zeros.
+template <typename T, int KIND, int NGRP>
+__global__ __launch_bounds__(TPB) void k_spmv_sell_sten(CsrDev A, const T *__restrict__ x,
+                                                        T *__restrict__ y, const T *__restrict__ b,
+                                                        T *__restrict__ partial, const int *done,
+                                                        long long *iter)
+{
+  typedef T tv __attribute__((ext_vector_type(SELL_V)));
+  if (done && *done) return;
+  if (iter && blockIdx.x == 0 && threadIdx.x == 0) *iter += 1;
+  const T *__restrict__ sval = sell_values<T>(A);
+  const int nb = gridDim.x;
+  const int spb = (A.nslice + nb - 1) / nb;
+  const int lb = xcd_block(blockIdx.x, nb);
+  const int s0 = lb * spb, s1 = min(A.nslice, s0 + spb);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), lane = threadIdx.x % 64;
+  const int nm1 = A.n - 1;
+  T acc = T(0);
+  for (int sl = s0 + wave; sl < s1; sl += TPB / 64) {
+    const unsigned vb = (unsigned)(sl * 64 * NGRP * SELL_V + lane * SELL_V) * sizeof(T);
+    const int rb = sl * 64 + lane;
+    tv v[NGRP];
+    T g[NGRP][SELL_V];
+#pragma unroll
+    for (int k = 0; k < NGRP; k++) v[k] = ldnt(at_byte<tv>(sval, vb + k * 64 * SELL_V * sizeof(T)));
+#pragma unroll
+    for (int k = 0; k < NGRP; k++)
+#pragma unroll
+      for (int u = 0; u < SELL_V; u++) {
+        const int c = min(max(rb + A.sten[k * SELL_V + u], 0), nm1);
+        g[k][u] = *at_byte<T>(x, (unsigned)c * sizeof(T));
+      }
+    T sum = T(0);
+#pragma unroll
+    for (int k = 0; k < NGRP; k++)
+#pragma unroll
+      for (int u = 0; u < SELL_V; u++) sum += v[k][u] * g[k][u];
+    if (rb < A.n) finish_row<T, 1, KIND>(rb, sum, x, y, b, acc);
+  }
+  if (KIND != 0) {
+    const T t = block_sum(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+  }
+}
+
 // ev0/ev1 (both or neither): the launch's own start and end timestamps
 // (hipExtLaunchKernelGGL), i.e. the kernel's duration without the dispatch gap
 template <typename T, int KIND>
@@ -330,7 +380,14 @@ static hipError_t spmv_k(const SpmvTune &t, const CsrDev &A, const T *val, const
   } while (0)
 #define SSF_VEC(G) SSF_LAUNCH((k_spmv_vec<T, G, KIND>), A, val, x, y, b, partial, done, iter)
 #define SSF_STR(I, G) SSF_LAUNCH((k_spmv_stream<T, I, G, KIND>), A, val, x, y, b, partial, done, iter)
-  if (t.format == 2 && A.nslice > 0) {
+  if (t.format == 2 && A.nslice > 0 && t.idx16 == 3 && A.sten_n > 0) {
+    switch ((A.sten_n + SELL_V - 1) / SELL_V) {
+#define SSF_STEN(G) case G: SSF_LAUNCH((k_spmv_sell_sten<T, KIND, G>), A, x, y, b, partial, done, iter); break;
+      SSF_STEN(1) SSF_STEN(2) SSF_STEN(3) SSF_STEN(4) SSF_STEN(5) SSF_STEN(6) SSF_STEN(7)
+      default: SSF_LAUNCH((k_spmv_sell_sten<T, KIND, 8>), A, x, y, b, partial, done, iter); break;
+#undef SSF_STEN
+    }
+  } else if (t.format == 2 && A.nslice > 0) {
     if (t.idx16 == 2 && A.sell_pid) {
       if (t.lanes >= 8) SSF_LAUNCH((k_spmv_sell_pat<T, KIND, 8>), A, x, y, b, partial, done, iter);
       else if (t.lanes >= 4) SSF_LAUNCH((k_spmv_sell_pat<T, KIND, 4>), A, x, y, b, partial, done, iter);

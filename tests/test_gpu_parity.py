@@ -242,6 +242,39 @@ def test_sell_row_patterns(ctx):
     c.close()
 
 
+def test_sell_stencil_form(ctx):
+    """Stencil form (index form 3): when every row's column offsets come from one
+    list of at most 32 (the heat cube's 27-point stencil) the layout gives each
+    offset its own slot in every row; products and solves equal the other forms'
+    (absent slots add exact zeros). A plate's offsets need a much wider layout than
+    its rows: no stencil form, and asking for it is refused."""
+    for prob in (mesh.heat_cube(17, seed=9), mesh.heat_cube(6, seed=10)):
+        adr, ci, a, rhs = O.assemble_problem(prob, "scr")
+        m = gpu.Matrix.from_csr(ctx, gpu.STORAGE_SCR, adr, ci)
+        m.set_values(a)
+        b = np.random.default_rng(6).standard_normal(prob.n)
+        m.set_tune(gpu.FP64, (2, 0, 64, 0))
+        y0 = m.gmxv(b)
+        x0, i0 = m.solve(rhs, ni=20000, res=1e-22)
+        for nb in (1, 3, 64):
+            m.set_tune(gpu.FP64, (2, 0, nb, 3))
+            assert np.array_equal(m.gmxv(b), y0), (prob.name, nb)
+        x3, i3 = m.solve(rhs, ni=20000, res=1e-22)
+        assert i3.iterations == i0.iterations and np.array_equal(x3, x0)
+        # the mixed solve on the stencil form
+        m.set_tune(gpu.MIXED, (2, 0, 64, 3))
+        xm, im = m.solve(rhs, ni=20000, res=1e-22, precision=gpu.MIXED)
+        assert im.status == 1 and np.linalg.norm(xm - x0) <= 1e-8 * np.linalg.norm(x0)
+        m.close()
+    prob = mesh.plate(20, 7, seed=11, nrhs=1)
+    adr, ci, a, rhs = O.assemble_problem(prob, "scr")
+    m = gpu.Matrix.from_csr(ctx, gpu.STORAGE_SCR, adr, ci)
+    m.set_values(a)
+    with pytest.raises(gpu.SsfError):
+        m.set_tune(gpu.FP64, (2, 0, 64, 3))
+    m.close()
+
+
 def test_autotune_keeps_parity(ctx):
     prob = mesh.heat_cube(12, seed=8)
     m, rhs = gpu.build_problem_matrix(ctx, prob)
