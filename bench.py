@@ -819,7 +819,10 @@ def main():
         "spmv_gbs": {"fp32": gbs32, "fp64": gbs64, "copy_kernel": copy_gbs},
         "roofline": {"bound": "hbm", "achieved": gbs32, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                      "frac": gbs32 * 1e9 / PEAK_HBM, "traffic": traffic,
-                     "kernel": {2: ("k_spmv_sell_pat<float, 1, %d> (FP32 SELL-64 SpMV of the mixed "
+                     "kernel": {2: ("k_spmv_sell_sten<float, 1, %d> (FP32 SELL-64 SpMV of the mixed "
+                                    "solve, stencil form)" % round((b32_act - 8 * prob.n) / 4 / (64 * ((prob.n + 63) // 64)) / 4)
+                                    if tune32[3] == 3 else
+                                    "k_spmv_sell_pat<float, 1, %d> (FP32 SELL-64 SpMV of the mixed "
                                     "solve, row patterns)" % (8 if tune32[1] >= 8 else 4 if tune32[1] >= 4 else 2)
                                     if tune32[3] == 2 else
                                     "k_spmv_sell<float, %d, 1> (FP32 SELL-64 SpMV of the mixed solve, %s)"
