@@ -431,7 +431,8 @@ int ssf_spmv_bench(ssf_matrix *m, int precision, int reps, double *usec);
  * receives {format (0 row-vector, 1 stream tiles, 2 SELL-64 slices), lanes per
  * row (formats 0, 1) or value loads in flight per lane (format 2 with row
  * patterns: 2, 4 or 8), blocks, idx16 (column form: 0 int32, 1 16-bit tile or
- * slice offsets, 2 SELL row patterns)}. */
+ * slice offsets, 2 SELL row patterns, 3 SELL stencil form: one slot per column
+ * offset in every row, offered when the matrix has at most 32 offsets)}. */
 int ssf_matrix_autotune(ssf_matrix *m, int precision, int reps, int32_t *tune_out);
 /* pin the SpMV tiling {format, lanes, blocks, idx16}; the result of a solve
  * depends only on the matrix, the right-hand side and the two tilings */
