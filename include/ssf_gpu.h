@@ -97,6 +97,21 @@ void *ssf_ctx_stream(ssf_ctx *ctx);
 int ssf_ctx_set_profiling(ssf_ctx *ctx, int on);
 int ssf_ctx_kernel_time(ssf_ctx *ctx, int which, double *avg_us, int64_t *count);
 
+/* ------------------------------------------------------------ DOF numbering */
+/* gtopology::gencodnum (GEFEL/gtopology.cpp:1619-1665), on the host: dof[nn*ndofn]
+ * holds the node flags on entry (0 fixed, 1 free, k > 1 coupled group k, < 0
+ * prescribed value index) and the code numbers on return: free DOFs get
+ * 1, 2, ... in the node order `ordering` (NULL = identity; else a permutation,
+ * e.g. from ssf_cuthill_mckee), each coupled group one number at its first DOF,
+ * 0 and negative flags unchanged. *ndof = the number of unknowns. */
+int ssf_gencodnum(int64_t nn, int64_t ndofn, int64_t *dof, const int64_t *ordering,
+                  int64_t *ndof);
+/* gtopology::give_code_numbers (GEFEL/gtopology.cpp:597-626, code numbers on the
+ * nodes): cn[ne][nne*ndofn] = the codes of element e's nodes conn[e*nne + a]
+ * (0-based) in node order, DOFs of a node in their order. */
+int ssf_give_code_numbers(int64_t ne, int64_t nne, const int64_t *conn, int64_t nn,
+                          int64_t ndofn, const int64_t *dof, int64_t *cn);
+
 /* ------------------------------------------------------ node renumbering */
 /* gtopology::cuthill_mckee_renumb (GEFEL/gtopology.cpp:6242-6355) on the edge graph
  * of adjacnodes_edge (:2632) of nn nodes and ne hex8 (nne 8) or quad4 (nne 4)

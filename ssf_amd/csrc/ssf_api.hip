@@ -248,16 +248,32 @@ bool build_sell_patterns(int64_t n, const std::vector<int> &off, const std::vect
 static std::vector<int> stencil_offsets(int64_t n, const std::vector<int> &rowptr,
                                         const std::vector<int> &col, int64_t tot)
 {
-  std::vector<int> u;
-  for (int64_t r = 0; r < n; r++)
-    for (int k = rowptr[r]; k < rowptr[r + 1]; k++) {
-      const int o = col[k] - (int)r;
-      if (std::find(u.begin(), u.end(), o) == u.end()) {
-        if ((int)u.size() == SELL_STEN_MAX) return {};
-        u.push_back(o);
+  // each host thread collects the offsets of its rows (giving up past
+  // SELL_STEN_MAX), then the lists are merged
+  const int T = host_threads();
+  std::vector<std::vector<int>> part(T);
+  std::atomic<bool> over{false};
+  parallel_tasks(T, [&](int t) {
+    std::vector<int> &u = part[t];
+    const int64_t b = n * t / T, e = n * (t + 1) / T;
+    for (int64_t r = b; r < e && !over.load(std::memory_order_relaxed); r++)
+      for (int k = rowptr[r]; k < rowptr[r + 1]; k++) {
+        const int o = col[k] - (int)r;
+        if (std::find(u.begin(), u.end(), o) == u.end()) {
+          if ((int)u.size() == SELL_STEN_MAX) {
+            over = true;
+            break;
+          }
+          u.push_back(o);
+        }
       }
-    }
+  });
+  if (over) return {};
+  std::vector<int> u;
+  for (const std::vector<int> &pt : part) u.insert(u.end(), pt.begin(), pt.end());
   std::sort(u.begin(), u.end());
+  u.erase(std::unique(u.begin(), u.end()), u.end());
+  if ((int)u.size() > SELL_STEN_MAX) return {};
   const int64_t w = ((int64_t)u.size() + SELL_V - 1) / SELL_V * SELL_V;
   if (u.empty() || w * 64 * ((n + 63) / 64) > tot + tot / 20) return {};
   return u;
@@ -288,7 +304,9 @@ bool build_sell(int64_t n, const std::vector<int> &rowptr, const std::vector<int
   if (!u.empty()) {
     // stencil form: every slice as wide as the offset list, slot k <-> u[k]
     const int64_t w = ((int64_t)u.size() + SELL_V - 1) / SELL_V * SELL_V;
-    if (w * 64 * nslice >= (1LL << 31)) {
+    // the stencil kernel addresses values and x by 32-bit byte offsets: keep the
+    // form only while both fit in 4 GiB at FP64
+    if (std::max<int64_t>(w * 64 * nslice, n) * (int64_t)sizeof(double) >= (int64_t(1) << 32)) {
       sten->clear();
       u.clear();
     } else {
@@ -442,7 +460,7 @@ extern "C" int ssf_matrix_destroy(ssf_matrix *m)
   dfree(m->rowptr, m->ctx->stream); dfree(m->col, m->ctx->stream); dfree(m->val, m->ctx->stream); dfree(m->val32, m->ctx->stream);
   dfree(m->diag, m->ctx->stream); dfree(m->diag32, m->ctx->stream); dfree(m->norm_a, m->ctx->stream);
   dfree(m->map_lo, m->ctx->stream); dfree(m->map_up, m->ctx->stream);
-  dfree(m->slot, m->ctx->stream); dfree(m->order, m->ctx->stream);
+  dfree(m->slot, m->ctx->stream); dfree(m->order, m->ctx->stream); dfree(m->asm_dup, m->ctx->stream);
   dfree(m->tiles, m->ctx->stream); dfree(m->col16, m->ctx->stream);
   dfree(m->sell_off, m->ctx->stream); dfree(m->sell_col, m->ctx->stream); dfree(m->sell_src, m->ctx->stream); dfree(m->sell_v64, m->ctx->stream); dfree(m->sell_v32, m->ctx->stream);
   dfree(m->sell_col16, m->ctx->stream);
@@ -463,6 +481,7 @@ void ssf::matrix_drop_assembly(ssf_matrix *m, bool host_pattern)
   (void)hipStreamSynchronize(m->ctx->stream);
   dfree(m->slot, m->ctx->stream);
   dfree(m->order, m->ctx->stream);
+  dfree(m->asm_dup, m->ctx->stream);
   m->asm_ne = 0;
   m->asm_ndofe = 0;
   m->colour_start.clear();
@@ -1003,27 +1022,45 @@ extern "C" int ssf_matrix_get_values(const ssf_matrix *m, double *a)
 
 // ----------------------------------------------------------------- assembly
 
-// Greedy first-fit colouring of the elements by shared nodes (at most 64
-// colours): elements of one colour share no node, hence no DOF and no matrix
-// entry, so one colour is assembled without atomics.
-static int colour_elements(int64_t nn, int64_t ne, int nne, const int64_t *conn,
+// Greedy first-fit colouring of the elements by their unknowns: elements of one
+// colour share no positive code number, hence no matrix entry and no entry of
+// the lift, so one colour is assembled without atomics. Code numbers <= 0 are
+// skipped by localize and the lift target, so they are no shared resource.
+// Coupled DOFs (one code number on several nodes) are covered because the
+// resource is the code number, not the node. Colours are taken 64 at a time:
+// elements that find none free in a window go to the next one, so heavily
+// shared unknowns (a DOF coupled over a whole face) only add colours.
+static int colour_elements(int64_t n, int64_t ne, int nd, const int64_t *cn,
                            std::vector<int> &order, std::vector<int64_t> &start)
 {
-  std::vector<uint64_t> used(nn, 0);
-  std::vector<int> colour(ne);
+  std::vector<uint64_t> used(n + 1, 0);
+  std::vector<int> colour(ne, -1);
+  std::vector<int64_t> todo(ne), next;
+  for (int64_t e = 0; e < ne; e++) todo[e] = e;
   int ncol = 0;
-  for (int64_t e = 0; e < ne; e++) {
-    uint64_t m = 0;
-    for (int a = 0; a < nne; a++) {
-      const int64_t nd = conn[e * nne + a];
-      SSF_CHECK(nd >= 0 && nd < nn, SSF_EINVAL, "element node index out of range");
-      m |= used[nd];
+  for (int win = 0; !todo.empty(); win++) {
+    SSF_CHECK(win < (1 << 16) / 64, SSF_EUNSUP, "element colouring needs more than 65536 colours");
+    if (win) std::fill(used.begin(), used.end(), 0);
+    next.clear();
+    for (int64_t e : todo) {
+      uint64_t m = 0;
+      for (int a = 0; a < nd; a++) {
+        const int64_t c = cn[e * nd + a];
+        if (c > 0) m |= used[c];
+      }
+      if (~m == 0) {
+        next.push_back(e);
+        continue;
+      }
+      const int c = __builtin_ctzll(~m);
+      colour[e] = 64 * win + c;
+      ncol = std::max(ncol, 64 * win + c + 1);
+      for (int a = 0; a < nd; a++) {
+        const int64_t k = cn[e * nd + a];
+        if (k > 0) used[k] |= (1ULL << c);
+      }
     }
-    SSF_CHECK(~m != 0, SSF_EUNSUP, "element colouring needs more than 64 colours");
-    const int c = __builtin_ctzll(~m);
-    colour[e] = c;
-    ncol = std::max(ncol, c + 1);
-    for (int a = 0; a < nne; a++) used[conn[e * nne + a]] |= (1ULL << c);
+    todo.swap(next);
   }
   start.assign(ncol + 1, 0);
   for (int64_t e = 0; e < ne; e++) start[colour[e] + 1]++;
@@ -1063,17 +1100,34 @@ static int prepare_assembly(ssf_matrix *m, int64_t nn, int64_t ne, int nne, int 
   }
   std::vector<int> order;
   std::vector<int64_t> start;
-  int rc;
-  if (conn) rc = colour_elements(nn, ne, nne, conn, order, start);
-  else {  // no node list: colour by code numbers (DOFs as the shared resource)
-    std::vector<int64_t> c2(ncn);
-    for (int64_t k = 0; k < ncn; k++) c2[k] = std::max<int64_t>(cn[k], 0);
-    rc = colour_elements(m->n + 1, ne, nd, c2.data(), order, start);
-  }
+  int rc = colour_elements(m->n, ne, nd, cn, order, start);
   if (rc) {
     dfree(d_cn, s);
     *d_cn_out = nullptr;
     return rc;
+  }
+  // elements in which one positive code number occurs twice (coupled DOFs of
+  // two of its nodes): their entries can meet in one slot, so the wavefront
+  // kernels add them one by one in the reference's order
+  std::vector<unsigned char> dup(ne, 0);
+  std::atomic<int> ndup{0};
+  parallel_for(ne, [&](int64_t b, int64_t e) {
+    std::vector<int64_t> c(nd);
+    for (int64_t el = b; el < e; el++) {
+      int k = 0;
+      for (int a = 0; a < nd; a++)
+        if (cn[el * nd + a] > 0) c[k++] = cn[el * nd + a];
+      std::sort(c.begin(), c.begin() + k);
+      if (std::adjacent_find(c.begin(), c.begin() + k) != c.begin() + k) {
+        dup[el] = 1;
+        ndup++;
+      }
+    }
+  });
+  dfree(m->asm_dup, s);
+  if (ndup.load()) {
+    SSF_HIP(dalloc(&m->asm_dup, ne, s));
+    SSF_HIP(hipMemcpyAsync(m->asm_dup, dup.data(), ne, hipMemcpyHostToDevice, s));
   }
   dfree(m->slot, s);
   dfree(m->order, s);
@@ -1130,6 +1184,14 @@ extern "C" int ssf_matrix_assemble(ssf_matrix *m, int kind, int64_t nn, const do
   const int nne = kind == SSF_ELEM_QUAD4_PSTRESS ? 4 : 8;
   const int nd = kind == SSF_ELEM_HEX8_ELAST ? 24 : 8;
   SSF_CHECK((int64_t)ne * nd * nd < (1LL << 31) * 4, SSF_EUNSUP, "ssf_matrix_assemble: slot map too large");
+  {
+    std::atomic<int> bad{0};
+    parallel_for(ne * nne, [&](int64_t b, int64_t e) {
+      for (int64_t k = b; k < e; k++)
+        if (conn[k] < 0 || conn[k] >= nn) bad = 1;
+    });
+    SSF_CHECK(!bad.load(), SSF_EINVAL, "ssf_matrix_assemble: element node index out of range");
+  }
   if (rhs_lift)  // prescribed values are read by the lift only
     for (int64_t k = 0; k < ne * nd; k++)
       if (cn[k] < 0) SSF_CHECK(pv && -cn[k] <= npv, SSF_EINVAL, "ssf_matrix_assemble: prescribed value index out of range");
@@ -1164,7 +1226,8 @@ extern "C" int ssf_matrix_assemble(ssf_matrix *m, int kind, int64_t nn, const do
   }
   for (size_t c = 0; e == hipSuccess && c + 1 < m->colour_start.size(); c++)
     chk(assemble_elements(kind, m->colour_start[c], m->colour_start[c + 1], ne, m->order, d_conn,
-                          d_cn, d_xyz, d_a, d_b, d_t, m->slot, m->a_ref, d_pv, d_lift, s));
+                          d_cn, d_xyz, d_a, d_b, d_t, m->slot, m->asm_dup, m->a_ref, d_pv, d_lift,
+                          s));
   if (e == hipSuccess) {
     int r2 = values_changed(m);
     if (r2) e = hipErrorUnknown;

@@ -12,8 +12,12 @@
 //      v_mfma_f64_16x16x4f64, 12 K-steps each; operands are formed on the fly
 //      from the LDS derivatives (G) and D (H);
 //   3. each lane adds its tile entries (and the mirror of tile (0,1)) into the
-//      reference-layout values through the element's slot map (coloured, so no
-//      other wave touches the same entries), plus the Dirichlet lift.
+//      reference-layout values through the element's slot map (coloured by code
+//      numbers, so no other wave touches the same entries), plus the Dirichlet
+//      lift. An element in which a code number repeats (coupled DOFs on two of
+//      its nodes) can hit one slot from two lanes: its K_e goes through LDS and
+//      one lane adds the entries in localize's order (row i, then column j,
+//      GEFEL/scr.cpp:422-458).
 #include "ssf_device.hpp"
 
 namespace ssf {
@@ -45,6 +49,7 @@ __global__ __launch_bounds__(TPB) void k_hex8_elast(int64_t p0, int64_t p1, int6
                                                     const double *__restrict__ E,
                                                     const double *__restrict__ NU,
                                                     const int *__restrict__ slot,
+                                                    const unsigned char *__restrict__ dup,
                                                     double *__restrict__ a_ref,
                                                     const double *__restrict__ pv,
                                                     double *__restrict__ lift)
@@ -139,7 +144,9 @@ __global__ __launch_bounds__(TPB) void k_hex8_elast(int64_t p0, int64_t p1, int6
   if (!active) return;
   // 3. scatter through the slot map (AoS per element: slot[p*576 + i*24 + j])
   const int *sl = slot + p * 576;
+  const bool serial = dup && dup[e];  // wave-uniform
   const bool want_lift = lift != nullptr;
+  const bool stage = want_lift || serial;
 #pragma unroll
   for (int t = 0; t < 3; t++) {
     const int ti = (t == 2) ? 1 : 0, tj = (t == 0) ? 0 : 1;
@@ -148,21 +155,50 @@ __global__ __launch_bounds__(TPB) void k_hex8_elast(int64_t p0, int64_t p1, int6
       const int i = 16 * ti + 4 * v + lk, j = 16 * tj + li;
       if (i >= 24 || j >= 24) continue;
       const double kv = acc[t][v];
-      int q = sl[i * 24 + j];
-      if (q >= 0) a_ref[q] += kv;
-      if (want_lift) s_k[wv][i * 24 + j] = kv;
-      if (t == 1) {  // mirror of tile (0,1) is tile (1,0)
-        q = sl[j * 24 + i];
+      if (!serial) {
+        const int q = sl[i * 24 + j];
         if (q >= 0) a_ref[q] += kv;
-        if (want_lift) s_k[wv][j * 24 + i] = kv;
+      }
+      if (stage) s_k[wv][i * 24 + j] = kv;
+      if (t == 1) {  // mirror of tile (0,1) is tile (1,0)
+        if (!serial) {
+          const int q = sl[j * 24 + i];
+          if (q >= 0) a_ref[q] += kv;
+        }
+        if (stage) s_k[wv][j * 24 + i] = kv;
       }
     }
   }
-  if (!want_lift) return;
+  if (!stage) return;
   // s_k is written and read by lanes of this wavefront only
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
   __builtin_amdgcn_wave_barrier();
+  if (serial) {
+    if (lane == 0)
+      for (int ij = 0; ij < 576; ij++) {
+        const int q = sl[ij];
+        if (q >= 0) a_ref[q] += s_k[wv][ij];
+      }
+    if (!want_lift) return;
+    if (lane == 0) {  // the lift row by row as well: two rows may share an unknown
+      for (int i = 0; i < 24; i++) {
+        const int ci = cn[(int64_t)e * 24 + i];
+        if (ci < 1) continue;
+        double f = 0.0;
+        bool any = false;
+        for (int j = 0; j < 24; j++) {
+          const int cj = cn[(int64_t)e * 24 + j];
+          if (cj < 0) {
+            f += s_k[wv][i * 24 + j] * pv[-cj - 1];
+            any = true;
+          }
+        }
+        if (any) lift[ci - 1] -= f;
+      }
+    }
+    return;
+  }
   if (lane < 24) {
     const int ci = cn[(int64_t)e * 24 + lane];
     if (ci >= 1) {
@@ -183,12 +219,13 @@ __global__ __launch_bounds__(TPB) void k_hex8_elast(int64_t p0, int64_t p1, int6
 hipError_t assemble_hex8_elast(int64_t p0, int64_t p1, int64_t ne_total, const int *order,
                                const int *conn, const int *cn, const double *xyz,
                                const double *ma, const double *mb, const int *slot,
-                               double *a_ref, const double *pv, double *lift, hipStream_t s)
+                               const unsigned char *dup, double *a_ref, const double *pv,
+                               double *lift, hipStream_t s)
 {
   if (p1 <= p0) return hipSuccess;
   const unsigned nb = (unsigned)((p1 - p0 + EPB - 1) / EPB);
   hipLaunchKernelGGL(k_hex8_elast, dim3(nb), dim3(TPB), 0, s, p0, p1, ne_total, order, conn, cn,
-                     xyz, ma, mb, slot, a_ref, pv, lift);
+                     xyz, ma, mb, slot, dup, a_ref, pv, lift);
   return hipGetLastError();
 }
 

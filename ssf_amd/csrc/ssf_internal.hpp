@@ -222,6 +222,8 @@ struct ssf_matrix {
   int64_t asm_ne = 0, asm_ndofe = 0;
   int *slot = nullptr;       // [asm_ne * ndofe * ndofe] reference-layout slot or -1
   int *order = nullptr;      // element ids sorted by colour
+  unsigned char *asm_dup = nullptr;  // [ne] 1: a positive code number repeats in the
+                                     // element (null when no element has one)
   std::vector<int64_t> colour_start;
   std::vector<int64_t> asm_cn_host;  // for identity checks
   // stream tiling: whole rows, at most STREAM_CAP entries per tile

@@ -451,17 +451,18 @@ __global__ __launch_bounds__(TPB) void k_assemble8(int64_t p0, int64_t p1, int64
 hipError_t assemble_hex8_elast(int64_t p0, int64_t p1, int64_t ne_total, const int *order,
                                const int *conn, const int *cn, const double *xyz,
                                const double *ma, const double *mb, const int *slot,
-                               double *a_ref, const double *pv, double *lift, hipStream_t s);
+                               const unsigned char *dup, double *a_ref, const double *pv,
+                               double *lift, hipStream_t s);
 
 hipError_t assemble_elements(int kind, int64_t p0, int64_t p1, int64_t ne_total,
                              const int *order, const int *conn, const int *cn,
                              const double *xyz, const double *mat_a, const double *mat_b,
-                             const double *thick, const int *slot, double *a_ref,
-                             const double *pv, double *lift, hipStream_t s)
+                             const double *thick, const int *slot, const unsigned char *dup,
+                             double *a_ref, const double *pv, double *lift, hipStream_t s)
 {
   if (p1 <= p0) return hipSuccess;
   if (kind == SSF_ELEM_HEX8_ELAST)
-    return assemble_hex8_elast(p0, p1, ne_total, order, conn, cn, xyz, mat_a, mat_b, slot,
+    return assemble_hex8_elast(p0, p1, ne_total, order, conn, cn, xyz, mat_a, mat_b, slot, dup,
                                a_ref, pv, lift, s);
   const unsigned nb = (unsigned)((p1 - p0 + TPB - 1) / TPB);
   if (kind == SSF_ELEM_HEX8_HEAT)

@@ -113,8 +113,8 @@ hipError_t vec_rate(int64_t n, double *v, const double *x, const double *d, doub
 hipError_t assemble_elements(int kind, int64_t p0, int64_t p1, int64_t ne_total,
                              const int *order, const int *conn, const int *cn,
                              const double *xyz, const double *mat_a, const double *mat_b,
-                             const double *thick, const int *slot, double *a_ref,
-                             const double *pv, double *lift, hipStream_t s);
+                             const double *thick, const int *slot, const unsigned char *dup,
+                             double *a_ref, const double *pv, double *lift, hipStream_t s);
 hipError_t localize_elements(int64_t p0, int64_t p1, int64_t ne_total, int nd,
                              const int *order, const int *slot, const double *lm,
                              double *a_ref, hipStream_t s);

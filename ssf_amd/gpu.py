@@ -111,6 +111,8 @@ SIGNATURES = {
     "ssf_np_step_linear": (_I, [_P, _P, _P, _D, _D, ctypes.POINTER(SolveParams), _P, _P, _P,
                                 ctypes.POINTER(SolveInfo)]),
     "ssf_cuthill_mckee": (_I, [_P, _L, _L, _L, _P, _I, _P]),
+    "ssf_gencodnum": (_I, [_L, _L, _P, _P, ctypes.POINTER(_L)]),
+    "ssf_give_code_numbers": (_I, [_L, _L, _P, _L, _L, _P, _P]),
     "ssf_newton_raphson": (_I, [_P, _P, ctypes.POINTER(NrParams), ctypes.POINTER(SolveParams),
                                 _P, _L, _P, _P, _P, _P, _P, _P, _P, _P, ctypes.POINTER(NrInfo)]),
     "ssf_matrix_ildl_factor": (_I, [_P, _P]),
@@ -398,6 +400,31 @@ def np_step_linear(K: Matrix, C: Matrix, A: Matrix, alpha: float, dt: float, rhs
     _check(getattr(lib(), fn)(K.h, C.h, A.h, float(alpha), float(dt), ctypes.byref(p),
                               rhs_ptr, lhs_ptr, tdlhs_ptr, ctypes.byref(info)))
     return info
+
+
+def gencodnum(flags: np.ndarray, ordering=None) -> tuple[np.ndarray, int]:
+    """gtopology::gencodnum on the host through the C ABI: node flags (nn, ndofn)
+    (0 fixed, 1 free, k > 1 coupled group, < 0 prescribed) -> code numbers and n."""
+    dof = np.array(flags, dtype=np.int64, order="C", copy=True)
+    if dof.ndim == 1:
+        dof = dof[:, None]
+    nd = _L()
+    o = None if ordering is None else _i64(ordering)
+    _check(lib().ssf_gencodnum(dof.shape[0], dof.shape[1], _p(dof), _p(o), ctypes.byref(nd)))
+    return dof, int(nd.value)
+
+
+def give_code_numbers(conn: np.ndarray, dof: np.ndarray) -> np.ndarray:
+    """gtopology::give_code_numbers for every element: (ne, nne * ndofn)."""
+    conn = _i64(conn)
+    dof = _i64(dof)
+    if dof.ndim == 1:
+        dof = dof[:, None]
+    ne, nne = conn.shape
+    cn = np.empty((ne, nne * dof.shape[1]), dtype=np.int64)
+    _check(lib().ssf_give_code_numbers(ne, nne, _p(conn), dof.shape[0], dof.shape[1], _p(dof),
+                                       _p(cn)))
+    return cn
 
 
 def cuthill_mckee(ctx: Context, nn: int, conn: np.ndarray, reverse=False) -> np.ndarray:

@@ -7,8 +7,9 @@ coordinates accumulated step by step; element nodes (+,+,+), (-,+,+), (-,-,+),
 quadrilaterals PREP/SEQMESHGEN/gensifquad.cpp:25-78.
 
 Equation numbers follow gtopology::gencodnum (GEFEL/gtopology.cpp:1619-1680):
-per-node DOF flags 0 = fixed, 1 = free, <0 = prescribed value index; free DOFs are
-numbered 1..n in node order x DOF order. The element code-number table matches
+per-node DOF flags 0 = fixed, 1 = free, k > 1 = coupled group, <0 = prescribed
+value index; free DOFs are numbered 1..n in node order x DOF order, a coupled
+group sharing the number of its first DOF. The element code-number table matches
 gtopology::give_code_numbers (GEFEL/gtopology.cpp:597-626).
 
 Everything here is host-side input construction (numpy); the arithmetic of the
@@ -110,15 +111,27 @@ def quad_grid(nex: int, ney: int, lx: float, ly: float):
     return coords, conn
 
 
-def codenum(flags: np.ndarray) -> tuple[np.ndarray, int]:
-    """gencodnum for flags in {0, 1, <0}: free DOFs numbered 1..n in node order."""
+def codenum(flags: np.ndarray, ordering=None) -> tuple[np.ndarray, int]:
+    """gencodnum (GEFEL/gtopology.cpp:1619-1665) for node flags (nn, ndofn): 0 fixed,
+    1 free, k > 1 coupled group k, < 0 prescribed. Nodes are visited in `ordering`
+    (default: node order), DOFs of a node in order; a free DOF takes the next
+    number, a coupled group one number at its first DOF; 0 and < 0 stay."""
     flags = np.asarray(flags, dtype=np.int64)
-    if (flags > 1).any():
-        raise ValueError("coupled DOF groups (flag > 1) need the general numbering")
-    free = flags == 1
-    num = np.cumsum(free.ravel()).reshape(flags.shape)
-    dof = np.where(free, num, flags).astype(np.int64)
-    return dof, int(free.sum())
+    shape = flags.shape
+    f2 = flags.reshape(shape[0], -1)
+    order = np.arange(shape[0]) if ordering is None else np.asarray(ordering, dtype=np.int64)
+    visit = f2[order].reshape(-1)
+    pos = np.nonzero(visit > 0)[0]
+    # a token per unknown: each free DOF its own (negative), a group its flag
+    key = np.where(visit[pos] == 1, -(pos + 1), visit[pos])
+    uniq, first, inv = np.unique(key, return_index=True, return_inverse=True)
+    rank = np.empty(uniq.size, dtype=np.int64)
+    rank[np.argsort(first, kind="stable")] = np.arange(1, uniq.size + 1)
+    out = visit.copy()
+    out[pos] = rank[inv]
+    dof = np.empty_like(f2)
+    dof[order] = out.reshape(f2.shape)
+    return dof.reshape(shape), int(uniq.size)
 
 
 def lognormal(rng: np.random.Generator, size: int, mean: float, sigma_log: float) -> np.ndarray:
