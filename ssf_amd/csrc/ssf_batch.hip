@@ -16,6 +16,15 @@
 
 namespace ssf {
 
+// ---------------------------------------------------------- work list
+
+// Block-table kernels are persistent: min(nblk, pgrid) workgroups stride over
+// the work list (the blocks of the systems still running), so systems that have
+// finished cost nothing and a launch covers any number of systems.
+__device__ __forceinline__ int wl_count(const BatchDev &B) { return B.nwl ? *B.nwl : B.nblk; }
+__device__ __forceinline__ int wl_block(const BatchDev &B, int i) { return B.nwl ? B.wl[i] : i; }
+static inline dim3 wl_grid(const BatchDev &B) { return dim3((unsigned)std::max(1, std::min(B.nblk, B.pgrid))); }
+
 // ------------------------------------------------------------ reductions
 
 // SELL-64 SpMM: one wavefront per slice, one lane per row holding the R sums of
@@ -31,11 +40,13 @@ __global__ __launch_bounds__(TPB) void k_bsell(BatchDev B, const T *__restrict__
   typedef T tv __attribute__((ext_vector_type(SELL_V)));
   typedef int iv __attribute__((ext_vector_type(SELL_V)));
   typedef short sv __attribute__((ext_vector_type(SELL_V)));
-  const int blk = blockIdx.x;
-  const int s = B.blk_sys[blk];
-  if (sys_done && sys_done[s]) return;
-  const int rend = B.blk_r1[blk];
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int wn = wl_count(B);
+  for (int wi = blockIdx.x; wi < wn; wi += gridDim.x) {
+  const int blk = wl_block(B, wi);
+  const int s = B.blk_sys[blk];
+  if (sys_done && sys_done[s]) continue;
+  const int rend = B.blk_r1[blk];
   T acc[R];
 #pragma unroll
   for (int r = 0; r < R; r++) acc[r] = T(0);
@@ -88,6 +99,7 @@ __global__ __launch_bounds__(TPB) void k_bsell(BatchDev B, const T *__restrict__
       const T t = block_sum(acc[r]);
       if (threadIdx.x == 0) partial[(int64_t)blk * R + r] = t;
     }
+  }
   }
 }
 
@@ -147,12 +159,15 @@ __global__ __launch_bounds__(TPB) void k_bspmm(BatchDev B, const T *__restrict__
 {
   constexpr int G = R * L;
   constexpr int NG = TPB / G;
-  const int blk = blockIdx.x;
-  const int s = B.blk_sys[blk];
-  if (sys_done && sys_done[s]) return;
-  const int r0 = B.blk_r0[blk], r1 = B.blk_r1[blk];
   const int grp = threadIdx.x / G, lane = threadIdx.x % G;
   const int r = lane % R, l = lane / R;
+  __shared__ T outp[R];
+  const int wn = wl_count(B);
+  for (int wi = blockIdx.x; wi < wn; wi += gridDim.x) {
+  const int blk = wl_block(B, wi);
+  const int s = B.blk_sys[blk];
+  if (sys_done && sys_done[s]) continue;
+  const int r0 = B.blk_r0[blk], r1 = B.blk_r1[blk];
   T acc = T(0);
   for (int row = r0 + grp; row < r1; row += NG) {
     const int beg = B.rowptr[row], end = B.rowptr[row + 1];
@@ -173,9 +188,9 @@ __global__ __launch_bounds__(TPB) void k_bspmm(BatchDev B, const T *__restrict__
   }
   if (KIND != 0) {
     // per right-hand side: thread t holds a partial sum for rhs t % R
-    __shared__ T outp[R];
     block_sum_r<T, R>(acc, outp);
     if (threadIdx.x < R) partial[(int64_t)blk * R + threadIdx.x] = outp[threadIdx.x];
+  }
   }
 }
 
@@ -192,9 +207,11 @@ __global__ __launch_bounds__(TPB) void k_bupdate(BatchDev B, BatchState<T> st,
 {
   __shared__ T alpha[R];
   __shared__ T outa[R], outb[R], dens[R];
-  const int blk = blockIdx.x;
+  const int wn = wl_count(B);
+  for (int wi = blockIdx.x; wi < wn; wi += gridDim.x) {
+  const int blk = wl_block(B, wi);
   const int s = B.blk_sys[blk];
-  if (st.sys_done[s]) return;
+  if (st.sys_done[s]) continue;
   sys_sum_block<T, R>(B, pden, s, dens);
   if (threadIdx.x < R) {
     const int r = threadIdx.x;
@@ -238,6 +255,7 @@ __global__ __launch_bounds__(TPB) void k_bupdate(BatchDev B, BatchState<T> st,
     prr[(int64_t)blk * R + threadIdx.x] = outa[threadIdx.x];
     prr[(int64_t)(B.nblk + blk) * R + threadIdx.x] = outb[threadIdx.x];
   }
+  }
 }
 
 // -------------------------------------------------------------- B3: direction
@@ -250,9 +268,11 @@ __global__ __launch_bounds__(TPB) void k_bdir(BatchDev B, BatchState<T> st,
   __shared__ T beta[R];
   __shared__ int upd[R];
   __shared__ T rrs[R], noms[R];
-  const int blk = blockIdx.x;
+  const int wn = wl_count(B);
+  for (int wi = blockIdx.x; wi < wn; wi += gridDim.x) {
+  const int blk = wl_block(B, wi);
   const int s = B.blk_sys[blk];
-  if (st.sys_done[s]) return;
+  if (st.sys_done[s]) continue;
   const bool first = blk == B.sys_blk0[s];
   sys_sum_block<T, R>(B, prr, s, rrs);
   sys_sum_block<T, R>(B, prr + (int64_t)B.nblk * R, s, noms);
@@ -298,9 +318,12 @@ __global__ __launch_bounds__(TPB) void k_bdir(BatchDev B, BatchState<T> st,
   __syncthreads();
   const int64_t e0 = (int64_t)B.blk_r0[blk] * R, e1 = (int64_t)B.blk_r1[blk] * R;
   const int r = threadIdx.x % R;
-  if (!upd[r]) return;
-  const T bt = beta[r];
-  for (int64_t e = e0 + threadIdx.x; e < e1; e += TPB) d[e] = bt * d[e] + h[e];
+  if (upd[r]) {
+    const T bt = beta[r];
+    for (int64_t e = e0 + threadIdx.x; e < e1; e += TPB) d[e] = bt * d[e] + h[e];
+  }
+  __syncthreads();  // beta / upd are rewritten for the next block
+  }
 }
 
 // applies the pending state changes of the iteration, then: a system is done when
@@ -344,7 +367,9 @@ __global__ __launch_bounds__(TPB) void k_binit(BatchDev B, const T *__restrict__
                                                T *__restrict__ prr)
 {
   __shared__ T outa[R], outb[R];
-  const int blk = blockIdx.x;
+  const int wn = wl_count(B);
+  for (int wi = blockIdx.x; wi < wn; wi += gridDim.x) {
+  const int blk = wl_block(B, wi);
   const int64_t e0 = (int64_t)B.blk_r0[blk] * R, e1 = (int64_t)B.blk_r1[blk] * R;
   T syy = T(0), srh = T(0);
   for (int64_t e = e0 + threadIdx.x; e < e1; e += TPB) {
@@ -361,6 +386,7 @@ __global__ __launch_bounds__(TPB) void k_binit(BatchDev B, const T *__restrict__
   if (threadIdx.x < R) {
     prr[(int64_t)blk * R + threadIdx.x] = outa[threadIdx.x];
     prr[(int64_t)(B.nblk + blk) * R + threadIdx.x] = outb[threadIdx.x];
+  }
   }
 }
 
@@ -409,11 +435,13 @@ __global__ __launch_bounds__(TPB) void k_bmx_xupdate(BatchDev B, BatchMix mx,
                                                      double *__restrict__ px)
 {
   __shared__ double outa[R];
-  const int blk = blockIdx.x;
-  const int s = B.blk_sys[blk];
-  if (sys_final[s]) return;
-  const int64_t e0 = (int64_t)B.blk_r0[blk] * R, e1 = (int64_t)B.blk_r1[blk] * R;
   const int r = threadIdx.x % R;
+  const int wn = wl_count(B);
+  for (int wi = blockIdx.x; wi < wn; wi += gridDim.x) {
+  const int blk = wl_block(B, wi);
+  const int s = B.blk_sys[blk];
+  if (sys_final[s]) continue;
+  const int64_t e0 = (int64_t)B.blk_r0[blk] * R, e1 = (int64_t)B.blk_r1[blk] * R;
   const int q = s * R + r;
   const bool fin = mx.final[q];
   const double inv = 1.0 / mx.scale[q];
@@ -428,6 +456,7 @@ __global__ __launch_bounds__(TPB) void k_bmx_xupdate(BatchDev B, BatchMix mx,
   }
   block_sum_r<double, R>(xx, outa);
   if (threadIdx.x < R) px[(int64_t)blk * R + threadIdx.x] = outa[threadIdx.x];
+  }
 }
 
 template <int R>
@@ -445,9 +474,12 @@ __global__ __launch_bounds__(TPB) void k_bmx_refresh(BatchDev B, BatchMix mx,
   __shared__ float cfac[R];
   __shared__ int rst[R], fin[R];
   __shared__ float outa[R], outb[R];
-  const int blk = blockIdx.x;
+  const int wn = wl_count(B);
+  for (int wi = blockIdx.x; wi < wn; wi += gridDim.x) {
+  const int blk = wl_block(B, wi);
   const int s = B.blk_sys[blk];
-  if (sys_final[s]) return;
+  if (sys_final[s]) continue;
+  __syncthreads();  // the previous block's reads of scl / cfac / rst / fin are done
   if (threadIdx.x < R) {
     const int q = s * R + threadIdx.x;
     const double rr64 = sys_sum<double, R>(B, part64, s, threadIdx.x);
@@ -481,6 +513,7 @@ __global__ __launch_bounds__(TPB) void k_bmx_refresh(BatchDev B, BatchMix mx,
   if (threadIdx.x < R) {
     prr[(int64_t)blk * R + threadIdx.x] = outa[threadIdx.x];
     prr[(int64_t)(B.nblk + blk) * R + threadIdx.x] = outb[threadIdx.x];
+  }
   }
 }
 
@@ -587,12 +620,15 @@ __global__ __launch_bounds__(TPB) void k_bmx_scale_rhs(BatchDev B, BatchMix mx,
                                                        const double *__restrict__ r64,
                                                        float *__restrict__ y32)
 {
-  const int blk = blockIdx.x;
-  const int s = B.blk_sys[blk];
-  const int64_t e0 = (int64_t)B.blk_r0[blk] * R, e1 = (int64_t)B.blk_r1[blk] * R;
   const int r = threadIdx.x % R;
-  const double sc = mx.scale[s * R + r];
-  for (int64_t e = e0 + threadIdx.x; e < e1; e += TPB) y32[e] = (float)(r64[e] * sc);
+  const int wn = wl_count(B);
+  for (int wi = blockIdx.x; wi < wn; wi += gridDim.x) {
+    const int blk = wl_block(B, wi);
+    const int s = B.blk_sys[blk];
+    const int64_t e0 = (int64_t)B.blk_r0[blk] * R, e1 = (int64_t)B.blk_r1[blk] * R;
+    const double sc = mx.scale[s * R + r];
+    for (int64_t e = e0 + threadIdx.x; e < e1; e += TPB) y32[e] = (float)(r64[e] * sc);
+  }
 }
 
 template <int R>
@@ -631,7 +667,7 @@ static hipError_t bspmm_r(const BatchTune &t, int kind, const BatchDev &B, const
                           const T *sval, const T *x, T *y, const T *b, T *partial,
                           const int *sys_done, hipStream_t s)
 {
-  dim3 grid(B.nblk), block(TPB);
+  dim3 grid = wl_grid(B), block(TPB);
   if (t.format == 2 && B.soff && sval) {
 #define SSF_BSELL(I, K) hipLaunchKernelGGL((k_bsell<T, R, I, K>), grid, block, 0, s, B, sval, x, y, b, partial, sys_done)
     const bool i16 = B.scol16 != nullptr;
@@ -683,7 +719,7 @@ static hipError_t biter_r(const BatchTune &t, const BatchDev &B, const BatchVec<
 {
   hipError_t e = bspmm_r<T, R>(t, 1, B, v.val, v.sval, v.d, v.p, nullptr, v.part, st.sys_done, s);
   if (e != hipSuccess) return e;
-  dim3 grid(B.nblk), block(TPB);
+  dim3 grid = wl_grid(B), block(TPB);
   T *prr = v.part + (int64_t)B.nblk * R;
   hipLaunchKernelGGL((k_bupdate<T, R>), grid, block, 0, s, B, st, v.part, zero, v.x, v.r, v.d,
                      v.p, v.diag, v.h, prr);
@@ -719,7 +755,7 @@ static hipError_t binit_r(const BatchDev &B, const BatchVec<T> &v, BatchState<T>
                           hipStream_t s)
 {
   T *prr = v.part + (int64_t)B.nblk * R;
-  hipLaunchKernelGGL((k_binit<T, R>), dim3(B.nblk), dim3(TPB), 0, s, B, v.y, v.diag, v.r, v.h,
+  hipLaunchKernelGGL((k_binit<T, R>), wl_grid(B), dim3(TPB), 0, s, B, v.y, v.diag, v.r, v.h,
                      v.d, prr);
   const int nq = B.nsys * R;
   hipLaunchKernelGGL((k_binit2<T, R>), dim3((nq + TPB - 1) / TPB), dim3(TPB), 0, s, B, st, prr,
@@ -780,12 +816,12 @@ static hipError_t bmx_start_r(const BatchTune &t64, const BatchTune &t32, const 
                               double delta, hipStream_t s)
 {
   BatchMix mx = mixview<R>(md);
-  dim3 grid(B.nblk), block(TPB);
+  dim3 grid = wl_grid(B), block(TPB);
   const int nq = B.nsys * R;
   const unsigned gq = (unsigned)((nq + TPB - 1) / TPB);
   // y.y and the initial FP64 residual r = b - A x
   double *pyy = v64.part + 2 * (int64_t)B.nblk * R;
-  hipLaunchKernelGGL(k_bdot_yy, grid, block, 0, s, B, R, v64.y, pyy);
+  hipLaunchKernelGGL(k_bdot_yy, dim3(B.nblk), block, 0, s, B, R, v64.y, pyy);
   hipError_t e = bspmm_r<double, R>(t64, 2, B, v64.val, v64.sval, v64.x, v64.r, v64.y, v64.part,
                                     nullptr, s);
   if (e != hipSuccess) return e;
@@ -809,7 +845,7 @@ static hipError_t bmx_refresh_r(const BatchTune &t64, const BatchDev &B,
                                 hipStream_t s)
 {
   BatchMix mx = mixview<R>(md);
-  dim3 grid(B.nblk), block(TPB);
+  dim3 grid = wl_grid(B), block(TPB);
   const int nq = B.nsys * R;
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s, md.active);
   double *px = v64.part + (int64_t)B.nblk * R;  // free in the mixed solve
@@ -825,6 +861,53 @@ static hipError_t bmx_refresh_r(const BatchTune &t64, const BatchDev &B,
                      v64.part, prr, px, res);
   hipLaunchKernelGGL((k_bmx_flags<R>), dim3((B.nsys + TPB - 1) / TPB), dim3(TPB), 0, s, B, mx, st,
                      md.sys_final, md.active);
+  return hipGetLastError();
+}
+
+// Work list of the blocks of the running systems (skip[s] == 0; every system when
+// skip is null), in system order: one workgroup scans the per-system block
+// counts 1024 systems at a time. With hist: hist[*ctr] = *active, *ctr += 1.
+constexpr int CTPB = 1024;
+__global__ __launch_bounds__(CTPB) void k_bcompact(BatchDev B, const int *__restrict__ skip,
+                                                   int *__restrict__ wl, int *__restrict__ nwl,
+                                                   const int *__restrict__ active,
+                                                   int *__restrict__ hist, int *__restrict__ ctr,
+                                                   int hist_cap)
+{
+  __shared__ int scan[CTPB];
+  int base = 0;
+  for (int s0 = 0; s0 < B.nsys; s0 += CTPB) {
+    const int sy = s0 + threadIdx.x;
+    int cnt = 0;
+    if (sy < B.nsys && !(skip && skip[sy])) cnt = B.sys_blk0[sy + 1] - B.sys_blk0[sy];
+    scan[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int off = 1; off < CTPB; off <<= 1) {
+      const int v = threadIdx.x >= off ? scan[threadIdx.x - off] : 0;
+      __syncthreads();
+      scan[threadIdx.x] += v;
+      __syncthreads();
+    }
+    const int start = base + scan[threadIdx.x] - cnt;
+    for (int k = 0; k < cnt; k++) wl[start + k] = B.sys_blk0[sy] + k;
+    base += scan[CTPB - 1];
+    __syncthreads();  // scan is rewritten by the next group of systems
+  }
+  if (threadIdx.x == 0) {
+    *nwl = base;
+    if (hist) {
+      const int c = *ctr;
+      if (c < hist_cap) hist[c] = *active;
+      *ctr = c + 1;
+    }
+  }
+}
+
+hipError_t batch_compact(const BatchDev &B, const int *skip, int *wl, int *nwl, const int *active,
+                         int *hist, int *ctr, int hist_cap, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_bcompact, dim3(1), dim3(CTPB), 0, s, B, skip, wl, nwl, active, hist, ctr,
+                     hist_cap);
   return hipGetLastError();
 }
 

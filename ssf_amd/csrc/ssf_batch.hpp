@@ -26,6 +26,12 @@ struct BatchDev {
   const short *scol16 = nullptr;    // offsets from the slice's first row
   const int *slice_row0 = nullptr;
   const int *blk_s0 = nullptr, *blk_s1 = nullptr;
+  // work list of the persistent kernels: the blocks of the systems still
+  // running, wl[0, *nwl) (rebuilt once per iteration chunk); null: every block.
+  // Block-table kernels launch min(nblk, pgrid) workgroups that stride over it.
+  const int *wl = nullptr;
+  const int *nwl = nullptr;
+  int pgrid = 2048;
 };
 
 struct BatchTune {
@@ -82,6 +88,11 @@ hipError_t batch_flags(const BatchDev &B, BatchState<T> st, int *active, hipStre
 hipError_t batch_norm_inf(const BatchDev &B, const double *val, double *blkmax, double *norm_a,
                           hipStream_t s);
 hipError_t offset_copy(int64_t n, const int *src, int *dst, int add, hipStream_t s);
+// work list of the blocks of the systems with skip[s] == 0 into wl / *nwl; with
+// hist, also hist[*ctr] = *active (the unfinished pairs counted before) and
+// *ctr += 1, the per-chunk record of a solve replayed from one graph
+hipError_t batch_compact(const BatchDev &B, const int *skip, int *wl, int *nwl, const int *active,
+                         int *hist, int *ctr, int hist_cap, hipStream_t s);
 hipError_t zero_int(int *p, hipStream_t s);
 hipError_t batch_mixed_start(const BatchTune &t64, const BatchTune &t32, const BatchDev &B,
                              const BatchVec<double> &v64, const BatchVec<float> &v32,

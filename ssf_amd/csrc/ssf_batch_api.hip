@@ -1,6 +1,11 @@
 // C ABI of the batched solver: many assembled matrices (independent systems, each
-// with R right-hand sides) solved together. Iteration chunks are captured once
-// into a hipGraph and replayed until every (system, rhs) pair has finished.
+// with R right-hand sides) solved together. The whole solve is one hipGraph
+// launch: the graph holds as many iteration chunks as the batch's previous solve
+// needed (plus one), each chunk ending with the stop tests and a new work list
+// of the systems still running, so the persistent kernels skip finished systems
+// and the chunks after the last pair has stopped cost a few dispatches each. A
+// batch's first solve, and one that needs more chunks, replays a one-chunk graph
+// from a host loop.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -47,6 +52,8 @@ struct StateArrays {
 
 }  // namespace
 
+constexpr int HIST_CAP = 1 << 16;  // chunks recorded per solve
+
 struct ssf_batch {
   ssf_ctx *ctx = nullptr;
   int nsys = 0, R = 1, nblk = 0;
@@ -73,10 +80,17 @@ struct ssf_batch {
   BatchMixDev mix;
   int *active = nullptr;
   int *active_host = nullptr;  // pinned, 2 entries
+  // work list of the persistent kernels and the per-chunk record of unfinished
+  // pairs (hist[k] after chunk k, ctr chunks so far)
+  int *wl = nullptr, *nwl = nullptr, *hist = nullptr, *ctr = nullptr;
   BatchTune tune64, tune32;
-  hipGraphExec_t graph[2] = {nullptr, nullptr};  // [precision]
+  // [precision]: the one-chunk graph and the whole-solve graph of sgraph_n chunks
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t sgraph[2] = {nullptr, nullptr};
+  int sgraph_n[2] = {0, 0};
   int graph_chunk[2] = {0, 0};
   double graph_res[2] = {0.0, 0.0}, graph_zero[2] = {0.0, 0.0};
+  int64_t learned[2] = {0, 0};  // chunks the last solve needed (0: none yet)
   BatchDev dev() const
   {
     BatchDev b;
@@ -96,6 +110,8 @@ struct ssf_batch {
     b.slice_row0 = slice_row0;
     b.blk_s0 = blk_s0;
     b.blk_s1 = blk_s1;
+    b.wl = wl;
+    b.nwl = nwl;
     return b;
   }
   template <typename T>
@@ -122,8 +138,10 @@ BatchVec<float> ssf_batch::vec<float>() const
 
 static void batch_release(ssf_batch *b)
 {
-  for (int k = 0; k < 2; k++)
+  for (int k = 0; k < 2; k++) {
     if (b->graph[k]) (void)hipGraphExecDestroy(b->graph[k]);
+    if (b->sgraph[k]) (void)hipGraphExecDestroy(b->sgraph[k]);
+  }
   bfree(b->rowptr); bfree(b->col); bfree(b->val); bfree(b->diag); bfree(b->val32); bfree(b->diag32);
   bfree(b->blk_sys); bfree(b->blk_r0); bfree(b->blk_r1); bfree(b->sys_blk0);
   bfree(b->soff); bfree(b->scol); bfree(b->scol16); bfree(b->slice_row0); bfree(b->blk_s0);
@@ -138,6 +156,7 @@ static void batch_release(ssf_batch *b)
   bfree(m.since_best); bfree(m.final); bfree(m.status); bfree(m.updates); bfree(m.sys_final);
   bfree(m.norm_a);
   bfree(b->active);
+  bfree(b->wl); bfree(b->nwl); bfree(b->hist); bfree(b->ctr);
   if (b->active_host) (void)hipHostFree(b->active_host);
   b->active_host = nullptr;
 }
@@ -226,6 +245,8 @@ extern "C" int ssf_batch_create(ssf_ctx *ctx, int64_t nsys, ssf_matrix *const *m
   c(balloc(&b->h, nv)); c(balloc(&b->y, nv)); c(balloc(&b->part, 3 * (int64_t)b->nblk * nrhs));
   c(b->st64.alloc(nq, nsys));
   c(balloc(&b->active, 1));
+  c(balloc(&b->wl, b->nblk)); c(balloc(&b->nwl, 1)); c(balloc(&b->hist, HIST_CAP));
+  c(balloc(&b->ctr, 1));
   c(balloc(&b->soff, b->nslice + 1)); c(balloc(&b->slice_row0, b->nslice));
   c(balloc(&b->blk_s0, b->nblk)); c(balloc(&b->blk_s1, b->nblk)); c(balloc(&b->sval64, b->stot));
   if (all16) c(balloc(&b->scol16, b->stot));
@@ -333,34 +354,65 @@ static hipError_t enqueue_chunk(ssf_batch *b, int mixed, int64_t c, const ssf_so
   }
   if (e != hipSuccess) return e;
   if (mixed)
-    return batch_mixed_refresh(b->tune64, B, b->vec<double>(), b->vec<float>(), b->st32.s, b->mix,
-                               p->res, s);
-  e = zero_int(b->active, s);
-  if (e == hipSuccess) e = batch_flags<double>(B, b->st64.s, b->active, s);
+    e = batch_mixed_refresh(b->tune64, B, b->vec<double>(), b->vec<float>(), b->st32.s, b->mix,
+                            p->res, s);
+  else {
+    e = zero_int(b->active, s);
+    if (e == hipSuccess) e = batch_flags<double>(B, b->st64.s, b->active, s);
+  }
+  // the next chunk's work list: the systems not yet finished
+  if (e == hipSuccess)
+    e = batch_compact(B, mixed ? b->mix.sys_final : b->st64.s.sys_done, b->wl, b->nwl, b->active,
+                      b->hist, b->ctr, HIST_CAP, s);
   return e;
 }
 
-// a full chunk is captured once into a graph (kernel arguments include res and
-// zero) and replayed; the graph holds kernel nodes only
-static int chunk_graph(ssf_batch *b, int mixed, int chunk, const ssf_solve_params *p)
+// nch full chunks captured into one graph (kernel arguments include res and
+// zero); the graph holds kernel nodes only
+static int capture_chunks(ssf_batch *b, int mixed, int chunk, int64_t nch,
+                          const ssf_solve_params *p, hipGraphExec_t *out)
 {
-  if (b->graph[mixed] && b->graph_chunk[mixed] == chunk && b->graph_res[mixed] == p->res &&
-      b->graph_zero[mixed] == p->zero)
-    return SSF_OK;
-  if (b->graph[mixed]) (void)hipGraphExecDestroy(b->graph[mixed]);
-  b->graph[mixed] = nullptr;
   hipStream_t s = b->ctx->stream;
   SSF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  hipError_t e = enqueue_chunk(b, mixed, chunk, p);
+  hipError_t e = hipSuccess;
+  for (int64_t k = 0; k < nch && e == hipSuccess; k++) e = enqueue_chunk(b, mixed, chunk, p);
   hipGraph_t g = nullptr;
   hipError_t e2 = hipStreamEndCapture(s, &g);
   if (e == hipSuccess) e = e2;
-  if (e == hipSuccess) e = hipGraphInstantiate(&b->graph[mixed], g, nullptr, nullptr, 0);
+  if (e == hipSuccess) e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
   if (g) (void)hipGraphDestroy(g);
   SSF_HIP(e);
-  b->graph_chunk[mixed] = chunk;
-  b->graph_res[mixed] = p->res;
-  b->graph_zero[mixed] = p->zero;
+  return SSF_OK;
+}
+
+// the one-chunk graph of the host loop, and the whole-solve graph of nch chunks
+// (nch <= 1: none); both are rebuilt when the chunk, res or zero change
+static int solve_graphs(ssf_batch *b, int mixed, int chunk, int64_t nch, const ssf_solve_params *p)
+{
+  const bool same = b->graph_chunk[mixed] == chunk && b->graph_res[mixed] == p->res &&
+                    b->graph_zero[mixed] == p->zero;
+  if (!same) {
+    for (hipGraphExec_t *g : {&b->graph[mixed], &b->sgraph[mixed]})
+      if (*g) (void)hipGraphExecDestroy(*g);
+    b->graph[mixed] = b->sgraph[mixed] = nullptr;
+    b->sgraph_n[mixed] = 0;
+    b->learned[mixed] = 0;
+    b->graph_chunk[mixed] = chunk;
+    b->graph_res[mixed] = p->res;
+    b->graph_zero[mixed] = p->zero;
+  }
+  if (!b->graph[mixed]) {
+    int rc = capture_chunks(b, mixed, chunk, 1, p, &b->graph[mixed]);
+    if (rc) return rc;
+  }
+  if (nch > 1 && b->sgraph_n[mixed] != nch) {
+    if (b->sgraph[mixed]) (void)hipGraphExecDestroy(b->sgraph[mixed]);
+    b->sgraph[mixed] = nullptr;
+    b->sgraph_n[mixed] = 0;
+    int rc = capture_chunks(b, mixed, chunk, nch, p, &b->sgraph[mixed]);
+    if (rc) return rc;
+    b->sgraph_n[mixed] = (int)nch;
+  }
   return SSF_OK;
 }
 
@@ -385,31 +437,49 @@ extern "C" int ssf_batch_solve_dev(ssf_batch *b, const ssf_solve_params *p, cons
   else SSF_HIP(hipMemsetAsync(b->x, 0, nv * sizeof(double), s));
   const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(p->inner_ni > 0 ? p->inner_ni : 32,
                                                                  std::max<int64_t>(p->ni, 1)));
+  SSF_CHECK(mixed || !p->iv, SSF_EUNSUP, "ssf_batch_solve_dev: the FP64 batched solve starts from x = 0");
+  // every block works in the set-up; the chunk record starts empty
+  SSF_HIP(zero_int(b->ctr, s));
+  SSF_HIP(batch_compact(B, nullptr, b->wl, b->nwl, nullptr, nullptr, nullptr, 0, s));
   if (mixed) {
     SSF_HIP(hipMemsetAsync(b->x32, 0, nv * sizeof(float), s));
     const double delta = (p->inner_res > 0.0 && p->inner_res < 1.0) ? p->inner_res : 0.1;
     SSF_HIP(batch_mixed_start(b->tune64, b->tune32, B, b->vec<double>(), b->vec<float>(), b->st32.s,
                               b->mix, p->res, delta, s));
+    SSF_HIP(batch_compact(B, b->mix.sys_final, b->wl, b->nwl, nullptr, nullptr, nullptr, 0, s));
   } else {
-    SSF_CHECK(!p->iv, SSF_EUNSUP, "ssf_batch_solve_dev: the FP64 batched solve starts from x = 0");
     SSF_HIP(batch_init<double>(B, b->vec<double>(), b->st64.s, p->zero, s));
   }
-  int rc = chunk_graph(b, mixed ? 1 : 0, chunk, p);
+  const int mk = mixed ? 1 : 0;
+  int rc = solve_graphs(b, mk, chunk, 0, p);  // drops graphs and record of other parameters
   if (rc) return rc;
+  // the whole solve in one launch: the chunks the last solve of these
+  // parameters needed, plus one (never more than ni iterations)
+  const int64_t nfull = p->ni / chunk;
+  const int64_t G = b->learned[mk] > 0 ? std::min<int64_t>(b->learned[mk] + 1, nfull) : 0;
+  rc = solve_graphs(b, mk, chunk, G, p);
+  if (rc) return rc;
+  int64_t it = 0;
+  bool finished = false;
+  if (G > 1) {
+    SSF_HIP(hipGraphLaunch(b->sgraph[mk], s));
+    SSF_HIP(hipMemcpyAsync(&b->active_host[0], b->active, sizeof(int), hipMemcpyDeviceToHost, s));
+    SSF_HIP(hipStreamSynchronize(s));
+    it = G * chunk;
+    finished = b->active_host[0] == 0;
+  }
   hipEvent_t ev[2];
   SSF_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
   SSF_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
-  int64_t it = 0;
   int k = 0;
   bool pending = false;
   hipError_t e = hipSuccess;
-  while (true) {
+  while (!finished) {
     bool launched = false;
     if (it < p->ni) {
       // the last chunk is shortened so that exactly ni iterations run
       const int64_t c = std::min<int64_t>(chunk, p->ni - it);
-      e = c == chunk ? hipGraphLaunch(b->graph[mixed ? 1 : 0], s)
-                     : enqueue_chunk(b, mixed ? 1 : 0, c, p);
+      e = c == chunk ? hipGraphLaunch(b->graph[mk], s) : enqueue_chunk(b, mk, c, p);
       if (e != hipSuccess) break;
       it += c;
       (void)hipMemcpyAsync(&b->active_host[k & 1], b->active, sizeof(int), hipMemcpyDeviceToHost, s);
@@ -428,6 +498,21 @@ extern "C" int ssf_batch_solve_dev(ssf_batch *b, const ssf_solve_params *p, cons
   (void)hipEventDestroy(ev[0]);
   (void)hipEventDestroy(ev[1]);
   SSF_HIP(e);
+  {
+    // chunks this solve needed: the first chunk after which no pair was running
+    int nch = 0;
+    SSF_HIP(hipMemcpy(&nch, b->ctr, sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<int> hist(std::min(nch, HIST_CAP));
+    if (!hist.empty())
+      SSF_HIP(hipMemcpy(hist.data(), b->hist, hist.size() * sizeof(int), hipMemcpyDeviceToHost));
+    int64_t need = nch;
+    for (size_t c = 0; c < hist.size(); c++)
+      if (hist[c] == 0) {
+        need = (int64_t)c + 1;
+        break;
+      }
+    b->learned[mk] = need;
+  }
   SSF_HIP(hipMemcpyAsync(lhs_dev, b->x, nv * sizeof(double), hipMemcpyDeviceToDevice, s));
   // per-pair results
   std::vector<int> status(nq), done(nq);
@@ -527,6 +612,8 @@ extern "C" int ssf_batch_autotune(ssf_batch *b, int precision, int reps, int32_t
   if (reps <= 0) reps = 5;
   hipStream_t s = b->ctx->stream;
   const BatchDev B = b->dev();
+  // time the products over every block
+  SSF_HIP(batch_compact(B, nullptr, b->wl, b->nwl, nullptr, nullptr, nullptr, 0, s));
   hipEvent_t t0, t1;
   SSF_HIP(hipEventCreate(&t0));
   SSF_HIP(hipEventCreate(&t1));
@@ -570,7 +657,10 @@ extern "C" int ssf_batch_autotune(ssf_batch *b, int precision, int reps, int32_t
   // a new tiling invalidates the captured chunk
   const int k = precision ? 1 : 0;
   if (b->graph[k]) (void)hipGraphExecDestroy(b->graph[k]);
-  b->graph[k] = nullptr;
+  if (b->sgraph[k]) (void)hipGraphExecDestroy(b->sgraph[k]);
+  b->graph[k] = b->sgraph[k] = nullptr;
+  b->sgraph_n[k] = 0;
+  b->graph_chunk[k] = 0;
   if (tune_out) {
     tune_out[0] = bt.format;
     tune_out[1] = bt.lanes;

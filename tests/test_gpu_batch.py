@@ -175,3 +175,25 @@ def test_batch_device_vectors_and_autotune(ctx):
         for r in range(R):
             assert rel(xt[s][r], xh[s][r]) <= 1e-8
             assert it[s * R + r].status == ih[s * R + r].status
+
+
+def test_batch_one_graph_launch_repeats_the_host_loop(ctx):
+    """A solve after the first replays the whole solve from one graph sized by the
+    last solve's chunk count; a harder right-hand side that needs more chunks
+    continues from the host loop. Results equal the first solve's bit for bit."""
+    probs = problems()
+    mats, rhs = build(ctx, probs, 2, seed=13)
+    b = gpu.Batch(ctx, mats, 2)
+    # an easy right-hand side first: a multiple of x0 = A-applied smooth vector
+    easy = [np.stack([m.gmxv(np.ones(p.n)), m.gmxv(np.ones(p.n))]) for m, p in zip(mats, probs)]
+    x0, i0 = b.solve(easy, ni=20000, res=RES)
+    for mixed in (gpu.FP64, gpu.MIXED):
+        xa, ia = b.solve(rhs, ni=20000, res=RES, precision=mixed)  # more chunks than learned
+        xb, ib = b.solve(rhs, ni=20000, res=RES, precision=mixed)  # one graph launch
+        xc, ic = b.solve(rhs, ni=20000, res=RES, precision=mixed)
+        for s in range(len(probs)):
+            for r in range(2):
+                assert np.array_equal(xa[s][r], xb[s][r]) and np.array_equal(xa[s][r], xc[s][r])
+                q = s * 2 + r
+                assert ia[q].iterations == ib[q].iterations == ic[q].iterations
+                assert ia[q].status == ib[q].status == ic[q].status
