@@ -32,6 +32,7 @@ import argparse
 import json
 import os
 import sys
+import threading
 import time
 
 import numpy as np
@@ -46,11 +47,90 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def cpu_info():
+    """CPU model and the threads this process may use (the box's host cores)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return {"model": model, "logical_cpus": os.cpu_count(), "available_to_process": avail}
+
+
+class OracleSolve(threading.Thread):
+    """The oracle (CPU restatement of the reference's symcomprow assembly and
+    cg_prec, one thread) on a problem, in the background: its own assembly from
+    the element data, then cg_prec from x = 0 to `res`. The ctypes calls release
+    the GIL, so it runs beside the GPU legs."""
+
+    def __init__(self, prob, res, ni=400000, rhs=None):
+        super().__init__(daemon=True)
+        self.prob, self.res, self.ni, self.rhs = prob, res, ni, rhs
+        self.err = None
+
+    def run(self):
+        try:
+            import oracle as O
+            t = time.perf_counter()
+            self.adr, self.ci, self.a, rhs = O.assemble_problem(self.prob, "scr")
+            self.y = rhs[0] if self.rhs is None else self.rhs
+            self.t_assemble = time.perf_counter() - t
+            t = time.perf_counter()
+            self.x, self.ani, self.ares, self.status = O.scr_cg_prec(self.adr, self.ci, self.a,
+                                                                     self.y, self.ni, self.res)
+            self.t_solve = time.perf_counter() - t
+        except Exception as e:  # reported by the caller
+            self.err = repr(e)
+
+    def summary(self):
+        return {"res": self.res, "status": int(self.status), "ani": int(self.ani),
+                "ares": float(self.ares), "assembly_s": self.t_assemble, "solve_s": self.t_solve,
+                "threads": 1}
+
+
+def cpu_parallel(adr, ci, a, y, its, k=10):
+    """P = the process's available CPUs (at most 16) concurrent oracle cg_prec runs
+    of k iterations each on the same system (independent vectors), as the host
+    would solve independent systems side by side; scaled to `its` iterations per
+    solve. Returns the cpu_baseline fields of this leg."""
+    import oracle as O
+    P = max(1, min(16, cpu_info()["available_to_process"]))
+    done = []
+
+    def one():
+        O.scr_cg_prec(adr, ci, a, y, k, 0.0)
+        done.append(1)
+
+    th = [threading.Thread(target=one) for _ in range(P)]
+    t = time.perf_counter()
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    dt = time.perf_counter() - t
+    return {"value": P / (dt / k * its), "unit": "solves/s", "cores": P,
+            "sample": f"{P} concurrent oracle cg_prec runs of {k} iterations each ({dt:.2f} s), "
+                      f"scaled to {its} iterations per solve"}
+
+
 def bench_batched(ctx, args, gpu, mesh, torch):
     """512 independent quad4 plane-stress plates (1e3..6e4 DOF, seed 4), 8 load cases
     each: 4096 solves in one batch. Compared with the drop-in path, which solves
     the 4096 systems one after the other with the FP64 GPU PCG (timed on every
-    `seq_stride`-th plate, all 8 load cases, and scaled to the 512 plates)."""
+    `seq_stride`-th plate, all 8 load cases, and scaled to the 512 plates).
+    Errors are against the batched FP64 recurrence run to 1e-28, itself checked
+    against the oracle's FP64 CPU cg_prec (to 1e-30) on a sample of plates. The
+    mixed form stops on its true residual, which bounds the error of these
+    high-condition plates less tightly than the FP64 recurrence's recursive
+    residual does: it runs at --batch-res-mixed (1e-22)."""
     t0 = time.time()
     probs = mesh.plate_batch(args.batch_count, seed=4, nrhs=8)
     mats, rhs = [], []
@@ -86,14 +166,34 @@ def bench_batched(ctx, args, gpu, mesh, torch):
         den.index_add_(0, idx, rr * rr)
         return torch.sqrt(num / den).reshape(-1).cpu().numpy()
 
+    oracle_check = None
+    if not args.no_cpu:
+        # x_ref against the oracle's FP64 CPU solve on a sample of plates (the
+        # largest among them), first load case
+        import oracle as O
+        xr = b.unpack(xref.cpu().numpy())
+        big = int(np.argmax([p.n for p in probs]))
+        sample = sorted({0, len(probs) // 3, 2 * len(probs) // 3, big})
+        errs = {}
+        t = time.perf_counter()
+        for s in sample:
+            adr, ci = mats[s].pattern()
+            xo, ani, _, st = O.scr_cg_prec(adr, ci, mats[s].get_values(), rhs[s][0], 400000, 1e-30)
+            errs[str(s)] = {"n": int(probs[s].n), "oracle_status": int(st),
+                            "oracle_iterations": int(ani + 1),
+                            "rel_err_x_ref": float(np.linalg.norm(xr[s][0] - xo) / np.linalg.norm(xo))}
+        oracle_check = {"plates": errs, "seconds": time.perf_counter() - t,
+                        "max_rel_err_x_ref": max(v["rel_err_x_ref"] for v in errs.values())}
+        log(f"[batch] x_ref vs the oracle: {oracle_check}")
     out = {}
-    for name, prec in (("fp64", gpu.FP64), ("mixed", gpu.MIXED)):
+    for name, prec, res in (("fp64", gpu.FP64, args.batch_res),
+                            ("mixed", gpu.MIXED, args.batch_res_mixed)):
         for _ in range(args.warmup):
-            b.solve_dev(y.data_ptr(), x.data_ptr(), ni=40000, res=args.batch_res, precision=prec)
+            b.solve_dev(y.data_ptr(), x.data_ptr(), ni=40000, res=res, precision=prec)
         torch.cuda.synchronize()
         t = time.perf_counter()
         for _ in range(args.steps):
-            infos = b.solve_dev(y.data_ptr(), x.data_ptr(), ni=40000, res=args.batch_res,
+            infos = b.solve_dev(y.data_ptr(), x.data_ptr(), ni=40000, res=res,
                                 precision=prec)
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t) / args.steps
@@ -102,7 +202,7 @@ def bench_batched(ctx, args, gpu, mesh, torch):
         st = np.array([i.status for i in infos])
         by_status = {int(k): {"pairs": int((st == k).sum()), "max_rel_err": float(err[st == k].max())}
                      for k in np.unique(st)}
-        out[name] = {"solves_per_s": nq / dt, "ms_per_batch": 1e3 * dt,
+        out[name] = {"solves_per_s": nq / dt, "ms_per_batch": 1e3 * dt, "res": res,
                      "max_rel_err": float(err.max()), "by_status": by_status,
                      "max_iterations": int(max(its)), "mean_iterations": float(np.mean(its))}
         log(f"[batch] {name}: {out[name]}")
@@ -129,12 +229,17 @@ def bench_batched(ctx, args, gpu, mesh, torch):
         m.close()
     return {
         "workload": f"{len(probs)} quad4 plates (seed 4), {ndof} DOF, 8 load cases each",
-        "solves": nq, "res": args.batch_res,
+        "solves": nq, "res": {"fp64": args.batch_res, "mixed": args.batch_res_mixed},
+        "error_reference": "batched FP64 recurrence to 1e-28 (x_ref), checked against the oracle",
+        "oracle_check_x_ref": oracle_check,
+        "launch": "one hipGraph launch per batched solve (the graph holds the chunks the "
+                  "previous solve needed, plus one; persistent kernels over the running systems)",
         "batched_mixed_solves_per_s": out["mixed"]["solves_per_s"],
         "batched_fp64_solves_per_s": out["fp64"]["solves_per_s"],
         "sequential_fp64_solves_per_s": seq,
         "speedup_mixed_vs_sequential_fp64": out["mixed"]["solves_per_s"] / seq,
         "speedup_mixed_vs_batched_fp64": out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"],
+        "speedup_fp64_batched_vs_sequential_fp64": out["fp64"]["solves_per_s"] / seq,
         "max_rel_err": {"mixed": out["mixed"]["max_rel_err"], "fp64": out["fp64"]["max_rel_err"]},
         "detail": out, "spmm_format_lanes": {"fp64": lanes64, "fp32": lanes32},
         "sequential_sample": f"every {stride}th plate ({len(sample)} plates x 8 load cases), "
@@ -145,9 +250,13 @@ def bench_batched(ctx, args, gpu, mesh, torch):
 def bench_served(ctx, args, gpu, mesh, torch):
     """Served solves: a Poisson stream of requests (seed 5), half hex8 elasticity
     columns and half heat cubes of 5e3..2e6 DOF (log-uniform), admitted under a
-    200 GB HBM budget. Latency = finished - submitted per request (server clock).
-    Compared with the drop-in path, which builds, assembles and solves the same
-    requests one after the other with the FP64 GPU PCG."""
+    200 GB HBM budget, for each arrival rate of --serve-rates and each precision
+    of --serve-precisions (FP64: the cg_prec recurrence per request; mixed: FP32
+    recurrences refreshed in FP64 every chunk). Latency = finished - submitted per
+    request (server clock). Compared with the drop-in path, which builds,
+    assembles and solves the same requests one after the other with the FP64 GPU
+    PCG; errors are against those drop-in solutions, which are checked against the
+    oracle's FP64 CPU cg_prec on the --serve-oracle smallest requests."""
     rng = np.random.default_rng(5)
     t0 = time.time()
     probs = [mesh.served_request(rng) for _ in range(args.serve_count)]
@@ -156,38 +265,7 @@ def bench_served(ctx, args, gpu, mesh, torch):
         f"generated in {time.time() - t0:.1f}s")
     out = {"workload": f"{len(probs)} requests (seed 5): hex8 elasticity / heat, "
                        f"5e3..2e6 DOF, {sum(ndof)} DOF in total",
-           "res": args.serve_res, "budget_bytes": 200 << 30, "rates": {}}
-    for rate in args.serve_rates:
-        arr = np.cumsum(np.random.default_rng(7).exponential(1.0 / rate, len(probs)))
-        srv = gpu.Server(budget_bytes=200 << 30, res=args.serve_res, chunk=32, prep_threads=4)
-        try:
-            tickets = []
-            start = time.perf_counter()
-            for p, ta in zip(probs, arr):
-                dt = ta - (time.perf_counter() - start)
-                if dt > 0:
-                    time.sleep(dt)
-                tickets.append(srv.submit(p))
-            res = [srv.wait(t) for t in tickets]
-            st = srv.stats()
-        finally:
-            srv.close()
-        times = np.array([r[2] for r in res])
-        lat = times[:, 3] - times[:, 0]
-        span = times[:, 3].max() - times[:, 0].min()
-        conv = sum(1 for r in res if r[1].status == 1)
-        out["rates"][str(rate)] = {
-            "offered_req_per_s": rate, "solves_per_s": len(probs) / span,
-            "p50_latency_s": float(np.percentile(lat, 50)),
-            "p99_latency_s": float(np.percentile(lat, 99)),
-            "mean_queue_s": float(np.mean(times[:, 1] - times[:, 0])),
-            "mean_assembly_s": float(np.mean(times[:, 2] - times[:, 1])),
-            "mean_solve_s": float(np.mean(times[:, 3] - times[:, 2])),
-            "converged": conv, "peak_bytes": st["peak_bytes"], "steps": st["steps"],
-            "prep_thread_s": {"pattern": st["pattern_s"], "setup": st["setup_s"],
-                              "assembly": st["assembly_s"]}}
-        log(f"[serve] rate {rate}/s: {out['rates'][str(rate)]}")
-        xs = [r[0] for r in res]
+           "res": args.serve_res, "budget_bytes": 200 << 30, "chunk": args.serve_chunk}
     # the drop-in path, one request after the other: pattern, assembly, FP64 solve
     t = time.perf_counter()
     seq_x = []
@@ -198,13 +276,64 @@ def bench_served(ctx, args, gpu, mesh, torch):
         m.close()
     dt = time.perf_counter() - t
     out["sequential_fp64_solves_per_s"] = len(probs) / dt
-    # the served results against the drop-in ones (same recurrence, FP64)
-    out["max_rel_diff_vs_sequential"] = float(max(
-        np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(xs, seq_x)))
-    best = max(v["solves_per_s"] for v in out["rates"].values())
-    out["speedup_served_vs_sequential_fp64"] = best / out["sequential_fp64_solves_per_s"]
-    log(f"[serve] sequential drop-in {out['sequential_fp64_solves_per_s']:.2f} solves/s, "
-        f"served best {best:.2f}; max rel diff {out['max_rel_diff_vs_sequential']:.1e}")
+    log(f"[serve] sequential drop-in {out['sequential_fp64_solves_per_s']:.2f} solves/s")
+    if not args.no_cpu and args.serve_oracle > 0:
+        import oracle as O
+        small = np.argsort(ndof)[:args.serve_oracle]
+        chk = {}
+        for k in small:
+            adr, ci, a, rhs_o = O.assemble_problem(probs[k], "scr")
+            xo, ani, _, st = O.scr_cg_prec(adr, ci, a, rhs_o[0], 400000, 1e-30)
+            chk[str(int(k))] = {"n": int(ndof[k]), "oracle_iterations": int(ani + 1),
+                                "rel_err_drop_in": float(np.linalg.norm(seq_x[k] - xo) /
+                                                         np.linalg.norm(xo))}
+        out["oracle_check_drop_in"] = chk
+        log(f"[serve] drop-in vs the oracle: {chk}")
+    for pname in args.serve_precisions:
+        prec = gpu.MIXED if pname == "mixed" else gpu.FP64
+        rates = {}
+        for rate in args.serve_rates:
+            arr = np.cumsum(np.random.default_rng(7).exponential(1.0 / rate, len(probs)))
+            srv = gpu.Server(budget_bytes=200 << 30, res=args.serve_res, chunk=args.serve_chunk,
+                             prep_threads=4, precision=prec)
+            try:
+                tickets = []
+                start = time.perf_counter()
+                for p, ta in zip(probs, arr):
+                    dt = ta - (time.perf_counter() - start)
+                    if dt > 0:
+                        time.sleep(dt)
+                    tickets.append(srv.submit(p))
+                res = [srv.wait(t) for t in tickets]
+                st = srv.stats()
+            finally:
+                srv.close()
+            times = np.array([r[2] for r in res])
+            lat = times[:, 3] - times[:, 0]
+            span = times[:, 3].max() - times[:, 0].min()
+            errs = np.array([np.linalg.norm(r[0] - b) / np.linalg.norm(b)
+                             for r, b in zip(res, seq_x)])
+            stat = np.array([r[1].status for r in res])
+            rates[str(rate)] = {
+                "offered_req_per_s": rate, "solves_per_s": len(probs) / span,
+                "p50_latency_s": float(np.percentile(lat, 50)),
+                "p99_latency_s": float(np.percentile(lat, 99)),
+                "max_latency_s": float(lat.max()),
+                "mean_queue_s": float(np.mean(times[:, 1] - times[:, 0])),
+                "mean_assembly_s": float(np.mean(times[:, 2] - times[:, 1])),
+                "mean_solve_s": float(np.mean(times[:, 3] - times[:, 2])),
+                "status_counts": {str(int(k)): int(v) for k, v in
+                                  zip(*np.unique(stat, return_counts=True))},
+                "max_rel_err_vs_drop_in": float(errs.max()),
+                "peak_bytes": st["peak_bytes"], "steps": st["steps"],
+                "prep_thread_s": {"pattern": st["pattern_s"], "setup": st["setup_s"],
+                                  "assembly": st["assembly_s"]}}
+            rates[str(rate)]["speedup_vs_sequential_fp64"] = (
+                rates[str(rate)]["solves_per_s"] / out["sequential_fp64_solves_per_s"])
+            log(f"[serve] {pname} rate {rate}/s: {rates[str(rate)]}")
+        out[pname] = rates
+    best = {p: max(v["solves_per_s"] for v in out[p].values()) for p in args.serve_precisions}
+    out["best_solves_per_s"] = best
     return out
 
 
@@ -376,6 +505,13 @@ def bench_elastic(ctx, args, gpu, mesh, torch):
     iterations of the oracle's cg_prec on the same system, scaled to the FP64
     iteration count."""
     nel = args.elast_nel
+    # the oracle's FP64 CPU solve of a mid-size column of the same family, beside
+    oref = None
+    if not args.no_cpu:
+        mid = mesh.elastic_column(args.elast_oracle_nel, args.elast_oracle_nel,
+                                  4 * args.elast_oracle_nel, seed=3)
+        oref = OracleSolve(mid, 1e-30)
+        oref.start()
     t0 = time.time()
     prob = mesh.elastic_column(nel, nel, 4 * nel, seed=3)
     m = gpu.Matrix.from_elements(ctx, gpu.STORAGE_SCR, prob.n, prob.elem_codes())
@@ -423,6 +559,25 @@ def bench_elastic(ctx, args, gpu, mesh, torch):
                      "spmv_bytes": act, "spmv_gbs": act / (spmv_us * 1e-6) / 1e9 if spmv_us else None}
         log(f"[elast] {name}: {out[name]}")
     out["speedup_mixed_vs_fp64"] = out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"]
+    out["error_reference"] = "the GPU FP64 recurrence to 1e-30 on this column"
+    if oref is not None:
+        # both forms on the mid-size column against the oracle's solution
+        mm, rr = gpu.build_problem_matrix(ctx, mid)
+        oref.join()
+        chk = {"workload": f"hex8 elasticity column {mid.name}, n={mid.n}"}
+        if oref.err:
+            chk["error"] = oref.err
+        else:
+            chk["oracle"] = oref.summary()
+            no = np.linalg.norm(oref.x)
+            for name, prec in (("fp64", gpu.FP64), ("mixed", gpu.MIXED)):
+                xm, im = mm.solve(rr[0], ni=400000, res=args.res, precision=prec,
+                                  inner_ni=args.interval)
+                chk[name] = {"rel_err": float(np.linalg.norm(xm - oref.x) / no),
+                             "status": int(im.status), "iterations": int(im.iterations)}
+        mm.close()
+        out["oracle_check_mid_size"] = chk
+        log(f"[elast] mid-size check against the oracle: {chk}")
     if not args.no_cpu:
         import oracle as O
         adr, ci = m.pattern()
@@ -609,6 +764,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched leg")
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--batch-res", type=float, default=1e-20)
+    ap.add_argument("--batch-res-mixed", type=float, default=1e-22)
     ap.add_argument("--seq-stride", type=int, default=16)
     ap.add_argument("--only-batch", action="store_true")
     ap.add_argument("--tiling64", type=int, nargs=4, default=None,
@@ -631,11 +787,17 @@ def main():
     ap.add_argument("--no-elast", action="store_true", help="skip the elasticity column leg")
     ap.add_argument("--elast-nel", type=int, default=64,
                     help="elasticity column nel x nel x 4 nel elements")
+    ap.add_argument("--elast-oracle-nel", type=int, default=16,
+                    help="column checked against the oracle: nel x nel x 4 nel")
     ap.add_argument("--ildl-steps", type=int, default=1)
     ap.add_argument("--bicg-steps", type=int, default=2)
     ap.add_argument("--serve-count", type=int, default=24)
     ap.add_argument("--serve-res", type=float, default=1e-22)
-    ap.add_argument("--serve-rates", type=float, nargs="+", default=[20.0])
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[50.0])
+    ap.add_argument("--serve-precisions", nargs="+", default=["fp64", "mixed"])
+    ap.add_argument("--serve-chunk", type=int, default=32)
+    ap.add_argument("--serve-oracle", type=int, default=3,
+                    help="smallest requests whose drop-in solution is checked by the oracle")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; the path shards by independent "
@@ -660,6 +822,12 @@ def main():
     t0 = time.time()
     prob = mesh.heat_cube(args.nel, seed=2)
     log(f"[bench] problem {prob.name}: n={prob.n} ne={prob.ne} ({time.time() - t0:.1f}s)")
+    # the reference's FP64 CPU solve of the same system (the oracle: its own
+    # assembly and cg_prec to 1e-30), beside the GPU legs
+    oref = None
+    if not args.no_cpu:
+        oref = OracleSolve(prob, 1e-30)
+        oref.start()
     ctx = gpu.Context(0)
     t0 = time.time()
     m = gpu.Matrix.from_elements(ctx, gpu.STORAGE_SCR, prob.n, prob.elem_codes())
@@ -736,21 +904,44 @@ def main():
         f"copy {copy_gbs:.0f} GB/s")
 
     cpu = None
-    if not args.no_cpu:
-        import oracle as O
-        adr, ci = m.pattern()
-        a = m.get_values()
-        k = args.cpu_iters
+    oracle_ref = None
+    if oref is not None:
+        # the timed solutions against the oracle's; the solutions of the last
+        # timed solves are kept
+        xs = {}
+        for name, fn in (("fp64", fp64_solve), ("mixed", mixed_solve)):
+            fn()
+            torch.cuda.synchronize()
+            xs[name] = x.cpu().numpy()
         t = time.perf_counter()
-        O.scr_cg_prec(adr, ci, a, rhs, k, 0.0)
-        dt = time.perf_counter() - t
-        per_it = dt / k
-        its = results["fp64"]["iterations"]
-        cpu = {"value": 1.0 / (per_it * its), "unit": "solves/s", "cores": 1, "kind": "port",
-               "sample": f"{k} iterations of the oracle cg_prec (scr, Jacobi, 1 thread) on the "
-                         f"same {prob.n}-unknown system, scaled to the {its} iterations of the "
-                         f"FP64 recurrence"}
-        log(f"[bench] cpu: {per_it * 1e3:.1f} ms/iteration -> {cpu['value']:.4f} solves/s")
+        oref.join()
+        waited = time.perf_counter() - t
+        if oref.err:
+            log(f"[bench] oracle failed: {oref.err}")
+        else:
+            xo = oref.x
+            no = np.linalg.norm(xo)
+            oracle_ref = oref.summary()
+            oracle_ref["waited_s"] = waited
+            for name in ("fp64", "mixed"):
+                results[name]["rel_err_vs_gpu_ref"] = results[name]["rel_err"]
+                results[name]["rel_err"] = float(np.linalg.norm(xs[name] - xo) / no)
+            oracle_ref["gpu_ref_rel_err"] = float(
+                np.linalg.norm(xref.cpu().numpy() - xo) / no)
+            its = results["fp64"]["iterations"]
+            per_it = oref.t_solve / (oref.ani + 1)
+            ci = cpu_info()
+            cpu = {"value": 1.0 / (per_it * its), "unit": "solves/s", "cores": 1, "kind": "port",
+                   "cpu_model": ci["model"], "cpus_available": ci["available_to_process"],
+                   "sample": f"the oracle's full cg_prec solve (scr, Jacobi, 1 thread) of the same "
+                             f"{prob.n}-unknown system to 1e-30 ({oref.ani + 1} iterations, "
+                             f"{oref.t_solve:.1f} s, run beside the GPU legs), per-iteration time "
+                             f"scaled to the {its} iterations of the FP64 recurrence at {args.res}"}
+            cpu["parallel"] = cpu_parallel(oref.adr, oref.ci, oref.a, oref.y, its)
+            log(f"[bench] oracle x: {oracle_ref}; errors fp64 {results['fp64']['rel_err']:.2e} "
+                f"mixed {results['mixed']['rel_err']:.2e}")
+            log(f"[bench] cpu: {per_it * 1e3:.1f} ms/iteration -> {cpu['value']:.4f} solves/s; "
+                f"{cpu['parallel']['cores']} threads: {cpu['parallel']['value']:.4f} solves/s")
 
     # HBM traffic per launch measured by rocprofv3 counters (calibrated on the copy
     # kernel) for this workload and tiling, when such a measurement is committed
@@ -813,7 +1004,12 @@ def main():
                    "spmv_tiling": {"fp64": tune64, "fp32": tune32}},
         "fp64_solves_per_s": fp64["solves_per_s"],
         "speedup_vs_fp64": mixed["solves_per_s"] / fp64["solves_per_s"],
-        "rel_err": {"mixed": mixed["rel_err"], "fp64": fp64["rel_err"]},
+        "rel_err": {"mixed": mixed["rel_err"], "fp64": fp64["rel_err"],
+                    "reference": ("the oracle's FP64 CPU cg_prec of the same system to 1e-30"
+                                  if oracle_ref else "the GPU FP64 recurrence to 1e-30"),
+                    "vs_gpu_fp64_ref": {"mixed": mixed.get("rel_err_vs_gpu_ref"),
+                                        "fp64": fp64.get("rel_err_vs_gpu_ref")}},
+        "oracle_reference_solve": oracle_ref,
         "iterations": {"mixed_fp32": mixed["iterations"], "mixed_refinements": mixed["refinements"],
                        "fp64": fp64["iterations"]},
         "spmv_gbs": {"fp32": gbs32, "fp64": gbs64, "copy_kernel": copy_gbs},

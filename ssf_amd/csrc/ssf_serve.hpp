@@ -16,7 +16,11 @@ struct SlotDesc {
   const double *sval;
   const double *diag;
   const double *y;
-  double *x, *r, *d, *p, *h;
+  double *x, *r, *d, *p, *h;  // FP64 recurrence; the mixed form uses x, r only
+  // mixed form: FP32 values and recurrence, ||A||_inf of the FP64 values
+  const float *sval32, *diag32;
+  float *x32, *r32, *d32, *p32, *h32;
+  const double *norm_a;
   int n;
   int sid;         // index of the slot's scalars in SlotState (stable while resident)
   int blk0, nblk;  // the slot's blocks in the block table
@@ -29,6 +33,12 @@ struct SlotState {
   double *sc;                 // 3 per slot: d.p, r.r, r.h of the running iteration
   long long *iter, *ani;
   int *done, *status, *pend, *pend_status, *brk;
+  // mixed form (reliable updates, as the single-system solve): ||b||^2, ||b - A x||^2
+  // at the last refresh and its best, scale of the FP32 quantities, delta^2,
+  // ||x||^2 of the refresh; final: the solve has ended with mstatus
+  double *nory64, *rr64, *best, *scale, *delta2, *xx;
+  int *bad, *since_best, *final, *mstatus;
+  long long *updates;
 };
 
 struct ServeDev {
@@ -47,5 +57,15 @@ hipError_t serve_iteration(const ServeDev &S, SlotState st, double res, double z
                            long long ni, hipStream_t s);
 // applies pending state; counts unfinished slots into *active (zeroed first)
 hipError_t serve_flags(const ServeDev &S, SlotState st, int *active, hipStream_t s);
+
+// mixed form: fresh slots start from x = 0 (FP64 residual y, FP32 recurrence on
+// y / ||y||); an iteration advances the FP32 recurrences; a refresh folds the FP32
+// corrections into the FP64 solutions, recomputes b - A x in FP64, tests it and
+// restarts the FP32 recurrences on it (ni: FP32 iterations at most)
+hipError_t serve_mixed_init(const ServeDev &S, SlotState st, double res, double delta,
+                            double zero, hipStream_t s);
+hipError_t serve_mixed_iteration(const ServeDev &S, SlotState st, hipStream_t s);
+hipError_t serve_mixed_refresh(const ServeDev &S, SlotState st, double res, long long ni,
+                               int *active, hipStream_t s);
 
 }  // namespace ssf

@@ -12,6 +12,9 @@
 //    for all slots in one launch sequence; finished slots are retired (solution
 //    copied to the host, device memory and budget released) and their places
 //    are taken by the next ready requests at the following step.
+//    In the mixed form a step is `chunk` iterations of every slot's FP32
+//    recurrence followed by every slot's FP64 refresh (reliable update), which
+//    decides which slots are finished.
 //
 // A request's device data stays where it was allocated: the kernels reach each
 // slot through a descriptor, so admission and retirement move no matrix data.
@@ -50,7 +53,9 @@ struct Request {
   bool finished = false;
   // device
   ssf_matrix *m = nullptr;
-  double *vec = nullptr;  // y, x, r, d, p, h: 6 n doubles
+  // FP64: y, x, r, d, p, h (6 n doubles); mixed: y, x, r (3 n doubles), then
+  // x32, r32, d32, p32, h32, diag32 (6 n floats); then ||A||_inf (1 double)
+  double *vec = nullptr;
   int64_t bytes = 0;
   int sid = -1;
 };
@@ -116,19 +121,21 @@ struct ssf_server {
 namespace {
 
 // device bytes of a resident request: SELL slices (offsets, 16-bit columns or
-// 32-bit ones, FP64 values), diagonal, six solve vectors
-int64_t resident_bytes(int64_t n, int64_t tot, bool col16)
+// 32-bit ones, FP64 values; FP32 values in the mixed form), diagonal, the solve
+// vectors (six FP64 ones, or three FP64 and six FP32 ones)
+int64_t resident_bytes(int64_t n, int64_t tot, bool col16, bool mixed)
 {
-  return 4 * ((n + 63) / 64 + 1) + (col16 ? 10 : 12) * tot + 8 * n + 48 * n + (1 << 16);
+  return 4 * ((n + 63) / 64 + 1) + (col16 ? 10 : 12) * tot + (mixed ? 4 * tot : 0) + 8 * n +
+         48 * n + (1 << 16);
 }
 
 // and while it is laid out and assembled: the device pattern, the layout build's
 // temporaries (row index, sort keys and values, twice), SELL columns of both
 // widths, SELL positions and values of the reference layout, element slot map
 // and colour order
-int64_t prep_bytes(const Request &q, int64_t negm, int64_t tot)
+int64_t prep_bytes(const Request &q, int64_t negm, int64_t tot, bool mixed)
 {
-  return resident_bytes(q.n, tot, false) + 2 * tot + 4 * (q.n + 1) + 4 * negm + 28 * negm +
+  return resident_bytes(q.n, tot, false, mixed) + 2 * tot + 4 * (q.n + 1) + 4 * negm + 28 * negm +
          24 * q.n + 16 * negm + 8 * negm + 4 * q.ne * q.ndofe * q.ndofe + 8 * q.ne;
 }
 
@@ -146,6 +153,7 @@ void prep_main(ssf_server *S, int w)
 {
   (void)hipSetDevice(S->device);
   ssf_ctx *ctx = S->prep_ctx[w];
+  const bool mixed = S->params.precision == SSF_PRECISION_MIXED;
   while (true) {
     Request *q = nullptr;
     {
@@ -226,7 +234,7 @@ void prep_main(ssf_server *S, int w)
     double dt_create = 0, dt_assemble = 0;
     // 2. the request's set-up bytes, then the matrix laid out from the pattern
     if (rc == SSF_OK) {
-      q->bytes = prep_bytes(*q, negm, tot);
+      q->bytes = prep_bytes(*q, negm, tot, mixed);
       rc = reserve(q->bytes);
       if (rc == SSF_OK) {
         std::unique_lock<std::mutex> lk(S->mu);
@@ -259,7 +267,8 @@ void prep_main(ssf_server *S, int w)
         matrix_keep_sell_only(q->m);
         dt_assemble = std::chrono::duration<double>(clk::now() - ta0).count();
         {  // only the SELL solve structures are left: keep their bytes reserved
-          const int64_t keep = resident_bytes(q->n, q->m->sell_tot, q->m->sell_col16 != nullptr);
+          const int64_t keep =
+              resident_bytes(q->n, q->m->sell_tot, q->m->sell_col16 != nullptr, mixed);
           std::unique_lock<std::mutex> lk(S->mu);
           if (keep < q->bytes) {
             S->used -= q->bytes - keep;
@@ -269,11 +278,27 @@ void prep_main(ssf_server *S, int w)
         }
         for (int64_t i = 0; i < q->n; i++) lift[i] += q->rhs[i];
         hipError_t e = hipMallocAsync((void **)&q->vec,
-                                      6 * (size_t)std::max<int64_t>(q->n, 1) * sizeof(double),
+                                      (6 * (size_t)std::max<int64_t>(q->n, 1) + 1) * sizeof(double),
                                       ctx->stream);
         if (e == hipSuccess)
           e = hipMemcpyAsync(q->vec, lift.data(), q->n * sizeof(double), hipMemcpyHostToDevice,
                              ctx->stream);
+        if (e == hipSuccess && mixed) {
+          // FP32 copy of the SELL values and of the diagonal; ||A||_inf of the
+          // FP64 values, for the rounding floor of the FP64 residual
+          ssf_matrix *m = q->m;
+          e = hipMallocAsync((void **)&m->sell_v32, std::max<int64_t>(m->sell_tot, 1) * sizeof(float),
+                             ctx->stream);
+          if (e == hipSuccess) e = to_f32(m->sell_tot, m->sell_v64, m->sell_v32, 1.0, ctx->stream);
+          float *f = (float *)(q->vec + 3 * q->n);
+          if (e == hipSuccess) e = to_f32(q->n, m->diag, f + 5 * q->n, 1.0, ctx->stream);
+          double *part = nullptr;
+          if (e == hipSuccess) e = hipMallocAsync((void **)&part, 1024 * sizeof(double), ctx->stream);
+          if (e == hipSuccess)
+            e = sell_row_absmax((int)q->n, m->sell_off, m->sell_v64, part, 1024, q->vec + 6 * q->n,
+                                ctx->stream);
+          if (part) (void)hipFreeAsync(part, ctx->stream);
+        }
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
         if (e != hipSuccess) {
           rc = SSF_EHIP;
@@ -364,6 +389,17 @@ hipError_t upload_tables(ssf_server *S, const std::vector<char> &fresh)
     D.d = q->vec + 3 * q->n;
     D.p = q->vec + 4 * q->n;
     D.h = q->vec + 5 * q->n;
+    {  // the mixed form's FP32 vectors follow its three FP64 ones
+      float *f = (float *)(q->vec + 3 * q->n);
+      D.x32 = f;
+      D.r32 = f + q->n;
+      D.d32 = f + 2 * q->n;
+      D.p32 = f + 3 * q->n;
+      D.h32 = f + 4 * q->n;
+      D.diag32 = f + 5 * q->n;
+      D.sval32 = m->sell_v32;
+      D.norm_a = q->vec + 6 * q->n;
+    }
     D.n = (int)q->n;
     D.sid = q->sid;
     D.fresh = fresh[k];
@@ -426,6 +462,10 @@ void loop_main(ssf_server *S)
   (void)hipSetDevice(S->device);
   hipStream_t s = S->ctx->stream;
   const ssf_solve_params P = S->params;
+  const bool mixed = P.precision == SSF_PRECISION_MIXED;
+  const double delta = (P.inner_res > 0.0 && P.inner_res < 1.0) ? P.inner_res : 0.1;
+  // a slot is finished when its recurrence stopped (FP64) or its solve ended (mixed)
+  int *fin_dev = mixed ? S->st.final : S->st.done;
   std::string fail;
   while (true) {
     std::vector<Request *> joined;
@@ -453,14 +493,19 @@ void loop_main(ssf_server *S)
         fresh.push_back(1);
       }
       e = upload_tables(S, fresh);
-      if (e == hipSuccess) e = serve_init(serve_view(S), S->st, P.zero, s);
+      if (e == hipSuccess)
+        e = mixed ? serve_mixed_init(serve_view(S), S->st, P.res, delta, P.zero, s)
+                  : serve_init(serve_view(S), S->st, P.zero, s);
     }
     const ServeDev V = serve_view(S);
     for (int k = 0; k < S->chunk && e == hipSuccess; k++)
-      e = serve_iteration(V, S->st, P.res, P.zero, P.ni, s);
-    if (e == hipSuccess) e = serve_flags(V, S->st, S->d_active, s);
+      e = mixed ? serve_mixed_iteration(V, S->st, s)
+                : serve_iteration(V, S->st, P.res, P.zero, P.ni, s);
     if (e == hipSuccess)
-      e = hipMemcpyAsync(S->h_done, S->st.done, S->max_slots * sizeof(int), hipMemcpyDeviceToHost, s);
+      e = mixed ? serve_mixed_refresh(V, S->st, P.res, P.ni, S->d_active, s)
+                : serve_flags(V, S->st, S->d_active, s);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(S->h_done, fin_dev, S->max_slots * sizeof(int), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     S->steps++;
     if (e != hipSuccess) {
@@ -492,12 +537,27 @@ void loop_main(ssf_server *S)
       if (e2 == hipSuccess) e2 = hipMemcpy(&ares, S->st.ares + q->sid, sizeof(double), hipMemcpyDeviceToHost);
       if (e2 == hipSuccess && q->n)
         e2 = hipMemcpy(q->x.data(), q->vec + q->n, q->n * sizeof(double), hipMemcpyDeviceToHost);
-      q->info.status = status;
-      q->info.outer = 0;
-      q->info.iterations = status == 3 ? 0 : (status == 1 || status == 2 ? ani + 1 : iter);
-      q->info.ani = (status == 0) ? -1 : ani;
-      q->info.ares = ares;
-      q->info.rel_residual = -1.0;
+      if (mixed) {  // as the single-system mixed solve reports
+        long long upd = 0;
+        double rr = 0.0, yy = 0.0;
+        if (e2 == hipSuccess) e2 = hipMemcpy(&status, S->st.mstatus + q->sid, sizeof(int), hipMemcpyDeviceToHost);
+        if (e2 == hipSuccess) e2 = hipMemcpy(&upd, S->st.updates + q->sid, sizeof(long long), hipMemcpyDeviceToHost);
+        if (e2 == hipSuccess) e2 = hipMemcpy(&rr, S->st.rr64 + q->sid, sizeof(double), hipMemcpyDeviceToHost);
+        if (e2 == hipSuccess) e2 = hipMemcpy(&yy, S->st.nory64 + q->sid, sizeof(double), hipMemcpyDeviceToHost);
+        q->info.status = status;
+        q->info.outer = upd;
+        q->info.iterations = status == 3 ? 0 : iter;
+        q->info.ani = q->info.iterations;
+        q->info.ares = yy > 0.0 ? rr / yy : 0.0;
+        q->info.rel_residual = yy > 0.0 ? std::sqrt(rr / yy) : 0.0;
+      } else {
+        q->info.status = status;
+        q->info.outer = 0;
+        q->info.iterations = status == 3 ? 0 : (status == 1 || status == 2 ? ani + 1 : iter);
+        q->info.ani = (status == 0) ? -1 : ani;
+        q->info.ares = ares;
+        q->info.rel_residual = -1.0;
+      }
       S->free_sids.push_back(q->sid);
       release_device(q, s);
       finish(S, q, e2 == hipSuccess ? SSF_OK : SSF_EHIP,
@@ -527,10 +587,15 @@ hipError_t alloc_state(ssf_server *S, int cap)
   c(salloc(&t.iter, cap)); c(salloc(&t.ani, cap)); c(salloc(&t.done, cap));
   c(salloc(&t.status, cap)); c(salloc(&t.pend, cap)); c(salloc(&t.pend_status, cap));
   c(salloc(&t.brk, cap));
+  c(salloc(&t.nory64, cap)); c(salloc(&t.rr64, cap)); c(salloc(&t.best, cap));
+  c(salloc(&t.scale, cap)); c(salloc(&t.delta2, cap)); c(salloc(&t.xx, cap));
+  c(salloc(&t.bad, cap)); c(salloc(&t.since_best, cap)); c(salloc(&t.final, cap));
+  c(salloc(&t.mstatus, cap)); c(salloc(&t.updates, cap));
   c(salloc(&S->d_active, 1));
   if (e == hipSuccess) c(hipHostMalloc((void **)&S->h_done, cap * sizeof(int), hipHostMallocDefault));
   // unused slots read as finished until a request takes them
   if (e == hipSuccess) c(hipMemset(t.done, 0xff, cap * sizeof(int)));
+  if (e == hipSuccess) c(hipMemset(t.final, 0xff, cap * sizeof(int)));
   return e;
 }
 
@@ -540,6 +605,8 @@ void free_all(ssf_server *S)
   sfree(t.nory); sfree(t.nom); sfree(t.ares); sfree(t.sc);
   sfree(t.iter); sfree(t.ani); sfree(t.done);
   sfree(t.status); sfree(t.pend); sfree(t.pend_status); sfree(t.brk);
+  sfree(t.nory64); sfree(t.rr64); sfree(t.best); sfree(t.scale); sfree(t.delta2); sfree(t.xx);
+  sfree(t.bad); sfree(t.since_best); sfree(t.final); sfree(t.mstatus); sfree(t.updates);
   sfree(S->d_active);
   sfree(S->d_slots); sfree(S->d_blk_slot); sfree(S->d_blk_s0); sfree(S->d_blk_s1);
   sfree(S->d_part);
@@ -578,8 +645,8 @@ extern "C" int ssf_server_create(int device, int64_t budget_bytes, const ssf_sol
   SSF_CHECK(p && out && budget_bytes > 0, SSF_EINVAL, "ssf_server_create: bad argument");
   SSF_CHECK(p->linsolver == SSF_SOLVER_CG && p->prec == SSF_PREC_DIAG, SSF_EUNSUP,
             "ssf_server_create: served solves are Jacobi-PCG (cg + diagprec)");
-  SSF_CHECK(p->precision == SSF_PRECISION_FP64 && !p->iv, SSF_EUNSUP,
-            "ssf_server_create: served solves run the FP64 recurrence from x = 0");
+  SSF_CHECK((p->precision == SSF_PRECISION_FP64 || p->precision == SSF_PRECISION_MIXED) && !p->iv,
+            SSF_EUNSUP, "ssf_server_create: served solves start from x = 0 (FP64 or mixed)");
   ssf_server *S = new ssf_server();
   S->device = device;
   S->params = *p;

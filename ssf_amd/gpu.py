@@ -552,11 +552,12 @@ class Batch:
 
 class Server:
     """Request loop on one GPU: element-mesh solves admitted under an HBM budget
-    and solved with continuous batching (FP64 Jacobi-PCG per request)."""
+    and solved with continuous batching (Jacobi-PCG per request: the FP64
+    recurrence, or precision=MIXED: FP32 recurrences refreshed in FP64 every chunk)."""
 
     def __init__(self, device=0, budget_bytes=200 << 30, ni=100000, res=1e-20, zero=1e-30,
-                 chunk=32, max_slots=1024, prep_threads=2):
-        p = SolveParams(SOLVER_CG, PREC_DIAG, ni, res, zero, 0, FP64, 0.0, 0, 0)
+                 chunk=32, max_slots=1024, prep_threads=2, precision=FP64, inner_res=1e-4):
+        p = SolveParams(SOLVER_CG, PREC_DIAG, ni, res, zero, 0, precision, inner_res, 0, 0)
         self.h = _P()
         _check(lib().ssf_server_create(device, int(budget_bytes), ctypes.byref(p), chunk,
                                        max_slots, prep_threads, ctypes.byref(self.h)))

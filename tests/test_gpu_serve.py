@@ -118,3 +118,50 @@ def test_server_ni_and_zero_rhs():
         assert i0.status == 3 and i0.iterations == 0 and not np.any(x0)
     finally:
         srv.close()
+
+
+def test_server_mixed_matches_oracle_and_alone_equals_in_mix():
+    """The mixed-precision served form: every result within 1e-8 of the oracle's
+    FP64 cg_prec solution, and bitwise the same alone and among other requests."""
+    probs = requests()
+    target = probs[1]
+    kw = dict(budget_bytes=1 << 30, res=1e-22, chunk=16, precision=gpu.MIXED)
+    srv = gpu.Server(**kw)
+    try:
+        x_alone, i_alone, _ = srv.wait(srv.submit(target))
+    finally:
+        srv.close()
+    srv = gpu.Server(prep_threads=3, **kw)
+    try:
+        tickets = []
+        for k, p in enumerate(probs):
+            tickets.append(srv.submit(p))
+            time.sleep(0.01 * k)
+        tickets.append(srv.submit(target))
+        out = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+    finally:
+        srv.close()
+    for p, (x, info, _) in zip(probs, out):
+        adr, ci, a, rhs_o = O.assemble_problem(p, "scr")
+        xo, _, _, st_o = O.scr_cg_prec(adr, ci, a, rhs_o[0], 100000, 1e-30)
+        assert info.status in (1, 4) and info.outer >= 1, (p.name, info.status)
+        assert rel(x, xo) <= 1e-8, (p.name, rel(x, xo))
+    for j in (1, len(probs)):
+        x, info, _ = out[j]
+        assert np.array_equal(x, x_alone)
+        assert info.iterations == i_alone.iterations and info.outer == i_alone.outer
+    assert st["reserved_bytes"] == 0
+
+
+def test_server_mixed_ni_and_zero_rhs():
+    p = mesh.cantilever(8, 3, 3)
+    srv = gpu.Server(budget_bytes=1 << 30, res=1e-22, ni=20, chunk=16, precision=gpu.MIXED)
+    try:
+        x, info, _ = srv.wait(srv.submit(p))
+        assert info.status == 0 and info.iterations >= 20
+        z = mesh.plate(9, 5, seed=33, nrhs=1)
+        x0, i0, _ = srv.wait(srv.submit(z, rhs=np.zeros(z.n)))
+        assert i0.status == 3 and i0.iterations == 0 and not np.any(x0)
+    finally:
+        srv.close()
