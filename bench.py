@@ -757,7 +757,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nel", type=int, default=128, help="elements per cube edge")
     ap.add_argument("--res", type=float, default=1e-22, help="squared relative residual test")
-    ap.add_argument("--interval", type=int, default=32,
+    ap.add_argument("--interval", type=int, default=64,
                     help="FP32 iterations between FP64 refinements (mixed)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-iters", type=int, default=20)
@@ -767,9 +767,10 @@ def main():
     ap.add_argument("--batch-res-mixed", type=float, default=1e-22)
     ap.add_argument("--seq-stride", type=int, default=16)
     ap.add_argument("--only-batch", action="store_true")
-    ap.add_argument("--tiling64", type=int, nargs=4, default=None,
-                    help="pin the FP64 SpMV tiling (format lanes blocks idx16) instead of autotuning")
-    ap.add_argument("--tiling32", type=int, nargs=4, default=None,
+    ap.add_argument("--tiling64", type=int, nargs="+", default=None,
+                    help="pin the FP64 SpMV tiling (format lanes blocks idx16 [map]) instead of "
+                         "autotuning")
+    ap.add_argument("--tiling32", type=int, nargs="+", default=None,
                     help="pin the FP32 SpMV tiling instead of autotuning")
     ap.add_argument("--no-serve", action="store_true", help="skip the served leg")
     ap.add_argument("--only-serve", action="store_true")

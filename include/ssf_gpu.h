@@ -443,14 +443,17 @@ int ssf_elements_bench(ssf_elements *el, const double *lhs_dev, const double *pv
 int ssf_spmv_bench(ssf_matrix *m, int precision, int reps, double *usec);
 /* SpMV autotuner: time every tiling of the FP64 (precision 0) or FP32 (1) SpMV
  * on this matrix and keep the fastest for later solves. tune_out (may be NULL)
- * receives {format (0 row-vector, 1 stream tiles, 2 SELL-64 slices), lanes per
- * row (formats 0, 1) or value loads in flight per lane (format 2 with row
- * patterns: 2, 4 or 8), blocks, idx16 (column form: 0 int32, 1 16-bit tile or
- * slice offsets, 2 SELL row patterns, 3 SELL stencil form: one slot per column
- * offset in every row, offered when the matrix has at most 32 offsets)}. */
+ * receives 5 values {format (0 row-vector, 1 stream tiles, 2 SELL-64 slices),
+ * lanes per row (formats 0, 1) or value loads in flight per lane (format 2 with
+ * row patterns: 2, 4 or 8), blocks (at most 8192), idx16 (column form: 0 int32,
+ * 1 16-bit tile or slice offsets, 2 SELL row patterns, 3 SELL stencil form: one
+ * slot per column offset in every row, offered when the matrix has at most 32
+ * offsets), map (SELL slices to wavefronts: 0 a contiguous range per block, 1
+ * each of the 8 XCDs a contiguous eighth whose slices its wavefronts take in
+ * turn; the default)}. */
 int ssf_matrix_autotune(ssf_matrix *m, int precision, int reps, int32_t *tune_out);
-/* pin the SpMV tiling {format, lanes, blocks, idx16}; the result of a solve
- * depends only on the matrix, the right-hand side and the two tilings */
+/* pin the SpMV tiling {format, lanes, blocks, idx16, map} (5 values); the result
+ * of a solve depends only on the matrix, the right-hand side and the two tilings */
 int ssf_matrix_set_tune(ssf_matrix *m, int precision, const int32_t *tune);
 /* Time `reps` launches of a plain device copy of `bytes` bytes (read + write
  * 2*bytes of HBM per launch): the achievable-bandwidth reference of the

@@ -1276,7 +1276,7 @@ static int ensure_ws(ssf_matrix *m, bool mixed)
   if (w.n != n || w.nb < nb) {
     free_workspace(w, m->ctx->stream);
     w.n = n;
-    w.nb = std::max(nb, 4096);
+    w.nb = std::max(nb, 8192);
     SSF_HIP(dalloc(&w.x, n, m->ctx->stream)); SSF_HIP(dalloc(&w.r, n, m->ctx->stream)); SSF_HIP(dalloc(&w.d, n, m->ctx->stream));
     SSF_HIP(dalloc(&w.p, n, m->ctx->stream)); SSF_HIP(dalloc(&w.h, n, m->ctx->stream)); SSF_HIP(dalloc(&w.y, n, m->ctx->stream));
     SSF_HIP(dalloc(&w.part, 4 * (size_t)w.nb, m->ctx->stream));
@@ -1829,22 +1829,27 @@ extern "C" int ssf_matrix_autotune(ssf_matrix *m, int precision, int reps, int32
           t.idx16 = i16;
           cand.push_back(t);
         }
+  // SELL block counts: a solve's element-wise kernels re-reduce the SpMV's
+  // per-block partials in every block, so more SpMV blocks than 2048 cost more
+  // there than they save in the product
   if (m->nslice > 0)
-    for (int b : {512, 1024, 2048, 4096})
+    for (int b : {256, 512, 1024, 2048})
       for (int i16 = 0; i16 <= 3; i16++)
-        for (int dep : {2, 4, 8}) {  // row patterns: value loads in flight per lane
-          if (b > std::max(8, (m->nslice + 3) / 4)) continue;
-          if (i16 == 1 && !m->sell_col16) continue;
-          if (i16 == 2 && !m->sell_pid) continue;
-          if (i16 == 3 && m->sten.empty()) continue;
-          if (i16 != 2 && dep != 2) continue;
-          SpmvTune t;
-          t.format = 2;
-          t.lanes = dep;
-          t.blocks = b;
-          t.idx16 = i16;
-          cand.push_back(t);
-        }
+        for (int dep : {2, 4, 8})  // row patterns: value loads in flight per lane
+          for (int map : {0, 1}) {
+            if (b > std::max(8, (m->nslice + 3) / 4)) continue;
+            if (i16 == 1 && !m->sell_col16) continue;
+            if (i16 == 2 && !m->sell_pid) continue;
+            if (i16 == 3 && m->sten.empty()) continue;
+            if (i16 != 2 && dep != 2) continue;
+            SpmvTune t;
+            t.format = 2;
+            t.lanes = dep;
+            t.blocks = b;
+            t.idx16 = i16;
+            t.map = map;
+            cand.push_back(t);
+          }
   SpmvTune best = precision ? m->tune32 : m->tune64;
   double best_t = 1e300;
   for (const SpmvTune &t : cand) {
@@ -1864,6 +1869,7 @@ extern "C" int ssf_matrix_autotune(ssf_matrix *m, int precision, int reps, int32
     tune_out[1] = best.lanes;
     tune_out[2] = best.blocks;
     tune_out[3] = best.idx16;
+    tune_out[4] = best.map;
   }
   return SSF_OK;
 }
@@ -1877,6 +1883,8 @@ extern "C" int ssf_matrix_set_tune(ssf_matrix *m, int precision, const int32_t *
   t.lanes = tune[1];
   t.blocks = tune[2];
   t.idx16 = tune[3];
+  t.map = tune[4];
+  SSF_CHECK(t.map == 0 || t.map == 1, SSF_EINVAL, "ssf_matrix_set_tune: slice map must be 0 or 1");
   SSF_CHECK((t.format == 0 && m->rowptr) || (t.format == 1 && m->ntiles > 0) ||
                 (t.format == 2 && m->nslice > 0),
             SSF_EINVAL,
@@ -1891,7 +1899,7 @@ extern "C" int ssf_matrix_set_tune(ssf_matrix *m, int precision, const int32_t *
     SSF_CHECK(t.lanes >= 0 && t.lanes <= 8, SSF_EINVAL,
               "ssf_matrix_set_tune: SELL depth must be in [0, 8] (value loads in flight per "
               "lane of the row-pattern form: 8 or more, 4 or more, else 2)");
-  SSF_CHECK(t.blocks >= 1 && t.blocks <= 4096, SSF_EINVAL, "ssf_matrix_set_tune: blocks out of [1, 4096]");
+  SSF_CHECK(t.blocks >= 1 && t.blocks <= 8192, SSF_EINVAL, "ssf_matrix_set_tune: blocks out of [1, 8192]");
   SSF_CHECK(t.idx16 == 0 || (t.idx16 == 1 && precision && t.format == 1 && m->col16) ||
                 (t.idx16 == 1 && t.format == 2 && m->sell_col16) ||
                 (t.idx16 == 2 && t.format == 2 && m->sell_pid) ||

@@ -125,7 +125,9 @@ struct SpmvTune {
   int blocks = 1024; // blocks of 256 threads (rows / tiles split in contiguous chunks)
   int idx16 = 0;     // column index form: 0 int32; 1 16-bit (stream tile- / SELL
                      // slice-relative); 2 SELL row patterns (a pattern id per row
-                     // and a shared dictionary of column offsets)
+                     // and a shared dictionary of column offsets); 3 SELL stencil form
+  int map = 1;       // SELL slices to waves: 0 a contiguous chunk per block; 1 XCD
+                     // interleave (each XCD a contiguous eighth, its waves in turn)
 };
 
 // Device view of a full CSR matrix and its stream tiling.
@@ -157,6 +159,7 @@ struct CsrDev {
   int sten[SELL_STEN_MAX] = {};
   const double *sell_v64 = nullptr;
   const float *sell_v32 = nullptr;
+  int slice_map = 1;  // SpmvTune::map of the launch
 };
 
 constexpr int SELL_V = 4;  // entries per lane and load (16-byte FP32, 32-byte FP64 loads)

@@ -136,6 +136,34 @@ __global__ __launch_bounds__(TPB) void k_spmv_stream(CsrDev A, const T *__restri
 
 // ---------------------------------------------------------------- SELL SpMV
 
+// The 64-row slices a wave streams: first, first + step, ... below last.
+// map 0: each block a contiguous range of slices, its waves in turn (blocks in
+// XCD-contiguous order). map 1 (XCD interleave): workgroups are dispatched
+// round-robin to the 8 XCDs; XCD c keeps the contiguous eighth c of the slices
+// and all of its waves take them in turn, so the concurrent waves of an XCD
+// stream one window of the value array (HBM pages and prefetch stay together)
+// while the x entries of neighbouring rows stay in that XCD's L2.
+__device__ __forceinline__ void wave_slices(int nslice, int map, int wave, int &first, int &last,
+                                            int &step)
+{
+  const int nb = gridDim.x;
+  constexpr int W = TPB / 64;
+  if (map == 1 && (nb & 7) == 0) {
+    const int xcd = blockIdx.x & 7, lb = blockIdx.x >> 3, nbx = nb >> 3;
+    const int spx = (nslice + 7) / 8;
+    const int x0 = min(nslice, xcd * spx);
+    first = x0 + lb * W + wave;
+    last = min(nslice, x0 + spx);
+    step = nbx * W;
+  } else {
+    const int spb = (nslice + nb - 1) / nb;
+    const int s0 = xcd_block(blockIdx.x, nb) * spb;
+    first = s0 + wave;
+    last = min(nslice, s0 + spb);
+    step = W;
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ const T *sell_values(const CsrDev &A);
 template <>
@@ -166,13 +194,11 @@ __global__ __launch_bounds__(TPB) void k_spmv_sell(CsrDev A, const T *__restrict
   if (done && *done) return;
   if (iter && blockIdx.x == 0 && threadIdx.x == 0) *iter += 1;
   const T *__restrict__ sval = sell_values<T>(A);
-  const int nb = gridDim.x;
-  const int spb = (A.nslice + nb - 1) / nb;
-  const int lb = xcd_block(blockIdx.x, nb);
-  const int s0 = lb * spb, s1 = min(A.nslice, s0 + spb);
-  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), lane = threadIdx.x % 64;
+  int sfirst, slast, sstep;
+  wave_slices(A.nslice, A.slice_map, wave, sfirst, slast, sstep);
   T acc = T(0);
-  for (int sl = s0 + wave; sl < s1; sl += TPB / 64) {
+  for (int sl = sfirst; sl < slast; sl += sstep) {
     const int off = A.sell_off[sl];
     const int w = (A.sell_off[sl + 1] - off) / 64;  // a multiple of SELL_V
     T sum = T(0);
@@ -269,16 +295,13 @@ __global__ __launch_bounds__(TPB) void k_spmv_sell_pat(CsrDev A, const T *__rest
   if (done && *done) return;
   if (iter && blockIdx.x == 0 && threadIdx.x == 0) *iter += 1;
   const T *__restrict__ sval = sell_values<T>(A);
-  const int nb = gridDim.x;
-  const int spb = (A.nslice + nb - 1) / nb;
-  const int lb = xcd_block(blockIdx.x, nb);
-  const int s0 = lb * spb, s1 = min(A.nslice, s0 + spb);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), lane = threadIdx.x % 64;
+  int s0w, s1, ST;
+  wave_slices(A.nslice, A.slice_map, wave, s0w, s1, ST);
   T acc = T(0);
   // slice bounds one slice ahead, pattern offsets one and pattern ids two
   // slices ahead of the one being summed
-  const int ST = TPB / 64;
-  int sl = s0 + wave;
+  int sl = s0w;
   int offc = 0, endc = 0, dofc = 0, pid1 = 0;
   if (sl < s1) {
     offc = A.sell_off[sl];
@@ -331,14 +354,12 @@ __global__ __launch_bounds__(TPB) void k_spmv_sell_sten(CsrDev A, const T *__res
   if (done && *done) return;
   if (iter && blockIdx.x == 0 && threadIdx.x == 0) *iter += 1;
   const T *__restrict__ sval = sell_values<T>(A);
-  const int nb = gridDim.x;
-  const int spb = (A.nslice + nb - 1) / nb;
-  const int lb = xcd_block(blockIdx.x, nb);
-  const int s0 = lb * spb, s1 = min(A.nslice, s0 + spb);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), lane = threadIdx.x % 64;
+  int sfirst, slast, sstep;
+  wave_slices(A.nslice, A.slice_map, wave, sfirst, slast, sstep);
   const int nm1 = A.n - 1;
   T acc = T(0);
-  for (int sl = s0 + wave; sl < s1; sl += TPB / 64) {
+  for (int sl = sfirst; sl < slast; sl += sstep) {
     const unsigned vb = (unsigned)(sl * 64 * NGRP * SELL_V + lane * SELL_V) * sizeof(T);
     const int rb = sl * 64 + lane;
     tv v[NGRP];
@@ -368,11 +389,13 @@ __global__ __launch_bounds__(TPB) void k_spmv_sell_sten(CsrDev A, const T *__res
 // ev0/ev1 (both or neither): the launch's own start and end timestamps
 // (hipExtLaunchKernelGGL), i.e. the kernel's duration without the dispatch gap
 template <typename T, int KIND>
-static hipError_t spmv_k(const SpmvTune &t, const CsrDev &A, const T *val, const T *x, T *y,
+static hipError_t spmv_k(const SpmvTune &t, const CsrDev &A0, const T *val, const T *x, T *y,
                          const T *b, T *partial, const int *done, long long *iter,
                          hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
 {
   dim3 grid(t.blocks), block(TPB);
+  CsrDev A = A0;
+  A.slice_map = t.map;
 #define SSF_LAUNCH(KERN, ...)                                                      \
   do {                                                                             \
     if (ev0) hipExtLaunchKernelGGL(KERN, grid, block, 0, s, ev0, ev1, 0, __VA_ARGS__); \

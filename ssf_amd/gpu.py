@@ -377,13 +377,18 @@ class Matrix:
         return a.value, b.value
 
     def autotune(self, precision=FP64, reps=5):
-        """Returns the chosen SpMV tiling (format, lanes, blocks, idx16)."""
-        out = np.zeros(4, np.int32)
+        """Returns the chosen SpMV tiling (format, lanes, blocks, idx16, map)."""
+        out = np.zeros(5, np.int32)
         _check(lib().ssf_matrix_autotune(self.h, precision, reps, _p(out)))
         return tuple(int(v) for v in out)
 
     def set_tune(self, precision, tune):
-        t = np.ascontiguousarray(tune, dtype=np.int32)
+        """Pins the SpMV tiling (format, lanes, blocks, idx16[, map]); map defaults
+        to 1 (XCD interleave of the SELL slices)."""
+        t = list(tune)
+        if len(t) == 4:
+            t.append(1)
+        t = np.ascontiguousarray(t, dtype=np.int32)
         _check(lib().ssf_matrix_set_tune(self.h, precision, _p(t)))
 
 

@@ -288,3 +288,34 @@ def test_autotune_keeps_parity(ctx):
     for precision in (gpu.FP64, gpu.MIXED):
         x, info = m.solve(rhs[0], ni=50000, res=1e-24, precision=precision)
         assert rel(x, xo) <= 1e-8, (precision, t64, t32)
+
+
+def test_sell_slice_maps(ctx):
+    """Both maps of SELL slices to wavefronts (0: a contiguous range per block;
+    1: each XCD a contiguous eighth taken in turn by its wavefronts) give the same
+    products in every index form, for block counts that do and do not divide
+    among the 8 XCDs, and solves within tolerance of the oracle."""
+    for prob in (mesh.heat_cube(17, seed=9), mesh.elastic_column(5, 6, 7, seed=10)):
+        adr, ci, a, rhs = O.assemble_problem(prob, "scr")
+        m = gpu.Matrix.from_csr(ctx, gpu.STORAGE_SCR, adr, ci)
+        m.set_values(a)
+        b = np.random.default_rng(7).standard_normal(prob.n)
+        m.set_tune(gpu.FP64, (2, 0, 64, 0, 0))
+        y0 = m.gmxv(b)
+        forms = [0, 1, 2] + ([3] if prob.kind == mesh.KIND_HEX8_HEAT else [])
+        for i16 in forms:
+            for nb in (3, 8, 64, 512):
+                for mp in (0, 1):
+                    m.set_tune(gpu.FP64, (2, 4, nb, i16, mp))
+                    assert np.array_equal(m.gmxv(b), y0), (prob.name, i16, nb, mp)
+        xo, *_ = O.scr_cg_prec(adr, ci, a, rhs[0], 50000, 1e-30)
+        for mp in (0, 1):
+            t = (2, 4, 64, forms[-1], mp)
+            m.set_tune(gpu.FP64, t)
+            m.set_tune(gpu.MIXED, t)
+            for precision in (gpu.FP64, gpu.MIXED):
+                x, info = m.solve(rhs[0], ni=50000, res=1e-24, precision=precision)
+                assert info.status in (1, 2, 4) and rel(x, xo) <= 1e-8, (prob.name, mp, precision)
+        with pytest.raises(gpu.SsfError):
+            m.set_tune(gpu.FP64, (2, 0, 64, 0, 2))
+        m.close()
