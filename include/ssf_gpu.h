@@ -212,6 +212,19 @@ int ssf_solve(ssf_matrix *m, const ssf_solve_params *p, const double *rhs, doubl
 /* same with device vectors (rhs_dev read-only, lhs_dev in/out), synchronous */
 int ssf_solve_dev(ssf_matrix *m, const ssf_solve_params *p, const double *rhs_dev,
                   double *lhs_dev, ssf_solve_info *info);
+/* ssf_solve_dev without any allocation. No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid it). The FP64 cg solver only
+ * (symcomprow with diagprec or noprecond, comprow with noprecond), with the
+ * results of ssf_solve_dev bit for bit, for n <= SSF_NOALLOC_MAX_N (SSF_EINVAL
+ * beyond). Its working vectors, reduction partials and recurrence scalars live in
+ * one device array of the library (4 SSF_NOALLOC_MAX_N + 32784 doubles, about
+ * 80 MB), which each GPU the library is loaded on holds until the process ends.
+ * Every call on a GPU shares that GPU's array: calls that overlap on one GPU race.
+ * Synchronous; no hipMalloc, hipFree or memset. */
+#define SSF_NOALLOC_MAX_N 2500000
+int ssf_solve_dev_noalloc(ssf_matrix *m, const ssf_solve_params *p, const double *rhs_dev,
+                          double *lhs_dev, ssf_solve_info *info);
 /* symcomprow::incomplete_ldl (GEFEL/scr.cpp:990-1022) with weight 0, as gmatrix
  * runs it (GEFEL/gmatrix.cpp:24,1237): the factor incdec[negm] of the current
  * values in the reference layout (L off the diagonal, D on it), copied to the

@@ -86,6 +86,8 @@ SIGNATURES = {
     "ssf_gmxv_dev": (_I, [_P, _P, _P]),
     "ssf_solve": (_I, [_P, ctypes.POINTER(SolveParams), _P, _P, ctypes.POINTER(SolveInfo)]),
     "ssf_solve_dev": (_I, [_P, ctypes.POINTER(SolveParams), _P, _P, ctypes.POINTER(SolveInfo)]),
+    "ssf_solve_dev_noalloc": (_I, [_P, ctypes.POINTER(SolveParams), _P, _P,
+                                   ctypes.POINTER(SolveInfo)]),
     "ssf_spmv_bench": (_I, [_P, _I, _I, ctypes.POINTER(_D)]),
     "ssf_spmv_bytes": (_I, [_P, _I, ctypes.POINTER(_D), ctypes.POINTER(_D)]),
     "ssf_copy_bench": (_I, [_P, _L, _I, ctypes.POINTER(_D)]),
@@ -364,6 +366,16 @@ class Matrix:
         info = SolveInfo(-1, -1.0, 0, 0, 0, 0, 0.0)
         _check(lib().ssf_solve_dev(self.h, ctypes.byref(p), ctypes.c_void_p(rhs_ptr),
                                    ctypes.c_void_p(x_ptr), ctypes.byref(info)))
+        return info
+
+    def solve_dev_noalloc(self, rhs_ptr, x_ptr, ni=100000, res=1e-20, prec=PREC_DIAG, zero=1e-30,
+                          iv=0):
+        """ssf_solve_dev_noalloc: the FP64 cg solve with the library's static scratch
+        array instead of any allocation (n <= 2 500 000)."""
+        p = SolveParams(SOLVER_CG, prec, ni, res, zero, iv, FP64, 0.0, 0, 0)
+        info = SolveInfo(-1, -1.0, 0, 0, 0, 0, 0.0)
+        _check(lib().ssf_solve_dev_noalloc(self.h, ctypes.byref(p), ctypes.c_void_p(rhs_ptr),
+                                           ctypes.c_void_p(x_ptr), ctypes.byref(info)))
         return info
 
     def spmv_bench(self, precision=FP64, reps=20):
