@@ -67,7 +67,29 @@ def build(verbose: bool = False, incremental: bool = False, jobs: int = 8) -> st
     return LIB
 
 
+
+
+DRIVER = os.path.join(ROOT, "tests", "cxx", "gmatrix_driver")
+
+
+def build_driver(verbose: bool = False) -> str:
+    """The host C++ caller of the boundary (tests/cxx/gmatrix_driver.cpp with the
+    gmatrix_gpu mirror of ssf_amd/host), linked against libssf_gpu.so."""
+    cxx = os.environ.get("CXX", "g++")
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+           "-I", os.path.join(HERE, "host"), os.path.join(HERE, "host", "gmatrix_gpu.cpp"),
+           os.path.join(ROOT, "tests", "cxx", "gmatrix_driver.cpp"), "-o", DRIVER + ".tmp",
+           "-L", HERE, "-lssf_gpu", "-Wl,-rpath,$ORIGIN/../../ssf_amd",
+           "-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
+
+
 if __name__ == "__main__":
     build(verbose="-v" in sys.argv, incremental="-i" in sys.argv)
+    build_driver(verbose="-v" in sys.argv)
     print("built", LIB)
     sys.exit(0)

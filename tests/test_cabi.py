@@ -37,3 +37,24 @@ def test_library_loads_and_reports_errors():
     h = ctypes.c_void_p()
     rc = l.ssf_matrix_create_csr(None, 11, 0, None, None, ctypes.byref(h))
     assert rc == -1 and b"NULL" in l.ssf_last_error()
+
+
+def _driver():
+    from ssf_amd import build
+    if not os.path.exists(build.DRIVER):
+        build.build_driver()
+    return build.DRIVER
+
+
+def test_struct_layouts_agree_with_the_compiled_caller():
+    """The C++ caller (compiled against include/ssf_gpu.h, whose header also holds
+    static_asserts of the layouts) and the ctypes structs of ssf_amd/gpu.py see
+    the same sizes and field offsets."""
+    import json
+    out = subprocess.run([_driver(), "--layout"], capture_output=True, text=True, check=True)
+    lay = json.loads(out.stdout)
+    for name, cls in (("ssf_solve_params", gpu.SolveParams), ("ssf_solve_info", gpu.SolveInfo)):
+        import ctypes
+        assert lay[name]["size"] == ctypes.sizeof(cls), name
+        for field, _ in cls._fields_:
+            assert lay[name][field] == getattr(cls, field).offset, (name, field)
