@@ -338,13 +338,13 @@ def bench_served(ctx, args, gpu, mesh, torch):
 
 
 def bench_assembly(ctx, args, gpu, mesh, torch=None):
-    """Element kernels + coloured assembly of the 64x64x256 hex8 elasticity column
-    (1 048 576 elements, 3 244 800 unknowns): device time of the element kernel
-    launches (HIP events around ssf_matrix_assemble's kernels are not exposed, so
-    the whole call is timed after a warm-up that builds the slot map)."""
+    """Element kernels + assembly by entry of the 64x64x256 hex8 elasticity column
+    (1 048 576 elements, 3 244 800 unknowns): the whole ssf_matrix_assemble call
+    (host uploads of the element arrays, element kernel, per-entry gather, value
+    expansion) after a warm-up call that builds the assembly plan."""
     prob = mesh.elastic_column(args.asm_nel, args.asm_nel, 4 * args.asm_nel, seed=3)
     m = gpu.Matrix.from_elements(ctx, gpu.STORAGE_SCR, prob.n, prob.elem_codes())
-    m.assemble(prob, lift=False)  # slot map and colouring
+    m.assemble(prob, lift=False)  # builds the assembly plan
     ts = []
     for _ in range(args.asm_reps):
         m.zero()

@@ -20,7 +20,7 @@ LIB = os.path.join(HERE, "libssf_gpu.so")
 SOURCES = ["ssf_kernels.hip", "ssf_solver.hip", "ssf_elast.hip", "ssf_api.hip", "ssf_batch.hip",
            "ssf_batch_api.hip", "ssf_serve.hip", "ssf_server.hip", "ssf_layout.hip",
            "ssf_post.hip", "ssf_pattern.hip", "ssf_bicg.hip", "ssf_ildl.hip", "ssf_nr.hip",
-           "ssf_renumber.hip", "ssf_codenum.hip", "ssf_noalloc.hip"]
+           "ssf_renumber.hip", "ssf_codenum.hip", "ssf_noalloc.hip", "ssf_assemble.hip"]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # packed FP32 instructions are disabled: hipcc forms them itself from float code

@@ -460,7 +460,7 @@ extern "C" int ssf_matrix_destroy(ssf_matrix *m)
   dfree(m->rowptr, m->ctx->stream); dfree(m->col, m->ctx->stream); dfree(m->val, m->ctx->stream); dfree(m->val32, m->ctx->stream);
   dfree(m->diag, m->ctx->stream); dfree(m->diag32, m->ctx->stream); dfree(m->norm_a, m->ctx->stream);
   dfree(m->map_lo, m->ctx->stream); dfree(m->map_up, m->ctx->stream);
-  dfree(m->slot, m->ctx->stream); dfree(m->order, m->ctx->stream); dfree(m->asm_dup, m->ctx->stream);
+  dfree(m->asm_cstart, m->ctx->stream); dfree(m->asm_cidx, m->ctx->stream);
   dfree(m->tiles, m->ctx->stream); dfree(m->col16, m->ctx->stream);
   dfree(m->sell_off, m->ctx->stream); dfree(m->sell_col, m->ctx->stream); dfree(m->sell_src, m->ctx->stream); dfree(m->sell_v64, m->ctx->stream); dfree(m->sell_v32, m->ctx->stream);
   dfree(m->sell_col16, m->ctx->stream);
@@ -479,12 +479,11 @@ extern "C" int ssf_matrix_destroy(ssf_matrix *m)
 void ssf::matrix_drop_assembly(ssf_matrix *m, bool host_pattern)
 {
   (void)hipStreamSynchronize(m->ctx->stream);
-  dfree(m->slot, m->ctx->stream);
-  dfree(m->order, m->ctx->stream);
-  dfree(m->asm_dup, m->ctx->stream);
+  dfree(m->asm_cstart, m->ctx->stream);
+  dfree(m->asm_cidx, m->ctx->stream);
   m->asm_ne = 0;
   m->asm_ndofe = 0;
-  m->colour_start.clear();
+  m->asm_nc = 0;
   m->asm_cn_host.clear();
   m->asm_cn_host.shrink_to_fit();
   if (host_pattern) {  // the matrix can then only be solved and multiplied
@@ -1022,63 +1021,15 @@ extern "C" int ssf_matrix_get_values(const ssf_matrix *m, double *a)
 
 // ----------------------------------------------------------------- assembly
 
-// Greedy first-fit colouring of the elements by their unknowns: elements of one
-// colour share no positive code number, hence no matrix entry and no entry of
-// the lift, so one colour is assembled without atomics. Code numbers <= 0 are
-// skipped by localize and the lift target, so they are no shared resource.
-// Coupled DOFs (one code number on several nodes) are covered because the
-// resource is the code number, not the node. Colours are taken 64 at a time:
-// elements that find none free in a window go to the next one, so heavily
-// shared unknowns (a DOF coupled over a whole face) only add colours.
-static int colour_elements(int64_t n, int64_t ne, int nd, const int64_t *cn,
-                           std::vector<int> &order, std::vector<int64_t> &start)
-{
-  std::vector<uint64_t> used(n + 1, 0);
-  std::vector<int> colour(ne, -1);
-  std::vector<int64_t> todo(ne), next;
-  for (int64_t e = 0; e < ne; e++) todo[e] = e;
-  int ncol = 0;
-  for (int win = 0; !todo.empty(); win++) {
-    SSF_CHECK(win < (1 << 16) / 64, SSF_EUNSUP, "element colouring needs more than 65536 colours");
-    if (win) std::fill(used.begin(), used.end(), 0);
-    next.clear();
-    for (int64_t e : todo) {
-      uint64_t m = 0;
-      for (int a = 0; a < nd; a++) {
-        const int64_t c = cn[e * nd + a];
-        if (c > 0) m |= used[c];
-      }
-      if (~m == 0) {
-        next.push_back(e);
-        continue;
-      }
-      const int c = __builtin_ctzll(~m);
-      colour[e] = 64 * win + c;
-      ncol = std::max(ncol, 64 * win + c + 1);
-      for (int a = 0; a < nd; a++) {
-        const int64_t k = cn[e * nd + a];
-        if (k > 0) used[k] |= (1ULL << c);
-      }
-    }
-    todo.swap(next);
-  }
-  start.assign(ncol + 1, 0);
-  for (int64_t e = 0; e < ne; e++) start[colour[e] + 1]++;
-  for (int c = 0; c < ncol; c++) start[c + 1] += start[c];
-  std::vector<int64_t> fillp(start.begin(), start.end() - 1);
-  order.resize(ne);
-  for (int64_t e = 0; e < ne; e++) order[fillp[colour[e]]++] = (int)e;
-  return SSF_OK;
-}
-
-// element set + slot map of the assembly (rebuilt when the code numbers change)
-static int prepare_assembly(ssf_matrix *m, int64_t nn, int64_t ne, int nne, int nd,
-                            const int64_t *conn, const int64_t *cn, bool aos,
-                            int **d_cn_out)
+// The assembly plan of a code-number table (rebuilt when the code numbers
+// change): the contributions of every element-matrix entry to the reference
+// layout, listed per entry in the reference's localize order (ssf_assemble.hip).
+// d_cn_out receives the table as int32 on the device (the caller frees it).
+static int prepare_assembly(ssf_matrix *m, int64_t ne, int nd, const int64_t *cn, int **d_cn_out)
 {
   hipStream_t s = m->ctx->stream;
   const int64_t ncn = ne * nd;
-  bool same = m->slot && m->asm_ne == ne && m->asm_ndofe == nd &&
+  bool same = m->asm_cstart && m->asm_ne == ne && m->asm_ndofe == nd &&
               (int64_t)m->asm_cn_host.size() == ncn &&
               std::equal(m->asm_cn_host.begin(), m->asm_cn_host.end(), cn);
   std::vector<int> cn32(ncn);
@@ -1098,73 +1049,39 @@ static int prepare_assembly(ssf_matrix *m, int64_t nn, int64_t ne, int nne, int 
     SSF_HIP(hipStreamSynchronize(s));  // cn32 is released on return
     return SSF_OK;
   }
-  std::vector<int> order;
-  std::vector<int64_t> start;
-  int rc = colour_elements(m->n, ne, nd, cn, order, start);
+  dfree(m->asm_cstart, s);
+  dfree(m->asm_cidx, s);
+  m->asm_ne = 0;
+  const int *d_adr = m->d_adr32, *d_ci = m->d_ci32;  // the device pattern, if kept
+  int *t_adr = nullptr, *t_ci = nullptr;
+  if (!d_adr) {  // reference-layout pattern for the entry search
+    std::vector<int> adr32(m->n + 1), ci32(m->negm);
+    parallel_for(m->n + 1, [&](int64_t b, int64_t e) {
+      for (int64_t i = b; i < e; i++) adr32[i] = (int)m->adr[i];
+    });
+    parallel_for(m->negm, [&](int64_t b, int64_t e) {
+      for (int64_t k = b; k < e; k++) ci32[k] = (int)m->ci[k];
+    });
+    SSF_HIP(dalloc(&t_adr, m->n + 1, s));
+    SSF_HIP(dalloc(&t_ci, m->negm, s));
+    SSF_HIP(hipMemcpyAsync(t_adr, adr32.data(), (m->n + 1) * sizeof(int), hipMemcpyHostToDevice, s));
+    if (m->negm)
+      SSF_HIP(hipMemcpyAsync(t_ci, ci32.data(), m->negm * sizeof(int), hipMemcpyHostToDevice, s));
+    SSF_HIP(hipStreamSynchronize(s));  // the host copies go out of scope
+    d_adr = t_adr;
+    d_ci = t_ci;
+  }
+  int rc = asm_plan_build(m->storage, m->n, m->negm, ne, nd, d_cn, d_adr, d_ci, s,
+                          &m->asm_cstart, &m->asm_cidx, &m->asm_nc);
+  dfree(t_adr, s);
+  dfree(t_ci, s);
   if (rc) {
     dfree(d_cn, s);
     *d_cn_out = nullptr;
     return rc;
   }
-  // elements in which one positive code number occurs twice (coupled DOFs of
-  // two of its nodes): their entries can meet in one slot, so the wavefront
-  // kernels add them one by one in the reference's order
-  std::vector<unsigned char> dup(ne, 0);
-  std::atomic<int> ndup{0};
-  parallel_for(ne, [&](int64_t b, int64_t e) {
-    std::vector<int64_t> c(nd);
-    for (int64_t el = b; el < e; el++) {
-      int k = 0;
-      for (int a = 0; a < nd; a++)
-        if (cn[el * nd + a] > 0) c[k++] = cn[el * nd + a];
-      std::sort(c.begin(), c.begin() + k);
-      if (std::adjacent_find(c.begin(), c.begin() + k) != c.begin() + k) {
-        dup[el] = 1;
-        ndup++;
-      }
-    }
-  });
-  dfree(m->asm_dup, s);
-  if (ndup.load()) {
-    SSF_HIP(dalloc(&m->asm_dup, ne, s));
-    SSF_HIP(hipMemcpyAsync(m->asm_dup, dup.data(), ne, hipMemcpyHostToDevice, s));
-  }
-  dfree(m->slot, s);
-  dfree(m->order, s);
-  SSF_HIP(dalloc(&m->order, ne, s));
-  SSF_HIP(hipMemcpyAsync(m->order, order.data(), ne * sizeof(int), hipMemcpyHostToDevice, s));
-  SSF_HIP(dalloc(&m->slot, ne * nd * nd, s));
-  if (m->sell_only) {  // the device pattern is at hand
-    SSF_HIP(build_slots(m->storage, aos ? 1 : 0, ne, nd, m->order, d_cn, m->d_adr32, m->d_ci32,
-                        m->slot, s));
-    SSF_HIP(hipStreamSynchronize(s));
-    m->asm_ne = ne;
-    m->asm_ndofe = nd;
-    m->colour_start = start;
-    m->asm_cn_host.assign(cn, cn + ncn);
-    return SSF_OK;
-  }
-  // reference-layout pattern for the search
-  std::vector<int> adr32(m->n + 1), ci32(m->negm);
-  parallel_for(m->n + 1, [&](int64_t b, int64_t e) {
-    for (int64_t i = b; i < e; i++) adr32[i] = (int)m->adr[i];
-  });
-  parallel_for(m->negm, [&](int64_t b, int64_t e) {
-    for (int64_t k = b; k < e; k++) ci32[k] = (int)m->ci[k];
-  });
-  int *d_adr = nullptr, *d_ci = nullptr;
-  SSF_HIP(dalloc(&d_adr, m->n + 1, s));
-  SSF_HIP(dalloc(&d_ci, m->negm, s));
-  SSF_HIP(hipMemcpyAsync(d_adr, adr32.data(), (m->n + 1) * sizeof(int), hipMemcpyHostToDevice, s));
-  if (m->negm)
-    SSF_HIP(hipMemcpyAsync(d_ci, ci32.data(), m->negm * sizeof(int), hipMemcpyHostToDevice, s));
-  SSF_HIP(build_slots(m->storage, aos ? 1 : 0, ne, nd, m->order, d_cn, d_adr, d_ci, m->slot, s));
-  SSF_HIP(hipStreamSynchronize(s));
-  dfree(d_adr, s);
-  dfree(d_ci, s);
   m->asm_ne = ne;
   m->asm_ndofe = nd;
-  m->colour_start = start;
   m->asm_cn_host.assign(cn, cn + ncn);
   return SSF_OK;
 }
@@ -1183,7 +1100,8 @@ extern "C" int ssf_matrix_assemble(ssf_matrix *m, int kind, int64_t nn, const do
   SSF_CHECK(ne >= 0 && nn >= 0 && ne < (1LL << 31) && nn < (1LL << 31), SSF_EINVAL, "ssf_matrix_assemble: bad sizes");
   const int nne = kind == SSF_ELEM_QUAD4_PSTRESS ? 4 : 8;
   const int nd = kind == SSF_ELEM_HEX8_ELAST ? 24 : 8;
-  SSF_CHECK((int64_t)ne * nd * nd < (1LL << 31) * 4, SSF_EUNSUP, "ssf_matrix_assemble: slot map too large");
+  SSF_CHECK((int64_t)ne * nd * nd < (1LL << 31), SSF_EUNSUP,
+            "ssf_matrix_assemble: more than 2^31 element-matrix entries");
   {
     std::atomic<int> bad{0};
     parallel_for(ne * nne, [&](int64_t b, int64_t e) {
@@ -1198,16 +1116,28 @@ extern "C" int ssf_matrix_assemble(ssf_matrix *m, int kind, int64_t nn, const do
   SSF_HIP(hipSetDevice(m->ctx->device));
   hipStream_t s = m->ctx->stream;
   int *d_cn = nullptr;
-  int rc = prepare_assembly(m, nn, ne, nne, nd, conn, cn, kind == SSF_ELEM_HEX8_ELAST, &d_cn);
+  int rc = prepare_assembly(m, ne, nd, cn, &d_cn);
   if (rc) return rc;
+  // the lift: elements with prescribed values and their rows, per unknown
+  std::vector<int> pidx, lstart, lidx;
+  int64_t npe = 0;
+  if (rhs_lift) lift_plan_host(m->n, ne, nd, cn, pidx, lstart, lidx, &npe);
   std::vector<int> conn32(ne * nne);
   parallel_for(ne * nne, [&](int64_t b, int64_t e) {
     for (int64_t k = b; k < e; k++) conn32[k] = (int)conn[k];
   });
-  int *d_conn = nullptr;
+  int *d_conn = nullptr, *d_pidx = nullptr, *d_lstart = nullptr, *d_lidx = nullptr;
   double *d_xyz = nullptr, *d_a = nullptr, *d_b = nullptr, *d_t = nullptr, *d_pv = nullptr, *d_lift = nullptr;
+  double *d_kbuf = nullptr, *d_fbuf = nullptr;
   hipError_t e = hipSuccess;
   auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
+  chk(dalloc(&d_kbuf, ne * (nd * (nd + 1) / 2), s));
+  if (rhs_lift) {
+    chk(dalloc(&d_pidx, ne, s));
+    chk(dalloc(&d_lstart, m->n + 1, s));
+    chk(dalloc(&d_lidx, lidx.size(), s));
+    chk(dalloc(&d_fbuf, npe * nd, s));
+  }
   chk(dalloc(&d_conn, ne * nne, s));
   chk(dalloc(&d_xyz, nn * 3, s));
   chk(dalloc(&d_a, ne, s));
@@ -1223,11 +1153,21 @@ extern "C" int ssf_matrix_assemble(ssf_matrix *m, int kind, int64_t nn, const do
     if (d_t) chk(hipMemcpyAsync(d_t, thick, ne * sizeof(double), hipMemcpyHostToDevice, s));
     if (d_pv) chk(hipMemcpyAsync(d_pv, pv, npv * sizeof(double), hipMemcpyHostToDevice, s));
     if (d_lift) chk(hipMemsetAsync(d_lift, 0, m->n * sizeof(double), s));
+    if (rhs_lift) {
+      chk(hipMemcpyAsync(d_pidx, pidx.data(), ne * sizeof(int), hipMemcpyHostToDevice, s));
+      chk(hipMemcpyAsync(d_lstart, lstart.data(), (m->n + 1) * sizeof(int), hipMemcpyHostToDevice, s));
+      if (!lidx.empty())
+        chk(hipMemcpyAsync(d_lidx, lidx.data(), lidx.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    }
   }
-  for (size_t c = 0; e == hipSuccess && c + 1 < m->colour_start.size(); c++)
-    chk(assemble_elements(kind, m->colour_start[c], m->colour_start[c + 1], ne, m->order, d_conn,
-                          d_cn, d_xyz, d_a, d_b, d_t, m->slot, m->asm_dup, m->a_ref, d_pv, d_lift,
-                          s));
+  // element matrices (and lift values) of every element, then the sums per entry
+  // (and per unknown) in the reference's order
+  if (e == hipSuccess)
+    chk(assemble_elements(kind, ne, d_conn, d_cn, d_xyz, d_a, d_b, d_t, d_kbuf, d_pidx, d_pv,
+                          d_fbuf, s));
+  if (e == hipSuccess)
+    chk(asm_gather(m->negm, nd, true, m->asm_cstart, m->asm_cidx, d_kbuf, m->a_ref, s));
+  if (e == hipSuccess && rhs_lift) chk(lift_gather(m->n, d_lstart, d_lidx, d_fbuf, d_lift, s));
   if (e == hipSuccess) {
     int r2 = values_changed(m);
     if (r2) e = hipErrorUnknown;
@@ -1236,7 +1176,8 @@ extern "C" int ssf_matrix_assemble(ssf_matrix *m, int kind, int64_t nn, const do
     chk(hipMemcpyAsync(rhs_lift, d_lift, m->n * sizeof(double), hipMemcpyDeviceToHost, s));
   chk(hipStreamSynchronize(s));
   dfree(d_conn, s); dfree(d_xyz, s); dfree(d_a, s); dfree(d_b, s); dfree(d_t, s); dfree(d_pv, s); dfree(d_lift, s);
-  dfree(d_cn, s);
+  dfree(d_cn, s); dfree(d_kbuf, s); dfree(d_fbuf, s); dfree(d_pidx, s); dfree(d_lstart, s);
+  dfree(d_lidx, s);
   SSF_HIP(e);
   return SSF_OK;
 }
@@ -1251,14 +1192,13 @@ extern "C" int ssf_matrix_localize_batch(ssf_matrix *m, int64_t ne, int64_t nc, 
   SSF_HIP(hipSetDevice(m->ctx->device));
   hipStream_t s = m->ctx->stream;
   int *d_cn = nullptr;
-  int rc = prepare_assembly(m, 0, ne, (int)nc, (int)nc, nullptr, cn, false, &d_cn);
+  int rc = prepare_assembly(m, ne, (int)nc, cn, &d_cn);
   if (rc) return rc;
   double *d_lm = nullptr;
   SSF_HIP(dalloc(&d_lm, ne * nc * nc, s));
   SSF_HIP(hipMemcpyAsync(d_lm, lm, ne * nc * nc * sizeof(double), hipMemcpyHostToDevice, s));
-  for (size_t c = 0; c + 1 < m->colour_start.size(); c++)
-    SSF_HIP(localize_elements(m->colour_start[c], m->colour_start[c + 1], ne, (int)nc, m->order,
-                              m->slot, d_lm, m->a_ref, s));
+  // every entry adds its contributions of the element matrices in element order
+  SSF_HIP(asm_gather(m->negm, (int)nc, false, m->asm_cstart, m->asm_cidx, d_lm, m->a_ref, s));
   rc = values_changed(m);
   SSF_HIP(hipStreamSynchronize(s));
   dfree(d_lm, s);

@@ -11,13 +11,11 @@
 //   2. K_e, padded to 32x32, is three 16x16 tiles (0,0), (0,1), (1,1) of
 //      v_mfma_f64_16x16x4f64, 12 K-steps each; operands are formed on the fly
 //      from the LDS derivatives (G) and D (H);
-//   3. each lane adds its tile entries (and the mirror of tile (0,1)) into the
-//      reference-layout values through the element's slot map (coloured by code
-//      numbers, so no other wave touches the same entries), plus the Dirichlet
-//      lift. An element in which a code number repeats (coupled DOFs on two of
-//      its nodes) can hit one slot from two lanes: its K_e goes through LDS and
-//      one lane adds the entries in localize's order (row i, then column j,
-//      GEFEL/scr.cpp:422-458).
+//   3. each lane writes its tile entries of the lower triangle (the mirror of
+//      tile (0,1) for the entries above the diagonal) to the element's 300
+//      values of the matrix buffer, which the assembly by entry adds into the
+//      reference layout in localize's order (ssf_assemble.hip); an element with
+//      prescribed values also writes its lift values -K_e u_p.
 #include "ssf_device.hpp"
 
 namespace ssf {
@@ -41,27 +39,24 @@ __device__ __forceinline__ void hex_nat1(double xi, double eta, double zeta, int
 
 constexpr int EPB = TPB / 64;  // elements (wavefronts) per block
 
-__global__ __launch_bounds__(TPB) void k_hex8_elast(int64_t p0, int64_t p1, int64_t ne_total,
-                                                    const int *__restrict__ order,
-                                                    const int *__restrict__ conn,
+__global__ __launch_bounds__(TPB) void k_hex8_elast(int64_t ne, const int *__restrict__ conn,
                                                     const int *__restrict__ cn,
                                                     const double *__restrict__ xyz,
                                                     const double *__restrict__ E,
                                                     const double *__restrict__ NU,
-                                                    const int *__restrict__ slot,
-                                                    const unsigned char *__restrict__ dup,
-                                                    double *__restrict__ a_ref,
+                                                    double *__restrict__ kbuf,
+                                                    const int *__restrict__ pidx,
                                                     const double *__restrict__ pv,
-                                                    double *__restrict__ lift)
+                                                    double *__restrict__ fbuf)
 {
   __shared__ double s_xyz[EPB][3][8];
   __shared__ double s_b[EPB][8][8][3];  // [gp][node][d/dx, d/dy, d/dz]
   __shared__ double s_w[EPB][8];        // |J| of each Gauss point (weights are 1)
   __shared__ double s_k[EPB][24 * 24];  // K_e for the lift (only if needed)
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int64_t p = p0 + (int64_t)blockIdx.x * EPB + wv;
-  const bool active = p < p1;
-  const int e = active ? order[p] : 0;
+  const int64_t ep = (int64_t)blockIdx.x * EPB + wv;
+  const bool active = ep < ne;
+  const int64_t e = active ? ep : 0;
   if (active && lane < 24) {
     const int a = lane & 7, c = lane >> 3;
     const int nd = conn[(int64_t)e * 8 + a];
@@ -142,11 +137,10 @@ __global__ __launch_bounds__(TPB) void k_hex8_elast(int64_t p0, int64_t p1, int6
     acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(gv[1], hv[1], acc[2], 0, 0, 0);  // (1,1)
   }
   if (!active) return;
-  // 3. scatter through the slot map (AoS per element: slot[p*576 + i*24 + j])
-  const int *sl = slot + p * 576;
-  const bool serial = dup && dup[e];  // wave-uniform
-  const bool want_lift = lift != nullptr;
-  const bool stage = want_lift || serial;
+  // 3. the lower triangle (i >= j) into the element's kbuf values at
+  //    i (i + 1) / 2 + j; the lift needs whole rows, from LDS
+  double *kb = kbuf + e * 300;
+  const bool want_lift = fbuf != nullptr && pidx[e] >= 0;  // wave-uniform
 #pragma unroll
   for (int t = 0; t < 3; t++) {
     const int ti = (t == 2) ? 1 : 0, tj = (t == 0) ? 0 : 1;
@@ -155,77 +149,40 @@ __global__ __launch_bounds__(TPB) void k_hex8_elast(int64_t p0, int64_t p1, int6
       const int i = 16 * ti + 4 * v + lk, j = 16 * tj + li;
       if (i >= 24 || j >= 24) continue;
       const double kv = acc[t][v];
-      if (!serial) {
-        const int q = sl[i * 24 + j];
-        if (q >= 0) a_ref[q] += kv;
-      }
-      if (stage) s_k[wv][i * 24 + j] = kv;
-      if (t == 1) {  // mirror of tile (0,1) is tile (1,0)
-        if (!serial) {
-          const int q = sl[j * 24 + i];
-          if (q >= 0) a_ref[q] += kv;
-        }
-        if (stage) s_k[wv][j * 24 + i] = kv;
+      if (t == 1) kb[j * (j + 1) / 2 + i] = kv;  // tile (0,1): i < j, stored as (j, i)
+      else if (i >= j) kb[i * (i + 1) / 2 + j] = kv;
+      if (want_lift) {
+        s_k[wv][i * 24 + j] = kv;
+        if (t == 1) s_k[wv][j * 24 + i] = kv;  // mirror of tile (0,1) is tile (1,0)
       }
     }
   }
-  if (!stage) return;
+  if (!want_lift) return;
   // s_k is written and read by lanes of this wavefront only
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
   __builtin_amdgcn_wave_barrier();
-  if (serial) {
-    if (lane == 0)
-      for (int ij = 0; ij < 576; ij++) {
-        const int q = sl[ij];
-        if (q >= 0) a_ref[q] += s_k[wv][ij];
-      }
-    if (!want_lift) return;
-    if (lane == 0) {  // the lift row by row as well: two rows may share an unknown
-      for (int i = 0; i < 24; i++) {
-        const int ci = cn[(int64_t)e * 24 + i];
-        if (ci < 1) continue;
-        double f = 0.0;
-        bool any = false;
-        for (int j = 0; j < 24; j++) {
-          const int cj = cn[(int64_t)e * 24 + j];
-          if (cj < 0) {
-            f += s_k[wv][i * 24 + j] * pv[-cj - 1];
-            any = true;
-          }
-        }
-        if (any) lift[ci - 1] -= f;
-      }
-    }
-    return;
-  }
   if (lane < 24) {
-    const int ci = cn[(int64_t)e * 24 + lane];
+    const int ci = cn[e * 24 + lane];
     if (ci >= 1) {
       double f = 0.0;
-      bool any = false;
       for (int j = 0; j < 24; j++) {
-        const int cj = cn[(int64_t)e * 24 + j];
-        if (cj < 0) {
-          f += s_k[wv][lane * 24 + j] * pv[-cj - 1];
-          any = true;
-        }
+        const int cj = cn[e * 24 + j];
+        if (cj < 0) f += s_k[wv][lane * 24 + j] * pv[-cj - 1];
       }
-      if (any) lift[ci - 1] -= f;
+      fbuf[(int64_t)pidx[e] * 24 + lane] = -f;
     }
   }
 }
 
-hipError_t assemble_hex8_elast(int64_t p0, int64_t p1, int64_t ne_total, const int *order,
-                               const int *conn, const int *cn, const double *xyz,
-                               const double *ma, const double *mb, const int *slot,
-                               const unsigned char *dup, double *a_ref, const double *pv,
-                               double *lift, hipStream_t s)
+hipError_t assemble_hex8_elast(int64_t ne, const int *conn, const int *cn, const double *xyz,
+                               const double *ma, const double *mb, double *kbuf, const int *pidx,
+                               const double *pv, double *fbuf, hipStream_t s)
 {
-  if (p1 <= p0) return hipSuccess;
-  const unsigned nb = (unsigned)((p1 - p0 + EPB - 1) / EPB);
-  hipLaunchKernelGGL(k_hex8_elast, dim3(nb), dim3(TPB), 0, s, p0, p1, ne_total, order, conn, cn,
-                     xyz, ma, mb, slot, dup, a_ref, pv, lift);
+  if (ne <= 0) return hipSuccess;
+  const unsigned nb = (unsigned)((ne + EPB - 1) / EPB);
+  hipLaunchKernelGGL(k_hex8_elast, dim3(nb), dim3(TPB), 0, s, ne, conn, cn, xyz, ma, mb, kbuf,
+                     pidx, pv, fbuf);
   return hipGetLastError();
 }
 

@@ -221,13 +221,13 @@ struct ssf_matrix {
   int *map_up = nullptr;     // ref entry -> mirrored full slot or -1
   bool values_set = false;
   bool val32_valid = false;
-  // element assembly structures (built at the first assemble / localize_batch)
-  int64_t asm_ne = 0, asm_ndofe = 0;
-  int *slot = nullptr;       // [asm_ne * ndofe * ndofe] reference-layout slot or -1
-  int *order = nullptr;      // element ids sorted by colour
-  unsigned char *asm_dup = nullptr;  // [ne] 1: a positive code number repeats in the
-                                     // element (null when no element has one)
-  std::vector<int64_t> colour_start;
+  // element assembly plan (built at the first assemble / localize_batch of a
+  // code-number table): the contributions e nd^2 + i nd + j of the element
+  // matrices to entry s, asm_cidx[asm_cstart[s] .. asm_cstart[s + 1]), in the
+  // reference's localize order
+  int64_t asm_ne = 0, asm_ndofe = 0, asm_nc = 0;
+  int *asm_cstart = nullptr;  // [negm + 1]
+  int *asm_cidx = nullptr;    // [asm_nc]
   std::vector<int64_t> asm_cn_host;  // for identity checks
   // stream tiling: whole rows, at most STREAM_CAP entries per tile
   int *tiles = nullptr;
@@ -325,7 +325,7 @@ int matrix_create(ssf_ctx *ctx, int storage, int64_t n, const int64_t *adr, cons
 // from the pattern (moved in); assembled, multiplied and solved in FP64
 int matrix_create_sell(ssf_ctx *ctx, int64_t n, std::vector<int64_t> &adr,
                        std::vector<int64_t> &ci, ssf_matrix **out);
-// releases the element-assembly structures (slot map, colour order) of a matrix
+// releases the element-assembly plan of a matrix
 // and, if host_pattern, its host copy of the reference-layout pattern
 void matrix_drop_assembly(ssf_matrix *m, bool host_pattern);
 // keeps only what the SELL-64 solve reads (slice offsets, columns of the

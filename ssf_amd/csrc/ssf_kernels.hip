@@ -1,7 +1,7 @@
 // gfx950 kernels of the SIFEL linear-system path: CSR SpMV with fused dots, the
 // preconditioned CG recurrence of GEFEL/scr.cpp:1093 (and its cg / comprow
 // variants), reference-layout <-> full-CSR value maps, and the batched element
-// kernels with atomic-free coloured assembly.
+// kernels, which write their matrices for the assembly by entry (ssf_assemble.hip).
 //
 // Reductions are deterministic: every block writes one partial per quantity, and
 // every consumer block re-reduces the whole partial array in the same fixed order,
@@ -187,53 +187,6 @@ hipError_t copy_kernel(int64_t n, const double *a, double *b, hipStream_t s)
 
 // ----------------------------------------------------------------- assembly
 
-// slot[(i*nd + j) * ne_total + p] (aos: slot[p * nd*nd + i*nd + j]) = reference-layout
-// entry of element order[p]'s
-// local pair (i, j), or -1 where localize skips it (code number <= 0, or the
-// upper triangle of symcomprow: GEFEL/scr.cpp:427-433).
-__global__ void k_build_slots(int storage, int aos, int64_t ne_total, int nd,
-                              const int *__restrict__ order, const int *__restrict__ cn,
-                              const int *__restrict__ adr, const int *__restrict__ ci,
-                              int *__restrict__ slot)
-{
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t nq = (int64_t)nd * nd;
-  const int64_t total = ne_total * nq;
-  if (t >= total) return;
-  const int64_t p = aos ? t / nq : t % ne_total;
-  const int ij = aos ? (int)(t % nq) : (int)(t / ne_total);
-  const int i = ij / nd, j = ij % nd;
-  const int e = order[p];
-  const int ii = cn[(int64_t)e * nd + i] - 1;
-  const int jj = cn[(int64_t)e * nd + j] - 1;
-  int sl = -1;
-  if (ii >= 0 && jj >= 0 && !(storage == SSF_STORAGE_SYMM_COMP_ROWS && ii < jj)) {
-    int lo = adr[ii], hi = adr[ii + 1] - 1;
-    while (lo <= hi) {
-      const int mid = (lo + hi) >> 1;
-      const int c = ci[mid];
-      if (c == jj) {
-        sl = mid;
-        break;
-      }
-      if (c < jj) lo = mid + 1;
-      else hi = mid - 1;
-    }
-  }
-  slot[t] = sl;
-}
-
-hipError_t build_slots(int storage, int aos, int64_t ne_total, int nd, const int *order,
-                       const int *cn, const int *ref_adr, const int *ref_ci, int *slot,
-                       hipStream_t s)
-{
-  const int64_t total = ne_total * nd * nd;
-  if (total == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_build_slots, dim3((unsigned)((total + TPB - 1) / TPB)), dim3(TPB), 0, s,
-                     storage, aos, ne_total, nd, order, cn, ref_adr, ref_ci, slot);
-  return hipGetLastError();
-}
-
 // natural derivatives of the trilinear hexahedron (GEFEL/basefun.cpp:2317-2375)
 __device__ __forceinline__ void hex_nat(double xi, double eta, double zeta, double *nx,
                                         double *ny, double *nz)
@@ -383,25 +336,24 @@ __device__ void ke_hex8_cap(const double *x, const double *y, const double *z, d
   }
 }
 
-// one thread per element of a colour; adds K_e into the reference-layout values
-// through the slot map, and -K_e u_p into the Dirichlet lift (colour-exclusive).
+// one thread per element: K_e (symmetric) into kbuf[e][36] as its lower triangle
+// (entry (i, j), j <= i, at i (i + 1) / 2 + j) and, for an element with prescribed
+// values (pidx[e] >= 0), its lift values -sum_j K_ij u_j over the prescribed j into
+// fbuf[pidx[e]][8] (rows with a code number >= 1)
 template <int KIND>
-__global__ __launch_bounds__(TPB) void k_assemble8(int64_t p0, int64_t p1, int64_t ne_total,
-                                                   const int *__restrict__ order,
-                                                   const int *__restrict__ conn,
+__global__ __launch_bounds__(TPB) void k_assemble8(int64_t ne, const int *__restrict__ conn,
                                                    const int *__restrict__ cn,
                                                    const double *__restrict__ xyz,
                                                    const double *__restrict__ ma,
                                                    const double *__restrict__ mb,
                                                    const double *__restrict__ th,
-                                                   const int *__restrict__ slot,
-                                                   double *__restrict__ a_ref,
+                                                   double *__restrict__ kbuf,
+                                                   const int *__restrict__ pidx,
                                                    const double *__restrict__ pv,
-                                                   double *__restrict__ lift)
+                                                   double *__restrict__ fbuf)
 {
-  const int64_t p = p0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= p1) return;
-  const int e = order[p];
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
   double K[64];
   if (KIND == SSF_ELEM_HEX8_HEAT || KIND == SSF_ELEM_HEX8_HEAT_CAP) {
     double x[8], y[8], z[8];
@@ -421,79 +373,47 @@ __global__ __launch_bounds__(TPB) void k_assemble8(int64_t p0, int64_t p1, int64
     }
     ke_quad4_pstress(x, y, ma[e], mb[e], th ? th[e] : 1.0, K);
   }
+  double *kb = kbuf + e * 36;
 #pragma unroll
-  for (int q = 0; q < 64; q++) {
-    const int s = slot[(int64_t)q * ne_total + p];
-    if (s >= 0) a_ref[s] += K[q];
-  }
-  if (lift) {
+  for (int i = 0; i < 8; i++)
+#pragma unroll
+    for (int j = 0; j <= i; j++) kb[i * (i + 1) / 2 + j] = K[i * 8 + j];
+  if (fbuf && pidx[e] >= 0) {
     int c[8];
-    bool any = false;
+#pragma unroll
+    for (int i = 0; i < 8; i++) c[i] = cn[e * 8 + i];
+    double *fb = fbuf + (int64_t)pidx[e] * 8;
 #pragma unroll
     for (int i = 0; i < 8; i++) {
-      c[i] = cn[(int64_t)e * 8 + i];
-      any |= c[i] < 0;
-    }
-    if (any) {
+      if (c[i] < 1) continue;
+      double f = 0.0;
 #pragma unroll
-      for (int i = 0; i < 8; i++) {
-        if (c[i] < 1) continue;
-        double f = 0.0;
-#pragma unroll
-        for (int j = 0; j < 8; j++)
-          if (c[j] < 0) f += K[i * 8 + j] * pv[-c[j] - 1];
-        lift[c[i] - 1] -= f;
-      }
+      for (int j = 0; j < 8; j++)
+        if (c[j] < 0) f += K[i * 8 + j] * pv[-c[j] - 1];
+      fb[i] = -f;
     }
   }
 }
 
-hipError_t assemble_hex8_elast(int64_t p0, int64_t p1, int64_t ne_total, const int *order,
-                               const int *conn, const int *cn, const double *xyz,
-                               const double *ma, const double *mb, const int *slot,
-                               const unsigned char *dup, double *a_ref, const double *pv,
-                               double *lift, hipStream_t s);
+hipError_t assemble_hex8_elast(int64_t ne, const int *conn, const int *cn, const double *xyz,
+                               const double *ma, const double *mb, double *kbuf, const int *pidx,
+                               const double *pv, double *fbuf, hipStream_t s);
 
-hipError_t assemble_elements(int kind, int64_t p0, int64_t p1, int64_t ne_total,
-                             const int *order, const int *conn, const int *cn,
+hipError_t assemble_elements(int kind, int64_t ne, const int *conn, const int *cn,
                              const double *xyz, const double *mat_a, const double *mat_b,
-                             const double *thick, const int *slot, const unsigned char *dup,
-                             double *a_ref, const double *pv, double *lift, hipStream_t s)
+                             const double *thick, double *kbuf, const int *pidx,
+                             const double *pv, double *fbuf, hipStream_t s)
 {
-  if (p1 <= p0) return hipSuccess;
+  if (ne <= 0) return hipSuccess;
   if (kind == SSF_ELEM_HEX8_ELAST)
-    return assemble_hex8_elast(p0, p1, ne_total, order, conn, cn, xyz, mat_a, mat_b, slot, dup,
-                               a_ref, pv, lift, s);
-  const unsigned nb = (unsigned)((p1 - p0 + TPB - 1) / TPB);
+    return assemble_hex8_elast(ne, conn, cn, xyz, mat_a, mat_b, kbuf, pidx, pv, fbuf, s);
+  const unsigned nb = (unsigned)((ne + TPB - 1) / TPB);
   if (kind == SSF_ELEM_HEX8_HEAT)
-    hipLaunchKernelGGL((k_assemble8<SSF_ELEM_HEX8_HEAT>), dim3(nb), dim3(TPB), 0, s, p0, p1, ne_total, order, conn, cn, xyz, mat_a, mat_b, thick, slot, a_ref, pv, lift);
+    hipLaunchKernelGGL((k_assemble8<SSF_ELEM_HEX8_HEAT>), dim3(nb), dim3(TPB), 0, s, ne, conn, cn, xyz, mat_a, mat_b, thick, kbuf, pidx, pv, fbuf);
   else if (kind == SSF_ELEM_HEX8_HEAT_CAP)
-    hipLaunchKernelGGL((k_assemble8<SSF_ELEM_HEX8_HEAT_CAP>), dim3(nb), dim3(TPB), 0, s, p0, p1, ne_total, order, conn, cn, xyz, mat_a, mat_b, thick, slot, a_ref, pv, lift);
+    hipLaunchKernelGGL((k_assemble8<SSF_ELEM_HEX8_HEAT_CAP>), dim3(nb), dim3(TPB), 0, s, ne, conn, cn, xyz, mat_a, mat_b, thick, kbuf, pidx, pv, fbuf);
   else
-    hipLaunchKernelGGL((k_assemble8<SSF_ELEM_QUAD4_PSTRESS>), dim3(nb), dim3(TPB), 0, s, p0, p1, ne_total, order, conn, cn, xyz, mat_a, mat_b, thick, slot, a_ref, pv, lift);
-  return hipGetLastError();
-}
-
-__global__ void k_localize(int64_t p0, int64_t p1, int64_t ne_total, int nd,
-                           const int *__restrict__ order, const int *__restrict__ slot,
-                           const double *__restrict__ lm, double *__restrict__ a_ref)
-{
-  const int64_t p = p0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= p1) return;
-  const int e = order[p];
-  const int nq = nd * nd;
-  for (int q = 0; q < nq; q++) {
-    const int s = slot[(int64_t)q * ne_total + p];
-    if (s >= 0) a_ref[s] += lm[(int64_t)e * nq + q];
-  }
-}
-
-hipError_t localize_elements(int64_t p0, int64_t p1, int64_t ne_total, int nd, const int *order,
-                             const int *slot, const double *lm, double *a_ref, hipStream_t s)
-{
-  if (p1 <= p0) return hipSuccess;
-  hipLaunchKernelGGL(k_localize, dim3((unsigned)((p1 - p0 + TPB - 1) / TPB)), dim3(TPB), 0, s,
-                     p0, p1, ne_total, nd, order, slot, lm, a_ref);
+    hipLaunchKernelGGL((k_assemble8<SSF_ELEM_QUAD4_PSTRESS>), dim3(nb), dim3(TPB), 0, s, ne, conn, cn, xyz, mat_a, mat_b, thick, kbuf, pidx, pv, fbuf);
   return hipGetLastError();
 }
 

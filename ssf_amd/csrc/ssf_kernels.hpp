@@ -93,10 +93,28 @@ hipError_t dot_partial(int64_t n, const double *a, const double *b, double *part
 hipError_t fill(int64_t n, double v, double *x, hipStream_t s);
 hipError_t reduce_to_host_slot(const double *part, int nb, double *out, hipStream_t s);
 
-// assembly
-hipError_t build_slots(int storage, int aos, int64_t ne_total, int nd, const int *order,
-                       const int *cn, const int *ref_adr, const int *ref_ci, int *slot,
-                       hipStream_t s);
+// assembly by entry (ssf_assemble.hip): the plan of a pattern and code-number table
+// (cn[ne*nd] int32 on the device, the reference-layout pattern adr / ci int32 on
+// the device): for every entry its contributions e nd^2 + i nd + j in the
+// reference's localize order, cidx[cstart[s] .. cstart[s+1])
+int asm_plan_build(int storage, int64_t n, int64_t negm, int64_t ne, int nd, const int *cn,
+                   const int *adr, const int *ci, hipStream_t s, int **cstart, int **cidx,
+                   int64_t *nc);
+// a[s] += its contributions from kbuf: element matrices in full (sym false, nd^2
+// per element) or as lower triangles (sym true, nd (nd + 1) / 2 per element)
+hipError_t asm_gather(int64_t negm, int nd, bool sym, const int *cstart, const int *cidx,
+                      const double *kbuf, double *a, hipStream_t s);
+// lift[d] += fbuf[lidx[k]] for k in [lstart[d], lstart[d+1])
+hipError_t lift_gather(int64_t n, const int *lstart, const int *lidx, const double *fbuf,
+                       double *lift, hipStream_t s);
+void lift_plan_host(int64_t n, int64_t ne, int nd, const int64_t *cn, std::vector<int> &pidx,
+                    std::vector<int> &lstart, std::vector<int> &lidx, int64_t *npe);
+// device bytes of building an assembly plan and assembling once (peak)
+inline int64_t asm_bytes(int64_t ne, int nd, int64_t negm)
+{
+  const int64_t ncmax = ne * nd * nd;
+  return 8 * ne * nd + 20 * ncmax + 4 * (negm + 1) + 8 * ne * nd * (nd + 1) / 2 + 4 * ne * nd;
+}
 // value and vector updates rounded as the reference's loops (no contraction)
 hipError_t vals_scale(int64_t n, double *a, double c, hipStream_t s);             // a *= c
 hipError_t vals_add(int64_t n, double *a, double c, const double *b, hipStream_t s);  // a += c b
@@ -110,13 +128,12 @@ hipError_t vec_sc2_add(int64_t n, double *f, const double *rhs, double alpha, do
 hipError_t vec_rate(int64_t n, double *v, const double *x, const double *d, double dt,
                     double alpha, hipStream_t s);                                 // v = (x - d) / dt / alpha
 
-hipError_t assemble_elements(int kind, int64_t p0, int64_t p1, int64_t ne_total,
-                             const int *order, const int *conn, const int *cn,
+// element matrices of every element as lower triangles into kbuf, and the lift
+// values of the elements with prescribed values (pidx[e] >= 0) into fbuf (fbuf
+// may be null)
+hipError_t assemble_elements(int kind, int64_t ne, const int *conn, const int *cn,
                              const double *xyz, const double *mat_a, const double *mat_b,
-                             const double *thick, const int *slot, const unsigned char *dup,
-                             double *a_ref, const double *pv, double *lift, hipStream_t s);
-hipError_t localize_elements(int64_t p0, int64_t p1, int64_t ne_total, int nd,
-                             const int *order, const int *slot, const double *lm,
-                             double *a_ref, hipStream_t s);
+                             const double *thick, double *kbuf, const int *pidx,
+                             const double *pv, double *fbuf, hipStream_t s);
 
 }  // namespace ssf

@@ -131,12 +131,12 @@ int64_t resident_bytes(int64_t n, int64_t tot, bool col16, bool mixed)
 
 // and while it is laid out and assembled: the device pattern, the layout build's
 // temporaries (row index, sort keys and values, twice), SELL columns of both
-// widths, SELL positions and values of the reference layout, element slot map
-// and colour order
+// widths, SELL positions and values of the reference layout, the assembly plan's
+// build and the element-matrix buffer
 int64_t prep_bytes(const Request &q, int64_t negm, int64_t tot, bool mixed)
 {
   return resident_bytes(q.n, tot, false, mixed) + 2 * tot + 4 * (q.n + 1) + 4 * negm + 28 * negm +
-         24 * q.n + 16 * negm + 8 * negm + 4 * q.ne * q.ndofe * q.ndofe + 8 * q.ne;
+         24 * q.n + 16 * negm + 8 * negm + asm_bytes(q.ne, (int)q.ndofe, negm) + 8 * q.ne;
 }
 
 // s: the stream the release is ordered on (device memory is stream-ordered)
