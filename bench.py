@@ -360,6 +360,21 @@ def bench_assembly(ctx, args, gpu, mesh, torch=None):
            "gflops_call": flop / dt / 1e9}
     log(f"[assembly] {out}")
     m.close()
+    if args.asm_heat_nel:
+        # the conductivity matrix of the heat cube (hex8, 8 DOFs per element)
+        hp = mesh.heat_cube(args.asm_heat_nel, seed=2)
+        mh = gpu.Matrix.from_elements(ctx, gpu.STORAGE_SCR, hp.n, hp.elem_codes())
+        mh.assemble(hp, lift=True)
+        th = []
+        for _ in range(args.asm_reps):
+            mh.zero()
+            t = time.perf_counter()
+            mh.assemble(hp, lift=True)
+            th.append(time.perf_counter() - t)
+        out["heat"] = {"workload": f"hex8 heat {args.asm_heat_nel}^3, {hp.ne} elements, n={hp.n}",
+                       "assemble_call_s": min(th), "elements_per_s": hp.ne / min(th)}
+        log(f"[assembly] heat {out['heat']}")
+        mh.close()
     if torch is not None:
         x = torch.tensor(np.random.default_rng(1).standard_normal(prob.n) * 1e-3,
                          dtype=torch.float64, device=torch.device("cuda:0"))
@@ -777,6 +792,7 @@ def main():
     ap.add_argument("--only-assembly", action="store_true")
     ap.add_argument("--asm-nel", type=int, default=64)
     ap.add_argument("--asm-reps", type=int, default=3)
+    ap.add_argument("--asm-heat-nel", type=int, default=128)
     ap.add_argument("--post-reps", type=int, default=10)
     ap.add_argument("--no-post", action="store_true", help="skip the post-solve element pass")
     ap.add_argument("--no-np", action="store_true", help="skip the nonstationary heat leg")

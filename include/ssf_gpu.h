@@ -461,7 +461,10 @@ int ssf_spmv_bench(ssf_matrix *m, int precision, int reps, double *usec);
  * row patterns: 2, 4 or 8), blocks (at most 8192), idx16 (column form: 0 int32,
  * 1 16-bit tile or slice offsets, 2 SELL row patterns, 3 SELL stencil form: one
  * slot per column offset in every row, offered when the matrix has at most 32
- * offsets), map (SELL slices to wavefronts: 0 a contiguous range per block, 1
+ * offsets and its slices fit 32-bit byte offsets; a slot the row lacks holds 0
+ * and multiplies x at the row plus that offset, so its products equal the other
+ * forms' for finite x only: an Inf or NaN in x makes NaN in rows that do not
+ * couple to it), map (SELL slices to wavefronts: 0 a contiguous range per block, 1
  * each of the 8 XCDs a contiguous eighth whose slices its wavefronts take in
  * turn; the default)}. */
 int ssf_matrix_autotune(ssf_matrix *m, int precision, int reps, int32_t *tune_out);

@@ -1131,7 +1131,8 @@ extern "C" int ssf_matrix_assemble(ssf_matrix *m, int kind, int64_t nn, const do
   double *d_kbuf = nullptr, *d_fbuf = nullptr;
   hipError_t e = hipSuccess;
   auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
-  chk(dalloc(&d_kbuf, ne * (nd * (nd + 1) / 2), s));
+  const int wsh = asm_wshift(kind);
+  chk(dalloc(&d_kbuf, asm_kbuf_count(ne, nd, wsh), s));
   if (rhs_lift) {
     chk(dalloc(&d_pidx, ne, s));
     chk(dalloc(&d_lstart, m->n + 1, s));
@@ -1166,7 +1167,7 @@ extern "C" int ssf_matrix_assemble(ssf_matrix *m, int kind, int64_t nn, const do
     chk(assemble_elements(kind, ne, d_conn, d_cn, d_xyz, d_a, d_b, d_t, d_kbuf, d_pidx, d_pv,
                           d_fbuf, s));
   if (e == hipSuccess)
-    chk(asm_gather(m->negm, nd, true, m->asm_cstart, m->asm_cidx, d_kbuf, m->a_ref, s));
+    chk(asm_gather(m->negm, nd, true, wsh, m->asm_cstart, m->asm_cidx, d_kbuf, m->a_ref, s));
   if (e == hipSuccess && rhs_lift) chk(lift_gather(m->n, d_lstart, d_lidx, d_fbuf, d_lift, s));
   if (e == hipSuccess) {
     int r2 = values_changed(m);
@@ -1198,7 +1199,7 @@ extern "C" int ssf_matrix_localize_batch(ssf_matrix *m, int64_t ne, int64_t nc, 
   SSF_HIP(dalloc(&d_lm, ne * nc * nc, s));
   SSF_HIP(hipMemcpyAsync(d_lm, lm, ne * nc * nc * sizeof(double), hipMemcpyHostToDevice, s));
   // every entry adds its contributions of the element matrices in element order
-  SSF_HIP(asm_gather(m->negm, (int)nc, false, m->asm_cstart, m->asm_cidx, d_lm, m->a_ref, s));
+  SSF_HIP(asm_gather(m->negm, (int)nc, false, 0, m->asm_cstart, m->asm_cidx, d_lm, m->a_ref, s));
   rc = values_changed(m);
   SSF_HIP(hipStreamSynchronize(s));
   dfree(d_lm, s);

@@ -167,10 +167,11 @@ int asm_plan_build(int storage, int64_t n, int64_t negm, int64_t ne, int nd, con
 }
 
 // a[s] += the contributions of entry s in list order; a contribution
-// e nd^2 + i nd + j reads the element matrix buffer at e P + (i, j): the full
-// matrix (P = nd^2) or its lower triangle (P = nd (nd + 1) / 2, symmetric K_e)
+// e nd^2 + i nd + j reads value t of element e in the element-matrix buffer: the
+// full matrix (P = nd^2, t = i nd + j) or its lower triangle (P = nd (nd + 1) / 2,
+// t of (max(i, j), min(i, j)), symmetric K_e), at (e / W) W P + t W + e % W
 template <bool SYM>
-__global__ __launch_bounds__(TPB) void k_asm_gather(int64_t negm, int nd,
+__global__ __launch_bounds__(TPB) void k_asm_gather(int64_t negm, int nd, int wshift,
                                                     const int *__restrict__ cstart,
                                                     const int *__restrict__ cidx,
                                                     const double *__restrict__ kbuf,
@@ -182,6 +183,7 @@ __global__ __launch_bounds__(TPB) void k_asm_gather(int64_t negm, int nd,
   if (k0 == k1) return;
   const int nd2 = nd * nd;
   const int64_t P = SYM ? (int64_t)nd * (nd + 1) / 2 : nd2;
+  const int wmask = (1 << wshift) - 1;
   double acc = a[s];
   for (int k = k0; k < k1; k++) {
     const int v = cidx[k];
@@ -191,18 +193,20 @@ __global__ __launch_bounds__(TPB) void k_asm_gather(int64_t negm, int nd,
       const int i = ij / nd, j = ij - i * nd;
       t = i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i;
     }
-    acc += kbuf[(int64_t)e * P + t];
+    acc += kbuf[(int64_t)(e & ~wmask) * P + ((int64_t)t << wshift) + (e & wmask)];
   }
   a[s] = acc;
 }
 
-hipError_t asm_gather(int64_t negm, int nd, bool sym, const int *cstart, const int *cidx,
-                      const double *kbuf, double *a, hipStream_t s)
+hipError_t asm_gather(int64_t negm, int nd, bool sym, int wshift, const int *cstart,
+                      const int *cidx, const double *kbuf, double *a, hipStream_t s)
 {
   if (negm == 0) return hipSuccess;
+  // (a mapping that gives each XCD a contiguous eighth of the entries measured
+  // 12 % slower on the 64x64x256 elasticity column: 2.93 against 2.62 ms)
   const unsigned g = (unsigned)((negm + TPB - 1) / TPB);
-  if (sym) hipLaunchKernelGGL(k_asm_gather<true>, dim3(g), dim3(TPB), 0, s, negm, nd, cstart, cidx, kbuf, a);
-  else hipLaunchKernelGGL(k_asm_gather<false>, dim3(g), dim3(TPB), 0, s, negm, nd, cstart, cidx, kbuf, a);
+  if (sym) hipLaunchKernelGGL(k_asm_gather<true>, dim3(g), dim3(TPB), 0, s, negm, nd, wshift, cstart, cidx, kbuf, a);
+  else hipLaunchKernelGGL(k_asm_gather<false>, dim3(g), dim3(TPB), 0, s, negm, nd, wshift, cstart, cidx, kbuf, a);
   return hipGetLastError();
 }
 

@@ -336,8 +336,10 @@ __device__ void ke_hex8_cap(const double *x, const double *y, const double *z, d
   }
 }
 
-// one thread per element: K_e (symmetric) into kbuf[e][36] as its lower triangle
-// (entry (i, j), j <= i, at i (i + 1) / 2 + j) and, for an element with prescribed
+// one thread per element: K_e (symmetric) into kbuf as its lower triangle (entry
+// (i, j), j <= i, is value t = i (i + 1) / 2 + j), in groups of 64 elements: value t
+// of element e at (e / 64) 64 36 + 64 t + e % 64, so that every store of a
+// wavefront writes 64 consecutive doubles; and, for an element with prescribed
 // values (pidx[e] >= 0), its lift values -sum_j K_ij u_j over the prescribed j into
 // fbuf[pidx[e]][8] (rows with a code number >= 1)
 template <int KIND>
@@ -373,11 +375,11 @@ __global__ __launch_bounds__(TPB) void k_assemble8(int64_t ne, const int *__rest
     }
     ke_quad4_pstress(x, y, ma[e], mb[e], th ? th[e] : 1.0, K);
   }
-  double *kb = kbuf + e * 36;
+  double *kb = kbuf + (e >> 6) * (64 * 36) + (e & 63);
 #pragma unroll
   for (int i = 0; i < 8; i++)
 #pragma unroll
-    for (int j = 0; j <= i; j++) kb[i * (i + 1) / 2 + j] = K[i * 8 + j];
+    for (int j = 0; j <= i; j++) kb[64 * (i * (i + 1) / 2 + j)] = K[i * 8 + j];
   if (fbuf && pidx[e] >= 0) {
     int c[8];
 #pragma unroll

@@ -100,10 +100,21 @@ hipError_t reduce_to_host_slot(const double *part, int nb, double *out, hipStrea
 int asm_plan_build(int storage, int64_t n, int64_t negm, int64_t ne, int nd, const int *cn,
                    const int *adr, const int *ci, hipStream_t s, int **cstart, int **cidx,
                    int64_t *nc);
-// a[s] += its contributions from kbuf: element matrices in full (sym false, nd^2
-// per element) or as lower triangles (sym true, nd (nd + 1) / 2 per element)
-hipError_t asm_gather(int64_t negm, int nd, bool sym, const int *cstart, const int *cidx,
-                      const double *kbuf, double *a, hipStream_t s);
+// The element-matrix buffer: P values per element (the full matrix, P = nd^2, or
+// its lower triangle, P = nd (nd + 1) / 2 with (i, j), j <= i, at i (i + 1) / 2 + j)
+// in groups of W = 2^wshift elements: value t of element e at
+// (e / W) W P + t W + e % W. The one-thread-per-element kernels write W = 64 (their
+// stores coalesce), the wavefront-per-element elasticity kernel and the caller's
+// matrices of localize_batch W = 1.
+inline int asm_wshift(int kind) { return kind == SSF_ELEM_HEX8_ELAST ? 0 : 6; }
+inline int64_t asm_kbuf_count(int64_t ne, int nd, int wshift)
+{
+  const int64_t w = int64_t(1) << wshift;
+  return (ne + w - 1) / w * w * ((int64_t)nd * (nd + 1) / 2);
+}
+// a[s] += its contributions from kbuf (lower triangles when sym, else full matrices)
+hipError_t asm_gather(int64_t negm, int nd, bool sym, int wshift, const int *cstart,
+                      const int *cidx, const double *kbuf, double *a, hipStream_t s);
 // lift[d] += fbuf[lidx[k]] for k in [lstart[d], lstart[d+1])
 hipError_t lift_gather(int64_t n, const int *lstart, const int *lidx, const double *fbuf,
                        double *lift, hipStream_t s);
@@ -113,7 +124,7 @@ void lift_plan_host(int64_t n, int64_t ne, int nd, const int64_t *cn, std::vecto
 inline int64_t asm_bytes(int64_t ne, int nd, int64_t negm)
 {
   const int64_t ncmax = ne * nd * nd;
-  return 8 * ne * nd + 20 * ncmax + 4 * (negm + 1) + 8 * ne * nd * (nd + 1) / 2 + 4 * ne * nd;
+  return 8 * ne * nd + 20 * ncmax + 4 * (negm + 1) + 8 * asm_kbuf_count(ne, nd, 6) + 4 * ne * nd;
 }
 // value and vector updates rounded as the reference's loops (no contraction)
 hipError_t vals_scale(int64_t n, double *a, double c, hipStream_t s);             // a *= c
