@@ -15,6 +15,18 @@ __device__ __forceinline__ int xcd_block(int b, int nb)
   return ((nb & 7) == 0) ? (b & 7) * (nb >> 3) + (b >> 3) : b;
 }
 
+// Orders LDS accesses between the lanes of one wavefront (data that lanes of
+// this wavefront wrote and other lanes of it read): the wavefront's LDS
+// operations execute in order, so it waits for its own outstanding LDS operations
+// only (s_waitcnt lgkmcnt(0)); global loads and stores stay in flight. The asm
+// statements keep the compiler from moving memory accesses across the point.
+__device__ __forceinline__ void wave_lds_sync()
+{
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // gfx9 encoding: vmcnt 63, expcnt 7, lgkmcnt 0
+  asm volatile("" ::: "memory");
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v)
 {

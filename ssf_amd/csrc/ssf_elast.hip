@@ -75,13 +75,6 @@ __device__ __forceinline__ void tri_ij(int idx, int &i, int &j)
   j = idx - r * (r + 1) / 2;
 }
 
-// LDS written and read by the lanes of one wavefront only
-__device__ __forceinline__ void wave_lds_sync()
-{
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-}
 
 // wavefronts of the element loop: every wavefront assembles elements w, w + nw, ...
 // so that its lane constants (shape-function factors, LDS positions of M and of

@@ -60,14 +60,8 @@ struct Shape<SSF_ELEM_QUAD4_PSTRESS> {
 constexpr double kGP = 0.577350269189626;  // GEFEL/intp.cpp:26-27, weights 1
 
 // The NGP lanes of an element are consecutive lanes of one wavefront and only
-// they touch the element's LDS records: a wavefront-local barrier with a
-// workgroup-scope fence (waits for the LDS writes) orders them.
-__device__ __forceinline__ void wave_sync()
-{
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-}
+// they touch the element's LDS records, so a wavefront-local LDS sync orders them.
+__device__ __forceinline__ void wave_sync() { wave_lds_sync(); }
 
 // Natural derivatives of the trilinear hexahedron (GEFEL/basefun.cpp:2317-2375),
 // e.g. dN_0/dxi = (1 + eta)(1 + zeta) / 8 for node 0 at (+,+,+). At a Gauss point

@@ -197,3 +197,32 @@ def test_batch_one_graph_launch_repeats_the_host_loop(ctx):
                 q = s * 2 + r
                 assert ia[q].iterations == ib[q].iterations == ic[q].iterations
                 assert ia[q].status == ib[q].status == ic[q].status
+
+
+def test_batch_plates_of_the_benchmark_conditioning_vs_oracle(ctx):
+    """Plates like the 512-plate benchmark batch (quad4 plane stress, aspect 4,
+    lognormal E with sigma_log 0.3, clamped edge; up to ~59k DOF), two load cases
+    each: the FP64 batch at res 1e-20 and the mixed batch at res 1e-22 (the
+    tolerances the benchmark runs) are within 1e-8 of the oracle's FP64 cg_prec
+    run to 1e-30 on the oracle's own assembly of the same plate."""
+    probs = [mesh.plate(340, 85, seed=41, nrhs=2), mesh.plate(160, 40, seed=42, nrhs=2),
+             mesh.plate(64, 16, seed=43, nrhs=2)]
+    mats, rhs = [], []
+    for p in probs:
+        m, r = gpu.build_problem_matrix(ctx, p)
+        mats.append(m)
+        rhs.append(r)
+    b = gpu.Batch(ctx, mats, 2)
+    x64, i64 = b.solve(rhs, ni=40000, res=1e-20)
+    xmx, imx = b.solve(rhs, ni=40000, res=1e-22, precision=gpu.MIXED)
+    for s, p in enumerate(probs):
+        adr, ci, a, _ = O.assemble_problem(p, "scr")
+        for r in range(2):
+            xo, ani, ares, st = O.scr_cg_prec(adr, ci, a, rhs[s][r], 60000, 1e-30)
+            assert st in (1, 2), (s, r, st)
+            assert i64[s * 2 + r].status == 1 and imx[s * 2 + r].status in (1, 4)
+            assert rel(x64[s][r], xo) <= 1e-8, (s, r, rel(x64[s][r], xo))
+            assert rel(xmx[s][r], xo) <= 1e-8, (s, r, rel(xmx[s][r], xo))
+    b.close()
+    for m in mats:
+        m.close()
