@@ -154,12 +154,14 @@ int ssf_matrix_set_values(ssf_matrix *m, const double *a);
 /* device -> a[negm] in the reference layout */
 int ssf_matrix_get_values(const ssf_matrix *m, double *a);
 /* gmatrix::localized (GEFEL/gmatrix.cpp:581) for a batch of element matrices:
- * lm[ne][nc*nc] row-major added into the device values in element order of the
- * colouring (symcomprow keeps ii>=jj, comprow all). cn[ne*nc] 1-based. */
+ * lm[ne][nc*nc] row-major added into the device values; every entry receives its
+ * contributions in localize's order (element, row, column; symcomprow keeps
+ * ii>=jj, comprow all; code numbers <= 0 skipped). cn[ne*nc] 1-based; code
+ * numbers may repeat within and across elements (coupled DOFs). */
 int ssf_matrix_localize_batch(ssf_matrix *m, int64_t ne, int64_t nc, const int64_t *cn,
                               const double *lm);
-/* Batched element kernel + atomic-free coloured assembly straight into the device
- * values (stiffness_matrix MEFEL/SRC/globmat.cpp:41 / conductivity_matrix
+/* Batched element kernel + atomic-free assembly by entry into the device values,
+ * each entry summed in localize's order (stiffness_matrix MEFEL/SRC/globmat.cpp:41 / conductivity_matrix
  * TRFEL/SRC/globmatt.cpp:19 with linhex / linhext / planeelemlq kernels).
  * xyz[nn*3] node coordinates (z ignored for quad4), conn[ne*nne] 0-based nodes,
  * cn[ne*ndofe] code numbers, mat_a = E (elasticity) or k (heat), mat_b = nu,

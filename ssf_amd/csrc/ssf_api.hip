@@ -1,4 +1,4 @@
-// C ABI of libssf_gpu: storage setup, values, coloured assembly, SpMV and the
+// C ABI of libssf_gpu: storage setup, values, assembly by entry, SpMV and the
 // CG solves behind gmatrix::solve_system (see include/ssf_gpu.h).
 #include <algorithm>
 #include <atomic>

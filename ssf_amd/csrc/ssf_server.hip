@@ -4,7 +4,7 @@
 //  submit (any host thread) -> pending queue (FIFO)
 //  prep workers (host threads, one HIP stream each): pattern of the request
 //    (symcomprow::initiate semantics, GEFEL/scr.cpp:556), wait until the request's
-//    device bytes fit the budget, upload, element kernels + coloured assembly on
+//    device bytes fit the budget, upload, element kernels + assembly by entry on
 //    the device, right-hand side with the Dirichlet lift, solve vectors
 //    -> ready queue
 //  solve loop (one host thread, its own stream): ready requests join the resident

@@ -289,7 +289,7 @@ class Matrix:
         _check(lib().ssf_matrix_localize_batch(self.h, ne, nc, _p(cn), _p(lm)))
 
     def assemble(self, prob: mesh.Problem, lift: bool = True):
-        """Element kernels + coloured assembly; returns the Dirichlet lift (or None)."""
+        """Element kernels + assembly by entry; returns the Dirichlet lift (or None)."""
         cn = _i64(prob.elem_codes())
         conn = _i64(prob.conn)
         xyz = _f64(prob.coords)

@@ -5,10 +5,10 @@ contribution of every element, duplicates included, in element order.
 
 Three couplings are covered for each element kind:
   * periodic across a thin mesh (one element across the tied axis): two nodes of
-    one element share code numbers, so its entries meet in one slot;
+    one element share code numbers, so its entries meet in one entry;
   * periodic across a thicker mesh: elements that share no node share unknowns;
-  * one unknown tied over a whole face: more elements share it than one window of
-    64 colours holds.
+  * one unknown tied over a whole face: hundreds of elements add into the same
+    entries.
 """
 import numpy as np
 import pytest
@@ -151,8 +151,8 @@ def test_coupled_device_pattern_assembles_as_the_host_one(ctx, name):
 
 @pytest.mark.parametrize("name", ["cantilever_20x10x10", "elast_in_element", "heat_across"])
 def test_localize_batch_matches_the_oracle(ctx, name):
-    """gmatrix::localized of the oracle's element matrices: every boundary
-    element colours freely (codes <= 0 are no shared resource)"""
+    """gmatrix::localized of the oracle's element matrices, on meshes with many
+    boundary elements (codes <= 0 are skipped, not shared)"""
     prob = mesh.cantilever(20, 10, 10) if name.startswith("cantilever") else CASES[name]()
     cn = prob.elem_codes()
     ke = O.element_matrices(prob)
