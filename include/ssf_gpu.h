@@ -463,7 +463,10 @@ int ssf_spmv_bench(ssf_matrix *m, int precision, int reps, double *usec);
  * row patterns: 2, 4 or 8), blocks (at most 8192), idx16 (column form: 0 int32,
  * 1 16-bit tile or slice offsets, 2 SELL row patterns, 3 SELL stencil form: one
  * slot per column offset in every row, offered when the matrix has at most 32
- * offsets and its slices fit 32-bit byte offsets; a slot the row lacks holds 0
+ * offsets from the row itself, or at most 96 from the first row of the row's
+ * block of 2 or 3 rows (the unknowns of a node numbered together: 81 for hex8
+ * elasticity), costs at most 5 % more entries than the compact layout, and its
+ * slices fit 32-bit byte offsets; a slot the row lacks holds 0
  * and multiplies x at the row plus that offset, so its products equal the other
  * forms' for finite x only: an Inf or NaN in x makes NaN in rows that do not
  * couple to it), map (SELL slices to wavefronts: 0 a contiguous range per block, 1

@@ -242,38 +242,44 @@ bool build_sell_patterns(int64_t n, const std::vector<int> &off, const std::vect
 // SELL-64 layout of a full CSR: slices of 64 rows, each row's entries in
 // column order, padded to the slice's longest row (rounded up to SELL_V) with
 // value 0 at the row's own column; src maps a SELL entry to its CSR entry.
-// Column offsets (col - row) that occur in the matrix, ascending, when there are
-// at most SELL_STEN_MAX of them and a layout with one slot per offset in every
-// row holds at most 5 % more entries than the compact one (tot); else empty.
+// The column offsets col - (r - r % B) of the rows r that occur in the matrix,
+// ascending, when there are at most maxn of them and a layout with one slot per
+// offset in every row holds at most 5 % more entries than the compact one
+// (tot); else empty. B = 1 gives the offsets from the row itself; B = 2, 3 those
+// from the first row of the row's block (the unknowns of one node numbered
+// together), whose lists do not depend on the row's place in its block.
 static std::vector<int> stencil_offsets(int64_t n, const std::vector<int> &rowptr,
-                                        const std::vector<int> &col, int64_t tot)
+                                        const std::vector<int> &col, int64_t tot, int B,
+                                        int maxn)
 {
-  // each host thread collects the offsets of its rows (giving up past
-  // SELL_STEN_MAX), then the lists are merged
+  // each host thread collects the offsets of its rows (giving up past maxn),
+  // then the lists are merged
   const int T = host_threads();
   std::vector<std::vector<int>> part(T);
   std::atomic<bool> over{false};
   parallel_tasks(T, [&](int t) {
     std::vector<int> &u = part[t];
     const int64_t b = n * t / T, e = n * (t + 1) / T;
-    for (int64_t r = b; r < e && !over.load(std::memory_order_relaxed); r++)
+    for (int64_t r = b; r < e && !over.load(std::memory_order_relaxed); r++) {
+      const int base = (int)(r - r % B);
       for (int k = rowptr[r]; k < rowptr[r + 1]; k++) {
-        const int o = col[k] - (int)r;
+        const int o = col[k] - base;
         if (std::find(u.begin(), u.end(), o) == u.end()) {
-          if ((int)u.size() == SELL_STEN_MAX) {
+          if ((int)u.size() == maxn) {
             over = true;
             break;
           }
           u.push_back(o);
         }
       }
+    }
   });
   if (over) return {};
   std::vector<int> u;
   for (const std::vector<int> &pt : part) u.insert(u.end(), pt.begin(), pt.end());
   std::sort(u.begin(), u.end());
   u.erase(std::unique(u.begin(), u.end()), u.end());
-  if ((int)u.size() > SELL_STEN_MAX) return {};
+  if ((int)u.size() > maxn) return {};
   const int64_t w = ((int64_t)u.size() + SELL_V - 1) / SELL_V * SELL_V;
   if (u.empty() || w * 64 * ((n + 63) / 64) > tot + tot / 20) return {};
   return u;
@@ -281,7 +287,7 @@ static std::vector<int> stencil_offsets(int64_t n, const std::vector<int> &rowpt
 
 bool build_sell(int64_t n, const std::vector<int> &rowptr, const std::vector<int> &col,
                 std::vector<int> &off, std::vector<int> &scol, std::vector<int> &src,
-                std::vector<short> &scol16, std::vector<int> *sten)
+                std::vector<short> &scol16, std::vector<int> *sten, int *sten_b)
 {
   const int64_t nslice = (n + 63) / 64;
   off.assign(nslice + 1, 0);
@@ -297,9 +303,18 @@ bool build_sell(int64_t n, const std::vector<int> &rowptr, const std::vector<int
   }
   off[nslice] = (int)tot;
   std::vector<int> u;
+  int B = 1;
   if (sten) {
-    u = stencil_offsets(n, rowptr, col, tot);
+    // offsets from the row (at most 32, held in the kernel arguments), else from
+    // the row's block of 2 or 3 unknowns (at most 96, read from device memory)
+    u = stencil_offsets(n, rowptr, col, tot, 1, SELL_STEN_MAX);
+    for (int bb = 2; bb <= 3 && u.empty(); bb++)
+      if (n % bb == 0) {
+        u = stencil_offsets(n, rowptr, col, tot, bb, SELL_STEN_MAX_B);
+        if (!u.empty()) B = bb;
+      }
     *sten = u;
+    if (sten_b) *sten_b = u.empty() ? 1 : B;
   }
   if (!u.empty()) {
     // stencil form: every slice as wide as the offset list, slot k <-> u[k]
@@ -308,6 +323,7 @@ bool build_sell(int64_t n, const std::vector<int> &rowptr, const std::vector<int
     // form only while both fit in 4 GiB at FP64
     if (std::max<int64_t>(w * 64 * nslice, n) * (int64_t)sizeof(double) >= (int64_t(1) << 32)) {
       sten->clear();
+      if (sten_b) *sten_b = 1;
       u.clear();
     } else {
       for (int64_t sl = 0; sl <= nslice; sl++) off[sl] = (int)(sl * 64 * w);
@@ -318,11 +334,12 @@ bool build_sell(int64_t n, const std::vector<int> &rowptr, const std::vector<int
         for (int64_t sl = b; sl < e; sl++)
           for (int64_t l = 0; l < 64; l++) {
             const int64_t r = sl * 64 + l;
+            const int base = (int)(r - r % B);
             int k = r < n ? rowptr[r] : 0;
             const int kend = r < n ? rowptr[r + 1] : 0;
             for (int64_t q = 0; q < w; q++) {
               const int64_t s = off[sl] + (q / SELL_V) * 64 * SELL_V + l * SELL_V + q % SELL_V;
-              if (q < (int64_t)u.size() && k < kend && col[k] - (int)r == u[q]) {
+              if (q < (int64_t)u.size() && k < kend && col[k] - base == u[q]) {
                 scol[s] = col[k];
                 src[s] = k;
                 k++;
@@ -464,6 +481,7 @@ extern "C" int ssf_matrix_destroy(ssf_matrix *m)
   dfree(m->tiles, m->ctx->stream); dfree(m->col16, m->ctx->stream);
   dfree(m->sell_off, m->ctx->stream); dfree(m->sell_col, m->ctx->stream); dfree(m->sell_src, m->ctx->stream); dfree(m->sell_v64, m->ctx->stream); dfree(m->sell_v32, m->ctx->stream);
   dfree(m->sell_col16, m->ctx->stream);
+  dfree(m->sten_d, m->ctx->stream);
   dfree(m->sell_pid, m->ctx->stream); dfree(m->sell_dict, m->ctx->stream);
   dfree(m->sell_dict_off, m->ctx->stream);
   dfree(m->sell_lo, m->ctx->stream); dfree(m->sell_up, m->ctx->stream);
@@ -676,7 +694,9 @@ int ssf::matrix_create(ssf_ctx *ctx, int storage, int64_t n, const int64_t *adr,
   const bool have_tiles = stream_tiles && build_tiles(n, rowptr, col, tiles, col16);
   std::vector<int> soff, scol, ssrc;
   std::vector<short> scol16;
-  const bool have_sell = n > 0 && build_sell(n, rowptr, col, soff, scol, ssrc, scol16, &m->sten);
+  const bool have_sell = n > 0 && build_sell(n, rowptr, col, soff, scol, ssrc, scol16, &m->sten,
+                                             &m->sten_b);
+  std::vector<int> sten_pad;  // (copied to the device below; lives until the final sync)
   hipError_t e = hipSuccess;
   auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
   chk(dalloc(&m->rowptr, n + 1, ctx->stream));
@@ -701,7 +721,14 @@ int ssf::matrix_create(ssf_ctx *ctx, int storage, int64_t n, const int64_t *adr,
     chk(dalloc(&m->sell_col, m->sell_tot, ctx->stream));
     chk(dalloc(&m->sell_src, m->sell_tot, ctx->stream));
     chk(dalloc(&m->sell_v64, m->sell_tot, ctx->stream));
+    // the offsets of every slot of a row: the list padded with 0 to the slot count
+    sten_pad = m->sten;
+    sten_pad.resize((m->sten.size() + SELL_V - 1) / SELL_V * SELL_V, 0);
+    if (!sten_pad.empty()) chk(dalloc(&m->sten_d, sten_pad.size(), ctx->stream));
     if (e == hipSuccess) {
+      if (!sten_pad.empty())
+        chk(hipMemcpyAsync(m->sten_d, sten_pad.data(), sten_pad.size() * sizeof(int),
+                           hipMemcpyHostToDevice, ctx->stream));
       chk(hipMemcpyAsync(m->sell_off, soff.data(), soff.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
       chk(hipMemcpyAsync(m->sell_col, scol.data(), scol.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
       chk(hipMemcpyAsync(m->sell_src, ssrc.data(), ssrc.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));

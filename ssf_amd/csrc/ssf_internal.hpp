@@ -131,7 +131,8 @@ struct SpmvTune {
 };
 
 // Device view of a full CSR matrix and its stream tiling.
-constexpr int SELL_STEN_MAX = 32;  // slots of the stencil form of the SELL layout
+constexpr int SELL_STEN_MAX = 32;    // slots of the stencil form held in the kernel arguments
+constexpr int SELL_STEN_MAX_B = 96;  // slots of the block stencil form (offsets in device memory)
 
 struct CsrDev {
   int n = 0;
@@ -153,10 +154,15 @@ struct CsrDev {
   const int *sell_dict = nullptr;
   const int *sell_dict_off = nullptr;
   // stencil form: every row of the layout has the same slots, slot k at column
-  // row + sten[k] (sten ascending; a slot the row lacks holds value 0 at the
-  // row's own column), so the columns need no stream and no dictionary
+  // base + sten[k] with base = row - row % sten_b (sten ascending; a slot the row
+  // lacks holds value 0), so the columns need no stream and no dictionary.
+  // sten_b = 1: offsets from the row itself, at most SELL_STEN_MAX of them, in
+  // sten; sten_b = 2, 3: offsets from the first row of the row's block of
+  // sten_b rows (the unknowns of one node), read from sten_d
   int sten_n = 0;
+  int sten_b = 1;
   int sten[SELL_STEN_MAX] = {};
+  const int *sten_d = nullptr;
   const double *sell_v64 = nullptr;
   const float *sell_v32 = nullptr;
   int slice_map = 1;  // SpmvTune::map of the launch
@@ -242,6 +248,8 @@ struct ssf_matrix {
   int *sell_dict = nullptr, *sell_dict_off = nullptr;
   int sell_npat = 0;
   std::vector<int> sten;  // stencil form of the SELL layout (empty: not stencil-aligned)
+  int sten_b = 1;         // its block: offsets from row - row % sten_b
+  int *sten_d = nullptr;  // device copy of sten
   double *sell_v64 = nullptr;
   float *sell_v32 = nullptr;
   // SELL-only matrices (laid out on the device, no full CSR): SELL positions of
@@ -281,7 +289,9 @@ struct ssf_matrix {
     c.sell_dict = sell_dict;
     c.sell_dict_off = sell_dict_off;
     c.sten_n = (int)sten.size();
+    c.sten_b = sten_b;
     for (size_t k = 0; k < sten.size() && k < (size_t)ssf::SELL_STEN_MAX; k++) c.sten[k] = sten[k];
+    c.sten_d = sten_d;
     c.sell_v64 = sell_v64;
     c.sell_v32 = sell_v32;
     return c;
@@ -337,7 +347,7 @@ void matrix_keep_sell_only(ssf_matrix *m);
 bool build_sell(int64_t n, const std::vector<int> &rowptr, const std::vector<int> &col,
                 std::vector<int> &off, std::vector<int> &scol, std::vector<int> &src,
                 std::vector<short> &scol16,
-                std::vector<int> *sten = nullptr);
+                std::vector<int> *sten = nullptr, int *sten_b = nullptr);
 // row patterns of a SELL-64 layout (off, scol from build_sell): per row of the
 // layout its column offsets from min(row, n-1), deduplicated into dict (each
 // pattern's entries start at a multiple of SELL_V, dict_off[p]); pid[r] = the

@@ -386,6 +386,78 @@ __global__ __launch_bounds__(TPB) void k_spmv_sell_sten(CsrDev A, const T *__res
   }
 }
 
+// NG groups of SELL_V slots of one row in the block stencil form: all value
+// loads, then all gathers (slot k at column base + so[k], clamped into range),
+// then the products in slot (= column) order. vb: byte offset of the first
+// group's values in sval; so: the groups' offsets (wave-uniform: scalar loads).
+template <typename T, int NG>
+__device__ __forceinline__ T sten_groups(const T *__restrict__ sval, unsigned vb,
+                                         const int *__restrict__ so, const T *__restrict__ x,
+                                         int base, int nm1, T sum)
+{
+  typedef T tv __attribute__((ext_vector_type(SELL_V)));
+  tv v[NG];
+  T g[NG][SELL_V];
+#pragma unroll
+  for (int k = 0; k < NG; k++) v[k] = ldnt(at_byte<tv>(sval, vb + k * 64 * SELL_V * sizeof(T)));
+#pragma unroll
+  for (int k = 0; k < NG; k++)
+#pragma unroll
+    for (int u = 0; u < SELL_V; u++) {
+      const int c = min(max(base + so[k * SELL_V + u], 0), nm1);
+      g[k][u] = *at_byte<T>(x, (unsigned)c * sizeof(T));
+    }
+#pragma unroll
+  for (int k = 0; k < NG; k++)
+#pragma unroll
+    for (int u = 0; u < SELL_V; u++) sum += v[k][u] * g[k][u];
+  return sum;
+}
+
+// SELL-64 in the block stencil form: slot k of row r sits at column
+// (r - r % B) + A.sten_d[k], the offsets counted from the first row of the row's
+// block of B rows. With B = 3 and the unknowns of a node numbered together (3D
+// elasticity), every row's columns are its node's 27 neighbour nodes times 3
+// unknowns: 81 slots, one list for all rows, where the offsets from the row
+// itself would differ with r % 3. A row takes D groups per round; the sum is in
+// slot order, which is column order, so the products equal the column forms'
+// bit for bit (absent slots hold 0 and add exact zeros for finite x).
+template <typename T, int KIND, int D, int B>
+__global__ __launch_bounds__(TPB) void k_spmv_sell_bsten(CsrDev A, const T *__restrict__ x,
+                                                         T *__restrict__ y, const T *__restrict__ b,
+                                                         T *__restrict__ partial, const int *done,
+                                                         long long *iter)
+{
+  if (done && *done) return;
+  if (iter && blockIdx.x == 0 && threadIdx.x == 0) *iter += 1;
+  const T *__restrict__ sval = sell_values<T>(A);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), lane = threadIdx.x % 64;
+  int sfirst, slast, sstep;
+  wave_slices(A.nslice, A.slice_map, wave, sfirst, slast, sstep);
+  const int nm1 = A.n - 1;
+  const int ng = (A.sten_n + SELL_V - 1) / SELL_V;  // groups per row
+  const int *__restrict__ so = A.sten_d;
+  T acc = T(0);
+  for (int sl = sfirst; sl < slast; sl += sstep) {
+    const unsigned vb = (unsigned)(sl * 64 * ng * SELL_V + lane * SELL_V) * sizeof(T);
+    const int rb = sl * 64 + lane;
+    const int base = B == 1 ? rb : rb - rb % B;
+    T sum = T(0);
+    int g0 = 0;
+    for (; g0 + D <= ng; g0 += D)
+      sum = sten_groups<T, D>(sval, vb + g0 * 64 * SELL_V * sizeof(T), so + g0 * SELL_V, x, base,
+                              nm1, sum);
+    for (; g0 < ng; g0++)
+      sum = sten_groups<T, 1>(sval, vb + g0 * 64 * SELL_V * sizeof(T), so + g0 * SELL_V, x, base,
+                              nm1, sum);
+    if (rb < A.n) finish_row<T, 1, KIND>(rb, sum, x, y, b, acc);
+  }
+  if (KIND != 0) {
+    const T t = block_sum(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+  }
+}
+
 // ev0/ev1 (both or neither): the launch's own start and end timestamps
 // (hipExtLaunchKernelGGL), i.e. the kernel's duration without the dispatch gap
 template <typename T, int KIND>
@@ -403,7 +475,12 @@ static hipError_t spmv_k(const SpmvTune &t, const CsrDev &A0, const T *val, cons
   } while (0)
 #define SSF_VEC(G) SSF_LAUNCH((k_spmv_vec<T, G, KIND>), A, val, x, y, b, partial, done, iter)
 #define SSF_STR(I, G) SSF_LAUNCH((k_spmv_stream<T, I, G, KIND>), A, val, x, y, b, partial, done, iter)
-  if (t.format == 2 && A.nslice > 0 && t.idx16 == 3 && A.sten_n > 0) {
+  if (t.format == 2 && A.nslice > 0 && t.idx16 == 3 && A.sten_n > 0 &&
+      (A.sten_b != 1 || A.sten_n > SELL_STEN_MAX)) {
+    if (A.sten_b == 3) SSF_LAUNCH((k_spmv_sell_bsten<T, KIND, 7, 3>), A, x, y, b, partial, done, iter);
+    else if (A.sten_b == 2) SSF_LAUNCH((k_spmv_sell_bsten<T, KIND, 5, 2>), A, x, y, b, partial, done, iter);
+    else SSF_LAUNCH((k_spmv_sell_bsten<T, KIND, 7, 1>), A, x, y, b, partial, done, iter);
+  } else if (t.format == 2 && A.nslice > 0 && t.idx16 == 3 && A.sten_n > 0) {
     switch ((A.sten_n + SELL_V - 1) / SELL_V) {
 #define SSF_STEN(G) case G: SSF_LAUNCH((k_spmv_sell_sten<T, KIND, G>), A, x, y, b, partial, done, iter); break;
       SSF_STEN(1) SSF_STEN(2) SSF_STEN(3) SSF_STEN(4) SSF_STEN(5) SSF_STEN(6) SSF_STEN(7)

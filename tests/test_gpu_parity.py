@@ -266,12 +266,49 @@ def test_sell_stencil_form(ctx):
         xm, im = m.solve(rhs, ni=20000, res=1e-22, precision=gpu.MIXED)
         assert im.status == 1 and np.linalg.norm(xm - x0) <= 1e-8 * np.linalg.norm(x0)
         m.close()
-    prob = mesh.plate(20, 7, seed=11, nrhs=1)
+    # a pattern whose rows share no offset list: no stencil form, and asking for
+    # it is refused
+    n = 3000
+    rng = np.random.default_rng(12)
+    low = [sorted(set(rng.integers(0, i + 1, 4).tolist()) | {i}) for i in range(n)]
+    adr = np.zeros(n + 1, np.int64)
+    adr[1:] = np.cumsum([len(r) for r in low])
+    ci = np.concatenate([np.array(r, np.int64) for r in low])
+    m = gpu.Matrix.from_csr(ctx, gpu.STORAGE_SCR, adr, ci)
+    m.set_values(rng.standard_normal(ci.shape[0]) + (ci == np.repeat(np.arange(n), np.diff(adr))) * 50)
+    with pytest.raises(gpu.SsfError):
+        m.set_tune(gpu.FP64, (2, 0, 64, 3))
+    m.close()
+
+
+@pytest.mark.parametrize("name", ["cantilever", "plate"])
+def test_sell_block_stencil_form(ctx, name):
+    """Block stencil form: with the unknowns of a node numbered together (3 for
+    hex8 elasticity, 2 for the quad4 plate), the offsets of every row counted from
+    the first row of its node (27 neighbour nodes x 3 = 81, 9 x 2 = 18) form one
+    list, so the layout gives each its own slot. Products equal the column forms'
+    bit for bit at every block count and slice map, and the FP64 and mixed solves
+    agree with the oracle."""
+    prob = mesh.cantilever(8, 3, 3) if name == "cantilever" else mesh.plate(20, 7, seed=11, nrhs=1)
     adr, ci, a, rhs = O.assemble_problem(prob, "scr")
     m = gpu.Matrix.from_csr(ctx, gpu.STORAGE_SCR, adr, ci)
     m.set_values(a)
-    with pytest.raises(gpu.SsfError):
-        m.set_tune(gpu.FP64, (2, 0, 64, 3))
+    b = np.random.default_rng(7).standard_normal(prob.n)
+    m.set_tune(gpu.FP64, (2, 0, 64, 0))
+    y0 = m.gmxv(b)
+    assert rel(y0, O.mxv_scr(adr, ci, a, b)) <= 1e-14
+    x0, i0 = m.solve(rhs[0], ni=50000, res=1e-22)
+    for nb in (1, 3, 64, 512):
+        for mp in (0, 1):
+            m.set_tune(gpu.FP64, (2, 0, nb, 3, mp))
+            assert np.array_equal(m.gmxv(b), y0), (nb, mp)
+    x3, i3 = m.solve(rhs[0], ni=50000, res=1e-22)
+    assert i3.iterations == i0.iterations and np.array_equal(x3, x0)
+    xo, *_ = O.scr_cg_prec(adr, ci, a, rhs[0], 50000, 1e-30)
+    assert rel(x3, xo) <= 1e-8
+    m.set_tune(gpu.MIXED, (2, 0, 64, 3, 1))
+    xm, im = m.solve(rhs[0], ni=50000, res=1e-22, precision=gpu.MIXED)
+    assert im.status in (1, 4) and rel(xm, xo) <= 1e-8, rel(xm, xo)
     m.close()
 
 
