@@ -460,7 +460,9 @@ int ssf_spmv_bench(ssf_matrix *m, int precision, int reps, double *usec);
  * on this matrix and keep the fastest for later solves. tune_out (may be NULL)
  * receives 5 values {format (0 row-vector, 1 stream tiles, 2 SELL-64 slices),
  * lanes per row (formats 0, 1) or value loads in flight per lane (format 2 with
- * row patterns: 2, 4 or 8), blocks (at most 8192), idx16 (column form: 0 int32,
+ * row patterns: 2, 4 or 8; stencil form: 3 or 4 takes that many groups of 4
+ * slots per round, other values all of a row's groups (7 per round for blocks
+ * of 3 rows, 5 for blocks of 2)), blocks (at most 8192), idx16 (column form: 0 int32,
  * 1 16-bit tile or slice offsets, 2 SELL row patterns, 3 SELL stencil form: one
  * slot per column offset in every row, offered when the matrix has at most 32
  * offsets from the row itself, or at most 96 from the first row of the row's

@@ -1803,13 +1803,17 @@ extern "C" int ssf_matrix_autotune(ssf_matrix *m, int precision, int reps, int32
   if (m->nslice > 0)
     for (int b : {256, 512, 1024, 2048})
       for (int i16 = 0; i16 <= 3; i16++)
-        for (int dep : {2, 4, 8})  // row patterns: value loads in flight per lane
+        // row patterns: value loads in flight per lane (2, 4, 8); stencil form:
+        // 3 or 4 groups of slots per round, or the default (2)
+        for (int dep : {2, 3, 4, 8})
           for (int map : {0, 1}) {
             if (b > std::max(8, (m->nslice + 3) / 4)) continue;
             if (i16 == 1 && !m->sell_col16) continue;
             if (i16 == 2 && !m->sell_pid) continue;
             if (i16 == 3 && m->sten.empty()) continue;
-            if (i16 != 2 && dep != 2) continue;
+            if (i16 == 2 && dep == 3) continue;
+            if (i16 == 3 && dep == 8) continue;
+            if (i16 < 2 && dep != 2) continue;
             SpmvTune t;
             t.format = 2;
             t.lanes = dep;

@@ -476,10 +476,23 @@ static hipError_t spmv_k(const SpmvTune &t, const CsrDev &A0, const T *val, cons
 #define SSF_VEC(G) SSF_LAUNCH((k_spmv_vec<T, G, KIND>), A, val, x, y, b, partial, done, iter)
 #define SSF_STR(I, G) SSF_LAUNCH((k_spmv_stream<T, I, G, KIND>), A, val, x, y, b, partial, done, iter)
   if (t.format == 2 && A.nslice > 0 && t.idx16 == 3 && A.sten_n > 0 &&
-      (A.sten_b != 1 || A.sten_n > SELL_STEN_MAX)) {
-    if (A.sten_b == 3) SSF_LAUNCH((k_spmv_sell_bsten<T, KIND, 7, 3>), A, x, y, b, partial, done, iter);
-    else if (A.sten_b == 2) SSF_LAUNCH((k_spmv_sell_bsten<T, KIND, 5, 2>), A, x, y, b, partial, done, iter);
-    else SSF_LAUNCH((k_spmv_sell_bsten<T, KIND, 7, 1>), A, x, y, b, partial, done, iter);
+      (A.sten_b != 1 || A.sten_n > SELL_STEN_MAX || t.lanes == 3 || t.lanes == 4)) {
+    // groups per round: t.lanes 3 or 4 takes that many, else 7 (5 for blocks of 2)
+#define SSF_BSTEN(D, B) SSF_LAUNCH((k_spmv_sell_bsten<T, KIND, D, B>), A, x, y, b, partial, done, iter)
+    if (A.sten_b == 3) {
+      if (t.lanes == 3) SSF_BSTEN(3, 3);
+      else if (t.lanes == 4) SSF_BSTEN(4, 3);
+      else SSF_BSTEN(7, 3);
+    } else if (A.sten_b == 2) {
+      if (t.lanes == 3) SSF_BSTEN(3, 2);
+      else if (t.lanes == 4) SSF_BSTEN(4, 2);
+      else SSF_BSTEN(5, 2);
+    } else {
+      if (t.lanes == 3) SSF_BSTEN(3, 1);
+      else if (t.lanes == 4) SSF_BSTEN(4, 1);
+      else SSF_BSTEN(7, 1);
+    }
+#undef SSF_BSTEN
   } else if (t.format == 2 && A.nslice > 0 && t.idx16 == 3 && A.sten_n > 0) {
     switch ((A.sten_n + SELL_V - 1) / SELL_V) {
 #define SSF_STEN(G) case G: SSF_LAUNCH((k_spmv_sell_sten<T, KIND, G>), A, x, y, b, partial, done, iter); break;

@@ -257,8 +257,9 @@ def test_sell_stencil_form(ctx):
         y0 = m.gmxv(b)
         x0, i0 = m.solve(rhs, ni=20000, res=1e-22)
         for nb in (1, 3, 64):
-            m.set_tune(gpu.FP64, (2, 0, nb, 3))
-            assert np.array_equal(m.gmxv(b), y0), (prob.name, nb)
+            for dep in (0, 3, 4):  # one round per row, or rounds of 3 or 4 groups
+                m.set_tune(gpu.FP64, (2, dep, nb, 3))
+                assert np.array_equal(m.gmxv(b), y0), (prob.name, nb, dep)
         x3, i3 = m.solve(rhs, ni=20000, res=1e-22)
         assert i3.iterations == i0.iterations and np.array_equal(x3, x0)
         # the mixed solve on the stencil form
@@ -300,8 +301,9 @@ def test_sell_block_stencil_form(ctx, name):
     x0, i0 = m.solve(rhs[0], ni=50000, res=1e-22)
     for nb in (1, 3, 64, 512):
         for mp in (0, 1):
-            m.set_tune(gpu.FP64, (2, 0, nb, 3, mp))
-            assert np.array_equal(m.gmxv(b), y0), (nb, mp)
+            for dep in (0, 3, 4):  # groups of slots per round: all, 3, 4
+                m.set_tune(gpu.FP64, (2, dep, nb, 3, mp))
+                assert np.array_equal(m.gmxv(b), y0), (nb, mp, dep)
     x3, i3 = m.solve(rhs[0], ni=50000, res=1e-22)
     assert i3.iterations == i0.iterations and np.array_equal(x3, x0)
     xo, *_ = O.scr_cg_prec(adr, ci, a, rhs[0], 50000, 1e-30)
