@@ -824,6 +824,9 @@ int ssf::matrix_create_sell_dev(ssf_ctx *ctx, int64_t n, int *d_adr, int *d_ci, 
     m->sell_col16 = L.col16;
     m->sell_lo = L.lo;
     m->sell_up = L.up;
+    m->sten = L.sten_h;  // stencil form (no column arrays) when the layout took it
+    m->sten_b = L.sten ? L.sten_b : 1;
+    m->sten_d = L.sten;
     chk(dalloc(&m->a_ref, negm, s));
     chk(dalloc(&m->sell_v64, m->sell_tot, s));
     chk(dalloc(&m->diag, n, s));
@@ -843,7 +846,7 @@ int ssf::matrix_create_sell_dev(ssf_ctx *ctx, int64_t n, int *d_adr, int *d_ci, 
   SpmvTune t;
   t.format = 2;
   t.blocks = (int)((std::max<int64_t>(8, std::min<int64_t>(2048, (m->nslice + 3) / 4)) + 7) / 8 * 8);
-  t.idx16 = m->sell_col16 ? 1 : 0;
+  t.idx16 = !m->sten.empty() ? 3 : m->sell_col16 ? 1 : 0;
   m->tune64 = m->tune32 = t;
   *out = m;
   return SSF_OK;
@@ -1808,6 +1811,7 @@ extern "C" int ssf_matrix_autotune(ssf_matrix *m, int precision, int reps, int32
         for (int dep : {2, 3, 4, 8})
           for (int map : {0, 1}) {
             if (b > std::max(8, (m->nslice + 3) / 4)) continue;
+            if (i16 == 0 && !m->sell_col) continue;
             if (i16 == 1 && !m->sell_col16) continue;
             if (i16 == 2 && !m->sell_pid) continue;
             if (i16 == 3 && m->sten.empty()) continue;
@@ -1872,14 +1876,15 @@ extern "C" int ssf_matrix_set_tune(ssf_matrix *m, int precision, const int32_t *
               "ssf_matrix_set_tune: SELL depth must be in [0, 8] (value loads in flight per "
               "lane of the row-pattern form: 8 or more, 4 or more, else 2)");
   SSF_CHECK(t.blocks >= 1 && t.blocks <= 8192, SSF_EINVAL, "ssf_matrix_set_tune: blocks out of [1, 8192]");
-  SSF_CHECK(t.idx16 == 0 || (t.idx16 == 1 && precision && t.format == 1 && m->col16) ||
+  SSF_CHECK((t.idx16 == 0 && (t.format != 2 || m->sell_col)) ||
+                (t.idx16 == 1 && precision && t.format == 1 && m->col16) ||
                 (t.idx16 == 1 && t.format == 2 && m->sell_col16) ||
                 (t.idx16 == 2 && t.format == 2 && m->sell_pid) ||
                 (t.idx16 == 3 && t.format == 2 && !m->sten.empty()),
             SSF_EINVAL,
-            "ssf_matrix_set_tune: index form 1 (16-bit columns) for the FP32 stream operator or "
-            "the SELL operator of a matrix that has them, 2 (row patterns) or 3 (stencil form) "
-            "for a SELL operator that has them");
+            "ssf_matrix_set_tune: index form 0 (int32 columns), 1 (16-bit columns) for the FP32 "
+            "stream operator or the SELL operator of a matrix that has them, 2 (row patterns) or "
+            "3 (stencil form) for a SELL operator that has them");
   if (precision) m->tune32 = t;
   else m->tune64 = t;
   return SSF_OK;

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <functional>
+#include <vector>
 
 #include "ssf_internal.hpp"
 
@@ -74,6 +75,12 @@ struct SellLayout {
   int64_t tot = 0;
   int *off = nullptr, *col = nullptr, *lo = nullptr, *up = nullptr;
   short *col16 = nullptr;
+  // stencil form (no column arrays): slot k of row r at column
+  // (r - r % sten_b) + sten[k]; sten on the device padded with 0 to the slot
+  // count, sten_h the list on the host
+  int *sten = nullptr;
+  int sten_b = 1;
+  std::vector<int> sten_h;
 };
 hipError_t sell_layout_build(int n, int64_t negm, const int *adr, const int *ci, hipStream_t s,
                              SellLayout *out);

@@ -11,8 +11,12 @@ namespace ssf {
 // matrix data.
 struct SlotDesc {
   const int *soff;      // slice offsets, nslice + 1
-  const int *scol;      // 32-bit columns (used when scol16 is null)
+  const int *scol;      // 32-bit columns (used when scol16 and sten are null)
   const short *scol16;  // 16-bit offsets from the slice's first row
+  // stencil form (when non-null): slot k of row r at column
+  // (r - r % sten_b) + sten[k], sten padded with 0 to the slot count
+  const int *sten;
+  int sten_n, sten_b;
   const double *sval;
   const double *diag;
   const double *y;

@@ -120,12 +120,13 @@ struct ssf_server {
 
 namespace {
 
-// device bytes of a resident request: SELL slices (offsets, 16-bit columns or
-// 32-bit ones, FP64 values; FP32 values in the mixed form), diagonal, the solve
-// vectors (six FP64 ones, or three FP64 and six FP32 ones)
-int64_t resident_bytes(int64_t n, int64_t tot, bool col16, bool mixed)
+// device bytes of a resident request: SELL slices (offsets, colbytes per entry of
+// columns: 0 in the stencil form, 2 or 4; FP64 values; FP32 values in the mixed
+// form), diagonal, the solve vectors (six FP64 ones, or three FP64 and six FP32
+// ones)
+int64_t resident_bytes(int64_t n, int64_t tot, int colbytes, bool mixed)
 {
-  return 4 * ((n + 63) / 64 + 1) + (col16 ? 10 : 12) * tot + (mixed ? 4 * tot : 0) + 8 * n +
+  return 4 * ((n + 63) / 64 + 1) + (8 + colbytes) * tot + (mixed ? 4 * tot : 0) + 8 * n +
          48 * n + (1 << 16);
 }
 
@@ -135,7 +136,7 @@ int64_t resident_bytes(int64_t n, int64_t tot, bool col16, bool mixed)
 // build and the element-matrix buffer
 int64_t prep_bytes(const Request &q, int64_t negm, int64_t tot, bool mixed)
 {
-  return resident_bytes(q.n, tot, false, mixed) + 2 * tot + 4 * (q.n + 1) + 4 * negm + 28 * negm +
+  return resident_bytes(q.n, tot, 4, mixed) + 2 * tot + 4 * (q.n + 1) + 4 * negm + 28 * negm +
          24 * q.n + 16 * negm + 8 * negm + asm_bytes(q.ne, (int)q.ndofe, negm) + 8 * q.ne;
 }
 
@@ -268,7 +269,8 @@ void prep_main(ssf_server *S, int w)
         dt_assemble = std::chrono::duration<double>(clk::now() - ta0).count();
         {  // only the SELL solve structures are left: keep their bytes reserved
           const int64_t keep =
-              resident_bytes(q->n, q->m->sell_tot, q->m->sell_col16 != nullptr, mixed);
+              resident_bytes(q->n, q->m->sell_tot,
+                             !q->m->sten.empty() ? 0 : q->m->sell_col16 ? 2 : 4, mixed);
           std::unique_lock<std::mutex> lk(S->mu);
           if (keep < q->bytes) {
             S->used -= q->bytes - keep;
@@ -381,6 +383,9 @@ hipError_t upload_tables(ssf_server *S, const std::vector<char> &fresh)
     D.soff = m->sell_off;
     D.scol = m->sell_col;
     D.scol16 = m->sell_col16;
+    D.sten = m->sten.empty() ? nullptr : m->sten_d;
+    D.sten_n = (int)m->sten.size();
+    D.sten_b = m->sten_b;
     D.sval = m->sell_v64;
     D.diag = m->diag;
     D.y = q->vec;

@@ -62,7 +62,13 @@ def test_device_pattern_matrix_assembles_and_solves_as_the_host_one(ctx, name):
     if lh is not None:
         assert np.array_equal(lh, ld)
     rhs = prob.rhs[0] + (lh if lh is not None else 0.0)
+    # the device layout takes the stencil form (no column arrays) where the host
+    # one does; then both run it, else both the 16-bit column form
     tune = [2, 8, 256, 1]
+    try:
+        md.set_tune(gpu.FP64, tune)
+    except gpu.SsfError:
+        tune = [2, 0, 256, 3]
     mh.set_tune(gpu.FP64, tune)
     md.set_tune(gpu.FP64, tune)
     xh, ih = mh.solve(rhs, ni=20000, res=1e-20)
@@ -88,3 +94,47 @@ def test_device_pattern_refuses_an_unknown_no_element_refers_to(ctx):
     prob = mesh.heat_cube(4, seed=56)
     with pytest.raises(gpu.SsfError):
         gpu.Matrix.from_elements_device(ctx, prob.n + 1, prob.elem_codes())
+
+
+@pytest.mark.parametrize("name", ["heat_17", "cantilever", "quad4", "heat_9"])
+def test_device_layout_stencil_form_matches_the_host_layout(ctx, name):
+    """The device-built layout decides on the stencil form (offsets from the row,
+    or from its node's block of 2 or 3 rows) by the host layout's rule: when the
+    host matrix has it, the device matrix has it with the same slots, and the
+    products are equal bit for bit; else the device matrix keeps its columns."""
+    prob = {"heat_17": lambda: mesh.heat_cube(17, seed=57),
+            "cantilever": lambda: mesh.cantilever(8, 3, 3),  # blocks of 3, 2.0 % of padding
+            "quad4": CASES["quad4"], "heat_9": CASES["hex8_heat"]}[name]()
+    cn = prob.elem_codes()
+    mh = gpu.Matrix.from_elements(ctx, gpu.STORAGE_SCR, prob.n, cn)
+    md = gpu.Matrix.from_elements_device(ctx, prob.n, cn)
+    mh.assemble(prob, lift=False)
+    md.assemble(prob, lift=False)
+    b = np.random.default_rng(58).standard_normal(prob.n)
+    try:
+        mh.set_tune(gpu.FP64, (2, 0, 64, 3))
+        host_sten = True
+    except gpu.SsfError:
+        host_sten = False
+    try:
+        md.set_tune(gpu.FP64, (2, 0, 64, 3))
+        dev_sten = True
+    except gpu.SsfError:
+        dev_sten = False
+    assert host_sten == dev_sten, (name, host_sten, dev_sten)
+    if name in ("heat_17", "cantilever", "quad4", "heat_9"):
+        assert dev_sten, name
+    if dev_sten:
+        for dep in (0, 3):
+            for mp in (0, 1):
+                mh.set_tune(gpu.FP64, (2, dep, 64, 3, mp))
+                md.set_tune(gpu.FP64, (2, dep, 64, 3, mp))
+                assert np.array_equal(md.gmxv(b), mh.gmxv(b)), (name, dep, mp)
+        a = mh.get_values()
+        adr, ci = mh.pattern()
+        ref = O.mxv_scr(adr, ci, a, b)
+        assert np.linalg.norm(md.gmxv(b) - ref) <= 1e-14 * np.linalg.norm(ref)
+        with pytest.raises(gpu.SsfError):  # no column arrays on the device matrix
+            md.set_tune(gpu.FP64, (2, 0, 64, 0))
+    mh.close()
+    md.close()
