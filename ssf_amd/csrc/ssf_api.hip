@@ -1816,7 +1816,9 @@ extern "C" int ssf_matrix_autotune(ssf_matrix *m, int precision, int reps, int32
             if (i16 == 2 && !m->sell_pid) continue;
             if (i16 == 3 && m->sten.empty()) continue;
             if (i16 == 2 && dep == 3) continue;
-            if (i16 == 3 && dep == 8) continue;
+            // (rounds of 3 or 4 groups pay off for the long rows of the block
+            // form; on the 27-slot heat rows they measured 1-2 us slower)
+            if (i16 == 3 && (dep == 8 || (m->sten_b == 1 && dep != 2))) continue;
             if (i16 < 2 && dep != 2) continue;
             SpmvTune t;
             t.format = 2;

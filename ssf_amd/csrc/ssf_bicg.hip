@@ -43,6 +43,10 @@ struct DualSell {
   int64_t tot = 0;
   int *off = nullptr, *col = nullptr;
   short *col16 = nullptr;              // col - 64 * slice, when every offset fits
+  // stencil form of the union layout (then no column array): slot k of row r at
+  // column (r - r % sten_b) + sten[k], sten padded with 0 to the slot count
+  int *sten = nullptr;
+  int sten_n = 0, sten_b = 1;
   int *src_a = nullptr, *src_t = nullptr;  // CSR slot of a_ij / a_ji, -1 if none
   double *va = nullptr, *vt = nullptr;
   uint64_t version = ~0ULL;            // values_version the values were taken at
@@ -59,13 +63,17 @@ struct DualDev {
   const int *off, *col;
   const short *col16;
   const double *va, *vt;
+  const int *sten;
+  int sten_b;
 };
 
 namespace {
 
 // DUAL: (p, pp) = (A d, A^T dd) from the pairs dv = (d, dd), one 16-byte gather
-// per entry, and the block partial of dd.p; otherwise q = A x (x, q plain)
-template <bool I16, bool DUAL>
+// per entry, and the block partial of dd.p; otherwise q = A x (x, q plain).
+// IDX: 0 int32 columns, 1 16-bit offsets from the slice's first row, 2 stencil
+// form (the values alone; columns from the row's block and the slot's offset)
+template <int IDX, bool DUAL>
 __global__ __launch_bounds__(TPB) void k_dual_spmv(DualDev L, const double2 *__restrict__ dv,
                                                    double2 *__restrict__ pv,
                                                    const double *__restrict__ x,
@@ -94,7 +102,12 @@ __global__ __launch_bounds__(TPB) void k_dual_spmv(DualDev L, const double2 *__r
       tv vt;
       if (DUAL) vt = ldnt((const tv *)(L.vt + k));
       int c[SELL_V];
-      if (I16) {
+      if (IDX == 2) {
+        const int r = sl * 64 + lane;
+        const int base = r - r % L.sten_b;
+#pragma unroll
+        for (int u = 0; u < SELL_V; u++) c[u] = min(max(base + L.sten[q + u], 0), L.n - 1);
+      } else if (IDX == 1) {
         const sv cc = ldnt((const sv *)(L.col16 + k));
 #pragma unroll
         for (int u = 0; u < SELL_V; u++) c[u] = sl * 64 + cc[u];
@@ -350,7 +363,7 @@ bool union_pattern(int64_t n, const std::vector<int64_t> &adr, const std::vector
 void dual_free(DualSell *L, hipStream_t s)
 {
   if (!L) return;
-  dfree(L->off, s); dfree(L->col, s); dfree(L->col16, s);
+  dfree(L->off, s); dfree(L->col, s); dfree(L->col16, s); dfree(L->sten, s);
   dfree(L->src_a, s); dfree(L->src_t, s); dfree(L->va, s); dfree(L->vt, s);
   dfree(L->rv, s); dfree(L->dv, s); dfree(L->pv, s); dfree(L->part, s); dfree(L->st, s);
   if (L->st_host) (void)hipHostFree(L->st_host);
@@ -369,11 +382,14 @@ static int dual_prepare(ssf_matrix *m)
       if (rc) return rc;
     }
     std::vector<int> urow, ucol, ua, ut;
-    std::vector<int> off, scol, src;
+    std::vector<int> off, scol, src, sten;
     std::vector<short> scol16;
+    int sten_b = 1;
     SSF_CHECK(union_pattern(n, adr, ci, urow, ucol, ua, ut) &&
-                  build_sell(n, urow, ucol, off, scol, src, scol16),
+                  build_sell(n, urow, ucol, off, scol, src, scol16, &sten, &sten_b),
               SSF_EUNSUP, "bicg: 2^31 entries or more in the union of A and A^T");
+    std::vector<int> sten_pad(sten);
+    sten_pad.resize((sten.size() + SELL_V - 1) / SELL_V * SELL_V, 0);
     const int64_t tot = off.back();
     std::vector<int> sa(tot), st(tot);
     parallel_for(tot, [&](int64_t b, int64_t e) {
@@ -397,8 +413,15 @@ static int dual_prepare(ssf_matrix *m)
         c(hipMemcpyAsync(*dst, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice, s));
     };
     up(&L->off, off);
-    if (!scol16.empty()) up(&L->col16, scol16);
-    else up(&L->col, scol);
+    if (!sten.empty()) {  // stencil form: the values alone
+      up(&L->sten, sten_pad);
+      L->sten_n = (int)sten.size();
+      L->sten_b = sten_b;
+    } else if (!scol16.empty()) {
+      up(&L->col16, scol16);
+    } else {
+      up(&L->col, scol);
+    }
     up(&L->src_a, sa);
     up(&L->src_t, st);
     c(dalloc(&L->va, tot, s));
@@ -430,15 +453,19 @@ template <bool DUAL>
 static hipError_t dual_launch(const DualSell *L, const double *x, double *q, const int *done,
                               long long *iter, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
 {
-  const DualDev dd{L->n, L->nslice, L->off, L->col, L->col16, L->va, L->vt};
+  const DualDev dd{L->n, L->nslice, L->off, L->col, L->col16, L->va, L->vt, L->sten, L->sten_b};
   dim3 grid(L->nbs), block(TPB);
-  if (L->col16) {
-    if (ev0) hipExtLaunchKernelGGL((k_dual_spmv<true, DUAL>), grid, block, 0, s, ev0, ev1, 0, dd, L->dv, L->pv, x, q, L->part, done, iter);
-    else hipLaunchKernelGGL((k_dual_spmv<true, DUAL>), grid, block, 0, s, dd, L->dv, L->pv, x, q, L->part, done, iter);
-  } else {
-    if (ev0) hipExtLaunchKernelGGL((k_dual_spmv<false, DUAL>), grid, block, 0, s, ev0, ev1, 0, dd, L->dv, L->pv, x, q, L->part, done, iter);
-    else hipLaunchKernelGGL((k_dual_spmv<false, DUAL>), grid, block, 0, s, dd, L->dv, L->pv, x, q, L->part, done, iter);
-  }
+#define SSF_DUAL(IDX)                                                                          \
+  do {                                                                                         \
+    if (ev0) hipExtLaunchKernelGGL((k_dual_spmv<IDX, DUAL>), grid, block, 0, s, ev0, ev1, 0, dd, \
+                                   L->dv, L->pv, x, q, L->part, done, iter);                   \
+    else hipLaunchKernelGGL((k_dual_spmv<IDX, DUAL>), grid, block, 0, s, dd, L->dv, L->pv, x, q, \
+                            L->part, done, iter);                                              \
+  } while (0)
+  if (L->sten) SSF_DUAL(2);
+  else if (L->col16) SSF_DUAL(1);
+  else SSF_DUAL(0);
+#undef SSF_DUAL
   return hipGetLastError();
 }
 

@@ -90,15 +90,15 @@ __device__ __forceinline__ T sv_row(const SlotDesc &D, const T *__restrict__ val
   const int off = D.soff[sl];
   const int w = (D.soff[sl + 1] - off) / 64;
   T sum = T(0);
-  if (D.sten) {  // stencil form: the values alone; slots in column order
+  if (D.sten) {  // stencil form: the values alone; slots in column order, 4 groups a round
     const int r = sl * 64 + lane;
     const int base = r - r % D.sten_b, nm1 = D.n - 1;
-    for (int q = 0; q < w; q += SELL_V) {
-      const int k = off + q * 64 + lane * SELL_V;
-      const tv vv = ldnt((const tv *)(val + k));
-#pragma unroll
-      for (int u = 0; u < SELL_V; u++) sum += vv[u] * x[min(max(base + D.sten[q + u], 0), nm1)];
-    }
+    const unsigned vb = (unsigned)(off + lane * SELL_V) * sizeof(T);
+    int q = 0;
+    for (; q + 4 * SELL_V <= w; q += 4 * SELL_V)
+      sum = sten_groups<T, 4>(val, vb + q * 64 * sizeof(T), D.sten + q, x, base, nm1, sum);
+    for (; q < w; q += SELL_V)
+      sum = sten_groups<T, 1>(val, vb + q * 64 * sizeof(T), D.sten + q, x, base, nm1, sum);
   } else if (D.scol16) {
     const T *__restrict__ xs = x + sl * 64;
     for (int q = 0; q < w; q += SELL_V) {

@@ -225,13 +225,6 @@ __global__ __launch_bounds__(TPB) void k_spmv_sell(CsrDev A, const T *__restrict
   }
 }
 
-// an element at a 32-bit byte offset from a base shared by the wavefront (the
-// base stays in scalar registers, the offset takes one vector register)
-template <typename U>
-__device__ __forceinline__ const U *at_byte(const void *base, unsigned off)
-{
-  return (const U *)((const char *)base + off);
-}
 
 // NG consecutive groups of SELL_V entries of one row in the row-pattern form: all
 // value loads, then all pattern-offset loads, then all gathers are issued before
@@ -384,34 +377,6 @@ __global__ __launch_bounds__(TPB) void k_spmv_sell_sten(CsrDev A, const T *__res
     const T t = block_sum(acc);
     if (threadIdx.x == 0) partial[blockIdx.x] = t;
   }
-}
-
-// NG groups of SELL_V slots of one row in the block stencil form: all value
-// loads, then all gathers (slot k at column base + so[k], clamped into range),
-// then the products in slot (= column) order. vb: byte offset of the first
-// group's values in sval; so: the groups' offsets (wave-uniform: scalar loads).
-template <typename T, int NG>
-__device__ __forceinline__ T sten_groups(const T *__restrict__ sval, unsigned vb,
-                                         const int *__restrict__ so, const T *__restrict__ x,
-                                         int base, int nm1, T sum)
-{
-  typedef T tv __attribute__((ext_vector_type(SELL_V)));
-  tv v[NG];
-  T g[NG][SELL_V];
-#pragma unroll
-  for (int k = 0; k < NG; k++) v[k] = ldnt(at_byte<tv>(sval, vb + k * 64 * SELL_V * sizeof(T)));
-#pragma unroll
-  for (int k = 0; k < NG; k++)
-#pragma unroll
-    for (int u = 0; u < SELL_V; u++) {
-      const int c = min(max(base + so[k * SELL_V + u], 0), nm1);
-      g[k][u] = *at_byte<T>(x, (unsigned)c * sizeof(T));
-    }
-#pragma unroll
-  for (int k = 0; k < NG; k++)
-#pragma unroll
-    for (int u = 0; u < SELL_V; u++) sum += v[k][u] * g[k][u];
-  return sum;
 }
 
 // SELL-64 in the block stencil form: slot k of row r sits at column
