@@ -475,7 +475,7 @@ extern "C" int ssf_matrix_destroy(ssf_matrix *m)
   if (m->a_ref == m->val) m->a_ref = nullptr;  // comprow: the same allocation
   dfree(m->a_ref, m->ctx->stream);
   dfree(m->rowptr, m->ctx->stream); dfree(m->col, m->ctx->stream); dfree(m->val, m->ctx->stream); dfree(m->val32, m->ctx->stream);
-  dfree(m->diag, m->ctx->stream); dfree(m->diag32, m->ctx->stream); dfree(m->norm_a, m->ctx->stream);
+  dfree(m->diag, m->ctx->stream); dfree(m->isd, m->ctx->stream); dfree(m->norm_a, m->ctx->stream);
   dfree(m->map_lo, m->ctx->stream); dfree(m->map_up, m->ctx->stream);
   dfree(m->asm_cstart, m->ctx->stream); dfree(m->asm_cidx, m->ctx->stream);
   dfree(m->tiles, m->ctx->stream); dfree(m->col16, m->ctx->stream);
@@ -520,7 +520,7 @@ void ssf::matrix_keep_sell_only(ssf_matrix *m)
   dfree(m->col, m->ctx->stream);
   dfree(m->val, m->ctx->stream);
   dfree(m->val32, m->ctx->stream);
-  dfree(m->diag32, m->ctx->stream);
+  dfree(m->isd, m->ctx->stream);
   dfree(m->tiles, m->ctx->stream);
   dfree(m->col16, m->ctx->stream);
   dfree(m->sell_src, m->ctx->stream);
@@ -1266,16 +1266,11 @@ static int ensure_ws(ssf_matrix *m, bool mixed)
   if (mixed && !m->val32_valid) {
     // a SELL-only matrix (device-built pattern) has no full CSR: its FP32 copy
     // and its norm come from the SELL values
-    if (!m->sell_only) {
-      if (!m->val32) SSF_HIP(dalloc(&m->val32, m->nnz, m->ctx->stream));
-      SSF_HIP(to_f32(m->nnz, m->val, m->val32, 1.0, m->ctx->stream));
-    }
-    if (!m->diag32) SSF_HIP(dalloc(&m->diag32, m->n, m->ctx->stream));
-    SSF_HIP(to_f32(m->n, m->diag, m->diag32, 1.0, m->ctx->stream));
-    if (m->sell_v64) {
-      if (!m->sell_v32) SSF_HIP(dalloc(&m->sell_v32, m->sell_tot, m->ctx->stream));
-      SSF_HIP(to_f32(m->sell_tot, m->sell_v64, m->sell_v32, 1.0, m->ctx->stream));
-    }
+    if (!m->sell_only && !m->val32) SSF_HIP(dalloc(&m->val32, m->nnz, m->ctx->stream));
+    if (!m->isd) SSF_HIP(dalloc(&m->isd, m->n, m->ctx->stream));
+    if (m->sell_v64 && !m->sell_v32) SSF_HIP(dalloc(&m->sell_v32, m->sell_tot, m->ctx->stream));
+    SSF_HIP(mixed_operator(m->csr(), m->diag, m->sell_only ? nullptr : m->val, m->val32,
+                           m->sell_v64, m->sell_v32, m->isd, m->ctx->stream));
     if (!m->norm_a) SSF_HIP(dalloc(&m->norm_a, 1, m->ctx->stream));
     if (m->sell_only)
       SSF_HIP(sell_row_absmax((int)m->n, m->sell_off, m->sell_v64, w.part, 1024, m->norm_a,
@@ -1511,10 +1506,10 @@ static int solve_mixed(ssf_matrix *m, const ssf_solve_params *p, const double *y
   // FP32 recurrence on the residual scaled by 1/||r0|| (the device keeps the
   // scale from here on), x32 = 0
   const double scale = slots[0] > 0.0 ? 1.0 / std::sqrt(slots[0]) : 1.0;
-  SSF_HIP(to_f32(m->n, w.r, w.y32, scale, s));
+  SSF_HIP(mixed_rhs32((int)m->n, w.r, m->isd, w.y32, scale, s));
   SSF_HIP(hipMemsetAsync(w.x32, 0, m->n * sizeof(float), s));
-  SSF_HIP(pcg_init<float>(m->tune32, true, false, A, m->val32, m->diag32, w.x32, w.y32, w.r32,
-                          w.h32, w.d32, w.part32, w.st32, 1e-30f, s));
+  SSF_HIP(pcg_init<float>(m->tune32, false, false, A, m->val32, nullptr, w.x32, w.y32, w.r32,
+                          nullptr, w.d32, w.part32, w.st32, 1e-30f, s));
   const double delta = (p->inner_res > 0.0 && p->inner_res < 1.0) ? p->inner_res : 0.1;
   SSF_HIP(mixed_init(w.mix, w.st32, dslot, m->norm_a, p->res, delta, s));
   hipEvent_t ev[2];
@@ -1533,16 +1528,16 @@ static int solve_mixed(ssf_matrix *m, const ssf_solve_params *p, const double *y
       // and starts on cold caches; the inner recurrence never stops after one step)
       const int64_t qs = c > 1 ? 1 : 0;
       for (int64_t q = 0; q < c && rc == SSF_OK; q++)
-        if (pcg_iteration<float>(m->tune32, MODE_MIXED_INNER, true, A, m->val32, m->diag32, w.x32,
-                                 w.r32, w.d32, w.p32, w.h32, w.part32, w.st32, 1.0f, 1e-30f, s,
+        if (pcg_iteration<float>(m->tune32, MODE_MIXED_INNER, false, A, m->val32, nullptr, w.x32,
+                                 w.r32, w.d32, w.p32, nullptr, w.part32, w.st32, 1.0f, 1e-30f, s,
                                  (m->ctx->prof && q == qs) ? m->ctx->prof_ev[k & 1][0] : nullptr,
                                  (m->ctx->prof && q == qs) ? m->ctx->prof_ev[k & 1][1] : nullptr) !=
             hipSuccess)
           rc = SSF_EHIP;
       if (rc == SSF_OK &&
-          mixed_update(m->tune64, m->tune32, A, m->val, m->diag32, y, x, w.r, w.part,
-                       w.part + w.nb, w.x32,
-                       w.r32, w.h32, w.d32, w.part32, w.st32, w.mix, p->res, s) != hipSuccess)
+          mixed_update(m->tune64, m->tune32, A, m->val, m->isd, y, x, w.r, w.part,
+                       w.part + w.nb, w.x32, w.r32, w.d32, w.part32, w.st32, w.mix, p->res,
+                       s) != hipSuccess)
         rc = SSF_EHIP;
       if (rc) {
         set_error("solve_mixed: kernel launch failed");

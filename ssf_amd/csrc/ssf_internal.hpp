@@ -219,7 +219,7 @@ struct ssf_matrix {
   double *val = nullptr;
   float *val32 = nullptr;
   double *diag = nullptr;
-  float *diag32 = nullptr;
+  double *isd = nullptr;     // 1/sqrt|a_ii|: the scaling of the FP32 operator (mixed solves)
   double *norm_a = nullptr;  // ||A||_inf, with the FP32 copy (mixed solves)
   // reference layout values on the device and the scatter into the full CSR
   double *a_ref = nullptr;   // == val for comprow

@@ -46,15 +46,24 @@ hipError_t pcg_iteration_prec(const SpmvTune &t, const CsrDev &A, const double *
                               hipEvent_t ev0, hipEvent_t ev1,
                               const std::function<hipError_t()> &prec);
 
-// mixed precision with reliable updates (ssf_solver.hip)
+// mixed precision with reliable updates (ssf_solver.hip). The FP32 operator is
+// float(S A S) with S = diag(1/sqrt|a_ii|): isd = diag(S) (n doubles), val32 from
+// the full CSR values (if val32 and A.rowptr), sval32 from the SELL values (if
+// sval32 and A.nslice)
+hipError_t mixed_operator(const CsrDev &A, const double *diag, const double *val, float *val32,
+                          const double *sval, float *sval32, double *isd, hipStream_t s);
+// r32 = float(r64 * scale * isd)
+hipError_t mixed_rhs32(int n, const double *r64, const double *isd, float *r32, double scale,
+                       hipStream_t s);
+// after pcg_init of the FP32 recurrence (CG, no preconditioner)
 hipError_t mixed_init(MixState *ms, PcgState<float> *st, const double *slots,
                       const double *norm_a, double res, double delta, hipStream_t s);
 // px: vec_blocks(t32, n) doubles of scratch (partials of ||x64||^2)
 hipError_t mixed_update(const SpmvTune &t64, const SpmvTune &t32, const CsrDev &A,
-                        const double *val64, const float *diag32, const double *b,
-                        double *x64, double *r64, double *part64, double *px, float *x32,
-                        float *r32, float *h32, float *d32, float *part32, PcgState<float> *st,
-                        MixState *ms, double res, hipStream_t s);
+                        const double *val64, const double *isd, const double *b, double *x64,
+                        double *r64, double *part64, double *px, float *x32, float *r32,
+                        float *d32, float *part32, PcgState<float> *st, MixState *ms,
+                        double res, hipStream_t s);
 // ||A||_inf into *out (device); partial: nb doubles of scratch
 hipError_t row_absmax(int n, const int *rowptr, const double *val, double *partial, int nb,
                       double *out, hipStream_t s);

@@ -591,7 +591,7 @@ __global__ __launch_bounds__(TPB) void k_pcg_update(int n, PcgState<T> *st,
   const T alpha = nom / den;
   int r0, r1;
   T rr = T(0), rh = T(0);
-  if constexpr (std::is_same<T, float>::value && JAC) {
+  if constexpr (std::is_same<T, float>::value) {
     // FP32 (the mixed inner solve): four rows per 16-byte access; every vector is
     // its own allocation, so row chunks aligned to 4 rows are 16-byte aligned
     chunk_al<4>(n, r0, r1);
@@ -600,16 +600,18 @@ __global__ __launch_bounds__(TPB) void k_pcg_update(int n, PcgState<T> *st,
       float4 xv = *(const float4 *)(x + i);
       const float4 dv = *(const float4 *)(d + i), pv = *(const float4 *)(p + i);
       float4 rv = *(const float4 *)(r + i);
-      const float4 gv = *(const float4 *)(diag + i);
-      float4 hv;
       xv.x += alpha * dv.x; xv.y += alpha * dv.y; xv.z += alpha * dv.z; xv.w += alpha * dv.w;
       rv.x -= alpha * pv.x; rv.y -= alpha * pv.y; rv.z -= alpha * pv.z; rv.w -= alpha * pv.w;
-      hv.x = rv.x / gv.x; hv.y = rv.y / gv.y; hv.z = rv.z / gv.z; hv.w = rv.w / gv.w;
       rr += rv.x * rv.x; rr += rv.y * rv.y; rr += rv.z * rv.z; rr += rv.w * rv.w;
-      rh += rv.x * hv.x; rh += rv.y * hv.y; rh += rv.z * hv.z; rh += rv.w * hv.w;
+      if (JAC) {
+        const float4 gv = *(const float4 *)(diag + i);
+        float4 hv;
+        hv.x = rv.x / gv.x; hv.y = rv.y / gv.y; hv.z = rv.z / gv.z; hv.w = rv.w / gv.w;
+        rh += rv.x * hv.x; rh += rv.y * hv.y; rh += rv.z * hv.z; rh += rv.w * hv.w;
+        *(float4 *)(h + i) = hv;
+      }
       *(float4 *)(x + i) = xv;
       *(float4 *)(r + i) = rv;
-      *(float4 *)(h + i) = hv;
     }
     r0 = r4;  // the last rows of the chunk one by one, below
   } else {
@@ -759,6 +761,7 @@ hipError_t pcg_iteration(const SpmvTune &t, int mode, bool jac, const CsrDev &A,
   } else {
     if (mode == MODE_SCR_CG_PREC) SSF_DIR(false, MODE_SCR_CG_PREC);
     else if (mode == MODE_SCR_CG) SSF_DIR(false, MODE_SCR_CG);
+    else if (mode == MODE_MIXED_INNER) SSF_DIR(false, MODE_MIXED_INNER);
     else SSF_DIR(false, MODE_CR_CG_PREC);
   }
 #undef SSF_DIR
@@ -862,12 +865,17 @@ hipError_t pcg_iteration_prec(const SpmvTune &t, const CsrDev &A, const double *
 
 // ------------------------------------------- mixed precision: reliable updates
 //
-// FP32 Jacobi-PCG on float(A) for the residual scaled by s = 1/||r|| (the FP64
-// residual norm at the last refresh, so FP32 values stay O(1)). Every K
-// iterations (and whenever the FP32 recurrence meets the test) the correction is
-// folded into the FP64 solution and the residual is recomputed in FP64:
-//   x64 += x32 / s, x32 = 0;  r64 = b - A x64 (FP64);  s' = 1/||r64||;
-//   r32 = s' r64, h32 = M^-1 r32, d32 *= s'/s;  FP64 test ||r64||^2 < res ||b||^2.
+// The FP32 inner solve is CG on the symmetrically scaled operator
+//   As = float(S A S),  S = diag(1/sqrt|a_ii|),
+// which is Jacobi-PCG on A in exact arithmetic (the iterates map by
+// x = S xs, r = S^-1 rs): the diagonal of As is 1, so the recurrence carries no
+// preconditioned residual and each iteration moves x, r, d, p only.
+// It runs on the residual scaled by s = 1/||r|| (the FP64 residual norm at the
+// last refresh, so FP32 values stay O(1)). Every K iterations (and whenever the
+// FP32 recurrence meets its test) the correction is folded into the FP64
+// solution and the residual is recomputed in FP64:
+//   x64 += S xs / s, xs = 0;  r64 = b - A x64 (FP64);  s' = 1/||r64||;
+//   rs = s' S r64, d *= s'/s;  FP64 test ||r64||^2 < res ||b||^2.
 // The FP32 search direction is carried across the refresh (rescaled with the
 // residual, which leaves alpha*d consistent), so the Krylov recurrence continues
 // on the corrected residual and the iteration count stays close to the FP64
@@ -875,8 +883,88 @@ hipError_t pcg_iteration_prec(const SpmvTune &t, const CsrDev &A, const double *
 
 namespace ssf {
 
-// x64 += x32 / s, x32 = 0; block partials of ||x64||^2 into px
+// 1/sqrt|a_ii| (1 where a_ii = 0)
+__global__ void k_inv_sqrt_diag(int n, const double *__restrict__ diag, double *__restrict__ isd)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double a = fabs(diag[i]);
+  isd[i] = a > 0.0 ? 1.0 / sqrt(a) : 1.0;
+}
+
+// the scaled FP32 copy of the full CSR: v32[k] = float(v[k] isd[i] isd[col[k]])
+__global__ void k_csr_scale32(int n, const int *__restrict__ rowptr, const int *__restrict__ col,
+                              const double *__restrict__ v, const double *__restrict__ isd,
+                              float *__restrict__ v32)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double si = isd[i];
+  for (int k = rowptr[i]; k < rowptr[i + 1]; k++) v32[k] = (float)(v[k] * si * isd[col[k]]);
+}
+
+// the same for the SELL-64 values; one thread per layout row. A slot's column
+// comes from whichever column form the layout holds (all agree on the entries
+// that are not padding, and padding holds 0)
+__global__ void k_sell_scale32(CsrDev A, const double *__restrict__ v,
+                               const double *__restrict__ isd, float *__restrict__ v32)
+{
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= A.nslice * 64) return;
+  const int sl = r / 64, lane = r % 64;
+  const int off = A.sell_off[sl];
+  const int w = (A.sell_off[sl + 1] - off) / 64;
+  const int rc = min(r, A.n - 1);
+  const double si = isd[rc];
+  for (int q = 0; q < w; q++) {
+    const int k = off + (q / SELL_V) * 64 * SELL_V + lane * SELL_V + q % SELL_V;
+    int c;
+    if (A.sell_col) c = A.sell_col[k];
+    else if (A.sell_col16) c = sl * 64 + A.sell_col16[k];
+    else if (A.sell_pid) c = rc + A.sell_dict[A.sell_dict_off[A.sell_pid[r]] + q];
+    else c = r - r % A.sten_b + A.sten_d[q];
+    c = min(max(c, 0), A.n - 1);
+    v32[k] = (float)(v[k] * si * isd[c]);
+  }
+}
+
+// b = float(a * scale * isd)
+__global__ void k_to_f32_scaled(int n, const double *__restrict__ a,
+                                const double *__restrict__ isd, float *__restrict__ b,
+                                double scale)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)(a[i] * scale * isd[i]);
+}
+
+hipError_t mixed_operator(const CsrDev &A, const double *diag, const double *val, float *val32,
+                          const double *sval, float *sval32, double *isd, hipStream_t s)
+{
+  const int n = A.n;
+  if (n <= 0) return hipSuccess;
+  const int g = (n + TPB - 1) / TPB;
+  hipLaunchKernelGGL(k_inv_sqrt_diag, dim3(g), dim3(TPB), 0, s, n, diag, isd);
+  if (val && val32 && A.rowptr)
+    hipLaunchKernelGGL(k_csr_scale32, dim3(g), dim3(TPB), 0, s, n, A.rowptr, A.col, val, isd,
+                       val32);
+  if (sval && sval32 && A.nslice > 0)
+    hipLaunchKernelGGL(k_sell_scale32, dim3((A.nslice * 64 + TPB - 1) / TPB), dim3(TPB), 0, s, A,
+                       sval, isd, sval32);
+  return hipGetLastError();
+}
+
+hipError_t mixed_rhs32(int n, const double *r64, const double *isd, float *r32, double scale,
+                       hipStream_t s)
+{
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_to_f32_scaled, dim3((n + TPB - 1) / TPB), dim3(TPB), 0, s, n, r64, isd,
+                     r32, scale);
+  return hipGetLastError();
+}
+
+// x64 += S x32 / s, x32 = 0; block partials of ||x64||^2 into px
 __global__ __launch_bounds__(TPB) void k_mx_xupdate(int n, const MixState *ms,
+                                                    const double *__restrict__ isd,
                                                     double *__restrict__ x64,
                                                     float *__restrict__ x32,
                                                     double *__restrict__ px)
@@ -887,7 +975,7 @@ __global__ __launch_bounds__(TPB) void k_mx_xupdate(int n, const MixState *ms,
   chunk(n, r0, r1);
   double xx = 0.0;
   for (int i = r0 + threadIdx.x; i < r1; i += TPB) {
-    const double xi = x64[i] + (double)x32[i] * inv;
+    const double xi = x64[i] + (double)x32[i] * inv * isd[i];
     x64[i] = xi;
     x32[i] = 0.0f;
     xx += xi * xi;
@@ -913,12 +1001,13 @@ __device__ __forceinline__ double refresh_scale(double rr64, double old)
   return rr64 > 0.0 ? 1.0 / sqrt(rr64) : old;
 }
 
+// rs = float(s' S r64), d *= s'/s (or d = rs on a restart); block partials of
+// rs.rs into prr[0, nb) and, as the recurrence's r.h, prr[nb, 2 nb)
 __global__ __launch_bounds__(TPB) void k_mx_refresh(int n, const MixState *ms,
                                                     const double *__restrict__ part64, int nb64,
                                                     const double *__restrict__ r64,
-                                                    const float *__restrict__ diag32,
+                                                    const double *__restrict__ isd,
                                                     float *__restrict__ r32,
-                                                    float *__restrict__ h32,
                                                     float *__restrict__ d32,
                                                     float *__restrict__ prr)
 {
@@ -927,25 +1016,21 @@ __global__ __launch_bounds__(TPB) void k_mx_refresh(int n, const MixState *ms,
   const double s = refresh_scale(rr64, ms->scale);
   const float c = (float)(s / ms->scale);
   // a true residual 1e4 x above its best means the carried direction is spoiled:
-  // restart it from the preconditioned residual
+  // restart it from the residual
   const bool restart = rr64 > 1e4 * ms->best;
   int r0, r1;
   chunk(n, r0, r1);
-  float rr = 0.f, rh = 0.f;
+  float rr = 0.f;
   for (int i = r0 + threadIdx.x; i < r1; i += TPB) {
-    const float ri = (float)(r64[i] * s);
-    const float hi = ri / diag32[i];
+    const float ri = (float)(r64[i] * s * isd[i]);
     r32[i] = ri;
-    h32[i] = hi;
-    d32[i] = restart ? hi : d32[i] * c;
+    d32[i] = restart ? ri : d32[i] * c;
     rr += ri * ri;
-    rh += ri * hi;
   }
   const float a = block_sum(rr);
-  const float b = block_sum(rh);
   if (threadIdx.x == 0) {
     prr[blockIdx.x] = a;
-    prr[gridDim.x + blockIdx.x] = b;
+    prr[gridDim.x + blockIdx.x] = a;
   }
 }
 
@@ -984,16 +1069,16 @@ __global__ __launch_bounds__(TPB) void k_mx_check(MixState *ms, PcgState<float> 
     ms->status = 4;
     return;
   }
-  const double s = refresh_scale(rr64, ms->scale);
-  ms->scale = s;
-  // next refresh once the scaled FP32 residual (norm 1 now) falls below
-  // delta, or below the final target if that is closer
-  st->nory = (float)fmax(ms->delta2, res * ms->nory64 * s * s);
+  ms->scale = refresh_scale(rr64, ms->scale);
+  // next refresh once the FP32 residual falls by delta from its value now, or
+  // to the final target if that is closer
+  st->nory = (float)((double)nomn * fmax(ms->delta2, res * ms->nory64 / rr64));
   st->nom[(st->iter + 1) & 1] = nomn;
   st->done = 0;
   st->status = 0;
 }
 
+// after pcg_init of the FP32 recurrence (st->nory = rs.rs of its start)
 __global__ void k_mx_init(MixState *ms, PcgState<float> *st, const double *__restrict__ slots,
                           const double *__restrict__ norm_a, double res, double delta)
 {
@@ -1009,12 +1094,13 @@ __global__ void k_mx_init(MixState *ms, PcgState<float> *st, const double *__res
   ms->updates = 0;
   ms->final = 0;
   ms->status = 0;
-  st->nory = (float)fmax(ms->delta2, res * slots[1] * ms->scale * ms->scale);
   if (slots[0] < res * slots[1]) {
     ms->final = 1;
     ms->status = 1;
     st->done = 1;
+    return;
   }
+  st->nory = (float)((double)st->nory * fmax(ms->delta2, res * slots[1] / slots[0]));
 }
 
 hipError_t mixed_init(MixState *ms, PcgState<float> *st, const double *slots,
@@ -1025,19 +1111,19 @@ hipError_t mixed_init(MixState *ms, PcgState<float> *st, const double *slots,
 }
 
 hipError_t mixed_update(const SpmvTune &t64, const SpmvTune &t32, const CsrDev &A,
-                        const double *val64, const float *diag32, const double *b,
-                        double *x64, double *r64, double *part64, double *px, float *x32,
-                        float *r32, float *h32, float *d32, float *part32, PcgState<float> *st,
-                        MixState *ms, double res, hipStream_t s)
+                        const double *val64, const double *isd, const double *b, double *x64,
+                        double *r64, double *part64, double *px, float *x32, float *r32,
+                        float *d32, float *part32, PcgState<float> *st, MixState *ms,
+                        double res, hipStream_t s)
 {
   const int n = A.n;
   const int nv = vec_blocks(t32, n);
-  hipLaunchKernelGGL(k_mx_xupdate, dim3(nv), dim3(TPB), 0, s, n, ms, x64, x32, px);
+  hipLaunchKernelGGL(k_mx_xupdate, dim3(nv), dim3(TPB), 0, s, n, ms, isd, x64, x32, px);
   hipError_t e = spmv<double>(t64, 2, A, val64, x64, r64, b, part64, &ms->final, nullptr, s);
   if (e != hipSuccess) return e;
   float *prr = part32 + t32.blocks;
   hipLaunchKernelGGL(k_mx_refresh, dim3(nv), dim3(TPB), 0, s, n, ms, part64, t64.blocks, r64,
-                     diag32, r32, h32, d32, prr);
+                     isd, r32, d32, prr);
   hipLaunchKernelGGL(k_mx_check, dim3(1), dim3(TPB), 0, s, ms, st, part64, t64.blocks, prr, nv,
                      px, res);
   return hipGetLastError();
