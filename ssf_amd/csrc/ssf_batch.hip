@@ -10,8 +10,10 @@
 //   B1  P = A D, partials d.p              (k_bspmm, kind 1)
 //   B2  x += a d, r -= a p, h = r / diag, partials r.r, r.h   (k_bupdate)
 //   B3  test, d = h + b d                   (k_bdir)
-// Mixed precision adds, once per chunk, the reliable update of ssf_solver.hip
-// per pair: x64 += x32 / s, r64 = b - A x64 in FP64, FP32 residual refresh.
+// Mixed precision runs the FP32 recurrence as CG on the symmetrically scaled
+// operator float(S A S), S = diag(1/sqrt|a_ii|) (h = r, no diagonal), and adds,
+// once per chunk, the reliable update of ssf_solver.hip per pair:
+// x64 += S x32 / s, r64 = b - A x64 in FP64, FP32 residual refresh.
 #include "ssf_batch.hpp"
 
 namespace ssf {
@@ -196,7 +198,7 @@ __global__ __launch_bounds__(TPB) void k_bspmm(BatchDev B, const T *__restrict__
 
 // ------------------------------------------------------------- B2: update
 
-template <typename T, int R>
+template <typename T, int R, bool JAC>
 __global__ __launch_bounds__(TPB) void k_bupdate(BatchDev B, BatchState<T> st,
                                                  const T *__restrict__ pden, T zero,
                                                  T *__restrict__ x, T *__restrict__ rr_,
@@ -244,13 +246,16 @@ __global__ __launch_bounds__(TPB) void k_bupdate(BatchDev B, BatchState<T> st,
       rr_[e] -= a * p[e];
     }
     const T ri = rr_[e];
-    const T hi = ri / diag[e / R];
-    h[e] = hi;
     srr += ri * ri;
-    srh += ri * hi;
+    if (JAC) {
+      const T hi = ri / diag[e / R];
+      h[e] = hi;
+      srh += ri * hi;
+    }
   }
   block_sum_r<T, R>(srr, outa);
-  block_sum_r<T, R>(srh, outb);
+  if (JAC) block_sum_r<T, R>(srh, outb);
+  else if (threadIdx.x < R) outb[threadIdx.x] = outa[threadIdx.x];
   if (threadIdx.x < R) {
     prr[(int64_t)blk * R + threadIdx.x] = outa[threadIdx.x];
     prr[(int64_t)(B.nblk + blk) * R + threadIdx.x] = outb[threadIdx.x];
@@ -359,8 +364,8 @@ __global__ void k_badvance(BatchDev B, BatchState<T> st, int *active)
 
 // --------------------------------------------------------------- init
 
-// r = y (x = 0), h = r / diag, d = h, partials y.y, r.h
-template <typename T, int R>
+// r = y (x = 0), h = r / diag (JAC) or r, d = h, partials y.y, r.h
+template <typename T, int R, bool JAC>
 __global__ __launch_bounds__(TPB) void k_binit(BatchDev B, const T *__restrict__ y,
                                                const T *__restrict__ diag, T *__restrict__ r_,
                                                T *__restrict__ h, T *__restrict__ d,
@@ -374,9 +379,9 @@ __global__ __launch_bounds__(TPB) void k_binit(BatchDev B, const T *__restrict__
   T syy = T(0), srh = T(0);
   for (int64_t e = e0 + threadIdx.x; e < e1; e += TPB) {
     const T yi = y[e];
-    const T hi = yi / diag[e / R];
+    const T hi = JAC ? yi / diag[e / R] : yi;
     r_[e] = yi;
-    h[e] = hi;
+    if (JAC) h[e] = hi;
     d[e] = hi;
     syy += yi * yi;
     srh += yi * hi;
@@ -425,11 +430,12 @@ struct BatchMix {
   long long *updates;
 };
 
-// x64 += x32 / s, x32 = 0 for the unfinished pairs; block partials of ||x64||^2
+// x64 += S x32 / s, x32 = 0 for the unfinished pairs; block partials of ||x64||^2
 // per rhs into px[blk * R + r]
 template <int R>
 __global__ __launch_bounds__(TPB) void k_bmx_xupdate(BatchDev B, BatchMix mx,
                                                      const int *__restrict__ sys_final,
+                                                     const double *__restrict__ isd,
                                                      double *__restrict__ x64,
                                                      float *__restrict__ x32,
                                                      double *__restrict__ px)
@@ -448,7 +454,7 @@ __global__ __launch_bounds__(TPB) void k_bmx_xupdate(BatchDev B, BatchMix mx,
   double xx = 0.0;
   if (!fin) {
     for (int64_t e = e0 + threadIdx.x; e < e1; e += TPB) {
-      const double xi = x64[e] + (double)x32[e] * inv;
+      const double xi = x64[e] + (double)x32[e] * inv * isd[e / R];
       x64[e] = xi;
       x32[e] = 0.0f;
       xx += xi * xi;
@@ -464,16 +470,15 @@ __global__ __launch_bounds__(TPB) void k_bmx_refresh(BatchDev B, BatchMix mx,
                                                      const int *__restrict__ sys_final,
                                                      const double *__restrict__ part64,
                                                      const double *__restrict__ r64,
-                                                     const float *__restrict__ diag32,
+                                                     const double *__restrict__ isd,
                                                      float *__restrict__ r32,
-                                                     float *__restrict__ h32,
                                                      float *__restrict__ d32,
                                                      float *__restrict__ prr)
 {
   __shared__ double scl[R];
   __shared__ float cfac[R];
   __shared__ int rst[R], fin[R];
-  __shared__ float outa[R], outb[R];
+  __shared__ float outa[R];
   const int wn = wl_count(B);
   for (int wi = blockIdx.x; wi < wn; wi += gridDim.x) {
   const int blk = wl_block(B, wi);
@@ -493,26 +498,22 @@ __global__ __launch_bounds__(TPB) void k_bmx_refresh(BatchDev B, BatchMix mx,
   __syncthreads();
   const int64_t e0 = (int64_t)B.blk_r0[blk] * R, e1 = (int64_t)B.blk_r1[blk] * R;
   const int r = threadIdx.x % R;
-  float srr = 0.f, srh = 0.f;
+  float srr = 0.f;
   if (!fin[r]) {
     const double sc = scl[r];
     const float c = cfac[r];
     const bool restart = rst[r];
     for (int64_t e = e0 + threadIdx.x; e < e1; e += TPB) {
-      const float ri = (float)(r64[e] * sc);
-      const float hi = ri / diag32[e / R];
+      const float ri = (float)(r64[e] * sc * isd[e / R]);
       r32[e] = ri;
-      h32[e] = hi;
-      d32[e] = restart ? hi : d32[e] * c;
+      d32[e] = restart ? ri : d32[e] * c;
       srr += ri * ri;
-      srh += ri * hi;
     }
   }
   block_sum_r<float, R>(srr, outa);
-  block_sum_r<float, R>(srh, outb);
   if (threadIdx.x < R) {
     prr[(int64_t)blk * R + threadIdx.x] = outa[threadIdx.x];
-    prr[(int64_t)(B.nblk + blk) * R + threadIdx.x] = outb[threadIdx.x];
+    prr[(int64_t)(B.nblk + blk) * R + threadIdx.x] = outa[threadIdx.x];
   }
   }
 }
@@ -557,9 +558,10 @@ __global__ void k_bmx_check(BatchDev B, BatchMix mx, BatchState<float> st,
     st.done[q] = 1;
     return;
   }
-  const double sc = rr64 > 0.0 ? 1.0 / sqrt(rr64) : mx.scale[q];
-  mx.scale[q] = sc;
-  st.nory[q] = (float)fmax(mx.delta2[q], res * mx.nory64[q] * sc * sc);
+  mx.scale[q] = rr64 > 0.0 ? 1.0 / sqrt(rr64) : mx.scale[q];
+  // next refresh once the FP32 residual falls by delta from its value now, or
+  // to the final target if that is closer
+  st.nory[q] = (float)((double)nomn * fmax(mx.delta2[q], res * mx.nory64[q] / rr64));
   st.nom[2 * q + (st.iter[q] & 1)] = nomn;
   st.done[q] = 0;
   st.status[q] = 0;
@@ -614,10 +616,11 @@ __global__ void k_bmx_init(BatchDev B, BatchMix mx, BatchState<float> st,
   }
 }
 
-// residual of the FP64 refresh in the units of the FP32 recurrence
+// residual of the FP64 refresh in the units of the FP32 recurrence: s S r64
 template <int R>
 __global__ __launch_bounds__(TPB) void k_bmx_scale_rhs(BatchDev B, BatchMix mx,
                                                        const double *__restrict__ r64,
+                                                       const double *__restrict__ isd,
                                                        float *__restrict__ y32)
 {
   const int r = threadIdx.x % R;
@@ -627,19 +630,51 @@ __global__ __launch_bounds__(TPB) void k_bmx_scale_rhs(BatchDev B, BatchMix mx,
     const int s = B.blk_sys[blk];
     const int64_t e0 = (int64_t)B.blk_r0[blk] * R, e1 = (int64_t)B.blk_r1[blk] * R;
     const double sc = mx.scale[s * R + r];
-    for (int64_t e = e0 + threadIdx.x; e < e1; e += TPB) y32[e] = (float)(r64[e] * sc);
+    for (int64_t e = e0 + threadIdx.x; e < e1; e += TPB) y32[e] = (float)(r64[e] * sc * isd[e / R]);
   }
 }
 
+// after k_binit2 of the FP32 recurrence (st.nory = y32.y32 of its start)
 template <int R>
 __global__ void k_bmx_init32(BatchDev B, BatchMix mx, BatchState<float> st, double res)
 {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= B.nsys * R) return;
-  const double sc = mx.scale[q];
-  st.nory[q] = (float)fmax(mx.delta2[q], res * mx.nory64[q] * sc * sc);
+  const double rr = mx.rr64[q];
+  st.nory[q] = rr > 0.0 ? (float)((double)st.nory[q] * fmax(mx.delta2[q], res * mx.nory64[q] / rr))
+                        : 0.f;
   if (mx.final[q]) st.done[q] = 1;
   st.status[q] = 0;
+}
+
+// the scaled FP32 SELL values of the mixed solve: v32 = float(v isd_row isd_col),
+// one thread per layout row (rows past a system's end hold padding, value 0)
+__global__ void k_bsell_scale32(BatchDev B, int nslice, const double *__restrict__ v,
+                                const double *__restrict__ isd, float *__restrict__ v32)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nslice * 64) return;
+  const int g = t / 64, lane = t % 64;
+  const int off = B.soff[g];
+  const int w = (B.soff[g + 1] - off) / 64;
+  const int64_t last = B.nrows - 1;
+  const int row0 = B.slice_row0[g];
+  const double si = isd[min((int64_t)row0 + lane, last)];
+  for (int q = 0; q < w; q++) {
+    const int k = off + (q / SELL_V) * 64 * SELL_V + lane * SELL_V + q % SELL_V;
+    int64_t c = B.scol16 ? (int64_t)row0 + B.scol16[k] : (int64_t)B.scol[k];
+    c = c < 0 ? 0 : (c > last ? last : c);
+    v32[k] = (float)(v[k] * si * isd[c]);
+  }
+}
+
+hipError_t batch_sell_scale32(const BatchDev &B, int nslice, const double *v, const double *isd,
+                              float *v32, hipStream_t s)
+{
+  if (nslice <= 0 || B.nrows <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_bsell_scale32, dim3((nslice * 64 + TPB - 1) / TPB), dim3(TPB), 0, s, B,
+                     nslice, v, isd, v32);
+  return hipGetLastError();
 }
 
 __global__ __launch_bounds__(TPB) void k_bdot_yy(BatchDev B, int R, const double *__restrict__ y,
@@ -721,12 +756,15 @@ static hipError_t biter_r(const BatchTune &t, const BatchDev &B, const BatchVec<
   if (e != hipSuccess) return e;
   dim3 grid = wl_grid(B), block(TPB);
   T *prr = v.part + (int64_t)B.nblk * R;
-  hipLaunchKernelGGL((k_bupdate<T, R>), grid, block, 0, s, B, st, v.part, zero, v.x, v.r, v.d,
-                     v.p, v.diag, v.h, prr);
-  if (mixed)
-    hipLaunchKernelGGL((k_bdir<T, R, true>), grid, block, 0, s, B, st, prr, res, v.d, v.h);
-  else
+  if (mixed) {  // the FP32 recurrence on the scaled operator: h = r
+    hipLaunchKernelGGL((k_bupdate<T, R, false>), grid, block, 0, s, B, st, v.part, zero, v.x, v.r,
+                       v.d, v.p, nullptr, nullptr, prr);
+    hipLaunchKernelGGL((k_bdir<T, R, true>), grid, block, 0, s, B, st, prr, res, v.d, v.r);
+  } else {
+    hipLaunchKernelGGL((k_bupdate<T, R, true>), grid, block, 0, s, B, st, v.part, zero, v.x, v.r,
+                       v.d, v.p, v.diag, v.h, prr);
     hipLaunchKernelGGL((k_bdir<T, R, false>), grid, block, 0, s, B, st, prr, res, v.d, v.h);
+  }
   hipLaunchKernelGGL((k_badvance<T, R>), dim3((B.nsys + TPB - 1) / TPB), dim3(TPB), 0, s, B, st,
                      nullptr);
   return hipGetLastError();
@@ -752,11 +790,15 @@ template hipError_t batch_iteration<float>(const BatchTune &, const BatchDev &,
 
 template <typename T, int R>
 static hipError_t binit_r(const BatchDev &B, const BatchVec<T> &v, BatchState<T> st, T zero,
-                          hipStream_t s)
+                          hipStream_t s, bool jac = true)
 {
   T *prr = v.part + (int64_t)B.nblk * R;
-  hipLaunchKernelGGL((k_binit<T, R>), wl_grid(B), dim3(TPB), 0, s, B, v.y, v.diag, v.r, v.h,
-                     v.d, prr);
+  if (jac)
+    hipLaunchKernelGGL((k_binit<T, R, true>), wl_grid(B), dim3(TPB), 0, s, B, v.y, v.diag, v.r,
+                       v.h, v.d, prr);
+  else
+    hipLaunchKernelGGL((k_binit<T, R, false>), wl_grid(B), dim3(TPB), 0, s, B, v.y, nullptr, v.r,
+                       nullptr, v.d, prr);
   const int nq = B.nsys * R;
   hipLaunchKernelGGL((k_binit2<T, R>), dim3((nq + TPB - 1) / TPB), dim3(TPB), 0, s, B, st, prr,
                      zero);
@@ -828,8 +870,8 @@ static hipError_t bmx_start_r(const BatchTune &t64, const BatchTune &t32, const 
   hipLaunchKernelGGL((k_bmx_init<R>), dim3(gq), dim3(TPB), 0, s, B, mx, st, v64.part, pyy, res,
                      delta, md.sys_final);
   // FP32 recurrence on the scaled residual (x32 = 0 set by the caller)
-  hipLaunchKernelGGL((k_bmx_scale_rhs<R>), grid, block, 0, s, B, mx, v64.r, v32.y);
-  e = binit_r<float, R>(B, v32, st, 1e-30f, s);
+  hipLaunchKernelGGL((k_bmx_scale_rhs<R>), grid, block, 0, s, B, mx, v64.r, md.isd, v32.y);
+  e = binit_r<float, R>(B, v32, st, 1e-30f, s, false);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL((k_bmx_init32<R>), dim3(gq), dim3(TPB), 0, s, B, mx, st, res);
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s, md.active);
@@ -849,14 +891,14 @@ static hipError_t bmx_refresh_r(const BatchTune &t64, const BatchDev &B,
   const int nq = B.nsys * R;
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s, md.active);
   double *px = v64.part + (int64_t)B.nblk * R;  // free in the mixed solve
-  hipLaunchKernelGGL((k_bmx_xupdate<R>), grid, block, 0, s, B, mx, md.sys_final, v64.x, v32.x,
-                     px);
+  hipLaunchKernelGGL((k_bmx_xupdate<R>), grid, block, 0, s, B, mx, md.sys_final, md.isd, v64.x,
+                     v32.x, px);
   hipError_t e = bspmm_r<double, R>(t64, 2, B, v64.val, v64.sval, v64.x, v64.r, v64.y,
                                     v64.part, md.sys_final, s);
   if (e != hipSuccess) return e;
   float *prr = v32.part + (int64_t)B.nblk * R;
   hipLaunchKernelGGL((k_bmx_refresh<R>), grid, block, 0, s, B, mx, md.sys_final, v64.part, v64.r,
-                     v32.diag, v32.r, v32.h, v32.d, prr);
+                     md.isd, v32.r, v32.d, prr);
   hipLaunchKernelGGL((k_bmx_check<R>), dim3((nq + TPB - 1) / TPB), dim3(TPB), 0, s, B, mx, st,
                      v64.part, prr, px, res);
   hipLaunchKernelGGL((k_bmx_flags<R>), dim3((B.nsys + TPB - 1) / TPB), dim3(TPB), 0, s, B, mx, st,

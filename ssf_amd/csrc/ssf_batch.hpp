@@ -68,6 +68,8 @@ struct BatchMixDev {
   long long *updates = nullptr;
   int *sys_final = nullptr;
   int *active = nullptr;
+  // per row: 1/sqrt|a_ii|, the scaling S of the FP32 operator float(S A S)
+  double *isd = nullptr;
 };
 
 // Y = A X (kind 0), with per-(block, rhs) partials of d.p (kind 1), or
@@ -94,6 +96,9 @@ hipError_t offset_copy(int64_t n, const int *src, int *dst, int add, hipStream_t
 hipError_t batch_compact(const BatchDev &B, const int *skip, int *wl, int *nwl, const int *active,
                          int *hist, int *ctr, int hist_cap, hipStream_t s);
 hipError_t zero_int(int *p, hipStream_t s);
+// the FP32 SELL values float(v_ij isd_i isd_j) of the nslice concatenated slices
+hipError_t batch_sell_scale32(const BatchDev &B, int nslice, const double *v, const double *isd,
+                              float *v32, hipStream_t s);
 hipError_t batch_mixed_start(const BatchTune &t64, const BatchTune &t32, const BatchDev &B,
                              const BatchVec<double> &v64, const BatchVec<float> &v32,
                              BatchState<float> st, const BatchMixDev &md, double res,

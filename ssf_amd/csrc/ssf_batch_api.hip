@@ -61,7 +61,7 @@ struct ssf_batch {
   std::vector<int64_t> row0;  // first global row of each system, nsys + 1
   int *rowptr = nullptr, *col = nullptr;
   double *val = nullptr, *diag = nullptr;
-  float *val32 = nullptr, *diag32 = nullptr;
+  float *val32 = nullptr;  // float(S A S), S = diag(1/sqrt|a_ii|) (mix.isd)
   int *blk_sys = nullptr, *blk_r0 = nullptr, *blk_r1 = nullptr, *sys_blk0 = nullptr;
   // SELL-64 slices of the systems, concatenated
   int nslice = 0;
@@ -73,7 +73,7 @@ struct ssf_batch {
   // vectors
   double *x = nullptr, *r = nullptr, *d = nullptr, *p = nullptr, *h = nullptr, *y = nullptr,
          *part = nullptr;
-  float *x32 = nullptr, *r32 = nullptr, *d32 = nullptr, *p32 = nullptr, *h32 = nullptr,
+  float *x32 = nullptr, *r32 = nullptr, *d32 = nullptr, *p32 = nullptr,
         *y32 = nullptr, *part32 = nullptr;
   StateArrays<double> st64;
   StateArrays<float> st32;
@@ -130,7 +130,7 @@ template <>
 BatchVec<float> ssf_batch::vec<float>() const
 {
   BatchVec<float> v;
-  v.val = val32; v.diag = diag32; v.x = x32; v.r = r32; v.d = d32; v.p = p32; v.h = h32;
+  v.val = val32; v.x = x32; v.r = r32; v.d = d32; v.p = p32;  // no diagonal, h = r
   v.y = y32; v.part = part32;
   v.sval = sval32;
   return v;
@@ -142,12 +142,12 @@ static void batch_release(ssf_batch *b)
     if (b->graph[k]) (void)hipGraphExecDestroy(b->graph[k]);
     if (b->sgraph[k]) (void)hipGraphExecDestroy(b->sgraph[k]);
   }
-  bfree(b->rowptr); bfree(b->col); bfree(b->val); bfree(b->diag); bfree(b->val32); bfree(b->diag32);
+  bfree(b->rowptr); bfree(b->col); bfree(b->val); bfree(b->diag); bfree(b->val32); bfree(b->mix.isd);
   bfree(b->blk_sys); bfree(b->blk_r0); bfree(b->blk_r1); bfree(b->sys_blk0);
   bfree(b->soff); bfree(b->scol); bfree(b->scol16); bfree(b->slice_row0); bfree(b->blk_s0);
   bfree(b->blk_s1); bfree(b->sval64); bfree(b->sval32);
   bfree(b->x); bfree(b->r); bfree(b->d); bfree(b->p); bfree(b->h); bfree(b->y); bfree(b->part);
-  bfree(b->x32); bfree(b->r32); bfree(b->d32); bfree(b->p32); bfree(b->h32); bfree(b->y32);
+  bfree(b->x32); bfree(b->r32); bfree(b->d32); bfree(b->p32); bfree(b->y32);
   bfree(b->part32);
   b->st64.release();
   b->st32.release();
@@ -317,9 +317,9 @@ static int ensure_mixed(ssf_batch *b)
   const int64_t nq = (int64_t)b->nsys * b->R, nv = b->nrows * b->R;
   hipError_t e = hipSuccess;
   auto c = [&](hipError_t x) { if (e == hipSuccess) e = x; };
-  c(balloc(&b->val32, b->nnz)); c(balloc(&b->diag32, b->nrows));
+  c(balloc(&b->val32, b->nnz)); c(balloc(&b->mix.isd, b->nrows));
   c(balloc(&b->x32, nv)); c(balloc(&b->r32, nv)); c(balloc(&b->d32, nv)); c(balloc(&b->p32, nv));
-  c(balloc(&b->h32, nv)); c(balloc(&b->y32, nv));
+  c(balloc(&b->y32, nv));
   c(balloc(&b->part32, 3 * (int64_t)b->nblk * b->R));
   c(b->st32.alloc(nq, b->nsys));
   BatchMixDev &m = b->mix;
@@ -330,9 +330,11 @@ static int ensure_mixed(ssf_batch *b)
   c(balloc(&m.norm_a, b->nsys));
   m.active = b->active;
   c(balloc(&b->sval32, b->stot));
-  if (e == hipSuccess) c(to_f32(b->nnz, b->val, b->val32, 1.0, b->ctx->stream));
-  if (e == hipSuccess) c(to_f32(b->stot, b->sval64, b->sval32, 1.0, b->ctx->stream));
-  if (e == hipSuccess) c(to_f32(b->nrows, b->diag, b->diag32, 1.0, b->ctx->stream));
+  if (e == hipSuccess) c(inv_sqrt_diag((int)b->nrows, b->diag, m.isd, b->ctx->stream));
+  if (e == hipSuccess)
+    c(csr_scale32((int)b->nrows, b->rowptr, b->col, b->val, m.isd, b->val32, b->ctx->stream));
+  if (e == hipSuccess)
+    c(batch_sell_scale32(b->dev(), b->nslice, b->sval64, m.isd, b->sval32, b->ctx->stream));
   if (e == hipSuccess) c(batch_norm_inf(b->dev(), b->val, b->part, m.norm_a, b->ctx->stream));
   SSF_HIP(e);
   return SSF_OK;

@@ -52,6 +52,10 @@ hipError_t pcg_iteration_prec(const SpmvTune &t, const CsrDev &A, const double *
 // sval32 and A.nslice)
 hipError_t mixed_operator(const CsrDev &A, const double *diag, const double *val, float *val32,
                           const double *sval, float *sval32, double *isd, hipStream_t s);
+// isd = 1/sqrt|diag| (1 where diag = 0); val32 = float(val_ij isd_i isd_j) of a CSR
+hipError_t inv_sqrt_diag(int n, const double *diag, double *isd, hipStream_t s);
+hipError_t csr_scale32(int n, const int *rowptr, const int *col, const double *val,
+                       const double *isd, float *val32, hipStream_t s);
 // r32 = float(r64 * scale * isd)
 hipError_t mixed_rhs32(int n, const double *r64, const double *isd, float *r32, double scale,
                        hipStream_t s);

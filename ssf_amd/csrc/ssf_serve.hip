@@ -2,7 +2,8 @@
 // resident request ("slot") by one iteration of the cg_prec recurrence
 // (GEFEL/scr.cpp:1142-1197), each slot through its own descriptor. The same
 // kernels, instantiated for float, run the FP32 recurrence of the mixed form,
-// whose FP64 refresh (reliable update) ends every chunk.
+// on the symmetrically scaled operator (no diagonal: h = r), whose FP64 refresh
+// (reliable update) ends every chunk.
 //
 // Block b of a launch works on the 64-row slices [blk_s0[b], blk_s1[b]) of slot
 // blk_slot[b] (the SELL-64 layout of the slot's own matrix). Every block leaves its partial sums; a small kernel with
@@ -76,7 +77,7 @@ __device__ __forceinline__ SvVec<double> sv_vec<double>(const SlotDesc &D)
 template <>
 __device__ __forceinline__ SvVec<float> sv_vec<float>(const SlotDesc &D)
 {
-  return {D.sval32, D.diag32, D.x32, D.r32, D.d32, D.p32, D.h32};
+  return {D.sval32, nullptr, D.x32, D.r32, D.d32, D.p32, D.r32};
 }
 
 // one row sum of the SELL layout: sum_k val * x[col] in column order
@@ -145,7 +146,7 @@ __global__ __launch_bounds__(TPB) void k_sv_spmv(ServeDev S, SlotState st)
 }
 
 // den = d.p; |den| < zero stops the slot (status 2); x += a d, r -= a p,
-// h = r / diag; slot sums r.r and r.h
+// h = r / diag (FP64; the FP32 recurrence has h = r); slot sums r.r and r.h
 template <typename T>
 __global__ __launch_bounds__(TPB) void k_sv_update(ServeDev S, SlotState st, double zero)
 {
@@ -175,12 +176,14 @@ __global__ __launch_bounds__(TPB) void k_sv_update(ServeDev S, SlotState st, dou
       V.r[i] -= a * V.p[i];
     }
     const T ri = V.r[i];
-    const T hi = ri / V.diag[i];
-    V.h[i] = hi;
     srr += ri * ri;
-    srh += ri * hi;
+    if (sizeof(T) == sizeof(double)) {
+      const T hi = ri / V.diag[i];
+      V.h[i] = hi;
+      srh += ri * hi;
+    }
   }
-  block_partials2<T>(S.part, srr, srh);
+  block_partials2<T>(S.part, srr, sizeof(T) == sizeof(double) ? srh : srr);
 }
 
 // FP64: stopping test |r.r / nory| < res (GEFEL/scr.cpp:1188), else d = h + b d;
@@ -344,10 +347,11 @@ hipError_t serve_flags(const ServeDev &S, SlotState st, int *active, hipStream_t
 
 // ----------------------------------------------- mixed form: reliable updates
 //
-// Per slot, the scheme of the single-system mixed solve: FP32 Jacobi-PCG on
-// float(A) for the residual scaled by s = 1/||r|| of the last refresh; at the end
-// of every chunk x64 += x32 / s, r64 = b - A x64 in FP64, the FP64 test, and the
-// FP32 recurrence restarts on s' r64 with its direction rescaled by s'/s.
+// Per slot, the scheme of the single-system mixed solve: FP32 CG on
+// float(S A S), S = diag(1/sqrt|a_ii|) (Jacobi-PCG on A in exact arithmetic),
+// for the residual scaled by s = 1/||r|| of the last refresh; at the end of every
+// chunk x64 += S x32 / s, r64 = b - A x64 in FP64, the FP64 test, and the FP32
+// recurrence restarts on s' S r64 with its direction rescaled by s'/s.
 
 // fresh slots: x64 = 0, x32 = 0, r64 = y; block partials of y.y
 __global__ __launch_bounds__(TPB) void k_svm_init(ServeDev S, SlotState st)
@@ -372,8 +376,8 @@ __device__ __forceinline__ double svm_scale(double rr, double old)
   return rr > 0.0 ? 1.0 / sqrt(rr) : old;
 }
 
-// fresh slots: the FP32 recurrence on y / ||y||: r32, h32 = d32 = r32 / diag;
-// block partials of r32.r32 and r32.h32
+// fresh slots: the FP32 recurrence on S y / ||y||: r32 = d32; block partials of
+// r32.r32 (twice: it is also the recurrence's r.h)
 __global__ __launch_bounds__(TPB) void k_svm_init32(ServeDev S, SlotState st)
 {
   const int blk = blockIdx.x;
@@ -381,17 +385,14 @@ __global__ __launch_bounds__(TPB) void k_svm_init32(ServeDev S, SlotState st)
   if (!D.fresh) return;
   const double sc = svm_scale(st.nory64[D.sid], 1.0);
   const int r0 = S.blk_s0[blk] * 64, r1 = min(D.n, S.blk_s1[blk] * 64);
-  float srr = 0.f, srh = 0.f;
+  float srr = 0.f;
   for (int i = r0 + threadIdx.x; i < r1; i += TPB) {
-    const float ri = (float)(D.y[i] * sc);
-    const float hi = ri / D.diag32[i];
+    const float ri = (float)(D.y[i] * sc * D.isd[i]);
     D.r32[i] = ri;
-    D.h32[i] = hi;
-    D.d32[i] = hi;
+    D.d32[i] = ri;
     srr += ri * ri;
-    srh += ri * hi;
   }
-  block_partials2<float>(S.part, srr, srh);
+  block_partials2<float>(S.part, srr, srr);
 }
 
 // one thread per fresh slot: the scalars of both recurrences; zero rhs: x = 0
@@ -412,8 +413,9 @@ __global__ void k_svm_init2(ServeDev S, SlotDesc *slots, SlotState st, double re
   st.updates[sid] = 0;
   st.final[sid] = 0;
   st.mstatus[sid] = 0;
-  st.nory[sid] = (double)(float)fmax(delta * delta, res * yy * sc * sc);
+  // refresh once the FP32 residual falls by delta, or to the target if closer
   const double nom = st.sc[3 * sid + 2];
+  st.nory[sid] = (double)(float)(nom * fmax(delta * delta, res));
   st.nom[2 * sid] = nom;
   st.nom[2 * sid + 1] = nom;
   st.iter[sid] = 0;
@@ -431,7 +433,7 @@ __global__ void k_svm_init2(ServeDev S, SlotDesc *slots, SlotState st, double re
   slots[k].fresh = 0;
 }
 
-// x64 += x32 / s, x32 = 0; block partials of ||x64||^2
+// x64 += S x32 / s, x32 = 0; block partials of ||x64||^2
 __global__ __launch_bounds__(TPB) void k_svm_xupdate(ServeDev S, SlotState st)
 {
   const int blk = blockIdx.x;
@@ -441,7 +443,7 @@ __global__ __launch_bounds__(TPB) void k_svm_xupdate(ServeDev S, SlotState st)
   const int r0 = S.blk_s0[blk] * 64, r1 = min(D.n, S.blk_s1[blk] * 64);
   double xx = 0.0;
   for (int i = r0 + threadIdx.x; i < r1; i += TPB) {
-    const double xi = D.x[i] + (double)D.x32[i] * inv;
+    const double xi = D.x[i] + (double)D.x32[i] * inv * D.isd[i];
     D.x[i] = xi;
     D.x32[i] = 0.0f;
     xx += xi * xi;
@@ -470,9 +472,9 @@ __global__ __launch_bounds__(TPB) void k_svm_resid(ServeDev S, SlotState st)
   block_partials2<double>(S.part, acc, 0.0);
 }
 
-// the FP32 recurrence restarts on s' r64 (s' = 1/||r64||), its direction rescaled
-// by s'/s, or restarted from h32 when ||r64|| rose 1e4-fold above its best;
-// block partials of r32.r32 and r32.h32. rr64 of the refresh is st.sc[3 sid].
+// the FP32 recurrence restarts on s' S r64 (s' = 1/||r64||), its direction
+// rescaled by s'/s, or restarted from r32 when ||r64|| rose 1e4-fold above its
+// best; block partials of r32.r32 (twice). rr64 of the refresh is st.sc[3 sid].
 __global__ __launch_bounds__(TPB) void k_svm_refresh(ServeDev S, SlotState st)
 {
   const int blk = blockIdx.x;
@@ -485,17 +487,14 @@ __global__ __launch_bounds__(TPB) void k_svm_refresh(ServeDev S, SlotState st)
   const float c = (float)(sc / old);
   const bool restart = rr64 > 1e4 * st.best[sid];
   const int r0 = S.blk_s0[blk] * 64, r1 = min(D.n, S.blk_s1[blk] * 64);
-  float srr = 0.f, srh = 0.f;
+  float srr = 0.f;
   for (int i = r0 + threadIdx.x; i < r1; i += TPB) {
-    const float ri = (float)(D.r[i] * sc);
-    const float hi = ri / D.diag32[i];
+    const float ri = (float)(D.r[i] * sc * D.isd[i]);
     D.r32[i] = ri;
-    D.h32[i] = hi;
-    D.d32[i] = restart ? hi : D.d32[i] * c;
+    D.d32[i] = restart ? ri : D.d32[i] * c;
     srr += ri * ri;
-    srh += ri * hi;
   }
-  block_partials2<float>(S.part, srr, srh);
+  block_partials2<float>(S.part, srr, srr);
 }
 
 // below this the FP64 residual is within the rounding error of its own
@@ -536,9 +535,10 @@ __global__ void k_svm_check(ServeDev S, SlotState st, double res, long long ni, 
         st.mstatus[sid] = 0;
         st.done[sid] = 1;
       } else {
-        const double sc = svm_scale(rr64, st.scale[sid]);
-        st.scale[sid] = sc;
-        st.nory[sid] = (double)(float)fmax(st.delta2[sid], res * st.nory64[sid] * sc * sc);
+        st.scale[sid] = svm_scale(rr64, st.scale[sid]);
+        // next refresh once the FP32 residual falls by delta from its value
+        // now (nomn), or to the final target if that is closer
+        st.nory[sid] = (double)(float)(nomn * fmax(st.delta2[sid], res * st.nory64[sid] / rr64));
         st.nom[2 * sid + (st.iter[sid] & 1)] = nomn;
         st.done[sid] = 0;
         st.status[sid] = 0;

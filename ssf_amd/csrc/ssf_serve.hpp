@@ -21,9 +21,11 @@ struct SlotDesc {
   const double *diag;
   const double *y;
   double *x, *r, *d, *p, *h;  // FP64 recurrence; the mixed form uses x, r only
-  // mixed form: FP32 values and recurrence, ||A||_inf of the FP64 values
-  const float *sval32, *diag32;
-  float *x32, *r32, *d32, *p32, *h32;
+  // mixed form: the FP32 operator float(S A S) with S = diag(isd),
+  // isd = 1/sqrt|a_ii|, its CG recurrence (h = r), ||A||_inf of the FP64 values
+  const float *sval32;
+  const double *isd;
+  float *x32, *r32, *d32, *p32;
   const double *norm_a;
   int n;
   int sid;         // index of the slot's scalars in SlotState (stable while resident)

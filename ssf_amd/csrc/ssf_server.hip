@@ -54,7 +54,7 @@ struct Request {
   // device
   ssf_matrix *m = nullptr;
   // FP64: y, x, r, d, p, h (6 n doubles); mixed: y, x, r (3 n doubles), then
-  // x32, r32, d32, p32, h32, diag32 (6 n floats); then ||A||_inf (1 double)
+  // x32, r32, d32, p32 (4 n floats), isd (n doubles); then ||A||_inf (1 double)
   double *vec = nullptr;
   int64_t bytes = 0;
   int sid = -1;
@@ -286,14 +286,15 @@ void prep_main(ssf_server *S, int w)
           e = hipMemcpyAsync(q->vec, lift.data(), q->n * sizeof(double), hipMemcpyHostToDevice,
                              ctx->stream);
         if (e == hipSuccess && mixed) {
-          // FP32 copy of the SELL values and of the diagonal; ||A||_inf of the
-          // FP64 values, for the rounding floor of the FP64 residual
+          // the scaled FP32 operator float(S A S) of the SELL values and
+          // S = 1/sqrt|diag|; ||A||_inf of the FP64 values, for the rounding
+          // floor of the FP64 residual
           ssf_matrix *m = q->m;
           e = hipMallocAsync((void **)&m->sell_v32, std::max<int64_t>(m->sell_tot, 1) * sizeof(float),
                              ctx->stream);
-          if (e == hipSuccess) e = to_f32(m->sell_tot, m->sell_v64, m->sell_v32, 1.0, ctx->stream);
-          float *f = (float *)(q->vec + 3 * q->n);
-          if (e == hipSuccess) e = to_f32(q->n, m->diag, f + 5 * q->n, 1.0, ctx->stream);
+          if (e == hipSuccess)
+            e = mixed_operator(m->csr(), m->diag, nullptr, nullptr, m->sell_v64, m->sell_v32,
+                               q->vec + 5 * q->n, ctx->stream);
           double *part = nullptr;
           if (e == hipSuccess) e = hipMallocAsync((void **)&part, 1024 * sizeof(double), ctx->stream);
           if (e == hipSuccess)
@@ -400,8 +401,7 @@ hipError_t upload_tables(ssf_server *S, const std::vector<char> &fresh)
       D.r32 = f + q->n;
       D.d32 = f + 2 * q->n;
       D.p32 = f + 3 * q->n;
-      D.h32 = f + 4 * q->n;
-      D.diag32 = f + 5 * q->n;
+      D.isd = q->vec + 5 * q->n;  // the two n-float slots after p32
       D.sval32 = m->sell_v32;
       D.norm_a = q->vec + 6 * q->n;
     }

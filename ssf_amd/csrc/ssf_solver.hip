@@ -937,16 +937,31 @@ __global__ void k_to_f32_scaled(int n, const double *__restrict__ a,
   if (i < n) b[i] = (float)(a[i] * scale * isd[i]);
 }
 
+hipError_t inv_sqrt_diag(int n, const double *diag, double *isd, hipStream_t s)
+{
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_inv_sqrt_diag, dim3((n + TPB - 1) / TPB), dim3(TPB), 0, s, n, diag, isd);
+  return hipGetLastError();
+}
+
+hipError_t csr_scale32(int n, const int *rowptr, const int *col, const double *val,
+                       const double *isd, float *val32, hipStream_t s)
+{
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_csr_scale32, dim3((n + TPB - 1) / TPB), dim3(TPB), 0, s, n, rowptr, col,
+                     val, isd, val32);
+  return hipGetLastError();
+}
+
 hipError_t mixed_operator(const CsrDev &A, const double *diag, const double *val, float *val32,
                           const double *sval, float *sval32, double *isd, hipStream_t s)
 {
   const int n = A.n;
   if (n <= 0) return hipSuccess;
-  const int g = (n + TPB - 1) / TPB;
-  hipLaunchKernelGGL(k_inv_sqrt_diag, dim3(g), dim3(TPB), 0, s, n, diag, isd);
-  if (val && val32 && A.rowptr)
-    hipLaunchKernelGGL(k_csr_scale32, dim3(g), dim3(TPB), 0, s, n, A.rowptr, A.col, val, isd,
-                       val32);
+  hipError_t e = inv_sqrt_diag(n, diag, isd, s);
+  if (e == hipSuccess && val && val32 && A.rowptr)
+    e = csr_scale32(n, A.rowptr, A.col, val, isd, val32, s);
+  if (e != hipSuccess) return e;
   if (sval && sval32 && A.nslice > 0)
     hipLaunchKernelGGL(k_sell_scale32, dim3((A.nslice * 64 + TPB - 1) / TPB), dim3(TPB), 0, s, A,
                        sval, isd, sval32);
