@@ -1033,7 +1033,12 @@ def main():
         "spmv_gbs": {"fp32": gbs32, "fp64": gbs64, "copy_kernel": copy_gbs},
         "roofline": {"bound": "hbm", "achieved": gbs32, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                      "frac": gbs32 * 1e9 / PEAK_HBM, "traffic": traffic,
-                     "kernel": {2: ("k_spmv_sell_sten<float, 1, %d> (FP32 SELL-64 SpMV of the mixed "
+                     "kernel": {2: ("k_spmv_half<float, 1, %d> (FP32 SpMV of the mixed solve, "
+                                    "symmetric half of the stencil form: each row's lower slots "
+                                    "stored once, the upper entries read from the mirrored rows)"
+                                    % (round((b32_act - 8 * prob.n) / (4 * prob.n)) - 1)
+                                    if tune32[3] == 4 else
+                                    "k_spmv_sell_sten<float, 1, %d> (FP32 SELL-64 SpMV of the mixed "
                                     "solve, stencil form)" % round((b32_act - 8 * prob.n) / 4 / (64 * ((prob.n + 63) // 64)) / 4)
                                     if tune32[3] == 3 else
                                     "k_spmv_sell_pat<float, 1, %d> (FP32 SELL-64 SpMV of the mixed "

@@ -143,6 +143,41 @@ static SpmvTune default_tune(int64_t n, int64_t nnz, int ntiles, int nslice, boo
   return t;
 }
 
+// the symmetric half of the stencil form: a symcomprow matrix whose stencil list
+// (offsets from the row itself) is symmetric about its 0 slot; sets half_nl (the
+// strictly lower slots, at most 15) and half_pad, or leaves half_nl = 0
+static void half_setup(ssf_matrix *m)
+{
+  m->half_nl = 0;
+  m->half_pad = 0;
+  const int ns = (int)m->sten.size();
+  if (m->storage != SSF_STORAGE_SYMM_COMP_ROWS || m->sten_b != 1 || ns % 2 == 0 || ns > 31 ||
+      m->nslice <= 0)
+    return;
+  const int nl = ns / 2;
+  if (m->sten[nl] != 0) return;
+  for (int k = 0; k < nl; k++)
+    if (m->sten[k] != -m->sten[ns - 1 - k]) return;
+  const int64_t pad = ((int64_t)m->nslice * 64 - m->sten[0] + 63) / 64 * 64;
+  // rows are addressed as 32-bit indices
+  if (pad >= (int64_t(1) << 31)) return;
+  m->half_nl = nl;
+  m->half_pad = pad;
+}
+
+// the half stencil form: blocks of 256 threads, one wavefront per slice and at
+// most 512 blocks (a few slices per wavefront)
+static void half_default_tune(ssf_matrix *m)
+{
+  SpmvTune t;
+  t.format = 2;
+  t.lanes = 1;
+  t.idx16 = 4;
+  const int64_t nb = std::max<int64_t>(8, std::min<int64_t>(512, (m->nslice + 3) / 4));
+  t.blocks = (int)((nb + 7) / 8 * 8);
+  m->tune64 = m->tune32 = t;
+}
+
 // Stream tiling: consecutive whole rows with at most STREAM_CAP entries per tile;
 // 16-bit column offsets relative to each tile's first row when they all fit.
 static bool build_tiles(int64_t n, const std::vector<int> &rowptr, const std::vector<int> &col,
@@ -482,6 +517,7 @@ extern "C" int ssf_matrix_destroy(ssf_matrix *m)
   dfree(m->sell_off, m->ctx->stream); dfree(m->sell_col, m->ctx->stream); dfree(m->sell_src, m->ctx->stream); dfree(m->sell_v64, m->ctx->stream); dfree(m->sell_v32, m->ctx->stream);
   dfree(m->sell_col16, m->ctx->stream);
   dfree(m->sten_d, m->ctx->stream);
+  dfree(m->half_v64, m->ctx->stream); dfree(m->half_v32, m->ctx->stream);
   dfree(m->sell_pid, m->ctx->stream); dfree(m->sell_dict, m->ctx->stream);
   dfree(m->sell_dict_off, m->ctx->stream);
   dfree(m->sell_lo, m->ctx->stream); dfree(m->sell_up, m->ctx->stream);
@@ -525,6 +561,7 @@ void ssf::matrix_keep_sell_only(ssf_matrix *m)
   dfree(m->col16, m->ctx->stream);
   dfree(m->sell_src, m->ctx->stream);
   dfree(m->sell_v32, m->ctx->stream);
+  dfree(m->half_v32, m->ctx->stream);
   if (m->sell_col16) dfree(m->sell_col, m->ctx->stream);
   dfree(m->sell_lo, m->ctx->stream);
   dfree(m->sell_up, m->ctx->stream);
@@ -793,12 +830,14 @@ int ssf::matrix_create(ssf_ctx *ctx, int storage, int64_t n, const int64_t *adr,
   if (m->sell_col16) m->tune64.idx16 = m->tune32.idx16 = (m->tune64.format == 2) ? 1 : 0;
   if (m->sell_pid && m->tune64.format == 2) m->tune64.idx16 = m->tune32.idx16 = 2;
   if (!m->sten.empty() && m->tune64.format == 2) m->tune64.idx16 = m->tune32.idx16 = 3;
+  half_setup(m);
+  if (m->half_nl > 0 && m->tune64.format == 2) half_default_tune(m);
   *out = m;
   return SSF_OK;
 }
 
 int ssf::matrix_create_sell_dev(ssf_ctx *ctx, int64_t n, int *d_adr, int *d_ci, int64_t negm,
-                                ssf_matrix **out)
+                                ssf_matrix **out, bool half_form)
 {
   hipStream_t s = ctx->stream;
   ssf_matrix *m = new ssf_matrix();
@@ -848,6 +887,8 @@ int ssf::matrix_create_sell_dev(ssf_ctx *ctx, int64_t n, int *d_adr, int *d_ci, 
   t.blocks = (int)((std::max<int64_t>(8, std::min<int64_t>(2048, (m->nslice + 3) / 4)) + 7) / 8 * 8);
   t.idx16 = !m->sten.empty() ? 3 : m->sell_col16 ? 1 : 0;
   m->tune64 = m->tune32 = t;
+  if (half_form) half_setup(m);
+  if (m->half_nl > 0) half_default_tune(m);
   *out = m;
   return SSF_OK;
 }
@@ -995,6 +1036,24 @@ extern "C" int ssf_matrix_pattern(const ssf_matrix *m, int64_t *adr, int64_t *ci
   return SSF_OK;
 }
 
+// (re)builds the half form of the FP64 (precision 0) or FP32 SELL values
+static int half_build(ssf_matrix *m, int precision)
+{
+  if (m->half_nl == 0) return SSF_OK;
+  hipStream_t s = m->ctx->stream;
+  const int64_t tot = (int64_t)(m->half_nl + 1) * m->half_pad;
+  if (precision == 0) {
+    if (!m->sell_v64) return SSF_OK;
+    if (!m->half_v64) SSF_HIP(dalloc(&m->half_v64, tot, s));
+    SSF_HIP(half_from_sell<double>(m->csr(), m->sell_v64, m->half_v64, s));
+  } else {
+    if (!m->sell_v32) return SSF_OK;
+    if (!m->half_v32) SSF_HIP(dalloc(&m->half_v32, tot, s));
+    SSF_HIP(half_from_sell<float>(m->csr(), m->sell_v32, m->half_v32, s));
+  }
+  return SSF_OK;
+}
+
 // after the reference-layout values changed: full CSR values, diagonal
 static int values_changed(ssf_matrix *m)
 {
@@ -1005,14 +1064,14 @@ static int values_changed(ssf_matrix *m)
                         m->sell_v64, m->diag, s));
     m->values_set = true;
     m->val32_valid = false;
-    return SSF_OK;
+    return half_build(m, 0);
   }
   if (m->storage == SSF_STORAGE_SYMM_COMP_ROWS) SSF_HIP(expand_ref(m->negm, m->map_lo, m->map_up, m->a_ref, m->val, s));
   SSF_HIP(extract_diag((int)m->n, m->rowptr, m->col, m->val, m->diag, nullptr, s));
   if (m->sell_src) SSF_HIP(sell_gather(m->sell_tot, m->sell_src, m->val, m->sell_v64, s));
   m->values_set = true;
   m->val32_valid = false;
-  return SSF_OK;
+  return half_build(m, 0);
 }
 
 extern "C" int ssf_matrix_zero(ssf_matrix *m)
@@ -1271,6 +1330,7 @@ static int ensure_ws(ssf_matrix *m, bool mixed)
     if (m->sell_v64 && !m->sell_v32) SSF_HIP(dalloc(&m->sell_v32, m->sell_tot, m->ctx->stream));
     SSF_HIP(mixed_operator(m->csr(), m->diag, m->sell_only ? nullptr : m->val, m->val32,
                            m->sell_v64, m->sell_v32, m->isd, m->ctx->stream));
+    if (int rc = half_build(m, 1)) return rc;
     if (!m->norm_a) SSF_HIP(dalloc(&m->norm_a, 1, m->ctx->stream));
     if (m->sell_only)
       SSF_HIP(sell_row_absmax((int)m->n, m->sell_off, m->sell_v64, w.part, 1024, m->norm_a,
@@ -1823,6 +1883,20 @@ extern "C" int ssf_matrix_autotune(ssf_matrix *m, int precision, int reps, int32
             t.map = map;
             cand.push_back(t);
           }
+  // symmetric half of the stencil form: blocks of 256 or 512 threads
+  if (m->half_nl > 0)
+    for (int b : {256, 512, 1024, 2048})
+      for (int tb : {1, 2})
+        for (int map : {0, 1}) {
+          if (b > std::max(8, (m->nslice + 3) / 4)) continue;
+          SpmvTune t;
+          t.format = 2;
+          t.lanes = tb;
+          t.blocks = b;
+          t.idx16 = 4;
+          t.map = map;
+          cand.push_back(t);
+        }
   SpmvTune best = precision ? m->tune32 : m->tune64;
   double best_t = 1e300;
   for (const SpmvTune &t : cand) {
@@ -1877,11 +1951,13 @@ extern "C" int ssf_matrix_set_tune(ssf_matrix *m, int precision, const int32_t *
                 (t.idx16 == 1 && precision && t.format == 1 && m->col16) ||
                 (t.idx16 == 1 && t.format == 2 && m->sell_col16) ||
                 (t.idx16 == 2 && t.format == 2 && m->sell_pid) ||
-                (t.idx16 == 3 && t.format == 2 && !m->sten.empty()),
+                (t.idx16 == 3 && t.format == 2 && !m->sten.empty()) ||
+                (t.idx16 == 4 && t.format == 2 && m->half_nl > 0 && (t.lanes == 1 || t.lanes == 2)),
             SSF_EINVAL,
             "ssf_matrix_set_tune: index form 0 (int32 columns), 1 (16-bit columns) for the FP32 "
-            "stream operator or the SELL operator of a matrix that has them, 2 (row patterns) or "
-            "3 (stencil form) for a SELL operator that has them");
+            "stream operator or the SELL operator of a matrix that has them, 2 (row patterns), "
+            "3 (stencil form) or 4 (symmetric half of the stencil form, lanes 1 or 2: blocks of "
+            "256 or 512 threads) for a SELL operator that has them");
   if (precision) m->tune32 = t;
   else m->tune64 = t;
   return SSF_OK;
@@ -1936,7 +2012,9 @@ extern "C" int ssf_spmv_bytes(const ssf_matrix *m, int precision, double *canoni
   double a = (double)m->nnz * (sv + ib) + 4.0 * (double)(m->n + 1) + 2.0 * sv * (double)m->n;
   if (t.format == 1) a += 4.0 * (double)(m->ntiles + 1);
   if (t.format == 2 && m->nslice > 0) {  // padded slices, slice offsets instead of row pointers
-    if (t.idx16 == 3 && !m->sten.empty())  // stencil form: the values alone
+    if (t.idx16 == 4 && m->half_nl > 0)  // half stencil form: the lower slots of every row once
+      a = (double)(m->half_nl + 1) * (double)m->n * sv + 2.0 * sv * (double)m->n;
+    else if (t.idx16 == 3 && !m->sten.empty())  // stencil form: the values alone
       a = (double)m->sell_tot * sv + 2.0 * sv * (double)m->n;
     else if (t.idx16 == 2 && m->sell_pid)  // a 2-byte pattern id per layout row instead of columns
       a = (double)m->sell_tot * sv + 2.0 * 64.0 * (double)m->nslice +

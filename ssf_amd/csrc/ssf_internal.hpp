@@ -125,7 +125,9 @@ struct SpmvTune {
   int blocks = 1024; // blocks of 256 threads (rows / tiles split in contiguous chunks)
   int idx16 = 0;     // column index form: 0 int32; 1 16-bit (stream tile- / SELL
                      // slice-relative); 2 SELL row patterns (a pattern id per row
-                     // and a shared dictionary of column offsets); 3 SELL stencil form
+                     // and a shared dictionary of column offsets); 3 SELL stencil form;
+                     // 4 symmetric half of the stencil form (lanes: 256-thread units
+                     // per block, 1 or 2)
   int map = 1;       // SELL slices to waves: 0 a contiguous chunk per block; 1 XCD
                      // interleave (each XCD a contiguous eighth, its waves in turn)
 };
@@ -165,6 +167,15 @@ struct CsrDev {
   const int *sten_d = nullptr;
   const double *sell_v64 = nullptr;
   const float *sell_v32 = nullptr;
+  // symmetric half of the stencil form (sten_b = 1, sten[k] = -sten[sten_n - 1 - k],
+  // sten[half_nl] = 0): the half_nl + 1 lower slots of every row, diagonal-major,
+  // half_v[k * half_pad + r] = a(r, r + sten[k]), 0 for rows r >= n; half_pad
+  // covers 64 nslice - sten[0] rows, so the mirrored slot a(r, r - sten[k]) =
+  // half_v[k * half_pad + r - sten[k]] needs no bounds test
+  int half_nl = 0;
+  int64_t half_pad = 0;
+  const double *half_v64 = nullptr;
+  const float *half_v32 = nullptr;
   int slice_map = 1;  // SpmvTune::map of the launch
 };
 
@@ -252,6 +263,12 @@ struct ssf_matrix {
   int *sten_d = nullptr;  // device copy of sten
   double *sell_v64 = nullptr;
   float *sell_v32 = nullptr;
+  // symmetric half of the stencil form (CsrDev::half_*), rebuilt from the SELL
+  // values whenever they change; half_nl = 0: the matrix has no such form
+  int half_nl = 0;
+  int64_t half_pad = 0;
+  double *half_v64 = nullptr;
+  float *half_v32 = nullptr;
   // SELL-only matrices (laid out on the device, no full CSR): SELL positions of
   // the reference-layout entries and the device copy of the pattern
   bool sell_only = false;
@@ -294,6 +311,10 @@ struct ssf_matrix {
     c.sten_d = sten_d;
     c.sell_v64 = sell_v64;
     c.sell_v32 = sell_v32;
+    c.half_nl = half_nl;
+    c.half_pad = half_pad;
+    c.half_v64 = half_v64;
+    c.half_v32 = half_v32;
     return c;
   }
   ssf::Workspace ws;
@@ -320,9 +341,10 @@ int64_t pattern_build_bytes(int64_t ne, int nd);  // its transient device bytes
 // entries of the SELL-64 layout of the full matrix of a device symcomprow pattern
 hipError_t sell_entries_dev(int64_t n, int64_t negm, const int *adr, const int *ci, hipStream_t s,
                             int64_t *tot);
-// matrix_create_sell from a device pattern (ownership of adr / ci passes to the matrix)
+// matrix_create_sell from a device pattern (ownership of adr / ci passes to the matrix);
+// half_form: also keep the symmetric half of a stencil layout for the solves
 int matrix_create_sell_dev(ssf_ctx *ctx, int64_t n, int *adr, int *ci, int64_t negm,
-                           ssf_matrix **out);
+                           ssf_matrix **out, bool half_form = true);
 // symbolic pattern of symcomprow / comprow from element code numbers (host)
 int pattern_from_elements(int storage, int64_t n, int64_t ne, int64_t ndofe, const int64_t *cn,
                           std::vector<int64_t> &adr, std::vector<int64_t> &ci);

@@ -21,6 +21,11 @@ hipError_t spmv(const SpmvTune &t, int kind, const CsrDev &A, const T *val, cons
                 const T *b, T *partial, const int *done, long long *iter, hipStream_t s,
                 hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 
+// the symmetric half of a stencil-form SELL layout (CsrDev::half_*): hv[k * half_pad
+// + r] = slot k of row r of sval for k <= A.half_nl, 0 for r >= n
+template <typename T>
+hipError_t half_from_sell(const CsrDev &A, const T *sval, T *hv, hipStream_t s);
+
 // PCG pieces (see ssf_kernels.hip for the recurrence).
 template <typename T>
 hipError_t pcg_init(const SpmvTune &t, bool jac, bool havex, const CsrDev &A, const T *val,

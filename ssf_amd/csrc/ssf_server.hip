@@ -248,7 +248,7 @@ void prep_main(ssf_server *S, int w)
     }
     if (rc == SSF_OK) {
       const auto tc0 = clk::now();
-      rc = matrix_create_sell_dev(ctx, q->n, d_adr, d_ci, negm, &q->m);
+      rc = matrix_create_sell_dev(ctx, q->n, d_adr, d_ci, negm, &q->m, false);
       dt_create = std::chrono::duration<double>(clk::now() - tc0).count();
       const auto ta0 = clk::now();
       if (rc == SSF_OK && q->m->nslice == 0) {

@@ -144,10 +144,9 @@ __global__ __launch_bounds__(TPB) void k_spmv_stream(CsrDev A, const T *__restri
 // stream one window of the value array (HBM pages and prefetch stay together)
 // while the x entries of neighbouring rows stay in that XCD's L2.
 __device__ __forceinline__ void wave_slices(int nslice, int map, int wave, int &first, int &last,
-                                            int &step)
+                                            int &step, int W = TPB / 64)
 {
   const int nb = gridDim.x;
-  constexpr int W = TPB / 64;
   if (map == 1 && (nb & 7) == 0) {
     const int xcd = blockIdx.x & 7, lb = blockIdx.x >> 3, nbx = nb >> 3;
     const int spx = (nslice + 7) / 8;
@@ -423,6 +422,113 @@ __global__ __launch_bounds__(TPB) void k_spmv_sell_bsten(CsrDev A, const T *__re
   }
 }
 
+template <typename T>
+__device__ __forceinline__ const T *half_values(const CsrDev &A);
+template <>
+__device__ __forceinline__ const double *half_values<double>(const CsrDev &A)
+{
+  return A.half_v64;
+}
+template <>
+__device__ __forceinline__ const float *half_values<float>(const CsrDev &A)
+{
+  return A.half_v32;
+}
+
+// Symmetric half of the stencil form (symcomprow matrices whose stencil list is
+// symmetric, sten[NL] = 0). Each row holds its NL + 1 lower slots, diagonal-major
+// (hv[k * pad + r] = a(r, r + sten[k])); the upper entry a(r, r - sten[k]) is
+// a(r - sten[k], r), slot k of the row |sten[k]| further on, read from the same
+// diagonal. The wavefronts of one XCD stream neighbouring slices together, so the
+// mirrored reads are L2 hits and HBM carries each stored value once: 14 values
+// per heat row instead of 28. Blocks of 256 or 512 threads, one wavefront per
+// slice. The row sums in column order (lower slots ascending, the diagonal, the
+// upper ones ascending), so the products equal the other forms' bit for bit.
+template <typename T, int KIND, int NL>
+__global__ __launch_bounds__(512) void k_spmv_half(CsrDev A, const T *__restrict__ x,
+                                                   T *__restrict__ y, const T *__restrict__ b,
+                                                   T *__restrict__ partial, const int *done,
+                                                   long long *iter)
+{
+  if (done && *done) return;
+  if (iter && blockIdx.x == 0 && threadIdx.x == 0) *iter += 1;
+  const T *__restrict__ hv = half_values<T>(A);
+  const int W = blockDim.x / 64;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), lane = threadIdx.x % 64;
+  int sfirst, slast, sstep;
+  wave_slices(A.nslice, A.slice_map, wave, sfirst, slast, sstep, W);
+  const int nm1 = A.n - 1;
+  const int64_t pad = A.half_pad;
+  T acc = T(0);
+  for (int sl = sfirst; sl < slast; sl += sstep) {
+    const int r = sl * 64 + lane;
+    T vo[NL + 1], vm[NL], gl[NL], gu[NL];
+#pragma unroll
+    for (int k = 0; k <= NL; k++) vo[k] = hv[k * pad + r];
+#pragma unroll
+    for (int k = 0; k < NL; k++) vm[k] = hv[k * pad + (r - A.sten[k])];
+#pragma unroll
+    for (int k = 0; k < NL; k++) {
+      gl[k] = x[min(max(r + A.sten[k], 0), nm1)];
+      gu[k] = x[min(r - A.sten[k], nm1)];
+    }
+    const T gd = x[min(r, nm1)];
+    T sum = T(0);
+#pragma unroll
+    for (int k = 0; k < NL; k++) sum += vo[k] * gl[k];
+    sum += vo[NL] * gd;
+#pragma unroll
+    for (int k = NL - 1; k >= 0; k--) sum += vm[k] * gu[k];
+    if (r < A.n) {
+      if (KIND == 2) {
+        sum = b[r] - sum;
+        acc += sum * sum;
+      } else if (KIND == 1) {
+        acc += gd * sum;
+      }
+      y[r] = sum;
+    }
+  }
+  if (KIND != 0) {
+    __shared__ T lds[8];
+    acc = wave_sum(acc);
+    if (lane == 0) lds[wave] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      T t = lds[0];
+      for (int i = 1; i < W; i++) t += lds[i];
+      partial[blockIdx.x] = t;
+    }
+  }
+}
+
+template <typename T>
+__global__ void k_half_from_sell(CsrDev A, const T *__restrict__ sval, T *__restrict__ hv)
+{
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= (int64_t)(A.half_nl + 1) * A.half_pad) return;
+  const int k = (int)(i / A.half_pad), r = (int)(i % A.half_pad);
+  T v = T(0);
+  if (r < A.n) {  // the stencil form's slices are all ng * SELL_V slots wide
+    const int64_t ng = (A.sten_n + SELL_V - 1) / SELL_V;
+    v = sval[(r / 64) * 64 * ng * SELL_V + (k / SELL_V) * 64 * SELL_V + (r % 64) * SELL_V +
+             k % SELL_V];
+  }
+  hv[i] = v;
+}
+
+template <typename T>
+hipError_t half_from_sell(const CsrDev &A, const T *sval, T *hv, hipStream_t s)
+{
+  const int64_t tot = (int64_t)(A.half_nl + 1) * A.half_pad;
+  if (tot <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_half_from_sell<T>, dim3((unsigned)((tot + TPB - 1) / TPB)), dim3(TPB), 0,
+                     s, A, sval, hv);
+  return hipGetLastError();
+}
+template hipError_t half_from_sell<double>(const CsrDev &, const double *, double *, hipStream_t);
+template hipError_t half_from_sell<float>(const CsrDev &, const float *, float *, hipStream_t);
+
 // ev0/ev1 (both or neither): the launch's own start and end timestamps
 // (hipExtLaunchKernelGGL), i.e. the kernel's duration without the dispatch gap
 template <typename T, int KIND>
@@ -440,7 +546,24 @@ static hipError_t spmv_k(const SpmvTune &t, const CsrDev &A0, const T *val, cons
   } while (0)
 #define SSF_VEC(G) SSF_LAUNCH((k_spmv_vec<T, G, KIND>), A, val, x, y, b, partial, done, iter)
 #define SSF_STR(I, G) SSF_LAUNCH((k_spmv_stream<T, I, G, KIND>), A, val, x, y, b, partial, done, iter)
-  if (t.format == 2 && A.nslice > 0 && t.idx16 == 3 && A.sten_n > 0 &&
+  const bool have_half = sizeof(T) == 8 ? A.half_v64 != nullptr : A.half_v32 != nullptr;
+  if (t.format == 2 && A.nslice > 0 && t.idx16 == 4 && A.half_nl > 0 && have_half) {
+    const dim3 hblock(t.lanes == 2 ? 512 : 256);
+#define SSF_HALF(NL)                                                                        \
+  case NL:                                                                                  \
+    if (ev0) hipExtLaunchKernelGGL((k_spmv_half<T, KIND, NL>), grid, hblock, 0, s, ev0, ev1, 0, \
+                                   A, x, y, b, partial, done, iter);                        \
+    else hipLaunchKernelGGL((k_spmv_half<T, KIND, NL>), grid, hblock, 0, s, A, x, y, b,     \
+                            partial, done, iter);                                           \
+    break;
+    switch (A.half_nl) {
+      SSF_HALF(1) SSF_HALF(2) SSF_HALF(3) SSF_HALF(4) SSF_HALF(5) SSF_HALF(6) SSF_HALF(7)
+      SSF_HALF(8) SSF_HALF(9) SSF_HALF(10) SSF_HALF(11) SSF_HALF(12) SSF_HALF(13)
+      SSF_HALF(14) SSF_HALF(15)
+      default: return hipErrorInvalidValue;
+    }
+#undef SSF_HALF
+  } else if (t.format == 2 && A.nslice > 0 && t.idx16 >= 3 && A.sten_n > 0 &&
       (A.sten_b != 1 || A.sten_n > SELL_STEN_MAX || t.lanes == 3 || t.lanes == 4)) {
     // groups per round: t.lanes 3 or 4 takes that many, else 7 (5 for blocks of 2)
 #define SSF_BSTEN(D, B) SSF_LAUNCH((k_spmv_sell_bsten<T, KIND, D, B>), A, x, y, b, partial, done, iter)
@@ -458,7 +581,7 @@ static hipError_t spmv_k(const SpmvTune &t, const CsrDev &A0, const T *val, cons
       else SSF_BSTEN(7, 1);
     }
 #undef SSF_BSTEN
-  } else if (t.format == 2 && A.nslice > 0 && t.idx16 == 3 && A.sten_n > 0) {
+  } else if (t.format == 2 && A.nslice > 0 && t.idx16 >= 3 && A.sten_n > 0) {
     switch ((A.sten_n + SELL_V - 1) / SELL_V) {
 #define SSF_STEN(G) case G: SSF_LAUNCH((k_spmv_sell_sten<T, KIND, G>), A, x, y, b, partial, done, iter); break;
       SSF_STEN(1) SSF_STEN(2) SSF_STEN(3) SSF_STEN(4) SSF_STEN(5) SSF_STEN(6) SSF_STEN(7)
@@ -565,6 +688,18 @@ __global__ __launch_bounds__(TPB) void k_pcg_init2(const T *__restrict__ part, i
   }
 }
 
+// 16-byte groups of a vector: four FP32 or two FP64 rows per access. Every vector
+// is its own allocation, so row chunks aligned to a group are 16-byte aligned.
+template <typename T>
+struct Grp {
+  static constexpr int L = 16 / (int)sizeof(T);
+  typedef T v __attribute__((ext_vector_type(16 / sizeof(T))));
+};
+// groups per thread loaded before a kernel knows the scalars of the previous
+// one (the re-reduction of its partials), so the two latencies overlap; a
+// block's chunk of a 2M-row vector (512 to 1024 blocks) takes 1 to 4 of them
+constexpr int PCG_PF = 4;
+
 template <typename T, bool JAC>
 __global__ __launch_bounds__(TPB) void k_pcg_update(int n, PcgState<T> *st,
                                                     const T *__restrict__ pden, int npden,
@@ -575,7 +710,24 @@ __global__ __launch_bounds__(TPB) void k_pcg_update(int n, PcgState<T> *st,
                                                     T *__restrict__ h, T *__restrict__ prr)
 {
   if (st->done) return;
+  typedef typename Grp<T>::v V;
+  constexpr int L = Grp<T>::L;
   const int nb = gridDim.x;
+  int r0, r1;
+  chunk_al<L>(n, r0, r1);
+  const int rL = r0 + (r1 - r0) / L * L;
+  V xv[PCG_PF], dv[PCG_PF], pv[PCG_PF], rv[PCG_PF], gv[PCG_PF];
+#pragma unroll
+  for (int j = 0; j < PCG_PF; j++) {
+    const int i = r0 + L * (threadIdx.x + j * TPB);
+    if (i < rL) {
+      xv[j] = *(const V *)(x + i);
+      dv[j] = *(const V *)(d + i);
+      pv[j] = *(const V *)(p + i);
+      rv[j] = *(const V *)(r + i);
+      if (JAC) gv[j] = *(const V *)(diag + i);
+    }
+  }
   const T den = reduce_partials(pden, npden);
   const long long it = st->iter;
   const T nom = st->nom[it & 1];
@@ -589,35 +741,40 @@ __global__ __launch_bounds__(TPB) void k_pcg_update(int n, PcgState<T> *st,
     return;
   }
   const T alpha = nom / den;
-  int r0, r1;
   T rr = T(0), rh = T(0);
-  if constexpr (std::is_same<T, float>::value) {
-    // FP32 (the mixed inner solve): four rows per 16-byte access; every vector is
-    // its own allocation, so row chunks aligned to 4 rows are 16-byte aligned
-    chunk_al<4>(n, r0, r1);
-    const int r4 = r0 + (r1 - r0) / 4 * 4;
-    for (int i = r0 + 4 * threadIdx.x; i < r4; i += 4 * TPB) {
-      float4 xv = *(const float4 *)(x + i);
-      const float4 dv = *(const float4 *)(d + i), pv = *(const float4 *)(p + i);
-      float4 rv = *(const float4 *)(r + i);
-      xv.x += alpha * dv.x; xv.y += alpha * dv.y; xv.z += alpha * dv.z; xv.w += alpha * dv.w;
-      rv.x -= alpha * pv.x; rv.y -= alpha * pv.y; rv.z -= alpha * pv.z; rv.w -= alpha * pv.w;
-      rr += rv.x * rv.x; rr += rv.y * rv.y; rr += rv.z * rv.z; rr += rv.w * rv.w;
-      if (JAC) {
-        const float4 gv = *(const float4 *)(diag + i);
-        float4 hv;
-        hv.x = rv.x / gv.x; hv.y = rv.y / gv.y; hv.z = rv.z / gv.z; hv.w = rv.w / gv.w;
-        rh += rv.x * hv.x; rh += rv.y * hv.y; rh += rv.z * hv.z; rh += rv.w * hv.w;
-        *(float4 *)(h + i) = hv;
-      }
-      *(float4 *)(x + i) = xv;
-      *(float4 *)(r + i) = rv;
+  // x += a d, r -= a p, h = r / diag, partials r.r and r.h, rows in order per thread
+  auto group = [&](V &xg, const V &dg, const V &pg, V &rg, const V &gg, int i) {
+#pragma unroll
+    for (int c = 0; c < L; c++) {
+      xg[c] += alpha * dg[c];
+      rg[c] -= alpha * pg[c];
+      rr += rg[c] * rg[c];
     }
-    r0 = r4;  // the last rows of the chunk one by one, below
-  } else {
-    chunk(n, r0, r1);
+    if (JAC) {
+      V hg;
+#pragma unroll
+      for (int c = 0; c < L; c++) {
+        hg[c] = rg[c] / gg[c];
+        rh += rg[c] * hg[c];
+      }
+      *(V *)(h + i) = hg;
+    }
+    *(V *)(x + i) = xg;
+    *(V *)(r + i) = rg;
+  };
+#pragma unroll
+  for (int j = 0; j < PCG_PF; j++) {
+    const int i = r0 + L * (threadIdx.x + j * TPB);
+    if (i < rL) group(xv[j], dv[j], pv[j], rv[j], gv[j], i);
   }
-  for (int i = r0 + threadIdx.x; i < r1; i += TPB) {
+  for (int i = r0 + L * (threadIdx.x + PCG_PF * TPB); i < rL; i += L * TPB) {
+    V xg = *(const V *)(x + i), rg = *(const V *)(r + i);
+    const V dg = *(const V *)(d + i), pg = *(const V *)(p + i);
+    V gg;
+    if (JAC) gg = *(const V *)(diag + i);
+    group(xg, dg, pg, rg, gg, i);
+  }
+  for (int i = rL + threadIdx.x; i < r1; i += TPB) {  // the chunk's last rows
     x[i] += alpha * d[i];
     const T ri = r[i] - alpha * p[i];
     r[i] = ri;
@@ -636,12 +793,55 @@ __global__ __launch_bounds__(TPB) void k_pcg_update(int n, PcgState<T> *st,
   }
 }
 
+// d = beta d + h over the block's chunk, the first groups of each thread given
+template <typename T>
+__device__ __forceinline__ void pcg_dir_rows(int r0, int rL, int r1, T beta, T *__restrict__ d,
+                                             const T *__restrict__ h,
+                                             const typename Grp<T>::v *dv,
+                                             const typename Grp<T>::v *hv)
+{
+  typedef typename Grp<T>::v V;
+  constexpr int L = Grp<T>::L;
+#pragma unroll
+  for (int j = 0; j < PCG_PF; j++) {
+    const int i = r0 + L * (threadIdx.x + j * TPB);
+    if (i < rL) {
+      V dg;
+#pragma unroll
+      for (int c = 0; c < L; c++) dg[c] = beta * dv[j][c] + hv[j][c];
+      *(V *)(d + i) = dg;
+    }
+  }
+  for (int i = r0 + L * (threadIdx.x + PCG_PF * TPB); i < rL; i += L * TPB) {
+    V dg = *(const V *)(d + i);
+    const V hg = *(const V *)(h + i);
+#pragma unroll
+    for (int c = 0; c < L; c++) dg[c] = beta * dg[c] + hg[c];
+    *(V *)(d + i) = dg;
+  }
+  for (int i = rL + threadIdx.x; i < r1; i += TPB) d[i] = beta * d[i] + h[i];
+}
+
 template <typename T, bool JAC, int MODE>
 __global__ __launch_bounds__(TPB) void k_pcg_dir(int n, PcgState<T> *st,
                                                  const T *__restrict__ prr, T res, T zero,
                                                  T *__restrict__ d, const T *__restrict__ h)
 {
   if (st->done) return;
+  typedef typename Grp<T>::v V;
+  constexpr int L = Grp<T>::L;
+  int r0, r1;
+  chunk_al<L>(n, r0, r1);
+  const int rL = r0 + (r1 - r0) / L * L;
+  V dv[PCG_PF], hv[PCG_PF];
+#pragma unroll
+  for (int j = 0; j < PCG_PF; j++) {
+    const int i = r0 + L * (threadIdx.x + j * TPB);
+    if (i < rL) {
+      dv[j] = *(const V *)(d + i);
+      hv[j] = *(const V *)(h + i);
+    }
+  }
   const int nb = gridDim.x;
   const T rr = reduce_partials(prr, nb);
   const T nomn = JAC ? reduce_partials(prr + nb, nb) : rr;
@@ -651,23 +851,7 @@ __global__ __launch_bounds__(TPB) void k_pcg_dir(int n, PcgState<T> *st,
   if (MODE == MODE_MIXED_INNER) {
     // the direction is always updated, then rr < nory (the refresh threshold)
     // pauses the recurrence until the FP64 refresh
-    const T beta = nomn / nom;
-    int r0, r1;
-    if constexpr (std::is_same<T, float>::value) {  // four rows per 16-byte access
-      chunk_al<4>(n, r0, r1);
-      const int r4 = r0 + (r1 - r0) / 4 * 4;
-      for (int i = r0 + 4 * threadIdx.x; i < r4; i += 4 * TPB) {
-        float4 dv = *(const float4 *)(d + i);
-        const float4 hv = *(const float4 *)(h + i);
-        dv.x = beta * dv.x + hv.x; dv.y = beta * dv.y + hv.y;
-        dv.z = beta * dv.z + hv.z; dv.w = beta * dv.w + hv.w;
-        *(float4 *)(d + i) = dv;
-      }
-      r0 = r4;
-    } else {
-      chunk(n, r0, r1);
-    }
-    for (int i = r0 + threadIdx.x; i < r1; i += TPB) d[i] = beta * d[i] + h[i];
+    pcg_dir_rows<T>(r0, rL, r1, nomn / nom, d, h, dv, hv);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       st->nom[(it + 1) & 1] = nomn;
       if (rr < nory) {
@@ -692,10 +876,7 @@ __global__ __launch_bounds__(TPB) void k_pcg_dir(int n, PcgState<T> *st,
     }
     return;
   }
-  const T beta = nomn / nom;
-  int r0, r1;
-  chunk(n, r0, r1);
-  for (int i = r0 + threadIdx.x; i < r1; i += TPB) d[i] = beta * d[i] + h[i];
+  pcg_dir_rows<T>(r0, rL, r1, nomn / nom, d, h, dv, hv);
   if (blockIdx.x == 0 && threadIdx.x == 0) st->nom[(it + 1) & 1] = nomn;
 }
 
@@ -892,7 +1073,9 @@ __global__ void k_inv_sqrt_diag(int n, const double *__restrict__ diag, double *
   isd[i] = a > 0.0 ? 1.0 / sqrt(a) : 1.0;
 }
 
-// the scaled FP32 copy of the full CSR: v32[k] = float(v[k] isd[i] isd[col[k]])
+// the scaled FP32 copy of the full CSR: v32[k] = float(v[k] (isd[i] isd[col[k]])), the
+// scale product taken first so that mirrored entries round alike (a symmetric
+// operator stays bitwise symmetric)
 __global__ void k_csr_scale32(int n, const int *__restrict__ rowptr, const int *__restrict__ col,
                               const double *__restrict__ v, const double *__restrict__ isd,
                               float *__restrict__ v32)
@@ -900,7 +1083,7 @@ __global__ void k_csr_scale32(int n, const int *__restrict__ rowptr, const int *
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double si = isd[i];
-  for (int k = rowptr[i]; k < rowptr[i + 1]; k++) v32[k] = (float)(v[k] * si * isd[col[k]]);
+  for (int k = rowptr[i]; k < rowptr[i + 1]; k++) v32[k] = (float)(v[k] * (si * isd[col[k]]));
 }
 
 // the same for the SELL-64 values; one thread per layout row. A slot's column
@@ -924,7 +1107,7 @@ __global__ void k_sell_scale32(CsrDev A, const double *__restrict__ v,
     else if (A.sell_pid) c = rc + A.sell_dict[A.sell_dict_off[A.sell_pid[r]] + q];
     else c = r - r % A.sten_b + A.sten_d[q];
     c = min(max(c, 0), A.n - 1);
-    v32[k] = (float)(v[k] * si * isd[c]);
+    v32[k] = (float)(v[k] * (si * isd[c]));
   }
 }
 

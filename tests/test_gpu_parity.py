@@ -282,6 +282,72 @@ def test_sell_stencil_form(ctx):
     m.close()
 
 
+def test_sell_half_stencil_form(ctx):
+    """Symmetric half of the stencil form (index form 4): the lower slots of every
+    row stored once, diagonal-major, and the upper entries read from the mirrored
+    rows. Products equal the column forms' bit for bit at every block count, block
+    size and slice map; FP64 and mixed solves on the same block partition equal the
+    stencil form's bit for bit (the FP32 operator is scaled symmetrically, so its
+    mirrored entries agree); the form is the default of a heat matrix, and refused
+    where the storage or the stencil is not symmetric about the row."""
+    # (cube, lower slots and diagonal): 27-point stencils; one free plane of 3 x 3 nodes
+    for prob, nlow in ((mesh.heat_cube(17, seed=9), 14), (mesh.heat_cube(6, seed=10), 14),
+                       (mesh.heat_cube(2, seed=3), 5)):
+        adr, ci, a, rhs = O.assemble_problem(prob, "scr")
+        m = gpu.Matrix.from_csr(ctx, gpu.STORAGE_SCR, adr, ci)
+        m.set_values(a)
+        n = prob.n
+        # the default tiling streams the lower slots and the diagonal of every row
+        can, act = m.spmv_bytes(gpu.FP64)
+        assert act == nlow * 8.0 * n + 16.0 * n, (prob.name, act)
+        b = np.random.default_rng(6).standard_normal(n)
+        m.set_tune(gpu.FP64, (2, 0, 64, 0))
+        y0 = m.gmxv(b)
+        assert rel(y0, O.mxv_scr(adr, ci, a, b)) <= 1e-14
+        for nb in (1, 3, 64, 512):
+            for tb in (1, 2):
+                for mp in (0, 1):
+                    m.set_tune(gpu.FP64, (2, tb, nb, 4, mp))
+                    assert np.array_equal(m.gmxv(b), y0), (prob.name, nb, tb, mp)
+        for precision in (gpu.FP64, gpu.MIXED):
+            m.set_tune(gpu.FP64, (2, 0, 64, 3))
+            m.set_tune(gpu.MIXED, (2, 0, 64, 3))
+            x3, i3 = m.solve(rhs[0], ni=20000, res=1e-22, precision=precision)
+            m.set_tune(gpu.FP64, (2, 1, 64, 4))
+            m.set_tune(gpu.MIXED, (2, 1, 64, 4))
+            x4, i4 = m.solve(rhs[0], ni=20000, res=1e-22, precision=precision)
+            assert i4.status == i3.status and i4.iterations == i3.iterations, (prob.name, precision)
+            assert np.array_equal(x4, x3), (prob.name, precision)
+        # other partitions (blocks of 512 threads, the contiguous slice map) reduce the
+        # dot products in another order: solves within tolerance of the oracle
+        xo, *_ = O.scr_cg_prec(adr, ci, a, rhs[0], 50000, 1e-30)
+        m.set_tune(gpu.FP64, (2, 2, 8, 4, 0))
+        m.set_tune(gpu.MIXED, (2, 2, 24, 4, 1))
+        for precision in (gpu.FP64, gpu.MIXED):
+            x4, i4 = m.solve(rhs[0], ni=20000, res=1e-22, precision=precision)
+            assert i4.status in (1, 4) and rel(x4, xo) <= 1e-8, (prob.name, precision)
+        # a values change rebuilds the half layout
+        m.set_values(2.0 * a)
+        m.set_tune(gpu.FP64, (2, 1, 64, 4))
+        assert np.array_equal(m.gmxv(b), 2.0 * y0)
+        m.close()
+        # comprow storage (values need not be symmetric): no half form
+        ac, cc, va, _ = O.assemble_problem(prob, "cr")
+        c = gpu.Matrix.from_csr(ctx, gpu.STORAGE_CR, ac, cc)
+        c.set_values(va)
+        with pytest.raises(gpu.SsfError):
+            c.set_tune(gpu.FP64, (2, 1, 64, 4))
+        c.close()
+    # the block stencil form of elasticity rows has no half form
+    prob = mesh.cantilever(8, 3, 3)
+    adr, ci, a, rhs = O.assemble_problem(prob, "scr")
+    m = gpu.Matrix.from_csr(ctx, gpu.STORAGE_SCR, adr, ci)
+    m.set_values(a)
+    with pytest.raises(gpu.SsfError):
+        m.set_tune(gpu.FP64, (2, 1, 64, 4))
+    m.close()
+
+
 @pytest.mark.parametrize("name", ["cantilever", "plate"])
 def test_sell_block_stencil_form(ctx, name):
     """Block stencil form: with the unknowns of a node numbered together (3 for
