@@ -1580,10 +1580,22 @@ static int solve_mixed(ssf_matrix *m, const ssf_solve_params *p, const double *y
   bool pending = false;
   int rc = SSF_OK;
   bool sample_valid[2] = {true, false};
+  // Chunk lengths. A chunk's iterations after the recurrence paused, and every
+  // iteration of a chunk enqueued after the one that converged, exit early (a
+  // few us each). The host knows the outcome of chunk k - 1 when it enqueues
+  // chunk k + 1: from the FP64 residual's rate of decrease per iteration (the
+  // smaller of its mean since the start and its rate over the last known chunk,
+  // as the residual of CG is irregular chunk by chunk and slows late) it
+  // estimates the iterations still needed and makes chunk k + 1 that long plus a
+  // margin, at most the interval; a chunk that ends early only refreshes sooner.
+  int64_t clen[2] = {interval, interval}, cnext = interval, iter_seen = 0;
+  const double rr_start = slots[0];
+  double rr_seen = slots[0];
   while (true) {
     bool launched_now = false;
     if (it < p->ni) {
-      const int64_t c = std::min<int64_t>(interval, p->ni - it);
+      const int64_t c = std::min<int64_t>(cnext, p->ni - it);
+      clen[k & 1] = c;
       // sampled: the chunk's second FP32 SpMV (the first follows the FP64 refresh
       // and starts on cold caches; the inner recurrence never stops after one step)
       const int64_t qs = c > 1 ? 1 : 0;
@@ -1605,6 +1617,8 @@ static int solve_mixed(ssf_matrix *m, const ssf_solve_params *p, const double *y
       }
       it += c;
       (void)hipMemcpyAsync(&w.mix_host[k & 1], w.mix, sizeof(MixState), hipMemcpyDeviceToHost, s);
+      (void)hipMemcpyAsync(&w.st32_host[k & 1], w.st32, sizeof(PcgState<float>),
+                           hipMemcpyDeviceToHost, s);
       (void)hipEventRecord(ev[k & 1], s);
       launched_now = true;
     }
@@ -1613,6 +1627,22 @@ static int solve_mixed(ssf_matrix *m, const ssf_solve_params *p, const double *y
       if (m->ctx->prof && sample_valid[(k - 1) & 1]) prof_add(m->ctx, 1, (k - 1) & 1);
       sample_valid[k & 1] = !w.mix_host[(k - 1) & 1].final;
       if (w.mix_host[(k - 1) & 1].final) break;
+      // iterations chunk k - 1 ran (st.iter counts the real ones, from -1) and the
+      // residual reduction they bought
+      const MixState &mk = w.mix_host[(k - 1) & 1];
+      const int64_t seen = w.st32_host[(k - 1) & 1].iter + 1;
+      cnext = interval;
+      const double target = p->res * mk.nory64;
+      if (seen > iter_seen && mk.rr64 > 0.0 && mk.rr64 < rr_seen && target > 0.0) {
+        const double rate = std::min(std::log(rr_start / mk.rr64) / (double)seen,
+                                     std::log(rr_seen / mk.rr64) / (double)(seen - iter_seen));
+        // still needed after chunk k (already enqueued)
+        const double need = std::log(std::max(mk.rr64 / target, 1.0)) / rate - (double)clen[k & 1];
+        const int64_t want = need <= 0.0 ? 8 : (int64_t)std::ceil(1.1 * need) + 8;
+        cnext = std::min<int64_t>(interval, std::max<int64_t>(8, want));
+      }
+      iter_seen = seen;
+      rr_seen = mk.rr64;
     }
     if (!launched_now) break;
     pending = true;

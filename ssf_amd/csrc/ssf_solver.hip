@@ -435,6 +435,10 @@ __device__ __forceinline__ const float *half_values<float>(const CsrDev &A)
   return A.half_v32;
 }
 
+// KIND as the other SpMV kernels, and 3: p = A d with the block partials of d.p,
+// r.r, r.p and p.p (b = r) at partial[j * gridDim.x + block], j = 0..3, for the
+// fused step of the FP32 recurrence (k_cg_fused_step).
+//
 // Symmetric half of the stencil form (symcomprow matrices whose stencil list is
 // symmetric, sten[NL] = 0). Each row holds its NL + 1 lower slots, diagonal-major
 // (hv[k * pad + r] = a(r, r + sten[k])); the upper entry a(r, r - sten[k]) is
@@ -459,7 +463,7 @@ __global__ __launch_bounds__(512) void k_spmv_half(CsrDev A, const T *__restrict
   wave_slices(A.nslice, A.slice_map, wave, sfirst, slast, sstep, W);
   const int nm1 = A.n - 1;
   const int64_t pad = A.half_pad;
-  T acc = T(0);
+  T acc = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
   for (int sl = sfirst; sl < slast; sl += sstep) {
     const int r = sl * 64 + lane;
     T vo[NL + 1], vm[NL], gl[NL], gu[NL];
@@ -480,7 +484,13 @@ __global__ __launch_bounds__(512) void k_spmv_half(CsrDev A, const T *__restrict
 #pragma unroll
     for (int k = NL - 1; k >= 0; k--) sum += vm[k] * gu[k];
     if (r < A.n) {
-      if (KIND == 2) {
+      if (KIND == 3) {  // the dots of the fused CG step; b holds the residual
+        const T rv = b[r];
+        acc += gd * sum;
+        a1 += rv * rv;
+        a2 += rv * sum;
+        a3 += sum * sum;
+      } else if (KIND == 2) {
         sum = b[r] - sum;
         acc += sum * sum;
       } else if (KIND == 1) {
@@ -489,7 +499,25 @@ __global__ __launch_bounds__(512) void k_spmv_half(CsrDev A, const T *__restrict
       y[r] = sum;
     }
   }
-  if (KIND != 0) {
+  if (KIND == 3) {
+    __shared__ T lds4[4][8];
+    acc = wave_sum(acc);
+    a1 = wave_sum(a1);
+    a2 = wave_sum(a2);
+    a3 = wave_sum(a3);
+    if (lane == 0) {
+      lds4[0][wave] = acc;
+      lds4[1][wave] = a1;
+      lds4[2][wave] = a2;
+      lds4[3][wave] = a3;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+      T t = lds4[threadIdx.x][0];
+      for (int i = 1; i < W; i++) t += lds4[threadIdx.x][i];
+      partial[threadIdx.x * gridDim.x + blockIdx.x] = t;
+    }
+  } else if (KIND != 0) {
     __shared__ T lds[8];
     acc = wave_sum(acc);
     if (lane == 0) lds[wave] = acc;
@@ -634,6 +662,32 @@ template hipError_t spmv<double>(const SpmvTune &, int, const CsrDev &, const do
 template hipError_t spmv<float>(const SpmvTune &, int, const CsrDev &, const float *,
                                 const float *, float *, const float *, float *, const int *,
                                 long long *, hipStream_t, hipEvent_t, hipEvent_t);
+
+// p = A d in the half stencil form with the four dots of the fused CG step
+template <typename T>
+static hipError_t spmv_half_dots(const SpmvTune &t, const CsrDev &A0, const T *d, T *p,
+                                 const T *r, T *partial, const int *done, long long *iter,
+                                 hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
+{
+  CsrDev A = A0;
+  A.slice_map = t.map;
+  const dim3 grid(t.blocks), block(t.lanes == 2 ? 512 : 256);
+#define SSF_HALF3(NL)                                                                        \
+  case NL:                                                                                   \
+    if (ev0) hipExtLaunchKernelGGL((k_spmv_half<T, 3, NL>), grid, block, 0, s, ev0, ev1, 0,   \
+                                   A, d, p, r, partial, done, iter);                         \
+    else hipLaunchKernelGGL((k_spmv_half<T, 3, NL>), grid, block, 0, s, A, d, p, r, partial, \
+                            done, iter);                                                     \
+    break;
+  switch (A.half_nl) {
+    SSF_HALF3(1) SSF_HALF3(2) SSF_HALF3(3) SSF_HALF3(4) SSF_HALF3(5) SSF_HALF3(6) SSF_HALF3(7)
+    SSF_HALF3(8) SSF_HALF3(9) SSF_HALF3(10) SSF_HALF3(11) SSF_HALF3(12) SSF_HALF3(13)
+    SSF_HALF3(14) SSF_HALF3(15)
+    default: return hipErrorInvalidValue;
+  }
+#undef SSF_HALF3
+  return hipGetLastError();
+}
 
 // ----------------------------------------------------------------------- PCG
 
@@ -880,6 +934,106 @@ __global__ __launch_bounds__(TPB) void k_pcg_dir(int n, PcgState<T> *st,
   if (blockIdx.x == 0 && threadIdx.x == 0) st->nom[(it + 1) & 1] = nomn;
 }
 
+// One step of the FP32 recurrence of the mixed solve after p = A d
+// (k_spmv_half<T, 3>): alpha = r.r / d.p, x += alpha d, r -= alpha p, and with
+// ||r - alpha p||^2 = r.r - 2 alpha r.p + alpha^2 p.p known from the same dots,
+// beta and d = r + beta d in the same pass. Two kernels per iteration instead of
+// three; r.r is the true dot of every step, so the recurrence's estimate carries
+// one step's rounding only. As in MODE_MIXED_INNER, the direction is updated and
+// then ||r||^2 < nory pauses the recurrence until the FP64 refresh.
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_cg_fused_step(int n, PcgState<T> *st,
+                                                       const T *__restrict__ part, int np,
+                                                       T zero, T *__restrict__ x,
+                                                       T *__restrict__ r, T *__restrict__ d,
+                                                       const T *__restrict__ p)
+{
+  if (st->done) return;
+  typedef typename Grp<T>::v V;
+  constexpr int L = Grp<T>::L;
+  int r0, r1;
+  chunk_al<L>(n, r0, r1);
+  const int rL = r0 + (r1 - r0) / L * L;
+  V xv[PCG_PF], dv[PCG_PF], pv[PCG_PF], rv[PCG_PF];
+#pragma unroll
+  for (int j = 0; j < PCG_PF; j++) {
+    const int i = r0 + L * (threadIdx.x + j * TPB);
+    if (i < rL) {
+      xv[j] = *(const V *)(x + i);
+      dv[j] = *(const V *)(d + i);
+      pv[j] = *(const V *)(p + i);
+      rv[j] = *(const V *)(r + i);
+    }
+  }
+  // the four dots, re-reduced in a fixed order by every block
+  T q[4] = {T(0), T(0), T(0), T(0)};
+  for (int i = threadIdx.x; i < np; i += TPB)
+#pragma unroll
+    for (int j = 0; j < 4; j++) q[j] += part[j * np + i];
+  __shared__ T lds[4][TPB / 64];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    q[j] = wave_sum(q[j]);
+    if ((threadIdx.x & 63) == 0) lds[j][threadIdx.x >> 6] = q[j];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    q[j] = lds[j][0];
+#pragma unroll
+    for (int w = 1; w < TPB / 64; w++) q[j] += lds[j][w];
+  }
+  const T dp = q[0], rr = q[1], rp = q[2], pp = q[3];
+  const long long it = st->iter;
+  if (fabs(dp) < zero) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->status = 2;
+      st->ani = it;
+      st->ares = rr;
+      st->done = 1;
+    }
+    return;
+  }
+  const T alpha = rr / dp;
+  const T rrn = rr - T(2) * alpha * rp + alpha * alpha * pp;
+  const T beta = rrn / rr;
+  auto group = [&](V &xg, V &dg, const V &pg, V &rg, int i) {
+#pragma unroll
+    for (int c = 0; c < L; c++) {
+      xg[c] += alpha * dg[c];
+      rg[c] -= alpha * pg[c];
+      dg[c] = rg[c] + beta * dg[c];
+    }
+    *(V *)(x + i) = xg;
+    *(V *)(r + i) = rg;
+    *(V *)(d + i) = dg;
+  };
+#pragma unroll
+  for (int j = 0; j < PCG_PF; j++) {
+    const int i = r0 + L * (threadIdx.x + j * TPB);
+    if (i < rL) group(xv[j], dv[j], pv[j], rv[j], i);
+  }
+  for (int i = r0 + L * (threadIdx.x + PCG_PF * TPB); i < rL; i += L * TPB) {
+    V xg = *(const V *)(x + i), dg = *(const V *)(d + i), rg = *(const V *)(r + i);
+    const V pg = *(const V *)(p + i);
+    group(xg, dg, pg, rg, i);
+  }
+  for (int i = rL + threadIdx.x; i < r1; i += TPB) {  // the chunk's last rows
+    x[i] += alpha * d[i];
+    const T ri = r[i] - alpha * p[i];
+    r[i] = ri;
+    d[i] = ri + beta * d[i];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->nom[(it + 1) & 1] = rrn;
+    if (rrn < st->nory) {
+      st->ani = it;
+      st->status = 1;
+      st->done = 1;
+    }
+  }
+}
+
 // the element-wise PCG kernels use at most this many blocks
 static int vec_blocks(const SpmvTune &t, int n)
 {
@@ -924,6 +1078,16 @@ hipError_t pcg_iteration(const SpmvTune &t, int mode, bool jac, const CsrDev &A,
                          PcgState<T> *st, T res, T zero, hipStream_t s, hipEvent_t ev0,
                          hipEvent_t ev1)
 {
+  const bool have_half = sizeof(T) == 8 ? A.half_v64 != nullptr : A.half_v32 != nullptr;
+  if (mode == MODE_MIXED_INNER && !jac && t.format == 2 && t.idx16 == 4 && A.half_nl > 0 &&
+      have_half) {
+    // the FP32 recurrence of the mixed solve in two kernels (k_cg_fused_step)
+    hipError_t e = spmv_half_dots<T>(t, A, d, p, r, part, &st->done, &st->iter, s, ev0, ev1);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_cg_fused_step<T>), dim3(vec_blocks(t, A.n)), dim3(TPB), 0, s, A.n, st,
+                       part, t.blocks, zero, x, r, d, p);
+    return hipGetLastError();
+  }
   hipError_t e = spmv<T>(t, 1, A, val, d, p, nullptr, part, &st->done, &st->iter, s, ev0, ev1);
   if (e != hipSuccess) return e;
   const int nv = vec_blocks(t, A.n);

@@ -286,10 +286,12 @@ def test_sell_half_stencil_form(ctx):
     """Symmetric half of the stencil form (index form 4): the lower slots of every
     row stored once, diagonal-major, and the upper entries read from the mirrored
     rows. Products equal the column forms' bit for bit at every block count, block
-    size and slice map; FP64 and mixed solves on the same block partition equal the
-    stencil form's bit for bit (the FP32 operator is scaled symmetrically, so its
-    mirrored entries agree); the form is the default of a heat matrix, and refused
-    where the storage or the stencil is not symmetric about the row."""
+    size and slice map, and FP64 solves on the same block partition equal the
+    stencil form's bit for bit. The mixed solve on this form runs its FP32
+    recurrence in two kernels per iteration (the SpMV kernel takes the four dots of
+    the step): within tolerance of the oracle in about the same iterations. The form
+    is the default of a heat matrix, and refused where the storage or the stencil
+    is not symmetric about the row."""
     # (cube, lower slots and diagonal): 27-point stencils; one free plane of 3 x 3 nodes
     for prob, nlow in ((mesh.heat_cube(17, seed=9), 14), (mesh.heat_cube(6, seed=10), 14),
                        (mesh.heat_cube(2, seed=3), 5)):
@@ -316,8 +318,14 @@ def test_sell_half_stencil_form(ctx):
             m.set_tune(gpu.FP64, (2, 1, 64, 4))
             m.set_tune(gpu.MIXED, (2, 1, 64, 4))
             x4, i4 = m.solve(rhs[0], ni=20000, res=1e-22, precision=precision)
-            assert i4.status == i3.status and i4.iterations == i3.iterations, (prob.name, precision)
-            assert np.array_equal(x4, x3), (prob.name, precision)
+            if precision == gpu.FP64:
+                assert i4.status == i3.status and i4.iterations == i3.iterations, prob.name
+                assert np.array_equal(x4, x3), prob.name
+            else:
+                assert i4.status == i3.status == 1, prob.name
+                assert abs(i4.iterations - i3.iterations) <= max(2, i3.iterations // 20), (
+                    prob.name, i4.iterations, i3.iterations)
+                assert rel(x4, x3) <= 1e-9, prob.name
         # other partitions (blocks of 512 threads, the contiguous slice map) reduce the
         # dot products in another order: solves within tolerance of the oracle
         xo, *_ = O.scr_cg_prec(adr, ci, a, rhs[0], 50000, 1e-30)
