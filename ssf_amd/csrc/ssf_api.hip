@@ -146,14 +146,78 @@ static SpmvTune default_tune(int64_t n, int64_t nnz, int ntiles, int nslice, boo
 // the symmetric half of the stencil form: a symcomprow matrix whose stencil list
 // (offsets from the row itself) is symmetric about its 0 slot; sets half_nl (the
 // strictly lower slots, at most 15) and half_pad, or leaves half_nl = 0
+// Block variant: the block stencil form of B = 2 or 3 unknowns per node whose
+// offsets are every unknown of a set of neighbour nodes symmetric about the node
+// itself (27 nodes of a hex8 mesh, 9 of a quad4 mesh); the stored entries get
+// their SELL slot and row in the block in half_slot (device).
+static void half_setup_block(ssf_matrix *m)
+{
+  const int B = m->sten_b, ns = (int)m->sten.size();
+  if (ns % B != 0 || m->n % B != 0) return;
+  std::vector<int> nodes;  // neighbour node offsets, ascending (sten is ascending)
+  for (int k = 0; k < ns; k++) {
+    const int d = m->sten[k], e = ((d % B) + B) % B, mo = (d - e) / B;
+    if (e != k % B) return;  // every unknown of every neighbour node, in order
+    if (e == 0) nodes.push_back(mo);
+    else if (nodes.back() != mo) return;
+  }
+  const int nm = (int)nodes.size(), nl = nm / 2;
+  if (nm % 2 == 0 || nodes[nl] != 0) return;
+  for (int k = 0; k < nl; k++)
+    if (nodes[k] != -nodes[nm - 1 - k]) return;
+  if (!((B == 3 && nl == 13) || (B == 2 && nl == 4))) return;  // the kernels' instances
+  const int64_t nn = m->n / B;
+  const int64_t pad = ((nn + 63) / 64 * 64 - nodes[0] + 63) / 64 * 64;
+  if (pad * B >= (int64_t(1) << 31)) return;
+  // (slot, row in block) of every stored entry: blocks j < nl, entry (t, e), then
+  // the diagonal block's lower triangle
+  std::vector<int> slot;
+  auto slot_of = [&](int d) {
+    return (int)(std::lower_bound(m->sten.begin(), m->sten.end(), d) - m->sten.begin());
+  };
+  for (int j = 0; j < nl; j++)
+    for (int t = 0; t < B; t++)
+      for (int e = 0; e < B; e++) {
+        slot.push_back(slot_of(B * nodes[j] + e));
+        slot.push_back(t);
+      }
+  for (int t = 0; t < B; t++)
+    for (int e = 0; e <= t; e++) {
+      slot.push_back(slot_of(e));
+      slot.push_back(t);
+    }
+  if (dalloc(&m->half_slot, slot.size(), m->ctx->stream) != hipSuccess ||
+      hipMemcpyAsync(m->half_slot, slot.data(), slot.size() * sizeof(int), hipMemcpyHostToDevice,
+                     m->ctx->stream) != hipSuccess ||
+      hipStreamSynchronize(m->ctx->stream) != hipSuccess) {
+    dfree(m->half_slot, m->ctx->stream);
+    return;
+  }
+  m->half_b = B;
+  m->half_m.assign(nodes.begin(), nodes.begin() + nl);
+  m->half_nl = nl;
+  m->half_pad = pad;
+}
+
+// values per row or node of the half form
+static int64_t half_stored(const ssf_matrix *m)
+{
+  const int B = m->half_b;
+  return B == 1 ? m->half_nl + 1 : (int64_t)m->half_nl * B * B + B * (B + 1) / 2;
+}
+
 static void half_setup(ssf_matrix *m)
 {
   m->half_nl = 0;
   m->half_pad = 0;
+  m->half_b = 1;
   const int ns = (int)m->sten.size();
-  if (m->storage != SSF_STORAGE_SYMM_COMP_ROWS || m->sten_b != 1 || ns % 2 == 0 || ns > 31 ||
-      m->nslice <= 0)
+  if (m->storage != SSF_STORAGE_SYMM_COMP_ROWS || m->nslice <= 0 || ns == 0) return;
+  if (m->sten_b == 2 || m->sten_b == 3) {
+    half_setup_block(m);
     return;
+  }
+  if (m->sten_b != 1 || ns % 2 == 0 || ns > 31) return;
   const int nl = ns / 2;
   if (m->sten[nl] != 0) return;
   for (int k = 0; k < nl; k++)
@@ -173,7 +237,8 @@ static void half_default_tune(ssf_matrix *m)
   t.format = 2;
   t.lanes = 1;
   t.idx16 = 4;
-  const int64_t nb = std::max<int64_t>(8, std::min<int64_t>(512, (m->nslice + 3) / 4));
+  const int64_t ns = (m->n / m->half_b + 63) / 64;  // slices of 64 rows or nodes
+  const int64_t nb = std::max<int64_t>(8, std::min<int64_t>(512, (ns + 3) / 4));
   t.blocks = (int)((nb + 7) / 8 * 8);
   m->tune64 = m->tune32 = t;
 }
@@ -518,6 +583,7 @@ extern "C" int ssf_matrix_destroy(ssf_matrix *m)
   dfree(m->sell_col16, m->ctx->stream);
   dfree(m->sten_d, m->ctx->stream);
   dfree(m->half_v64, m->ctx->stream); dfree(m->half_v32, m->ctx->stream);
+  dfree(m->half_slot, m->ctx->stream);
   dfree(m->sell_pid, m->ctx->stream); dfree(m->sell_dict, m->ctx->stream);
   dfree(m->sell_dict_off, m->ctx->stream);
   dfree(m->sell_lo, m->ctx->stream); dfree(m->sell_up, m->ctx->stream);
@@ -1041,15 +1107,26 @@ static int half_build(ssf_matrix *m, int precision)
 {
   if (m->half_nl == 0) return SSF_OK;
   hipStream_t s = m->ctx->stream;
-  const int64_t tot = (int64_t)(m->half_nl + 1) * m->half_pad;
+  const int nv = (int)half_stored(m);
+  // entries per row or node in memory (the block variant in 16-byte groups)
+  const int64_t per = m->half_b == 1 ? nv
+                      : (int64_t)half_groups(precision ? 4 : 8, m->half_b, m->half_nl) *
+                            (precision ? 4 : 2);
+  const int64_t tot = per * m->half_pad;
   if (precision == 0) {
     if (!m->sell_v64) return SSF_OK;
     if (!m->half_v64) SSF_HIP(dalloc(&m->half_v64, tot, s));
-    SSF_HIP(half_from_sell<double>(m->csr(), m->sell_v64, m->half_v64, s));
+    if (m->half_b > 1)
+      SSF_HIP(half_from_sell_block<double>(m->csr(), m->half_slot, nv, m->sell_v64, m->half_v64, s));
+    else
+      SSF_HIP(half_from_sell<double>(m->csr(), m->sell_v64, m->half_v64, s));
   } else {
     if (!m->sell_v32) return SSF_OK;
     if (!m->half_v32) SSF_HIP(dalloc(&m->half_v32, tot, s));
-    SSF_HIP(half_from_sell<float>(m->csr(), m->sell_v32, m->half_v32, s));
+    if (m->half_b > 1)
+      SSF_HIP(half_from_sell_block<float>(m->csr(), m->half_slot, nv, m->sell_v32, m->half_v32, s));
+    else
+      SSF_HIP(half_from_sell<float>(m->csr(), m->sell_v32, m->half_v32, s));
   }
   return SSF_OK;
 }
@@ -2043,7 +2120,7 @@ extern "C" int ssf_spmv_bytes(const ssf_matrix *m, int precision, double *canoni
   if (t.format == 1) a += 4.0 * (double)(m->ntiles + 1);
   if (t.format == 2 && m->nslice > 0) {  // padded slices, slice offsets instead of row pointers
     if (t.idx16 == 4 && m->half_nl > 0)  // half stencil form: the lower slots of every row once
-      a = (double)(m->half_nl + 1) * (double)m->n * sv + 2.0 * sv * (double)m->n;
+      a = (double)half_stored(m) * (double)(m->n / m->half_b) * sv + 2.0 * sv * (double)m->n;
     else if (t.idx16 == 3 && !m->sten.empty())  // stencil form: the values alone
       a = (double)m->sell_tot * sv + 2.0 * sv * (double)m->n;
     else if (t.idx16 == 2 && m->sell_pid)  // a 2-byte pattern id per layout row instead of columns

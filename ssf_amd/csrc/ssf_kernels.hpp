@@ -25,6 +25,11 @@ hipError_t spmv(const SpmvTune &t, int kind, const CsrDev &A, const T *val, cons
 // + r] = slot k of row r of sval for k <= A.half_nl, 0 for r >= n
 template <typename T>
 hipError_t half_from_sell(const CsrDev &A, const T *sval, T *hv, hipStream_t s);
+// the block variant (A.half_b > 1): stored entry i of every node from SELL slot
+// slot[2 i] of row half_b q + slot[2 i + 1]; nv entries per node
+template <typename T>
+hipError_t half_from_sell_block(const CsrDev &A, const int *slot, int nv, const T *sval, T *hv,
+                                hipStream_t s);
 
 // PCG pieces (see ssf_kernels.hip for the recurrence).
 template <typename T>

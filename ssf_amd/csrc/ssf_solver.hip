@@ -530,6 +530,183 @@ __global__ __launch_bounds__(512) void k_spmv_half(CsrDev A, const T *__restrict
   }
 }
 
+// Block variant of the half form (A.half_b = B = 2, 3; CsrDev::half_*): one lane
+// per node, its B rows summed in column order (lower neighbour nodes ascending,
+// the diagonal block, the upper nodes ascending; within a node its unknowns in
+// order), which is the block stencil form's slot order, so the products equal
+// the other forms' bit for bit. A slice of 64 nodes keeps its entries together
+// in 16-byte groups per lane: a wavefront streams one contiguous run per slice,
+// and a B x B block takes a few 16-byte loads (3 for FP32 3 x 3 blocks) instead
+// of B^2 single ones. The upper block towards node q - m_j is the transpose of
+// block j of that node, read from the same groups of its slice.
+template <typename T, int S, int G>
+__device__ __forceinline__ void half_groups_load(const T *__restrict__ base, int g0, T *ev)
+{
+  typedef T gv __attribute__((ext_vector_type(G)));
+#pragma unroll
+  for (int u = 0; u < S; u++) {
+    const gv v = *(const gv *)(base + (g0 + u) * 64 * G);
+#pragma unroll
+    for (int c = 0; c < G; c++) ev[u * G + c] = v[c];
+  }
+}
+
+template <typename T, int KIND, int B, int NLN>
+__global__ __launch_bounds__(512) void k_spmv_halfb(CsrDev A, const T *__restrict__ x,
+                                                    T *__restrict__ y, const T *__restrict__ b,
+                                                    T *__restrict__ partial, const int *done,
+                                                    long long *iter)
+{
+  if (done && *done) return;
+  if (iter && blockIdx.x == 0 && threadIdx.x == 0) *iter += 1;
+  const T *__restrict__ hv = half_values<T>(A);
+  const int W = blockDim.x / 64;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), lane = threadIdx.x % 64;
+  const int nn = A.n / B;
+  int sfirst, slast, sstep;
+  wave_slices((nn + 63) / 64, A.slice_map, wave, sfirst, slast, sstep, W);
+  constexpr int G = 16 / (int)sizeof(T);                     // entries per group
+  constexpr int BB = B * B, ND = NLN * BB;                  // the diagonal block's first entry
+  constexpr int NG = half_groups((int)sizeof(T), B, NLN);   // groups per node
+  constexpr int S = (G - 1 + BB + G - 1) / G;               // groups a block spans
+  constexpr int SD = (G - 1 + B * (B + 1) / 2 + G - 1) / G;  // groups the diagonal spans
+  T acc = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
+  for (int sl = sfirst; sl < slast; sl += sstep) {
+    const int q = sl * 64 + lane;
+    const T *__restrict__ own = hv + ((int64_t)sl * NG * 64 + lane) * G;
+    T sum[B];
+#pragma unroll
+    for (int t = 0; t < B; t++) sum[t] = T(0);
+#pragma unroll
+    for (int j = 0; j < NLN; j++) {  // lower neighbour nodes
+      const int c = min(max(q + A.half_m[j], 0), nn - 1);
+      T xc[B], ev[S * G];
+#pragma unroll
+      for (int e = 0; e < B; e++) xc[e] = x[B * c + e];
+      half_groups_load<T, S, G>(own, (j * BB) / G, ev);
+      const int o = (j * BB) % G;
+#pragma unroll
+      for (int t = 0; t < B; t++)
+#pragma unroll
+        for (int e = 0; e < B; e++) sum[t] += ev[o + t * B + e] * xc[e];
+    }
+    T xd[B], dv[SD * G];
+    const int cd = min(q, nn - 1);
+#pragma unroll
+    for (int e = 0; e < B; e++) xd[e] = x[B * cd + e];
+    half_groups_load<T, SD, G>(own, ND / G, dv);
+#pragma unroll
+    for (int t = 0; t < B; t++)  // the diagonal block, its upper entries mirrored
+#pragma unroll
+      for (int e = 0; e < B; e++)
+        sum[t] += dv[ND % G + (e <= t ? t * (t + 1) / 2 + e : e * (e + 1) / 2 + t)] * xd[e];
+#pragma unroll
+    for (int j = NLN - 1; j >= 0; j--) {  // upper neighbour nodes
+      const int qm = q - A.half_m[j];
+      const T *__restrict__ mir = hv + ((int64_t)(qm >> 6) * NG * 64 + (qm & 63)) * G;
+      const int c = min(qm, nn - 1);
+      T xc[B], ev[S * G];
+#pragma unroll
+      for (int e = 0; e < B; e++) xc[e] = x[B * c + e];
+      half_groups_load<T, S, G>(mir, (j * BB) / G, ev);
+      const int o = (j * BB) % G;
+#pragma unroll
+      for (int t = 0; t < B; t++)
+#pragma unroll
+        for (int e = 0; e < B; e++) sum[t] += ev[o + e * B + t] * xc[e];
+    }
+    if (q < nn) {
+#pragma unroll
+      for (int t = 0; t < B; t++) {
+        const int r = B * q + t;
+        T st = sum[t];
+        if (KIND == 3) {
+          const T rv = b[r];
+          acc += xd[t] * st;
+          a1 += rv * rv;
+          a2 += rv * st;
+          a3 += st * st;
+        } else if (KIND == 2) {
+          st = b[r] - st;
+          acc += st * st;
+        } else if (KIND == 1) {
+          acc += xd[t] * st;
+        }
+        y[r] = st;
+      }
+    }
+  }
+  if (KIND == 3) {
+    __shared__ T lds4[4][8];
+    acc = wave_sum(acc);
+    a1 = wave_sum(a1);
+    a2 = wave_sum(a2);
+    a3 = wave_sum(a3);
+    if (lane == 0) {
+      lds4[0][wave] = acc;
+      lds4[1][wave] = a1;
+      lds4[2][wave] = a2;
+      lds4[3][wave] = a3;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+      T t = lds4[threadIdx.x][0];
+      for (int i = 1; i < W; i++) t += lds4[threadIdx.x][i];
+      partial[threadIdx.x * gridDim.x + blockIdx.x] = t;
+    }
+  } else if (KIND != 0) {
+    __shared__ T lds[8];
+    acc = wave_sum(acc);
+    if (lane == 0) lds[wave] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      T t = lds[0];
+      for (int i = 1; i < W; i++) t += lds[i];
+      partial[blockIdx.x] = t;
+    }
+  }
+}
+
+// group layout of k_spmv_halfb: element i of the layout belongs to slice
+// i / (ng 64 G), group i / (64 G) % ng, lane i / G % 64 and entry
+// (group) G + i % G of that node, taken from SELL slot slot[2 entry] of row
+// half_b q + slot[2 entry + 1] (0 past the node's nv entries or past n)
+template <typename T>
+__global__ void k_half_from_sell_b(CsrDev A, const int *__restrict__ slot, int nv, int ng,
+                                   const T *__restrict__ sval, T *__restrict__ hv)
+{
+  constexpr int G = 16 / (int)sizeof(T);
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= (int64_t)ng * G * A.half_pad) return;
+  const int64_t sl = i / ((int64_t)ng * 64 * G);
+  const int g = (int)(i / (64 * G) % ng), l = (int)(i / G % 64);
+  const int idx = g * G + (int)(i % G), q = (int)(sl * 64 + l);
+  T v = T(0);
+  if (idx < nv && q < A.n / A.half_b) {
+    const int k = slot[2 * idx], r = A.half_b * q + slot[2 * idx + 1];
+    const int64_t ngs = (A.sten_n + SELL_V - 1) / SELL_V;
+    v = sval[(r / 64) * 64 * ngs * SELL_V + (k / SELL_V) * 64 * SELL_V + (r % 64) * SELL_V +
+             k % SELL_V];
+  }
+  hv[i] = v;
+}
+
+template <typename T>
+hipError_t half_from_sell_block(const CsrDev &A, const int *slot, int nv, const T *sval, T *hv,
+                                hipStream_t s)
+{
+  const int ng = half_groups((int)sizeof(T), A.half_b, A.half_nl);
+  const int64_t tot = (int64_t)ng * (16 / sizeof(T)) * A.half_pad;
+  if (tot <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_half_from_sell_b<T>, dim3((unsigned)((tot + TPB - 1) / TPB)), dim3(TPB), 0,
+                     s, A, slot, nv, ng, sval, hv);
+  return hipGetLastError();
+}
+template hipError_t half_from_sell_block<double>(const CsrDev &, const int *, int, const double *,
+                                                 double *, hipStream_t);
+template hipError_t half_from_sell_block<float>(const CsrDev &, const int *, int, const float *,
+                                                float *, hipStream_t);
+
 template <typename T>
 __global__ void k_half_from_sell(CsrDev A, const T *__restrict__ sval, T *__restrict__ hv)
 {
@@ -575,7 +752,19 @@ static hipError_t spmv_k(const SpmvTune &t, const CsrDev &A0, const T *val, cons
 #define SSF_VEC(G) SSF_LAUNCH((k_spmv_vec<T, G, KIND>), A, val, x, y, b, partial, done, iter)
 #define SSF_STR(I, G) SSF_LAUNCH((k_spmv_stream<T, I, G, KIND>), A, val, x, y, b, partial, done, iter)
   const bool have_half = sizeof(T) == 8 ? A.half_v64 != nullptr : A.half_v32 != nullptr;
-  if (t.format == 2 && A.nslice > 0 && t.idx16 == 4 && A.half_nl > 0 && have_half) {
+  if (t.format == 2 && A.nslice > 0 && t.idx16 == 4 && A.half_nl > 0 && have_half &&
+      A.half_b > 1) {
+    const dim3 hblock(t.lanes == 2 ? 512 : 256);
+#define SSF_HALFB(B, NLN)                                                                        \
+  if (ev0) hipExtLaunchKernelGGL((k_spmv_halfb<T, KIND, B, NLN>), grid, hblock, 0, s, ev0, ev1, 0, \
+                                 A, x, y, b, partial, done, iter);                              \
+  else hipLaunchKernelGGL((k_spmv_halfb<T, KIND, B, NLN>), grid, hblock, 0, s, A, x, y, b,       \
+                          partial, done, iter);
+    if (A.half_b == 3 && A.half_nl == 13) { SSF_HALFB(3, 13) }
+    else if (A.half_b == 2 && A.half_nl == 4) { SSF_HALFB(2, 4) }
+    else return hipErrorInvalidValue;
+#undef SSF_HALFB
+  } else if (t.format == 2 && A.nslice > 0 && t.idx16 == 4 && A.half_nl > 0 && have_half) {
     const dim3 hblock(t.lanes == 2 ? 512 : 256);
 #define SSF_HALF(NL)                                                                        \
   case NL:                                                                                  \
@@ -672,6 +861,18 @@ static hipError_t spmv_half_dots(const SpmvTune &t, const CsrDev &A0, const T *d
   CsrDev A = A0;
   A.slice_map = t.map;
   const dim3 grid(t.blocks), block(t.lanes == 2 ? 512 : 256);
+  if (A.half_b > 1) {
+#define SSF_HALFB3(B, NLN)                                                                      \
+  if (ev0) hipExtLaunchKernelGGL((k_spmv_halfb<T, 3, B, NLN>), grid, block, 0, s, ev0, ev1, 0,   \
+                                 A, d, p, r, partial, done, iter);                              \
+  else hipLaunchKernelGGL((k_spmv_halfb<T, 3, B, NLN>), grid, block, 0, s, A, d, p, r, partial,  \
+                          done, iter);
+    if (A.half_b == 3 && A.half_nl == 13) { SSF_HALFB3(3, 13) }
+    else if (A.half_b == 2 && A.half_nl == 4) { SSF_HALFB3(2, 4) }
+    else return hipErrorInvalidValue;
+#undef SSF_HALFB3
+    return hipGetLastError();
+  }
 #define SSF_HALF3(NL)                                                                        \
   case NL:                                                                                   \
     if (ev0) hipExtLaunchKernelGGL((k_spmv_half<T, 3, NL>), grid, block, 0, s, ev0, ev1, 0,   \

@@ -346,14 +346,6 @@ def test_sell_half_stencil_form(ctx):
         with pytest.raises(gpu.SsfError):
             c.set_tune(gpu.FP64, (2, 1, 64, 4))
         c.close()
-    # the block stencil form of elasticity rows has no half form
-    prob = mesh.cantilever(8, 3, 3)
-    adr, ci, a, rhs = O.assemble_problem(prob, "scr")
-    m = gpu.Matrix.from_csr(ctx, gpu.STORAGE_SCR, adr, ci)
-    m.set_values(a)
-    with pytest.raises(gpu.SsfError):
-        m.set_tune(gpu.FP64, (2, 1, 64, 4))
-    m.close()
 
 
 @pytest.mark.parametrize("name", ["cantilever", "plate"])
@@ -363,11 +355,17 @@ def test_sell_block_stencil_form(ctx, name):
     the first row of its node (27 neighbour nodes x 3 = 81, 9 x 2 = 18) form one
     list, so the layout gives each its own slot. Products equal the column forms'
     bit for bit at every block count and slice map, and the FP64 and mixed solves
-    agree with the oracle."""
+    agree with the oracle. Its symmetric half (index form 4: per node the blocks
+    towards the lower neighbour nodes and the diagonal block's lower triangle, the
+    upper blocks read transposed from the mirrored nodes) is the default tiling
+    and gives the same products and FP64 solves bit for bit."""
     prob = mesh.cantilever(8, 3, 3) if name == "cantilever" else mesh.plate(20, 7, seed=11, nrhs=1)
     adr, ci, a, rhs = O.assemble_problem(prob, "scr")
     m = gpu.Matrix.from_csr(ctx, gpu.STORAGE_SCR, adr, ci)
     m.set_values(a)
+    B, nlow = (3, 13 * 9 + 6) if name == "cantilever" else (2, 4 * 4 + 3)
+    can, act = m.spmv_bytes(gpu.FP64)  # the default tiling streams the half form
+    assert act == nlow * 8.0 * (prob.n // B) + 16.0 * prob.n, act
     b = np.random.default_rng(7).standard_normal(prob.n)
     m.set_tune(gpu.FP64, (2, 0, 64, 0))
     y0 = m.gmxv(b)
@@ -378,13 +376,27 @@ def test_sell_block_stencil_form(ctx, name):
             for dep in (0, 3, 4):  # groups of slots per round: all, 3, 4
                 m.set_tune(gpu.FP64, (2, dep, nb, 3, mp))
                 assert np.array_equal(m.gmxv(b), y0), (nb, mp, dep)
+            for tb in (1, 2):  # the half form, blocks of 256 or 512 threads
+                m.set_tune(gpu.FP64, (2, tb, nb, 4, mp))
+                assert np.array_equal(m.gmxv(b), y0), (nb, mp, tb)
+    m.set_tune(gpu.FP64, (2, 4, 512, 3, 1))
     x3, i3 = m.solve(rhs[0], ni=50000, res=1e-22)
     assert i3.iterations == i0.iterations and np.array_equal(x3, x0)
+    # the half form's wavefronts take 64 nodes, not 64 rows, so its dot products
+    # are reduced in another order: the solve agrees to rounding
+    m.set_tune(gpu.FP64, (2, 1, 64, 4, 1))
+    x4, i4 = m.solve(rhs[0], ni=50000, res=1e-22)
+    assert abs(i4.iterations - i3.iterations) <= 1 and rel(x4, x3) <= 1e-12
     xo, *_ = O.scr_cg_prec(adr, ci, a, rhs[0], 50000, 1e-30)
-    assert rel(x3, xo) <= 1e-8
-    m.set_tune(gpu.MIXED, (2, 0, 64, 3, 1))
-    xm, im = m.solve(rhs[0], ni=50000, res=1e-22, precision=gpu.MIXED)
-    assert im.status in (1, 4) and rel(xm, xo) <= 1e-8, rel(xm, xo)
+    assert rel(x3, xo) <= 1e-8 and rel(x4, xo) <= 1e-8
+    for t in ((2, 0, 64, 3, 1), (2, 1, 64, 4, 1), (2, 2, 8, 4, 0)):
+        m.set_tune(gpu.MIXED, t)
+        xm, im = m.solve(rhs[0], ni=50000, res=1e-22, precision=gpu.MIXED)
+        assert im.status in (1, 4) and rel(xm, xo) <= 1e-8, (t, rel(xm, xo))
+    # a values change rebuilds the half layout
+    m.set_values(2.0 * a)
+    m.set_tune(gpu.FP64, (2, 1, 64, 4))
+    assert np.array_equal(m.gmxv(b), 2.0 * y0)
     m.close()
 
 
