@@ -1108,11 +1108,7 @@ static int half_build(ssf_matrix *m, int precision)
   if (m->half_nl == 0) return SSF_OK;
   hipStream_t s = m->ctx->stream;
   const int nv = (int)half_stored(m);
-  // entries per row or node in memory (the block variant in 16-byte groups)
-  const int64_t per = m->half_b == 1 ? nv
-                      : (int64_t)half_groups(precision ? 4 : 8, m->half_b, m->half_nl) *
-                            (precision ? 4 : 2);
-  const int64_t tot = per * m->half_pad;
+  const int64_t tot = nv * m->half_pad;
   if (precision == 0) {
     if (!m->sell_v64) return SSF_OK;
     if (!m->half_v64) SSF_HIP(dalloc(&m->half_v64, tot, s));
@@ -1910,6 +1906,53 @@ static int time_spmv(ssf_matrix *m, int precision, const SpmvTune &t, int reps, 
   return SSF_OK;
 }
 
+// Mean time of one iteration of the recurrence a solve runs with tiling t: the
+// FP64 drop-in's cg_prec step (product with d.p, update, direction) or the mixed
+// solve's FP32 step (on the half form the product with the step's dots and the
+// fused update), on the workspace vectors with a state that never stops. The
+// element-wise kernels re-reduce the product's per-block partials, so the block
+// count weighs on both.
+static int time_iteration(ssf_matrix *m, int precision, const SpmvTune &t, int reps, double *usec)
+{
+  hipStream_t s = m->ctx->stream;
+  Workspace &w = m->ws;
+  const CsrDev A = m->csr();
+  hipEvent_t a, b;
+  SSF_HIP(hipEventCreate(&a));
+  SSF_HIP(hipEventCreate(&b));
+  if (precision == 0) {
+    PcgState<double> st{};
+    st.nory = 1.0;
+    st.nom[0] = st.nom[1] = 1.0;
+    SSF_HIP(fill(m->n, 1.0, w.r, s));
+    SSF_HIP(hipMemcpyAsync(w.st, &st, sizeof st, hipMemcpyHostToDevice, s));
+  } else {
+    PcgState<float> st{};
+    st.nory = -1.0f;  // ||r||^2 < nory never holds
+    st.nom[0] = st.nom[1] = 1.0f;
+    SSF_HIP(to_f32(m->n, w.d, w.r32, 1.0, s));
+    SSF_HIP(hipMemcpyAsync(w.st32, &st, sizeof st, hipMemcpyHostToDevice, s));
+  }
+  auto launch = [&]() -> hipError_t {
+    if (precision == 0)  // res = 0 and zero = 0: the tests never stop the recurrence
+      return pcg_iteration<double>(t, MODE_SCR_CG_PREC, true, A, m->val, m->diag, w.x, w.r, w.d,
+                                   w.p, w.h, w.part, w.st, 0.0, 0.0, s);
+    return pcg_iteration<float>(t, MODE_MIXED_INNER, false, A, m->val32, nullptr, w.x32, w.r32,
+                                w.d32, w.p32, nullptr, w.part32, w.st32, 0.0f, 0.0f, s);
+  };
+  SSF_HIP(launch());
+  SSF_HIP(hipEventRecord(a, s));
+  for (int i = 0; i < reps; i++) SSF_HIP(launch());
+  SSF_HIP(hipEventRecord(b, s));
+  SSF_HIP(hipEventSynchronize(b));
+  float ms = 0.f;
+  SSF_HIP(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  *usec = 1000.0 * ms / reps;
+  return SSF_OK;
+}
+
 extern "C" int ssf_spmv_bench(ssf_matrix *m, int precision, int reps, double *usec)
 {
   SSF_ENTER();
@@ -1925,8 +1968,10 @@ extern "C" int ssf_spmv_bench(ssf_matrix *m, int precision, int reps, double *us
 
 // SpMV autotuner: times every tiling candidate (row-vector kernel: lanes per row x
 // blocks; stream kernel: lanes per row x blocks, with 16-bit column offsets for
-// the FP32 operator when they fit) on this matrix and keeps the fastest. Solves
-// use the chosen tiling from then on. tune_out = {format, lanes, blocks, idx16}.
+// the FP32 operator when they fit; the SELL forms) on this matrix, each as one
+// iteration of the recurrence its solve runs (time_iteration), and keeps the
+// fastest. Solves use the chosen tiling from then on.
+// tune_out = {format, lanes, blocks, idx16, map}.
 extern "C" int ssf_matrix_autotune(ssf_matrix *m, int precision, int reps, int32_t *tune_out)
 {
   SSF_ENTER();
@@ -2009,7 +2054,7 @@ extern "C" int ssf_matrix_autotune(ssf_matrix *m, int precision, int reps, int32
   for (const SpmvTune &t : cand) {
     if (t.blocks > m->ws.nb) continue;
     double us = 0;
-    rc = time_spmv(m, precision, t, reps, &us);
+    rc = time_iteration(m, precision, t, reps, &us);
     if (rc) return rc;
     if (us < best_t) {
       best_t = us;

@@ -181,11 +181,10 @@ struct CsrDev {
   // entries, the half_b x half_b blocks towards its half_nl lower neighbour nodes
   // q + half_m[j] (half_m ascending, negative), entry (t, e) of block j as entry
   // i = (j half_b + t) half_b + e, then the lower triangle of its diagonal block,
-  // entry (t, e <= t) as i = half_nl half_b^2 + t (t + 1) / 2 + e. Slice-major in
-  // 16-byte groups of G = 16 / sizeof(value) entries (half_groups per node): entry
-  // i of node 64 s + l at half_v[((s ng + i / G) 64 + l) G + i % G]; 0 for
-  // nodes beyond n / half_b and entries beyond nv. half_pad (a multiple of 64)
-  // covers 64 ceil(n / half_b / 64) - half_m[0] nodes.
+  // entry (t, e <= t) as i = half_nl half_b^2 + t (t + 1) / 2 + e. Slice-major:
+  // entry i of node 64 s + l at half_v[(s nv + i) 64 + l]; 0 for nodes beyond
+  // n / half_b. half_pad (a multiple of 64) covers 64 ceil(n / half_b / 64) -
+  // half_m[0] nodes.
   int half_b = 1;
   int half_m[16] = {};
   int slice_map = 1;  // SpmvTune::map of the launch
@@ -193,19 +192,6 @@ struct CsrDev {
 
 constexpr int SELL_V = 4;  // entries per lane and load (16-byte FP32, 32-byte FP64 loads)
 
-// 16-byte groups per node of the block half form (CsrDev::half_b > 1) for values of
-// vsize bytes: enough that the groups a block's B^2 entries (or the diagonal
-// block's) start in, and the ones after it that a load of the block's span of
-// groups reaches, all lie inside the node
-__host__ __device__ constexpr int half_groups(int vsize, int B, int nln)
-{
-  const int G = 16 / vsize, bb = B * B, nd = nln * bb, nv = nd + B * (B + 1) / 2;
-  const int span = (G - 1 + bb + G - 1) / G;     // groups any run of bb entries can span
-  const int dspan = (G - 1 + nv - nd + G - 1) / G;
-  int ng = (nv + G - 1) / G;
-  for (int j = 0; j < nln; j++) ng = ng > (j * bb) / G + span ? ng : (j * bb) / G + span;
-  return ng > nd / G + dspan ? ng : nd / G + dspan;
-}
 
 struct Workspace {  // vectors and reduction scratch of one solve shape
   int64_t n = 0;

@@ -534,23 +534,12 @@ __global__ __launch_bounds__(512) void k_spmv_half(CsrDev A, const T *__restrict
 // per node, its B rows summed in column order (lower neighbour nodes ascending,
 // the diagonal block, the upper nodes ascending; within a node its unknowns in
 // order), which is the block stencil form's slot order, so the products equal
-// the other forms' bit for bit. A slice of 64 nodes keeps its entries together
-// in 16-byte groups per lane: a wavefront streams one contiguous run per slice,
-// and a B x B block takes a few 16-byte loads (3 for FP32 3 x 3 blocks) instead
-// of B^2 single ones. The upper block towards node q - m_j is the transpose of
-// block j of that node, read from the same groups of its slice.
-template <typename T, int S, int G>
-__device__ __forceinline__ void half_groups_load(const T *__restrict__ base, int g0, T *ev)
-{
-  typedef T gv __attribute__((ext_vector_type(G)));
-#pragma unroll
-  for (int u = 0; u < S; u++) {
-    const gv v = *(const gv *)(base + (g0 + u) * 64 * G);
-#pragma unroll
-    for (int c = 0; c < G; c++) ev[u * G + c] = v[c];
-  }
-}
-
+// the other forms' bit for bit. The entries of a slice of 64 nodes lie together
+// (entry i of the slice's nodes in 64 consecutive values), so a wavefront streams
+// one contiguous run per slice and every load reads 256 contiguous bytes; 16-byte
+// groups per lane were measured as fast with 30 % more traffic, the mirrored
+// blocks then pulling whole groups. The upper block towards node q - m_j is the
+// transpose of block j of that node, read from the same place of its slice.
 template <typename T, int KIND, int B, int NLN>
 __global__ __launch_bounds__(512) void k_spmv_halfb(CsrDev A, const T *__restrict__ x,
                                                     T *__restrict__ y, const T *__restrict__ b,
@@ -565,55 +554,47 @@ __global__ __launch_bounds__(512) void k_spmv_halfb(CsrDev A, const T *__restric
   const int nn = A.n / B;
   int sfirst, slast, sstep;
   wave_slices((nn + 63) / 64, A.slice_map, wave, sfirst, slast, sstep, W);
-  constexpr int G = 16 / (int)sizeof(T);                     // entries per group
-  constexpr int BB = B * B, ND = NLN * BB;                  // the diagonal block's first entry
-  constexpr int NG = half_groups((int)sizeof(T), B, NLN);   // groups per node
-  constexpr int S = (G - 1 + BB + G - 1) / G;               // groups a block spans
-  constexpr int SD = (G - 1 + B * (B + 1) / 2 + G - 1) / G;  // groups the diagonal spans
+  constexpr int ND = NLN * B * B;            // the diagonal block's first entry
+  constexpr int NV = ND + B * (B + 1) / 2;   // entries per node
   T acc = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
   for (int sl = sfirst; sl < slast; sl += sstep) {
     const int q = sl * 64 + lane;
-    const T *__restrict__ own = hv + ((int64_t)sl * NG * 64 + lane) * G;
+    const T *__restrict__ own = hv + (int64_t)sl * NV * 64 + lane;
     T sum[B];
 #pragma unroll
     for (int t = 0; t < B; t++) sum[t] = T(0);
 #pragma unroll
     for (int j = 0; j < NLN; j++) {  // lower neighbour nodes
       const int c = min(max(q + A.half_m[j], 0), nn - 1);
-      T xc[B], ev[S * G];
+      T xc[B];
 #pragma unroll
       for (int e = 0; e < B; e++) xc[e] = x[B * c + e];
-      half_groups_load<T, S, G>(own, (j * BB) / G, ev);
-      const int o = (j * BB) % G;
 #pragma unroll
       for (int t = 0; t < B; t++)
 #pragma unroll
-        for (int e = 0; e < B; e++) sum[t] += ev[o + t * B + e] * xc[e];
+        for (int e = 0; e < B; e++) sum[t] += own[((j * B + t) * B + e) * 64] * xc[e];
     }
-    T xd[B], dv[SD * G];
+    T xd[B];
     const int cd = min(q, nn - 1);
 #pragma unroll
     for (int e = 0; e < B; e++) xd[e] = x[B * cd + e];
-    half_groups_load<T, SD, G>(own, ND / G, dv);
 #pragma unroll
     for (int t = 0; t < B; t++)  // the diagonal block, its upper entries mirrored
 #pragma unroll
       for (int e = 0; e < B; e++)
-        sum[t] += dv[ND % G + (e <= t ? t * (t + 1) / 2 + e : e * (e + 1) / 2 + t)] * xd[e];
+        sum[t] += own[(ND + (e <= t ? t * (t + 1) / 2 + e : e * (e + 1) / 2 + t)) * 64] * xd[e];
 #pragma unroll
     for (int j = NLN - 1; j >= 0; j--) {  // upper neighbour nodes
       const int qm = q - A.half_m[j];
-      const T *__restrict__ mir = hv + ((int64_t)(qm >> 6) * NG * 64 + (qm & 63)) * G;
+      const T *__restrict__ mir = hv + (int64_t)(qm >> 6) * NV * 64 + (qm & 63);
       const int c = min(qm, nn - 1);
-      T xc[B], ev[S * G];
+      T xc[B];
 #pragma unroll
       for (int e = 0; e < B; e++) xc[e] = x[B * c + e];
-      half_groups_load<T, S, G>(mir, (j * BB) / G, ev);
-      const int o = (j * BB) % G;
 #pragma unroll
       for (int t = 0; t < B; t++)
 #pragma unroll
-        for (int e = 0; e < B; e++) sum[t] += ev[o + e * B + t] * xc[e];
+        for (int e = 0; e < B; e++) sum[t] += mir[((j * B + e) * B + t) * 64] * xc[e];
     }
     if (q < nn) {
 #pragma unroll
@@ -667,25 +648,21 @@ __global__ __launch_bounds__(512) void k_spmv_halfb(CsrDev A, const T *__restric
   }
 }
 
-// group layout of k_spmv_halfb: element i of the layout belongs to slice
-// i / (ng 64 G), group i / (64 G) % ng, lane i / G % 64 and entry
-// (group) G + i % G of that node, taken from SELL slot slot[2 entry] of row
-// half_b q + slot[2 entry + 1] (0 past the node's nv entries or past n)
+// entry idx of node q (q < half_pad) from SELL slot slot[2 idx] of row
+// half_b q + slot[2 idx + 1], in the slice-major order of k_spmv_halfb
 template <typename T>
-__global__ void k_half_from_sell_b(CsrDev A, const int *__restrict__ slot, int nv, int ng,
+__global__ void k_half_from_sell_b(CsrDev A, const int *__restrict__ slot, int nv,
                                    const T *__restrict__ sval, T *__restrict__ hv)
 {
-  constexpr int G = 16 / (int)sizeof(T);
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= (int64_t)ng * G * A.half_pad) return;
-  const int64_t sl = i / ((int64_t)ng * 64 * G);
-  const int g = (int)(i / (64 * G) % ng), l = (int)(i / G % 64);
-  const int idx = g * G + (int)(i % G), q = (int)(sl * 64 + l);
+  if (i >= (int64_t)nv * A.half_pad) return;
+  const int64_t sl = i / (64 * (int64_t)nv);
+  const int idx = (int)(i / 64 % nv), q = (int)(sl * 64 + i % 64);
   T v = T(0);
-  if (idx < nv && q < A.n / A.half_b) {
+  if (q < A.n / A.half_b) {
     const int k = slot[2 * idx], r = A.half_b * q + slot[2 * idx + 1];
-    const int64_t ngs = (A.sten_n + SELL_V - 1) / SELL_V;
-    v = sval[(r / 64) * 64 * ngs * SELL_V + (k / SELL_V) * 64 * SELL_V + (r % 64) * SELL_V +
+    const int64_t ng = (A.sten_n + SELL_V - 1) / SELL_V;
+    v = sval[(r / 64) * 64 * ng * SELL_V + (k / SELL_V) * 64 * SELL_V + (r % 64) * SELL_V +
              k % SELL_V];
   }
   hv[i] = v;
@@ -695,11 +672,10 @@ template <typename T>
 hipError_t half_from_sell_block(const CsrDev &A, const int *slot, int nv, const T *sval, T *hv,
                                 hipStream_t s)
 {
-  const int ng = half_groups((int)sizeof(T), A.half_b, A.half_nl);
-  const int64_t tot = (int64_t)ng * (16 / sizeof(T)) * A.half_pad;
+  const int64_t tot = (int64_t)nv * A.half_pad;
   if (tot <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_half_from_sell_b<T>, dim3((unsigned)((tot + TPB - 1) / TPB)), dim3(TPB), 0,
-                     s, A, slot, nv, ng, sval, hv);
+                     s, A, slot, nv, sval, hv);
   return hipGetLastError();
 }
 template hipError_t half_from_sell_block<double>(const CsrDev &, const int *, int, const double *,
