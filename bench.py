@@ -535,8 +535,16 @@ def bench_elastic(ctx, args, gpu, mesh, torch):
     lift = m.assemble(prob)
     rhs = prob.rhs[0] + (lift if lift is not None else 0.0)
     t_asm = time.time() - t0
-    tune64 = m.autotune(gpu.FP64)
-    tune32 = m.autotune(gpu.MIXED)
+    if args.elast_tiling64:
+        m.set_tune(gpu.FP64, args.elast_tiling64)
+        tune64 = tuple(args.elast_tiling64)
+    else:
+        tune64 = m.autotune(gpu.FP64)
+    if args.elast_tiling32:
+        m.set_tune(gpu.MIXED, args.elast_tiling32)
+        tune32 = tuple(args.elast_tiling32)
+    else:
+        tune32 = m.autotune(gpu.MIXED)
     log(f"[elast] n={prob.n} nnz={m.nnz}: pattern {t_pat:.1f}s, assembly {t_asm:.2f}s, "
         f"tilings {tune64} {tune32}")
     dev = torch.device("cuda:0")
@@ -788,6 +796,10 @@ def main():
                          "autotuning")
     ap.add_argument("--tiling32", type=int, nargs="+", default=None,
                     help="pin the FP32 SpMV tiling instead of autotuning")
+    ap.add_argument("--elast-tiling64", type=int, nargs="+", default=None,
+                    help="pin the elasticity leg's FP64 SpMV tiling instead of autotuning")
+    ap.add_argument("--elast-tiling32", type=int, nargs="+", default=None,
+                    help="pin the elasticity leg's FP32 SpMV tiling instead of autotuning")
     ap.add_argument("--no-serve", action="store_true", help="skip the served leg")
     ap.add_argument("--only-serve", action="store_true")
     ap.add_argument("--only-assembly", action="store_true")
