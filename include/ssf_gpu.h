@@ -325,9 +325,10 @@ int ssf_server_submit(ssf_server *s, int kind, int64_t n, int64_t nn, const doub
  * assembled, finished, in seconds since the server was created */
 int ssf_server_wait(ssf_server *s, int64_t ticket, double *x, ssf_solve_info *info,
                     double *times);
-/* stats[7]: finished requests, loop steps, peak reserved bytes, reserved bytes,
- * and the microseconds spent by the prep threads on patterns, on matrix set-up
- * and on device assembly, summed over requests */
+/* stats[8]: finished requests, loop steps, peak reserved bytes, reserved bytes,
+ * the microseconds spent by the prep threads on patterns, on matrix set-up and
+ * on device assembly, summed over requests, and the requests whose solve
+ * streams the symmetric half of the stencil form */
 int ssf_server_stats(ssf_server *s, int64_t *stats);
 
 /* ------------------------------------------------------- post-solve element pass */

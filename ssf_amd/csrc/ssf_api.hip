@@ -187,10 +187,14 @@ static void half_setup_block(ssf_matrix *m)
       slot.push_back(t);
     }
   if (dalloc(&m->half_slot, slot.size(), m->ctx->stream) != hipSuccess ||
+      dalloc(&m->half_m_d, nl, m->ctx->stream) != hipSuccess ||
       hipMemcpyAsync(m->half_slot, slot.data(), slot.size() * sizeof(int), hipMemcpyHostToDevice,
                      m->ctx->stream) != hipSuccess ||
+      hipMemcpyAsync(m->half_m_d, nodes.data(), nl * sizeof(int), hipMemcpyHostToDevice,
+                     m->ctx->stream) != hipSuccess ||
       hipStreamSynchronize(m->ctx->stream) != hipSuccess) {
-    dfree(m->half_slot, m->ctx->stream);
+    dfree(m->half_slot, m->ctx->stream); dfree(m->half_m_d, m->ctx->stream);
+    dfree(m->half_m_d, m->ctx->stream);
     return;
   }
   m->half_b = B;
@@ -583,7 +587,7 @@ extern "C" int ssf_matrix_destroy(ssf_matrix *m)
   dfree(m->sell_col16, m->ctx->stream);
   dfree(m->sten_d, m->ctx->stream);
   dfree(m->half_v64, m->ctx->stream); dfree(m->half_v32, m->ctx->stream);
-  dfree(m->half_slot, m->ctx->stream);
+  dfree(m->half_slot, m->ctx->stream); dfree(m->half_m_d, m->ctx->stream);
   dfree(m->sell_pid, m->ctx->stream); dfree(m->sell_dict, m->ctx->stream);
   dfree(m->sell_dict_off, m->ctx->stream);
   dfree(m->sell_lo, m->ctx->stream); dfree(m->sell_up, m->ctx->stream);
@@ -1125,6 +1129,13 @@ static int half_build(ssf_matrix *m, int precision)
       SSF_HIP(half_from_sell<float>(m->csr(), m->sell_v32, m->half_v32, s));
   }
   return SSF_OK;
+}
+
+int ssf::matrix_half_build(ssf_matrix *m, int precision) { return half_build(m, precision); }
+
+int64_t ssf::matrix_half_entries(const ssf_matrix *m)
+{
+  return m->half_nl > 0 ? half_stored(m) * m->half_pad : 0;
 }
 
 // after the reference-layout values changed: full CSR values, diagonal

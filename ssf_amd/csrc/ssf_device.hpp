@@ -101,6 +101,87 @@ __device__ __forceinline__ T sten_groups(const T *__restrict__ sval, unsigned vb
 }
 
 
+// Row r of the symmetric half of the stencil form (CsrDev::half_*): NL lower
+// slots at offsets off(k) (ascending, negative; off is an accessor, so offsets
+// held in kernel arguments stay in scalar registers), stored diagonal-major in
+// hv[k pad + r], then the diagonal at k = NL; the upper entry a(r, r - off(k)) is
+// slot k of row r - off(k). The sum runs in column order (lower slots ascending,
+// the diagonal, the upper ones ascending), all loads first. xr = x[min(r, n-1)].
+template <typename T, int NL, typename Off>
+__device__ __forceinline__ T half_row(const T *__restrict__ hv, int64_t pad, Off off,
+                                      const T *__restrict__ x, int r, int nm1, T &xr)
+{
+  T vo[NL + 1], vm[NL], gl[NL], gu[NL];
+#pragma unroll
+  for (int k = 0; k <= NL; k++) vo[k] = hv[k * pad + r];
+#pragma unroll
+  for (int k = 0; k < NL; k++) vm[k] = hv[k * pad + (r - off(k))];
+#pragma unroll
+  for (int k = 0; k < NL; k++) {
+    gl[k] = x[min(max(r + off(k), 0), nm1)];
+    gu[k] = x[min(r - off(k), nm1)];
+  }
+  xr = x[min(r, nm1)];
+  T sum = T(0);
+#pragma unroll
+  for (int k = 0; k < NL; k++) sum += vo[k] * gl[k];
+  sum += vo[NL] * xr;
+#pragma unroll
+  for (int k = NL - 1; k >= 0; k--) sum += vm[k] * gu[k];
+  return sum;
+}
+
+// Node q of the block variant (B unknowns per node, NLN lower neighbour nodes q +
+// m(j), m an accessor as above): the B row sums of the node in column order
+// (lower nodes ascending, the diagonal block, the upper nodes ascending; within
+// a node its unknowns in order) into sum, x of the node's own unknowns into xd.
+// Slice-major storage: entry i of node 64 s + l at hv[(s NV + i) 64 + l], NV =
+// NLN B^2 + B (B + 1) / 2, block j's entry (t, e) at i = (j B + t) B + e, the
+// diagonal block's (t, e <= t) at NLN B^2 + t (t + 1) / 2 + e; the upper block
+// towards q - m(j) is the transpose of block j of that node.
+template <typename T, int B, int NLN, typename Off>
+__device__ __forceinline__ void half_node(const T *__restrict__ hv, Off m,
+                                          const T *__restrict__ x, int q, int nn, T (&sum)[B],
+                                          T (&xd)[B])
+{
+  constexpr int ND = NLN * B * B, NV = ND + B * (B + 1) / 2;
+  const T *__restrict__ own = hv + (int64_t)(q >> 6) * NV * 64 + (q & 63);
+#pragma unroll
+  for (int t = 0; t < B; t++) sum[t] = T(0);
+#pragma unroll
+  for (int j = 0; j < NLN; j++) {  // lower neighbour nodes
+    const int c = min(max(q + m(j), 0), nn - 1);
+    T xc[B];
+#pragma unroll
+    for (int e = 0; e < B; e++) xc[e] = x[B * c + e];
+#pragma unroll
+    for (int t = 0; t < B; t++)
+#pragma unroll
+      for (int e = 0; e < B; e++) sum[t] += own[((j * B + t) * B + e) * 64] * xc[e];
+  }
+  const int cd = min(q, nn - 1);
+#pragma unroll
+  for (int e = 0; e < B; e++) xd[e] = x[B * cd + e];
+#pragma unroll
+  for (int t = 0; t < B; t++)  // the diagonal block, its upper entries mirrored
+#pragma unroll
+    for (int e = 0; e < B; e++)
+      sum[t] += own[(ND + (e <= t ? t * (t + 1) / 2 + e : e * (e + 1) / 2 + t)) * 64] * xd[e];
+#pragma unroll
+  for (int j = NLN - 1; j >= 0; j--) {  // upper neighbour nodes
+    const int qm = q - m(j);
+    const T *__restrict__ mir = hv + (int64_t)(qm >> 6) * NV * 64 + (qm & 63);
+    const int c = min(qm, nn - 1);
+    T xc[B];
+#pragma unroll
+    for (int e = 0; e < B; e++) xc[e] = x[B * c + e];
+#pragma unroll
+    for (int t = 0; t < B; t++)
+#pragma unroll
+      for (int e = 0; e < B; e++) sum[t] += mir[((j * B + e) * B + t) * 64] * xc[e];
+  }
+}
+
 // dot of one CSR row with x by G lanes; the sum is valid in the group's lane 0
 template <typename T, int G>
 __device__ __forceinline__ T row_dot(const int *__restrict__ rowptr, const int *__restrict__ col,

@@ -283,6 +283,7 @@ struct ssf_matrix {
   float *half_v32 = nullptr;
   int half_b = 1;
   std::vector<int> half_m;   // block variant: lower node offsets, ascending
+  int *half_m_d = nullptr;   // their device copy
   int *half_slot = nullptr;  // block variant: (SELL slot, row in block) of every stored entry
   // SELL-only matrices (laid out on the device, no full CSR): SELL positions of
   // the reference-layout entries and the device copy of the pattern
@@ -374,6 +375,11 @@ int matrix_create(ssf_ctx *ctx, int storage, int64_t n, const int64_t *adr, cons
 // from the pattern (moved in); assembled, multiplied and solved in FP64
 int matrix_create_sell(ssf_ctx *ctx, int64_t n, std::vector<int64_t> &adr,
                        std::vector<int64_t> &ci, ssf_matrix **out);
+// (re)builds the half stencil form of the FP64 (precision 0) or FP32 SELL values
+// when the matrix has one (half_nl > 0)
+int matrix_half_build(ssf_matrix *m, int precision);
+// values of the half form per precision (0 when the matrix has none)
+int64_t matrix_half_entries(const ssf_matrix *m);
 // releases the element-assembly plan of a matrix
 // and, if host_pattern, its host copy of the reference-layout pattern
 void matrix_drop_assembly(ssf_matrix *m, bool host_pattern);

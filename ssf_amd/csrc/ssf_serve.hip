@@ -121,8 +121,52 @@ __device__ __forceinline__ T sv_row(const SlotDesc &D, const T *__restrict__ val
   return sum;
 }
 
+template <typename T>
+__device__ __forceinline__ const T *sv_half(const SlotDesc &D);
+template <>
+__device__ __forceinline__ const double *sv_half<double>(const SlotDesc &D)
+{
+  return D.half_v;
+}
+template <>
+__device__ __forceinline__ const float *sv_half<float>(const SlotDesc &D)
+{
+  return D.half_v32;
+}
+
+// y = A x over the block's part of the slot in the half form; f(row, (A x)_row,
+// x_row) for every row of the matrix. Row variant: the block's 64-row slices;
+// block variant (3 unknowns per node): the 64-node slices t whose first row
+// slice 3 t is the block's, so the slot's blocks cover every node once.
+template <typename T, typename F>
+__device__ __forceinline__ void sv_half_rows(const SlotDesc &D, const T *__restrict__ hv,
+                                             const T *__restrict__ x, int s0, int s1, F f)
+{
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  auto off = [&](int k) { return D.half_off[k]; };
+  if (D.half_b == 1) {
+    for (int sl = s0 + wave; sl < s1; sl += TPB / 64) {
+      const int r = sl * 64 + lane;
+      T xr;
+      const T sum = half_row<T, 13>(hv, D.half_pad, off, x, r, D.n - 1, xr);
+      if (r < D.n) f(r, sum, xr);
+    }
+  } else {
+    const int nn = D.n / 3;
+    for (int t = (s0 + 2) / 3 + wave; t < (s1 + 2) / 3; t += TPB / 64) {
+      const int q = t * 64 + lane;
+      T sum[3], xd[3];
+      half_node<T, 3, 13>(hv, off, x, q, nn, sum, xd);
+      if (q < nn)
+#pragma unroll
+        for (int u = 0; u < 3; u++) f(3 * q + u, sum[u], xd[u]);
+    }
+  }
+}
+
 // p = A d over the block's slices: one wavefront per slice, one lane per row,
-// SELL_V entries per load (16-bit or 32-bit columns), row sums in column order
+// SELL_V entries per load (16-bit or 32-bit columns), row sums in column order;
+// or in the half form
 template <typename T>
 __global__ __launch_bounds__(TPB) void k_sv_spmv(ServeDev S, SlotState st)
 {
@@ -132,14 +176,21 @@ __global__ __launch_bounds__(TPB) void k_sv_spmv(ServeDev S, SlotState st)
   if (st.done[sid]) return;
   const SvVec<T> V = sv_vec<T>(D);
   const int s0 = S.blk_s0[blk], s1 = S.blk_s1[blk];
-  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   T acc = T(0);
-  for (int sl = s0 + wave; sl < s1; sl += TPB / 64) {
-    const T sum = sv_row<T>(D, V.val, V.d, sl, lane);
-    const int row = sl * 64 + lane;
-    if (row < D.n) {
+  if (const T *hv = sv_half<T>(D)) {
+    sv_half_rows<T>(D, hv, V.d, s0, s1, [&](int row, T sum, T dr) {
       V.p[row] = sum;
-      acc += V.d[row] * sum;
+      acc += dr * sum;
+    });
+  } else {
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    for (int sl = s0 + wave; sl < s1; sl += TPB / 64) {
+      const T sum = sv_row<T>(D, V.val, V.d, sl, lane);
+      const int row = sl * 64 + lane;
+      if (row < D.n) {
+        V.p[row] = sum;
+        acc += V.d[row] * sum;
+      }
     }
   }
   block_partials2<T>(S.part, acc, T(0));
@@ -458,15 +509,23 @@ __global__ __launch_bounds__(TPB) void k_svm_resid(ServeDev S, SlotState st)
   const SlotDesc D = S.slots[S.blk_slot[blk]];
   if (st.final[D.sid]) return;
   const int s0 = S.blk_s0[blk], s1 = S.blk_s1[blk];
-  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   double acc = 0.0;
-  for (int sl = s0 + wave; sl < s1; sl += TPB / 64) {
-    const double ax = sv_row<double>(D, D.sval, D.x, sl, lane);
-    const int row = sl * 64 + lane;
-    if (row < D.n) {
+  if (D.half_v) {
+    sv_half_rows<double>(D, D.half_v, D.x, s0, s1, [&](int row, double ax, double) {
       const double ri = D.y[row] - ax;
       D.r[row] = ri;
       acc += ri * ri;
+    });
+  } else {
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    for (int sl = s0 + wave; sl < s1; sl += TPB / 64) {
+      const double ax = sv_row<double>(D, D.sval, D.x, sl, lane);
+      const int row = sl * 64 + lane;
+      if (row < D.n) {
+        const double ri = D.y[row] - ax;
+        D.r[row] = ri;
+        acc += ri * ri;
+      }
     }
   }
   block_partials2<double>(S.part, acc, 0.0);

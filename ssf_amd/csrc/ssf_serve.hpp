@@ -18,6 +18,15 @@ struct SlotDesc {
   const int *sten;
   int sten_n, sten_b;
   const double *sval;
+  // symmetric half of the stencil form (CsrDev::half_*; 13 lower slots from the
+  // row or 13 lower neighbour nodes of 3 unknowns), or null: then the solve
+  // streams the slices above. half_off: the lower offsets (stencil offsets from
+  // the row, or node offsets), ascending
+  const double *half_v;
+  const float *half_v32;
+  const int *half_off;
+  int64_t half_pad;
+  int half_b;
   const double *diag;
   const double *y;
   double *x, *r, *d, *p, *h;  // FP64 recurrence; the mixed form uses x, r only

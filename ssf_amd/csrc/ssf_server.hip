@@ -111,6 +111,7 @@ struct ssf_server {
   // statistics
   int64_t completed = 0, steps = 0;
   double t_pattern = 0, t_create = 0, t_assemble = 0;  // summed over requests (prep threads)
+  int64_t half_requests = 0;  // requests whose solve streams the half stencil form
 
   double now() const
   {
@@ -124,10 +125,20 @@ namespace {
 // columns: 0 in the stencil form, 2 or 4; FP64 values; FP32 values in the mixed
 // form), diagonal, the solve vectors (six FP64 ones, or three FP64 and six FP32
 // ones)
-int64_t resident_bytes(int64_t n, int64_t tot, int colbytes, bool mixed)
+// half: values per precision of the half stencil form the solve streams instead of
+// the slices' values (0: the slices' values are kept)
+int64_t resident_bytes(int64_t n, int64_t tot, int colbytes, bool mixed, int64_t half = 0)
 {
-  return 4 * ((n + 63) / 64 + 1) + (8 + colbytes) * tot + (mixed ? 4 * tot : 0) + 8 * n +
-         48 * n + (1 << 16);
+  const int64_t vals = half > 0 ? (8 + (mixed ? 4 : 0)) * half : (8 + (mixed ? 4 : 0)) * tot;
+  return 4 * ((n + 63) / 64 + 1) + colbytes * tot + vals + 8 * n + 48 * n + (1 << 16);
+}
+
+// the half form a served request keeps: 13 lower slots from the row or 13 lower
+// neighbour nodes of 3 unknowns (the instances of the served kernels)
+bool served_half(const ssf_matrix *m)
+{
+  return m->half_nl == 13 && (m->half_b == 1 || m->half_b == 3) && m->half_v64 &&
+         (m->half_b == 1 ? m->sten_d != nullptr : m->half_m_d != nullptr);
 }
 
 // and while it is laid out and assembled: the device pattern, the layout build's
@@ -136,8 +147,12 @@ int64_t resident_bytes(int64_t n, int64_t tot, int colbytes, bool mixed)
 // build and the element-matrix buffer
 int64_t prep_bytes(const Request &q, int64_t negm, int64_t tot, bool mixed)
 {
-  return resident_bytes(q.n, tot, 4, mixed) + 2 * tot + 4 * (q.n + 1) + 4 * negm + 28 * negm +
-         24 * q.n + 16 * negm + 8 * negm + asm_bytes(q.ne, (int)q.ndofe, negm) + 8 * q.ne;
+  // the half form while both layouts are held: at most tot + 2^14 values per
+  // precision (its padding reaches one stencil offset beyond the last slice)
+  const int64_t half = (8 + (mixed ? 4 : 0)) * (tot + (1 << 14));
+  return resident_bytes(q.n, tot, 4, mixed) + half + 2 * tot + 4 * (q.n + 1) + 4 * negm +
+         28 * negm + 24 * q.n + 16 * negm + 8 * negm + asm_bytes(q.ne, (int)q.ndofe, negm) +
+         8 * q.ne;
 }
 
 // s: the stream the release is ordered on (device memory is stream-ordered)
@@ -248,7 +263,7 @@ void prep_main(ssf_server *S, int w)
     }
     if (rc == SSF_OK) {
       const auto tc0 = clk::now();
-      rc = matrix_create_sell_dev(ctx, q->n, d_adr, d_ci, negm, &q->m, false);
+      rc = matrix_create_sell_dev(ctx, q->n, d_adr, d_ci, negm, &q->m, true);
       dt_create = std::chrono::duration<double>(clk::now() - tc0).count();
       const auto ta0 = clk::now();
       if (rc == SSF_OK && q->m->nslice == 0) {
@@ -267,17 +282,6 @@ void prep_main(ssf_server *S, int w)
         matrix_drop_assembly(q->m, true);
         matrix_keep_sell_only(q->m);
         dt_assemble = std::chrono::duration<double>(clk::now() - ta0).count();
-        {  // only the SELL solve structures are left: keep their bytes reserved
-          const int64_t keep =
-              resident_bytes(q->n, q->m->sell_tot,
-                             !q->m->sten.empty() ? 0 : q->m->sell_col16 ? 2 : 4, mixed);
-          std::unique_lock<std::mutex> lk(S->mu);
-          if (keep < q->bytes) {
-            S->used -= q->bytes - keep;
-            q->bytes = keep;
-            S->cv_budget.notify_all();
-          }
-        }
         for (int64_t i = 0; i < q->n; i++) lift[i] += q->rhs[i];
         hipError_t e = hipMallocAsync((void **)&q->vec,
                                       (6 * (size_t)std::max<int64_t>(q->n, 1) + 1) * sizeof(double),
@@ -301,11 +305,33 @@ void prep_main(ssf_server *S, int w)
             e = sell_row_absmax((int)q->n, m->sell_off, m->sell_v64, part, 1024, q->vec + 6 * q->n,
                                 ctx->stream);
           if (part) (void)hipFreeAsync(part, ctx->stream);
+          if (e == hipSuccess && served_half(m) && matrix_half_build(m, 1) != SSF_OK)
+            e = hipErrorUnknown;
+        }
+        // with the half form the solve streams it alone: the slices' values go
+        int64_t half = 0;
+        if (e == hipSuccess && served_half(q->m) && (!mixed || q->m->half_v32)) {
+          half = matrix_half_entries(q->m);
+          dfree(q->m->sell_v64, ctx->stream);
+          dfree(q->m->sell_v32, ctx->stream);
+        } else if (e == hipSuccess) {
+          dfree(q->m->half_v64, ctx->stream);
+          dfree(q->m->half_v32, ctx->stream);
         }
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
         if (e != hipSuccess) {
           rc = SSF_EHIP;
           err = std::string("ssf_server: ") + hipGetErrorString(e);
+        } else {  // only the solve structures are left: keep their bytes reserved
+          const int64_t keep =
+              resident_bytes(q->n, q->m->sell_tot,
+                             !q->m->sten.empty() ? 0 : q->m->sell_col16 ? 2 : 4, mixed, half);
+          std::unique_lock<std::mutex> lk(S->mu);
+          if (keep < q->bytes) {
+            S->used -= q->bytes - keep;
+            q->bytes = keep;
+            S->cv_budget.notify_all();
+          }
         }
       } else {
         err = ssf_last_error();
@@ -325,6 +351,7 @@ void prep_main(ssf_server *S, int w)
     S->t_pattern += dt_pattern;
     S->t_create += dt_create;
     S->t_assemble += dt_assemble;
+    if (rc == SSF_OK && q->m && !q->m->sell_v64 && q->m->half_v64) S->half_requests++;
     if (rc != SSF_OK) {
       lk.unlock();
       release_device(q, ctx->stream);
@@ -388,6 +415,11 @@ hipError_t upload_tables(ssf_server *S, const std::vector<char> &fresh)
     D.sten_n = (int)m->sten.size();
     D.sten_b = m->sten_b;
     D.sval = m->sell_v64;
+    D.half_v = m->sell_v64 ? nullptr : m->half_v64;  // the half form replaced the slices' values
+    D.half_v32 = m->sell_v64 ? nullptr : m->half_v32;
+    D.half_off = m->half_b == 1 ? m->sten_d : m->half_m_d;
+    D.half_pad = m->half_pad;
+    D.half_b = m->half_b;
     D.diag = m->diag;
     D.y = q->vec;
     D.x = q->vec + q->n;
@@ -760,9 +792,10 @@ extern "C" int ssf_server_wait(ssf_server *S, int64_t ticket, double *x, ssf_sol
   return SSF_OK;
 }
 
-// stats[7]: completed requests, loop steps, peak reserved bytes, bytes now, and
-// the microseconds the prep threads spent on patterns, on matrix set-up (CSR,
-// SELL, upload) and on device assembly, summed over requests
+// stats[8]: completed requests, loop steps, peak reserved bytes, bytes now, the
+// microseconds the prep threads spent on patterns, on matrix set-up (CSR, SELL,
+// upload) and on device assembly, summed over requests, and the requests laid
+// out in the half stencil form
 extern "C" int ssf_server_stats(ssf_server *S, int64_t *stats)
 {
   SSF_ENTER();
@@ -775,5 +808,6 @@ extern "C" int ssf_server_stats(ssf_server *S, int64_t *stats)
   stats[4] = (int64_t)(1e6 * S->t_pattern);
   stats[5] = (int64_t)(1e6 * S->t_create);
   stats[6] = (int64_t)(1e6 * S->t_assemble);
+  stats[7] = S->half_requests;
   return SSF_OK;
 }

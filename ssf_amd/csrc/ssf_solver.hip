@@ -462,27 +462,11 @@ __global__ __launch_bounds__(512) void k_spmv_half(CsrDev A, const T *__restrict
   int sfirst, slast, sstep;
   wave_slices(A.nslice, A.slice_map, wave, sfirst, slast, sstep, W);
   const int nm1 = A.n - 1;
-  const int64_t pad = A.half_pad;
   T acc = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
   for (int sl = sfirst; sl < slast; sl += sstep) {
     const int r = sl * 64 + lane;
-    T vo[NL + 1], vm[NL], gl[NL], gu[NL];
-#pragma unroll
-    for (int k = 0; k <= NL; k++) vo[k] = hv[k * pad + r];
-#pragma unroll
-    for (int k = 0; k < NL; k++) vm[k] = hv[k * pad + (r - A.sten[k])];
-#pragma unroll
-    for (int k = 0; k < NL; k++) {
-      gl[k] = x[min(max(r + A.sten[k], 0), nm1)];
-      gu[k] = x[min(r - A.sten[k], nm1)];
-    }
-    const T gd = x[min(r, nm1)];
-    T sum = T(0);
-#pragma unroll
-    for (int k = 0; k < NL; k++) sum += vo[k] * gl[k];
-    sum += vo[NL] * gd;
-#pragma unroll
-    for (int k = NL - 1; k >= 0; k--) sum += vm[k] * gu[k];
+    T gd;
+    T sum = half_row<T, NL>(hv, A.half_pad, [&](int k) { return A.sten[k]; }, x, r, nm1, gd);
     if (r < A.n) {
       if (KIND == 3) {  // the dots of the fused CG step; b holds the residual
         const T rv = b[r];
@@ -554,48 +538,11 @@ __global__ __launch_bounds__(512) void k_spmv_halfb(CsrDev A, const T *__restric
   const int nn = A.n / B;
   int sfirst, slast, sstep;
   wave_slices((nn + 63) / 64, A.slice_map, wave, sfirst, slast, sstep, W);
-  constexpr int ND = NLN * B * B;            // the diagonal block's first entry
-  constexpr int NV = ND + B * (B + 1) / 2;   // entries per node
   T acc = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
   for (int sl = sfirst; sl < slast; sl += sstep) {
     const int q = sl * 64 + lane;
-    const T *__restrict__ own = hv + (int64_t)sl * NV * 64 + lane;
-    T sum[B];
-#pragma unroll
-    for (int t = 0; t < B; t++) sum[t] = T(0);
-#pragma unroll
-    for (int j = 0; j < NLN; j++) {  // lower neighbour nodes
-      const int c = min(max(q + A.half_m[j], 0), nn - 1);
-      T xc[B];
-#pragma unroll
-      for (int e = 0; e < B; e++) xc[e] = x[B * c + e];
-#pragma unroll
-      for (int t = 0; t < B; t++)
-#pragma unroll
-        for (int e = 0; e < B; e++) sum[t] += own[((j * B + t) * B + e) * 64] * xc[e];
-    }
-    T xd[B];
-    const int cd = min(q, nn - 1);
-#pragma unroll
-    for (int e = 0; e < B; e++) xd[e] = x[B * cd + e];
-#pragma unroll
-    for (int t = 0; t < B; t++)  // the diagonal block, its upper entries mirrored
-#pragma unroll
-      for (int e = 0; e < B; e++)
-        sum[t] += own[(ND + (e <= t ? t * (t + 1) / 2 + e : e * (e + 1) / 2 + t)) * 64] * xd[e];
-#pragma unroll
-    for (int j = NLN - 1; j >= 0; j--) {  // upper neighbour nodes
-      const int qm = q - A.half_m[j];
-      const T *__restrict__ mir = hv + (int64_t)(qm >> 6) * NV * 64 + (qm & 63);
-      const int c = min(qm, nn - 1);
-      T xc[B];
-#pragma unroll
-      for (int e = 0; e < B; e++) xc[e] = x[B * c + e];
-#pragma unroll
-      for (int t = 0; t < B; t++)
-#pragma unroll
-        for (int e = 0; e < B; e++) sum[t] += mir[((j * B + e) * B + t) * 64] * xc[e];
-    }
+    T sum[B], xd[B];
+    half_node<T, B, NLN>(hv, [&](int j) { return A.half_m[j]; }, x, q, nn, sum, xd);
     if (q < nn) {
 #pragma unroll
       for (int t = 0; t < B; t++) {

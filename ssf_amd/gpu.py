@@ -608,11 +608,11 @@ class Server:
         return x, info, times
 
     def stats(self):
-        s = np.zeros(7, np.int64)
+        s = np.zeros(8, np.int64)
         _check(lib().ssf_server_stats(self.h, _p(s)))
         return {"finished": int(s[0]), "steps": int(s[1]), "peak_bytes": int(s[2]),
                 "reserved_bytes": int(s[3]), "pattern_s": s[4] * 1e-6, "setup_s": s[5] * 1e-6,
-                "assembly_s": s[6] * 1e-6}
+                "assembly_s": s[6] * 1e-6, "half_form_requests": int(s[7])}
 
     def close(self):
         if self.h:

@@ -35,8 +35,13 @@ def test_server_matches_single_and_oracle(ctx):
     try:
         tickets = [srv.submit(p) for p in probs]
         out = [srv.wait(t) for t in tickets]
+        st = srv.stats()
     finally:
         srv.close()
+    # the heat cubes (27-point stencils) and the cantilever (27 nodes of 3
+    # unknowns) solve on the half stencil form; the 3 x 4 x 6 column (too much
+    # surface for the stencil form) and the plates keep their slices
+    assert st["half_form_requests"] == 3, st
     for p, (x, info, times) in zip(probs, out):
         m, rhs = gpu.build_problem_matrix(ctx, p)
         x1, i1 = m.solve(rhs[0], ni=100000, res=RES)
@@ -152,6 +157,7 @@ def test_server_mixed_matches_oracle_and_alone_equals_in_mix():
         assert np.array_equal(x, x_alone)
         assert info.iterations == i_alone.iterations and info.outer == i_alone.outer
     assert st["reserved_bytes"] == 0
+    assert st["half_form_requests"] == 3, st  # the target (3 x 4 x 6 column) keeps its slices
 
 
 def test_server_mixed_ni_and_zero_rhs():
