@@ -1,9 +1,9 @@
 // Kernels of the served solves (gfx950): one launch sequence advances every
 // resident request ("slot") by one iteration of the cg_prec recurrence
-// (GEFEL/scr.cpp:1142-1197), each slot through its own descriptor. The same
-// kernels, instantiated for float, run the FP32 recurrence of the mixed form,
-// on the symmetrically scaled operator (no diagonal: h = r), whose FP64 refresh
-// (reliable update) ends every chunk.
+// (GEFEL/scr.cpp:1142-1197), each slot through its own descriptor. The mixed
+// form runs an FP32 CG on the symmetrically scaled operator in two passes per
+// iteration (the product with four dots, then the fused step k_sv_step), and an
+// FP64 refresh (reliable update) ends every chunk.
 //
 // Block b of a launch works on the 64-row slices [blk_s0[b], blk_s1[b]) of slot
 // blk_slot[b] (the SELL-64 layout of the slot's own matrix). Every block leaves its partial sums; a small kernel with
@@ -30,6 +30,16 @@ __device__ __forceinline__ void block_partials2(double *__restrict__ part, T a, 
     part[2 * blockIdx.x] = (double)ta;
     part[2 * blockIdx.x + 1] = (double)tb;
   }
+}
+
+// block partials of four sums into part[4 * block + j]
+template <typename T>
+__device__ __forceinline__ void block_partials4(double *__restrict__ part, T a, T b, T c, T d)
+{
+  const T v[4] = {block_sum(a), block_sum(b), block_sum(c), block_sum(d)};
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int j = 0; j < 4; j++) part[4 * blockIdx.x + j] = (double)v[j];
 }
 
 // slots a kernel skips: 0 finished or paused (done), 1 not fresh, 2 final (mixed)
@@ -164,10 +174,30 @@ __device__ __forceinline__ void sv_half_rows(const SlotDesc &D, const T *__restr
   }
 }
 
+// one block per slot: the slot's four sums of block_partials4, in block order,
+// into st.sc4[4 sid + j] (slots the previous kernel skipped are skipped)
+__global__ __launch_bounds__(TPB) void k_sv_reduce4(ServeDev S, SlotState st)
+{
+  const SlotDesc D = S.slots[blockIdx.x];
+  const int sid = D.sid;
+  if (st.done[sid]) return;
+  double a[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int i = threadIdx.x; i < D.nblk; i += TPB)
+#pragma unroll
+    for (int j = 0; j < 4; j++) a[j] += S.part[4 * (D.blk0 + i) + j];
+#pragma unroll
+  for (int j = 0; j < 4; j++) a[j] = block_sum(a[j]);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int j = 0; j < 4; j++) st.sc4[4 * sid + j] = a[j];
+}
+
 // p = A d over the block's slices: one wavefront per slice, one lane per row,
 // SELL_V entries per load (16-bit or 32-bit columns), row sums in column order;
 // or in the half form
-template <typename T>
+// DOTS4: also r.r, r.p and p.p (block_partials4), for the fused step of the
+// mixed form's FP32 recurrence (k_sv_step)
+template <typename T, bool DOTS4>
 __global__ __launch_bounds__(TPB) void k_sv_spmv(ServeDev S, SlotState st)
 {
   const int blk = blockIdx.x;
@@ -176,28 +206,74 @@ __global__ __launch_bounds__(TPB) void k_sv_spmv(ServeDev S, SlotState st)
   if (st.done[sid]) return;
   const SvVec<T> V = sv_vec<T>(D);
   const int s0 = S.blk_s0[blk], s1 = S.blk_s1[blk];
-  T acc = T(0);
+  T acc = T(0), rr = T(0), rp = T(0), pp = T(0);
+  auto row_done = [&](int row, T sum, T dr) {
+    V.p[row] = sum;
+    acc += dr * sum;
+    if (DOTS4) {
+      const T rv = V.r[row];
+      rr += rv * rv;
+      rp += rv * sum;
+      pp += sum * sum;
+    }
+  };
   if (const T *hv = sv_half<T>(D)) {
-    sv_half_rows<T>(D, hv, V.d, s0, s1, [&](int row, T sum, T dr) {
-      V.p[row] = sum;
-      acc += dr * sum;
-    });
+    sv_half_rows<T>(D, hv, V.d, s0, s1, row_done);
   } else {
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     for (int sl = s0 + wave; sl < s1; sl += TPB / 64) {
       const T sum = sv_row<T>(D, V.val, V.d, sl, lane);
       const int row = sl * 64 + lane;
-      if (row < D.n) {
-        V.p[row] = sum;
-        acc += V.d[row] * sum;
-      }
+      if (row < D.n) row_done(row, sum, V.d[row]);
     }
   }
-  block_partials2<T>(S.part, acc, T(0));
+  if (DOTS4) block_partials4<T>(S.part, acc, rr, rp, pp);
+  else block_partials2<T>(S.part, acc, T(0));
+}
+
+// The mixed form's FP32 step after k_sv_spmv<T, true> and k_sv_reduce4: alpha =
+// r.r / d.p, x += alpha d, r -= alpha p and, with ||r - alpha p||^2 = r.r -
+// 2 alpha r.p + alpha^2 p.p from the same dots, d = r + beta d in the same pass
+// (as the single-system k_cg_fused_step); then ||r||^2 < nory pauses the slot
+// until the chunk's refresh. |d.p| < zero stops it (status 2, GEFEL/scr.cpp:1148).
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_sv_step(ServeDev S, SlotState st, double zero)
+{
+  const int blk = blockIdx.x;
+  const SlotDesc D = S.slots[S.blk_slot[blk]];
+  const int sid = D.sid;
+  if (st.done[sid]) return;
+  const SvVec<T> V = sv_vec<T>(D);
+  const T dp = (T)st.sc4[4 * sid], rr = (T)st.sc4[4 * sid + 1];
+  const T rp = (T)st.sc4[4 * sid + 2], pp = (T)st.sc4[4 * sid + 3];
+  const long long it = st.iter[sid];
+  if (fabs(dp) < (T)zero) {
+    if (blk == D.blk0 && threadIdx.x == 0) {
+      st.brk[sid] = 1;
+      st.ani[sid] = it;
+      st.ares[sid] = (double)rr;
+    }
+    return;
+  }
+  const T a = rr / dp;
+  const T rrn = rr - T(2) * a * rp + a * a * pp;
+  const T bt = rrn / rr;
+  if (blk == D.blk0 && threadIdx.x == 0) {
+    st.nom[2 * sid + ((it + 1) & 1)] = (double)rrn;
+    st.pend[sid] = SV_ADVANCE | (rrn < (T)st.nory[sid] ? SV_STOP : 0);
+    st.pend_status[sid] = 1;
+  }
+  const int r0 = S.blk_s0[blk] * 64, r1 = min(D.n, S.blk_s1[blk] * 64);
+  for (int i = r0 + threadIdx.x; i < r1; i += TPB) {
+    V.x[i] += a * V.d[i];
+    const T ri = V.r[i] - a * V.p[i];
+    V.r[i] = ri;
+    V.d[i] = ri + bt * V.d[i];
+  }
 }
 
 // den = d.p; |den| < zero stops the slot (status 2); x += a d, r -= a p,
-// h = r / diag (FP64; the FP32 recurrence has h = r); slot sums r.r and r.h
+// h = r / diag; slot sums r.r and r.h (the FP64 recurrence)
 template <typename T>
 __global__ __launch_bounds__(TPB) void k_sv_update(ServeDev S, SlotState st, double zero)
 {
@@ -237,11 +313,9 @@ __global__ __launch_bounds__(TPB) void k_sv_update(ServeDev S, SlotState st, dou
   block_partials2<T>(S.part, srr, sizeof(T) == sizeof(double) ? srh : srr);
 }
 
-// FP64: stopping test |r.r / nory| < res (GEFEL/scr.cpp:1188), else d = h + b d;
-// the iteration cap ni ends the slot with status 0 after its ni-th iteration.
-// MIXED (the FP32 recurrence): d = h + b d always, then r.r < nory pauses the
-// slot until the chunk's refresh (the refresh applies the cap).
-template <typename T, bool MIXED>
+// stopping test |r.r / nory| < res (GEFEL/scr.cpp:1188), else d = h + b d; the
+// iteration cap ni ends the slot with status 0 after its ni-th iteration
+template <typename T>
 __global__ __launch_bounds__(TPB) void k_sv_dir(ServeDev S, SlotState st, double res,
                                                 long long ni)
 {
@@ -254,29 +328,19 @@ __global__ __launch_bounds__(TPB) void k_sv_dir(ServeDev S, SlotState st, double
   const double nomn = st.sc[3 * sid + 2];
   const long long it = st.iter[sid];
   const double nom = st.nom[2 * sid + (it & 1)];
-  bool stop;
-  if (MIXED) {
-    stop = false;
-    if (blk == D.blk0 && threadIdx.x == 0) {
-      st.nom[2 * sid + ((it + 1) & 1)] = nomn;
-      st.pend[sid] = SV_ADVANCE | ((T)rr < (T)st.nory[sid] ? SV_STOP : 0);
+  const bool stop = fabs(rr / st.nory[sid]) < res;
+  if (blk == D.blk0 && threadIdx.x == 0) {
+    st.nom[2 * sid + ((it + 1) & 1)] = nomn;
+    if (stop) {
+      st.ani[sid] = it;
+      st.ares[sid] = nomn / st.nory[sid];
+      st.pend[sid] = SV_STOP;
       st.pend_status[sid] = 1;
-    }
-  } else {
-    stop = fabs(rr / st.nory[sid]) < res;
-    if (blk == D.blk0 && threadIdx.x == 0) {
-      st.nom[2 * sid + ((it + 1) & 1)] = nomn;
-      if (stop) {
-        st.ani[sid] = it;
-        st.ares[sid] = nomn / st.nory[sid];
-        st.pend[sid] = SV_STOP;
-        st.pend_status[sid] = 1;
-      } else if (it + 1 >= ni) {
-        st.pend[sid] = SV_ADVANCE | SV_STOP;
-        st.pend_status[sid] = 0;
-      } else {
-        st.pend[sid] = SV_ADVANCE;
-      }
+    } else if (it + 1 >= ni) {
+      st.pend[sid] = SV_ADVANCE | SV_STOP;
+      st.pend_status[sid] = 0;
+    } else {
+      st.pend[sid] = SV_ADVANCE;
     }
   }
   if (stop) return;
@@ -374,12 +438,12 @@ hipError_t serve_iteration(const ServeDev &S, SlotState st, double res, double z
                            long long ni, hipStream_t s)
 {
   if (S.nslot == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_sv_spmv<double>, dim3(S.nblk), dim3(TPB), 0, s, S, st);
+  hipLaunchKernelGGL((k_sv_spmv<double, false>), dim3(S.nblk), dim3(TPB), 0, s, S, st);
   hipLaunchKernelGGL(k_sv_reduce, dim3(S.nslot), dim3(TPB), 0, s, S, st, st.sc, nullptr, 3, SK_DONE);
   hipLaunchKernelGGL(k_sv_update<double>, dim3(S.nblk), dim3(TPB), 0, s, S, st, zero);
   hipLaunchKernelGGL(k_sv_reduce, dim3(S.nslot), dim3(TPB), 0, s, S, st, st.sc + 1, st.sc + 2, 3,
                      SK_DONE);
-  hipLaunchKernelGGL((k_sv_dir<double, false>), dim3(S.nblk), dim3(TPB), 0, s, S, st, res, ni);
+  hipLaunchKernelGGL(k_sv_dir<double>, dim3(S.nblk), dim3(TPB), 0, s, S, st, res, ni);
   hipLaunchKernelGGL(k_sv_advance, dim3((S.nslot + TPB - 1) / TPB), dim3(TPB), 0, s, S, st,
                      nullptr);
   return hipGetLastError();
@@ -627,12 +691,10 @@ hipError_t serve_mixed_init(const ServeDev &S, SlotState st, double res, double 
 hipError_t serve_mixed_iteration(const ServeDev &S, SlotState st, hipStream_t s)
 {
   if (S.nslot == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_sv_spmv<float>, dim3(S.nblk), dim3(TPB), 0, s, S, st);
-  hipLaunchKernelGGL(k_sv_reduce, dim3(S.nslot), dim3(TPB), 0, s, S, st, st.sc, nullptr, 3, SK_DONE);
-  hipLaunchKernelGGL(k_sv_update<float>, dim3(S.nblk), dim3(TPB), 0, s, S, st, 1e-30);
-  hipLaunchKernelGGL(k_sv_reduce, dim3(S.nslot), dim3(TPB), 0, s, S, st, st.sc + 1, st.sc + 2, 3,
-                     SK_DONE);
-  hipLaunchKernelGGL((k_sv_dir<float, true>), dim3(S.nblk), dim3(TPB), 0, s, S, st, 1.0, 0LL);
+  // two passes over the vectors: the product with its four dots, the fused step
+  hipLaunchKernelGGL((k_sv_spmv<float, true>), dim3(S.nblk), dim3(TPB), 0, s, S, st);
+  hipLaunchKernelGGL(k_sv_reduce4, dim3(S.nslot), dim3(TPB), 0, s, S, st);
+  hipLaunchKernelGGL(k_sv_step<float>, dim3(S.nblk), dim3(TPB), 0, s, S, st, 1e-30);
   hipLaunchKernelGGL(k_sv_advance, dim3((S.nslot + TPB - 1) / TPB), dim3(TPB), 0, s, S, st,
                      nullptr);
   return hipGetLastError();

@@ -46,6 +46,7 @@ struct SlotDesc {
 struct SlotState {
   double *nory, *nom, *ares;  // nom: 2 per slot (iteration parity)
   double *sc;                 // 3 per slot: d.p, r.r, r.h of the running iteration
+  double *sc4;                // 4 per slot: d.p, r.r, r.p, p.p (the mixed form's fused step)
   long long *iter, *ani;
   int *done, *status, *pend, *pend_status, *brk;
   // mixed form (reliable updates, as the single-system solve): ||b||^2, ||b - A x||^2
@@ -60,7 +61,7 @@ struct ServeDev {
   int nslot = 0, nblk = 0;
   const SlotDesc *slots = nullptr;
   const int *blk_slot = nullptr, *blk_s0 = nullptr, *blk_s1 = nullptr;  // local slices
-  double *part = nullptr;  // 2 per block
+  double *part = nullptr;  // 4 per block (2 for most kernels)
 };
 
 // 64-row slices of a slot per block: the partition depends only on the slot

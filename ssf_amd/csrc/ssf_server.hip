@@ -393,7 +393,7 @@ hipError_t ensure_tables(ssf_server *S, int nslot, int nblk)
     if ((e = salloc(&S->d_blk_slot, cap)) != hipSuccess) return e;
     if ((e = salloc(&S->d_blk_s0, cap)) != hipSuccess) return e;
     if ((e = salloc(&S->d_blk_s1, cap)) != hipSuccess) return e;
-    if ((e = salloc(&S->d_part, 2 * (size_t)cap)) != hipSuccess) return e;
+    if ((e = salloc(&S->d_part, 4 * (size_t)cap)) != hipSuccess) return e;
     S->cap_blk = cap;
   }
   return e;
@@ -620,7 +620,7 @@ hipError_t alloc_state(ssf_server *S, int cap)
   auto c = [&](hipError_t x) { if (e == hipSuccess) e = x; };
   SlotState &t = S->st;
   c(salloc(&t.nory, cap)); c(salloc(&t.nom, 2 * (size_t)cap)); c(salloc(&t.ares, cap));
-  c(salloc(&t.sc, 3 * (size_t)cap));
+  c(salloc(&t.sc, 3 * (size_t)cap)); c(salloc(&t.sc4, 4 * (size_t)cap));
   c(salloc(&t.iter, cap)); c(salloc(&t.ani, cap)); c(salloc(&t.done, cap));
   c(salloc(&t.status, cap)); c(salloc(&t.pend, cap)); c(salloc(&t.pend_status, cap));
   c(salloc(&t.brk, cap));
@@ -639,7 +639,7 @@ hipError_t alloc_state(ssf_server *S, int cap)
 void free_all(ssf_server *S)
 {
   SlotState &t = S->st;
-  sfree(t.nory); sfree(t.nom); sfree(t.ares); sfree(t.sc);
+  sfree(t.nory); sfree(t.nom); sfree(t.ares); sfree(t.sc); sfree(t.sc4);
   sfree(t.iter); sfree(t.ani); sfree(t.done);
   sfree(t.status); sfree(t.pend); sfree(t.pend_status); sfree(t.brk);
   sfree(t.nory64); sfree(t.rr64); sfree(t.best); sfree(t.scale); sfree(t.delta2); sfree(t.xx);
