@@ -308,7 +308,7 @@ def bench_served(ctx, args, gpu, mesh, torch):
         for rate in args.serve_rates:
             arr = np.cumsum(np.random.default_rng(7).exponential(1.0 / rate, len(probs)))
             srv = gpu.Server(budget_bytes=200 << 30, res=args.serve_res, chunk=args.serve_chunk,
-                             prep_threads=4, precision=prec)
+                             prep_threads=args.serve_prep, precision=prec)
             try:
                 tickets = []
                 start = time.perf_counter()
@@ -802,7 +802,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nel", type=int, default=128, help="elements per cube edge")
     ap.add_argument("--res", type=float, default=1e-22, help="squared relative residual test")
-    ap.add_argument("--interval", type=int, default=64,
+    ap.add_argument("--interval", type=int, default=128,
                     help="FP32 iterations between FP64 refinements (mixed)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-iters", type=int, default=20)
@@ -847,6 +847,8 @@ def main():
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[50.0])
     ap.add_argument("--serve-precisions", nargs="+", default=["fp64", "mixed"])
     ap.add_argument("--serve-chunk", type=int, default=32)
+    ap.add_argument("--serve-prep", type=int, default=4,
+                    help="server prep threads (pattern, layout and assembly of new requests)")
     ap.add_argument("--serve-oracle", type=int, default=3,
                     help="smallest requests whose drop-in solution is checked by the oracle")
     args = ap.parse_args()

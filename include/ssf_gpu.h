@@ -67,7 +67,7 @@ typedef struct ssf_solve_params {
   double inner_res;    /* mixed: delta, relative FP32 residual reduction between two
                           FP64 refreshes (0 = 0.1)                               */
   int64_t inner_ni;    /* mixed: FP32 iterations between two FP64 refinements
-                          (reliable updates); 0 = 32                              */
+                          (reliable updates); 0 = 128 (single systems), 32 (batch) */
   int64_t max_outer;   /* reserved (0)                                            */
 } ssf_solve_params;
 

@@ -1626,7 +1626,9 @@ static int solve_mixed(ssf_matrix *m, const ssf_solve_params *p, const double *y
   Workspace &w = m->ws;
   const CsrDev A = m->csr();
   if (!p->iv) SSF_HIP(hipMemsetAsync(x, 0, m->n * sizeof(double), s));
-  const int64_t interval = p->inner_ni > 0 ? p->inner_ni : 32;
+  // (128 measured best on heat 128^3 and the 64 x 64 x 256 elasticity column, from
+  // 16 to 512; at 512 the elasticity FP32 recurrence needs five times the iterations)
+  const int64_t interval = p->inner_ni > 0 ? p->inner_ni : 128;
   // ||b||^2 and the initial FP64 residual
   double *dslot = w.part + 3 * w.nb;
   SSF_HIP(dot_partial(m->n, y, y, w.part, 256, s));

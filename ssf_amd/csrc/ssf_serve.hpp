@@ -64,7 +64,8 @@ struct ServeDev {
   double *part = nullptr;  // 4 per block (2 for most kernels)
 };
 
-// 64-row slices of a slot per block: the partition depends only on the slot
+// 64-row slices of a slot per block (12 for a slot in the half form of 3-unknown
+// nodes): the partition depends only on the slot
 constexpr int SERVE_SLICES_PER_BLOCK = 16;
 
 hipError_t serve_init(const ServeDev &S, SlotState st, double zero, hipStream_t s);

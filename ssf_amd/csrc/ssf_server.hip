@@ -442,10 +442,13 @@ hipError_t upload_tables(ssf_server *S, const std::vector<char> &fresh)
     D.fresh = fresh[k];
     D.blk0 = (int)bs.size();
     const int ns = m->nslice;
-    for (int t = 0; t < ns; t += SERVE_SLICES_PER_BLOCK) {
+    // a request in the half form of 3-unknown nodes takes 12 row slices per
+    // block: 4 node slices, one per wavefront
+    const int spb = (D.half_v && m->half_b == 3) ? 12 : SERVE_SLICES_PER_BLOCK;
+    for (int t = 0; t < ns; t += spb) {
       bs.push_back((int)k);
       b0.push_back(t);
-      b1.push_back(std::min(ns, t + SERVE_SLICES_PER_BLOCK));
+      b1.push_back(std::min(ns, t + spb));
     }
     D.nblk = (int)bs.size() - D.blk0;
   }
