@@ -559,13 +559,13 @@ static void free_workspace(Workspace &w, hipStream_t s)
 {
   dfree(w.x, s); dfree(w.r, s); dfree(w.d, s); dfree(w.p, s); dfree(w.h, s); dfree(w.y, s);
   dfree(w.x32, s); dfree(w.r32, s); dfree(w.d32, s); dfree(w.p32, s); dfree(w.h32, s); dfree(w.y32, s);
-  dfree(w.part, s); dfree(w.part32, s); dfree(w.st, s); dfree(w.st32, s); dfree(w.mix, s);
-  if (w.mix_host) (void)hipHostFree(w.mix_host);
-  w.mix_host = nullptr;
+  dfree(w.part, s); dfree(w.part32, s); dfree(w.st, s); dfree(w.probe, s);
+  w.st32 = nullptr;
+  w.mix = nullptr;
+  if (w.probe_host) (void)hipHostFree(w.probe_host);
+  w.probe_host = nullptr;
   if (w.st_host) (void)hipHostFree(w.st_host);
-  if (w.st32_host) (void)hipHostFree(w.st32_host);
   w.st_host = nullptr;
-  w.st32_host = nullptr;
   w.n = 0;
 }
 
@@ -1401,10 +1401,10 @@ static int ensure_ws(ssf_matrix *m, bool mixed)
     SSF_HIP(dalloc(&w.x32, n, m->ctx->stream)); SSF_HIP(dalloc(&w.r32, n, m->ctx->stream)); SSF_HIP(dalloc(&w.d32, n, m->ctx->stream));
     SSF_HIP(dalloc(&w.p32, n, m->ctx->stream)); SSF_HIP(dalloc(&w.h32, n, m->ctx->stream)); SSF_HIP(dalloc(&w.y32, n, m->ctx->stream));
     SSF_HIP(dalloc(&w.part32, 4 * (size_t)w.nb, m->ctx->stream));
-    SSF_HIP(dalloc(&w.st32, 1, m->ctx->stream));
-    SSF_HIP(hipHostMalloc((void **)&w.st32_host, 2 * sizeof(PcgState<float>), hipHostMallocDefault));
-    SSF_HIP(dalloc(&w.mix, 1, m->ctx->stream));
-    SSF_HIP(hipHostMalloc((void **)&w.mix_host, 2 * sizeof(MixState), hipHostMallocDefault));
+    SSF_HIP(dalloc(&w.probe, 1, m->ctx->stream));
+    w.mix = &w.probe->mix;
+    w.st32 = &w.probe->st;
+    SSF_HIP(hipHostMalloc((void **)&w.probe_host, 2 * sizeof(MixProbe), hipHostMallocDefault));
   }
   if (mixed && !m->val32_valid) {
     // a SELL-only matrix (device-built pattern) has no full CSR: its FP32 copy
@@ -1658,41 +1658,60 @@ static int solve_mixed(ssf_matrix *m, const ssf_solve_params *p, const double *y
                           nullptr, w.d32, w.part32, w.st32, 1e-30f, s));
   const double delta = (p->inner_res > 0.0 && p->inner_res < 1.0) ? p->inner_res : 0.1;
   SSF_HIP(mixed_init(w.mix, w.st32, dslot, m->norm_a, p->res, delta, s));
+  // The FP32 iterations go out in units of at most MIX_UNIT iterations, the
+  // host one unit ahead of what it has read back: after each unit both states
+  // are copied to the host and an event is recorded, and unit u + 1 is enqueued
+  // once unit u - 1 is known. A unit may end with the FP64 refresh, which closes
+  // the chunk of iterations since the previous refresh. A chunk closes
+  //  - after `interval` iterations, or when ni is spent;
+  //  - as soon as a unit shows the FP32 recurrence paused (its residual met the
+  //    refresh threshold): the iterations enqueued after the pause exit early
+  //    (a few us each) and change nothing, so the refresh sees the iterate of
+  //    the pause.
+  // Units are shortened to end one iteration (and 5 %) after where the FP32
+  // residual's mean rate of decrease since the last refresh puts the pause, and
+  // the unit after a refresh that may end the solve (its chunk aimed at the
+  // final residual) is short, so that few iterations are enqueued past a pause.
+  // The refresh itself waits for the pause to be seen: placed on the prediction
+  // it came early on heat 128^3 (the first iterations of CG fall fastest) and
+  // cost another refresh and 21 iterations. Every decision reads device states
+  // only, so a solve repeats bit for bit.
+  constexpr int64_t MIX_UNIT = 16;
+  struct Unit {
+    int64_t len;
+    bool refresh, sample;
+  } unit[2];
   hipEvent_t ev[2];
   SSF_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
   SSF_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
-  int64_t it = 0;
-  int k = 0;
-  bool pending = false;
+  MixProbe *ph = w.probe_host;
+  // unit 0: the state after mixed_init, taken as a refresh
+  SSF_HIP(hipMemcpyAsync(&ph[0], w.probe, sizeof(MixProbe), hipMemcpyDeviceToHost, s));
+  SSF_HIP(hipEventRecord(ev[0], s));
+  unit[0] = {0, true, false};
+  int64_t u = 1, it = 0, in_chunk = 0, next_len = std::min<int64_t>(MIX_UNIT, interval);
+  bool next_close = false;
+  bool sample_valid[2] = {false, false};
+  double rate = 0.0, base_lrr = 0.0;
+  int64_t base_iter = 0;
   int rc = SSF_OK;
-  bool sample_valid[2] = {true, false};
-  // Chunk lengths. A chunk's iterations after the recurrence paused, and every
-  // iteration of a chunk enqueued after the one that converged, exit early (a
-  // few us each). The host knows the outcome of chunk k - 1 when it enqueues
-  // chunk k + 1: from the FP64 residual's rate of decrease per iteration (the
-  // smaller of its mean since the start and its rate over the last known chunk,
-  // as the residual of CG is irregular chunk by chunk and slows late) it
-  // estimates the iterations still needed and makes chunk k + 1 that long plus a
-  // margin, at most the interval; a chunk that ends early only refreshes sooner.
-  int64_t clen[2] = {interval, interval}, cnext = interval, iter_seen = 0;
-  const double rr_start = slots[0];
-  double rr_seen = slots[0];
   while (true) {
     bool launched_now = false;
     if (it < p->ni) {
-      const int64_t c = std::min<int64_t>(cnext, p->ni - it);
-      clen[k & 1] = c;
-      // sampled: the chunk's second FP32 SpMV (the first follows the FP64 refresh
-      // and starts on cold caches; the inner recurrence never stops after one step)
+      const int64_t c = std::min<int64_t>(std::min<int64_t>(next_len, interval - in_chunk), p->ni - it);
+      const bool close = next_close || in_chunk + c >= interval || it + c >= p->ni;
+      // sampled: the unit's second FP32 SpMV (the first after a refresh starts on
+      // cold caches)
       const int64_t qs = c > 1 ? 1 : 0;
+      const int slot = (int)(u & 1);
       for (int64_t q = 0; q < c && rc == SSF_OK; q++)
         if (pcg_iteration<float>(m->tune32, MODE_MIXED_INNER, false, A, m->val32, nullptr, w.x32,
                                  w.r32, w.d32, w.p32, nullptr, w.part32, w.st32, 1.0f, 1e-30f, s,
-                                 (m->ctx->prof && q == qs) ? m->ctx->prof_ev[k & 1][0] : nullptr,
-                                 (m->ctx->prof && q == qs) ? m->ctx->prof_ev[k & 1][1] : nullptr) !=
+                                 (m->ctx->prof && q == qs) ? m->ctx->prof_ev[slot][0] : nullptr,
+                                 (m->ctx->prof && q == qs) ? m->ctx->prof_ev[slot][1] : nullptr) !=
             hipSuccess)
           rc = SSF_EHIP;
-      if (rc == SSF_OK &&
+      if (rc == SSF_OK && close &&
           mixed_update(m->tune64, m->tune32, A, m->val, m->isd, y, x, w.r, w.part,
                        w.part + w.nb, w.x32, w.r32, w.d32, w.part32, w.st32, w.mix, p->res,
                        s) != hipSuccess)
@@ -1702,37 +1721,48 @@ static int solve_mixed(ssf_matrix *m, const ssf_solve_params *p, const double *y
         break;
       }
       it += c;
-      (void)hipMemcpyAsync(&w.mix_host[k & 1], w.mix, sizeof(MixState), hipMemcpyDeviceToHost, s);
-      (void)hipMemcpyAsync(&w.st32_host[k & 1], w.st32, sizeof(PcgState<float>),
-                           hipMemcpyDeviceToHost, s);
-      (void)hipEventRecord(ev[k & 1], s);
+      in_chunk = close ? 0 : in_chunk + c;
+      (void)hipMemcpyAsync(&ph[slot], w.probe, sizeof(MixProbe), hipMemcpyDeviceToHost, s);
+      (void)hipEventRecord(ev[slot], s);
+      unit[slot] = {c, close, m->ctx->prof && c > 0};
       launched_now = true;
     }
-    if (pending) {
-      (void)hipEventSynchronize(ev[(k - 1) & 1]);
-      if (m->ctx->prof && sample_valid[(k - 1) & 1]) prof_add(m->ctx, 1, (k - 1) & 1);
-      sample_valid[k & 1] = !w.mix_host[(k - 1) & 1].final;
-      if (w.mix_host[(k - 1) & 1].final) break;
-      // iterations chunk k - 1 ran (st.iter counts the real ones, from -1) and the
-      // residual reduction they bought
-      const MixState &mk = w.mix_host[(k - 1) & 1];
-      const int64_t seen = w.st32_host[(k - 1) & 1].iter + 1;
-      cnext = interval;
-      const double target = p->res * mk.nory64;
-      if (seen > iter_seen && mk.rr64 > 0.0 && mk.rr64 < rr_seen && target > 0.0) {
-        const double rate = std::min(std::log(rr_start / mk.rr64) / (double)seen,
-                                     std::log(rr_seen / mk.rr64) / (double)(seen - iter_seen));
-        // still needed after chunk k (already enqueued)
-        const double need = std::log(std::max(mk.rr64 / target, 1.0)) / rate - (double)clen[k & 1];
-        const int64_t want = need <= 0.0 ? 8 : (int64_t)std::ceil(1.1 * need) + 8;
-        cnext = std::min<int64_t>(interval, std::max<int64_t>(8, want));
-      }
-      iter_seen = seen;
-      rr_seen = mk.rr64;
+    // unit u - 1 is known
+    const int ps = (int)((u - 1) & 1);
+    (void)hipEventSynchronize(ev[ps]);
+    if (sample_valid[ps]) prof_add(m->ctx, 1, ps);
+    sample_valid[ps] = false;
+    const MixProbe &pv = ph[ps];
+    if (pv.mix.final || !launched_now) break;
+    const Unit &cur = unit[u & 1];
+    // unit u's sample is a real iteration unless the recurrence is paused
+    sample_valid[u & 1] = cur.sample && !pv.st.done;
+    // FP32 residual and its rate of decrease since the last refresh
+    const int64_t iter_v = pv.st.iter + 1;
+    const double rr = (double)pv.st.nom[(pv.st.iter + 1) & 1];
+    if (unit[ps].refresh) {
+      base_iter = iter_v;
+      base_lrr = rr > 0.0 ? std::log(rr) : 0.0;
+    } else if (iter_v > base_iter && rr > 0.0 && std::log(rr) < base_lrr) {
+      rate = (base_lrr - std::log(rr)) / (double)(iter_v - base_iter);
     }
-    if (!launched_now) break;
-    pending = true;
-    k++;
+    // the current chunk aims at the final residual: its threshold is
+    // res ||b||^2, not delta^2 of the residual at its refresh
+    const bool aims_final = p->res * pv.mix.nory64 >= pv.mix.delta2 * pv.mix.rr64;
+    next_close = false;
+    next_len = MIX_UNIT;
+    if (cur.refresh) {
+      if (aims_final) next_len = 4;
+    } else if (pv.st.done) {
+      next_close = true;  // paused during unit u - 1: unit u ran no iteration
+      next_len = 0;
+    } else if (rate > 0.0 && rr > 0.0 && pv.st.nory > 0.0f) {
+      // iterations after unit u until the residual meets the threshold
+      const double left = std::log(rr / (double)pv.st.nory) / rate - (double)cur.len;
+      if (left < (double)MIX_UNIT)
+        next_len = std::max<int64_t>(2, (int64_t)std::ceil(left + 0.05 * std::max(left, 0.0)) + 1);
+    }
+    u++;
   }
   (void)hipStreamSynchronize(s);
   (void)hipEventDestroy(ev[0]);

@@ -110,6 +110,13 @@ struct MixState {
   int status;      // 1 converged, 4 FP64 floor reached above res
 };
 
+// The mixed solve's two device states side by side: one copy brings both to the
+// host.
+struct MixProbe {
+  MixState mix;
+  PcgState<float> st;
+};
+
 // Krylov recurrence variants of the reference.
 enum PcgMode : int {
   MODE_SCR_CG_PREC = 0,  // symcomprow::cg_prec, Jacobi, squared test   GEFEL/scr.cpp:1093
@@ -202,11 +209,11 @@ struct Workspace {  // vectors and reduction scratch of one solve shape
   double *part = nullptr;   // 4 * nb doubles
   float *part32 = nullptr;  // 4 * nb floats
   PcgState<double> *st = nullptr;
-  PcgState<float> *st32 = nullptr;
+  PcgState<float> *st32 = nullptr;      // &probe->st (mixed solves)
   PcgState<double> *st_host = nullptr;  // pinned
-  PcgState<float> *st32_host = nullptr; // pinned
-  MixState *mix = nullptr;
-  MixState *mix_host = nullptr;         // pinned, 2 entries
+  MixProbe *probe = nullptr;            // holds *mix and *st32
+  MixProbe *probe_host = nullptr;       // pinned, 2 entries
+  MixState *mix = nullptr;              // &probe->mix
 };
 
 struct DualSell;  // union layout of A and A^T of a comprow matrix (ssf_bicg.hip)
