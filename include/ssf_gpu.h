@@ -457,8 +457,10 @@ int ssf_elements_bench(ssf_elements *el, const double *lhs_dev, const double *pv
  * vectors; returns the mean kernel time in microseconds measured with HIP
  * events on the context's stream. */
 int ssf_spmv_bench(ssf_matrix *m, int precision, int reps, double *usec);
-/* SpMV autotuner: time every tiling of the FP64 (precision 0) or FP32 (1) SpMV
- * on this matrix and keep the fastest for later solves. tune_out (may be NULL)
+/* SpMV autotuner: for every tiling of the FP64 (precision 0) or FP32 (1) SpMV on
+ * this matrix, time one iteration of the recurrence a solve runs with it (the
+ * product and the element-wise kernels), and keep the fastest for later solves.
+ * tune_out (may be NULL)
  * receives 5 values {format (0 row-vector, 1 stream tiles, 2 SELL-64 slices),
  * lanes per row (formats 0, 1) or value loads in flight per lane (format 2 with
  * row patterns: 2, 4 or 8; stencil form: 3 or 4 takes that many groups of 4
@@ -472,7 +474,11 @@ int ssf_spmv_bench(ssf_matrix *m, int precision, int reps, double *usec);
  * slices fit 32-bit byte offsets; a slot the row lacks holds 0
  * and multiplies x at the row plus that offset, so its products equal the other
  * forms' for finite x only: an Inf or NaN in x makes NaN in rows that do not
- * couple to it), map (SELL slices to wavefronts: 0 a contiguous range per block, 1
+ * couple to it; 4 the symmetric half of the stencil form, for symm_comp_rows
+ * matrices whose stencil is symmetric: the lower slots of a row (or of a node's
+ * block of 2 or 3 rows) and its diagonal stored once, the upper entries read from
+ * the mirrored rows, lanes 1 or 2 = blocks of 256 or 512 threads; its products
+ * equal the other forms' bit for bit), map (SELL slices to wavefronts: 0 a contiguous range per block, 1
  * each of the 8 XCDs a contiguous eighth whose slices its wavefronts take in
  * turn; the default)}. */
 int ssf_matrix_autotune(ssf_matrix *m, int precision, int reps, int32_t *tune_out);

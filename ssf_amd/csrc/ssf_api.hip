@@ -193,7 +193,7 @@ static void half_setup_block(ssf_matrix *m)
       hipMemcpyAsync(m->half_m_d, nodes.data(), nl * sizeof(int), hipMemcpyHostToDevice,
                      m->ctx->stream) != hipSuccess ||
       hipStreamSynchronize(m->ctx->stream) != hipSuccess) {
-    dfree(m->half_slot, m->ctx->stream); dfree(m->half_m_d, m->ctx->stream);
+    dfree(m->half_slot, m->ctx->stream);
     dfree(m->half_m_d, m->ctx->stream);
     return;
   }
