@@ -65,7 +65,7 @@ typedef struct ssf_solve_params {
   int32_t iv;          /* 0: start from x = 0, 1: start from lhs                  */
   int32_t precision;   /* SSF_PRECISION_*                                         */
   double inner_res;    /* mixed: delta, relative FP32 residual reduction between two
-                          FP64 refreshes (0 = 0.1)                               */
+                          FP64 refreshes (0 = 1e-4)                              */
   int64_t inner_ni;    /* mixed: FP32 iterations between two FP64 refinements
                           (reliable updates); 0 = 128 (single systems), 32 (batch) */
   int64_t max_outer;   /* reserved (0)                                            */

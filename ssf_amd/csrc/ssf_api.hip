@@ -1656,7 +1656,7 @@ static int solve_mixed(ssf_matrix *m, const ssf_solve_params *p, const double *y
   SSF_HIP(hipMemsetAsync(w.x32, 0, m->n * sizeof(float), s));
   SSF_HIP(pcg_init<float>(m->tune32, false, false, A, m->val32, nullptr, w.x32, w.y32, w.r32,
                           nullptr, w.d32, w.part32, w.st32, 1e-30f, s));
-  const double delta = (p->inner_res > 0.0 && p->inner_res < 1.0) ? p->inner_res : 0.1;
+  const double delta = (p->inner_res > 0.0 && p->inner_res < 1.0) ? p->inner_res : 1e-4;
   SSF_HIP(mixed_init(w.mix, w.st32, dslot, m->norm_a, p->res, delta, s));
   // The FP32 iterations go out in units of at most MIX_UNIT iterations, the
   // host one unit ahead of what it has read back: after each unit both states

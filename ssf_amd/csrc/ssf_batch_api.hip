@@ -445,7 +445,7 @@ extern "C" int ssf_batch_solve_dev(ssf_batch *b, const ssf_solve_params *p, cons
   SSF_HIP(batch_compact(B, nullptr, b->wl, b->nwl, nullptr, nullptr, nullptr, 0, s));
   if (mixed) {
     SSF_HIP(hipMemsetAsync(b->x32, 0, nv * sizeof(float), s));
-    const double delta = (p->inner_res > 0.0 && p->inner_res < 1.0) ? p->inner_res : 0.1;
+    const double delta = (p->inner_res > 0.0 && p->inner_res < 1.0) ? p->inner_res : 1e-4;
     SSF_HIP(batch_mixed_start(b->tune64, b->tune32, B, b->vec<double>(), b->vec<float>(), b->st32.s,
                               b->mix, p->res, delta, s));
     SSF_HIP(batch_compact(B, b->mix.sys_final, b->wl, b->nwl, nullptr, nullptr, nullptr, 0, s));

@@ -503,7 +503,7 @@ void loop_main(ssf_server *S)
   hipStream_t s = S->ctx->stream;
   const ssf_solve_params P = S->params;
   const bool mixed = P.precision == SSF_PRECISION_MIXED;
-  const double delta = (P.inner_res > 0.0 && P.inner_res < 1.0) ? P.inner_res : 0.1;
+  const double delta = (P.inner_res > 0.0 && P.inner_res < 1.0) ? P.inner_res : 1e-4;
   // a slot is finished when its recurrence stopped (FP64) or its solve ended (mixed)
   int *fin_dev = mixed ? S->st.final : S->st.done;
   std::string fail;

@@ -406,13 +406,13 @@ class Matrix:
 
 def np_step_linear(K: Matrix, C: Matrix, A: Matrix, alpha: float, dt: float, rhs_ptr, lhs_ptr,
                    tdlhs_ptr, ni=100000, res=1e-20, precision=FP64, zero=1e-30, inner_ni=0,
-                   form="v"):
+                   form="v", inner_res=1e-4):
     """One generalized-trapezoidal step on device vectors: lhs (nodal values) and
     tdlhs (their rates) are updated; returns the solve info. form "v" is TRFEL's
     one_step_linear (npsolvert.cpp:367), "d" the step of linear_nonstat_solv_dform
     (npsolvert.cpp:1071)."""
     fn = {"v": "ssf_np_step_linear", "d": "ssf_np_step_dform"}[form]
-    p = SolveParams(SOLVER_CG, PREC_DIAG, ni, res, zero, 1, precision, 0.0, inner_ni, 0)
+    p = SolveParams(SOLVER_CG, PREC_DIAG, ni, res, zero, 1, precision, inner_res, inner_ni, 0)
     info = SolveInfo(-1, -1.0, 0, 0, 0, 0, 0.0)
     _check(getattr(lib(), fn)(K.h, C.h, A.h, float(alpha), float(dt), ctypes.byref(p),
                               rhs_ptr, lhs_ptr, tdlhs_ptr, ctypes.byref(info)))
