@@ -13,7 +13,9 @@
 // Mixed precision runs the FP32 recurrence as CG on the symmetrically scaled
 // operator float(S A S), S = diag(1/sqrt|a_ii|) (h = r, no diagonal), and adds,
 // once per chunk, the reliable update of ssf_solver.hip per pair:
-// x64 += S x32 / s, r64 = b - A x64 in FP64, FP32 residual refresh.
+// x64 += S x32 / s, r64 = b - A x64 in FP64, FP32 residual refresh. On SELL
+// slices its iteration is two passes: the SpMM with the partials of d.p, r.r,
+// r.p and p.p (k_bsell, kind 3), then x, r and d in one step (k_bstep).
 #include "ssf_batch.hpp"
 
 namespace ssf {
@@ -49,9 +51,13 @@ __global__ __launch_bounds__(TPB) void k_bsell(BatchDev B, const T *__restrict__
   const int s = B.blk_sys[blk];
   if (sys_done && sys_done[s]) continue;
   const int rend = B.blk_r1[blk];
-  T acc[R];
+  // KIND 3 (fused FP32 step, b = r): d.p, r.r, r.p, p.p per right-hand side
+  constexpr int NA = KIND == 3 ? 4 : 1;
+  T acc[NA][R];
 #pragma unroll
-  for (int r = 0; r < R; r++) acc[r] = T(0);
+  for (int k = 0; k < NA; k++)
+#pragma unroll
+    for (int r = 0; r < R; r++) acc[k][r] = T(0);
   for (int sl = B.blk_s0[blk] + wave; sl < B.blk_s1[blk]; sl += TPB / 64) {
     const int off = B.soff[sl];
     const int w = (B.soff[sl + 1] - off) / 64;
@@ -85,20 +91,45 @@ __global__ __launch_bounds__(TPB) void k_bsell(BatchDev B, const T *__restrict__
 #pragma unroll
       for (int r = 0; r < R; r++) {
         T v = sum[r];
-        if (KIND == 2) {
+        if (KIND == 3) {
+          const T rv = b[q0 + r];
+          acc[0][r] += x[q0 + r] * v;
+          acc[1][r] += rv * rv;
+          acc[2][r] += rv * v;
+          acc[3][r] += v * v;
+        } else if (KIND == 2) {
           v = b[q0 + r] - v;
-          acc[r] += v * v;
+          acc[0][r] += v * v;
         } else if (KIND == 1) {
-          acc[r] += x[q0 + r] * v;
+          acc[0][r] += x[q0 + r] * v;
         }
         y[q0 + r] = v;
       }
     }
   }
-  if (KIND != 0) {
+  if (KIND == 3) {
+    // the 4 R sums in one pass: wave sums, then wave 0's lanes over the waves
+    __shared__ T red[TPB / 64][4 * R];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const T t = wave_sum(acc[k][r]);
+        if (lane == 0) red[wave][k * R + r] = t;
+      }
+    __syncthreads();
+    if (threadIdx.x < 4 * R) {
+      T t = red[0][threadIdx.x];
+#pragma unroll
+      for (int w = 1; w < TPB / 64; w++) t += red[w][threadIdx.x];
+      const int k = threadIdx.x / R, r = threadIdx.x % R;
+      partial[((int64_t)k * B.nblk + blk) * R + r] = t;
+    }
+    __syncthreads();  // red is rewritten for the next block
+  } else if (KIND != 0) {
 #pragma unroll
     for (int r = 0; r < R; r++) {
-      const T t = block_sum(acc[r]);
+      const T t = block_sum(acc[0][r]);
       if (threadIdx.x == 0) partial[(int64_t)blk * R + r] = t;
     }
   }
@@ -260,6 +291,71 @@ __global__ __launch_bounds__(TPB) void k_bupdate(BatchDev B, BatchState<T> st,
     prr[(int64_t)blk * R + threadIdx.x] = outa[threadIdx.x];
     prr[(int64_t)(B.nblk + blk) * R + threadIdx.x] = outb[threadIdx.x];
   }
+  }
+}
+
+// ------------------------------------------- B2 + B3 fused (mixed, FP32)
+
+// One step of the FP32 recurrence after P = A D with the four dots (k_bsell,
+// KIND 3): alpha = r.r / d.p, x += alpha d, r -= alpha p, and with ||r - alpha
+// p||^2 = r.r - 2 alpha r.p + alpha^2 p.p, beta and d = r + beta d in the same
+// pass, as k_cg_fused_step does for one system. The new r.r < nory pauses the
+// pair until the refresh; state changes are left pending for k_badvance.
+template <typename T, int R>
+__global__ __launch_bounds__(TPB) void k_bstep(BatchDev B, BatchState<T> st,
+                                               const T *__restrict__ part, T zero,
+                                               T *__restrict__ x, T *__restrict__ rv,
+                                               T *__restrict__ d, const T *__restrict__ p)
+{
+  __shared__ T dots[4][R];
+  __shared__ T alpha[R], beta[R];
+  const int wn = wl_count(B);
+  for (int wi = blockIdx.x; wi < wn; wi += gridDim.x) {
+  const int blk = wl_block(B, wi);
+  const int s = B.blk_sys[blk];
+  if (st.sys_done[s]) continue;
+  const bool first = blk == B.sys_blk0[s];
+#pragma unroll
+  for (int k = 0; k < 4; k++) sys_sum_block<T, R>(B, part + (int64_t)k * B.nblk * R, s, dots[k]);
+  if (threadIdx.x < R) {
+    const int r = threadIdx.x;
+    const int q = s * R + r;
+    T a = T(0), bt = T(0);
+    if (!st.done[q] && !st.brk[q]) {
+      const T dp = dots[0][r], rr = dots[1][r], rp = dots[2][r], pp = dots[3][r];
+      const long long it = st.iter[q];
+      if (fabs(dp) < zero) {
+        if (first) {  // |d.Ap| < zero: the pair stops, applied by k_badvance
+          st.brk[q] = 1;
+          st.ani[q] = it;
+          st.ares[q] = rr;
+        }
+      } else {
+        a = rr / dp;
+        const T rrn = rr - T(2) * a * rp + a * a * pp;
+        bt = rrn / rr;
+        if (first) {
+          st.nom[2 * q + ((it + 1) & 1)] = rrn;
+          st.pend[q] = PEND_ADVANCE | (rrn < st.nory[q] ? PEND_STOP : 0);
+          st.pend_status[q] = 1;
+        }
+      }
+    }
+    alpha[r] = a;
+    beta[r] = bt;
+  }
+  __syncthreads();
+  const int64_t e0 = (int64_t)B.blk_r0[blk] * R, e1 = (int64_t)B.blk_r1[blk] * R;
+  const int r = threadIdx.x % R;  // TPB is a multiple of R
+  const T a = alpha[r], bt = beta[r];
+  if (a != T(0))
+    for (int64_t e = e0 + threadIdx.x; e < e1; e += TPB) {
+      x[e] += a * d[e];
+      const T ri = rv[e] - a * p[e];
+      rv[e] = ri;
+      d[e] = ri + bt * d[e];
+    }
+  __syncthreads();  // alpha / beta are rewritten for the next block
   }
 }
 
@@ -708,7 +804,9 @@ static hipError_t bspmm_r(const BatchTune &t, int kind, const BatchDev &B, const
     const bool i16 = B.scol16 != nullptr;
     if (kind == 0) { if (i16) SSF_BSELL(true, 0); else SSF_BSELL(false, 0); }
     else if (kind == 1) { if (i16) SSF_BSELL(true, 1); else SSF_BSELL(false, 1); }
-    else { if (i16) SSF_BSELL(true, 2); else SSF_BSELL(false, 2); }
+    else if (kind == 2) { if (i16) SSF_BSELL(true, 2); else SSF_BSELL(false, 2); }
+    else if (i16) SSF_BSELL(true, 3);
+    else SSF_BSELL(false, 3);
 #undef SSF_BSELL
     return hipGetLastError();
   }
@@ -752,9 +850,19 @@ template <typename T, int R>
 static hipError_t biter_r(const BatchTune &t, const BatchDev &B, const BatchVec<T> &v,
                           BatchState<T> st, T res, T zero, bool mixed, hipStream_t s)
 {
+  dim3 grid = wl_grid(B), block(TPB);
+  if (mixed && t.format == 2 && B.soff && v.sval) {
+    // the FP32 recurrence in two passes: SpMM with the four dots, fused step
+    hipError_t e = bspmm_r<T, R>(t, 3, B, v.val, v.sval, v.d, v.p, v.r, v.part, st.sys_done, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_bstep<T, R>), grid, block, 0, s, B, st, v.part, zero, v.x, v.r, v.d,
+                       v.p);
+    hipLaunchKernelGGL((k_badvance<T, R>), dim3((B.nsys + TPB - 1) / TPB), dim3(TPB), 0, s, B,
+                       st, nullptr);
+    return hipGetLastError();
+  }
   hipError_t e = bspmm_r<T, R>(t, 1, B, v.val, v.sval, v.d, v.p, nullptr, v.part, st.sys_done, s);
   if (e != hipSuccess) return e;
-  dim3 grid = wl_grid(B), block(TPB);
   T *prr = v.part + (int64_t)B.nblk * R;
   if (mixed) {  // the FP32 recurrence on the scaled operator: h = r
     hipLaunchKernelGGL((k_bupdate<T, R, false>), grid, block, 0, s, B, st, v.part, zero, v.x, v.r,

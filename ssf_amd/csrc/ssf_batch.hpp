@@ -57,7 +57,7 @@ struct BatchVec {  // interleaved [row * R + r]
   const T *val = nullptr, *diag = nullptr;
   const T *sval = nullptr;  // SELL values
   T *x = nullptr, *r = nullptr, *d = nullptr, *p = nullptr, *h = nullptr, *y = nullptr;
-  T *part = nullptr;  // 3 * nblk * R
+  T *part = nullptr;  // 3 * nblk * R (FP64), 4 * nblk * R (FP32)
 };
 
 struct BatchMixDev {

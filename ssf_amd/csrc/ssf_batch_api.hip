@@ -320,7 +320,8 @@ static int ensure_mixed(ssf_batch *b)
   c(balloc(&b->val32, b->nnz)); c(balloc(&b->mix.isd, b->nrows));
   c(balloc(&b->x32, nv)); c(balloc(&b->r32, nv)); c(balloc(&b->d32, nv)); c(balloc(&b->p32, nv));
   c(balloc(&b->y32, nv));
-  c(balloc(&b->part32, 3 * (int64_t)b->nblk * b->R));
+  // (4 partials per block and right-hand side: the fused FP32 step's dots)
+  c(balloc(&b->part32, 4 * (int64_t)b->nblk * b->R));
   c(b->st32.alloc(nq, b->nsys));
   BatchMixDev &m = b->mix;
   c(balloc(&m.nory64, nq)); c(balloc(&m.rr64, nq)); c(balloc(&m.best, nq)); c(balloc(&m.scale, nq));
