@@ -1768,10 +1768,10 @@ static int solve_mixed(ssf_matrix *m, const ssf_solve_params *p, const double *y
   (void)hipEventDestroy(ev[0]);
   (void)hipEventDestroy(ev[1]);
   if (rc) return rc;
-  MixState ms;
-  PcgState<float> st;
-  SSF_HIP(hipMemcpy(&ms, w.mix, sizeof ms, hipMemcpyDeviceToHost));
-  SSF_HIP(hipMemcpy(&st, w.st32, sizeof st, hipMemcpyDeviceToHost));
+  MixProbe pr;
+  SSF_HIP(hipMemcpy(&pr, w.probe, sizeof pr, hipMemcpyDeviceToHost));
+  const MixState &ms = pr.mix;
+  const PcgState<float> &st = pr.st;
   info->status = ms.final ? ms.status : 0;
   info->outer = ms.updates;
   info->iterations = st.iter + 1;
