@@ -2078,9 +2078,11 @@ extern "C" int ssf_matrix_autotune(ssf_matrix *m, int precision, int reps, int32
             t.map = map;
             cand.push_back(t);
           }
-  // symmetric half of the stencil form: blocks of 256 or 512 threads
+  // symmetric half of the stencil form: blocks of 256 or 512 threads; up to 4096
+  // blocks (the elasticity column's block products gain 2-4 % in the solves over
+  // 2048, more than the element-wise kernels' longer re-reduction costs)
   if (m->half_nl > 0)
-    for (int b : {256, 512, 1024, 2048})
+    for (int b : {256, 512, 1024, 2048, 4096})
       for (int tb : {1, 2})
         for (int map : {0, 1}) {
           if (b > std::max(8, (m->nslice + 3) / 4)) continue;
